@@ -1,0 +1,171 @@
+"""Benchmark: plain FP64 EMC iterations on one MI355X.
+
+Workload (default): 251^3 model from 400 Gaussian atoms in a radius-30 sphere; 260x257
+detector as two panels with a 10-px gap; 100,000 sparse Poisson frames, mean 200 photons
+with per-frame scale U(0.5, 1.5); num_div 10 (50,100 orientations); scale optimisation on.
+A step is one full EMC iteration (maximize + model update + scale normalisation) through
+EMCRecon.run_iteration with inputs resident in HBM; output files are not written.
+
+Prints one JSON line: frame x orientation pairs/s, the roofline of the dominant kernel
+(likelihood) from HIP events on the engine's stream, and a CPU baseline (the oracle's
+OpenMP restatement of the reference engine) timed on a bounded orientation sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'frame×orientation pairs/s per EMC iter; speedup vs plain FP64 GPU at ≤1e-3 rel err'
+HBM_PEAK_GBS = 8000.
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_recon(prob, cfg, device=0):
+    import dragonfly_amd as df
+    from dragonfly_amd.iterate import Iterate
+    det = prob['det']
+    f = prob['frames']
+    cdet = df.CDetector.from_arrays(det['qvals'], det['corr'], det['mask'], det['detd'], det['ewald_rad'])
+    ds = df.CDataset.from_arrays(cdet, f['ones'], f['multi'], f['place_ones'], f['place_multi'],
+                                 f['count_multi'])
+    itr = Iterate()
+    itr.params.need_scaling = cfg['need_scaling']
+    itr.params.beta_factor = 1.
+    itr.params.verbosity = 0
+    itr.dets = [cdet]
+    itr.set_data(ds)
+    itr.set_quat(df.Quaternion(cfg['num_div']))
+    from dragonfly_amd import synth
+    model = df.Model(cfg['size'])
+    model.allocate('')
+    model.model1 = synth.start_model(cfg['size'], itr.mean_count[0], det['mask'])
+    itr.set_model(model)
+    itr.parse_scale('')
+    itr.calc_beta(None if cfg['beta'] == 'auto' else float(cfg['beta']))
+    recon = df.EMCRecon(device=device)
+    recon.write_output = False
+    recon.set_iterate(itr)
+    return recon, itr
+
+
+def cpu_baseline(prob, itr, num_rot_sample, threads):
+    """Oracle (OpenMP restatement of the reference engine) on all frames and the first
+    `num_rot_sample` orientations; pairs/s scales linearly in the orientation count."""
+    from oracle import core
+    f = prob['frames']
+    quats = itr.quat.quats[:num_rot_sample].copy()
+    fp = dict(scale=np.ones(itr.tot_num_data), beta=itr.beta_start.copy(),
+              blacklist=itr.blacklist, fcounts=itr.fcounts, sum_fact=itr.sum_fact,
+              mean_count=itr.mean_count[0])
+    t0 = time.perf_counter()
+    core.maximize(prob['det'], quats, itr.model.model1 if itr.model.model1 is not None else None,
+                  f, fp, dict(iteration=1, need_scaling=itr.params.need_scaling, update_scale=1),
+                  num_threads=threads)
+    dt = time.perf_counter() - t0
+    pairs = itr.tot_num_data * quats.shape[0]
+    return pairs / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--config', default='cfg2')
+    ap.add_argument('--num-data', type=int, default=None)
+    ap.add_argument('--num-div', type=int, default=None)
+    ap.add_argument('--cpu-rot', type=int, default=128, help='orientations in the CPU sample')
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU only: this build has no multi-GPU path (the work '
+              'would shard by frames; see DESIGN.md). Run with --gpus 1.', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from dragonfly_amd import synth
+    t0 = time.perf_counter()
+    log('generating %s workload ...' % args.config)
+    prob = synth.make_problem(args.config, device='cuda', num_data=args.num_data,
+                              num_div=args.num_div, batch=1024)
+    cfg = prob['cfg']
+    log('generated in %.1f s' % (time.perf_counter() - t0))
+    # host copy of the start model for the CPU baseline before the GPU changes it
+    recon, itr = build_recon(prob, cfg)
+    start_model = itr.model.model1.copy()
+    f = prob['frames']
+    mask = prob['det']['mask']
+    good_o = mask[f['place_ones']] < 1
+    good_m = mask[f['place_multi']] < 1
+    D = itr.tot_num_data
+    nbar = (good_o.sum() + good_m.sum()) / D
+    mbar = good_m.sum() / D
+    R = itr.quat.num_rot
+    pairs = (D - itr.num_blacklist) * R
+
+    for it in range(1, args.warmup + 1):
+        itr.params.iteration = it
+        recon.run_iteration()
+        log('warmup iteration %d: %s' % (it, json.dumps({k: round(v, 1) for k, v in recon.last_stats['stage_ms'].items()})))
+    torch.cuda.synchronize()
+    lik_ms, stage_tot, nchunks = [], {}, 1
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        itr.params.iteration = args.warmup + 1 + k
+        recon.run_iteration()
+        st = recon.last_stats['stage_ms']
+        lik_ms.append(st['likelihood'])
+        nchunks = recon.last_stats['num_chunks']
+        for name, v in st.items():
+            stage_tot[name] = stage_tot.get(name, 0.) + v
+        log('iteration %d: %s' % (itr.params.iteration, json.dumps({k2: round(v, 1) for k2, v in st.items()})))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = elapsed / args.steps * 1e3
+    value = pairs * args.steps / elapsed
+
+    # roofline of the likelihood kernel: algorithmic bytes per pair 12*nbar + 4*mbar
+    # (int32 pixel index + f64 log W per photon pixel, int32 count per multi pixel)
+    lik_launch_ms = float(np.mean(lik_ms)) / nchunks
+    bytes_per_launch = pairs / nchunks * (12. * nbar + 4. * mbar)
+    achieved = bytes_per_launch / (lik_launch_ms * 1e-3) / 1e9
+    roof = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel='likelihood_kernel',
+                launch_ms=round(lik_launch_ms, 3))
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        itr.model.model1 = start_model
+        v_cpu, dt = cpu_baseline(prob, itr, args.cpu_rot, threads)
+        cpu = dict(value=round(v_cpu, 1), unit='frame×orientation pairs/s', cores=threads,
+                   kind='port', sample='iteration 1 of the oracle engine (OpenMP restatement of the '
+                   'reference) on all %d frames x first %d orientations: %.1f s' % (D, args.cpu_rot, dt))
+    out = {
+        'metric': METRIC, 'value': round(value, 1), 'unit': 'frame×orientation pairs/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(ms_per_step, 2),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': 'synthetic',
+        'config': {'workload': '%s: %d frames, %d orientations (num_div %d), %d^3 model, %d pixels, '
+                               'need_scaling %d' % (args.config, D, R, cfg['num_div'], cfg['size'],
+                                                    len(mask), cfg['need_scaling']),
+                   'path': 'plain FP64', 'frames': D, 'num_rot': R, 'num_pix': int(len(mask)),
+                   'mean_photon_pixels': round(float(nbar), 2)},
+        'stage_ms_mean': {k: round(v / args.steps, 2) for k, v in stage_tot.items()},
+        'roofline': roof,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,229 @@
+"""ctypes binding of libemc_hip.so (declared in include/emc_hip.h).
+
+The GPU engine is the only compute path: if the library is missing or no GPU is present,
+the calls below raise instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libemc_hip.so')
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lp = C.POINTER(C.c_long)
+_u8p = C.POINTER(C.c_uint8)
+
+# enum emc_field
+FIELDS = {
+    'u': (0, 'f8', 'rot'), 'max_exp': (1, 'f8', 'data'), 'p_norm': (2, 'f8', 'data'),
+    'rmax': (3, 'i4', 'data'), 'likelihood': (4, 'f8', 'data'), 'info': (5, 'f8', 'data'),
+    'psum_d': (6, 'f8', 'data'), 'scale': (7, 'f8', 'data'), 'model1': (8, 'f8', 'vol'),
+    'model2': (9, 'f8', 'vol'), 'inter_weight': (10, 'f8', 'vol'), 'psum_r': (11, 'f8', 'rot'),
+    'prob': (12, 'f8', 'data_rot'),
+}
+
+STAGES = ('expand', 'likelihood', 'rescale', 'max', 'norm', 'posterior', 'tomogram', 'merge')
+
+
+class IterOpts(C.Structure):
+    _fields_ = [('iteration', C.c_int), ('need_scaling', C.c_int), ('update_scale', C.c_int),
+                ('known_scale', C.c_int)]
+
+
+class IterStats(C.Structure):
+    _fields_ = [('avg_likelihood', C.c_double), ('mutual_info', C.c_double),
+                ('rescale', C.c_double), ('stage_ms', C.c_double * 8), ('num_chunks', C.c_int)]
+
+
+class EMCError(RuntimeError):
+    """Error reported by the engine (negative return code)."""
+
+
+_lib = None
+
+
+def lib():
+    """Load libemc_hip.so, raising if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError('libemc_hip.so is not built; run `python -m dragonfly_amd.build`')
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            'emc_create': (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+            'emc_destroy': (None, [C.c_void_p]),
+            'emc_last_error': (C.c_char_p, []),
+            'emc_synchronize': (C.c_int, [C.c_void_p]),
+            'emc_num_rot': (C.c_int, [C.c_int]),
+            'emc_quat_gen': (C.c_int, [C.c_int, _dp]),
+            'emc_set_detector': (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _u8p]),
+            'emc_set_quats': (C.c_int, [C.c_void_p, C.c_int, _dp]),
+            'emc_set_model': (C.c_int, [C.c_void_p, C.c_long, _dp]),
+            'emc_set_frames_sparse': (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _lp, _lp, _ip, _ip, _ip]),
+            'emc_set_frame_params': (C.c_int, [C.c_void_p, _dp, _dp, _u8p, _ip, _dp, C.c_double, C.c_int]),
+            'emc_set_beta': (C.c_int, [C.c_void_p, _dp]),
+            'emc_set_scale': (C.c_int, [C.c_void_p, _dp]),
+            'emc_maximize': (C.c_int, [C.c_void_p, C.POINTER(IterOpts), C.POINTER(IterStats)]),
+            'emc_update_model': (C.c_int, [C.c_void_p, C.c_double, _dp]),
+            'emc_normalize_scale': (C.c_int, [C.c_void_p, _dp]),
+            'emc_slice_gen': (C.c_int, [C.c_void_p, _dp, _dp]),
+            'emc_slice_merge': (C.c_int, [C.c_void_p, _dp, _dp]),
+            'emc_get_field': (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
+            'emc_clear_merge': (C.c_int, [C.c_void_p]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise EMCError('engine error %d: %s' % (rc, lib().emc_last_error().decode()))
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def num_rot(num_div):
+    n = lib().emc_num_rot(int(num_div))
+    if n < 0:
+        raise ValueError('num_div %d out of range' % num_div)
+    return n
+
+
+def quat_gen(num_div):
+    """quat_gen (dragonfly/src/quaternion.c:811-842): (num_rot, 5) array, bit-exact."""
+    q = np.zeros((num_rot(num_div), 5))
+    n = lib().emc_quat_gen(int(num_div), _ptr(q))
+    if n != q.shape[0]:
+        raise EMCError('emc_quat_gen failed (%d)' % n)
+    return q
+
+
+class Engine:
+    """One device-resident EMC problem (an ``emc_engine`` handle)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().emc_create(int(device), C.byref(h)))
+        self._h = h
+        self.num_pix = self.num_rot = self.num_data = self.size = 0
+
+    def close(self):
+        if getattr(self, '_h', None):
+            lib().emc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_detector(self, qvals, corr, mask):
+        qvals = np.ascontiguousarray(qvals, dtype='f8').reshape(-1, 3)
+        corr = np.ascontiguousarray(corr, dtype='f8')
+        mask = np.ascontiguousarray(mask, dtype='u1')
+        if not (qvals.shape[0] == corr.shape[0] == mask.shape[0]):
+            raise ValueError('detector arrays disagree on num_pix')
+        _check(lib().emc_set_detector(self._h, corr.shape[0], _ptr(qvals), _ptr(corr), _ptr(mask, _u8p)))
+        self.num_pix = corr.shape[0]
+
+    def set_quats(self, quats):
+        quats = np.ascontiguousarray(quats, dtype='f8')
+        if quats.ndim != 2 or quats.shape[1] != 5:
+            raise ValueError('quats must be (num_rot, 5)')
+        _check(lib().emc_set_quats(self._h, quats.shape[0], _ptr(quats)))
+        self.num_rot = quats.shape[0]
+
+    def set_model(self, model1):
+        model1 = np.ascontiguousarray(model1, dtype='f8')
+        if model1.ndim != 3 or len(set(model1.shape)) != 1:
+            raise ValueError('model1 must be a cube')
+        _check(lib().emc_set_model(self._h, model1.shape[0], _ptr(model1)))
+        self.size = model1.shape[0]
+
+    def set_frames(self, ones, multi, ones_accum, multi_accum, place_ones, place_multi, count_multi):
+        a = [np.ascontiguousarray(x, dtype=t) for x, t in (
+            (ones, 'i4'), (multi, 'i4'), (ones_accum, 'i8'), (multi_accum, 'i8'),
+            (place_ones, 'i4'), (place_multi, 'i4'), (count_multi, 'i4'))]
+        for i in (4, 5, 6):
+            if a[i].size == 0:
+                a[i] = np.zeros(1, 'i4')
+        _check(lib().emc_set_frames_sparse(
+            self._h, a[0].shape[0], _ptr(a[0], _ip), _ptr(a[1], _ip), _ptr(a[2], _lp),
+            _ptr(a[3], _lp), _ptr(a[4], _ip), _ptr(a[5], _ip), _ptr(a[6], _ip)))
+        self.num_data = a[0].shape[0]
+
+    def set_frame_params(self, scale, beta, blacklist, fcounts, sum_fact, mean_count, num_blacklist=0):
+        a = [np.ascontiguousarray(x, dtype=t) for x, t in (
+            (scale, 'f8'), (beta, 'f8'), (blacklist, 'u1'), (fcounts, 'i4'), (sum_fact, 'f8'))]
+        if any(x.shape[0] != self.num_data for x in a):
+            raise ValueError('per-frame arrays must have num_data entries')
+        _check(lib().emc_set_frame_params(self._h, _ptr(a[0]), _ptr(a[1]), _ptr(a[2], _u8p),
+                                          _ptr(a[3], _ip), _ptr(a[4]), float(mean_count),
+                                          int(num_blacklist)))
+
+    def set_beta(self, beta):
+        beta = np.ascontiguousarray(beta, dtype='f8')
+        if beta.shape[0] != self.num_data:
+            raise ValueError('beta must have num_data entries')
+        _check(lib().emc_set_beta(self._h, _ptr(beta)))
+
+    def set_scale(self, scale):
+        scale = np.ascontiguousarray(scale, dtype='f8')
+        if scale.shape[0] != self.num_data:
+            raise ValueError('scale must have num_data entries')
+        _check(lib().emc_set_scale(self._h, _ptr(scale)))
+
+    def maximize(self, iteration=1, need_scaling=0, update_scale=1, known_scale=0):
+        opts = IterOpts(int(iteration), int(need_scaling), int(update_scale), int(known_scale))
+        st = IterStats()
+        _check(lib().emc_maximize(self._h, C.byref(opts), C.byref(st)))
+        return dict(avg_likelihood=st.avg_likelihood, mutual_info=st.mutual_info,
+                    rescale=st.rescale, stage_ms=dict(zip(STAGES, list(st.stage_ms))),
+                    num_chunks=st.num_chunks)
+
+    def update_model(self, alpha=0.):
+        rms = C.c_double()
+        _check(lib().emc_update_model(self._h, float(alpha), C.byref(rms)))
+        return rms.value
+
+    def normalize_scale(self):
+        m = C.c_double()
+        _check(lib().emc_normalize_scale(self._h, C.byref(m)))
+        return m.value
+
+    def clear_merge(self):
+        _check(lib().emc_clear_merge(self._h))
+
+    def slice_gen(self, quat):
+        quat = np.ascontiguousarray(quat, dtype='f8')
+        out = np.empty(self.num_pix)
+        _check(lib().emc_slice_gen(self._h, _ptr(quat), _ptr(out)))
+        return out
+
+    def slice_merge(self, quat, view):
+        quat = np.ascontiguousarray(quat, dtype='f8')
+        view = np.ascontiguousarray(view, dtype='f8')
+        if view.shape[0] != self.num_pix:
+            raise ValueError('view must have num_pix entries')
+        _check(lib().emc_slice_merge(self._h, _ptr(quat), _ptr(view)))
+
+    def synchronize(self):
+        _check(lib().emc_synchronize(self._h))
+
+    def get(self, name):
+        fid, dt, kind = FIELDS[name]
+        shape = {'rot': (self.num_rot,), 'data': (self.num_data,),
+                 'vol': (self.size,) * 3, 'data_rot': (self.num_data, self.num_rot)}[kind]
+        out = np.empty(shape, dtype=dt)
+        _check(lib().emc_get_field(self._h, fid, out.ctypes.data_as(C.c_void_p), out.size))
+        return out

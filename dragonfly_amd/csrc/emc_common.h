@@ -1,0 +1,22 @@
+// Shared definitions of the MI355X EMC engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace emc {
+
+// Orientations handled by one wavefront: one lane per orientation, so a photon's row of
+// log W (pixel-major, orientation-contiguous) is one 512-byte coalesced read.
+constexpr int kRTile = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / 64;
+
+// Reference constants (dragonfly/src/max_internal.h:6-8).
+constexpr double kProbMin = 1.e-6;
+constexpr double kPdiffThresh = 14.;
+constexpr double kMaxExpStart = -1.e100;
+
+}  // namespace emc

@@ -1,0 +1,636 @@
+// C ABI of the MI355X EMC engine: device-resident state and the one-iteration driver.
+// The ABI is declared (with the reference interfaces each entry point replaces) in
+// include/emc_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/emc_hip.h"
+#include "emc_common.h"
+#include "emc_kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t err_ = (expr);                                                            \
+        if (err_ != hipSuccess)                                                              \
+            return fail(EMC_E_HIP, "%s failed: %s", #expr, hipGetErrorString(err_));        \
+    } while (0)
+
+// Device buffer with an element count.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count) {
+        if (count == n && p) return 0;
+        release();
+        if (count == 0) return 0;
+        hipError_t err = hipMalloc((void **)&p, count * sizeof(T));
+        if (err != hipSuccess) {
+            p = nullptr;
+            return fail(EMC_E_NOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                        hipGetErrorString(err));
+        }
+        n = count;
+        return 0;
+    }
+    int upload(const T *src, size_t count, hipStream_t s) {
+        int rc = alloc(count);
+        if (rc) return rc;
+        if (count) HIP_TRY(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+void make_rot(const double *q, double *rot) {
+    // make_rot_quat, dragonfly/src/model.c:759-786
+    double q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    double q01 = q0 * q1, q02 = q0 * q2, q03 = q0 * q3, q11 = q1 * q1, q12 = q1 * q2;
+    double q13 = q1 * q3, q22 = q2 * q2, q23 = q2 * q3, q33 = q3 * q3;
+    rot[0] = (1. - 2. * (q22 + q33));
+    rot[1] = 2. * (q12 + q03);
+    rot[2] = 2. * (q13 - q02);
+    rot[3] = 2. * (q12 - q03);
+    rot[4] = (1. - 2. * (q11 + q33));
+    rot[5] = 2. * (q01 + q23);
+    rot[6] = 2. * (q02 + q13);
+    rot[7] = 2. * (q23 - q01);
+    rot[8] = (1. - 2. * (q11 + q22));
+}
+
+constexpr int kPixPerGroup = 1024;
+constexpr int kFramesPerPsumPart = 512;
+constexpr int kModelParts = 1024;
+
+}  // namespace
+
+struct emc_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[10] = {};
+
+    // detector
+    int num_pix = 0;
+    std::vector<uint8_t> h_mask;
+    DevBuf<double> qvals, corr;
+    DevBuf<uint8_t> mask;
+    // orientations (rots padded to Rp with identity)
+    int num_rot = 0, Rp = 0;
+    DevBuf<double> quats, rots, logwr;
+    // model
+    long size = 0;
+    DevBuf<double> model1, model2, weight;
+    // frames
+    int num_data = 0;
+    std::vector<int> h_ones, h_multi, h_place_ones, h_place_multi, h_count_multi;
+    std::vector<long> h_ones_accum, h_multi_accum;
+    bool lists_dirty = true;
+    DevBuf<long> lik_off, tom_off;
+    DevBuf<int> lik_pix, tom_frame;
+    DevBuf<double> lik_cnt, tom_cnt;
+    // per-frame state
+    DevBuf<double> scale, beta, sum_fact;
+    DevBuf<uint8_t> blacklist;
+    DevBuf<int> fcounts;
+    double mean_count = 0.;
+    int num_blacklist = 0;
+    bool have_params = false;
+    // work and results
+    int C = 0;
+    DevBuf<double> logw, upart, u, R, psum_r, psum_part, scalars, model_part;
+    DevBuf<double> max_exp, p_norm, likelihood, info, psum_d;
+    DevBuf<int> rmax;
+    DevBuf<double> one_rot, one_view;
+    bool have_results = false;
+};
+
+namespace {
+
+int check_ready(emc_engine *e) {
+    if (!e) return fail(EMC_E_ARG, "null engine");
+    if (!e->num_pix) return fail(EMC_E_STATE, "detector not set");
+    if (!e->num_rot) return fail(EMC_E_STATE, "quaternions not set");
+    if (!e->size) return fail(EMC_E_STATE, "model not set");
+    if (!e->num_data) return fail(EMC_E_STATE, "frames not set");
+    if (!e->have_params) return fail(EMC_E_STATE, "frame parameters not set");
+    return 0;
+}
+
+// Photon lists of every frame in two device layouts:
+//  * likelihood list: per frame, (pixel, count) for mask < 1, ones then multi
+//    (the order calculate_prob sums them, max_prob.c:158-170)
+//  * tomogram list: per pixel with mask < 2, (frame, count) with frames ascending
+//    (the order update_tomogram_nobg accumulates them, max_update.c:196-209)
+int build_lists(emc_engine *e) {
+    const int N = e->num_data, P = e->num_pix;
+    const uint8_t *mask = e->h_mask.data();
+    std::vector<long> loff(N + 1, 0);
+    std::vector<long> tcount(P + 1, 0);
+    for (int d = 0; d < N; ++d) {
+        long n = 0;
+        for (int k = 0; k < e->h_ones[d]; ++k) {
+            int p = e->h_place_ones[e->h_ones_accum[d] + k];
+            if (p < 0 || p >= P) return fail(EMC_E_ARG, "frame %d: pixel %d out of range", d, p);
+            if (mask[p] < 1) ++n;
+            if (mask[p] < 2) ++tcount[p + 1];
+        }
+        for (int k = 0; k < e->h_multi[d]; ++k) {
+            int p = e->h_place_multi[e->h_multi_accum[d] + k];
+            if (p < 0 || p >= P) return fail(EMC_E_ARG, "frame %d: pixel %d out of range", d, p);
+            if (mask[p] < 1) ++n;
+            if (mask[p] < 2) ++tcount[p + 1];
+        }
+        loff[d + 1] = loff[d] + n;
+    }
+    for (int p = 0; p < P; ++p) tcount[p + 1] += tcount[p];
+    std::vector<int> lpix(loff[N]);
+    std::vector<double> lcnt(loff[N]);
+    std::vector<int> tframe(tcount[P]);
+    std::vector<double> tcnt(tcount[P]);
+    std::vector<long> tfill(tcount.begin(), tcount.end() - 1);
+    for (int d = 0; d < N; ++d) {
+        long at = loff[d];
+        for (int k = 0; k < e->h_ones[d]; ++k) {
+            int p = e->h_place_ones[e->h_ones_accum[d] + k];
+            if (mask[p] < 1) { lpix[at] = p; lcnt[at] = 1.; ++at; }
+            if (mask[p] < 2) { tframe[tfill[p]] = d; tcnt[tfill[p]] = 1.; ++tfill[p]; }
+        }
+        for (int k = 0; k < e->h_multi[d]; ++k) {
+            long j = e->h_multi_accum[d] + k;
+            int p = e->h_place_multi[j];
+            double c = (double)e->h_count_multi[j];
+            if (mask[p] < 1) { lpix[at] = p; lcnt[at] = c; ++at; }
+            if (mask[p] < 2) { tframe[tfill[p]] = d; tcnt[tfill[p]] = c; ++tfill[p]; }
+        }
+    }
+    int rc;
+    if ((rc = e->lik_off.upload(loff.data(), loff.size(), e->stream))) return rc;
+    if ((rc = e->lik_pix.upload(lpix.data(), lpix.size() ? lpix.size() : 0, e->stream))) return rc;
+    if ((rc = e->lik_cnt.upload(lcnt.data(), lcnt.size(), e->stream))) return rc;
+    if ((rc = e->tom_off.upload(tcount.data(), tcount.size(), e->stream))) return rc;
+    if ((rc = e->tom_frame.upload(tframe.data(), tframe.size(), e->stream))) return rc;
+    if ((rc = e->tom_cnt.upload(tcnt.data(), tcnt.size(), e->stream))) return rc;
+    // empty lists still need a valid pointer for the kernels
+    if (!e->lik_pix.p && (rc = e->lik_pix.alloc(1))) return rc;
+    if (!e->lik_cnt.p && (rc = e->lik_cnt.alloc(1))) return rc;
+    if (!e->tom_frame.p && (rc = e->tom_frame.alloc(1))) return rc;
+    if (!e->tom_cnt.p && (rc = e->tom_cnt.alloc(1))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->lists_dirty = false;
+    return 0;
+}
+
+// Work buffers. The posterior matrix R/P holds all frames x all orientations (the keep
+// rule needs each frame's global maximum before any normalisation); log W / view hold one
+// orientation chunk of C columns, C chosen to fit the remaining memory.
+int ensure_work(emc_engine *e) {
+    int rc;
+    const size_t N = e->num_data, Rp = e->Rp, P = e->num_pix;
+    if ((rc = e->R.alloc(N * Rp))) return rc;
+    size_t freeb = 0, totalb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+    const size_t reserve = (size_t)4 << 30;
+    size_t budget = freeb > reserve ? freeb - reserve : 0;
+    const size_t cap = (size_t)48 << 30;
+    if (e->logw.n) budget += e->logw.n * sizeof(double);
+    if (budget > cap) budget = cap;
+    size_t cmax = budget / (P * sizeof(double) + 1);
+    cmax = cmax / emc::kRTile * emc::kRTile;
+    if (cmax < (size_t)emc::kRTile) return fail(EMC_E_NOMEM, "no device memory for an orientation chunk");
+    const int C = (int)(cmax < Rp ? cmax : Rp);
+    if (C != e->C || !e->logw.p) {
+        if ((rc = e->logw.alloc(P * (size_t)C))) return rc;
+        e->C = C;
+    }
+    const size_t ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
+    if ((rc = e->upart.alloc(ngroups * (size_t)e->C))) return rc;
+    if ((rc = e->u.alloc(Rp))) return rc;
+    if ((rc = e->psum_r.alloc(Rp))) return rc;
+    const size_t nparts = (N + kFramesPerPsumPart - 1) / kFramesPerPsumPart;
+    if ((rc = e->psum_part.alloc(nparts * (size_t)e->num_rot))) return rc;
+    if ((rc = e->scalars.alloc(8))) return rc;
+    if ((rc = e->model_part.alloc(kModelParts))) return rc;
+    if ((rc = e->max_exp.alloc(N))) return rc;
+    if ((rc = e->p_norm.alloc(N))) return rc;
+    if ((rc = e->likelihood.alloc(N))) return rc;
+    if ((rc = e->info.alloc(N))) return rc;
+    if ((rc = e->psum_d.alloc(N))) return rc;
+    if ((rc = e->rmax.alloc(N))) return rc;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *emc_last_error(void) { return g_last_error.c_str(); }
+
+int emc_create(int device, emc_engine **out) {
+    if (!out) return fail(EMC_E_ARG, "null output pointer");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(EMC_E_ARG, "device %d not present (%d devices)", device, ndev);
+    HIP_TRY(hipSetDevice(device));
+    emc_engine *e = new emc_engine();
+    e->device = device;
+    hipError_t err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (err != hipSuccess) {
+        delete e;
+        return fail(EMC_E_HIP, "hipStreamCreate failed: %s", hipGetErrorString(err));
+    }
+    for (auto &ev : e->ev) (void)hipEventCreate(&ev);
+    *out = e;
+    return 0;
+}
+
+void emc_destroy(emc_engine *e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    for (DevBuf<double> *b : {&e->qvals, &e->corr, &e->quats, &e->rots, &e->logwr, &e->model1,
+                              &e->model2, &e->weight, &e->lik_cnt, &e->tom_cnt, &e->scale,
+                              &e->beta, &e->sum_fact, &e->logw, &e->upart, &e->u, &e->R,
+                              &e->psum_r, &e->psum_part, &e->scalars, &e->model_part,
+                              &e->max_exp, &e->p_norm, &e->likelihood, &e->info, &e->psum_d,
+                              &e->one_rot, &e->one_view})
+        b->release();
+    e->mask.release();
+    e->blacklist.release();
+    e->lik_off.release();
+    e->tom_off.release();
+    e->lik_pix.release();
+    e->tom_frame.release();
+    e->fcounts.release();
+    e->rmax.release();
+    for (auto &ev : e->ev) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int emc_synchronize(emc_engine *e) {
+    if (!e) return fail(EMC_E_ARG, "null engine");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int emc_set_detector(emc_engine *e, int num_pix, const double *qvals, const double *corr,
+                     const uint8_t *raw_mask) {
+    if (!e || num_pix <= 0 || !qvals || !corr || !raw_mask) return fail(EMC_E_ARG, "emc_set_detector: bad argument");
+    HIP_TRY(hipSetDevice(e->device));
+    int rc;
+    if ((rc = e->qvals.upload(qvals, 3 * (size_t)num_pix, e->stream))) return rc;
+    if ((rc = e->corr.upload(corr, num_pix, e->stream))) return rc;
+    if ((rc = e->mask.upload(raw_mask, num_pix, e->stream))) return rc;
+    e->h_mask.assign(raw_mask, raw_mask + num_pix);
+    e->num_pix = num_pix;
+    e->lists_dirty = true;
+    e->have_results = false;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int emc_set_quats(emc_engine *e, int num_rot, const double *quats) {
+    if (!e || num_rot <= 0 || !quats) return fail(EMC_E_ARG, "emc_set_quats: bad argument");
+    HIP_TRY(hipSetDevice(e->device));
+    const int Rp = (num_rot + emc::kRTile - 1) / emc::kRTile * emc::kRTile;
+    std::vector<double> rot((size_t)Rp * 9, 0.), lw(Rp, 0.);
+    for (int r = 0; r < Rp; ++r) {
+        if (r < num_rot) {
+            make_rot(&quats[5 * (size_t)r], &rot[9 * (size_t)r]);
+            lw[r] = std::log(quats[5 * (size_t)r + 4]);  // log(w_r), max_prob.c:125
+        } else {
+            rot[9 * (size_t)r] = rot[9 * (size_t)r + 4] = rot[9 * (size_t)r + 8] = 1.;
+        }
+    }
+    int rc;
+    if ((rc = e->quats.upload(quats, 5 * (size_t)num_rot, e->stream))) return rc;
+    if ((rc = e->rots.upload(rot.data(), rot.size(), e->stream))) return rc;
+    if ((rc = e->logwr.upload(lw.data(), lw.size(), e->stream))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->num_rot = num_rot;
+    e->Rp = Rp;
+    e->have_results = false;
+    return 0;
+}
+
+int emc_set_model(emc_engine *e, long size, const double *model1) {
+    if (!e || size < 3 || !model1) return fail(EMC_E_ARG, "emc_set_model: bad argument");
+    HIP_TRY(hipSetDevice(e->device));
+    const size_t vol = (size_t)size * size * size;
+    int rc;
+    if ((rc = e->model1.upload(model1, vol, e->stream))) return rc;
+    if ((rc = e->model2.alloc(vol))) return rc;
+    if ((rc = e->weight.alloc(vol))) return rc;
+    emc::launch_fill(e->stream, e->model2.p, vol, 0.);
+    emc::launch_fill(e->stream, e->weight.p, vol, 0.);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->size = size;
+    return 0;
+}
+
+int emc_set_frames_sparse(emc_engine *e, int num_data, const int *ones, const int *multi,
+                          const long *ones_accum, const long *multi_accum, const int *place_ones,
+                          const int *place_multi, const int *count_multi) {
+    if (!e || num_data <= 0 || !ones || !multi || !ones_accum || !multi_accum)
+        return fail(EMC_E_ARG, "emc_set_frames_sparse: bad argument");
+    long ones_total = ones_accum[num_data - 1] + ones[num_data - 1];
+    long multi_total = multi_accum[num_data - 1] + multi[num_data - 1];
+    if ((ones_total && !place_ones) || (multi_total && (!place_multi || !count_multi)))
+        return fail(EMC_E_ARG, "emc_set_frames_sparse: missing photon arrays");
+    for (int d = 0; d < num_data; ++d)
+        if (ones[d] < 0 || multi[d] < 0 || ones_accum[d] < 0 || multi_accum[d] < 0 ||
+            ones_accum[d] + ones[d] > ones_total || multi_accum[d] + multi[d] > multi_total)
+            return fail(EMC_E_ARG, "emc_set_frames_sparse: inconsistent offsets at frame %d", d);
+    e->h_ones.assign(ones, ones + num_data);
+    e->h_multi.assign(multi, multi + num_data);
+    e->h_ones_accum.assign(ones_accum, ones_accum + num_data);
+    e->h_multi_accum.assign(multi_accum, multi_accum + num_data);
+    e->h_place_ones.assign(place_ones, place_ones + ones_total);
+    e->h_place_multi.assign(place_multi, place_multi + multi_total);
+    e->h_count_multi.assign(count_multi, count_multi + multi_total);
+    e->num_data = num_data;
+    e->lists_dirty = true;
+    e->have_params = false;
+    e->have_results = false;
+    return 0;
+}
+
+int emc_set_frame_params(emc_engine *e, const double *scale, const double *beta,
+                         const uint8_t *blacklist, const int *fcounts, const double *sum_fact,
+                         double mean_count, int num_blacklist) {
+    if (!e || !e->num_data) return fail(EMC_E_STATE, "emc_set_frame_params: frames not set");
+    if (!scale || !beta || !blacklist || !fcounts || !sum_fact)
+        return fail(EMC_E_ARG, "emc_set_frame_params: bad argument");
+    if (num_blacklist < 0 || num_blacklist >= e->num_data)
+        return fail(EMC_E_ARG, "emc_set_frame_params: num_blacklist %d out of range", num_blacklist);
+    HIP_TRY(hipSetDevice(e->device));
+    const size_t N = e->num_data;
+    int rc;
+    if ((rc = e->scale.upload(scale, N, e->stream))) return rc;
+    if ((rc = e->beta.upload(beta, N, e->stream))) return rc;
+    if ((rc = e->blacklist.upload(blacklist, N, e->stream))) return rc;
+    if ((rc = e->fcounts.upload(fcounts, N, e->stream))) return rc;
+    if ((rc = e->sum_fact.upload(sum_fact, N, e->stream))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->mean_count = mean_count;
+    e->num_blacklist = num_blacklist;
+    e->have_params = true;
+    return 0;
+}
+
+int emc_set_beta(emc_engine *e, const double *beta) {
+    if (!e || !beta || !e->beta.p) return fail(EMC_E_STATE, "emc_set_beta: frame parameters not set");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipMemcpyAsync(e->beta.p, beta, e->beta.n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int emc_set_scale(emc_engine *e, const double *scale) {
+    if (!e || !scale || !e->scale.p) return fail(EMC_E_STATE, "emc_set_scale: frame parameters not set");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipMemcpyAsync(e->scale.p, scale, e->scale.n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int emc_clear_merge(emc_engine *e) {
+    if (!e || !e->size) return fail(EMC_E_STATE, "emc_clear_merge: model not set");
+    HIP_TRY(hipSetDevice(e->device));
+    const size_t vol = (size_t)e->size * e->size * e->size;
+    emc::launch_fill(e->stream, e->model2.p, vol, 0.);
+    emc::launch_fill(e->stream, e->weight.p, vol, 0.);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats) {
+    int rc;
+    if ((rc = check_ready(e))) return rc;
+    if (!opts) return fail(EMC_E_ARG, "emc_maximize: null options");
+    HIP_TRY(hipSetDevice(e->device));
+    if (e->lists_dirty && (rc = build_lists(e))) return rc;
+    if ((rc = ensure_work(e))) return rc;
+
+    hipStream_t s = e->stream;
+    const int N = e->num_data, Rn = e->num_rot, P = e->num_pix, C = e->C;
+    const size_t ldR = e->Rp, vol = (size_t)e->size * e->size * e->size;
+    const int ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
+    const int use_scale = opts->need_scaling && (opts->iteration > 1 || opts->known_scale);
+    const int want_psum_d = opts->need_scaling && opts->update_scale;
+    float ms_expand = 0.f, ms_lik = 0.f, ms_tomo = 0.f, ms_merge = 0.f;
+
+    // allocate_memory: zero the merge targets (maximize.c:93-94)
+    emc::launch_fill(s, e->model2.p, vol, 0.);
+    emc::launch_fill(s, e->weight.p, vol, 0.);
+
+    // expand + likelihood, chunk by chunk over orientations
+    for (int r0 = 0; r0 < e->Rp; r0 += C) {
+        const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
+        float a = 0.f, b = 0.f;
+        HIP_TRY(hipEventRecord(e->ev[0], s));
+        emc::launch_expand(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
+                           P, r0, Cc, kPixPerGroup, e->logw.p, e->upart.p);
+        emc::launch_u_reduce(s, e->upart.p, ngroups, Cc, r0, Rn, e->u.p);
+        HIP_TRY(hipEventRecord(e->ev[1], s));
+        emc::launch_likelihood(s, e->logw.p, Cc, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
+                               e->blacklist.p, N, e->R.p, ldR, r0);
+        HIP_TRY(hipEventRecord(e->ev[2], s));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventSynchronize(e->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&a, e->ev[0], e->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&b, e->ev[1], e->ev[2]));
+        ms_expand += a;
+        ms_lik += b;
+    }
+    HIP_TRY(hipEventRecord(e->ev[3], s));
+    emc::launch_rescale(s, e->quats.p, e->u.p, Rn, e->mean_count, e->scalars.p);
+    HIP_TRY(hipEventRecord(e->ev[4], s));
+    emc::launch_frame_max(s, e->R.p, ldR, Rn, N, e->logwr.p, e->u.p, e->scale.p, use_scale,
+                          e->scalars.p, e->blacklist.p, e->max_exp.p, e->rmax.p);
+    HIP_TRY(hipEventRecord(e->ev[5], s));
+    emc::launch_frame_norm(s, e->R.p, ldR, Rn, N, e->beta.p, e->blacklist.p, e->max_exp.p,
+                           e->rmax.p, e->p_norm.p);
+    HIP_TRY(hipEventRecord(e->ev[6], s));
+    emc::launch_posterior(s, e->R.p, ldR, Rn, N, e->quats.p, e->u.p, e->beta.p, e->sum_fact.p,
+                          e->blacklist.p, e->max_exp.p, e->rmax.p, e->p_norm.p, e->scalars.p,
+                          want_psum_d, e->likelihood.p, e->info.p, e->psum_d.p);
+    emc::launch_psum_r(s, e->R.p, ldR, Rn, N, kFramesPerPsumPart, e->scale.p, opts->need_scaling,
+                       e->blacklist.p, e->psum_part.p, e->psum_r.p);
+    HIP_TRY(hipEventRecord(e->ev[7], s));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventSynchronize(e->ev[7]));
+    float ms_rescale = 0.f, ms_max = 0.f, ms_norm = 0.f, ms_post = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms_rescale, e->ev[3], e->ev[4]));
+    HIP_TRY(hipEventElapsedTime(&ms_max, e->ev[4], e->ev[5]));
+    HIP_TRY(hipEventElapsedTime(&ms_norm, e->ev[5], e->ev[6]));
+    HIP_TRY(hipEventElapsedTime(&ms_post, e->ev[6], e->ev[7]));
+
+    // tomogram + merge, chunk by chunk
+    for (int r0 = 0; r0 < e->Rp; r0 += C) {
+        const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
+        float a = 0.f, b = 0.f;
+        HIP_TRY(hipEventRecord(e->ev[0], s));
+        emc::launch_tomogram(s, e->R.p, ldR, r0, Cc, e->tom_off.p, e->tom_frame.p, e->tom_cnt.p,
+                             e->blacklist.p, e->psum_r.p, Rn, P, e->logw.p);
+        HIP_TRY(hipEventRecord(e->ev[1], s));
+        emc::launch_merge(s, e->rots.p, r0, Cc, Rn, e->logw.p, e->psum_r.p, e->qvals.p,
+                          e->corr.p, e->mask.p, P, kPixPerGroup, e->size, e->model2.p,
+                          e->weight.p);
+        HIP_TRY(hipEventRecord(e->ev[2], s));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventSynchronize(e->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&a, e->ev[0], e->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&b, e->ev[1], e->ev[2]));
+        ms_tomo += a;
+        ms_merge += b;
+    }
+    // combine_information_mpi statistics and update_scale
+    emc::launch_frame_stats(s, e->likelihood.p, e->info.p, e->blacklist.p, N, N - e->num_blacklist,
+                            e->scalars.p);
+    if (want_psum_d) emc::launch_scale(s, e->fcounts.p, e->psum_d.p, e->blacklist.p, N, e->scale.p);
+    HIP_TRY(hipGetLastError());
+    double host_scalars[3];
+    HIP_TRY(hipMemcpyAsync(host_scalars, e->scalars.p, sizeof host_scalars, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    e->have_results = true;
+    if (stats) {
+        stats->rescale = host_scalars[0];
+        stats->avg_likelihood = host_scalars[1];
+        stats->mutual_info = host_scalars[2];
+        const float ms[8] = {ms_expand, ms_lik, ms_rescale, ms_max, ms_norm, ms_post, ms_tomo, ms_merge};
+        for (int i = 0; i < 8; ++i) stats->stage_ms[i] = ms[i];
+        stats->num_chunks = (e->Rp + C - 1) / C;
+    }
+    return 0;
+}
+
+int emc_update_model(emc_engine *e, double alpha, double *rms_change) {
+    if (!e || !e->size) return fail(EMC_E_STATE, "emc_update_model: model not set");
+    HIP_TRY(hipSetDevice(e->device));
+    int rc;
+    if ((rc = e->model_part.alloc(kModelParts))) return rc;
+    if ((rc = e->scalars.alloc(8))) return rc;
+    emc::launch_update_model(e->stream, e->model1.p, e->model2.p, e->weight.p, e->size, alpha,
+                             e->model_part.p, kModelParts, e->scalars.p + 3);
+    HIP_TRY(hipGetLastError());
+    double change = 0.;
+    HIP_TRY(hipMemcpyAsync(&change, e->scalars.p + 3, sizeof change, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    const long vol = e->size * e->size * e->size;
+    if (rms_change) *rms_change = std::sqrt(change / 1 / vol);
+    return 0;
+}
+
+int emc_normalize_scale(emc_engine *e, double *mean_scale) {
+    if (!e || !e->have_params || !e->size) return fail(EMC_E_STATE, "emc_normalize_scale: state not set");
+    HIP_TRY(hipSetDevice(e->device));
+    int rc;
+    if ((rc = e->scalars.alloc(8))) return rc;
+    emc::launch_normalize_scale(e->stream, e->scale.p, e->blacklist.p, e->num_data, e->model1.p,
+                                e->size * e->size * e->size, e->scalars.p + 4);
+    HIP_TRY(hipGetLastError());
+    double m = 0.;
+    HIP_TRY(hipMemcpyAsync(&m, e->scalars.p + 4, sizeof m, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (mean_scale) *mean_scale = m;
+    return 0;
+}
+
+int emc_slice_gen(emc_engine *e, const double *quat, double *view_out) {
+    if (!e || !quat || !view_out) return fail(EMC_E_ARG, "emc_slice_gen: bad argument");
+    if (!e->num_pix || !e->size) return fail(EMC_E_STATE, "emc_slice_gen: detector/model not set");
+    HIP_TRY(hipSetDevice(e->device));
+    double rot[9];
+    make_rot(quat, rot);
+    int rc;
+    if ((rc = e->one_rot.upload(rot, 9, e->stream))) return rc;
+    if ((rc = e->one_view.alloc(e->num_pix))) return rc;
+    emc::launch_slice_gen(e->stream, e->one_rot.p, e->model1.p, e->size, e->qvals.p, e->corr.p,
+                          e->num_pix, e->one_view.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(view_out, e->one_view.p, e->num_pix * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int emc_slice_merge(emc_engine *e, const double *quat, const double *view) {
+    if (!e || !quat || !view) return fail(EMC_E_ARG, "emc_slice_merge: bad argument");
+    if (!e->num_pix || !e->size) return fail(EMC_E_STATE, "emc_slice_merge: detector/model not set");
+    HIP_TRY(hipSetDevice(e->device));
+    double rot[9];
+    make_rot(quat, rot);
+    int rc;
+    if ((rc = e->one_rot.upload(rot, 9, e->stream))) return rc;
+    if ((rc = e->one_view.upload(view, e->num_pix, e->stream))) return rc;
+    emc::launch_slice_merge(e->stream, e->one_rot.p, e->one_view.p, e->qvals.p, e->corr.p,
+                            e->mask.p, e->num_pix, e->size, e->model2.p, e->weight.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int emc_get_field(emc_engine *e, int field, void *dst, size_t count) {
+    if (!e || !dst) return fail(EMC_E_ARG, "emc_get_field: bad argument");
+    HIP_TRY(hipSetDevice(e->device));
+    const size_t N = e->num_data, Rn = e->num_rot, vol = (size_t)e->size * e->size * e->size;
+    const void *src = nullptr;
+    size_t n = 0, elem = sizeof(double);
+    bool needs_results = true;
+    switch (field) {
+        case EMC_FIELD_U: src = e->u.p; n = Rn; break;
+        case EMC_FIELD_MAX_EXP: src = e->max_exp.p; n = N; break;
+        case EMC_FIELD_P_NORM: src = e->p_norm.p; n = N; break;
+        case EMC_FIELD_RMAX: src = e->rmax.p; n = N; elem = sizeof(int); break;
+        case EMC_FIELD_LIKELIHOOD: src = e->likelihood.p; n = N; break;
+        case EMC_FIELD_INFO: src = e->info.p; n = N; break;
+        case EMC_FIELD_PSUM_D: src = e->psum_d.p; n = N; break;
+        case EMC_FIELD_PSUM_R: src = e->psum_r.p; n = Rn; break;
+        case EMC_FIELD_SCALE: src = e->scale.p; n = N; needs_results = false; break;
+        case EMC_FIELD_MODEL1: src = e->model1.p; n = vol; needs_results = false; break;
+        case EMC_FIELD_MODEL2: src = e->model2.p; n = vol; needs_results = false; break;
+        case EMC_FIELD_INTER_WEIGHT: src = e->weight.p; n = vol; needs_results = false; break;
+        case EMC_FIELD_PROB: {
+            if (!e->have_results) return fail(EMC_E_STATE, "emc_get_field: no iteration has run");
+            if (count != N * Rn) return fail(EMC_E_ARG, "emc_get_field: count %zu != %zu", count, N * Rn);
+            HIP_TRY(hipMemcpy2DAsync(dst, Rn * sizeof(double), e->R.p, e->Rp * sizeof(double),
+                                     Rn * sizeof(double), N, hipMemcpyDeviceToHost, e->stream));
+            HIP_TRY(hipStreamSynchronize(e->stream));
+            return 0;
+        }
+        default: return fail(EMC_E_ARG, "emc_get_field: unknown field %d", field);
+    }
+    if (needs_results && !e->have_results) return fail(EMC_E_STATE, "emc_get_field: no iteration has run");
+    if (!src) return fail(EMC_E_STATE, "emc_get_field: field %d not allocated", field);
+    if (count != n) return fail(EMC_E_ARG, "emc_get_field: count %zu != %zu", count, n);
+    HIP_TRY(hipMemcpyAsync(dst, src, n * elem, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+// Host launchers of the EMC kernels (defined in emc_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace emc {
+
+void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
+                   const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
+                   int r0, int C, int pix_per_group, double *logw, double *upart);
+void launch_u_reduce(hipStream_t s, const double *upart, int ngroups, int C, int r0, int num_rot,
+                     double *u);
+void launch_rescale(hipStream_t s, const double *quats, const double *u, int num_rot,
+                    double mean_count, double *scalars);
+void launch_likelihood(hipStream_t s, const double *logw, int C, const long *off, const int *pix,
+                       const double *cnt, const uint8_t *blacklist, int num_data, double *G,
+                       size_t ldG, int r0);
+void launch_frame_max(hipStream_t s, double *R, size_t ldR, int num_rot, int num_data,
+                      const double *logwr, const double *u, const double *scale, int use_scale,
+                      const double *scalars, const uint8_t *blacklist, double *max_exp, int *rmax);
+void launch_frame_norm(hipStream_t s, const double *R, size_t ldR, int num_rot, int num_data,
+                       const double *beta, const uint8_t *blacklist, const double *max_exp,
+                       const int *rmax, double *p_norm);
+void launch_posterior(hipStream_t s, double *R, size_t ldR, int num_rot, int num_data,
+                      const double *quats, const double *u, const double *beta,
+                      const double *sum_fact, const uint8_t *blacklist, const double *max_exp,
+                      const int *rmax, const double *p_norm, const double *scalars,
+                      int want_psum_d, double *likelihood, double *info, double *psum_d);
+void launch_psum_r(hipStream_t s, const double *P, size_t ldP, int num_rot, int num_data,
+                   int frames_per_part, const double *scale, int need_scaling,
+                   const uint8_t *blacklist, double *part, double *psum_r);
+void launch_tomogram(hipStream_t s, const double *P, size_t ldP, int r0, int C, const long *toff,
+                     const int *tframe, const double *tcnt, const uint8_t *blacklist,
+                     const double *psum_r, int num_rot, int num_pix, double *view);
+void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot, const double *view,
+                  const double *psum_r, const double *qvals, const double *corr,
+                  const uint8_t *mask, int num_pix, int pix_per_group, long size, double *model2,
+                  double *weight);
+void launch_slice_gen(hipStream_t s, const double *rot, const double *model, long size,
+                      const double *qvals, const double *corr, int num_pix, double *out);
+void launch_slice_merge(hipStream_t s, const double *rot, const double *view, const double *qvals,
+                        const double *corr, const uint8_t *mask, int num_pix, long size,
+                        double *model2, double *weight);
+void launch_frame_stats(hipStream_t s, const double *likelihood, const double *info,
+                        const uint8_t *blacklist, int num_data, int denom, double *scalars);
+void launch_scale(hipStream_t s, const int *fcounts, const double *psum_d, const uint8_t *blacklist,
+                  int num_data, double *scale);
+void launch_fill(hipStream_t s, double *a, size_t n, double v);
+void launch_update_model(hipStream_t s, double *model1, double *model2, const double *w, long size,
+                         double alpha, double *part, int nparts, double *out_change);
+void launch_normalize_scale(hipStream_t s, double *scale, const uint8_t *blacklist, int num_data,
+                            double *model1, long vol, double *mean_out);
+
+}  // namespace emc

@@ -1,0 +1,721 @@
+// HIP kernels of the plain FP64 EMC iteration for gfx950 (MI355X).
+//
+// Layout in HBM (all row-major):
+//   logW / view   [num_pix][C]      one orientation chunk of C (multiple of 64) columns:
+//                                   pixel-major, orientation-contiguous, so a photon's row is
+//                                   a 512-byte coalesced read by one wavefront
+//   R / P         [num_data][Rp]    per-frame log-likelihood, then posterior, in place;
+//                                   Rp = num_rot rounded up to 64
+//   model         [size][size][size] x-major (index x*size*size + y*size + z)
+// Every kernel keeps the reference's operation order for the values it computes
+// (the library is compiled with -ffp-contract=off, so no multiply-add is fused).
+#include <cfloat>
+
+#include "emc_common.h"
+#include "emc_kernels.h"
+
+namespace emc {
+
+// ---- geometry ------------------------------------------------------------------------
+
+// Rotated voxel position of a pixel and its trilinear cell, as slice_gen3d / slice_merge3d
+// (dragonfly/src/model.c:19-45 and :184-207): p = R q + center, cell = trunc(p).
+struct Cell {
+    long x, y, z;
+    double fx, fy, fz;
+};
+
+__device__ __forceinline__ bool locate(const double *rot, double q0, double q1, double q2,
+                                       double center, long size, Cell &c) {
+    double p0 = rot[0] * q0 + rot[1] * q1 + rot[2] * q2 + center;
+    double p1 = rot[3] * q0 + rot[4] * q1 + rot[5] * q2 + center;
+    double p2 = rot[6] * q0 + rot[7] * q1 + rot[8] * q2 + center;
+    c.x = (long)p0;
+    c.y = (long)p1;
+    c.z = (long)p2;
+    if (c.x < 0 || c.x > size - 2 || c.y < 0 || c.y > size - 2 || c.z < 0 || c.z > size - 2)
+        return false;
+    c.fx = p0 - c.x;
+    c.fy = p1 - c.y;
+    c.fz = p2 - c.z;
+    return true;
+}
+
+// Interpolated model value times corr, DBL_MIN outside the grid (model.c:35-57).
+__device__ __forceinline__ double slice_value(const double *__restrict__ model, long size,
+                                              const double *rot, double q0, double q1, double q2,
+                                              double corr, double center) {
+    Cell c;
+    if (!locate(rot, q0, q1, q2, center, size, c)) return DBL_MIN;
+    const long s2 = size * size;
+    const long x1 = (c.x + 1) % size, y1 = (c.y + 1) % size, z1 = (c.z + 1) % size;
+    const double cx = 1. - c.fx, cy = 1. - c.fy, cz = 1. - c.fz;
+    const double fx = c.fx, fy = c.fy, fz = c.fz;
+    double v = cx * cy * cz * model[c.x * s2 + c.y * size + c.z] +
+               cx * cy * fz * model[c.x * s2 + c.y * size + z1] +
+               cx * fy * cz * model[c.x * s2 + y1 * size + c.z] +
+               cx * fy * fz * model[c.x * s2 + y1 * size + z1] +
+               fx * cy * cz * model[x1 * s2 + c.y * size + c.z] +
+               fx * cy * fz * model[x1 * s2 + c.y * size + z1] +
+               fx * fy * cz * model[x1 * s2 + y1 * size + c.z] +
+               fx * fy * fz * model[x1 * s2 + y1 * size + z1];
+    return v * corr;
+}
+
+// Scatter one pixel value into model2/weight (model.c:209-242); w = view / corr.
+__device__ __forceinline__ void merge_value(double *__restrict__ model2, double *__restrict__ weight,
+                                            long size, const Cell &c, double w) {
+    const long s2 = size * size;
+    const long x1 = (c.x + 1) % size, y1 = (c.y + 1) % size, z1 = (c.z + 1) % size;
+    const double cx = 1. - c.fx, cy = 1. - c.fy, cz = 1. - c.fz;
+    const double fx = c.fx, fy = c.fy, fz = c.fz;
+    const long idx[8] = {c.x * s2 + c.y * size + c.z, c.x * s2 + c.y * size + z1,
+                         c.x * s2 + y1 * size + c.z, c.x * s2 + y1 * size + z1,
+                         x1 * s2 + c.y * size + c.z, x1 * s2 + c.y * size + z1,
+                         x1 * s2 + y1 * size + c.z, x1 * s2 + y1 * size + z1};
+    const double f[8] = {cx * cy * cz, cx * cy * fz, cx * fy * cz, cx * fy * fz,
+                         fx * cy * cz, fx * cy * fz, fx * fy * cz, fx * fy * fz};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        unsafeAtomicAdd(&weight[idx[k]], f[k]);
+        unsafeAtomicAdd(&model2[idx[k]], f[k] * w);
+    }
+}
+
+// ---- expand: log W for one orientation chunk, plus partial u_r ---------------------------
+// Block = 64 orientations x one pixel group. Compute with lanes along pixels (neighbouring
+// pixels hit neighbouring voxels), transpose through LDS, store with lanes along
+// orientations (coalesced rows of logW). u_r (max_prob.c:36-38) sums W over mask < 1.
+__global__ __launch_bounds__(kBlock) void expand_kernel(
+    const double *__restrict__ rots, const double *__restrict__ model, long size,
+    const double *__restrict__ qvals, const double *__restrict__ corr,
+    const uint8_t *__restrict__ mask, int num_pix, int r0, int C, int pix_per_group,
+    double *__restrict__ logw, double *__restrict__ upart) {
+    __shared__ double s_rot[kRTile * 9];
+    __shared__ double s_tile[64][kRTile + 1];
+    __shared__ double s_u[kWavesPerBlock][kRTile];
+    const int tid = threadIdx.x;
+    const int c0 = blockIdx.x * kRTile;
+    const int g = blockIdx.y;
+    for (int i = tid; i < kRTile * 9; i += kBlock) s_rot[i] = rots[(size_t)(r0 + c0) * 9 + i];
+    __syncthreads();
+    const double center = (double)(size / 2);
+    const int lane = tid & 63, wave = tid >> 6;
+    const int t_begin = g * pix_per_group;
+    const int t_end = min(num_pix, t_begin + pix_per_group);
+    double uacc = 0.;
+    for (int tb = t_begin; tb < t_end; tb += 64) {
+        const int t = tb + lane;
+        if (t < t_end) {
+            const double q0 = qvals[3 * (size_t)t], q1 = qvals[3 * (size_t)t + 1],
+                         q2 = qvals[3 * (size_t)t + 2], cr = corr[t];
+            for (int k = 0; k < kRTile / kWavesPerBlock; ++k) {
+                const int rr = wave + kWavesPerBlock * k;
+                s_tile[lane][rr] = slice_value(model, size, &s_rot[rr * 9], q0, q1, q2, cr, center);
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < 64 / kWavesPerBlock; ++k) {
+            const int p = wave + kWavesPerBlock * k;
+            const int tt = tb + p;
+            if (tt < t_end) {
+                const double W = s_tile[p][lane];
+                if (mask[tt] < 1) uacc += W;
+                logw[(size_t)tt * C + c0 + lane] = log(W);
+            }
+        }
+        __syncthreads();
+    }
+    s_u[wave][lane] = uacc;
+    __syncthreads();
+    if (tid < kRTile)
+        upart[(size_t)g * C + c0 + tid] = s_u[0][tid] + s_u[1][tid] + s_u[2][tid] + s_u[3][tid];
+}
+
+// u[r0 + c] = sum over pixel groups, fixed order.
+__global__ void u_reduce_kernel(const double *__restrict__ upart, int ngroups, int C, int r0,
+                                int num_rot, double *__restrict__ u) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C || r0 + c >= num_rot) return;
+    double acc = 0.;
+    for (int g = 0; g < ngroups; ++g) acc += upart[(size_t)g * C + c];
+    u[r0 + c] = acc;
+}
+
+// rescale = mean_count / sum_r w_r u_r (max_prob.c:40, :59); one block, fixed order.
+__global__ __launch_bounds__(kBlock) void rescale_kernel(const double *__restrict__ quats,
+                                                         const double *__restrict__ u, int num_rot,
+                                                         double mean_count,
+                                                         double *__restrict__ scalars) {
+    __shared__ double s[kBlock];
+    double acc = 0.;
+    for (int r = threadIdx.x; r < num_rot; r += kBlock) acc += quats[5 * (size_t)r + 4] * u[r];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) scalars[0] = mean_count / s[0] * 1;
+}
+
+// ---- likelihood: G_dr = sum_t K_dt log W_rt over one frame's photons (mask < 1) ------------
+// One wavefront = one frame x 64 orientations; the frame's (pixel, count) list is read
+// with scalar loads (wave-uniform) and each photon costs one 512-byte row read of logW.
+__global__ __launch_bounds__(kBlock) void likelihood_kernel(
+    const double *__restrict__ logw, int C, const long *__restrict__ off,
+    const int *__restrict__ pix, const double *__restrict__ cnt,
+    const uint8_t *__restrict__ blacklist, int num_data, double *__restrict__ G, size_t ldG,
+    int r0) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int d = blockIdx.x * kWavesPerBlock + wave;
+    if (d >= num_data || blacklist[d]) return;
+    const int c = blockIdx.y * kRTile + lane;
+    const double *__restrict__ col = logw + c;
+    const long k1 = off[d + 1];
+    long k = off[d];
+    double acc = 0.;
+    for (; k + 4 <= k1; k += 4) {
+        const double v0 = col[(size_t)pix[k] * C], v1 = col[(size_t)pix[k + 1] * C];
+        const double v2 = col[(size_t)pix[k + 2] * C], v3 = col[(size_t)pix[k + 3] * C];
+        acc += cnt[k] * v0;
+        acc += cnt[k + 1] * v1;
+        acc += cnt[k + 2] * v2;
+        acc += cnt[k + 3] * v3;
+    }
+    for (; k < k1; ++k) acc += cnt[k] * col[(size_t)pix[k] * C];
+    G[(size_t)d * ldG + r0 + c] = acc;
+}
+
+// ---- block helpers -----------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double *s) {
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    double r = s[0];
+    __syncthreads();
+    return r;
+}
+
+// ---- per-frame maximum: R = log w_r - phi_d u_r + G, first arg-max (max_prob.c:122-127,
+// :199-203, normalize_prob :223-241) ----------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void frame_max_kernel(
+    double *__restrict__ R, size_t ldR, int num_rot, const double *__restrict__ logwr,
+    const double *__restrict__ u, const double *__restrict__ scale, int use_scale,
+    const double *__restrict__ scalars, const uint8_t *__restrict__ blacklist,
+    double *__restrict__ max_exp, int *__restrict__ rmax) {
+    __shared__ double s_v[kBlock];
+    __shared__ int s_i[kBlock];
+    const int d = blockIdx.x;
+    if (blacklist[d]) {
+        if (threadIdx.x == 0) {
+            max_exp[d] = kMaxExpStart;
+            rmax[d] = -1;
+        }
+        return;
+    }
+    const double phi = use_scale ? scale[d] : scalars[0];
+    double *row = R + (size_t)d * ldR;
+    double best = kMaxExpStart;
+    int bi = -1;
+    for (int r = threadIdx.x; r < num_rot; r += kBlock) {
+        const double v = (logwr[r] - u[r] * phi) + row[r];
+        row[r] = v;
+        if (v > best) {
+            best = v;
+            bi = r;
+        }
+    }
+    s_v[threadIdx.x] = best;
+    s_i[threadIdx.x] = bi;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            const double ov = s_v[threadIdx.x + w];
+            const int oi = s_i[threadIdx.x + w];
+            if (ov > s_v[threadIdx.x] || (ov == s_v[threadIdx.x] && oi >= 0 && oi < s_i[threadIdx.x])) {
+                s_v[threadIdx.x] = ov;
+                s_i[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        max_exp[d] = s_v[0];
+        rmax[d] = (s_v[0] == kMaxExpStart) ? -1 : s_i[0];
+    }
+}
+
+// Kept set of the reference's single-thread keep list (max_prob.c:185-203): the first
+// arg-max rf; r > rf kept if R + 14/beta > M; r < rf kept if R > M - 14/beta.
+__device__ __forceinline__ bool kept(double v, int r, int rf, double M, double c) {
+    if (r == rf) return true;
+    return r > rf ? (v + c > M) : (v > M - c);
+}
+
+// p_norm_d = sum_kept exp(beta (R - M)) (normalize_prob, max_prob.c:245-250).
+__global__ __launch_bounds__(kBlock) void frame_norm_kernel(
+    const double *__restrict__ R, size_t ldR, int num_rot, const double *__restrict__ beta,
+    const uint8_t *__restrict__ blacklist, const double *__restrict__ max_exp,
+    const int *__restrict__ rmax, double *__restrict__ p_norm) {
+    __shared__ double s[kBlock];
+    const int d = blockIdx.x;
+    const int rf = rmax[d];
+    if (blacklist[d] || rf < 0) {
+        if (threadIdx.x == 0) p_norm[d] = 0.;
+        return;
+    }
+    const double b = beta[d], M = max_exp[d], c = kPdiffThresh / b;
+    const double *row = R + (size_t)d * ldR;
+    double acc = 0.;
+    for (int r = threadIdx.x; r < num_rot; r += kBlock) {
+        const double v = row[r];
+        if (kept(v, r, rf, M, c)) acc += exp(b * (v - M));
+    }
+    const double tot = block_sum(acc, s);
+    if (threadIdx.x == 0) p_norm[d] = tot;
+}
+
+// Posterior and per-frame statistics (calc_psum_r, max_update.c:95-130):
+// P = exp(beta (R - M)) / p_norm on the kept set, 0 elsewhere (written in place);
+// likelihood += P (R - sum_fact); psum_d += P u_r rescale; info += P log(P / w_r) for P >= 1e-6.
+__global__ __launch_bounds__(kBlock) void posterior_kernel(
+    double *__restrict__ R, size_t ldR, int num_rot, const double *__restrict__ quats,
+    const double *__restrict__ u, const double *__restrict__ beta,
+    const double *__restrict__ sum_fact, const uint8_t *__restrict__ blacklist,
+    const double *__restrict__ max_exp, const int *__restrict__ rmax,
+    const double *__restrict__ p_norm, const double *__restrict__ scalars, int want_psum_d,
+    double *__restrict__ likelihood, double *__restrict__ info, double *__restrict__ psum_d) {
+    __shared__ double s[kBlock];
+    const int d = blockIdx.x;
+    double *row = R + (size_t)d * ldR;
+    const int rf = rmax[d];
+    if (blacklist[d] || rf < 0) {
+        for (int r = threadIdx.x; r < num_rot; r += kBlock) row[r] = 0.;
+        if (threadIdx.x == 0) {
+            likelihood[d] = 0.;
+            info[d] = 0.;
+            if (want_psum_d) psum_d[d] = 0.;
+        }
+        return;
+    }
+    const double b = beta[d], M = max_exp[d], c = kPdiffThresh / b, pn = p_norm[d];
+    const double sf = sum_fact[d], rescale = scalars[0];
+    double lk = 0., inf = 0., ps = 0.;
+    for (int r = threadIdx.x; r < num_rot; r += kBlock) {
+        const double v = row[r];
+        double prob = 0.;
+        if (kept(v, r, rf, M, c)) {
+            prob = exp(b * (v - M)) / pn;
+            lk += prob * (v - sf);
+            ps += prob * u[r] * rescale;
+            if (!(prob < kProbMin)) inf += prob * log(prob / quats[5 * (size_t)r + 4] * 1);
+        }
+        row[r] = prob;
+    }
+    const double tl = block_sum(lk, s);
+    const double ti = block_sum(inf, s);
+    const double tp = block_sum(ps, s);
+    if (threadIdx.x == 0) {
+        likelihood[d] = tl;
+        info[d] = ti;
+        if (want_psum_d) psum_d[d] = tp;
+    }
+}
+
+// psum_r = sum_d P_dr phi'_d, phi' = scale if need_scaling else 1 (max_update.c:108-116).
+__global__ __launch_bounds__(kBlock) void psum_r_partial_kernel(
+    const double *__restrict__ P, size_t ldP, int num_rot, int num_data, int frames_per_part,
+    const double *__restrict__ scale, int need_scaling, const uint8_t *__restrict__ blacklist,
+    double *__restrict__ part) {
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    const int part_id = blockIdx.y;
+    if (r >= num_rot) return;
+    const int d0 = part_id * frames_per_part, d1 = min(num_data, d0 + frames_per_part);
+    double acc = 0.;
+    for (int d = d0; d < d1; ++d) {
+        if (blacklist[d]) continue;
+        const double p = P[(size_t)d * ldP + r];
+        acc += need_scaling ? p * scale[d] : p;
+    }
+    part[(size_t)part_id * num_rot + r] = acc;
+}
+
+__global__ void psum_r_reduce_kernel(const double *__restrict__ part, int nparts, int num_rot,
+                                     double *__restrict__ psum_r) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= num_rot) return;
+    double acc = 0.;
+    for (int i = 0; i < nparts; ++i) acc += part[(size_t)i * num_rot + r];
+    psum_r[r] = acc;
+}
+
+// ---- tomogram: view_rt = sum_{d: P > 1e-6} P_dr K_dt / psum_r (update_tomogram_nobg,
+// max_update.c:142-230), gathered per pixel over its frames (d ascending, as the
+// reference accumulates) ---------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void tomogram_kernel(
+    const double *__restrict__ P, size_t ldP, int r0, int C, const long *__restrict__ toff,
+    const int *__restrict__ tframe, const double *__restrict__ tcnt,
+    const uint8_t *__restrict__ blacklist, const double *__restrict__ psum_r, int num_rot,
+    int num_pix, double *__restrict__ view) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * kWavesPerBlock + wave;
+    if (t >= num_pix) return;
+    const int c = blockIdx.y * kRTile + lane;
+    const double *__restrict__ col = P + r0 + c;
+    const long k1 = toff[t + 1];
+    double acc = 0.;
+    for (long k = toff[t]; k < k1; ++k) {
+        const int d = tframe[k];
+        if (blacklist[d]) continue;
+        const double p = col[(size_t)d * ldP];
+        if (p > kProbMin) acc += tcnt[k] * p;
+    }
+    const double ps = (r0 + c < num_rot) ? psum_r[r0 + c] : 0.;
+    if (ps > 0.) acc /= ps;
+    view[(size_t)t * C + c] = acc;
+}
+
+// ---- merge: scatter views into model2 / inter_weight (merge_tomogram, max_update.c:33-36;
+// slice_merge3d model.c:171-244), only orientations with psum_r > 0 and pixels mask <= 1 ----
+__global__ __launch_bounds__(kBlock) void merge_kernel(
+    const double *__restrict__ rots, int r0, int C, int num_rot, const double *__restrict__ view,
+    const double *__restrict__ psum_r, const double *__restrict__ qvals,
+    const double *__restrict__ corr, const uint8_t *__restrict__ mask, int num_pix,
+    int pix_per_group, long size, double *__restrict__ model2, double *__restrict__ weight) {
+    __shared__ double s_rot[kRTile * 9];
+    __shared__ double s_tile[64][kRTile + 1];
+    __shared__ int s_on[kRTile];
+    const int tid = threadIdx.x;
+    const int c0 = blockIdx.x * kRTile;
+    const int g = blockIdx.y;
+    for (int i = tid; i < kRTile * 9; i += kBlock) s_rot[i] = rots[(size_t)(r0 + c0) * 9 + i];
+    if (tid < kRTile) {
+        const int r = r0 + c0 + tid;
+        s_on[tid] = (r < num_rot) && (psum_r[r] > 0.);
+    }
+    __syncthreads();
+    const double center = (double)(size / 2);
+    const int lane = tid & 63, wave = tid >> 6;
+    const int t_begin = g * pix_per_group;
+    const int t_end = min(num_pix, t_begin + pix_per_group);
+    for (int tb = t_begin; tb < t_end; tb += 64) {
+        for (int k = 0; k < 64 / kWavesPerBlock; ++k) {
+            const int p = wave + kWavesPerBlock * k;
+            if (tb + p < t_end) s_tile[p][lane] = view[(size_t)(tb + p) * C + c0 + lane];
+        }
+        __syncthreads();
+        const int t = tb + lane;
+        if (t < t_end && mask[t] <= 1) {
+            const double q0 = qvals[3 * (size_t)t], q1 = qvals[3 * (size_t)t + 1],
+                         q2 = qvals[3 * (size_t)t + 2], cr = corr[t];
+            for (int k = 0; k < kRTile / kWavesPerBlock; ++k) {
+                const int rr = wave + kWavesPerBlock * k;
+                if (!s_on[rr]) continue;
+                Cell cell;
+                if (!locate(&s_rot[rr * 9], q0, q1, q2, center, size, cell)) continue;
+                merge_value(model2, weight, size, cell, s_tile[lane][rr] / cr);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- single-view geometry (Model.slice_gen / slice_merge) -------------------------------
+__global__ void slice_gen_kernel(const double *__restrict__ rot, const double *__restrict__ model,
+                                 long size, const double *__restrict__ qvals,
+                                 const double *__restrict__ corr, int num_pix,
+                                 double *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_pix) return;
+    out[t] = slice_value(model, size, rot, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1],
+                         qvals[3 * (size_t)t + 2], corr[t], (double)(size / 2));
+}
+
+__global__ void slice_merge_kernel(const double *__restrict__ rot, const double *__restrict__ view,
+                                   const double *__restrict__ qvals,
+                                   const double *__restrict__ corr,
+                                   const uint8_t *__restrict__ mask, int num_pix, long size,
+                                   double *__restrict__ model2, double *__restrict__ weight) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_pix || mask[t] > 1) return;
+    Cell cell;
+    if (!locate(rot, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1], qvals[3 * (size_t)t + 2],
+                (double)(size / 2), size, cell))
+        return;
+    merge_value(model2, weight, size, cell, view[t] / corr[t]);
+}
+
+// ---- per-frame statistics and scale ------------------------------------------------------
+// sums of likelihood and info over non-blacklisted frames (max_prob.c:367-374).
+__global__ __launch_bounds__(kBlock) void frame_stats_kernel(
+    const double *__restrict__ likelihood, const double *__restrict__ info,
+    const uint8_t *__restrict__ blacklist, int num_data, int denom, double *__restrict__ scalars) {
+    __shared__ double s[kBlock];
+    double a = 0., b = 0.;
+    for (int d = threadIdx.x; d < num_data; d += kBlock)
+        if (!blacklist[d]) {
+            a += likelihood[d];
+            b += info[d];
+        }
+    const double ta = block_sum(a, s);
+    const double tb = block_sum(b, s);
+    if (threadIdx.x == 0) {
+        scalars[1] = ta / denom;
+        scalars[2] = tb / denom;
+    }
+}
+
+// update_scale: scale_d = fcounts_d / psum_d (max_scale.c:13-15).
+__global__ void scale_kernel(const int *__restrict__ fcounts, const double *__restrict__ psum_d,
+                             const uint8_t *__restrict__ blacklist, int num_data,
+                             double *__restrict__ scale) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= num_data || blacklist[d]) return;
+    scale[d] = fcounts[d] / psum_d[d];
+}
+
+__global__ void fill_kernel(double *__restrict__ a, size_t n, double v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+
+// ---- model update (EMCRecon.update_model, recon.pyx:112-149) ------------------------------
+// Friedel symmetrisation, model.c:470-490. Voxel pairs with z < center are each visited once
+// by the reference loop; pairs on the z == center plane are visited twice (once from each
+// end, in x-major order), which is reproduced here so the result is the same bit for bit.
+__global__ void friedel_offplane_kernel(double *__restrict__ a, const double *__restrict__ w,
+                                        int size) {
+    const int lo = (size % 2 == 0) ? 1 : 0, c = size / 2;
+    const int n = size - lo, nz = c - lo;
+    const long total = (long)n * n * nz;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total;
+         i += (long)gridDim.x * blockDim.x) {
+        const int z = lo + (int)(i % nz);
+        const int y = lo + (int)((i / nz) % n);
+        const int x = lo + (int)(i / ((long)nz * n));
+        const long v1 = (long)x * size * size + (long)y * size + z;
+        const long v2 = (long)(2 * c - x) * size * size + (long)(2 * c - y) * size + (2 * c - z);
+        double ai = 0.5 * (a[v1] * w[v1] + a[v2] * w[v2]);
+        const double aw = 0.5 * (w[v1] + w[v2]);
+        ai = (aw > 0) ? ai / aw : 0.;
+        a[v1] = ai;
+        a[v2] = ai;
+    }
+}
+
+__global__ void friedel_plane_kernel(double *__restrict__ a, const double *__restrict__ w, int size) {
+    const int lo = (size % 2 == 0) ? 1 : 0, c = size / 2;
+    const int n = size - lo;
+    const long total = (long)n * n;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total;
+         i += (long)gridDim.x * blockDim.x) {
+        const int y = lo + (int)(i % n), x = lo + (int)(i / n);
+        const int xm = 2 * c - x, ym = 2 * c - y;
+        // this thread owns the pair if (x, y) is visited first
+        if (xm < x || (xm == x && ym < y)) continue;
+        const long v1 = (long)x * size * size + (long)y * size + c;
+        const long v2 = (long)xm * size * size + (long)ym * size + c;
+        double ai = 0.5 * (a[v1] * w[v1] + a[v2] * w[v2]);
+        double aw = 0.5 * (w[v1] + w[v2]);
+        ai = (aw > 0) ? ai / aw : 0.;
+        a[v1] = ai;
+        a[v2] = ai;
+        if (v1 == v2) continue;
+        ai = 0.5 * (a[v2] * w[v2] + a[v1] * w[v1]);
+        aw = 0.5 * (w[v2] + w[v1]);
+        ai = (aw > 0) ? ai / aw : 0.;
+        a[v2] = ai;
+        a[v1] = ai;
+    }
+}
+
+// model2 /= weight where weight > 0; partial sums of (model2 - model1)^2; blend into model1.
+__global__ __launch_bounds__(kBlock) void model_divide_blend_kernel(
+    double *__restrict__ model1, double *__restrict__ model2, const double *__restrict__ w,
+    long vol, double alpha, double *__restrict__ part) {
+    __shared__ double s[kBlock];
+    double acc = 0.;
+    for (long x = blockIdx.x * (long)kBlock + threadIdx.x; x < vol; x += (long)gridDim.x * kBlock) {
+        double m2 = model2[x];
+        if (w[x] > 0.) m2 /= w[x];
+        model2[x] = m2;
+        const double diff = m2 - model1[x];
+        acc += diff * diff;
+        model1[x] = (alpha > 0.) ? alpha * model1[x] + (1 - alpha) * m2 : m2;
+    }
+    const double t = block_sum(acc, s);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kBlock) void sum_parts_kernel(const double *__restrict__ part, int n,
+                                                           double *__restrict__ out) {
+    __shared__ double s[kBlock];
+    double acc = 0.;
+    for (int i = threadIdx.x; i < n; i += kBlock) acc += part[i];
+    const double t = block_sum(acc, s);
+    if (threadIdx.x == 0) *out = t;
+}
+
+// Iterate.normalize_scale (iterate.pyx:361-384)
+__global__ __launch_bounds__(kBlock) void scale_mean_kernel(const double *__restrict__ scale,
+                                                            const uint8_t *__restrict__ blacklist,
+                                                            int num_data, double *__restrict__ out) {
+    __shared__ double s[kBlock];
+    __shared__ double sc[kBlock];
+    double acc = 0., cnt = 0.;
+    for (int d = threadIdx.x; d < num_data; d += kBlock)
+        if (!blacklist[d]) {
+            acc += scale[d];
+            cnt += 1.;
+        }
+    const double t = block_sum(acc, s);
+    const double n = block_sum(cnt, sc);
+    if (threadIdx.x == 0) *out = t / n;
+}
+
+__global__ void scale_divide_kernel(double *__restrict__ scale, const uint8_t *__restrict__ blacklist,
+                                    int num_data, const double *__restrict__ mean) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < num_data && !blacklist[d]) scale[d] /= *mean;
+}
+
+__global__ void volume_scale_kernel(double *__restrict__ a, long n, const double *__restrict__ f) {
+    const double v = *f;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        a[i] *= v;
+}
+
+}  // namespace emc
+
+// ---- launchers (host) --------------------------------------------------------------------
+namespace emc {
+
+static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+static inline unsigned grid_cap(size_t blocks) { return (unsigned)(blocks < 65536 ? blocks : 65536); }
+
+void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
+                   const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
+                   int r0, int C, int pix_per_group, double *logw, double *upart) {
+    dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
+    expand_kernel<<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0, C,
+                                          pix_per_group, logw, upart);
+}
+
+void launch_u_reduce(hipStream_t s, const double *upart, int ngroups, int C, int r0, int num_rot,
+                     double *u) {
+    u_reduce_kernel<<<cdiv(C, 256), 256, 0, s>>>(upart, ngroups, C, r0, num_rot, u);
+}
+
+void launch_rescale(hipStream_t s, const double *quats, const double *u, int num_rot,
+                    double mean_count, double *scalars) {
+    rescale_kernel<<<1, kBlock, 0, s>>>(quats, u, num_rot, mean_count, scalars);
+}
+
+void launch_likelihood(hipStream_t s, const double *logw, int C, const long *off, const int *pix,
+                       const double *cnt, const uint8_t *blacklist, int num_data, double *G,
+                       size_t ldG, int r0) {
+    dim3 grid(cdiv(num_data, kWavesPerBlock), C / kRTile);
+    likelihood_kernel<<<grid, kBlock, 0, s>>>(logw, C, off, pix, cnt, blacklist, num_data, G, ldG, r0);
+}
+
+void launch_frame_max(hipStream_t s, double *R, size_t ldR, int num_rot, int num_data,
+                      const double *logwr, const double *u, const double *scale, int use_scale,
+                      const double *scalars, const uint8_t *blacklist, double *max_exp, int *rmax) {
+    frame_max_kernel<<<num_data, kBlock, 0, s>>>(R, ldR, num_rot, logwr, u, scale, use_scale,
+                                                 scalars, blacklist, max_exp, rmax);
+}
+
+void launch_frame_norm(hipStream_t s, const double *R, size_t ldR, int num_rot, int num_data,
+                       const double *beta, const uint8_t *blacklist, const double *max_exp,
+                       const int *rmax, double *p_norm) {
+    frame_norm_kernel<<<num_data, kBlock, 0, s>>>(R, ldR, num_rot, beta, blacklist, max_exp, rmax,
+                                                  p_norm);
+}
+
+void launch_posterior(hipStream_t s, double *R, size_t ldR, int num_rot, int num_data,
+                      const double *quats, const double *u, const double *beta,
+                      const double *sum_fact, const uint8_t *blacklist, const double *max_exp,
+                      const int *rmax, const double *p_norm, const double *scalars,
+                      int want_psum_d, double *likelihood, double *info, double *psum_d) {
+    posterior_kernel<<<num_data, kBlock, 0, s>>>(R, ldR, num_rot, quats, u, beta, sum_fact,
+                                                 blacklist, max_exp, rmax, p_norm, scalars,
+                                                 want_psum_d, likelihood, info, psum_d);
+}
+
+void launch_psum_r(hipStream_t s, const double *P, size_t ldP, int num_rot, int num_data,
+                   int frames_per_part, const double *scale, int need_scaling,
+                   const uint8_t *blacklist, double *part, double *psum_r) {
+    const int nparts = (int)cdiv(num_data, frames_per_part);
+    dim3 grid(cdiv(num_rot, kBlock), nparts);
+    psum_r_partial_kernel<<<grid, kBlock, 0, s>>>(P, ldP, num_rot, num_data, frames_per_part,
+                                                  scale, need_scaling, blacklist, part);
+    psum_r_reduce_kernel<<<cdiv(num_rot, 256), 256, 0, s>>>(part, nparts, num_rot, psum_r);
+}
+
+void launch_tomogram(hipStream_t s, const double *P, size_t ldP, int r0, int C, const long *toff,
+                     const int *tframe, const double *tcnt, const uint8_t *blacklist,
+                     const double *psum_r, int num_rot, int num_pix, double *view) {
+    dim3 grid(cdiv(num_pix, kWavesPerBlock), C / kRTile);
+    tomogram_kernel<<<grid, kBlock, 0, s>>>(P, ldP, r0, C, toff, tframe, tcnt, blacklist, psum_r,
+                                            num_rot, num_pix, view);
+}
+
+void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot, const double *view,
+                  const double *psum_r, const double *qvals, const double *corr,
+                  const uint8_t *mask, int num_pix, int pix_per_group, long size, double *model2,
+                  double *weight) {
+    dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
+    merge_kernel<<<grid, kBlock, 0, s>>>(rots, r0, C, num_rot, view, psum_r, qvals, corr, mask,
+                                         num_pix, pix_per_group, size, model2, weight);
+}
+
+void launch_slice_gen(hipStream_t s, const double *rot, const double *model, long size,
+                      const double *qvals, const double *corr, int num_pix, double *out) {
+    slice_gen_kernel<<<cdiv(num_pix, 256), 256, 0, s>>>(rot, model, size, qvals, corr, num_pix, out);
+}
+
+void launch_slice_merge(hipStream_t s, const double *rot, const double *view, const double *qvals,
+                        const double *corr, const uint8_t *mask, int num_pix, long size,
+                        double *model2, double *weight) {
+    slice_merge_kernel<<<cdiv(num_pix, 256), 256, 0, s>>>(rot, view, qvals, corr, mask, num_pix,
+                                                          size, model2, weight);
+}
+
+void launch_frame_stats(hipStream_t s, const double *likelihood, const double *info,
+                        const uint8_t *blacklist, int num_data, int denom, double *scalars) {
+    frame_stats_kernel<<<1, kBlock, 0, s>>>(likelihood, info, blacklist, num_data, denom, scalars);
+}
+
+void launch_scale(hipStream_t s, const int *fcounts, const double *psum_d, const uint8_t *blacklist,
+                  int num_data, double *scale) {
+    scale_kernel<<<cdiv(num_data, 256), 256, 0, s>>>(fcounts, psum_d, blacklist, num_data, scale);
+}
+
+void launch_fill(hipStream_t s, double *a, size_t n, double v) {
+    fill_kernel<<<grid_cap(cdiv(n, 256)), 256, 0, s>>>(a, n, v);
+}
+
+void launch_update_model(hipStream_t s, double *model1, double *model2, const double *w, long size,
+                         double alpha, double *part, int nparts, double *out_change) {
+    const long vol = size * size * size;
+    friedel_offplane_kernel<<<grid_cap(cdiv(vol / 2 + 1, 256)), 256, 0, s>>>(model2, w, (int)size);
+    friedel_plane_kernel<<<cdiv((size_t)size * size, 256), 256, 0, s>>>(model2, w, (int)size);
+    model_divide_blend_kernel<<<nparts, kBlock, 0, s>>>(model1, model2, w, vol, alpha, part);
+    sum_parts_kernel<<<1, kBlock, 0, s>>>(part, nparts, out_change);
+}
+
+void launch_normalize_scale(hipStream_t s, double *scale, const uint8_t *blacklist, int num_data,
+                            double *model1, long vol, double *mean_out) {
+    scale_mean_kernel<<<1, kBlock, 0, s>>>(scale, blacklist, num_data, mean_out);
+    volume_scale_kernel<<<grid_cap(cdiv(vol, 256)), 256, 0, s>>>(model1, vol, mean_out);
+    scale_divide_kernel<<<cdiv(num_data, 256), 256, 0, s>>>(scale, blacklist, num_data, mean_out);
+}
+
+}  // namespace emc

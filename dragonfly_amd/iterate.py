@@ -1,0 +1,266 @@
+"""Iterate: the per-iteration state (mirror of dragonfly/iterate.pyx:21-640).
+
+Holds parameters, detector, photon data, orientations, model and the per-frame arrays, and
+computes the per-frame precompute of dragonfly/src/iterate.c (frame counts, mean count,
+sum of log-factorials, beta).
+"""
+import configparser
+import math
+import os.path as op
+
+import numpy as np
+
+from .config import MyConfigParser
+from .detector import CDetector
+from .emcfile import CDataset
+from .model import Model
+from .params import EMCParams
+from .quaternion import Quaternion
+
+_LNFACT = None
+
+
+def lnfact(counts):
+    """log(n!) as GSL's gsl_sf_lnfact: log of the double-rounded n! for n <= 170, else
+    lgamma(n + 1)."""
+    global _LNFACT
+    if _LNFACT is None:
+        _LNFACT = np.array([math.log(float(math.factorial(n))) for n in range(171)])
+    counts = np.asarray(counts, dtype='i8')
+    out = np.empty(counts.shape)
+    small = counts <= 170
+    out[small] = _LNFACT[counts[small]]
+    out[~small] = [math.lgamma(n + 1.) for n in counts[~small]]
+    return out
+
+
+def frame_counts(ones, multi, place_ones, place_multi, count_multi, mask):
+    """fcounts: photons per frame on mask < 1 pixels (calc_frame_counts, iterate.c:24-35)."""
+    frame_o = np.repeat(np.arange(len(ones)), ones)
+    frame_m = np.repeat(np.arange(len(multi)), multi)
+    good_o = mask[place_ones] < 1
+    good_m = mask[place_multi] < 1
+    f = np.bincount(frame_o[good_o], minlength=len(ones)).astype('i8')
+    f += np.bincount(frame_m[good_m], weights=count_multi[good_m], minlength=len(ones)).astype('i8')
+    return f.astype('i4')
+
+
+def sum_log_factorials(multi, place_multi, count_multi, mask):
+    """sum_fact: sum of log(K!) over a frame's multi-photon pixels with mask < 1
+    (calc_sum_fact, iterate.c:85-117), accumulated in pixel order."""
+    out = np.zeros(len(multi))
+    frame_m = np.repeat(np.arange(len(multi)), multi)
+    good = mask[place_multi] < 1
+    lf = lnfact(count_multi)
+    np.add.at(out, frame_m[good], lf[good])
+    return out
+
+
+class Iterate:
+    """State for each EMC iteration. Construct from a config file, or set the pieces with
+    set_params / set_data / set_quat / set_model."""
+
+    def __init__(self, config_fname='', section_name='emc', resume=False):
+        self.params = EMCParams()
+        self.dets = []
+        self.data = None
+        self.quat = None
+        self.model = None
+        self.tot_num_data = 0
+        self.num_blacklist = 0
+        self.fcounts = self.scale = self.bgscale = None
+        self.beta = self.beta_start = self.sum_fact = self.blacklist = None
+        self.mean_count = None
+        self.rescale = None
+        self.likelihood = self.mutual_info = self.rms_change = 0.
+        if config_fname != '':
+            self.from_config(config_fname, section_name, resume=resume)
+
+    # ---- configuration pipeline (iterate.pyx:40-182) ----------------------------------
+    def from_config(self, config_fname, section_name='emc', resume=False):
+        param = EMCParams().from_config(config_fname, section_name)
+        self.set_params(param)
+        config_folder = op.dirname(config_fname)
+        config = MyConfigParser()
+        config.read(config_fname)
+        det_fname = config.get(section_name, 'in_detector_file', fallback=None)
+        if det_fname is None:
+            raise ValueError('Need in_detector_file (in_detector_list is not implemented).')
+        if ':::' in det_fname:
+            sname, oname = det_fname.split(':::')
+            det_fname = config.get(sname, oname)
+        det = CDetector(op.join(config_folder, det_fname), norm=True)
+        ph_fname = config.get(section_name, 'in_photons_file', fallback=None)
+        ph_flist = config.get(section_name, 'in_photons_list', fallback=None)
+        if ph_fname is not None and ph_flist is not None:
+            raise ValueError('Both in_photons_file and in_photons_list specified. Pick one.')
+        if ph_fname is not None:
+            if ':::' in ph_fname:
+                sname, oname = ph_fname.split(':::')
+                ph_fname = config.get(sname, oname)
+            frames = CDataset(op.join(config_folder, ph_fname), det)
+        elif ph_flist is not None:
+            if ':::' in ph_flist:
+                sname, oname = ph_flist.split(':::')
+                ph_flist = config.get(sname, oname)
+            with open(ph_flist) as fptr:
+                fnames = [op.join(config_folder, line.strip()) for line in fptr if line.strip()]
+            frames = CDataset(fnames[0], det)
+            for fn in fnames[1:]:
+                frames.append(CDataset(fn, det))
+        else:
+            raise ValueError('Need either in_photons_file or in_photons_list.')
+        self.dets = [det]
+        self.set_data(frames)
+        self.set_quat(Quaternion().from_config(config_fname, section_name))
+        try:
+            model_file = config.get_filename(section_name, 'start_model_file')
+        except configparser.NoOptionError:
+            model_file = ''
+        try:
+            scale_file = config.get_filename(section_name, 'scale_file')
+        except configparser.NoOptionError:
+            scale_file = ''
+        if resume:
+            with open(param.log_fname) as fp:
+                last_iter = int(fp.readlines()[-1].split()[0])
+            self.params.start_iter = last_iter + 1
+            model_file = op.join(param.output_folder, 'output_%.3d.npz' % last_iter)
+            scale_file = model_file
+        model = Model(self.calculate_size(max(d.qmax() for d in self.dets)), 1)
+        model_mean = self.mean_count[0] / (self.dets[0].raw_mask == 0).sum() * 2.
+        if model_file.endswith('.npz') and op.isfile(model_file):
+            with np.load(model_file) as f:
+                model.allocate('', model_mean)
+                model.model1 = np.ascontiguousarray(f['intens'].reshape((model.size,) * 3))
+        else:
+            model.allocate(model_file, model_mean, fixed_seed=bool(param.fixed_seed))
+        self.set_model(model)
+        if self.params.need_scaling == 1:
+            self.parse_scale(scale_file)
+        sel_string = config.get(section_name, 'selection', fallback=None)
+        try:
+            blacklist_file = config.get_filename(section_name, 'blacklist_file')
+        except configparser.NoOptionError:
+            blacklist_file = ''
+        self.parse_blacklist(blacklist_file, sel_string)
+        beta_str = config.get(section_name, 'beta', fallback='auto')
+        self.calc_beta(None if beta_str == 'auto' else float(beta_str))
+        return self
+
+    def set_params(self, param):
+        self.params = param
+
+    def set_quat(self, quaternion):
+        self.quat = quaternion
+
+    def set_model(self, model):
+        self.model = model
+
+    def set_data(self, in_dset):
+        """Attach photon data and compute the per-frame precompute (iterate.pyx:241-266)."""
+        if self.tot_num_data != 0:
+            raise ValueError('Please use update_data() to change dataset')
+        total = 0
+        for ds in in_dset:
+            ds.num_offset = total
+            total += ds.num_data
+            if ds.ftype != 'sparse':
+                raise NotImplementedError('only sparse photon data are implemented')
+        self.tot_num_data = total
+        self.data = in_dset
+        if not self.dets:
+            self.dets = [in_dset.det]
+        if any(ds.det is not self.dets[0] for ds in in_dset):
+            raise NotImplementedError('all datasets must share one detector')
+        mask = self.dets[0].raw_mask
+        sp = self.sparse_arrays()
+        self.fcounts = frame_counts(sp['ones'], sp['multi'], sp['place_ones'], sp['place_multi'],
+                                    sp['count_multi'], mask)
+        self.mean_count = np.array([self.fcounts.astype('f8').sum() / total])
+        self.rescale = np.zeros(1)
+        self.sum_fact = sum_log_factorials(sp['multi'], sp['place_multi'], sp['count_multi'], mask)
+        self.blacklist = np.zeros(total, dtype='u1')
+
+    def sparse_arrays(self):
+        """All datasets concatenated into one frame list (linked list order)."""
+        parts = list(self.data)
+        cat = {k: np.concatenate([getattr(p, k) for p in parts]).astype('i4')
+               for k in ('ones', 'multi', 'place_ones', 'place_multi', 'count_multi')}
+        from .emcfile import _accum
+        cat['ones_accum'] = _accum(cat['ones'])
+        cat['multi_accum'] = _accum(cat['multi'])
+        return cat
+
+    def parse_scale(self, fname, bg=False):
+        """Scale factors from a file, or ones (iterate.pyx:284-315)."""
+        if bg:
+            raise NotImplementedError('background scale factors are not implemented')
+        if not fname or not op.isfile(fname):
+            if self.tot_num_data == 0:
+                raise AttributeError('Need tot_num_data to initialize scale factors')
+            scale = np.ones(self.tot_num_data)
+        elif fname.endswith('.npz'):
+            with np.load(fname) as f:
+                scale = np.asarray(f['scale'], dtype='f8')
+        else:
+            scale = np.loadtxt(fname, dtype='f8').ravel()
+        if self.tot_num_data < scale.shape[0]:
+            print('More scale factors than required. Taking only first', self.tot_num_data)
+            scale = scale[:self.tot_num_data]
+        elif self.tot_num_data > scale.shape[0]:
+            print('Insufficient scale factors in file. Setting rest to unity')
+            scale = np.append(scale, np.ones(self.tot_num_data - scale.shape[0]))
+        self.scale = np.ascontiguousarray(scale, dtype='f8').copy()
+
+    def parse_blacklist(self, fname, sel_string=None, refresh=False):
+        """Blacklist from a 0/1 text file and/or odd_only / even_only (iterate.pyx:317-359)."""
+        if self.tot_num_data == 0:
+            raise AttributeError('Need to define tot_num_data before generating blacklist')
+        if self.blacklist is None or refresh:
+            self.blacklist = np.zeros(self.tot_num_data, dtype='u1')
+        if fname and op.isfile(fname):
+            arr = np.loadtxt(fname, dtype='u1')
+            if arr.shape[0] != self.tot_num_data:
+                raise ValueError('Mismatched number of frames in blacklist file')
+            self.blacklist[:] = arr
+        if sel_string is None:
+            return
+        if sel_string == 'odd_only':
+            curr = 0
+        elif sel_string == 'even_only':
+            curr = 1
+        else:
+            raise ValueError('Unrecognized sel_string, %s' % sel_string)
+        for d in range(self.tot_num_data):
+            if self.blacklist[d] == 0:
+                self.blacklist[d] = curr
+                curr = 1 - curr
+
+    def normalize_scale(self):
+        """Mean-normalise scale factors into the model (iterate.pyx:361-384), host arrays."""
+        good = self.blacklist == 0
+        mean_scale = self.scale[good].mean()
+        if self.model is not None and self.model.model1 is not None:
+            self.model.model1 *= mean_scale
+        self.scale[good] /= mean_scale
+        self.rms_change *= mean_scale
+        return mean_scale
+
+    def calc_beta(self, setval=None):
+        """beta_start per frame (calc_beta, iterate.c:66-83)."""
+        if self.data is None:
+            raise AttributeError('Add data first before calculating beta for each frame')
+        start = -1. if setval is None else float(setval)
+        if not self.params.need_scaling and start < 0:
+            start = math.exp(-6.5 * math.pow(self.mean_count[0] * 1.e-5, 0.15))
+        if start > 0:
+            self.beta_start = np.full(self.tot_num_data, start)
+        else:
+            self.beta_start = np.array([math.exp(-6.5 * math.pow(f * 1.e-5, 0.15))
+                                        for f in self.fcounts.tolist()])
+        self.beta = self.beta_start.copy()
+
+    @staticmethod
+    def calculate_size(qmax):
+        return int(2 * np.ceil(qmax) + 3)

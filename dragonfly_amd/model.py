@@ -1,0 +1,78 @@
+"""Model: the 3D intensity volume (mirror of dragonfly/model.pyx:18-313, 3D, one mode)."""
+import os
+
+import numpy as np
+
+
+class Model:
+    """model1 (current estimate), model2 (merged numerator) and inter_weight, each
+    (size, size, size) float64, index x*size*size + y*size + z as the reference."""
+
+    def __init__(self, size=0, num_modes=1, model_type='3d'):
+        if model_type.lower() != '3d':
+            raise NotImplementedError('only 3d models are implemented')
+        if num_modes != 1:
+            raise NotImplementedError('num_modes > 1 is not implemented')
+        self.mtype = 'MODEL_3D'
+        self.num_modes = num_modes
+        self.size = int(size)
+        self.model1 = self.model2 = self.inter_weight = None
+        self._engine = None
+        self._stale = False
+
+    @property
+    def center(self):
+        return self.size // 2
+
+    @property
+    def vol(self):
+        return self.size ** 3
+
+    def allocate(self, fname='', model_mean=1., rank=0, fixed_seed=False):
+        """Load model1 from a raw float64 file, or draw a random start (model.pyx:43-103):
+        uniform [0, model_mean), numpy's legacy generator seeded 0x5EED when fixed_seed."""
+        shape = (self.size,) * 3
+        self.model2 = np.zeros(shape)
+        self.inter_weight = np.zeros(shape)
+        if fname and os.path.isfile(fname):
+            if fname.endswith('.npy'):
+                arr = np.load(fname)
+            else:
+                raw = np.fromfile(fname, dtype='f8')
+                fsize = int(np.rint(raw.size ** (1 / 3.)))
+                if fsize != self.size:
+                    raise ValueError('Wrong volume size in binary file (%d)' % fsize, fsize)
+                arr = raw
+            self.model1 = np.ascontiguousarray(arr, dtype='f8').reshape(shape).copy()
+        else:
+            rs = np.random.RandomState(0x5EED) if fixed_seed else np.random
+            self.model1 = rs.random_sample(shape) * model_mean
+        return self
+
+    # device mirror: the engine owns the live copies while a reconstruction runs
+    def _bind(self, engine):
+        self._engine = engine
+        self._stale = False
+
+    def _mark_stale(self):
+        self._stale = True
+
+    def sync(self):
+        """Refresh the host arrays from the engine after an iteration."""
+        if self._engine is not None and self._stale:
+            self.model1 = self._engine.get('model1')
+            self.model2 = self._engine.get('model2')
+            self.inter_weight = self._engine.get('inter_weight')
+            self._stale = False
+        return self
+
+    def slice_gen(self, quat, det, engine):
+        """View of model1 at one orientation (Model.slice_gen, model.pyx:105-131), on the GPU."""
+        engine.set_detector(det.qvals, det.corr, det.raw_mask)
+        engine.set_model(self.model1)
+        return engine.slice_gen(quat)
+
+    def slice_merge(self, quat, view, det, engine):
+        """Merge one view into the engine's model2/inter_weight (model.pyx:133-150)."""
+        engine.set_detector(det.qvals, det.corr, det.raw_mask)
+        engine.slice_merge(quat, view)

@@ -1,0 +1,146 @@
+/*
+ * emc_hip.h -- C ABI of the MI355X EMC engine (libemc_hip.so).
+ *
+ * One engine handle holds, in HBM, everything one EMC iteration reads and writes:
+ * detector geometry, orientation samples, the 3D intensity model, the sparse photon
+ * frames and the per-frame state. emc_maximize() runs one full E+M step on the GPU
+ * (expand -> likelihood -> normalise -> posterior -> tomogram -> merge -> scale) and
+ * replaces the reference's C entry point
+ *     double maximize(struct max_data*)            dragonfly/src/maximize.h:45, maximize.c:8-56
+ * The setters replace the fields the reference's Cython objects fill before calling it
+ * (struct iterate, dragonfly/src/iterate.h:11-38, filled by dragonfly/iterate.pyx:40-182);
+ * the getters replace the numpy-view properties of EMCRecon (dragonfly/recon.pyx:294-415).
+ *
+ * Conventions: plain pointers and sizes; all input pointers are HOST pointers that are
+ * copied into engine-owned device memory; getters copy device results into caller host
+ * buffers. Every call returns 0 on success or a negative EMC_E* code; emc_last_error()
+ * gives the message of the last failure on the calling thread. A handle is used by one
+ * caller at a time (not re-entrant); calls on it are ordered on the engine's stream and
+ * every getter synchronises that stream.
+ */
+#ifndef EMC_HIP_H
+#define EMC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMC_OK 0
+#define EMC_E_ARG (-1)      /* bad argument (null pointer, size mismatch, out of range) */
+#define EMC_E_STATE (-2)    /* a required setter has not been called */
+#define EMC_E_HIP (-3)      /* HIP runtime error (allocation, launch, copy) */
+#define EMC_E_NOMEM (-4)    /* device memory budget exceeded */
+
+typedef struct emc_engine emc_engine;
+
+/* Options of one iteration; the fields of struct params that maximize() reads
+ * (dragonfly/src/params.h, set from config by dragonfly/params.pyx:67-128). */
+typedef struct emc_iter_opts {
+    int iteration;      /* params.iteration, 1-based (max_prob.c:124) */
+    int need_scaling;   /* params.need_scaling (max_prob.c:124, max_update.c:108) */
+    int update_scale;   /* params.update_scale (maximize.c:151) */
+    int known_scale;    /* params.known_scale (max_prob.c:124) */
+} emc_iter_opts;
+
+/* Results of one iteration that the reference returns or stores in struct iterate. */
+typedef struct emc_iter_stats {
+    double avg_likelihood; /* return value of maximize() (max_prob.c:370-374) */
+    double mutual_info;    /* iter->mutual_info (max_prob.c:369-373) */
+    double rescale;        /* iter->rescale[0] (max_prob.c:59) */
+    double stage_ms[8];    /* expand, likelihood, rescale, max, norm, posterior, tomogram, merge+scale */
+    int num_chunks;        /* orientation chunks the iteration was split into (launches per stage) */
+} emc_iter_stats;
+
+/* Per-frame / per-orientation / volume fields readable with emc_get_field(). */
+enum emc_field {
+    EMC_FIELD_U = 0,          /* f64[num_rot]   u[0][r]      (max_prob.c:38)  recon.pyx:371 */
+    EMC_FIELD_MAX_EXP,        /* f64[num_data]  max_exp      (max_prob.c:234) recon.pyx:363 */
+    EMC_FIELD_P_NORM,         /* f64[num_data]  p_norm       (max_prob.c:245) recon.pyx:367 */
+    EMC_FIELD_RMAX,           /* i32[num_data]  rmax         (max_prob.c:237) recon.pyx:397 */
+    EMC_FIELD_LIKELIHOOD,     /* f64[num_data]  likelihood   (max_update.c:105) recon.pyx:393 */
+    EMC_FIELD_INFO,           /* f64[num_data]  info         (max_update.c:128) recon.pyx:389 */
+    EMC_FIELD_PSUM_D,         /* f64[num_data]  psum_d       (max_update.c:113) recon.pyx:358 */
+    EMC_FIELD_SCALE,          /* f64[num_data]  iter->scale  (max_scale.c:15) */
+    EMC_FIELD_MODEL1,         /* f64[size^3]    mod->model1 */
+    EMC_FIELD_MODEL2,         /* f64[size^3]    mod->model2  (max_prob.c:281) */
+    EMC_FIELD_INTER_WEIGHT,   /* f64[size^3]    mod->inter_weight (max_prob.c:282) */
+    EMC_FIELD_PSUM_R,         /* f64[num_rot]   psum_r per orientation (max_update.c:109,116) */
+    EMC_FIELD_PROB,           /* f64[num_data][num_rot] posterior P_dr, 0 where not kept
+                                 (dense form of max_data.prob/place_prob, recon.pyx:409-415) */
+    EMC_FIELD_COUNT
+};
+
+/* ---- lifetime ------------------------------------------------------------------------ */
+int emc_create(int device, emc_engine **out);
+void emc_destroy(emc_engine *e);
+const char *emc_last_error(void);
+int emc_synchronize(emc_engine *e);
+
+/* ---- orientation sampling (host; no GPU needed) --------------------------------------
+ * quat_gen(num_div, quat)   dragonfly/src/quaternion.c:811-842, bit-exact.
+ * quats_out receives emc_num_rot(num_div) rows of (q0, q1, q2, q3, weight).            */
+int emc_num_rot(int num_div);
+int emc_quat_gen(int num_div, double *quats_out);
+
+/* ---- inputs ----------------------------------------------------------------------------
+ * detector: struct detector qvals/corr/raw_mask  (dragonfly/src/detector.h:6-16,
+ *           filled by dragonfly/detector.pyx:31-74 / setters :161-206)               */
+int emc_set_detector(emc_engine *e, int num_pix, const double *qvals, const double *corr,
+                     const uint8_t *raw_mask);
+/* orientations: struct quaternion quats[num_rot][5] (dragonfly/src/quaternion.h:11-15) */
+int emc_set_quats(emc_engine *e, int num_rot, const double *quats);
+/* model: struct model model1[size^3], one mode (dragonfly/src/model.h:15-20,
+ *        Model.allocate dragonfly/model.pyx:43-103)                                   */
+int emc_set_model(emc_engine *e, long size, const double *model1);
+/* sparse photon frames: struct dataset ones, multi, place_ones, place_multi, count_multi
+ *        (dragonfly/src/emcfile.h:12-32, parse_binarydataset emcfile.c:3-57)          */
+int emc_set_frames_sparse(emc_engine *e, int num_data, const int *ones, const int *multi,
+                          const long *ones_accum, const long *multi_accum,
+                          const int *place_ones, const int *place_multi,
+                          const int *count_multi);
+/* per-frame state of struct iterate: scale, beta, blacklist, fcounts, sum_fact,
+ *        mean_count[0], num_blacklist (iterate.h:17-24,33; iterate.c:3-117)          */
+int emc_set_frame_params(emc_engine *e, const double *scale, const double *beta,
+                         const uint8_t *blacklist, const int *fcounts,
+                         const double *sum_fact, double mean_count, int num_blacklist);
+/* per-iteration beta written by EMCRecon.update_beta (recon.pyx:151-174)             */
+int emc_set_beta(emc_engine *e, const double *beta);
+/* scale factors (Iterate.parse_scale, iterate.pyx:284-315)                             */
+int emc_set_scale(emc_engine *e, const double *scale);
+
+/* ---- the iteration --------------------------------------------------------------------
+ * maximize(struct max_data*)  dragonfly/src/maximize.c:8-56. Writes model2,
+ * inter_weight, the per-frame fields, and (need_scaling && update_scale) scale.         */
+int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats);
+
+/* EMCRecon.update_model, 3D point group 1 (dragonfly/recon.pyx:112-149 with
+ * symmetrize_friedel dragonfly/src/model.c:470-490): Friedel-symmetrise model2 with
+ * inter_weight, divide, rms change, model1 = alpha*model1 + (1-alpha)*model2.          */
+int emc_update_model(emc_engine *e, double alpha, double *rms_change);
+/* Iterate.normalize_scale (dragonfly/iterate.pyx:361-384): mean of scale over
+ * non-blacklisted frames; model1 *= mean; scale /= mean. Returns the mean.             */
+int emc_normalize_scale(emc_engine *e, double *mean_scale);
+
+/* ---- geometry on the device ----------------------------------------------------------
+ * Model.slice_gen (dragonfly/model.pyx:105-131 -> slice_gen3d model.c:11-59): view of
+ * model1 at one quaternion.                                                             */
+int emc_slice_gen(emc_engine *e, const double *quat, double *view_out);
+/* Model.slice_merge (dragonfly/model.pyx:133-150 -> slice_merge3d model.c:171-244):
+ * merge one view into model2 / inter_weight.                                            */
+int emc_slice_merge(emc_engine *e, const double *quat, const double *view);
+
+/* ---- outputs ------------------------------------------------------------------------- */
+/* Copy field `field` (enum emc_field) into dst; count is the element count of dst and
+ * must equal the field's size.                                                          */
+int emc_get_field(emc_engine *e, int field, void *dst, size_t count);
+/* Zero model2 and inter_weight (max_data allocate_memory, maximize.c:93-94).            */
+int emc_clear_merge(emc_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMC_HIP_H */

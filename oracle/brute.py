@@ -1,0 +1,159 @@
+"""Dense brute-force EMC iteration for tiny inputs -- test infrastructure only.
+
+An independent formulation of one reference iteration (dragonfly/src/maximize.c:8-56 and
+the functions it calls) written as dense numpy over all (frame, orientation, pixel)
+triples, with no keep lists, no linear searches and no C code. It pins the oracle's
+orc_maximize: both are derived from the reference's semantics separately, so a dropped term,
+a wrong mask threshold or a swapped operand in either shows up as a mismatch.
+"""
+import numpy as np
+
+DBL_MIN = np.finfo(np.float64).tiny
+
+
+def rotation(q):
+    """make_rot_quat (model.c:759-786)."""
+    q0, q1, q2, q3 = q[:4]
+    return np.array([
+        [1. - 2. * (q2 * q2 + q3 * q3), 2. * (q1 * q2 + q0 * q3), 2. * (q1 * q3 - q0 * q2)],
+        [2. * (q1 * q2 - q0 * q3), 1. - 2. * (q1 * q1 + q3 * q3), 2. * (q0 * q1 + q2 * q3)],
+        [2. * (q0 * q2 + q1 * q3), 2. * (q2 * q3 - q0 * q1), 1. - 2. * (q1 * q1 + q2 * q2)]])
+
+
+def _cells(q, qvals, size):
+    p = qvals @ rotation(q).T + size // 2
+    cell = np.trunc(p).astype(np.int64)
+    inb = np.all((cell >= 0) & (cell <= size - 2), axis=1)
+    return p, cell, inb
+
+
+def _corners(cell, f, size):
+    for bx in (0, 1):
+        for by in (0, 1):
+            for bz in (0, 1):
+                w = ((f[:, 0] if bx else 1 - f[:, 0]) * (f[:, 1] if by else 1 - f[:, 1])
+                     * (f[:, 2] if bz else 1 - f[:, 2]))
+                idx = (((cell[:, 0] + bx) % size) * size + (cell[:, 1] + by) % size) * size \
+                    + (cell[:, 2] + bz) % size
+                yield w, idx
+
+
+def slice_view(model, q, qvals, corr):
+    """slice_gen3d (model.c:11-59): trilinear view times corr, DBL_MIN outside the grid."""
+    size = model.shape[0]
+    p, cell, inb = _cells(q, qvals, size)
+    f = p - cell
+    flat = model.ravel()
+    val = np.zeros(len(corr))
+    for w, idx in _corners(np.where(inb[:, None], cell, 0), f, size):
+        val += w * flat[idx]
+    return np.where(inb, val * corr, DBL_MIN)
+
+
+def merge_view(model2, weight, q, view, qvals, corr, mask):
+    """slice_merge3d (model.c:171-244) with np.add.at."""
+    size = model2.shape[0]
+    p, cell, inb = _cells(q, qvals, size)
+    sel = inb & (mask <= 1)
+    f = (p - cell)[sel]
+    w_val = view[sel] / corr[sel]
+    for w, idx in _corners(cell[sel], f, size):
+        np.add.at(weight.ravel(), idx, w)
+        np.add.at(model2.ravel(), idx, w * w_val)
+
+
+def dense_counts(frames, num_pix):
+    D = len(frames['ones'])
+    K = np.zeros((D, num_pix), dtype=np.int64)
+    for d in range(D):
+        o0, m0 = frames['ones_accum'][d], frames['multi_accum'][d]
+        K[d, frames['place_ones'][o0:o0 + frames['ones'][d]]] = 1
+        sl = slice(m0, m0 + frames['multi'][d])
+        K[d, frames['place_multi'][sl]] = frames['count_multi'][sl]
+    return K
+
+
+def iterate(det, quats, model1, frames, fparams, flags):
+    """One iteration; returns the same fields as oracle.core.maximize."""
+    qvals, corr, mask = det['qvals'], det['corr'], det['mask']
+    Rn, P = quats.shape[0], len(corr)
+    size = model1.shape[0]
+    beta, scale = fparams['beta'], fparams['scale']
+    black = np.asarray(fparams['blacklist']).astype(bool)
+    sum_fact = fparams['sum_fact']
+    need_scaling = flags.get('need_scaling', 0)
+    update_scale = flags.get('update_scale', 1)
+    use_scale = need_scaling and (flags.get('iteration', 1) > 1 or flags.get('known_scale', 0))
+    K = dense_counts(frames, P)
+    D = K.shape[0]
+    W = np.array([slice_view(model1, q, qvals, corr) for q in quats])
+    good = mask < 1
+    u = W[:, good].sum(1)
+    w_r = quats[:, 4]
+    rescale = fparams['mean_count'] / (w_r * u).sum()
+    with np.errstate(divide='ignore'):
+        logW = np.log(W)
+    R = np.full((D, Rn), -np.inf)
+    for d in range(D):
+        if black[d]:
+            continue
+        phi = scale[d] if use_scale else rescale
+        t = np.nonzero((K[d] > 0) & good)[0]
+        R[d] = np.log(w_r) - u * phi + (K[d, t][None, :] * logW[:, t]).sum(1)
+    M = np.full(D, -1.e100)
+    rmax = np.full(D, -1, dtype=np.int64)
+    Pm = np.zeros((D, Rn))
+    like, info, psum_d, p_norm = np.zeros(D), np.zeros(D), np.zeros(D), np.zeros(D)
+    for d in range(D):
+        if black[d]:
+            continue
+        valid = R[d] > -1.e100
+        if not valid.any():
+            continue
+        M[d] = R[d][valid].max()
+        rmax[d] = int(np.nonzero(R[d] == M[d])[0][0])
+        kept = R[d] > M[d] - 14. / beta[d]
+        e = np.where(kept, np.exp(beta[d] * (R[d] - M[d])), 0.)
+        p_norm[d] = e.sum()
+        Pm[d] = e / p_norm[d]
+        like[d] = (Pm[d] * np.where(kept, R[d] - sum_fact[d], 0.)).sum()
+        big = kept & (Pm[d] >= 1e-6)
+        info[d] = (Pm[d][big] * np.log(Pm[d][big] / w_r[big])).sum()
+        psum_d[d] = (Pm[d] * u * rescale).sum()
+    phi_p = np.where(black, 0., scale if need_scaling else 1.)
+    psum_r = (Pm * phi_p[:, None]).sum(0)
+    Psig = np.where(Pm > 1e-6, Pm, 0.)
+    Psig[black] = 0.
+    K2 = np.where((mask < 2)[None, :], K, 0)
+    views = Psig.T @ K2
+    model2 = np.zeros((size,) * 3)
+    weight = np.zeros((size,) * 3)
+    for r in range(Rn):
+        if psum_r[r] > 0:
+            merge_view(model2, weight, quats[r], views[r] / psum_r[r], qvals, corr, mask)
+    nb = D - int(black.sum())
+    out = dict(u=u, rescale=rescale, max_exp=M, rmax=rmax, p_norm=p_norm, likelihood=like,
+               info=info, psum_d=psum_d, psum_r=psum_r, prob=Pm, model2=model2,
+               inter_weight=weight, R=R, avg_likelihood=like[~black].sum() / nb,
+               mutual_info=info[~black].sum() / nb)
+    if need_scaling and update_scale:
+        new_scale = scale.copy()
+        new_scale[~black] = fparams['fcounts'][~black] / psum_d[~black]
+        out['scale'] = new_scale
+    return out
+
+
+def update_model(model1, model2, weight, alpha=0.):
+    """EMCRecon.update_model in Cython order with a vectorised Friedel average."""
+    size = model1.shape[0]
+    flipped_a = model2[::-1, ::-1, ::-1]
+    flipped_w = weight[::-1, ::-1, ::-1]
+    if size % 2 == 0:
+        raise ValueError('vectorised Friedel average is written for odd sizes')
+    num = 0.5 * (model2 * weight + flipped_a * flipped_w)
+    den = 0.5 * (weight + flipped_w)
+    sym = np.where(den > 0, num / np.where(den > 0, den, 1.), 0.)
+    m2 = np.where(weight > 0, sym / np.where(weight > 0, weight, 1.), sym)
+    rms = np.sqrt(((m2 - model1) ** 2).mean())
+    new1 = alpha * model1 + (1 - alpha) * m2 if alpha > 0 else m2
+    return new1, m2, rms

@@ -1,0 +1,86 @@
+/*
+ * CPU oracle for one EMC iteration -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference engine's algorithm (duaneloh/Dragonfly,
+ * Cython engine, dragonfly/src/), used by tests/, by the smoke check and by the
+ * CPU-baseline leg of bench.py as the checker. Nothing in the product links it.
+ *
+ * Each function cites the reference file:line it follows. Parity status of every
+ * function is listed in the header comment of emc_oracle.c.
+ */
+#ifndef EMC_ORACLE_H
+#define EMC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_problem {
+    /* detector (dragonfly/src/detector.h:6-16) */
+    int num_pix;
+    const double *qvals;      /* [num_pix][3] voxel units */
+    const double *corr;       /* [num_pix] */
+    const uint8_t *mask;      /* [num_pix] 0 good, 1 likelihood-excluded, 2 bad */
+    /* orientations (dragonfly/src/quaternion.h:11-15), [num_rot][5] = q0..q3, weight */
+    int num_rot;
+    const double *quats;
+    /* model (dragonfly/src/model.h:15-20), one mode */
+    long size;
+    const double *model1;
+    double *model2;           /* out: merged numerator */
+    double *inter_weight;     /* out: merged weights */
+    /* frames (dragonfly/src/emcfile.h:12-32): ftype 0 sparse, 1 dense int */
+    int num_data;
+    int ftype;
+    const int *ones, *multi;
+    const long *ones_accum, *multi_accum;
+    const int *place_ones, *place_multi, *count_multi;
+    const int *int_frames;    /* [num_data][num_pix] when ftype == 1 */
+    /* per-frame state (dragonfly/src/iterate.h:11-38) */
+    const double *beta;
+    double *scale;            /* in/out when need_scaling && update_scale */
+    const uint8_t *blacklist;
+    const int *fcounts;
+    const double *sum_fact;
+    double mean_count;
+    int num_blacklist;
+    /* flags (dragonfly/src/params.h) */
+    int need_scaling, update_scale, iteration, known_scale;
+    int num_threads;          /* OpenMP threads; 1 gives the canonical single-thread keep rule */
+} orc_problem;
+
+typedef struct orc_result {
+    double *u;                /* [num_rot] */
+    double *max_exp;          /* [num_data] */
+    double *p_norm;           /* [num_data] */
+    int *rmax;                /* [num_data] */
+    double *likelihood;       /* [num_data] */
+    double *info;             /* [num_data] */
+    double *psum_d;           /* [num_data], only when need_scaling && update_scale */
+    double *prob_dense;       /* optional [num_data][num_rot] posterior dump (0 where not kept) */
+    double rescale;
+    double mutual_info;
+    double avg_likelihood;
+    double seconds[5];        /* rescale, prob, norm, update, total */
+} orc_result;
+
+int orc_num_rot(int num_div);
+int orc_quat_gen(int num_div, double *quats);
+void orc_make_rot_quat(const double *q, double *rot9);
+void orc_slice_gen3d(const double *quat, const double *model, long size,
+                     int num_pix, const double *qvals, const double *corr, double *slice);
+void orc_slice_merge3d(const double *quat, const double *slice, double *model2,
+                       double *weight, long size, int num_pix, const double *qvals,
+                       const double *corr, const uint8_t *mask);
+void orc_symmetrize_friedel(double *array, const double *weights, int size);
+double orc_maximize(const orc_problem *p, orc_result *res);
+double orc_update_model(double *model1, double *model2, const double *inter_weight,
+                        long size, double alpha);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,82 @@
+"""Small seeded EMC problems shared by the CPU and GPU tests, plus runners that feed the
+same inputs to the oracle (CPU) and to the engine (GPU)."""
+import numpy as np
+
+from dragonfly_amd import synth
+from dragonfly_amd.iterate import frame_counts, sum_log_factorials
+
+SPARSE_KEYS = ('ones', 'multi', 'ones_accum', 'multi_accum', 'place_ones', 'place_multi',
+               'count_multi')
+
+
+def tiny_config(size=21, nx=22, ny=20, atoms=6, num_data=60, mean_count=40., num_div=2,
+                phi=None, gap=0, stoprad=2.):
+    return dict(size=size, atoms=atoms, atom_radius=max(2., size / 6.), sigma=1.2, nx=nx, ny=ny,
+                gap=gap, stoprad=stoprad, num_data=num_data, mean_count=mean_count, phi=phi,
+                num_div=num_div, need_scaling=0, beta=1.0, iters=2)
+
+
+def build(cfg, quats=None, start_seed=4, num_ave=500, device='cpu'):
+    """Detector, frames, start model, per-frame arrays and orientations for a config dict."""
+    size = cfg['size']
+    det = synth.flat_detector(cfg['nx'], cfg['ny'], qmax=(size - 3) / 2. - 0.05,
+                              stoprad=cfg['stoprad'], gap=cfg['gap'])
+    assert synth.model_size(det['qvals']) == size
+    intens = synth.make_intensity(size, cfg['atoms'], cfg['atom_radius'], cfg['sigma'], seed=0)
+    frames = synth.simulate_frames(intens, det, cfg['num_data'], cfg['mean_count'], phi=cfg['phi'],
+                                   num_ave=num_ave, batch=256, device=device)
+    if quats is None:
+        from dragonfly_amd import _lib
+        quats = _lib.quat_gen(cfg['num_div'])
+    mask = det['mask']
+    fc = frame_counts(frames['ones'], frames['multi'], frames['place_ones'], frames['place_multi'],
+                      frames['count_multi'], mask)
+    sf = sum_log_factorials(frames['multi'], frames['place_multi'], frames['count_multi'], mask)
+    D = len(frames['ones'])
+    mean_count = fc.astype('f8').sum() / D
+    model1 = synth.start_model(size, mean_count, mask, seed=start_seed)
+    fparams = dict(scale=np.ones(D), beta=np.full(D, float(cfg.get('beta_value', 1.0))),
+                   blacklist=np.zeros(D, 'u1'), fcounts=fc, sum_fact=sf, mean_count=mean_count,
+                   num_blacklist=0)
+    return dict(det=det, frames={k: frames[k] for k in SPARSE_KEYS}, quats=quats, model1=model1,
+                fparams=fparams, intens=intens, true_scale=frames['true_scale'])
+
+
+def run_oracle(p, flags, model1=None, dense_prob=True, num_threads=1):
+    from oracle import core
+    fp = dict(p['fparams'])
+    fp['scale'] = np.array(fp['scale'], dtype='f8')
+    out = core.maximize(p['det'], p['quats'], p['model1'] if model1 is None else model1,
+                        p['frames'], fp, flags, num_threads=num_threads, dense_prob=dense_prob)
+    out['scale'] = fp['scale']
+    return out
+
+
+def engine_for(p):
+    from dragonfly_amd._lib import Engine
+    eng = Engine(0)
+    eng.set_detector(p['det']['qvals'], p['det']['corr'], p['det']['mask'])
+    eng.set_quats(p['quats'])
+    eng.set_model(p['model1'])
+    f = p['frames']
+    eng.set_frames(*[f[k] for k in SPARSE_KEYS])
+    fp = p['fparams']
+    eng.set_frame_params(fp['scale'], fp['beta'], fp['blacklist'], fp['fcounts'], fp['sum_fact'],
+                         fp['mean_count'], fp['num_blacklist'])
+    return eng
+
+
+def run_engine(eng, flags):
+    st = eng.maximize(flags.get('iteration', 1), flags.get('need_scaling', 0),
+                      flags.get('update_scale', 1), flags.get('known_scale', 0))
+    out = {k: eng.get(k) for k in ('u', 'max_exp', 'p_norm', 'rmax', 'likelihood', 'info',
+                                   'psum_d', 'psum_r', 'model2', 'inter_weight', 'scale', 'prob')}
+    out.update(st)
+    return out
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype='f8')
+    b = np.asarray(b, dtype='f8')
+    scale = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / scale)
