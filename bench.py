@@ -83,7 +83,7 @@ def main():
     ap.add_argument('--config', default='cfg2')
     ap.add_argument('--num-data', type=int, default=None)
     ap.add_argument('--num-div', type=int, default=None)
-    ap.add_argument('--cpu-rot', type=int, default=128, help='orientations in the CPU sample')
+    ap.add_argument('--cpu-rot', type=int, default=3072, help='orientations in the CPU sample')
     ap.add_argument('--no-cpu', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1:

@@ -3,6 +3,7 @@
 // include/emc_hip.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -82,6 +83,34 @@ void make_rot(const double *q, double *rot) {
     rot[8] = (1. - 2. * (q11 + q22));
 }
 
+// Engine pixel order: pixels sorted along a Morton (Z-order) curve of their transverse
+// q-coordinates (qx, qy) quantised to the mean pixel spacing, so every 64 consecutive pixels
+// form a compact detector patch (an 8x8 tile on a regular grid). perm[new] = old.
+std::vector<int> morton_order(int num_pix, const double *qvals) {
+    double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+    for (int t = 0; t < num_pix; ++t) {
+        x0 = std::min(x0, qvals[3 * t]);
+        x1 = std::max(x1, qvals[3 * t]);
+        y0 = std::min(y0, qvals[3 * t + 1]);
+        y1 = std::max(y1, qvals[3 * t + 1]);
+    }
+    double area = (x1 - x0) * (y1 - y0);
+    double step = area > 0 ? std::sqrt(area / num_pix) : 1.;
+    if (!(step > 0)) step = 1.;
+    std::vector<std::pair<uint64_t, int>> key(num_pix);
+    for (int t = 0; t < num_pix; ++t) {
+        uint32_t ix = (uint32_t)std::min(65535.0, std::max(0.0, std::floor((qvals[3 * t] - x0) / step + 0.5)));
+        uint32_t iy = (uint32_t)std::min(65535.0, std::max(0.0, std::floor((qvals[3 * t + 1] - y0) / step + 0.5)));
+        uint64_t code = 0;
+        for (int b = 0; b < 16; ++b) code |= (uint64_t)((ix >> b) & 1) << (2 * b + 1) | (uint64_t)((iy >> b) & 1) << (2 * b);
+        key[t] = {code, t};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> perm(num_pix);
+    for (int t = 0; t < num_pix; ++t) perm[t] = key[t].second;
+    return perm;
+}
+
 constexpr int kPixPerGroup = 1024;
 constexpr int kFramesPerPsumPart = 512;
 constexpr int kModelParts = 1024;
@@ -95,9 +124,11 @@ struct emc_engine {
 
     // detector
     int num_pix = 0;
-    std::vector<uint8_t> h_mask;
-    DevBuf<double> qvals, corr;
+    std::vector<uint8_t> h_mask;       // caller's pixel order
+    std::vector<int> perm, inv;        // perm[engine] = caller, inv[caller] = engine
+    DevBuf<double> qvals, corr;        // engine pixel order
     DevBuf<uint8_t> mask;
+    DevBuf<int> d_perm;
     // orientations (rots padded to Rp with identity)
     int num_rot = 0, Rp = 0;
     DevBuf<double> quats, rots, logwr;
@@ -125,7 +156,13 @@ struct emc_engine {
     DevBuf<double> max_exp, p_norm, likelihood, info, psum_d;
     DevBuf<int> rmax;
     DevBuf<double> one_rot, one_view;
+    DevBuf<int> d_count;
+    // inter_weight summed over every orientation: data-independent, reused whenever all
+    // orientations have psum_r > 0
+    DevBuf<double> weight_all;
+    bool weight_all_valid = false;
     bool have_results = false;
+    int last_unmerged = -1;
 };
 
 namespace {
@@ -156,13 +193,13 @@ int build_lists(emc_engine *e) {
             int p = e->h_place_ones[e->h_ones_accum[d] + k];
             if (p < 0 || p >= P) return fail(EMC_E_ARG, "frame %d: pixel %d out of range", d, p);
             if (mask[p] < 1) ++n;
-            if (mask[p] < 2) ++tcount[p + 1];
+            if (mask[p] < 2) ++tcount[e->inv[p] + 1];
         }
         for (int k = 0; k < e->h_multi[d]; ++k) {
             int p = e->h_place_multi[e->h_multi_accum[d] + k];
             if (p < 0 || p >= P) return fail(EMC_E_ARG, "frame %d: pixel %d out of range", d, p);
             if (mask[p] < 1) ++n;
-            if (mask[p] < 2) ++tcount[p + 1];
+            if (mask[p] < 2) ++tcount[e->inv[p] + 1];
         }
         loff[d + 1] = loff[d] + n;
     }
@@ -175,16 +212,16 @@ int build_lists(emc_engine *e) {
     for (int d = 0; d < N; ++d) {
         long at = loff[d];
         for (int k = 0; k < e->h_ones[d]; ++k) {
-            int p = e->h_place_ones[e->h_ones_accum[d] + k];
-            if (mask[p] < 1) { lpix[at] = p; lcnt[at] = 1.; ++at; }
-            if (mask[p] < 2) { tframe[tfill[p]] = d; tcnt[tfill[p]] = 1.; ++tfill[p]; }
+            int p = e->h_place_ones[e->h_ones_accum[d] + k], q = e->inv[p];
+            if (mask[p] < 1) { lpix[at] = q; lcnt[at] = 1.; ++at; }
+            if (mask[p] < 2) { tframe[tfill[q]] = d; tcnt[tfill[q]] = 1.; ++tfill[q]; }
         }
         for (int k = 0; k < e->h_multi[d]; ++k) {
             long j = e->h_multi_accum[d] + k;
-            int p = e->h_place_multi[j];
+            int p = e->h_place_multi[j], q = e->inv[p];
             double c = (double)e->h_count_multi[j];
-            if (mask[p] < 1) { lpix[at] = p; lcnt[at] = c; ++at; }
-            if (mask[p] < 2) { tframe[tfill[p]] = d; tcnt[tfill[p]] = c; ++tfill[p]; }
+            if (mask[p] < 1) { lpix[at] = q; lcnt[at] = c; ++at; }
+            if (mask[p] < 2) { tframe[tfill[q]] = d; tcnt[tfill[q]] = c; ++tfill[q]; }
         }
     }
     int rc;
@@ -279,6 +316,9 @@ void emc_destroy(emc_engine *e) {
                               &e->one_rot, &e->one_view})
         b->release();
     e->mask.release();
+    e->d_perm.release();
+    e->d_count.release();
+    e->weight_all.release();
     e->blacklist.release();
     e->lik_off.release();
     e->tom_off.release();
@@ -301,14 +341,32 @@ int emc_set_detector(emc_engine *e, int num_pix, const double *qvals, const doub
                      const uint8_t *raw_mask) {
     if (!e || num_pix <= 0 || !qvals || !corr || !raw_mask) return fail(EMC_E_ARG, "emc_set_detector: bad argument");
     HIP_TRY(hipSetDevice(e->device));
+    std::vector<int> perm = morton_order(num_pix, qvals);
+    std::vector<int> inv(num_pix);
+    std::vector<double> pq(3 * (size_t)num_pix), pc(num_pix);
+    std::vector<uint8_t> pm(num_pix);
+    for (int t = 0; t < num_pix; ++t) {
+        const int o = perm[t];
+        inv[o] = t;
+        pq[3 * (size_t)t] = qvals[3 * (size_t)o];
+        pq[3 * (size_t)t + 1] = qvals[3 * (size_t)o + 1];
+        pq[3 * (size_t)t + 2] = qvals[3 * (size_t)o + 2];
+        pc[t] = corr[o];
+        pm[t] = raw_mask[o];
+    }
     int rc;
-    if ((rc = e->qvals.upload(qvals, 3 * (size_t)num_pix, e->stream))) return rc;
-    if ((rc = e->corr.upload(corr, num_pix, e->stream))) return rc;
-    if ((rc = e->mask.upload(raw_mask, num_pix, e->stream))) return rc;
+    if ((rc = e->qvals.upload(pq.data(), pq.size(), e->stream))) return rc;
+    if ((rc = e->corr.upload(pc.data(), pc.size(), e->stream))) return rc;
+    if ((rc = e->mask.upload(pm.data(), pm.size(), e->stream))) return rc;
+    if ((rc = e->d_perm.upload(perm.data(), perm.size(), e->stream))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
     e->h_mask.assign(raw_mask, raw_mask + num_pix);
+    e->perm.swap(perm);
+    e->inv.swap(inv);
     e->num_pix = num_pix;
     e->lists_dirty = true;
     e->have_results = false;
+    e->weight_all_valid = false;
     HIP_TRY(hipStreamSynchronize(e->stream));
     return 0;
 }
@@ -334,6 +392,7 @@ int emc_set_quats(emc_engine *e, int num_rot, const double *quats) {
     e->num_rot = num_rot;
     e->Rp = Rp;
     e->have_results = false;
+    e->weight_all_valid = false;
     return 0;
 }
 
@@ -349,6 +408,7 @@ int emc_set_model(emc_engine *e, long size, const double *model1) {
     emc::launch_fill(e->stream, e->weight.p, vol, 0.);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(e->stream));
+    if (size != e->size) e->weight_all_valid = false;
     e->size = size;
     return 0;
 }
@@ -491,6 +551,37 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     HIP_TRY(hipEventElapsedTime(&ms_norm, e->ev[5], e->ev[6]));
     HIP_TRY(hipEventElapsedTime(&ms_post, e->ev[6], e->ev[7]));
 
+    // inter_weight does not depend on the data: when every orientation merges (all
+    // psum_r > 0) it is the all-orientation sum, computed once and reused
+    int unmerged = 0;
+    if ((rc = e->d_count.alloc(1))) return rc;
+    emc::launch_count_unmerged(s, e->psum_r.p, Rn, e->d_count.p);
+    HIP_TRY(hipMemcpyAsync(&unmerged, e->d_count.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    e->last_unmerged = unmerged;
+    const bool cached_weight = (unmerged == 0);
+    if (cached_weight) {
+        HIP_TRY(hipEventRecord(e->ev[0], s));
+        if (!e->weight_all_valid) {
+            if ((rc = e->weight_all.alloc(vol))) return rc;
+            emc::launch_fill(s, e->weight_all.p, vol, 0.);
+            for (int r0 = 0; r0 < e->Rp; r0 += C) {
+                const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
+                emc::launch_merge(s, e->rots.p, r0, Cc, Rn, nullptr, e->psum_r.p, 1, e->qvals.p,
+                                  e->corr.p, e->mask.p, P, e->size, nullptr, e->weight_all.p, 0, 1);
+            }
+            HIP_TRY(hipGetLastError());
+            e->weight_all_valid = true;
+        }
+        HIP_TRY(hipMemcpyAsync(e->weight.p, e->weight_all.p, vol * sizeof(double),
+                               hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipEventRecord(e->ev[1], s));
+        HIP_TRY(hipEventSynchronize(e->ev[1]));
+        float a = 0.f;
+        HIP_TRY(hipEventElapsedTime(&a, e->ev[0], e->ev[1]));
+        ms_merge += a;
+    }
+
     // tomogram + merge, chunk by chunk
     for (int r0 = 0; r0 < e->Rp; r0 += C) {
         const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
@@ -499,9 +590,9 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         emc::launch_tomogram(s, e->R.p, ldR, r0, Cc, e->tom_off.p, e->tom_frame.p, e->tom_cnt.p,
                              e->blacklist.p, e->psum_r.p, Rn, P, e->logw.p);
         HIP_TRY(hipEventRecord(e->ev[1], s));
-        emc::launch_merge(s, e->rots.p, r0, Cc, Rn, e->logw.p, e->psum_r.p, e->qvals.p,
-                          e->corr.p, e->mask.p, P, kPixPerGroup, e->size, e->model2.p,
-                          e->weight.p);
+        emc::launch_merge(s, e->rots.p, r0, Cc, Rn, e->logw.p, e->psum_r.p, 0, e->qvals.p,
+                          e->corr.p, e->mask.p, P, e->size, e->model2.p, e->weight.p, 1,
+                          cached_weight ? 0 : 1);
         HIP_TRY(hipEventRecord(e->ev[2], s));
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventSynchronize(e->ev[2]));
@@ -572,7 +663,7 @@ int emc_slice_gen(emc_engine *e, const double *quat, double *view_out) {
     if ((rc = e->one_rot.upload(rot, 9, e->stream))) return rc;
     if ((rc = e->one_view.alloc(e->num_pix))) return rc;
     emc::launch_slice_gen(e->stream, e->one_rot.p, e->model1.p, e->size, e->qvals.p, e->corr.p,
-                          e->num_pix, e->one_view.p);
+                          e->d_perm.p, e->num_pix, e->one_view.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(view_out, e->one_view.p, e->num_pix * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -589,7 +680,7 @@ int emc_slice_merge(emc_engine *e, const double *quat, const double *view) {
     if ((rc = e->one_rot.upload(rot, 9, e->stream))) return rc;
     if ((rc = e->one_view.upload(view, e->num_pix, e->stream))) return rc;
     emc::launch_slice_merge(e->stream, e->one_rot.p, e->one_view.p, e->qvals.p, e->corr.p,
-                            e->mask.p, e->num_pix, e->size, e->model2.p, e->weight.p);
+                            e->mask.p, e->d_perm.p, e->num_pix, e->size, e->model2.p, e->weight.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(e->stream));
     return 0;

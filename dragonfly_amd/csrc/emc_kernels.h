@@ -36,14 +36,16 @@ void launch_tomogram(hipStream_t s, const double *P, size_t ldP, int r0, int C, 
                      const int *tframe, const double *tcnt, const uint8_t *blacklist,
                      const double *psum_r, int num_rot, int num_pix, double *view);
 void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot, const double *view,
-                  const double *psum_r, const double *qvals, const double *corr,
-                  const uint8_t *mask, int num_pix, int pix_per_group, long size, double *model2,
-                  double *weight);
+                  const double *psum_r, int all_on, const double *qvals, const double *corr,
+                  const uint8_t *mask, int num_pix, long size, double *model2, double *weight,
+                  int do_model, int do_weight);
+void launch_count_unmerged(hipStream_t s, const double *psum_r, int num_rot, int *count);
 void launch_slice_gen(hipStream_t s, const double *rot, const double *model, long size,
-                      const double *qvals, const double *corr, int num_pix, double *out);
+                      const double *qvals, const double *corr, const int *perm, int num_pix,
+                      double *out);
 void launch_slice_merge(hipStream_t s, const double *rot, const double *view, const double *qvals,
-                        const double *corr, const uint8_t *mask, int num_pix, long size,
-                        double *model2, double *weight);
+                        const double *corr, const uint8_t *mask, const int *perm, int num_pix,
+                        long size, double *model2, double *weight);
 void launch_frame_stats(hipStream_t s, const double *likelihood, const double *info,
                         const uint8_t *blacklist, int num_data, int denom, double *scalars);
 void launch_scale(hipStream_t s, const int *fcounts, const double *psum_d, const uint8_t *blacklist,

@@ -10,6 +10,7 @@
 // Every kernel keeps the reference's operation order for the values it computes
 // (the library is compiled with -ffp-contract=off, so no multiply-add is fused).
 #include <cfloat>
+#include <climits>
 
 #include "emc_common.h"
 #include "emc_kernels.h"
@@ -176,13 +177,13 @@ __global__ __launch_bounds__(kBlock) void likelihood_kernel(
     const long k1 = off[d + 1];
     long k = off[d];
     double acc = 0.;
-    for (; k + 4 <= k1; k += 4) {
-        const double v0 = col[(size_t)pix[k] * C], v1 = col[(size_t)pix[k + 1] * C];
-        const double v2 = col[(size_t)pix[k + 2] * C], v3 = col[(size_t)pix[k + 3] * C];
-        acc += cnt[k] * v0;
-        acc += cnt[k + 1] * v1;
-        acc += cnt[k + 2] * v2;
-        acc += cnt[k + 3] * v3;
+    // eight independent row loads in flight, then the additions in list order
+    for (; k + 8 <= k1; k += 8) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = col[(size_t)pix[k + j] * C];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc += cnt[k + j] * v[j];
     }
     for (; k < k1; ++k) acc += cnt[k] * col[(size_t)pix[k] * C];
     G[(size_t)d * ldG + r0 + c] = acc;
@@ -369,11 +370,20 @@ __global__ __launch_bounds__(kBlock) void tomogram_kernel(
     const int c = blockIdx.y * kRTile + lane;
     const double *__restrict__ col = P + r0 + c;
     const long k1 = toff[t + 1];
+    long k = toff[t];
     double acc = 0.;
-    for (long k = toff[t]; k < k1; ++k) {
-        const int d = tframe[k];
-        if (blacklist[d]) continue;
-        const double p = col[(size_t)d * ldP];
+    // P rows of blacklisted frames are zero (posterior_kernel), so they add nothing here.
+    // Eight independent row loads in flight, then the additions in frame order.
+    for (; k + 8 <= k1; k += 8) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = col[(size_t)tframe[k + j] * ldP];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (v[j] > kProbMin) acc += tcnt[k + j] * v[j];
+    }
+    for (; k < k1; ++k) {
+        const double p = col[(size_t)tframe[k] * ldP];
         if (p > kProbMin) acc += tcnt[k] * p;
     }
     const double ps = (r0 + c < num_rot) ? psum_r[r0 + c] : 0.;
@@ -383,72 +393,168 @@ __global__ __launch_bounds__(kBlock) void tomogram_kernel(
 
 // ---- merge: scatter views into model2 / inter_weight (merge_tomogram, max_update.c:33-36;
 // slice_merge3d model.c:171-244), only orientations with psum_r > 0 and pixels mask <= 1 ----
-__global__ __launch_bounds__(kBlock) void merge_kernel(
+//
+// Pixels are stored in 8x8 q-space tiles (Morton order), so the 64 lanes of a wavefront hold
+// one compact detector patch. For one orientation the patch lands in a small voxel box: the
+// wavefront accumulates its 8-corner contributions in a private LDS box with LDS atomics and
+// then adds each touched voxel to HBM once. A box larger than kBoxCap falls back to direct
+// HBM atomics. The voxel sums are the same as the reference's up to summation order.
+constexpr int kBoxCap = 1024;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ __forceinline__ void corner_weights(const Cell &c, double *f) {
+    const double cx = 1. - c.fx, cy = 1. - c.fy, cz = 1. - c.fz;
+    const double fx = c.fx, fy = c.fy, fz = c.fz;
+    f[0] = cx * cy * cz; f[1] = cx * cy * fz; f[2] = cx * fy * cz; f[3] = cx * fy * fz;
+    f[4] = fx * cy * cz; f[5] = fx * cy * fz; f[6] = fx * fy * cz; f[7] = fx * fy * fz;
+}
+
+template <bool kModel, bool kWeight>
+__global__ __launch_bounds__(kBlock) void merge_box_kernel(
     const double *__restrict__ rots, int r0, int C, int num_rot, const double *__restrict__ view,
-    const double *__restrict__ psum_r, const double *__restrict__ qvals,
-    const double *__restrict__ corr, const uint8_t *__restrict__ mask, int num_pix,
-    int pix_per_group, long size, double *__restrict__ model2, double *__restrict__ weight) {
+    const double *__restrict__ psum_r, int all_on, const double *__restrict__ qvals,
+    const double *__restrict__ corr, const uint8_t *__restrict__ mask, int num_pix, long size,
+    double *__restrict__ model2, double *__restrict__ weight) {
+    constexpr int kPer = (kModel ? kBoxCap : 0) + (kWeight ? kBoxCap : 0);
     __shared__ double s_rot[kRTile * 9];
-    __shared__ double s_tile[64][kRTile + 1];
     __shared__ int s_on[kRTile];
-    const int tid = threadIdx.x;
+    __shared__ double s_box[kWavesPerBlock][kPer];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c0 = blockIdx.x * kRTile;
-    const int g = blockIdx.y;
     for (int i = tid; i < kRTile * 9; i += kBlock) s_rot[i] = rots[(size_t)(r0 + c0) * 9 + i];
     if (tid < kRTile) {
         const int r = r0 + c0 + tid;
-        s_on[tid] = (r < num_rot) && (psum_r[r] > 0.);
+        s_on[tid] = (r < num_rot) && (all_on || psum_r[r] > 0.);
     }
     __syncthreads();
+    const int t = (blockIdx.y * kWavesPerBlock + wave) * 64 + lane;
+    const bool pix_ok = t < num_pix && mask[t] <= 1;
+    double q0 = 0., q1 = 0., q2 = 0., cr = 1.;
+    if (pix_ok) {
+        q0 = qvals[3 * (size_t)t];
+        q1 = qvals[3 * (size_t)t + 1];
+        q2 = qvals[3 * (size_t)t + 2];
+        cr = corr[t];
+    }
+    if (!__any(pix_ok)) return;
+    double *bm = &s_box[wave][0];
+    double *bw = bm + (kModel ? kBoxCap : 0);
     const double center = (double)(size / 2);
-    const int lane = tid & 63, wave = tid >> 6;
-    const int t_begin = g * pix_per_group;
-    const int t_end = min(num_pix, t_begin + pix_per_group);
-    for (int tb = t_begin; tb < t_end; tb += 64) {
-        for (int k = 0; k < 64 / kWavesPerBlock; ++k) {
-            const int p = wave + kWavesPerBlock * k;
-            if (tb + p < t_end) s_tile[p][lane] = view[(size_t)(tb + p) * C + c0 + lane];
-        }
-        __syncthreads();
-        const int t = tb + lane;
-        if (t < t_end && mask[t] <= 1) {
-            const double q0 = qvals[3 * (size_t)t], q1 = qvals[3 * (size_t)t + 1],
-                         q2 = qvals[3 * (size_t)t + 2], cr = corr[t];
-            for (int k = 0; k < kRTile / kWavesPerBlock; ++k) {
-                const int rr = wave + kWavesPerBlock * k;
-                if (!s_on[rr]) continue;
-                Cell cell;
-                if (!locate(&s_rot[rr * 9], q0, q1, q2, center, size, cell)) continue;
-                merge_value(model2, weight, size, cell, s_tile[lane][rr] / cr);
+    const long s2 = size * size;
+    for (int cc = 0; cc < kRTile; ++cc) {
+        if (!s_on[cc]) continue;
+        Cell cell;
+        const bool ok = pix_ok && locate(&s_rot[cc * 9], q0, q1, q2, center, size, cell);
+        double w = 0.;
+        if (kModel && ok) w = view[(size_t)t * C + c0 + cc] / cr;
+        const int xmax = wave_max(ok ? (int)cell.x : -1);
+        if (xmax < 0) continue;
+        const int xmin = wave_min(ok ? (int)cell.x : INT_MAX);
+        const int ymin = wave_min(ok ? (int)cell.y : INT_MAX), ymax = wave_max(ok ? (int)cell.y : -1);
+        const int zmin = wave_min(ok ? (int)cell.z : INT_MAX), zmax = wave_max(ok ? (int)cell.z : -1);
+        const int dx = xmax - xmin + 2, dy = ymax - ymin + 2, dz = zmax - zmin + 2;
+        const int bvol = dx * dy * dz;
+        double f[8];
+        if (ok) corner_weights(cell, f);
+        if (bvol <= kBoxCap) {
+            for (int i = lane; i < bvol; i += 64) {
+                if (kModel) bm[i] = 0.;
+                if (kWeight) bw[i] = 0.;
+            }
+            wave_sync();
+            if (ok) {
+                const int bx = (int)cell.x - xmin, by = (int)cell.y - ymin, bz = (int)cell.z - zmin;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int li = ((bx + (k >> 2)) * dy + (by + ((k >> 1) & 1))) * dz + bz + (k & 1);
+                    if (kWeight) unsafeAtomicAdd(&bw[li], f[k]);
+                    if (kModel) unsafeAtomicAdd(&bm[li], f[k] * w);
+                }
+            }
+            wave_sync();
+            for (int i = lane; i < bvol; i += 64) {
+                const int iz = i % dz, iy = (i / dz) % dy, ix = i / (dz * dy);
+                const long g = (long)(xmin + ix) * s2 + (long)(ymin + iy) * size + (zmin + iz);
+                if (kModel) {
+                    const double v = bm[i];
+                    if (v != 0.) unsafeAtomicAdd(&model2[g], v);
+                }
+                if (kWeight) {
+                    const double v = bw[i];
+                    if (v != 0.) unsafeAtomicAdd(&weight[g], v);
+                }
+            }
+            wave_sync();
+        } else if (ok) {
+            const long x1 = cell.x + 1, y1 = cell.y + 1, z1 = cell.z + 1;
+            const long idx[8] = {cell.x * s2 + cell.y * size + cell.z, cell.x * s2 + cell.y * size + z1,
+                                 cell.x * s2 + y1 * size + cell.z, cell.x * s2 + y1 * size + z1,
+                                 x1 * s2 + cell.y * size + cell.z, x1 * s2 + cell.y * size + z1,
+                                 x1 * s2 + y1 * size + cell.z, x1 * s2 + y1 * size + z1};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (kWeight) unsafeAtomicAdd(&weight[idx[k]], f[k]);
+                if (kModel) unsafeAtomicAdd(&model2[idx[k]], f[k] * w);
             }
         }
-        __syncthreads();
     }
 }
 
-// ---- single-view geometry (Model.slice_gen / slice_merge) -------------------------------
+// count of orientations with psum_r <= 0 (they are not merged, max_update.c:34-36); one block
+__global__ __launch_bounds__(kBlock) void count_unmerged_kernel(const double *__restrict__ psum_r,
+                                                                int num_rot, int *__restrict__ count) {
+    __shared__ int s[kBlock];
+    int local = 0;
+    for (int r = threadIdx.x; r < num_rot; r += kBlock) local += !(psum_r[r] > 0.);
+    s[threadIdx.x] = local;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = s[0];
+}
+
+// ---- single-view geometry (Model.slice_gen / slice_merge); perm maps the engine's pixel
+// order to the caller's --------------------------------------------------------------------
 __global__ void slice_gen_kernel(const double *__restrict__ rot, const double *__restrict__ model,
                                  long size, const double *__restrict__ qvals,
-                                 const double *__restrict__ corr, int num_pix,
-                                 double *__restrict__ out) {
+                                 const double *__restrict__ corr, const int *__restrict__ perm,
+                                 int num_pix, double *__restrict__ out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= num_pix) return;
-    out[t] = slice_value(model, size, rot, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1],
-                         qvals[3 * (size_t)t + 2], corr[t], (double)(size / 2));
+    out[perm[t]] = slice_value(model, size, rot, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1],
+                               qvals[3 * (size_t)t + 2], corr[t], (double)(size / 2));
 }
 
 __global__ void slice_merge_kernel(const double *__restrict__ rot, const double *__restrict__ view,
                                    const double *__restrict__ qvals,
                                    const double *__restrict__ corr,
-                                   const uint8_t *__restrict__ mask, int num_pix, long size,
-                                   double *__restrict__ model2, double *__restrict__ weight) {
+                                   const uint8_t *__restrict__ mask, const int *__restrict__ perm,
+                                   int num_pix, long size, double *__restrict__ model2,
+                                   double *__restrict__ weight) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= num_pix || mask[t] > 1) return;
     Cell cell;
     if (!locate(rot, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1], qvals[3 * (size_t)t + 2],
                 (double)(size / 2), size, cell))
         return;
-    merge_value(model2, weight, size, cell, view[t] / corr[t]);
+    merge_value(model2, weight, size, cell, view[perm[t]] / corr[t]);
 }
 
 // ---- per-frame statistics and scale ------------------------------------------------------
@@ -668,23 +774,35 @@ void launch_tomogram(hipStream_t s, const double *P, size_t ldP, int r0, int C, 
 }
 
 void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot, const double *view,
-                  const double *psum_r, const double *qvals, const double *corr,
-                  const uint8_t *mask, int num_pix, int pix_per_group, long size, double *model2,
-                  double *weight) {
-    dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
-    merge_kernel<<<grid, kBlock, 0, s>>>(rots, r0, C, num_rot, view, psum_r, qvals, corr, mask,
-                                         num_pix, pix_per_group, size, model2, weight);
+                  const double *psum_r, int all_on, const double *qvals, const double *corr,
+                  const uint8_t *mask, int num_pix, long size, double *model2, double *weight,
+                  int do_model, int do_weight) {
+    dim3 grid(C / kRTile, cdiv(num_pix, 64 * kWavesPerBlock));
+    if (do_model && do_weight)
+        merge_box_kernel<true, true><<<grid, kBlock, 0, s>>>(rots, r0, C, num_rot, view, psum_r, all_on,
+                                                             qvals, corr, mask, num_pix, size, model2, weight);
+    else if (do_model)
+        merge_box_kernel<true, false><<<grid, kBlock, 0, s>>>(rots, r0, C, num_rot, view, psum_r, all_on,
+                                                              qvals, corr, mask, num_pix, size, model2, weight);
+    else if (do_weight)
+        merge_box_kernel<false, true><<<grid, kBlock, 0, s>>>(rots, r0, C, num_rot, view, psum_r, all_on,
+                                                              qvals, corr, mask, num_pix, size, model2, weight);
+}
+
+void launch_count_unmerged(hipStream_t s, const double *psum_r, int num_rot, int *count) {
+    count_unmerged_kernel<<<1, kBlock, 0, s>>>(psum_r, num_rot, count);
 }
 
 void launch_slice_gen(hipStream_t s, const double *rot, const double *model, long size,
-                      const double *qvals, const double *corr, int num_pix, double *out) {
-    slice_gen_kernel<<<cdiv(num_pix, 256), 256, 0, s>>>(rot, model, size, qvals, corr, num_pix, out);
+                      const double *qvals, const double *corr, const int *perm, int num_pix,
+                      double *out) {
+    slice_gen_kernel<<<cdiv(num_pix, 256), 256, 0, s>>>(rot, model, size, qvals, corr, perm, num_pix, out);
 }
 
 void launch_slice_merge(hipStream_t s, const double *rot, const double *view, const double *qvals,
-                        const double *corr, const uint8_t *mask, int num_pix, long size,
-                        double *model2, double *weight) {
-    slice_merge_kernel<<<cdiv(num_pix, 256), 256, 0, s>>>(rot, view, qvals, corr, mask, num_pix,
+                        const double *corr, const uint8_t *mask, const int *perm, int num_pix,
+                        long size, double *model2, double *weight) {
+    slice_merge_kernel<<<cdiv(num_pix, 256), 256, 0, s>>>(rot, view, qvals, corr, mask, perm, num_pix,
                                                           size, model2, weight);
 }
 

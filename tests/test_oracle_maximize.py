@@ -1,0 +1,145 @@
+"""The oracle's full iteration (orc_maximize, restating dragonfly/src/maximize.c:8-56 with
+max_prob.c, max_update.c, max_scale.c) pinned without the reference's MPI/GSL build:
+
+* against oracle/brute.py, an independent dense formulation (no keep lists, no C), on tiny
+  inputs, unscaled and scaled, first and later iterations, with blacklisted frames;
+* against closed forms: a single orientation, the posterior summing to one, the rescale
+  identity sum_r w_r u_r rescale = mean_count, the scale update psum_d * scale = fcounts;
+* planted mistakes: the brute-force comparison must fail when the oracle's inputs emulate a
+  dropped term or a wrong mask threshold, so it is sensitive to those errors.
+"""
+import numpy as np
+import pytest
+
+import problems as P
+from oracle import brute
+
+TOL = 1e-11
+KEYS = ('u', 'max_exp', 'p_norm', 'likelihood', 'info', 'model2', 'inter_weight', 'prob')
+
+
+@pytest.fixture(scope='module')
+def tiny():
+    return P.build(P.tiny_config(num_data=50))
+
+
+@pytest.fixture(scope='module')
+def tiny_scaled():
+    return P.build(P.tiny_config(num_data=50, phi=(0.5, 1.5), gap=2))
+
+
+def _diff(o, b):
+    errs = {k: P.rel_err(o[k], b[k]) for k in KEYS}
+    for k in ('rescale', 'avg_likelihood', 'mutual_info'):
+        errs[k] = abs(o[k] - b[k]) / max(abs(b[k]), 1e-300)
+    return errs
+
+
+def _check(o, b):
+    errs = _diff(o, b)
+    bad = {k: v for k, v in errs.items() if not v <= TOL}
+    assert not bad, bad
+    assert np.array_equal(o['rmax'], b['rmax'])
+
+
+def test_unscaled_vs_brute(tiny):
+    flags = dict(iteration=1, need_scaling=0)
+    _check(P.run_oracle(tiny, flags), brute.iterate(tiny['det'], tiny['quats'], tiny['model1'],
+                                                    tiny['frames'], tiny['fparams'], flags))
+
+
+@pytest.mark.parametrize('iteration', [1, 2])
+def test_scaled_vs_brute(tiny_scaled, iteration):
+    p = tiny_scaled
+    fp = dict(p['fparams'], scale=np.asarray(p['true_scale'], 'f8').copy())
+    flags = dict(iteration=iteration, need_scaling=1, update_scale=1)
+    b = brute.iterate(p['det'], p['quats'], p['model1'], p['frames'], fp, flags)
+    o = P.run_oracle(dict(p, fparams=fp), flags)
+    _check(o, b)
+    assert P.rel_err(o['psum_d'], b['psum_d']) <= TOL
+    assert P.rel_err(o['scale'], b['scale']) <= TOL
+
+
+def test_blacklist_and_beta_vs_brute(tiny):
+    D = len(tiny['frames']['ones'])
+    bl = np.zeros(D, 'u1')
+    bl[[0, 5, 9]] = 1
+    fp = dict(tiny['fparams'], blacklist=bl, num_blacklist=3, beta=np.linspace(0.1, 1., D))
+    flags = dict(iteration=1, need_scaling=0)
+    o = P.run_oracle(dict(tiny, fparams=fp), flags)
+    _check(o, brute.iterate(tiny['det'], tiny['quats'], tiny['model1'], tiny['frames'], fp, flags))
+    assert (o['rmax'][[0, 5, 9]] == -1).all()
+    assert (o['prob'][[0, 5, 9]] == 0).all()
+
+
+def test_single_orientation_closed_form(tiny):
+    """One orientation of weight 1: every frame has P = 1, info = 0 and
+    likelihood = R - sum_fact with R = -u rescale + sum_t K log W."""
+    from oracle import core
+    q = np.array([[1., 0., 0., 0., 1.]])
+    p = dict(tiny, quats=q)
+    o = P.run_oracle(p, dict(iteration=1, need_scaling=0))
+    det = tiny['det']
+    W = core.slice_gen3d(q[0], tiny['model1'], det['qvals'], det['corr'])
+    good = det['mask'] < 1
+    u = W[good].sum()
+    rescale = tiny['fparams']['mean_count'] / u
+    K = brute.dense_counts(tiny['frames'], len(W))
+    R = -u * rescale + (K[:, good] * np.log(W[good])).sum(1)
+    assert np.allclose(o['prob'][:, 0], 1., rtol=1e-14)
+    assert np.allclose(o['info'], 0., atol=1e-15)
+    assert np.allclose(o['likelihood'], R - tiny['fparams']['sum_fact'], rtol=1e-12)
+    assert abs(o['rescale'] - rescale) <= 1e-14 * rescale
+    # tomogram of P = 1 everywhere is the photon sum over frames / number of frames,
+    # merged once with inter_weight of the one orientation
+    S = tiny['model1'].shape[0]
+    m2 = np.zeros((S,) * 3)
+    w2 = np.zeros((S,) * 3)
+    view = np.where(det['mask'] < 2, K.sum(0), 0) / K.shape[0]
+    core.slice_merge3d(q[0], view, m2, w2, det['qvals'], det['corr'], det['mask'])
+    assert P.rel_err(o['model2'], m2) < 1e-13 and P.rel_err(o['inter_weight'], w2) < 1e-13
+
+
+def test_invariants(tiny_scaled):
+    p = tiny_scaled
+    fp = dict(p['fparams'], scale=np.ones(len(p['frames']['ones'])))
+    o = P.run_oracle(dict(p, fparams=fp), dict(iteration=1, need_scaling=1, update_scale=1))
+    assert np.allclose(o['prob'].sum(1), 1., rtol=1e-13)
+    w = p['quats'][:, 4]
+    assert abs((w * o['u']).sum() * o['rescale'] - fp['mean_count']) < 1e-12 * fp['mean_count']
+    assert np.allclose(o['scale'] * o['psum_d'], fp['fcounts'], rtol=1e-13)
+    # the kept set is exactly {r : R > M - 14/beta}: P of dropped orientations is 0 and the
+    # smallest kept P is above exp(-14) / p_norm
+    kept = o['prob'] > 0
+    assert (o['prob'][kept].reshape(-1) >= 0).all()
+    pmin = np.where(kept, o['prob'], np.inf).min(1)
+    assert (pmin * o['p_norm'] >= np.exp(-14.) * (1 - 1e-9)).all()
+
+
+@pytest.mark.parametrize('mistake', ['dropped_sum_fact', 'mask1_in_likelihood', 'no_log_weight',
+                                     'scale_instead_of_rescale'])
+def test_planted_mistakes_are_detected(tiny, mistake):
+    """Emulate a plausible oracle mistake through its inputs and check that the
+    brute-force comparison flags it."""
+    flags = dict(iteration=1, need_scaling=0)
+    p = dict(tiny)
+    fp = dict(tiny['fparams'])
+    det = dict(tiny['det'])
+    if mistake == 'dropped_sum_fact':
+        fp['sum_fact'] = np.zeros_like(fp['sum_fact'])
+    elif mistake == 'mask1_in_likelihood':
+        m = det['mask'].copy()
+        m[m == 1] = 0
+        det['mask'] = m
+    elif mistake == 'no_log_weight':
+        q = tiny['quats'].copy()
+        q[:, 4] = 1.
+        p['quats'] = q
+    elif mistake == 'scale_instead_of_rescale':
+        flags = dict(iteration=2, need_scaling=1, update_scale=0)
+        fp['scale'] = np.ones(len(fp['scale']))
+    o = P.run_oracle(dict(p, fparams=fp, det=det), flags)
+    b = brute.iterate(tiny['det'], tiny['quats'], tiny['model1'], tiny['frames'], tiny['fparams'],
+                      dict(iteration=1, need_scaling=0))
+    errs = _diff(o, b)
+    assert max(errs.values()) > 1e-6, (mistake, errs)
