@@ -85,6 +85,8 @@ def main():
     ap.add_argument('--num-div', type=int, default=None)
     ap.add_argument('--cpu-rot', type=int, default=3072, help='orientations in the CPU sample')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-calibration-mib', type=int, default=0,
+                    help='run one device copy of this many MiB first (counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build has no multi-GPU path (the work '
@@ -93,6 +95,14 @@ def main():
 
     import torch
     from dragonfly_amd import synth
+    if args.copy_calibration_mib:
+        n = args.copy_calibration_mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device='cuda')
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log('copy calibration: %d MiB read + %d MiB written' % (args.copy_calibration_mib, args.copy_calibration_mib))
+        del a, b
     t0 = time.perf_counter()
     log('generating %s workload ...' % args.config)
     prob = synth.make_problem(args.config, device='cuda', num_data=args.num_data,

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -143,6 +144,16 @@ struct emc_engine {
     DevBuf<long> lik_off, tom_off;
     DevBuf<int> lik_pix, tom_frame;
     DevBuf<double> lik_cnt, tom_cnt;
+    // per (pixel, frame range) offsets into the tomogram list for the frame-partitioned form
+    DevBuf<long> tom_off2;
+    int tom_parts = 8, tomo_width32 = 0;
+    // upper bound on the orientation chunk width (0: memory decides)
+    int max_chunk_cols = 0;
+    DevBuf<double> tom_partial;
+    // tile form of the tomogram list: tiles of 128 engine pixels -> frames -> photons
+    DevBuf<long> ft_off, fe_off;
+    DevBuf<int> ft_frame, fe_code;
+    long tile_frames = 0, tile_entries = 0;
     // per-frame state
     DevBuf<double> scale, beta, sum_fact;
     DevBuf<uint8_t> blacklist;
@@ -161,8 +172,12 @@ struct emc_engine {
     // orientations have psum_r > 0
     DevBuf<double> weight_all;
     bool weight_all_valid = false;
+
     bool have_results = false;
     int last_unmerged = -1;
+    // orientation tiles (of 64) per wavefront in the two photon-gather kernels; tomogram
+    // form (2: frame-partitioned gather, 1: pixel-tile LDS accumulation, 0: per-pixel gather)
+    int lik_tiles = 1, tomo_tiles = 1, tomo_tile_mode = 2;
 };
 
 namespace {
@@ -224,7 +239,76 @@ int build_lists(emc_engine *e) {
             if (mask[p] < 2) { tframe[tfill[q]] = d; tcnt[tfill[q]] = c; ++tfill[q]; }
         }
     }
+    // frame-range offsets: toff2[t * nparts + x] = first photon of pixel t in frame range x
+    const int nparts = e->tom_parts;
+    std::vector<long> toff2((size_t)P * nparts + 1);
+    for (int t = 0; t < P; ++t) {
+        long k = tcount[t];
+        for (int x = 0; x < nparts; ++x) {
+            const long lo = (long)N * x / nparts;
+            while (k < tcount[t + 1] && tframe[k] < lo) ++k;
+            toff2[(size_t)t * nparts + x] = k;
+        }
+    }
+    toff2[(size_t)P * nparts] = tcount[P];
+
+    // tile form: photons of mask < 2 pixels grouped by 128-pixel tile, then frame
+    const int ntiles = (P + 127) / 128;
+    std::vector<long> tf_count(ntiles + 1, 0), te_count(ntiles + 1, 0);
+    std::vector<int> last(ntiles, -1);
+    auto visit = [&](auto &&fn) {
+        for (int d = 0; d < N; ++d) {
+            for (int k = 0; k < e->h_ones[d]; ++k) {
+                int p = e->h_place_ones[e->h_ones_accum[d] + k];
+                if (mask[p] < 2) fn(d, e->inv[p], 1);
+            }
+            for (int k = 0; k < e->h_multi[d]; ++k) {
+                long j = e->h_multi_accum[d] + k;
+                int p = e->h_place_multi[j];
+                if (mask[p] < 2) fn(d, e->inv[p], e->h_count_multi[j]);
+            }
+        }
+    };
+    bool big_count = false;
+    visit([&](int d, int q, int c) {
+        const int tl = q / 128;
+        if (last[tl] != d) { last[tl] = d; ++tf_count[tl + 1]; }
+        ++te_count[tl + 1];
+        if (c >= (1 << 23) || c < 0) big_count = true;
+    });
+    if (big_count) return fail(EMC_E_ARG, "photon count too large for the tile tomogram list");
+    for (int tl = 0; tl < ntiles; ++tl) {
+        tf_count[tl + 1] += tf_count[tl];
+        te_count[tl + 1] += te_count[tl];
+    }
+    const long nf = tf_count[ntiles], ne = te_count[ntiles];
+    std::vector<int> ft_frame(nf > 0 ? nf : 1), fe_code(ne > 0 ? ne : 1);
+    std::vector<long> fe_off(nf + 1, 0);
+    std::vector<long> fpos(tf_count.begin(), tf_count.end() - 1), epos(te_count.begin(), te_count.end() - 1);
+    std::fill(last.begin(), last.end(), -1);
+    // entries of one (tile, frame) are contiguous because frames are visited in order
+    visit([&](int d, int q, int c) {
+        const int tl = q / 128;
+        if (last[tl] != d) {
+            last[tl] = d;
+            ft_frame[fpos[tl]] = d;
+            fe_off[fpos[tl]] = epos[tl];
+            ++fpos[tl];
+        }
+        fe_code[epos[tl]++] = (c << 8) | (q & 127);
+    });
+    fe_off[nf] = ne;
+    // fe_off of the first frame of each tile equals te_count[tile]; fill ends: the frame
+    // after the last of a tile starts the next tile's entries, so fe_off is monotone
+    e->tile_frames = nf;
+    e->tile_entries = ne;
+
     int rc;
+    if ((rc = e->tom_off2.upload(toff2.data(), toff2.size(), e->stream))) return rc;
+    if ((rc = e->ft_off.upload(tf_count.data(), tf_count.size(), e->stream))) return rc;
+    if ((rc = e->ft_frame.upload(ft_frame.data(), ft_frame.size(), e->stream))) return rc;
+    if ((rc = e->fe_off.upload(fe_off.data(), fe_off.size(), e->stream))) return rc;
+    if ((rc = e->fe_code.upload(fe_code.data(), fe_code.size(), e->stream))) return rc;
     if ((rc = e->lik_off.upload(loff.data(), loff.size(), e->stream))) return rc;
     if ((rc = e->lik_pix.upload(lpix.data(), lpix.size() ? lpix.size() : 0, e->stream))) return rc;
     if ((rc = e->lik_cnt.upload(lcnt.data(), lcnt.size(), e->stream))) return rc;
@@ -252,17 +336,23 @@ int ensure_work(emc_engine *e) {
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     const size_t reserve = (size_t)4 << 30;
     size_t budget = freeb > reserve ? freeb - reserve : 0;
-    const size_t cap = (size_t)48 << 30;
+    const size_t cap = (size_t)120 << 30;
     if (e->logw.n) budget += e->logw.n * sizeof(double);
+    if (e->tom_partial.n) budget += e->tom_partial.n * sizeof(double);
     if (budget > cap) budget = cap;
-    size_t cmax = budget / (P * sizeof(double) + 1);
+    // per orientation column: log W / view, plus the partial views of the partitioned tomogram
+    const size_t per_col = P * sizeof(double) * (1 + (e->tomo_tile_mode == 2 ? e->tom_parts : 0));
+    size_t cmax = budget / per_col;
+    if (e->max_chunk_cols > 0 && cmax > (size_t)e->max_chunk_cols) cmax = e->max_chunk_cols;
     cmax = cmax / emc::kRTile * emc::kRTile;
     if (cmax < (size_t)emc::kRTile) return fail(EMC_E_NOMEM, "no device memory for an orientation chunk");
     const int C = (int)(cmax < Rp ? cmax : Rp);
     if (C != e->C || !e->logw.p) {
+        e->tom_partial.release();
         if ((rc = e->logw.alloc(P * (size_t)C))) return rc;
         e->C = C;
     }
+    if (e->tomo_tile_mode == 2 && (rc = e->tom_partial.alloc((size_t)e->tom_parts * P * (size_t)C))) return rc;
     const size_t ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
     if ((rc = e->upart.alloc(ngroups * (size_t)e->C))) return rc;
     if ((rc = e->u.alloc(Rp))) return rc;
@@ -300,6 +390,13 @@ int emc_create(int device, emc_engine **out) {
         return fail(EMC_E_HIP, "hipStreamCreate failed: %s", hipGetErrorString(err));
     }
     for (auto &ev : e->ev) (void)hipEventCreate(&ev);
+    if (const char *v = getenv("EMC_LIK_TILES")) e->lik_tiles = atoi(v);
+    if (const char *v = getenv("EMC_TOMO_TILES")) e->tomo_tiles = atoi(v);
+    if (const char *v = getenv("EMC_TOMO_TILE_MODE")) e->tomo_tile_mode = atoi(v);
+    if (const char *v = getenv("EMC_TOMO_PARTS")) e->tom_parts = std::max(1, atoi(v));
+    if (const char *v = getenv("EMC_TOMO_WIDTH32")) e->tomo_width32 = atoi(v);
+    if (const char *v = getenv("EMC_MAX_CHUNK_COLS")) e->max_chunk_cols = atoi(v);
+
     *out = e;
     return 0;
 }
@@ -319,9 +416,16 @@ void emc_destroy(emc_engine *e) {
     e->d_perm.release();
     e->d_count.release();
     e->weight_all.release();
+
     e->blacklist.release();
     e->lik_off.release();
     e->tom_off.release();
+    e->tom_off2.release();
+    e->tom_partial.release();
+    e->ft_off.release();
+    e->fe_off.release();
+    e->ft_frame.release();
+    e->fe_code.release();
     e->lik_pix.release();
     e->tom_frame.release();
     e->fcounts.release();
@@ -499,7 +603,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
 
     hipStream_t s = e->stream;
     const int N = e->num_data, Rn = e->num_rot, P = e->num_pix, C = e->C;
-    const size_t ldR = e->Rp, vol = (size_t)e->size * e->size * e->size;
+    const size_t vol = (size_t)e->size * e->size * e->size;
     const int ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
     const int use_scale = opts->need_scaling && (opts->iteration > 1 || opts->known_scale);
     const int want_psum_d = opts->need_scaling && opts->update_scale;
@@ -518,8 +622,8 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
                            P, r0, Cc, kPixPerGroup, e->logw.p, e->upart.p);
         emc::launch_u_reduce(s, e->upart.p, ngroups, Cc, r0, Rn, e->u.p);
         HIP_TRY(hipEventRecord(e->ev[1], s));
-        emc::launch_likelihood(s, e->logw.p, Cc, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
-                               e->blacklist.p, N, e->R.p, ldR, r0);
+        emc::launch_likelihood(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
+                               e->blacklist.p, N, e->R.p, r0, e->lik_tiles);
         HIP_TRY(hipEventRecord(e->ev[2], s));
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventSynchronize(e->ev[2]));
@@ -531,16 +635,16 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     HIP_TRY(hipEventRecord(e->ev[3], s));
     emc::launch_rescale(s, e->quats.p, e->u.p, Rn, e->mean_count, e->scalars.p);
     HIP_TRY(hipEventRecord(e->ev[4], s));
-    emc::launch_frame_max(s, e->R.p, ldR, Rn, N, e->logwr.p, e->u.p, e->scale.p, use_scale,
+    emc::launch_frame_max(s, e->R.p, Rn, N, e->logwr.p, e->u.p, e->scale.p, use_scale,
                           e->scalars.p, e->blacklist.p, e->max_exp.p, e->rmax.p);
     HIP_TRY(hipEventRecord(e->ev[5], s));
-    emc::launch_frame_norm(s, e->R.p, ldR, Rn, N, e->beta.p, e->blacklist.p, e->max_exp.p,
+    emc::launch_frame_norm(s, e->R.p, Rn, N, e->beta.p, e->blacklist.p, e->max_exp.p,
                            e->rmax.p, e->p_norm.p);
     HIP_TRY(hipEventRecord(e->ev[6], s));
-    emc::launch_posterior(s, e->R.p, ldR, Rn, N, e->quats.p, e->u.p, e->beta.p, e->sum_fact.p,
+    emc::launch_posterior(s, e->R.p, Rn, N, e->quats.p, e->u.p, e->beta.p, e->sum_fact.p,
                           e->blacklist.p, e->max_exp.p, e->rmax.p, e->p_norm.p, e->scalars.p,
                           want_psum_d, e->likelihood.p, e->info.p, e->psum_d.p);
-    emc::launch_psum_r(s, e->R.p, ldR, Rn, N, kFramesPerPsumPart, e->scale.p, opts->need_scaling,
+    emc::launch_psum_r(s, e->R.p, Rn, N, kFramesPerPsumPart, e->scale.p, opts->need_scaling,
                        e->blacklist.p, e->psum_part.p, e->psum_r.p);
     HIP_TRY(hipEventRecord(e->ev[7], s));
     HIP_TRY(hipGetLastError());
@@ -587,8 +691,16 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
         float a = 0.f, b = 0.f;
         HIP_TRY(hipEventRecord(e->ev[0], s));
-        emc::launch_tomogram(s, e->R.p, ldR, r0, Cc, e->tom_off.p, e->tom_frame.p, e->tom_cnt.p,
-                             e->blacklist.p, e->psum_r.p, Rn, P, e->logw.p);
+        if (e->tomo_tile_mode == 2)
+            emc::launch_tomogram_parts(s, e->R.p, N, r0, Cc, e->tom_off2.p, e->tom_parts,
+                                       e->tom_frame.p, e->tom_cnt.p, e->psum_r.p, Rn, P,
+                                       e->tom_partial.p, e->logw.p, e->tomo_width32);
+        else if (e->tomo_tile_mode == 1)
+            emc::launch_tomogram_tiles(s, e->R.p, N, r0, Cc, e->ft_off.p, e->ft_frame.p,
+                                       e->fe_off.p, e->fe_code.p, e->psum_r.p, Rn, P, e->logw.p);
+        else
+            emc::launch_tomogram(s, e->R.p, N, r0, Cc, e->tom_off.p, e->tom_frame.p,
+                                 e->tom_cnt.p, e->psum_r.p, Rn, P, e->logw.p, e->tomo_tiles);
         HIP_TRY(hipEventRecord(e->ev[1], s));
         emc::launch_merge(s, e->rots.p, r0, Cc, Rn, e->logw.p, e->psum_r.p, 0, e->qvals.p,
                           e->corr.p, e->mask.p, P, e->size, e->model2.p, e->weight.p, 1,
@@ -709,8 +821,13 @@ int emc_get_field(emc_engine *e, int field, void *dst, size_t count) {
         case EMC_FIELD_PROB: {
             if (!e->have_results) return fail(EMC_E_STATE, "emc_get_field: no iteration has run");
             if (count != N * Rn) return fail(EMC_E_ARG, "emc_get_field: count %zu != %zu", count, N * Rn);
-            HIP_TRY(hipMemcpy2DAsync(dst, Rn * sizeof(double), e->R.p, e->Rp * sizeof(double),
-                                     Rn * sizeof(double), N, hipMemcpyDeviceToHost, e->stream));
+            // tile-major [Rp/64][N][64] to the caller's [N][num_rot]
+            for (size_t j = 0; j * emc::kRTile < Rn; ++j) {
+                const size_t w = std::min((size_t)emc::kRTile, Rn - j * emc::kRTile);
+                HIP_TRY(hipMemcpy2DAsync((double *)dst + j * emc::kRTile, Rn * sizeof(double),
+                                         e->R.p + j * N * emc::kRTile, emc::kRTile * sizeof(double),
+                                         w * sizeof(double), N, hipMemcpyDeviceToHost, e->stream));
+            }
             HIP_TRY(hipStreamSynchronize(e->stream));
             return 0;
         }

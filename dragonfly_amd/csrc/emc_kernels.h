@@ -15,26 +15,34 @@ void launch_u_reduce(hipStream_t s, const double *upart, int ngroups, int C, int
                      double *u);
 void launch_rescale(hipStream_t s, const double *quats, const double *u, int num_rot,
                     double mean_count, double *scalars);
-void launch_likelihood(hipStream_t s, const double *logw, int C, const long *off, const int *pix,
-                       const double *cnt, const uint8_t *blacklist, int num_data, double *G,
-                       size_t ldG, int r0);
-void launch_frame_max(hipStream_t s, double *R, size_t ldR, int num_rot, int num_data,
+void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
+                       const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
+                       double *G, int r0, int tiles);
+void launch_frame_max(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *logwr, const double *u, const double *scale, int use_scale,
                       const double *scalars, const uint8_t *blacklist, double *max_exp, int *rmax);
-void launch_frame_norm(hipStream_t s, const double *R, size_t ldR, int num_rot, int num_data,
+void launch_frame_norm(hipStream_t s, const double *R, int num_rot, int num_data,
                        const double *beta, const uint8_t *blacklist, const double *max_exp,
                        const int *rmax, double *p_norm);
-void launch_posterior(hipStream_t s, double *R, size_t ldR, int num_rot, int num_data,
+void launch_posterior(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *quats, const double *u, const double *beta,
                       const double *sum_fact, const uint8_t *blacklist, const double *max_exp,
                       const int *rmax, const double *p_norm, const double *scalars,
                       int want_psum_d, double *likelihood, double *info, double *psum_d);
-void launch_psum_r(hipStream_t s, const double *P, size_t ldP, int num_rot, int num_data,
+void launch_psum_r(hipStream_t s, const double *P, int num_rot, int num_data,
                    int frames_per_part, const double *scale, int need_scaling,
                    const uint8_t *blacklist, double *part, double *psum_r);
-void launch_tomogram(hipStream_t s, const double *P, size_t ldP, int r0, int C, const long *toff,
-                     const int *tframe, const double *tcnt, const uint8_t *blacklist,
-                     const double *psum_r, int num_rot, int num_pix, double *view);
+void launch_tomogram(hipStream_t s, const double *P, int num_data, int r0, int C,
+                     const long *toff, const int *tframe, const double *tcnt, const double *psum_r,
+                     int num_rot, int num_pix, double *view, int tiles);
+void launch_tomogram_tiles(hipStream_t s, const double *P, int num_data, int r0, int C,
+                           const long *ft_off, const int *ft_frame, const long *fe_off,
+                           const int *fe_code, const double *psum_r, int num_rot, int num_pix,
+                           double *view);
+void launch_tomogram_parts(hipStream_t s, const double *P, int num_data, int r0, int C,
+                           const long *toff2, int nparts, const int *tframe, const double *tcnt,
+                           const double *psum_r, int num_rot, int num_pix, double *partial,
+                           double *view, int width32);
 void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot, const double *view,
                   const double *psum_r, int all_on, const double *qvals, const double *corr,
                   const uint8_t *mask, int num_pix, long size, double *model2, double *weight,

@@ -1,11 +1,14 @@
 // HIP kernels of the plain FP64 EMC iteration for gfx950 (MI355X).
 //
 // Layout in HBM (all row-major):
-//   logW / view   [num_pix][C]      one orientation chunk of C (multiple of 64) columns:
-//                                   pixel-major, orientation-contiguous, so a photon's row is
-//                                   a 512-byte coalesced read by one wavefront
-//   R / P         [num_data][Rp]    per-frame log-likelihood, then posterior, in place;
-//                                   Rp = num_rot rounded up to 64
+//   logW / view   [C/64][num_pix][64]   one orientation chunk of C (multiple of 64) columns,
+//                                       tile-major: the 64-orientation slab of every pixel is one
+//                                       contiguous block and a photon's row is a 512-byte
+//                                       coalesced read by one wavefront
+//   R / P         [Rp/64][num_data][64] per-frame log-likelihood, then posterior, in place;
+//                                       Rp = num_rot rounded up to 64, same tile-major form
+// Element (row i, orientation r) of a tile-major buffer with tile stride ts is at
+// (r / 64) * ts + i * 64 + r % 64.
 //   model         [size][size][size] x-major (index x*size*size + y*size + z)
 // Every kernel keeps the reference's operation order for the values it computes
 // (the library is compiled with -ffp-contract=off, so no multiply-add is fused).
@@ -122,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
             if (tt < t_end) {
                 const double W = s_tile[p][lane];
                 if (mask[tt] < 1) uacc += W;
-                logw[(size_t)tt * C + c0 + lane] = log(W);
+                logw[(size_t)(c0 >> 6) * ((size_t)num_pix * kRTile) + (size_t)tt * kRTile + lane] = log(W);
             }
         }
         __syncthreads();
@@ -161,32 +164,50 @@ __global__ __launch_bounds__(kBlock) void rescale_kernel(const double *__restric
 }
 
 // ---- likelihood: G_dr = sum_t K_dt log W_rt over one frame's photons (mask < 1) ------------
-// One wavefront = one frame x 64 orientations; the frame's (pixel, count) list is read
-// with scalar loads (wave-uniform) and each photon costs one 512-byte row read of logW.
+// One wavefront = one frame x NT*64 orientations; the frame's (pixel, count) list is read
+// with scalar loads (wave-uniform) and each photon costs NT 512-byte row reads of logW.
+// Row loads of several photons are issued before the additions, which stay in list order.
+template <int NT>
 __global__ __launch_bounds__(kBlock) void likelihood_kernel(
-    const double *__restrict__ logw, int C, const long *__restrict__ off,
+    const double *__restrict__ logw, int C, size_t tsW, const long *__restrict__ off,
     const int *__restrict__ pix, const double *__restrict__ cnt,
-    const uint8_t *__restrict__ blacklist, int num_data, double *__restrict__ G, size_t ldG,
+    const uint8_t *__restrict__ blacklist, int num_data, double *__restrict__ G, size_t tsG,
     int r0) {
+    constexpr int U = NT >= 4 ? 2 : (NT == 2 ? 4 : 8);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int d = blockIdx.x * kWavesPerBlock + wave;
     if (d >= num_data || blacklist[d]) return;
-    const int c = blockIdx.y * kRTile + lane;
-    const double *__restrict__ col = logw + c;
+    const int c0 = blockIdx.y * (kRTile * NT);
+    const int nt = min(NT, (C - c0) / kRTile);
+    const double *__restrict__ col = logw + (size_t)(c0 >> 6) * tsW + lane;
     const long k1 = off[d + 1];
     long k = off[d];
-    double acc = 0.;
-    // eight independent row loads in flight, then the additions in list order
-    for (; k + 8 <= k1; k += 8) {
-        double v[8];
+    double acc[NT];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = col[(size_t)pix[k + j] * C];
+    for (int j = 0; j < NT; ++j) acc[j] = 0.;
+    for (; k + U <= k1; k += U) {
+        double v[U][NT];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) acc += cnt[k + j] * v[j];
+        for (int u = 0; u < U; ++u) {
+            const size_t row = (size_t)pix[k + u] * kRTile;
+#pragma unroll
+            for (int j = 0; j < NT; ++j) v[u][j] = (j < nt) ? col[row + tsW * j] : 0.;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int j = 0; j < NT; ++j) acc[j] += cnt[k + u] * v[u][j];
     }
-    for (; k < k1; ++k) acc += cnt[k] * col[(size_t)pix[k] * C];
-    G[(size_t)d * ldG + r0 + c] = acc;
+    for (; k < k1; ++k) {
+        const size_t row = (size_t)pix[k] * kRTile;
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+            if (j < nt) acc[j] += cnt[k] * col[row + tsW * j];
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+        if (j < nt) G[(size_t)((r0 + c0) / kRTile + j) * tsG + (size_t)d * kRTile + lane] = acc[j];
 }
 
 // ---- block helpers -----------------------------------------------------------------------
@@ -205,7 +226,7 @@ __device__ __forceinline__ double block_sum(double v, double *s) {
 // ---- per-frame maximum: R = log w_r - phi_d u_r + G, first arg-max (max_prob.c:122-127,
 // :199-203, normalize_prob :223-241) ----------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void frame_max_kernel(
-    double *__restrict__ R, size_t ldR, int num_rot, const double *__restrict__ logwr,
+    double *__restrict__ R, size_t tsR, int num_rot, const double *__restrict__ logwr,
     const double *__restrict__ u, const double *__restrict__ scale, int use_scale,
     const double *__restrict__ scalars, const uint8_t *__restrict__ blacklist,
     double *__restrict__ max_exp, int *__restrict__ rmax) {
@@ -220,12 +241,13 @@ __global__ __launch_bounds__(kBlock) void frame_max_kernel(
         return;
     }
     const double phi = use_scale ? scale[d] : scalars[0];
-    double *row = R + (size_t)d * ldR;
+    double *row = R + (size_t)d * kRTile;
     double best = kMaxExpStart;
     int bi = -1;
     for (int r = threadIdx.x; r < num_rot; r += kBlock) {
-        const double v = (logwr[r] - u[r] * phi) + row[r];
-        row[r] = v;
+        double &x = row[(size_t)(r >> 6) * tsR + (r & 63)];
+        const double v = (logwr[r] - u[r] * phi) + x;
+        x = v;
         if (v > best) {
             best = v;
             bi = r;
@@ -260,7 +282,7 @@ __device__ __forceinline__ bool kept(double v, int r, int rf, double M, double c
 
 // p_norm_d = sum_kept exp(beta (R - M)) (normalize_prob, max_prob.c:245-250).
 __global__ __launch_bounds__(kBlock) void frame_norm_kernel(
-    const double *__restrict__ R, size_t ldR, int num_rot, const double *__restrict__ beta,
+    const double *__restrict__ R, size_t tsR, int num_rot, const double *__restrict__ beta,
     const uint8_t *__restrict__ blacklist, const double *__restrict__ max_exp,
     const int *__restrict__ rmax, double *__restrict__ p_norm) {
     __shared__ double s[kBlock];
@@ -271,10 +293,10 @@ __global__ __launch_bounds__(kBlock) void frame_norm_kernel(
         return;
     }
     const double b = beta[d], M = max_exp[d], c = kPdiffThresh / b;
-    const double *row = R + (size_t)d * ldR;
+    const double *row = R + (size_t)d * kRTile;
     double acc = 0.;
     for (int r = threadIdx.x; r < num_rot; r += kBlock) {
-        const double v = row[r];
+        const double v = row[(size_t)(r >> 6) * tsR + (r & 63)];
         if (kept(v, r, rf, M, c)) acc += exp(b * (v - M));
     }
     const double tot = block_sum(acc, s);
@@ -285,7 +307,7 @@ __global__ __launch_bounds__(kBlock) void frame_norm_kernel(
 // P = exp(beta (R - M)) / p_norm on the kept set, 0 elsewhere (written in place);
 // likelihood += P (R - sum_fact); psum_d += P u_r rescale; info += P log(P / w_r) for P >= 1e-6.
 __global__ __launch_bounds__(kBlock) void posterior_kernel(
-    double *__restrict__ R, size_t ldR, int num_rot, const double *__restrict__ quats,
+    double *__restrict__ R, size_t tsR, int num_rot, const double *__restrict__ quats,
     const double *__restrict__ u, const double *__restrict__ beta,
     const double *__restrict__ sum_fact, const uint8_t *__restrict__ blacklist,
     const double *__restrict__ max_exp, const int *__restrict__ rmax,
@@ -293,10 +315,10 @@ __global__ __launch_bounds__(kBlock) void posterior_kernel(
     double *__restrict__ likelihood, double *__restrict__ info, double *__restrict__ psum_d) {
     __shared__ double s[kBlock];
     const int d = blockIdx.x;
-    double *row = R + (size_t)d * ldR;
+    double *row = R + (size_t)d * kRTile;
     const int rf = rmax[d];
     if (blacklist[d] || rf < 0) {
-        for (int r = threadIdx.x; r < num_rot; r += kBlock) row[r] = 0.;
+        for (int r = threadIdx.x; r < num_rot; r += kBlock) row[(size_t)(r >> 6) * tsR + (r & 63)] = 0.;
         if (threadIdx.x == 0) {
             likelihood[d] = 0.;
             info[d] = 0.;
@@ -308,7 +330,8 @@ __global__ __launch_bounds__(kBlock) void posterior_kernel(
     const double sf = sum_fact[d], rescale = scalars[0];
     double lk = 0., inf = 0., ps = 0.;
     for (int r = threadIdx.x; r < num_rot; r += kBlock) {
-        const double v = row[r];
+        double &x = row[(size_t)(r >> 6) * tsR + (r & 63)];
+        const double v = x;
         double prob = 0.;
         if (kept(v, r, rf, M, c)) {
             prob = exp(b * (v - M)) / pn;
@@ -316,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void posterior_kernel(
             ps += prob * u[r] * rescale;
             if (!(prob < kProbMin)) inf += prob * log(prob / quats[5 * (size_t)r + 4] * 1);
         }
-        row[r] = prob;
+        x = prob;
     }
     const double tl = block_sum(lk, s);
     const double ti = block_sum(inf, s);
@@ -330,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void posterior_kernel(
 
 // psum_r = sum_d P_dr phi'_d, phi' = scale if need_scaling else 1 (max_update.c:108-116).
 __global__ __launch_bounds__(kBlock) void psum_r_partial_kernel(
-    const double *__restrict__ P, size_t ldP, int num_rot, int num_data, int frames_per_part,
+    const double *__restrict__ P, size_t tsP, int num_rot, int num_data, int frames_per_part,
     const double *__restrict__ scale, int need_scaling, const uint8_t *__restrict__ blacklist,
     double *__restrict__ part) {
     const int r = blockIdx.x * kBlock + threadIdx.x;
@@ -340,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void psum_r_partial_kernel(
     double acc = 0.;
     for (int d = d0; d < d1; ++d) {
         if (blacklist[d]) continue;
-        const double p = P[(size_t)d * ldP + r];
+        const double p = P[(size_t)(r >> 6) * tsP + (size_t)d * kRTile + (r & 63)];
         acc += need_scaling ? p * scale[d] : p;
     }
     part[(size_t)part_id * num_rot + r] = acc;
@@ -357,38 +380,213 @@ __global__ void psum_r_reduce_kernel(const double *__restrict__ part, int nparts
 
 // ---- tomogram: view_rt = sum_{d: P > 1e-6} P_dr K_dt / psum_r (update_tomogram_nobg,
 // max_update.c:142-230), gathered per pixel over its frames (d ascending, as the
-// reference accumulates) ---------------------------------------------------------------------
+// reference accumulates). One wavefront = one pixel x NT*64 orientations. P rows of
+// blacklisted frames are zero (posterior_kernel), so they add nothing here.
+template <int NT>
 __global__ __launch_bounds__(kBlock) void tomogram_kernel(
-    const double *__restrict__ P, size_t ldP, int r0, int C, const long *__restrict__ toff,
-    const int *__restrict__ tframe, const double *__restrict__ tcnt,
-    const uint8_t *__restrict__ blacklist, const double *__restrict__ psum_r, int num_rot,
-    int num_pix, double *__restrict__ view) {
+    const double *__restrict__ P, size_t tsP, int r0, int C, size_t tsW,
+    const long *__restrict__ toff, const int *__restrict__ tframe, const double *__restrict__ tcnt,
+    const double *__restrict__ psum_r, int num_rot, int num_pix, double *__restrict__ view) {
+    constexpr int U = NT >= 4 ? 2 : (NT == 2 ? 4 : 8);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int t = blockIdx.x * kWavesPerBlock + wave;
     if (t >= num_pix) return;
-    const int c = blockIdx.y * kRTile + lane;
-    const double *__restrict__ col = P + r0 + c;
+    const int c0 = blockIdx.y * (kRTile * NT);
+    const int nt = min(NT, (C - c0) / kRTile);
+    const double *__restrict__ col = P + (size_t)((r0 + c0) >> 6) * tsP + lane;
     const long k1 = toff[t + 1];
     long k = toff[t];
-    double acc = 0.;
-    // P rows of blacklisted frames are zero (posterior_kernel), so they add nothing here.
-    // Eight independent row loads in flight, then the additions in frame order.
-    for (; k + 8 <= k1; k += 8) {
-        double v[8];
+    double acc[NT];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = col[(size_t)tframe[k + j] * ldP];
+    for (int j = 0; j < NT; ++j) acc[j] = 0.;
+    for (; k + U <= k1; k += U) {
+        double v[U][NT];
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if (v[j] > kProbMin) acc += tcnt[k + j] * v[j];
+        for (int u = 0; u < U; ++u) {
+            const size_t row = (size_t)tframe[k + u] * kRTile;
+#pragma unroll
+            for (int j = 0; j < NT; ++j) v[u][j] = (j < nt) ? col[row + tsP * j] : 0.;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+                if (v[u][j] > kProbMin) acc[j] += tcnt[k + u] * v[u][j];
     }
     for (; k < k1; ++k) {
-        const double p = col[(size_t)tframe[k] * ldP];
+        const size_t row = (size_t)tframe[k] * kRTile;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const double p = (j < nt) ? col[row + tsP * j] : 0.;
+            if (p > kProbMin) acc[j] += tcnt[k] * p;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        if (j >= nt) continue;
+        const int r = r0 + c0 + kRTile * j + lane;
+        const double ps = (r < num_rot) ? psum_r[r] : 0.;
+        view[(size_t)((c0 >> 6) + j) * tsW + (size_t)t * kRTile + lane] = (ps > 0.) ? acc[j] / ps : acc[j];
+    }
+}
+
+// Frame-partitioned form of the tomogram. Frames are split into nparts contiguous ranges and
+// block b works on range b % nparts: with blocks dispatched round-robin over the 8 XCDs
+// (nparts a multiple of 8), each XCD's L2 holds only its own frames' P slab of the
+// orientation tile. Each range writes a partial view; tomogram_reduce_kernel adds the
+// partials in range order and divides by psum_r. toff2[t * nparts + x] is the first photon of
+// pixel t with a frame in range x (the per-pixel list is frame-ascending).
+__global__ __launch_bounds__(kBlock) void tomogram_part_kernel(
+    const double *__restrict__ P, size_t tsP, int r0, int C, const long *__restrict__ toff2,
+    int nparts, const int *__restrict__ tframe, const double *__restrict__ tcnt, int num_pix,
+    double *__restrict__ partial) {
+    constexpr int U = 8;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int part = blockIdx.x % nparts;
+    const int t = (blockIdx.x / nparts) * kWavesPerBlock + wave;
+    if (t >= num_pix) return;
+    const int c0 = blockIdx.y * kRTile;
+    const double *__restrict__ col = P + (size_t)((r0 + c0) >> 6) * tsP + lane;
+    const size_t at = (size_t)t * nparts + part;
+    long k = toff2[at];
+    const long k1 = toff2[at + 1];
+    double acc = 0.;
+    for (; k + U <= k1; k += U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = col[(size_t)tframe[k + u] * kRTile];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (v[u] > kProbMin) acc += tcnt[k + u] * v[u];
+    }
+    for (; k < k1; ++k) {
+        const double p = col[(size_t)tframe[k] * kRTile];
         if (p > kProbMin) acc += tcnt[k] * p;
     }
-    const double ps = (r0 + c < num_rot) ? psum_r[r0 + c] : 0.;
-    if (ps > 0.) acc /= ps;
-    view[(size_t)t * C + c] = acc;
+    const size_t tsW = (size_t)num_pix * kRTile, tsPart = tsW * (C / kRTile);
+    partial[part * tsPart + (size_t)(c0 >> 6) * tsW + (size_t)t * kRTile + lane] = acc;
+}
+
+// Same as tomogram_part_kernel with a wavefront covering 32 orientations: lanes 0-31 take the
+// even photons of the pixel's list and lanes 32-63 the odd ones (two 256-byte P segments per
+// load instruction); the halves are added at the end. The L2 slab of one frame range for one
+// 32-orientation block is half as large.
+__global__ __launch_bounds__(kBlock) void tomogram_part32_kernel(
+    const double *__restrict__ P, size_t tsP, int r0, int C, const long *__restrict__ toff2,
+    int nparts, const int *__restrict__ tframe, const double *__restrict__ tcnt, int num_pix,
+    double *__restrict__ partial) {
+    constexpr int U = 8;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, half = lane >> 5;
+    const int part = blockIdx.x % nparts;
+    const int t = (blockIdx.x / nparts) * kWavesPerBlock + wave;
+    if (t >= num_pix) return;
+    const int c0 = blockIdx.y * (kRTile / 2);          // chunk-local first orientation (multiple of 32)
+    const int rg = r0 + c0;                             // global first orientation
+    const double *__restrict__ col = P + (size_t)(rg >> 6) * tsP + (rg & 63) + (lane & 31);
+    const size_t at = (size_t)t * nparts + part;
+    long k = toff2[at];
+    const long k1 = toff2[at + 1];
+    double acc = 0.;
+    for (; k + 2 * U <= k1; k += 2 * U) {
+        double v[U], c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long kk = k + 2 * u + half;
+            v[u] = col[(size_t)tframe[kk] * kRTile];
+            c[u] = tcnt[kk];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (v[u] > kProbMin) acc += c[u] * v[u];
+    }
+    for (; k < k1; k += 2) {
+        const long kk = k + half;
+        if (kk < k1) {
+            const double p = col[(size_t)tframe[kk] * kRTile];
+            if (p > kProbMin) acc += tcnt[kk] * p;
+        }
+    }
+    acc += __shfl_down(acc, 32);
+    if (half == 0) {
+        const size_t tsW = (size_t)num_pix * kRTile, tsPart = tsW * (C / kRTile);
+        partial[part * tsPart + (size_t)(c0 >> 6) * tsW + (size_t)t * kRTile + (c0 & 63) + lane] = acc;
+    }
+}
+
+__global__ void tomogram_reduce_kernel(const double *__restrict__ partial, int nparts, int r0,
+                                       int C, int num_pix, const double *__restrict__ psum_r,
+                                       int num_rot, double *__restrict__ view) {
+    const size_t tsW = (size_t)num_pix * kRTile, n = tsW * (C / kRTile);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        double acc = 0.;
+        for (int x = 0; x < nparts; ++x) acc += partial[x * n + i];
+        const int r = r0 + (int)(i / tsW) * kRTile + (int)(i % kRTile);
+        const double ps = (r < num_rot) ? psum_r[r] : 0.;
+        view[i] = (ps > 0.) ? acc / ps : acc;
+    }
+}
+
+// Tile form of the tomogram. Engine pixels are grouped in tiles of kTomoTile consecutive
+// pixels (compact detector patches, Morton order). For a tile the photons are listed frame by
+// frame: ft_off[tile]..ft_off[tile+1] index the tile's distinct frames (ft_frame), and
+// fe_off[f]..fe_off[f+1] that frame's photons in the tile (fe_code = count << 8 | local pixel).
+// A block = one tile x 64 orientations: each frame's P row segment is loaded once and added
+// into LDS accumulators for every photon of that frame in the tile, so bright tiles reuse
+// each 512-byte P load several times. Accumulation order across waves is not fixed.
+constexpr int kTomoTile = 128;
+
+__global__ __launch_bounds__(kBlock) void tomogram_tile_kernel(
+    const double *__restrict__ P, size_t tsP, int r0, int C, size_t tsW,
+    const long *__restrict__ ft_off, const int *__restrict__ ft_frame, const long *__restrict__ fe_off,
+    const int *__restrict__ fe_code, const double *__restrict__ psum_r, int num_rot, int num_pix,
+    double *__restrict__ view) {
+    __shared__ double s_acc[kTomoTile][kRTile];
+    const int tile = blockIdx.x;
+    const int c0 = blockIdx.y * kRTile;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int i = threadIdx.x; i < kTomoTile * kRTile; i += kBlock) (&s_acc[0][0])[i] = 0.;
+    __syncthreads();
+    const double *__restrict__ col = P + (size_t)((r0 + c0) >> 6) * tsP + lane;
+    const long f1 = ft_off[tile + 1];
+    long f = ft_off[tile] + wave;
+    // four frames per step (one per wave of the block), two frames' rows in flight per wave
+    for (; f + kWavesPerBlock < f1; f += 2 * kWavesPerBlock) {
+        const long g = f + kWavesPerBlock;
+        const double pa = col[(size_t)ft_frame[f] * kRTile];
+        const double pb = col[(size_t)ft_frame[g] * kRTile];
+        if (pa > kProbMin)
+            for (long e = fe_off[f]; e < fe_off[f + 1]; ++e) {
+                const int code = fe_code[e];
+                unsafeAtomicAdd(&s_acc[code & 255][lane], (double)(code >> 8) * pa);
+            }
+        if (pb > kProbMin)
+            for (long e = fe_off[g]; e < fe_off[g + 1]; ++e) {
+                const int code = fe_code[e];
+                unsafeAtomicAdd(&s_acc[code & 255][lane], (double)(code >> 8) * pb);
+            }
+    }
+    for (; f < f1; f += kWavesPerBlock) {
+        const double pa = col[(size_t)ft_frame[f] * kRTile];
+        if (pa > kProbMin)
+            for (long e = fe_off[f]; e < fe_off[f + 1]; ++e) {
+                const int code = fe_code[e];
+                unsafeAtomicAdd(&s_acc[code & 255][lane], (double)(code >> 8) * pa);
+            }
+    }
+    __syncthreads();
+    const int r = r0 + c0 + lane;
+    const double ps = (r < num_rot) ? psum_r[r] : 0.;
+    for (int lp = wave; lp < kTomoTile; lp += kWavesPerBlock) {
+        const int t = tile * kTomoTile + lp;
+        if (t >= num_pix) break;
+        const double acc = s_acc[lp][lane];
+        view[(size_t)(c0 >> 6) * tsW + (size_t)t * kRTile + lane] = (ps > 0.) ? acc / ps : acc;
+    }
 }
 
 // ---- merge: scatter views into model2 / inter_weight (merge_tomogram, max_update.c:33-36;
@@ -461,7 +659,7 @@ __global__ __launch_bounds__(kBlock) void merge_box_kernel(
         Cell cell;
         const bool ok = pix_ok && locate(&s_rot[cc * 9], q0, q1, q2, center, size, cell);
         double w = 0.;
-        if (kModel && ok) w = view[(size_t)t * C + c0 + cc] / cr;
+        if (kModel && ok) w = view[(size_t)(c0 >> 6) * ((size_t)num_pix * kRTile) + (size_t)t * kRTile + cc] / cr;
         const int xmax = wave_max(ok ? (int)cell.x : -1);
         if (xmax < 0) continue;
         const int xmin = wave_min(ok ? (int)cell.x : INT_MAX);
@@ -724,53 +922,94 @@ void launch_rescale(hipStream_t s, const double *quats, const double *u, int num
     rescale_kernel<<<1, kBlock, 0, s>>>(quats, u, num_rot, mean_count, scalars);
 }
 
-void launch_likelihood(hipStream_t s, const double *logw, int C, const long *off, const int *pix,
-                       const double *cnt, const uint8_t *blacklist, int num_data, double *G,
-                       size_t ldG, int r0) {
-    dim3 grid(cdiv(num_data, kWavesPerBlock), C / kRTile);
-    likelihood_kernel<<<grid, kBlock, 0, s>>>(logw, C, off, pix, cnt, blacklist, num_data, G, ldG, r0);
+void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
+                       const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
+                       double *G, int r0, int tiles) {
+    const unsigned gx = cdiv(num_data, kWavesPerBlock);
+    const size_t tsW = (size_t)num_pix * kRTile, tsG = (size_t)num_data * kRTile;
+#define EMC_LIK(NT)                                                                              \
+    likelihood_kernel<NT><<<dim3(gx, cdiv(C, kRTile * NT)), kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, \
+                                                                          blacklist, num_data, G, tsG, r0)
+    if (tiles >= 4) EMC_LIK(4);
+    else if (tiles == 2) EMC_LIK(2);
+    else EMC_LIK(1);
+#undef EMC_LIK
 }
 
-void launch_frame_max(hipStream_t s, double *R, size_t ldR, int num_rot, int num_data,
+void launch_frame_max(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *logwr, const double *u, const double *scale, int use_scale,
                       const double *scalars, const uint8_t *blacklist, double *max_exp, int *rmax) {
-    frame_max_kernel<<<num_data, kBlock, 0, s>>>(R, ldR, num_rot, logwr, u, scale, use_scale,
+    frame_max_kernel<<<num_data, kBlock, 0, s>>>(R, (size_t)num_data * kRTile, num_rot, logwr, u, scale, use_scale,
                                                  scalars, blacklist, max_exp, rmax);
 }
 
-void launch_frame_norm(hipStream_t s, const double *R, size_t ldR, int num_rot, int num_data,
+void launch_frame_norm(hipStream_t s, const double *R, int num_rot, int num_data,
                        const double *beta, const uint8_t *blacklist, const double *max_exp,
                        const int *rmax, double *p_norm) {
-    frame_norm_kernel<<<num_data, kBlock, 0, s>>>(R, ldR, num_rot, beta, blacklist, max_exp, rmax,
+    frame_norm_kernel<<<num_data, kBlock, 0, s>>>(R, (size_t)num_data * kRTile, num_rot, beta, blacklist, max_exp, rmax,
                                                   p_norm);
 }
 
-void launch_posterior(hipStream_t s, double *R, size_t ldR, int num_rot, int num_data,
+void launch_posterior(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *quats, const double *u, const double *beta,
                       const double *sum_fact, const uint8_t *blacklist, const double *max_exp,
                       const int *rmax, const double *p_norm, const double *scalars,
                       int want_psum_d, double *likelihood, double *info, double *psum_d) {
-    posterior_kernel<<<num_data, kBlock, 0, s>>>(R, ldR, num_rot, quats, u, beta, sum_fact,
+    posterior_kernel<<<num_data, kBlock, 0, s>>>(R, (size_t)num_data * kRTile, num_rot, quats, u, beta, sum_fact,
                                                  blacklist, max_exp, rmax, p_norm, scalars,
                                                  want_psum_d, likelihood, info, psum_d);
 }
 
-void launch_psum_r(hipStream_t s, const double *P, size_t ldP, int num_rot, int num_data,
+void launch_psum_r(hipStream_t s, const double *P, int num_rot, int num_data,
                    int frames_per_part, const double *scale, int need_scaling,
                    const uint8_t *blacklist, double *part, double *psum_r) {
     const int nparts = (int)cdiv(num_data, frames_per_part);
     dim3 grid(cdiv(num_rot, kBlock), nparts);
-    psum_r_partial_kernel<<<grid, kBlock, 0, s>>>(P, ldP, num_rot, num_data, frames_per_part,
+    psum_r_partial_kernel<<<grid, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, num_rot, num_data, frames_per_part,
                                                   scale, need_scaling, blacklist, part);
     psum_r_reduce_kernel<<<cdiv(num_rot, 256), 256, 0, s>>>(part, nparts, num_rot, psum_r);
 }
 
-void launch_tomogram(hipStream_t s, const double *P, size_t ldP, int r0, int C, const long *toff,
-                     const int *tframe, const double *tcnt, const uint8_t *blacklist,
-                     const double *psum_r, int num_rot, int num_pix, double *view) {
-    dim3 grid(cdiv(num_pix, kWavesPerBlock), C / kRTile);
-    tomogram_kernel<<<grid, kBlock, 0, s>>>(P, ldP, r0, C, toff, tframe, tcnt, blacklist, psum_r,
-                                            num_rot, num_pix, view);
+void launch_tomogram(hipStream_t s, const double *P, int num_data, int r0, int C,
+                     const long *toff, const int *tframe, const double *tcnt, const double *psum_r,
+                     int num_rot, int num_pix, double *view, int tiles) {
+    const unsigned gx = cdiv(num_pix, kWavesPerBlock);
+    const size_t tsP = (size_t)num_data * kRTile, tsW = (size_t)num_pix * kRTile;
+#define EMC_TOMO(NT)                                                                             \
+    tomogram_kernel<NT><<<dim3(gx, cdiv(C, kRTile * NT)), kBlock, 0, s>>>(P, tsP, r0, C, tsW, toff, tframe, \
+                                                                        tcnt, psum_r, num_rot, num_pix, view)
+    if (tiles >= 4) EMC_TOMO(4);
+    else if (tiles == 2) EMC_TOMO(2);
+    else EMC_TOMO(1);
+#undef EMC_TOMO
+}
+
+void launch_tomogram_tiles(hipStream_t s, const double *P, int num_data, int r0, int C,
+                           const long *ft_off, const int *ft_frame, const long *fe_off,
+                           const int *fe_code, const double *psum_r, int num_rot, int num_pix,
+                           double *view) {
+    dim3 grid(cdiv(num_pix, kTomoTile), C / kRTile);
+    tomogram_tile_kernel<<<grid, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, r0, C,
+                                                 (size_t)num_pix * kRTile, ft_off, ft_frame, fe_off,
+                                                 fe_code, psum_r, num_rot, num_pix, view);
+}
+
+void launch_tomogram_parts(hipStream_t s, const double *P, int num_data, int r0, int C,
+                           const long *toff2, int nparts, const int *tframe, const double *tcnt,
+                           const double *psum_r, int num_rot, int num_pix, double *partial,
+                           double *view, int width32) {
+    if (width32) {
+        dim3 grid(cdiv(num_pix, kWavesPerBlock) * nparts, C / (kRTile / 2));
+        tomogram_part32_kernel<<<grid, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, r0, C, toff2,
+                                                       nparts, tframe, tcnt, num_pix, partial);
+    } else {
+        dim3 grid(cdiv(num_pix, kWavesPerBlock) * nparts, C / kRTile);
+        tomogram_part_kernel<<<grid, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, r0, C, toff2,
+                                                     nparts, tframe, tcnt, num_pix, partial);
+    }
+    const size_t n = (size_t)num_pix * C;
+    tomogram_reduce_kernel<<<grid_cap(cdiv(n, 256)), 256, 0, s>>>(partial, nparts, r0, C, num_pix,
+                                                                  psum_r, num_rot, view);
 }
 
 void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot, const double *view,
@@ -778,15 +1017,13 @@ void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot,
                   const uint8_t *mask, int num_pix, long size, double *model2, double *weight,
                   int do_model, int do_weight) {
     dim3 grid(C / kRTile, cdiv(num_pix, 64 * kWavesPerBlock));
-    if (do_model && do_weight)
-        merge_box_kernel<true, true><<<grid, kBlock, 0, s>>>(rots, r0, C, num_rot, view, psum_r, all_on,
-                                                             qvals, corr, mask, num_pix, size, model2, weight);
-    else if (do_model)
-        merge_box_kernel<true, false><<<grid, kBlock, 0, s>>>(rots, r0, C, num_rot, view, psum_r, all_on,
-                                                              qvals, corr, mask, num_pix, size, model2, weight);
-    else if (do_weight)
-        merge_box_kernel<false, true><<<grid, kBlock, 0, s>>>(rots, r0, C, num_rot, view, psum_r, all_on,
-                                                              qvals, corr, mask, num_pix, size, model2, weight);
+#define EMC_MERGE(M, W)                                                                          \
+    merge_box_kernel<M, W><<<grid, kBlock, 0, s>>>(rots, r0, C, num_rot, view, psum_r, all_on, qvals, \
+                                                   corr, mask, num_pix, size, model2, weight)
+    if (do_model && do_weight) EMC_MERGE(true, true);
+    else if (do_model) EMC_MERGE(true, false);
+    else if (do_weight) EMC_MERGE(false, true);
+#undef EMC_MERGE
 }
 
 void launch_count_unmerged(hipStream_t s, const double *psum_r, int num_rot, int *count) {

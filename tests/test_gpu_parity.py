@@ -190,3 +190,18 @@ def test_cfg1_geometry_two_iterations():
         assert abs(rms_g - rms_o) <= TOL * abs(rms_o)
         assert P.rel_err(eng.get('model1'), model1) <= TOL
     eng.close()
+
+
+@pytest.mark.parametrize('mode', ['2', '1', '0'])
+def test_orientation_chunks_and_tomogram_forms(tiny, monkeypatch, mode):
+    """Orientations split into 64-wide chunks (seven chunks for 420 orientations), with each
+    tomogram form (frame-partitioned, pixel-tile, per-pixel gather)."""
+    monkeypatch.setenv('EMC_MAX_CHUNK_COLS', '64')
+    monkeypatch.setenv('EMC_TOMO_TILE_MODE', mode)
+    flags = dict(iteration=1, need_scaling=0)
+    o = P.run_oracle(tiny, flags)
+    eng = P.engine_for(tiny)
+    g = P.run_engine(eng, flags)
+    eng.close()
+    assert g['num_chunks'] == 7
+    _compare(o, g, 'chunked mode=' + mode)
