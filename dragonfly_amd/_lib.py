@@ -63,6 +63,7 @@ def lib():
             'emc_set_quats': (C.c_int, [C.c_void_p, C.c_int, _dp]),
             'emc_set_model': (C.c_int, [C.c_void_p, C.c_long, _dp]),
             'emc_set_frames_sparse': (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _lp, _lp, _ip, _ip, _ip]),
+            'emc_set_frames_dense': (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip]),
             'emc_set_frame_params': (C.c_int, [C.c_void_p, _dp, _dp, _u8p, _ip, _dp, C.c_double, C.c_int]),
             'emc_set_beta': (C.c_int, [C.c_void_p, _dp]),
             'emc_set_scale': (C.c_int, [C.c_void_p, _dp]),
@@ -161,6 +162,13 @@ class Engine:
             self._h, a[0].shape[0], _ptr(a[0], _ip), _ptr(a[1], _ip), _ptr(a[2], _lp),
             _ptr(a[3], _lp), _ptr(a[4], _ip), _ptr(a[5], _ip), _ptr(a[6], _ip)))
         self.num_data = a[0].shape[0]
+
+    def set_frames_dense(self, frames):
+        frames = np.ascontiguousarray(frames, dtype='i4')
+        if frames.ndim != 2:
+            raise ValueError('dense frames must be (num_data, num_pix)')
+        _check(lib().emc_set_frames_dense(self._h, frames.shape[0], frames.shape[1], _ptr(frames, _ip)))
+        self.num_data = frames.shape[0]
 
     def set_frame_params(self, scale, beta, blacklist, fcounts, sum_fact, mean_count, num_blacklist=0):
         a = [np.ascontiguousarray(x, dtype=t) for x, t in (

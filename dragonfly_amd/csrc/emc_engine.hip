@@ -140,6 +140,9 @@ struct emc_engine {
     int num_data = 0;
     std::vector<int> h_ones, h_multi, h_place_ones, h_place_multi, h_count_multi;
     std::vector<long> h_ones_accum, h_multi_accum;
+    std::vector<uint8_t> h_dense;   // 1 for frames given as dense integer frames
+    bool any_dense = false;
+    DevBuf<uint8_t> dense;
     bool lists_dirty = true;
     DevBuf<long> lik_off, tom_off;
     DevBuf<int> lik_pix, tom_frame;
@@ -164,6 +167,7 @@ struct emc_engine {
     // work and results
     int C = 0;
     DevBuf<double> logw, upart, u, R, psum_r, psum_part, scalars, model_part;
+    DevBuf<int> ubad, rbad;   // orientation has log W = -inf / NaN on a likelihood pixel
     DevBuf<double> max_exp, p_norm, likelihood, info, psum_d;
     DevBuf<int> rmax;
     DevBuf<double> one_rot, one_view;
@@ -305,6 +309,9 @@ int build_lists(emc_engine *e) {
 
     int rc;
     if ((rc = e->tom_off2.upload(toff2.data(), toff2.size(), e->stream))) return rc;
+    e->any_dense = false;
+    for (uint8_t v : e->h_dense) e->any_dense |= (v != 0);
+    if ((rc = e->dense.upload(e->h_dense.data(), e->h_dense.size(), e->stream))) return rc;
     if ((rc = e->ft_off.upload(tf_count.data(), tf_count.size(), e->stream))) return rc;
     if ((rc = e->ft_frame.upload(ft_frame.data(), ft_frame.size(), e->stream))) return rc;
     if ((rc = e->fe_off.upload(fe_off.data(), fe_off.size(), e->stream))) return rc;
@@ -355,6 +362,8 @@ int ensure_work(emc_engine *e) {
     if (e->tomo_tile_mode == 2 && (rc = e->tom_partial.alloc((size_t)e->tom_parts * P * (size_t)C))) return rc;
     const size_t ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
     if ((rc = e->upart.alloc(ngroups * (size_t)e->C))) return rc;
+    if ((rc = e->ubad.alloc(ngroups * (size_t)e->C))) return rc;
+    if ((rc = e->rbad.alloc(Rp))) return rc;
     if ((rc = e->u.alloc(Rp))) return rc;
     if ((rc = e->psum_r.alloc(Rp))) return rc;
     const size_t nparts = (N + kFramesPerPsumPart - 1) / kFramesPerPsumPart;
@@ -413,6 +422,9 @@ void emc_destroy(emc_engine *e) {
                               &e->one_rot, &e->one_view})
         b->release();
     e->mask.release();
+    e->dense.release();
+    e->ubad.release();
+    e->rbad.release();
     e->d_perm.release();
     e->d_count.release();
     e->weight_all.release();
@@ -537,10 +549,39 @@ int emc_set_frames_sparse(emc_engine *e, int num_data, const int *ones, const in
     e->h_place_ones.assign(place_ones, place_ones + ones_total);
     e->h_place_multi.assign(place_multi, place_multi + multi_total);
     e->h_count_multi.assign(count_multi, count_multi + multi_total);
+    e->h_dense.assign(num_data, 0);
     e->num_data = num_data;
     e->lists_dirty = true;
     e->have_params = false;
     e->have_results = false;
+    return 0;
+}
+
+int emc_set_frames_dense(emc_engine *e, int num_data, int num_pix, const int *frames) {
+    if (!e || num_data <= 0 || num_pix <= 0 || !frames)
+        return fail(EMC_E_ARG, "emc_set_frames_dense: bad argument");
+    if (e->num_pix && num_pix != e->num_pix)
+        return fail(EMC_E_ARG, "emc_set_frames_dense: %d pixels, detector has %d", num_pix, e->num_pix);
+    // photon lists of the non-zero counts; the dense rule for zero counts is applied per
+    // orientation in the likelihood (frame_max_kernel)
+    std::vector<int> ones(num_data, 0), multi(num_data, 0), po, pm, cm;
+    std::vector<long> oa(num_data, 0), ma(num_data, 0);
+    for (int d = 0; d < num_data; ++d) {
+        oa[d] = (long)po.size();
+        ma[d] = (long)pm.size();
+        const int *f = frames + (size_t)d * num_pix;
+        for (int t = 0; t < num_pix; ++t) {
+            if (f[t] < 0) return fail(EMC_E_ARG, "emc_set_frames_dense: negative count");
+            if (f[t] == 1) { po.push_back(t); ++ones[d]; }
+            else if (f[t] > 1) { pm.push_back(t); cm.push_back(f[t]); ++multi[d]; }
+        }
+    }
+    if (po.empty()) po.push_back(0);
+    if (pm.empty()) { pm.push_back(0); cm.push_back(0); }
+    int rc = emc_set_frames_sparse(e, num_data, ones.data(), multi.data(), oa.data(), ma.data(),
+                                   po.data(), pm.data(), cm.data());
+    if (rc) return rc;
+    e->h_dense.assign(num_data, 1);
     return 0;
 }
 
@@ -619,8 +660,8 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         float a = 0.f, b = 0.f;
         HIP_TRY(hipEventRecord(e->ev[0], s));
         emc::launch_expand(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
-                           P, r0, Cc, kPixPerGroup, e->logw.p, e->upart.p);
-        emc::launch_u_reduce(s, e->upart.p, ngroups, Cc, r0, Rn, e->u.p);
+                           P, r0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p);
+        emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, r0, Rn, e->u.p, e->rbad.p);
         HIP_TRY(hipEventRecord(e->ev[1], s));
         emc::launch_likelihood(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
                                e->blacklist.p, N, e->R.p, r0, e->lik_tiles);
@@ -636,7 +677,8 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     emc::launch_rescale(s, e->quats.p, e->u.p, Rn, e->mean_count, e->scalars.p);
     HIP_TRY(hipEventRecord(e->ev[4], s));
     emc::launch_frame_max(s, e->R.p, Rn, N, e->logwr.p, e->u.p, e->scale.p, use_scale,
-                          e->scalars.p, e->blacklist.p, e->max_exp.p, e->rmax.p);
+                          e->scalars.p, e->blacklist.p, e->any_dense ? e->dense.p : nullptr,
+                          e->rbad.p, e->max_exp.p, e->rmax.p);
     HIP_TRY(hipEventRecord(e->ev[5], s));
     emc::launch_frame_norm(s, e->R.p, Rn, N, e->beta.p, e->blacklist.p, e->max_exp.p,
                            e->rmax.p, e->p_norm.p);

@@ -10,9 +10,9 @@ namespace emc {
 
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
-                   int r0, int C, int pix_per_group, double *logw, double *upart);
-void launch_u_reduce(hipStream_t s, const double *upart, int ngroups, int C, int r0, int num_rot,
-                     double *u);
+                   int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad);
+void launch_u_reduce(hipStream_t s, const double *upart, const int *ubad, int ngroups, int C,
+                     int r0, int num_rot, double *u, int *rbad);
 void launch_rescale(hipStream_t s, const double *quats, const double *u, int num_rot,
                     double mean_count, double *scalars);
 void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
@@ -20,7 +20,8 @@ void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, co
                        double *G, int r0, int tiles);
 void launch_frame_max(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *logwr, const double *u, const double *scale, int use_scale,
-                      const double *scalars, const uint8_t *blacklist, double *max_exp, int *rmax);
+                      const double *scalars, const uint8_t *blacklist, const uint8_t *dense,
+                      const int *rbad, double *max_exp, int *rmax);
 void launch_frame_norm(hipStream_t s, const double *R, int num_rot, int num_data,
                        const double *beta, const uint8_t *blacklist, const double *max_exp,
                        const int *rmax, double *p_norm);

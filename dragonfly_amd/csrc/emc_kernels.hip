@@ -94,10 +94,11 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
     const double *__restrict__ rots, const double *__restrict__ model, long size,
     const double *__restrict__ qvals, const double *__restrict__ corr,
     const uint8_t *__restrict__ mask, int num_pix, int r0, int C, int pix_per_group,
-    double *__restrict__ logw, double *__restrict__ upart) {
+    double *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad) {
     __shared__ double s_rot[kRTile * 9];
     __shared__ double s_tile[64][kRTile + 1];
     __shared__ double s_u[kWavesPerBlock][kRTile];
+    __shared__ int s_bad[kWavesPerBlock][kRTile];
     const int tid = threadIdx.x;
     const int c0 = blockIdx.x * kRTile;
     const int g = blockIdx.y;
@@ -108,6 +109,7 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
     const int t_begin = g * pix_per_group;
     const int t_end = min(num_pix, t_begin + pix_per_group);
     double uacc = 0.;
+    int bad = 0;  // some mask < 1 pixel has W <= 0 (log W = -inf or NaN)
     for (int tb = t_begin; tb < t_end; tb += 64) {
         const int t = tb + lane;
         if (t < t_end) {
@@ -124,26 +126,38 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
             const int tt = tb + p;
             if (tt < t_end) {
                 const double W = s_tile[p][lane];
-                if (mask[tt] < 1) uacc += W;
+                if (mask[tt] < 1) {
+                    uacc += W;
+                    bad |= !(W > 0.);
+                }
                 logw[(size_t)(c0 >> 6) * ((size_t)num_pix * kRTile) + (size_t)tt * kRTile + lane] = log(W);
             }
         }
         __syncthreads();
     }
     s_u[wave][lane] = uacc;
+    s_bad[wave][lane] = bad;
     __syncthreads();
-    if (tid < kRTile)
+    if (tid < kRTile) {
         upart[(size_t)g * C + c0 + tid] = s_u[0][tid] + s_u[1][tid] + s_u[2][tid] + s_u[3][tid];
+        ubad[(size_t)g * C + c0 + tid] = s_bad[0][tid] | s_bad[1][tid] | s_bad[2][tid] | s_bad[3][tid];
+    }
 }
 
 // u[r0 + c] = sum over pixel groups, fixed order.
-__global__ void u_reduce_kernel(const double *__restrict__ upart, int ngroups, int C, int r0,
-                                int num_rot, double *__restrict__ u) {
+__global__ void u_reduce_kernel(const double *__restrict__ upart, const int *__restrict__ ubad,
+                                int ngroups, int C, int r0, int num_rot, double *__restrict__ u,
+                                int *__restrict__ rbad) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C || r0 + c >= num_rot) return;
     double acc = 0.;
-    for (int g = 0; g < ngroups; ++g) acc += upart[(size_t)g * C + c];
+    int bad = 0;
+    for (int g = 0; g < ngroups; ++g) {
+        acc += upart[(size_t)g * C + c];
+        bad |= ubad[(size_t)g * C + c];
+    }
     u[r0 + c] = acc;
+    rbad[r0 + c] = bad;
 }
 
 // rescale = mean_count / sum_r w_r u_r (max_prob.c:40, :59); one block, fixed order.
@@ -229,10 +243,14 @@ __global__ __launch_bounds__(kBlock) void frame_max_kernel(
     double *__restrict__ R, size_t tsR, int num_rot, const double *__restrict__ logwr,
     const double *__restrict__ u, const double *__restrict__ scale, int use_scale,
     const double *__restrict__ scalars, const uint8_t *__restrict__ blacklist,
+    const uint8_t *__restrict__ dense, const int *__restrict__ rbad,
     double *__restrict__ max_exp, int *__restrict__ rmax) {
     __shared__ double s_v[kBlock];
     __shared__ int s_i[kBlock];
     const int d = blockIdx.x;
+    // dense frames sum K log W over every mask < 1 pixel (max_prob.c:173-177): an orientation
+    // with log W = -inf or NaN on such a pixel gives 0 * -inf = NaN and is never kept
+    const bool dense_frame = dense != nullptr && dense[d];
     if (blacklist[d]) {
         if (threadIdx.x == 0) {
             max_exp[d] = kMaxExpStart;
@@ -246,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void frame_max_kernel(
     int bi = -1;
     for (int r = threadIdx.x; r < num_rot; r += kBlock) {
         double &x = row[(size_t)(r >> 6) * tsR + (r & 63)];
-        const double v = (logwr[r] - u[r] * phi) + x;
+        const double v = (dense_frame && rbad[r]) ? -INFINITY : (logwr[r] - u[r] * phi) + x;
         x = v;
         if (v > best) {
             best = v;
@@ -906,15 +924,15 @@ static inline unsigned grid_cap(size_t blocks) { return (unsigned)(blocks < 6553
 
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
-                   int r0, int C, int pix_per_group, double *logw, double *upart) {
+                   int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad) {
     dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
     expand_kernel<<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0, C,
-                                          pix_per_group, logw, upart);
+                                          pix_per_group, logw, upart, ubad);
 }
 
-void launch_u_reduce(hipStream_t s, const double *upart, int ngroups, int C, int r0, int num_rot,
-                     double *u) {
-    u_reduce_kernel<<<cdiv(C, 256), 256, 0, s>>>(upart, ngroups, C, r0, num_rot, u);
+void launch_u_reduce(hipStream_t s, const double *upart, const int *ubad, int ngroups, int C,
+                     int r0, int num_rot, double *u, int *rbad) {
+    u_reduce_kernel<<<cdiv(C, 256), 256, 0, s>>>(upart, ubad, ngroups, C, r0, num_rot, u, rbad);
 }
 
 void launch_rescale(hipStream_t s, const double *quats, const double *u, int num_rot,
@@ -938,9 +956,10 @@ void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, co
 
 void launch_frame_max(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *logwr, const double *u, const double *scale, int use_scale,
-                      const double *scalars, const uint8_t *blacklist, double *max_exp, int *rmax) {
+                      const double *scalars, const uint8_t *blacklist, const uint8_t *dense,
+                      const int *rbad, double *max_exp, int *rmax) {
     frame_max_kernel<<<num_data, kBlock, 0, s>>>(R, (size_t)num_data * kRTile, num_rot, logwr, u, scale, use_scale,
-                                                 scalars, blacklist, max_exp, rmax);
+                                                 scalars, blacklist, dense, rbad, max_exp, rmax);
 }
 
 void launch_frame_norm(hipStream_t s, const double *R, int num_rot, int num_data,

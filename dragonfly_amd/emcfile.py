@@ -80,6 +80,16 @@ class CDataset:
             raise ValueError('sparse arrays are inconsistent with ones/multi counts')
         return ds
 
+    @classmethod
+    def from_dense(cls, det, frames, fname='in-memory'):
+        """Dense integer frames (ftype DENSE_INT), shape (num_data, num_pix)."""
+        ds = cls(det=det)
+        ds.fname = fname
+        ds._ftype = DENSE_INT
+        ds.int_frames = np.ascontiguousarray(frames, dtype='i4')
+        ds.num_data, ds.num_pix = ds.int_frames.shape
+        return ds
+
     def append(self, next_dset):
         curr = self
         while curr.next is not None:
@@ -116,6 +126,8 @@ class CDataset:
     def mean_count(self):
         """Mean masked photon count per frame (emcfile.c:171-193)."""
         m = self.det.raw_mask
+        if self._ftype == DENSE_INT:
+            return self.int_frames[:, m < 1].sum() / self.num_data
         tot = (m[self.place_ones] < 1).sum() + self.count_multi[m[self.place_multi] < 1].sum()
         return tot / self.num_data
 
@@ -177,6 +189,16 @@ class EMCWriter:
             for arrs in (self._po, self._pm, self._cm):
                 for a in arrs:
                     a.tofile(f)
+
+
+def write_dense(fname, frames):
+    """Write dense integer frames (num_data, num_pix) as a DENSE_INT .emc file."""
+    frames = np.ascontiguousarray(frames, dtype='i4')
+    header = np.zeros(HEADER_BYTES // 4, dtype='i4')
+    header[0], header[1], header[2] = frames.shape[0], frames.shape[1], DENSE_INT
+    with open(fname, 'wb') as f:
+        header.tofile(f)
+        frames.tofile(f)
 
 
 def write_sparse(fname, num_pix, ones, multi, place_ones, place_multi, count_multi):

@@ -162,25 +162,44 @@ class Iterate:
         if self.tot_num_data != 0:
             raise ValueError('Please use update_data() to change dataset')
         total = 0
+        kinds = set()
         for ds in in_dset:
             ds.num_offset = total
             total += ds.num_data
-            if ds.ftype != 'sparse':
-                raise NotImplementedError('only sparse photon data are implemented')
+            kinds.add(ds.ftype)
+        if not kinds <= {'sparse', 'dense_integer'} or len(kinds) != 1:
+            raise NotImplementedError('photon data must be all sparse or all dense-integer frames')
         self.tot_num_data = total
         self.data = in_dset
+        self.dense = kinds == {'dense_integer'}
         if not self.dets:
             self.dets = [in_dset.det]
         if any(ds.det is not self.dets[0] for ds in in_dset):
             raise NotImplementedError('all datasets must share one detector')
         mask = self.dets[0].raw_mask
-        sp = self.sparse_arrays()
-        self.fcounts = frame_counts(sp['ones'], sp['multi'], sp['place_ones'], sp['place_multi'],
-                                    sp['count_multi'], mask)
+        if self.dense:
+            fr = self.dense_frames()
+            # calc_frame_counts counts every pixel of a dense frame (iterate.c:38-46);
+            # calc_sum_fact only mask < 1 pixels (iterate.c:103-108)
+            self.fcounts = fr.sum(1, dtype='i8').astype('i4')
+            good = mask < 1
+            self.sum_fact = np.zeros(total)
+            for d in range(total):
+                k = fr[d, good]
+                k = k[k > 1]
+                self.sum_fact[d] = lnfact(k).sum() if k.size else 0.
+        else:
+            sp = self.sparse_arrays()
+            self.fcounts = frame_counts(sp['ones'], sp['multi'], sp['place_ones'], sp['place_multi'],
+                                        sp['count_multi'], mask)
+            self.sum_fact = sum_log_factorials(sp['multi'], sp['place_multi'], sp['count_multi'], mask)
         self.mean_count = np.array([self.fcounts.astype('f8').sum() / total])
         self.rescale = np.zeros(1)
-        self.sum_fact = sum_log_factorials(sp['multi'], sp['place_multi'], sp['count_multi'], mask)
         self.blacklist = np.zeros(total, dtype='u1')
+
+    def dense_frames(self):
+        """All dense datasets stacked, (tot_num_data, num_pix) int32."""
+        return np.concatenate([p.int_frames for p in self.data]).astype('i4')
 
     def sparse_arrays(self):
         """All datasets concatenated into one frame list (linked list order)."""

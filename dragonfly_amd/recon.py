@@ -65,9 +65,12 @@ class EMCRecon:
         eng.set_detector(det.qvals, det.corr, det.raw_mask)
         eng.set_quats(itr.quat.quats)
         eng.set_model(itr.model.model1)
-        sp = itr.sparse_arrays()
-        eng.set_frames(sp['ones'], sp['multi'], sp['ones_accum'], sp['multi_accum'],
-                       sp['place_ones'], sp['place_multi'], sp['count_multi'])
+        if getattr(itr, 'dense', False):
+            eng.set_frames_dense(itr.dense_frames())
+        else:
+            sp = itr.sparse_arrays()
+            eng.set_frames(sp['ones'], sp['multi'], sp['ones_accum'], sp['multi_accum'],
+                           sp['place_ones'], sp['place_multi'], sp['count_multi'])
         itr.num_blacklist = int((itr.blacklist != 0).sum())
         eng.set_frame_params(itr.scale, itr.beta_start, itr.blacklist, itr.fcounts,
                              itr.sum_fact, itr.mean_count[0], itr.num_blacklist)

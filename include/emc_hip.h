@@ -101,6 +101,11 @@ int emc_set_frames_sparse(emc_engine *e, int num_data, const int *ones, const in
                           const long *ones_accum, const long *multi_accum,
                           const int *place_ones, const int *place_multi,
                           const int *count_multi);
+/* dense integer frames (ftype DENSE_INT, emcfile.c:39-43): frames[num_data][num_pix].
+ * The likelihood then sums K log W over every mask < 1 pixel (max_prob.c:173-177), so an
+ * orientation with log W = -inf on any such pixel is dropped for these frames; the tomogram
+ * adds every pixel (max_update.c:211-214).                                             */
+int emc_set_frames_dense(emc_engine *e, int num_data, int num_pix, const int *frames);
 /* per-frame state of struct iterate: scale, beta, blacklist, fcounts, sum_fact,
  *        mean_count[0], num_blacklist (iterate.h:17-24,33; iterate.c:3-117)          */
 int emc_set_frame_params(emc_engine *e, const double *scale, const double *beta,

@@ -63,6 +63,8 @@ def merge_view(model2, weight, q, view, qvals, corr, mask):
 
 
 def dense_counts(frames, num_pix):
+    if 'int_frames' in frames:
+        return np.asarray(frames['int_frames'], dtype=np.int64)
     D = len(frames['ones'])
     K = np.zeros((D, num_pix), dtype=np.int64)
     for d in range(D):
@@ -86,6 +88,7 @@ def iterate(det, quats, model1, frames, fparams, flags):
     use_scale = need_scaling and (flags.get('iteration', 1) > 1 or flags.get('known_scale', 0))
     K = dense_counts(frames, P)
     D = K.shape[0]
+    dense = 'int_frames' in frames
     W = np.array([slice_view(model1, q, qvals, corr) for q in quats])
     good = mask < 1
     u = W[:, good].sum(1)
@@ -98,8 +101,11 @@ def iterate(det, quats, model1, frames, fparams, flags):
         if black[d]:
             continue
         phi = scale[d] if use_scale else rescale
-        t = np.nonzero((K[d] > 0) & good)[0]
-        R[d] = np.log(w_r) - u * phi + (K[d, t][None, :] * logW[:, t]).sum(1)
+        # sparse frames sum their photons; dense frames every mask < 1 pixel, where a zero
+        # count times log W = -inf is NaN (max_prob.c:173-177)
+        t = np.nonzero(good)[0] if dense else np.nonzero((K[d] > 0) & good)[0]
+        with np.errstate(invalid='ignore'):
+            R[d] = np.log(w_r) - u * phi + (K[d, t][None, :] * logW[:, t]).sum(1)
     M = np.full(D, -1.e100)
     rmax = np.full(D, -1, dtype=np.int64)
     Pm = np.zeros((D, Rn))

@@ -80,3 +80,30 @@ def rel_err(a, b):
     b = np.asarray(b, dtype='f8')
     scale = max(np.abs(b).max(), 1e-300)
     return float(np.abs(a - b).max() / scale)
+
+
+def densify(p):
+    """The same problem with its frames as dense integer frames (ftype DENSE_INT) and the
+    dense-frame precompute: fcounts over every pixel (iterate.c:38-46)."""
+    from oracle import brute
+    K = brute.dense_counts(p['frames'], len(p['det']['corr'])).astype('i4')
+    q = dict(p)
+    q['frames'] = {'int_frames': K}
+    fp = dict(p['fparams'])
+    fp['fcounts'] = K.sum(1).astype('i4')
+    fp['mean_count'] = fp['fcounts'].astype('f8').sum() / K.shape[0]
+    q['fparams'] = fp
+    return q
+
+
+def engine_for_dense(p):
+    from dragonfly_amd._lib import Engine
+    eng = Engine(0)
+    eng.set_detector(p['det']['qvals'], p['det']['corr'], p['det']['mask'])
+    eng.set_quats(p['quats'])
+    eng.set_model(p['model1'])
+    eng.set_frames_dense(p['frames']['int_frames'])
+    fp = p['fparams']
+    eng.set_frame_params(fp['scale'], fp['beta'], fp['blacklist'], fp['fcounts'], fp['sum_fact'],
+                         fp['mean_count'], fp['num_blacklist'])
+    return eng

@@ -205,3 +205,20 @@ def test_orientation_chunks_and_tomogram_forms(tiny, monkeypatch, mode):
     eng.close()
     assert g['num_chunks'] == 7
     _compare(o, g, 'chunked mode=' + mode)
+
+
+@pytest.mark.parametrize('zero_block', [False, True])
+def test_dense_integer_frames(tiny, zero_block):
+    """Dense integer frames through emc_set_frames_dense against the oracle's dense branch."""
+    p = P.densify(tiny)
+    if zero_block:
+        m = tiny['model1'].copy()
+        c = m.shape[0] // 2
+        m[c + 3:c + 6, c - 2:c + 3, :] = 0.
+        p['model1'] = m
+    flags = dict(iteration=1, need_scaling=0)
+    o = P.run_oracle(p, flags)
+    eng = P.engine_for_dense(p)
+    g = P.run_engine(eng, flags)
+    eng.close()
+    _compare(o, g, 'dense zero_block=%s' % zero_block)

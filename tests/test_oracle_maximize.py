@@ -143,3 +143,22 @@ def test_planted_mistakes_are_detected(tiny, mistake):
                       dict(iteration=1, need_scaling=0))
     errs = _diff(o, b)
     assert max(errs.values()) > 1e-6, (mistake, errs)
+
+
+@pytest.mark.parametrize('zero_block', [False, True])
+def test_dense_frames_vs_brute(tiny, zero_block):
+    """Dense integer frames (ftype DENSE_INT): with a zeroed model block, orientations whose
+    slice has log W = -inf on a likelihood pixel drop out of every dense frame."""
+    p = P.densify(tiny)
+    if zero_block:
+        m = tiny['model1'].copy()
+        c = m.shape[0] // 2
+        m[c + 3:c + 6, c - 2:c + 3, :] = 0.
+        p['model1'] = m
+    flags = dict(iteration=1, need_scaling=0)
+    o = P.run_oracle(p, flags)
+    b = brute.iterate(p['det'], p['quats'], p['model1'], p['frames'], p['fparams'], flags)
+    _check(o, b)
+    if zero_block:
+        sparse = P.run_oracle(dict(tiny, model1=p['model1']), flags)
+        assert (o['prob'] == 0).sum() > (sparse['prob'] == 0).sum()
