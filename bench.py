@@ -7,8 +7,8 @@ A step is one full EMC iteration (maximize + model update + scale normalisation)
 EMCRecon.run_iteration with inputs resident in HBM; output files are not written.
 
 Prints one JSON line: frame x orientation pairs/s, the roofline of the dominant kernel
-(likelihood) from HIP events on the engine's stream, and a CPU baseline (the oracle's
-OpenMP restatement of the reference engine) timed on a bounded orientation sample.
+from HIP events on the engine's stream, and a CPU baseline (the oracle's OpenMP restatement
+of the reference engine) timed on a bounded orientation sample.
 """
 import argparse
 import json
@@ -75,6 +75,62 @@ def cpu_baseline(prob, itr, num_rot_sample, threads):
     return pairs / dt, dt
 
 
+def photon_stats(prob, itr):
+    """Mean photon pixels per frame on likelihood pixels (mask < 1) and on tomogram pixels
+    (mask < 2), and the multi-photon part of each."""
+    f = prob['frames']
+    mask = prob['det']['mask']
+    D = itr.tot_num_data
+    out = {}
+    for key, lim in (('lik', 1), ('tomo', 2)):
+        go = mask[f['place_ones']] < lim
+        gm = mask[f['place_multi']] < lim
+        out[key] = ((go.sum() + gm.sum()) / D, gm.sum() / D)
+    return out
+
+
+def load_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy), or None."""
+    path = os.path.join(ROOT, 'profiles', 'pmc_traffic.json')
+    try:
+        with open(path) as fh:
+            tab = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    ent = tab.get(workload, {}).get(kernel)
+    return None if ent is None else float(ent['bytes_per_launch'])
+
+
+def roofline(prob, itr, stage_tot, steps, nchunks, pairs, workload):
+    """Roofline of the dominant gather kernel (likelihood or tomogram gather, whichever takes
+    longer), timed with HIP events on the engine's stream.
+
+    Algorithmic bytes per (frame, orientation) pair: 12 per photon pixel (int32 index + f64
+    gather of log W resp. P) + 4 per multi-photon pixel (int32 count), i.e. 12 n + 4 m, over
+    mask < 1 pixels for the likelihood and mask < 2 pixels for the tomogram. Every gather is
+    counted as if it came from HBM; the kernels serve most of them from L2 and the Infinity
+    Cache, so `frac` can exceed 1 and `traffic` (measured HBM-side bytes) is the figure that
+    shows what actually crossed the memory interface."""
+    ps = photon_stats(prob, itr)
+    kern = {'likelihood': ('likelihood_kernel', ps['lik']),
+            'tomogram': ('tomogram_part_kernel', ps['tomo'])}
+    stage = max(kern, key=lambda k: stage_tot.get(k, 0.))
+    name, (n, m) = kern[stage]
+    launch_ms = stage_tot[stage] / steps / nchunks
+    bytes_per_launch = pairs / nchunks * (12. * n + 4. * m)
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = load_traffic(workload, name)
+    out = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+               frac=round(achieved / HBM_PEAK_GBS, 4),
+               traffic=None if traffic is None else round(traffic),
+               kernel=name, launch_ms=round(launch_ms, 3),
+               algorithmic_bytes=round(bytes_per_launch))
+    if traffic is not None:
+        out['hbm_frac_measured'] = round(traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -112,13 +168,9 @@ def main():
     # host copy of the start model for the CPU baseline before the GPU changes it
     recon, itr = build_recon(prob, cfg)
     start_model = itr.model.model1.copy()
-    f = prob['frames']
     mask = prob['det']['mask']
-    good_o = mask[f['place_ones']] < 1
-    good_m = mask[f['place_multi']] < 1
     D = itr.tot_num_data
-    nbar = (good_o.sum() + good_m.sum()) / D
-    mbar = good_m.sum() / D
+    nbar = photon_stats(prob, itr)['lik'][0]
     R = itr.quat.num_rot
     pairs = (D - itr.num_blacklist) * R
 
@@ -127,13 +179,12 @@ def main():
         recon.run_iteration()
         log('warmup iteration %d: %s' % (it, json.dumps({k: round(v, 1) for k, v in recon.last_stats['stage_ms'].items()})))
     torch.cuda.synchronize()
-    lik_ms, stage_tot, nchunks = [], {}, 1
+    stage_tot, nchunks = {}, 1
     t_start = time.perf_counter()
     for k in range(args.steps):
         itr.params.iteration = args.warmup + 1 + k
         recon.run_iteration()
         st = recon.last_stats['stage_ms']
-        lik_ms.append(st['likelihood'])
         nchunks = recon.last_stats['num_chunks']
         for name, v in st.items():
             stage_tot[name] = stage_tot.get(name, 0.) + v
@@ -143,15 +194,9 @@ def main():
     ms_per_step = elapsed / args.steps * 1e3
     value = pairs * args.steps / elapsed
 
-    # roofline of the likelihood kernel: algorithmic bytes per pair 12*nbar + 4*mbar
-    # (int32 pixel index + f64 log W per photon pixel, int32 count per multi pixel)
-    lik_launch_ms = float(np.mean(lik_ms)) / nchunks
-    bytes_per_launch = pairs / nchunks * (12. * nbar + 4. * mbar)
-    achieved = bytes_per_launch / (lik_launch_ms * 1e-3) / 1e9
-    roof = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel='likelihood_kernel',
-                launch_ms=round(lik_launch_ms, 3))
-
+    default_size = args.num_data is None and args.num_div is None
+    roof = roofline(prob, itr, stage_tot, args.steps, nchunks, pairs,
+                    args.config if default_size else None)
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -25,7 +25,8 @@ FIELDS = {
     'prob': (12, 'f8', 'data_rot'),
 }
 
-STAGES = ('expand', 'likelihood', 'rescale', 'max', 'norm', 'posterior', 'tomogram', 'merge')
+STAGES = ('expand', 'likelihood', 'rescale', 'max', 'norm', 'posterior', 'tomogram', 'reduce',
+          'merge')
 
 
 class IterOpts(C.Structure):
@@ -35,7 +36,7 @@ class IterOpts(C.Structure):
 
 class IterStats(C.Structure):
     _fields_ = [('avg_likelihood', C.c_double), ('mutual_info', C.c_double),
-                ('rescale', C.c_double), ('stage_ms', C.c_double * 8), ('num_chunks', C.c_int)]
+                ('rescale', C.c_double), ('stage_ms', C.c_double * len(STAGES)), ('num_chunks', C.c_int)]
 
 
 class EMCError(RuntimeError):

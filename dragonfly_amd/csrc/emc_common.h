@@ -13,6 +13,9 @@ namespace emc {
 constexpr int kRTile = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
+// MI355X dispatches consecutive workgroups of a launch round-robin over its 8 XCDs, each with
+// its own L2.
+constexpr int kNumXcd = 8;
 
 // Reference constants (dragonfly/src/max_internal.h:6-8).
 constexpr double kProbMin = 1.e-6;

@@ -182,6 +182,7 @@ struct emc_engine {
     // orientation tiles (of 64) per wavefront in the two photon-gather kernels; tomogram
     // form (2: frame-partitioned gather, 1: pixel-tile LDS accumulation, 0: per-pixel gather)
     int lik_tiles = 1, tomo_tiles = 1, tomo_tile_mode = 2;
+    int lik_unroll = 8, tomo_unroll = 16, tomo_seq = 0;
 };
 
 namespace {
@@ -405,6 +406,9 @@ int emc_create(int device, emc_engine **out) {
     if (const char *v = getenv("EMC_TOMO_PARTS")) e->tom_parts = std::max(1, atoi(v));
     if (const char *v = getenv("EMC_TOMO_WIDTH32")) e->tomo_width32 = atoi(v);
     if (const char *v = getenv("EMC_MAX_CHUNK_COLS")) e->max_chunk_cols = atoi(v);
+    if (const char *v = getenv("EMC_LIK_UNROLL")) e->lik_unroll = atoi(v);
+    if (const char *v = getenv("EMC_TOMO_UNROLL")) e->tomo_unroll = atoi(v);
+    if (const char *v = getenv("EMC_TOMO_SEQ")) e->tomo_seq = atoi(v);
 
     *out = e;
     return 0;
@@ -648,7 +652,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     const int ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
     const int use_scale = opts->need_scaling && (opts->iteration > 1 || opts->known_scale);
     const int want_psum_d = opts->need_scaling && opts->update_scale;
-    float ms_expand = 0.f, ms_lik = 0.f, ms_tomo = 0.f, ms_merge = 0.f;
+    float ms_expand = 0.f, ms_lik = 0.f, ms_tomo = 0.f, ms_reduce = 0.f, ms_merge = 0.f;
 
     // allocate_memory: zero the merge targets (maximize.c:93-94)
     emc::launch_fill(s, e->model2.p, vol, 0.);
@@ -664,7 +668,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, r0, Rn, e->u.p, e->rbad.p);
         HIP_TRY(hipEventRecord(e->ev[1], s));
         emc::launch_likelihood(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
-                               e->blacklist.p, N, e->R.p, r0, e->lik_tiles);
+                               e->blacklist.p, N, e->R.p, r0, e->lik_tiles, e->lik_unroll);
         HIP_TRY(hipEventRecord(e->ev[2], s));
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventSynchronize(e->ev[2]));
@@ -731,12 +735,14 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     // tomogram + merge, chunk by chunk
     for (int r0 = 0; r0 < e->Rp; r0 += C) {
         const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
-        float a = 0.f, b = 0.f;
+        float a = 0.f, b = 0.f, c = 0.f;
         HIP_TRY(hipEventRecord(e->ev[0], s));
-        if (e->tomo_tile_mode == 2)
+        const bool parts = e->tomo_tile_mode == 2;
+        if (parts)
             emc::launch_tomogram_parts(s, e->R.p, N, r0, Cc, e->tom_off2.p, e->tom_parts,
                                        e->tom_frame.p, e->tom_cnt.p, e->psum_r.p, Rn, P,
-                                       e->tom_partial.p, e->logw.p, e->tomo_width32);
+                                       e->tom_partial.p, e->logw.p, e->tomo_width32,
+                                       e->tomo_unroll, e->tomo_seq);
         else if (e->tomo_tile_mode == 1)
             emc::launch_tomogram_tiles(s, e->R.p, N, r0, Cc, e->ft_off.p, e->ft_frame.p,
                                        e->fe_off.p, e->fe_code.p, e->psum_r.p, Rn, P, e->logw.p);
@@ -744,16 +750,22 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
             emc::launch_tomogram(s, e->R.p, N, r0, Cc, e->tom_off.p, e->tom_frame.p,
                                  e->tom_cnt.p, e->psum_r.p, Rn, P, e->logw.p, e->tomo_tiles);
         HIP_TRY(hipEventRecord(e->ev[1], s));
+        if (parts)
+            emc::launch_tomogram_reduce(s, e->tom_partial.p, e->tom_parts, r0, Cc, e->psum_r.p,
+                                        Rn, P, e->logw.p);
+        HIP_TRY(hipEventRecord(e->ev[2], s));
         emc::launch_merge(s, e->rots.p, r0, Cc, Rn, e->logw.p, e->psum_r.p, 0, e->qvals.p,
                           e->corr.p, e->mask.p, P, e->size, e->model2.p, e->weight.p, 1,
                           cached_weight ? 0 : 1);
-        HIP_TRY(hipEventRecord(e->ev[2], s));
+        HIP_TRY(hipEventRecord(e->ev[8], s));
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventSynchronize(e->ev[2]));
+        HIP_TRY(hipEventSynchronize(e->ev[8]));
         HIP_TRY(hipEventElapsedTime(&a, e->ev[0], e->ev[1]));
         HIP_TRY(hipEventElapsedTime(&b, e->ev[1], e->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&c, e->ev[2], e->ev[8]));
         ms_tomo += a;
-        ms_merge += b;
+        ms_reduce += b;
+        ms_merge += c;
     }
     // combine_information_mpi statistics and update_scale
     emc::launch_frame_stats(s, e->likelihood.p, e->info.p, e->blacklist.p, N, N - e->num_blacklist,
@@ -768,8 +780,9 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         stats->rescale = host_scalars[0];
         stats->avg_likelihood = host_scalars[1];
         stats->mutual_info = host_scalars[2];
-        const float ms[8] = {ms_expand, ms_lik, ms_rescale, ms_max, ms_norm, ms_post, ms_tomo, ms_merge};
-        for (int i = 0; i < 8; ++i) stats->stage_ms[i] = ms[i];
+        const float ms[EMC_NUM_STAGES] = {ms_expand, ms_lik, ms_rescale, ms_max, ms_norm,
+                                          ms_post, ms_tomo, ms_reduce, ms_merge};
+        for (int i = 0; i < EMC_NUM_STAGES; ++i) stats->stage_ms[i] = ms[i];
         stats->num_chunks = (e->Rp + C - 1) / C;
     }
     return 0;

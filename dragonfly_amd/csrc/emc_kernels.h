@@ -17,7 +17,7 @@ void launch_rescale(hipStream_t s, const double *quats, const double *u, int num
                     double mean_count, double *scalars);
 void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
                        const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
-                       double *G, int r0, int tiles);
+                       double *G, int r0, int tiles, int unroll);
 void launch_frame_max(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *logwr, const double *u, const double *scale, int use_scale,
                       const double *scalars, const uint8_t *blacklist, const uint8_t *dense,
@@ -43,7 +43,9 @@ void launch_tomogram_tiles(hipStream_t s, const double *P, int num_data, int r0,
 void launch_tomogram_parts(hipStream_t s, const double *P, int num_data, int r0, int C,
                            const long *toff2, int nparts, const int *tframe, const double *tcnt,
                            const double *psum_r, int num_rot, int num_pix, double *partial,
-                           double *view, int width32);
+                           double *view, int width32, int unroll, int seq);
+void launch_tomogram_reduce(hipStream_t s, const double *partial, int nparts, int r0, int C,
+                            const double *psum_r, int num_rot, int num_pix, double *view);
 void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot, const double *view,
                   const double *psum_r, int all_on, const double *qvals, const double *corr,
                   const uint8_t *mask, int num_pix, long size, double *model2, double *weight,

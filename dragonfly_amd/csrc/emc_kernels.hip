@@ -181,13 +181,12 @@ __global__ __launch_bounds__(kBlock) void rescale_kernel(const double *__restric
 // One wavefront = one frame x NT*64 orientations; the frame's (pixel, count) list is read
 // with scalar loads (wave-uniform) and each photon costs NT 512-byte row reads of logW.
 // Row loads of several photons are issued before the additions, which stay in list order.
-template <int NT>
+template <int NT, int U>
 __global__ __launch_bounds__(kBlock) void likelihood_kernel(
     const double *__restrict__ logw, int C, size_t tsW, const long *__restrict__ off,
     const int *__restrict__ pix, const double *__restrict__ cnt,
     const uint8_t *__restrict__ blacklist, int num_data, double *__restrict__ G, size_t tsG,
     int r0) {
-    constexpr int U = NT >= 4 ? 2 : (NT == 2 ? 4 : 8);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int d = blockIdx.x * kWavesPerBlock + wave;
@@ -455,15 +454,27 @@ __global__ __launch_bounds__(kBlock) void tomogram_kernel(
 // orientation tile. Each range writes a partial view; tomogram_reduce_kernel adds the
 // partials in range order and divides by psum_r. toff2[t * nparts + x] is the first photon of
 // pixel t with a frame in range x (the per-pixel list is frame-ascending).
+//
+// kSeq orders the blocks so that each XCD walks its ranges one after another (all pixels of
+// range x, then of range x + 8, ...) instead of interleaving them, which keeps one range's
+// P slab in that XCD's L2 at a time; nparts is then a multiple of 8.
+template <int U, bool kSeq>
 __global__ __launch_bounds__(kBlock) void tomogram_part_kernel(
     const double *__restrict__ P, size_t tsP, int r0, int C, const long *__restrict__ toff2,
     int nparts, const int *__restrict__ tframe, const double *__restrict__ tcnt, int num_pix,
-    double *__restrict__ partial) {
-    constexpr int U = 8;
+    int ngroups, double *__restrict__ partial) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int part = blockIdx.x % nparts;
-    const int t = (blockIdx.x / nparts) * kWavesPerBlock + wave;
+    int part, group;
+    if (kSeq) {
+        const int seq = blockIdx.x / kNumXcd;
+        part = (seq / ngroups) * kNumXcd + blockIdx.x % kNumXcd;
+        group = seq % ngroups;
+    } else {
+        part = blockIdx.x % nparts;
+        group = blockIdx.x / nparts;
+    }
+    const int t = group * kWavesPerBlock + wave;
     if (t >= num_pix) return;
     const int c0 = blockIdx.y * kRTile;
     const double *__restrict__ col = P + (size_t)((r0 + c0) >> 6) * tsP + lane;
@@ -942,15 +953,17 @@ void launch_rescale(hipStream_t s, const double *quats, const double *u, int num
 
 void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
                        const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
-                       double *G, int r0, int tiles) {
+                       double *G, int r0, int tiles, int unroll) {
     const unsigned gx = cdiv(num_data, kWavesPerBlock);
     const size_t tsW = (size_t)num_pix * kRTile, tsG = (size_t)num_data * kRTile;
-#define EMC_LIK(NT)                                                                              \
-    likelihood_kernel<NT><<<dim3(gx, cdiv(C, kRTile * NT)), kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, \
-                                                                          blacklist, num_data, G, tsG, r0)
-    if (tiles >= 4) EMC_LIK(4);
-    else if (tiles == 2) EMC_LIK(2);
-    else EMC_LIK(1);
+#define EMC_LIK(NT, U)                                                                           \
+    likelihood_kernel<NT, U><<<dim3(gx, cdiv(C, kRTile * NT)), kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, \
+                                                                             blacklist, num_data, G, tsG, r0)
+    if (tiles >= 4) EMC_LIK(4, 2);
+    else if (tiles == 2) EMC_LIK(2, 4);
+    else if (unroll >= 16) EMC_LIK(1, 16);
+    else if (unroll == 4) EMC_LIK(1, 4);
+    else EMC_LIK(1, 8);
 #undef EMC_LIK
 }
 
@@ -1016,16 +1029,29 @@ void launch_tomogram_tiles(hipStream_t s, const double *P, int num_data, int r0,
 void launch_tomogram_parts(hipStream_t s, const double *P, int num_data, int r0, int C,
                            const long *toff2, int nparts, const int *tframe, const double *tcnt,
                            const double *psum_r, int num_rot, int num_pix, double *partial,
-                           double *view, int width32) {
+                           double *view, int width32, int unroll, int seq) {
     if (width32) {
         dim3 grid(cdiv(num_pix, kWavesPerBlock) * nparts, C / (kRTile / 2));
         tomogram_part32_kernel<<<grid, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, r0, C, toff2,
                                                        nparts, tframe, tcnt, num_pix, partial);
     } else {
-        dim3 grid(cdiv(num_pix, kWavesPerBlock) * nparts, C / kRTile);
-        tomogram_part_kernel<<<grid, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, r0, C, toff2,
-                                                     nparts, tframe, tcnt, num_pix, partial);
+        const int ngroups = (int)cdiv(num_pix, kWavesPerBlock);
+        dim3 grid(ngroups * nparts, C / kRTile);
+        const bool sq = seq && nparts % kNumXcd == 0;
+#define EMC_TP(U, SQ)                                                                            \
+    tomogram_part_kernel<U, SQ><<<grid, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, r0, C, toff2, \
+                                                        nparts, tframe, tcnt, num_pix, ngroups, partial)
+        if (sq) {
+            if (unroll >= 16) EMC_TP(16, true); else EMC_TP(8, true);
+        } else {
+            if (unroll >= 16) EMC_TP(16, false); else EMC_TP(8, false);
+        }
+#undef EMC_TP
     }
+}
+
+void launch_tomogram_reduce(hipStream_t s, const double *partial, int nparts, int r0, int C,
+                            const double *psum_r, int num_rot, int num_pix, double *view) {
     const size_t n = (size_t)num_pix * C;
     tomogram_reduce_kernel<<<grid_cap(cdiv(n, 256)), 256, 0, s>>>(partial, nparts, r0, C, num_pix,
                                                                   psum_r, num_rot, view);

@@ -45,12 +45,16 @@ typedef struct emc_iter_opts {
     int known_scale;    /* params.known_scale (max_prob.c:124) */
 } emc_iter_opts;
 
+#define EMC_NUM_STAGES 9
+
 /* Results of one iteration that the reference returns or stores in struct iterate. */
 typedef struct emc_iter_stats {
     double avg_likelihood; /* return value of maximize() (max_prob.c:370-374) */
     double mutual_info;    /* iter->mutual_info (max_prob.c:369-373) */
     double rescale;        /* iter->rescale[0] (max_prob.c:59) */
-    double stage_ms[8];    /* expand, likelihood, rescale, max, norm, posterior, tomogram, merge+scale */
+    double stage_ms[EMC_NUM_STAGES]; /* GPU ms per stage (HIP events on the engine stream):
+                              expand, likelihood, rescale, max, norm, posterior, tomogram
+                              (gather), tomogram reduce, merge */
     int num_chunks;        /* orientation chunks the iteration was split into (launches per stage) */
 } emc_iter_stats;
 

@@ -51,6 +51,8 @@ def test_struct_layouts_match_header():
     assert [f[0] for f in _lib.IterOpts._fields_] == re.findall(r'int (\w+);', opts)
     fields = re.findall(r'(?:double|int) (\w+)', stats)
     assert [f[0] for f in _lib.IterStats._fields_] == fields
+    nstages = int(re.search(r'#define EMC_NUM_STAGES (\d+)', text).group(1))
+    assert nstages == len(_lib.STAGES) == _lib.IterStats.stage_ms.size // 8
     enum = re.search(r'enum emc_field \{(.*?)\};', text, re.S).group(1)
     enum = re.sub(r'/\*.*?\*/', '', enum, flags=re.S)
     names = [n.strip().split('=')[0].strip() for n in enum.split(',') if n.strip()]

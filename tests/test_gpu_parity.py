@@ -192,11 +192,17 @@ def test_cfg1_geometry_two_iterations():
     eng.close()
 
 
-@pytest.mark.parametrize('mode', ['2', '1', '0'])
+@pytest.mark.parametrize('mode', ['2', '2-seq16', '1', '0'])
 def test_orientation_chunks_and_tomogram_forms(tiny, monkeypatch, mode):
     """Orientations split into 64-wide chunks (seven chunks for 420 orientations), with each
-    tomogram form (frame-partitioned, pixel-tile, per-pixel gather)."""
+    tomogram form (frame-partitioned, frame-partitioned with 16 ranges walked in sequence per
+    XCD and 16-deep gathers, pixel-tile, per-pixel gather)."""
     monkeypatch.setenv('EMC_MAX_CHUNK_COLS', '64')
+    if mode == '2-seq16':
+        for k, v in (('EMC_TOMO_SEQ', '1'), ('EMC_TOMO_PARTS', '16'), ('EMC_TOMO_UNROLL', '16'),
+                     ('EMC_LIK_UNROLL', '16')):
+            monkeypatch.setenv(k, v)
+        mode = '2'
     monkeypatch.setenv('EMC_TOMO_TILE_MODE', mode)
     flags = dict(iteration=1, need_scaling=0)
     o = P.run_oracle(tiny, flags)
