@@ -29,16 +29,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_recon(prob, cfg, device=0):
+def build_recon(prob, cfg, device=0, dense=None, precision=0, start=None):
     import dragonfly_amd as df
     from dragonfly_amd.iterate import Iterate
     det = prob['det']
     f = prob['frames']
     cdet = df.CDetector.from_arrays(det['qvals'], det['corr'], det['mask'], det['detd'], det['ewald_rad'])
-    ds = df.CDataset.from_arrays(cdet, f['ones'], f['multi'], f['place_ones'], f['place_multi'],
-                                 f['count_multi'])
+    if dense is not None:
+        ds = df.CDataset.from_dense(cdet, dense)
+    else:
+        ds = df.CDataset.from_arrays(cdet, f['ones'], f['multi'], f['place_ones'], f['place_multi'],
+                                     f['count_multi'])
     itr = Iterate()
     itr.params.need_scaling = cfg['need_scaling']
+    itr.params.alpha = cfg.get('alpha', 0.)
     itr.params.beta_factor = 1.
     itr.params.verbosity = 0
     itr.dets = [cdet]
@@ -47,11 +51,12 @@ def build_recon(prob, cfg, device=0):
     from dragonfly_amd import synth
     model = df.Model(cfg['size'])
     model.allocate('')
-    model.model1 = synth.start_model(cfg['size'], itr.mean_count[0], det['mask'])
+    model.model1 = synth.start_model(cfg['size'], itr.mean_count[0], det['mask']) if start is None \
+        else start.copy()
     itr.set_model(model)
     itr.parse_scale('')
     itr.calc_beta(None if cfg['beta'] == 'auto' else float(cfg['beta']))
-    recon = df.EMCRecon(device=device)
+    recon = df.EMCRecon(device=device, precision=precision)
     recon.write_output = False
     recon.set_iterate(itr)
     return recon, itr
@@ -131,6 +136,93 @@ def roofline(prob, itr, stage_tot, steps, nchunks, pairs, workload):
     return out
 
 
+def run_steps(recon, itr, warmup, steps, keep_models=False):
+    """warmup + steps iterations; returns elapsed seconds of the timed steps, the summed
+    stage times, the chunk count, and per iteration (avg log-likelihood, model1 copy)."""
+    import torch
+    trace = []
+    for it in range(1, warmup + 1):
+        itr.params.iteration = it
+        recon.run_iteration()
+        trace.append((itr.likelihood, itr.model.model1.copy() if keep_models else None))
+        log('warmup iteration %d: %s' % (it, json.dumps({k: round(v, 1) for k, v in recon.last_stats['stage_ms'].items()})))
+    torch.cuda.synchronize()
+    stage_tot, nchunks, elapsed = {}, 1, 0.
+    for k in range(steps):
+        itr.params.iteration = warmup + 1 + k
+        t0 = time.perf_counter()
+        recon.run_iteration()
+        torch.cuda.synchronize()
+        elapsed += time.perf_counter() - t0
+        st = recon.last_stats['stage_ms']
+        nchunks = recon.last_stats['num_chunks']
+        for name, v in st.items():
+            stage_tot[name] = stage_tot.get(name, 0.) + v
+        log('iteration %d: %s' % (itr.params.iteration, json.dumps({k2: round(v, 1) for k2, v in st.items()})))
+        trace.append((itr.likelihood, itr.model.model1.copy() if keep_models else None))
+    return elapsed, stage_tot, nchunks, trace
+
+
+def bench_dense(args):
+    """Dense integer frames: the mixed-precision path (FP16 products on matrix cores) timed,
+    then the plain FP64 path on the same frames from the same start model, with the model and
+    log-likelihood relative errors of every iteration."""
+    import torch
+    from dragonfly_amd import synth
+    log('generating %s workload ...' % args.config)
+    prob = synth.make_problem(args.config, device='cuda', num_data=args.num_data,
+                              num_div=args.num_div, batch=512)
+    cfg = prob['cfg']
+    mask = prob['det']['mask']
+    K = synth.dense_frames(prob['frames'], len(mask))
+    recon, itr = build_recon(prob, cfg, dense=K, precision=args.precision)
+    start = itr.model.model1.copy()
+    D, R = itr.tot_num_data, itr.quat.num_rot
+    pairs = (D - itr.num_blacklist) * R
+    el_mp, st_mp, nch_mp, tr_mp = run_steps(recon, itr, args.warmup, args.steps, keep_models=True)
+    recon.free()
+    del recon
+    torch.cuda.empty_cache()
+    out_fp = None
+    if not args.no_fp64:
+        recon, itr2 = build_recon(prob, cfg, dense=K, precision=0, start=start)
+        el_fp, st_fp, nch_fp, tr_fp = run_steps(recon, itr2, args.warmup, args.steps, keep_models=True)
+        recon.free()
+        model_err = [float(np.abs(a[1] - b[1]).max() / np.abs(b[1]).max()) for a, b in zip(tr_mp, tr_fp)]
+        like_err = [abs(a[0] - b[0]) / abs(b[0]) for a, b in zip(tr_mp, tr_fp)]
+        out_fp = dict(fp64_value=round(pairs * args.steps / el_fp, 1),
+                      fp64_ms_per_step=round(el_fp / args.steps * 1e3, 2),
+                      speedup_vs_fp64=round(el_fp / el_mp, 2),
+                      model_rel_err=[float('%.3g' % x) for x in model_err],
+                      loglik_rel_err=[float('%.3g' % x) for x in like_err],
+                      fp64_stage_ms_mean={k: round(v / args.steps, 2) for k, v in st_fp.items()})
+    value = pairs * args.steps / el_mp
+    terms = 2 if args.precision == 2 else 1
+    P = len(mask)
+    launch_ms = st_mp['likelihood'] / args.steps / nch_mp
+    flops = 2. * D * (R / nch_mp) * P * terms
+    achieved = flops / (launch_ms * 1e-3) / 1e12
+    roof = dict(bound='mfma', achieved=round(achieved, 1), peak=2500., unit='TFLOP/s',
+                frac=round(achieved / 2500., 4), traffic=None, kernel='mp_gemm_kernel (likelihood)',
+                launch_ms=round(launch_ms, 3), algorithmic_flops=flops)
+    out = {
+        'metric': METRIC, 'value': round(value, 1), 'unit': 'frame×orientation pairs/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(el_mp / args.steps * 1e3, 2), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f16 products / f32 accumulate / f64 rest',
+        'data': 'synthetic',
+        'config': {'workload': '%s: %d dense frames, %d orientations (num_div %d), %d^3 model, %d pixels'
+                               % (args.config, D, R, cfg['num_div'], cfg['size'], P),
+                   'path': 'mixed precision, %d-term FP16 log W' % terms, 'frames': D, 'num_rot': R,
+                   'num_pix': P, 'mean_photons': round(float(K.sum() / D), 1)},
+        'stage_ms_mean': {k: round(v / args.steps, 2) for k, v in st_mp.items()},
+        'roofline': roof, 'cpu_baseline': None,
+    }
+    if out_fp:
+        out.update(out_fp)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -141,6 +233,10 @@ def main():
     ap.add_argument('--num-div', type=int, default=None)
     ap.add_argument('--cpu-rot', type=int, default=3072, help='orientations in the CPU sample')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--precision', type=int, default=2,
+                    help='dense-frame configurations: 1 or 2 FP16 terms of log W')
+    ap.add_argument('--no-fp64', action='store_true',
+                    help='dense-frame configurations: skip the FP64 comparison run')
     ap.add_argument('--copy-calibration-mib', type=int, default=0,
                     help='run one device copy of this many MiB first (counter calibration)')
     args = ap.parse_args()
@@ -159,6 +255,9 @@ def main():
         torch.cuda.synchronize()
         log('copy calibration: %d MiB read + %d MiB written' % (args.copy_calibration_mib, args.copy_calibration_mib))
         del a, b
+    if synth.CONFIGS.get(args.config, {}).get('dense'):
+        bench_dense(args)
+        return
     t0 = time.perf_counter()
     log('generating %s workload ...' % args.config)
     prob = synth.make_problem(args.config, device='cuda', num_data=args.num_data,

@@ -65,6 +65,7 @@ def lib():
             'emc_set_model': (C.c_int, [C.c_void_p, C.c_long, _dp]),
             'emc_set_frames_sparse': (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _lp, _lp, _ip, _ip, _ip]),
             'emc_set_frames_dense': (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip]),
+            'emc_set_precision': (C.c_int, [C.c_void_p, C.c_int]),
             'emc_set_frame_params': (C.c_int, [C.c_void_p, _dp, _dp, _u8p, _ip, _dp, C.c_double, C.c_int]),
             'emc_set_beta': (C.c_int, [C.c_void_p, _dp]),
             'emc_set_scale': (C.c_int, [C.c_void_p, _dp]),
@@ -171,6 +172,10 @@ class Engine:
         _check(lib().emc_set_frames_dense(self._h, frames.shape[0], frames.shape[1], _ptr(frames, _ip)))
         self.num_data = frames.shape[0]
 
+
+    def set_precision(self, mode):
+        """0: plain FP64; 1: FP16 log W on matrix cores (dense frames); 2: FP16 hi + lo."""
+        _check(lib().emc_set_precision(self._h, int(mode)))
     def set_frame_params(self, scale, beta, blacklist, fcounts, sum_fact, mean_count, num_blacklist=0):
         a = [np.ascontiguousarray(x, dtype=t) for x, t in (
             (scale, 'f8'), (beta, 'f8'), (blacklist, 'u1'), (fcounts, 'i4'), (sum_fact, 'f8'))]

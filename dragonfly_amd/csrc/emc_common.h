@@ -17,6 +17,14 @@ constexpr int kWavesPerBlock = kBlock / 64;
 // its own L2.
 constexpr int kNumXcd = 8;
 
+// Mixed-precision product tiles (emc_mp.hip): block tile kGemmBM x kGemmBN, kGemmBK halves
+// of K per step, 8 waves; kGemmGroupM m-tiles per panel of the block order.
+constexpr int kGemmBM = 256;
+constexpr int kGemmBN = 256;
+constexpr int kGemmBK = 32;
+constexpr int kGemmThreads = 512;
+constexpr int kGemmGroupM = 8;
+
 // Reference constants (dragonfly/src/max_internal.h:6-8).
 constexpr double kProbMin = 1.e-6;
 constexpr double kPdiffThresh = 14.;

@@ -183,6 +183,15 @@ struct emc_engine {
     // form (2: frame-partitioned gather, 1: pixel-tile LDS accumulation, 0: per-pixel gather)
     int lik_tiles = 1, tomo_tiles = 1, tomo_tile_mode = 2;
     int lik_unroll = 8, tomo_unroll = 16, tomo_seq = 0;
+
+    // mixed-precision dense-frame path (0: plain FP64; 1: FP16 log W; 2: FP16 hi + lo log W)
+    int precision = 0;
+    int max_count = 0;                  // largest photon count in the frames
+    size_t mp_ldp = 0, mp_ldd = 0;      // padded pixel / frame counts (K of the two products)
+    size_t mp_rows_d = 0, mp_rows_t = 0, mp_rows_c = 0;
+    bool mp_built = false;
+    DevBuf<_Float16> klik, ktom, lw_hi, lw_lo, p16;
+    DevBuf<double> center_t, center_base;
 };
 
 namespace {
@@ -229,6 +238,7 @@ int build_lists(emc_engine *e) {
     std::vector<int> tframe(tcount[P]);
     std::vector<double> tcnt(tcount[P]);
     std::vector<long> tfill(tcount.begin(), tcount.end() - 1);
+    int max_count = 1;
     for (int d = 0; d < N; ++d) {
         long at = loff[d];
         for (int k = 0; k < e->h_ones[d]; ++k) {
@@ -240,6 +250,7 @@ int build_lists(emc_engine *e) {
             long j = e->h_multi_accum[d] + k;
             int p = e->h_place_multi[j], q = e->inv[p];
             double c = (double)e->h_count_multi[j];
+            max_count = std::max(max_count, e->h_count_multi[j]);
             if (mask[p] < 1) { lpix[at] = q; lcnt[at] = c; ++at; }
             if (mask[p] < 2) { tframe[tfill[q]] = d; tcnt[tfill[q]] = c; ++tfill[q]; }
         }
@@ -329,9 +340,13 @@ int build_lists(emc_engine *e) {
     if (!e->tom_frame.p && (rc = e->tom_frame.alloc(1))) return rc;
     if (!e->tom_cnt.p && (rc = e->tom_cnt.alloc(1))) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
+    e->max_count = max_count;
     e->lists_dirty = false;
+    e->mp_built = false;
     return 0;
 }
+
+size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
 
 // Work buffers. The posterior matrix R/P holds all frames x all orientations (the keep
 // rule needs each frame's global maximum before any normalisation); log W / view hold one
@@ -347,9 +362,23 @@ int ensure_work(emc_engine *e) {
     const size_t cap = (size_t)120 << 30;
     if (e->logw.n) budget += e->logw.n * sizeof(double);
     if (e->tom_partial.n) budget += e->tom_partial.n * sizeof(double);
+    budget += (e->lw_hi.n + e->lw_lo.n + e->p16.n) * sizeof(_Float16);
+    const bool mp = e->precision > 0;
+    if (mp) {
+        e->mp_ldp = round_up(P, emc::kGemmBK);
+        e->mp_ldd = round_up(N, emc::kGemmBK);
+        e->mp_rows_d = round_up(N, emc::kGemmBM);
+        e->mp_rows_t = round_up(P, emc::kGemmBM);
+        const size_t kbytes = (e->mp_rows_d * e->mp_ldp + e->mp_rows_t * e->mp_ldd) * sizeof(_Float16);
+        const size_t have = (e->klik.n + e->ktom.n) * sizeof(_Float16);
+        budget = budget + have > kbytes ? budget + have - kbytes : 0;
+    }
     if (budget > cap) budget = cap;
-    // per orientation column: log W / view, plus the partial views of the partitioned tomogram
-    const size_t per_col = P * sizeof(double) * (1 + (e->tomo_tile_mode == 2 ? e->tom_parts : 0));
+    // per orientation column: log W / view, plus the partial views of the partitioned
+    // tomogram (plain path) or the FP16 log W rows and posterior column (mixed path)
+    const size_t per_col = mp ? P * sizeof(double) + (e->precision == 2 ? 2 : 1) * e->mp_ldp * sizeof(_Float16) +
+                                    e->mp_ldd * sizeof(_Float16)
+                              : P * sizeof(double) * (1 + (e->tomo_tile_mode == 2 ? e->tom_parts : 0));
     size_t cmax = budget / per_col;
     if (e->max_chunk_cols > 0 && cmax > (size_t)e->max_chunk_cols) cmax = e->max_chunk_cols;
     cmax = cmax / emc::kRTile * emc::kRTile;
@@ -360,7 +389,44 @@ int ensure_work(emc_engine *e) {
         if ((rc = e->logw.alloc(P * (size_t)C))) return rc;
         e->C = C;
     }
-    if (e->tomo_tile_mode == 2 && (rc = e->tom_partial.alloc((size_t)e->tom_parts * P * (size_t)C))) return rc;
+    if (mp) {
+        e->tom_partial.release();
+        const size_t rows_c = round_up((size_t)e->C, emc::kGemmBN);
+        if (rows_c != e->mp_rows_c || !e->lw_hi.p || (e->precision == 2) != (e->lw_lo.p != nullptr)) {
+            e->lw_hi.release();
+            e->lw_lo.release();
+            e->p16.release();
+            if ((rc = e->lw_hi.alloc(rows_c * e->mp_ldp))) return rc;
+            if (e->precision == 2 && (rc = e->lw_lo.alloc(rows_c * e->mp_ldp))) return rc;
+            if ((rc = e->p16.alloc(rows_c * e->mp_ldd))) return rc;
+            // padding rows and columns stay zero: the kernels only write valid entries
+            HIP_TRY(hipMemsetAsync(e->lw_hi.p, 0, e->lw_hi.n * sizeof(_Float16), e->stream));
+            if (e->lw_lo.p) HIP_TRY(hipMemsetAsync(e->lw_lo.p, 0, e->lw_lo.n * sizeof(_Float16), e->stream));
+            HIP_TRY(hipMemsetAsync(e->p16.p, 0, e->p16.n * sizeof(_Float16), e->stream));
+            e->mp_rows_c = rows_c;
+        }
+        if (!e->mp_built) {
+            if ((rc = e->klik.alloc(e->mp_rows_d * e->mp_ldp))) return rc;
+            if ((rc = e->ktom.alloc(e->mp_rows_t * e->mp_ldd))) return rc;
+            HIP_TRY(hipMemsetAsync(e->klik.p, 0, e->klik.n * sizeof(_Float16), e->stream));
+            HIP_TRY(hipMemsetAsync(e->ktom.p, 0, e->ktom.n * sizeof(_Float16), e->stream));
+            emc::launch_build_klik(e->stream, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p, (int)N, e->mp_ldp, e->klik.p);
+            emc::launch_build_ktom(e->stream, e->tom_off.p, e->tom_frame.p, e->tom_cnt.p, (int)P, e->mp_ldd, e->ktom.p);
+            HIP_TRY(hipGetLastError());
+            e->mp_built = true;
+        }
+        if ((rc = e->center_t.alloc(P))) return rc;
+        if ((rc = e->center_base.alloc(N))) return rc;
+    } else {
+        e->lw_hi.release();
+        e->lw_lo.release();
+        e->p16.release();
+        e->klik.release();
+        e->ktom.release();
+        e->mp_built = false;
+        e->mp_rows_c = 0;
+        if (e->tomo_tile_mode == 2 && (rc = e->tom_partial.alloc((size_t)e->tom_parts * P * (size_t)C))) return rc;
+    }
     const size_t ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
     if ((rc = e->upart.alloc(ngroups * (size_t)e->C))) return rc;
     if ((rc = e->ubad.alloc(ngroups * (size_t)e->C))) return rc;
@@ -589,6 +655,16 @@ int emc_set_frames_dense(emc_engine *e, int num_data, int num_pix, const int *fr
     return 0;
 }
 
+int emc_set_precision(emc_engine *e, int mode) {
+    if (!e) return fail(EMC_E_ARG, "null engine");
+    if (mode < 0 || mode > 2) return fail(EMC_E_ARG, "emc_set_precision: mode %d (0, 1 or 2)", mode);
+    if (mode != e->precision) {
+        e->precision = mode;
+        e->mp_built = false;
+    }
+    return 0;
+}
+
 int emc_set_frame_params(emc_engine *e, const double *scale, const double *beta,
                          const uint8_t *blacklist, const int *fcounts, const double *sum_fact,
                          double mean_count, int num_blacklist) {
@@ -644,6 +720,13 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     if (!opts) return fail(EMC_E_ARG, "emc_maximize: null options");
     HIP_TRY(hipSetDevice(e->device));
     if (e->lists_dirty && (rc = build_lists(e))) return rc;
+    const bool mp = e->precision > 0;
+    if (mp) {
+        for (uint8_t v : e->h_dense)
+            if (!v) return fail(EMC_E_STATE, "emc_maximize: the mixed-precision path needs dense frames");
+        if (e->max_count > 2048)
+            return fail(EMC_E_STATE, "emc_maximize: photon count %d is not exact in FP16 (limit 2048)", e->max_count);
+    }
     if ((rc = ensure_work(e))) return rc;
 
     hipStream_t s = e->stream;
@@ -658,17 +741,47 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     emc::launch_fill(s, e->model2.p, vol, 0.);
     emc::launch_fill(s, e->weight.p, vol, 0.);
 
+    // mixed path: per-pixel centre of log W and its per-frame photon sum, from this model
+    if (mp) {
+        HIP_TRY(hipEventRecord(e->ev[0], s));
+        emc::launch_center(s, e->rots.p, Rn, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p, P,
+                           e->center_t.p);
+        emc::launch_center_base(s, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p, e->center_t.p, N,
+                                e->center_base.p);
+        HIP_TRY(hipEventRecord(e->ev[1], s));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventSynchronize(e->ev[1]));
+        float a = 0.f;
+        HIP_TRY(hipEventElapsedTime(&a, e->ev[0], e->ev[1]));
+        ms_expand += a;
+    }
+
     // expand + likelihood, chunk by chunk over orientations
     for (int r0 = 0; r0 < e->Rp; r0 += C) {
         const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
         float a = 0.f, b = 0.f;
         HIP_TRY(hipEventRecord(e->ev[0], s));
-        emc::launch_expand(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
-                           P, r0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p);
+        if (mp)
+            emc::launch_expand_mp(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
+                                  e->center_t.p, P, r0, Cc, kPixPerGroup, e->mp_ldp, e->precision == 2,
+                                  e->lw_hi.p, e->lw_lo.p, e->upart.p, e->ubad.p);
+        else
+            emc::launch_expand(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
+                               P, r0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p);
         emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, r0, Rn, e->u.p, e->rbad.p);
         HIP_TRY(hipEventRecord(e->ev[1], s));
-        emc::launch_likelihood(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
-                               e->blacklist.p, N, e->R.p, r0, e->lik_tiles, e->lik_unroll);
+        if (mp) {
+            const size_t ts = (size_t)N * emc::kRTile;
+            if (emc::launch_mp_gemm(s, e->precision == 2 ? 2 : 1, 0, e->klik.p, e->mp_ldp, e->lw_hi.p,
+                                    e->lw_lo.p, e->mp_ldp, (int)e->mp_ldp, (int)e->mp_rows_d,
+                                    (int)round_up(Cc, emc::kGemmBN), N, Cc,
+                                    e->R.p + (size_t)(r0 / emc::kRTile) * ts, ts, e->center_base.p,
+                                    nullptr, 0, 1.))
+                return fail(EMC_E_STATE, "likelihood product: shape does not fit the tiles");
+        } else {
+            emc::launch_likelihood(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
+                                   e->blacklist.p, N, e->R.p, r0, e->lik_tiles, e->lik_unroll);
+        }
         HIP_TRY(hipEventRecord(e->ev[2], s));
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventSynchronize(e->ev[2]));
@@ -737,8 +850,17 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
         float a = 0.f, b = 0.f, c = 0.f;
         HIP_TRY(hipEventRecord(e->ev[0], s));
-        const bool parts = e->tomo_tile_mode == 2;
-        if (parts)
+        const bool parts = !mp && e->tomo_tile_mode == 2;
+        if (mp) {
+            // P^T K on matrix cores: P stored as fp16(P * 2^15) where P > 1e-6
+            const double pscale = 32768.;
+            emc::launch_posterior_fp16(s, e->R.p, N, r0, Cc, e->mp_ldd, pscale, e->p16.p);
+            if (emc::launch_mp_gemm(s, 1, 1, e->ktom.p, e->mp_ldd, e->p16.p, nullptr, e->mp_ldd,
+                                    (int)e->mp_ldd, (int)e->mp_rows_t, (int)round_up(Cc, emc::kGemmBN), P,
+                                    Cc, e->logw.p, (size_t)P * emc::kRTile, nullptr, e->psum_r.p + r0,
+                                    Rn - r0, 1. / pscale))
+                return fail(EMC_E_STATE, "tomogram product: shape does not fit the tiles");
+        } else if (parts)
             emc::launch_tomogram_parts(s, e->R.p, N, r0, Cc, e->tom_off2.p, e->tom_parts,
                                        e->tom_frame.p, e->tom_cnt.p, e->psum_r.p, Rn, P,
                                        e->tom_partial.p, e->logw.p, e->tomo_width32,

@@ -67,4 +67,27 @@ void launch_update_model(hipStream_t s, double *model1, double *model2, const do
 void launch_normalize_scale(hipStream_t s, double *scale, const uint8_t *blacklist, int num_data,
                             double *model1, long vol, double *mean_out);
 
+// ---- mixed-precision path (emc_mp.hip) ----
+size_t gemm_lds_bytes(int nb);
+int launch_mp_gemm(hipStream_t s, int nb, int epi, const _Float16 *A, size_t lda, const _Float16 *B0,
+                   const _Float16 *B1, size_t ldb, int K, int Mp, int Np, int M_valid, int N_valid,
+                   double *out, size_t ts, const double *row_add, const double *col_div,
+                   int div_valid, double scale);
+void launch_build_klik(hipStream_t s, const long *off, const int *pix, const double *cnt, int num_data,
+                       size_t ldp, _Float16 *klik);
+void launch_build_ktom(hipStream_t s, const long *toff, const int *tframe, const double *tcnt,
+                       int num_pix, size_t ldd, _Float16 *ktom);
+void launch_center(hipStream_t s, const double *rots, int num_rot, const double *model, long size,
+                   const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
+                   double *center_t);
+void launch_center_base(hipStream_t s, const long *off, const int *pix, const double *cnt,
+                        const double *center_t, int num_data, double *base);
+void launch_expand_mp(hipStream_t s, const double *rots, const double *model, long size,
+                      const double *qvals, const double *corr, const uint8_t *mask,
+                      const double *center_t, int num_pix, int r0, int C, int pix_per_group,
+                      size_t ldp, int two_terms, _Float16 *lw_hi, _Float16 *lw_lo, double *upart,
+                      int *ubad);
+void launch_posterior_fp16(hipStream_t s, const double *P, int num_data, int r0, int C, size_t ldd,
+                           double pscale, _Float16 *p16);
+
 }  // namespace emc

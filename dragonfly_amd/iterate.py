@@ -183,11 +183,11 @@ class Iterate:
             # calc_sum_fact only mask < 1 pixels (iterate.c:103-108)
             self.fcounts = fr.sum(1, dtype='i8').astype('i4')
             good = mask < 1
+            # sum of log(K!) over mask < 1 pixels, from a table of log(k!) per count value
+            table = lnfact(np.arange(int(fr.max()) + 1 if fr.size else 1))
             self.sum_fact = np.zeros(total)
-            for d in range(total):
-                k = fr[d, good]
-                k = k[k > 1]
-                self.sum_fact[d] = lnfact(k).sum() if k.size else 0.
+            for d0 in range(0, total, 512):
+                self.sum_fact[d0:d0 + 512] = table[fr[d0:d0 + 512][:, good]].sum(1)
         else:
             sp = self.sparse_arrays()
             self.fcounts = frame_counts(sp['ones'], sp['multi'], sp['place_ones'], sp['place_multi'],
@@ -199,7 +199,10 @@ class Iterate:
 
     def dense_frames(self):
         """All dense datasets stacked, (tot_num_data, num_pix) int32."""
-        return np.concatenate([p.int_frames for p in self.data]).astype('i4')
+        parts = [p.int_frames for p in self.data]
+        if len(parts) == 1:
+            return np.ascontiguousarray(parts[0], dtype='i4')
+        return np.concatenate(parts).astype('i4')
 
     def sparse_arrays(self):
         """All datasets concatenated into one frame list (linked list order)."""

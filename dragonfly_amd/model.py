@@ -16,9 +16,42 @@ class Model:
         self.mtype = 'MODEL_3D'
         self.num_modes = num_modes
         self.size = int(size)
-        self.model1 = self.model2 = self.inter_weight = None
         self._engine = None
         self._stale = False
+        self._model1 = self._model2 = self._weight = None
+
+    # While bound to an engine the live volumes are on the GPU; reading one after an
+    # iteration copies it back, assigning model1 sends it to the engine.
+    @property
+    def model1(self):
+        self.sync()
+        return self._model1
+
+    @model1.setter
+    def model1(self, arr):
+        self._model1 = arr
+        if self._engine is not None and arr is not None:
+            self.sync()
+            self._model1 = arr
+            self._engine.set_model(arr)
+
+    @property
+    def model2(self):
+        self.sync()
+        return self._model2
+
+    @model2.setter
+    def model2(self, arr):
+        self._model2 = arr
+
+    @property
+    def inter_weight(self):
+        self.sync()
+        return self._weight
+
+    @inter_weight.setter
+    def inter_weight(self, arr):
+        self._weight = arr
 
     @property
     def center(self):
@@ -60,10 +93,10 @@ class Model:
     def sync(self):
         """Refresh the host arrays from the engine after an iteration."""
         if self._engine is not None and self._stale:
-            self.model1 = self._engine.get('model1')
-            self.model2 = self._engine.get('model2')
-            self.inter_weight = self._engine.get('inter_weight')
             self._stale = False
+            self._model1 = self._engine.get('model1')
+            self._model2 = self._engine.get('model2')
+            self._weight = self._engine.get('inter_weight')
         return self
 
     def slice_gen(self, quat, det, engine):

@@ -28,10 +28,14 @@ class EMCRecon:
         ...     recon.run_iteration()
     """
 
-    def __init__(self, num_threads=-1, device=None):
-        # num_threads is kept for interface compatibility; the GPU engine does not use it
+    def __init__(self, num_threads=-1, device=None, precision=0):
+        # num_threads is kept for interface compatibility; the GPU engine does not use it.
+        # precision selects the arithmetic of the photon contractions for dense frames
+        # (0: FP64 as the reference; 1, 2: FP16 products on matrix cores, see
+        # Engine.set_precision).
         self.num_threads = num_threads
         self.device = int(os.environ.get('DRAGONFLY_DEVICE', '0')) if device is None else device
+        self.precision = precision
         self.engine = None
         self.iter = None
         self.write_output = True
@@ -61,6 +65,7 @@ class EMCRecon:
         if self.engine is None:
             self.engine = _lib.Engine(self.device)
         eng = self.engine
+        eng.set_precision(self.precision)
         det = itr.dets[0]
         eng.set_detector(det.qvals, det.corr, det.raw_mask)
         eng.set_quats(itr.quat.quats)

@@ -29,6 +29,10 @@ CONFIGS = {
     'cfg2': dict(size=251, atoms=400, atom_radius=30., sigma=1.5, nx=260, ny=257, gap=10,
                  stoprad=5., num_data=100000, mean_count=200., phi=(0.5, 1.5), num_div=10,
                  need_scaling=1, beta='auto', iters=5),
+    # dense integer frames (ftype DENSE_INT) of ~5000 photons for the mixed-precision path
+    'cfg3': dict(size=251, atoms=400, atom_radius=30., sigma=1.5, nx=260, ny=257, gap=10,
+                 stoprad=5., num_data=20000, mean_count=5000., phi=None, num_div=10,
+                 need_scaling=0, beta='auto', iters=5, dense=True, alpha=0.5),
 }
 
 
@@ -221,6 +225,17 @@ def simulate_frames(intens, det, num_data, mean_count, phi=None, device='cpu', b
     out['ones_accum'] = np.concatenate([[0], np.cumsum(ones)[:-1]]).astype('i8')
     out['multi_accum'] = np.concatenate([[0], np.cumsum(multi)[:-1]]).astype('i8')
     return out
+
+
+def dense_frames(frames, num_pix):
+    """(num_data, num_pix) int32 counts from the sparse lists of simulate_frames."""
+    D = len(frames['ones'])
+    K = np.zeros((D, num_pix), dtype='i4')
+    rows = np.repeat(np.arange(D), frames['ones'])
+    K[rows, frames['place_ones']] = 1
+    rows = np.repeat(np.arange(D), frames['multi'])
+    K[rows, frames['place_multi']] = frames['count_multi']
+    return K
 
 
 def start_model(size, mean_count, mask, seed=4):

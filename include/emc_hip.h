@@ -110,6 +110,13 @@ int emc_set_frames_sparse(emc_engine *e, int num_data, const int *ones, const in
  * orientation with log W = -inf on any such pixel is dropped for these frames; the tomogram
  * adds every pixel (max_update.c:211-214).                                             */
 int emc_set_frames_dense(emc_engine *e, int num_data, int num_pix, const int *frames);
+/* arithmetic of the two photon contractions. No counterpart in the reference (which
+ * computes everything in FP64): 0 = plain FP64 (default, reproduces the reference);
+ * 1 = dense frames only: likelihood and tomogram as FP16 products on matrix cores with
+ * FP32 accumulation, log W centred per pixel and rounded once to FP16; 2 = as 1 with
+ * log W carried as an FP16 hi + lo pair. Every other step stays FP64. Modes 1 and 2 need
+ * frames set with emc_set_frames_dense and counts <= 2048.                              */
+int emc_set_precision(emc_engine *e, int mode);
 /* per-frame state of struct iterate: scale, beta, blacklist, fcounts, sum_fact,
  *        mean_count[0], num_blacklist (iterate.h:17-24,33; iterate.c:3-117)          */
 int emc_set_frame_params(emc_engine *e, const double *scale, const double *beta,
