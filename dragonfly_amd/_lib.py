@@ -65,7 +65,18 @@ def lib():
             'emc_set_model': (C.c_int, [C.c_void_p, C.c_long, _dp]),
             'emc_set_frames_sparse': (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _lp, _lp, _ip, _ip, _ip]),
             'emc_set_frames_dense': (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip]),
+            'emc_voronoi_subset': (C.c_int, [C.c_int, _dp, C.c_int, _dp, _ip]),
+            'emc_nn_forward': (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t]),
+            'emc_nn_backward_data': (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t]),
+            'emc_nn_backward_weight': (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_size_t]),
+            'emc_nn_transpose': (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_void_p, C.c_size_t]),
+            'emc_nn_softmax_xent': (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_float, C.c_void_p, C.c_void_p, C.c_size_t]),
+            'emc_nn_colsum': (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_void_p]),
+            'emc_nn_adam': (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_size_t, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t]),
+            'emc_nn_polar_hist': (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t]),
+            'emc_nn_topk': (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_void_p]),
             'emc_set_precision': (C.c_int, [C.c_void_p, C.c_int]),
+            'emc_set_refine': (C.c_int, [C.c_void_p, C.c_int, _ip, _lp, _ip]),
             'emc_set_frame_params': (C.c_int, [C.c_void_p, _dp, _dp, _u8p, _ip, _dp, C.c_double, C.c_int]),
             'emc_set_beta': (C.c_int, [C.c_void_p, _dp]),
             'emc_set_scale': (C.c_int, [C.c_void_p, _dp]),
@@ -99,6 +110,15 @@ def num_rot(num_div):
     if n < 0:
         raise ValueError('num_div %d out of range' % num_div)
     return n
+
+
+def voronoi_subset(qcoarse, qfine):
+    """Index of the nearest coarse orientation of every fine orientation (quaternion.c:844-862)."""
+    qc = np.ascontiguousarray(qcoarse, dtype='f8')
+    qf = np.ascontiguousarray(qfine, dtype='f8')
+    out = np.zeros(len(qf), dtype='i4')
+    _check(lib().emc_voronoi_subset(len(qc), _ptr(qc), len(qf), _ptr(qf), _ptr(out, _ip)))
+    return out
 
 
 def quat_gen(num_div):
@@ -172,6 +192,18 @@ class Engine:
         _check(lib().emc_set_frames_dense(self._h, frames.shape[0], frames.shape[1], _ptr(frames, _ip)))
         self.num_data = frames.shape[0]
 
+
+    def set_refine(self, num_coarse, quat_mapping, rel_off, rel_quat):
+        """Per-frame orientation subsets (refinement); quat_mapping None switches it off."""
+        if quat_mapping is None:
+            _check(lib().emc_set_refine(self._h, 0, None, None, None))
+            return
+        qm = np.ascontiguousarray(quat_mapping, dtype='i4')
+        ro = np.ascontiguousarray(rel_off, dtype='i8')
+        rq = np.ascontiguousarray(rel_quat, dtype='i4')
+        if rq.size == 0:
+            rq = np.zeros(1, 'i4')
+        _check(lib().emc_set_refine(self._h, int(num_coarse), _ptr(qm, _ip), _ptr(ro, _lp), _ptr(rq, _ip)))
 
     def set_precision(self, mode):
         """0: plain FP64; 1: FP16 log W on matrix cores (dense frames); 2: FP16 hi + lo."""

@@ -25,6 +25,11 @@ constexpr int kGemmBK = 32;
 constexpr int kGemmThreads = 512;
 constexpr int kGemmGroupM = 8;
 
+// Brick merge: bricks of kBrick^3 voxels accumulated in LDS; at most kUnitItems work items
+// (orientation, 64-pixel patch) per block.
+constexpr int kBrick = 16;
+constexpr int kUnitItems = 2048;
+
 // Reference constants (dragonfly/src/max_internal.h:6-8).
 constexpr double kProbMin = 1.e-6;
 constexpr double kPdiffThresh = 14.;

@@ -192,6 +192,25 @@ struct emc_engine {
     bool mp_built = false;
     DevBuf<_Float16> klik, ktom, lw_hi, lw_lo, p16;
     DevBuf<double> center_t, center_base;
+
+    // refinement: coarse cell of every orientation and per-frame bit rows of allowed cells
+    bool refine = false;
+    int rel_words = 0;
+    DevBuf<int> qmap;
+    DevBuf<uint32_t> relbits;
+
+    // brick merge (1, default) or per-wave voxel boxes (0); the brick form needs an
+    // orientation-major view, which the frame-partitioned tomogram and the mixed path write
+    int merge_mode = 1;
+    struct BrickChunk {
+        int r0 = -1, Cc = 0, num_units = 0;
+        DevBuf<int> list;
+        DevBuf<int4> units;
+    };
+    std::vector<BrickChunk> bricks;
+    bool bricks_valid = false;
+    DevBuf<int> brick_counts, brick_fill;
+    DevBuf<long> brick_off;
 };
 
 namespace {
@@ -348,6 +367,68 @@ int build_lists(emc_engine *e) {
 
 size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
 
+// Per-chunk brick item lists and work units of the brick merge (geometry only: rebuilt when
+// rotations, detector, model size or chunk width change).
+int build_bricks(emc_engine *e) {
+    hipStream_t s = e->stream;
+    const int P = e->num_pix, C = e->C, Rn = e->num_rot;
+    const int nbx = (int)((e->size + emc::kBrick - 1) / emc::kBrick);
+    const size_t nb = (size_t)nbx * nbx * nbx;
+    const int npatch = (P + 63) / 64;
+    int rc;
+    if ((rc = e->brick_counts.alloc(nb))) return rc;
+    if ((rc = e->brick_fill.alloc(nb))) return rc;
+    if ((rc = e->brick_off.alloc(nb))) return rc;
+    std::vector<int> counts(nb);
+    std::vector<long> off(nb);
+    e->bricks.clear();
+    for (int r0 = 0; r0 < e->Rp; r0 += C) {
+        const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
+        e->bricks.emplace_back();
+        emc_engine::BrickChunk &bc = e->bricks.back();
+        bc.r0 = r0;
+        bc.Cc = Cc;
+        HIP_TRY(hipMemsetAsync(e->brick_counts.p, 0, nb * sizeof(int), s));
+        emc::launch_brick_items(s, e->rots.p, r0, Cc, Rn, e->qvals.p, e->mask.p, P, e->size, npatch, nbx, 0,
+                                e->brick_counts.p, nullptr, nullptr, nullptr);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(counts.data(), e->brick_counts.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        long total = 0;
+        std::vector<int4> units;
+        for (size_t b = 0; b < nb; ++b) {
+            off[b] = total;
+            for (int k = 0; k < counts[b]; k += emc::kUnitItems)
+                units.push_back(make_int4((int)b, (int)(total + k),
+                                          (int)(total + std::min(counts[b], k + emc::kUnitItems)), 0));
+            total += counts[b];
+        }
+        if (total >= INT32_MAX) return fail(EMC_E_ARG, "brick merge: too many work items");
+        if ((rc = bc.list.alloc(total > 0 ? total : 1))) return rc;
+        if ((rc = e->brick_off.upload(off.data(), nb, s))) return rc;
+        HIP_TRY(hipMemsetAsync(e->brick_fill.p, 0, nb * sizeof(int), s));
+        emc::launch_brick_items(s, e->rots.p, r0, Cc, Rn, e->qvals.p, e->mask.p, P, e->size, npatch, nbx, 1,
+                                nullptr, e->brick_off.p, e->brick_fill.p, bc.list.p);
+        HIP_TRY(hipGetLastError());
+        bc.num_units = (int)units.size();
+        if (!units.empty() && (rc = bc.units.upload(units.data(), units.size(), s))) return rc;
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    e->bricks_valid = true;
+    return 0;
+}
+
+int merge_chunk(emc_engine *e, int chunk, const double *view, bool all_on, bool do_model,
+                bool do_weight) {
+    const emc_engine::BrickChunk &bc = e->bricks[chunk];
+    const int nbx = (int)((e->size + emc::kBrick - 1) / emc::kBrick);
+    emc::launch_merge_bricks(e->stream, bc.units.p, bc.num_units, bc.list.p, e->rots.p, bc.r0, e->num_rot,
+                             view, e->psum_r.p, all_on, e->qvals.p, e->corr.p, e->mask.p, e->num_pix,
+                             e->size, (e->num_pix + 63) / 64, nbx, e->model2.p, e->weight.p, do_model,
+                             do_weight);
+    return 0;
+}
+
 // Work buffers. The posterior matrix R/P holds all frames x all orientations (the keep
 // rule needs each frame's global maximum before any normalisation); log W / view hold one
 // orientation chunk of C columns, C chosen to fit the remaining memory.
@@ -385,6 +466,7 @@ int ensure_work(emc_engine *e) {
     if (cmax < (size_t)emc::kRTile) return fail(EMC_E_NOMEM, "no device memory for an orientation chunk");
     const int C = (int)(cmax < Rp ? cmax : Rp);
     if (C != e->C || !e->logw.p) {
+        e->bricks_valid = false;
         e->tom_partial.release();
         if ((rc = e->logw.alloc(P * (size_t)C))) return rc;
         e->C = C;
@@ -475,6 +557,7 @@ int emc_create(int device, emc_engine **out) {
     if (const char *v = getenv("EMC_LIK_UNROLL")) e->lik_unroll = atoi(v);
     if (const char *v = getenv("EMC_TOMO_UNROLL")) e->tomo_unroll = atoi(v);
     if (const char *v = getenv("EMC_TOMO_SEQ")) e->tomo_seq = atoi(v);
+    if (const char *v = getenv("EMC_MERGE_MODE")) e->merge_mode = atoi(v);
 
     *out = e;
     return 0;
@@ -553,6 +636,7 @@ int emc_set_detector(emc_engine *e, int num_pix, const double *qvals, const doub
     e->lists_dirty = true;
     e->have_results = false;
     e->weight_all_valid = false;
+    e->bricks_valid = false;
     HIP_TRY(hipStreamSynchronize(e->stream));
     return 0;
 }
@@ -579,6 +663,8 @@ int emc_set_quats(emc_engine *e, int num_rot, const double *quats) {
     e->Rp = Rp;
     e->have_results = false;
     e->weight_all_valid = false;
+    e->bricks_valid = false;
+    e->refine = false;  // the orientation -> coarse cell map no longer applies
     return 0;
 }
 
@@ -594,7 +680,7 @@ int emc_set_model(emc_engine *e, long size, const double *model1) {
     emc::launch_fill(e->stream, e->weight.p, vol, 0.);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(e->stream));
-    if (size != e->size) e->weight_all_valid = false;
+    if (size != e->size) e->weight_all_valid = false, e->bricks_valid = false;
     e->size = size;
     return 0;
 }
@@ -624,6 +710,7 @@ int emc_set_frames_sparse(emc_engine *e, int num_data, const int *ones, const in
     e->lists_dirty = true;
     e->have_params = false;
     e->have_results = false;
+    e->refine = false;  // per-frame orientation sets belong to the previous frames
     return 0;
 }
 
@@ -652,6 +739,43 @@ int emc_set_frames_dense(emc_engine *e, int num_data, int num_pix, const int *fr
                                    po.data(), pm.data(), cm.data());
     if (rc) return rc;
     e->h_dense.assign(num_data, 1);
+    return 0;
+}
+
+int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const long *rel_off,
+                   const int *rel_quat) {
+    if (!e) return fail(EMC_E_ARG, "null engine");
+    if (!quat_mapping) {
+        e->refine = false;
+        return 0;
+    }
+    if (!e->num_rot || !e->num_data)
+        return fail(EMC_E_STATE, "emc_set_refine: set quaternions and frames first");
+    if (num_coarse <= 0 || !rel_off || !rel_quat) return fail(EMC_E_ARG, "emc_set_refine: bad argument");
+    HIP_TRY(hipSetDevice(e->device));
+    const int N = e->num_data, words = (num_coarse + 31) / 32;
+    std::vector<int> qm(e->Rp, 0);
+    for (int r = 0; r < e->num_rot; ++r) {
+        if (quat_mapping[r] < 0 || quat_mapping[r] >= num_coarse)
+            return fail(EMC_E_ARG, "emc_set_refine: quat_mapping[%d] = %d out of range", r, quat_mapping[r]);
+        qm[r] = quat_mapping[r];
+    }
+    if (rel_off[0] != 0) return fail(EMC_E_ARG, "emc_set_refine: rel_off[0] must be 0");
+    std::vector<uint32_t> bits((size_t)N * words, 0u);
+    for (int d = 0; d < N; ++d) {
+        if (rel_off[d + 1] < rel_off[d]) return fail(EMC_E_ARG, "emc_set_refine: rel_off not ascending");
+        for (long k = rel_off[d]; k < rel_off[d + 1]; ++k) {
+            const int c = rel_quat[k];
+            if (c < 0 || c >= num_coarse) return fail(EMC_E_ARG, "emc_set_refine: rel_quat %d out of range", c);
+            bits[(size_t)d * words + (c >> 5)] |= 1u << (c & 31);
+        }
+    }
+    int rc;
+    if ((rc = e->qmap.upload(qm.data(), qm.size(), e->stream))) return rc;
+    if ((rc = e->relbits.upload(bits.data(), bits.size(), e->stream))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->rel_words = words;
+    e->refine = true;
     return 0;
 }
 
@@ -795,12 +919,13 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     HIP_TRY(hipEventRecord(e->ev[4], s));
     emc::launch_frame_max(s, e->R.p, Rn, N, e->logwr.p, e->u.p, e->scale.p, use_scale,
                           e->scalars.p, e->blacklist.p, e->any_dense ? e->dense.p : nullptr,
-                          e->rbad.p, e->max_exp.p, e->rmax.p);
+                          e->rbad.p, e->refine ? e->qmap.p : nullptr,
+                          e->refine ? e->relbits.p : nullptr, e->rel_words, e->max_exp.p, e->rmax.p);
     HIP_TRY(hipEventRecord(e->ev[5], s));
     emc::launch_frame_norm(s, e->R.p, Rn, N, e->beta.p, e->blacklist.p, e->max_exp.p,
                            e->rmax.p, e->p_norm.p);
     HIP_TRY(hipEventRecord(e->ev[6], s));
-    emc::launch_posterior(s, e->R.p, Rn, N, e->quats.p, e->u.p, e->beta.p, e->sum_fact.p,
+    emc::launch_posterior(s, e->R.p, Rn, N, e->logwr.p, e->u.p, e->beta.p, e->sum_fact.p,
                           e->blacklist.p, e->max_exp.p, e->rmax.p, e->p_norm.p, e->scalars.p,
                           want_psum_d, e->likelihood.p, e->info.p, e->psum_d.p);
     emc::launch_psum_r(s, e->R.p, Rn, N, kFramesPerPsumPart, e->scale.p, opts->need_scaling,
@@ -816,6 +941,17 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
 
     // inter_weight does not depend on the data: when every orientation merges (all
     // psum_r > 0) it is the all-orientation sum, computed once and reused
+    // brick merge when the view is orientation-major (frame-partitioned tomogram, mixed path)
+    const bool bricks = e->merge_mode == 1 && (mp || e->tomo_tile_mode == 2);
+    if (bricks && !e->bricks_valid) {
+        HIP_TRY(hipEventRecord(e->ev[0], s));
+        if ((rc = build_bricks(e))) return rc;
+        HIP_TRY(hipEventRecord(e->ev[1], s));
+        HIP_TRY(hipEventSynchronize(e->ev[1]));
+        float a = 0.f;
+        HIP_TRY(hipEventElapsedTime(&a, e->ev[0], e->ev[1]));
+        ms_merge += a;
+    }
     int unmerged = 0;
     if ((rc = e->d_count.alloc(1))) return rc;
     emc::launch_count_unmerged(s, e->psum_r.p, Rn, e->d_count.p);
@@ -828,10 +964,19 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         if (!e->weight_all_valid) {
             if ((rc = e->weight_all.alloc(vol))) return rc;
             emc::launch_fill(s, e->weight_all.p, vol, 0.);
-            for (int r0 = 0; r0 < e->Rp; r0 += C) {
-                const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
-                emc::launch_merge(s, e->rots.p, r0, Cc, Rn, nullptr, e->psum_r.p, 1, e->qvals.p,
-                                  e->corr.p, e->mask.p, P, e->size, nullptr, e->weight_all.p, 0, 1);
+            if (bricks) {
+                // the brick kernel adds into e->weight: accumulate there, then keep a copy
+                emc::launch_fill(s, e->weight.p, vol, 0.);
+                for (size_t k = 0; k < e->bricks.size(); ++k)
+                    if ((rc = merge_chunk(e, (int)k, nullptr, true, false, true))) return rc;
+                HIP_TRY(hipMemcpyAsync(e->weight_all.p, e->weight.p, vol * sizeof(double),
+                                       hipMemcpyDeviceToDevice, s));
+            } else {
+                for (int r0 = 0; r0 < e->Rp; r0 += C) {
+                    const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
+                    emc::launch_merge(s, e->rots.p, r0, Cc, Rn, nullptr, e->psum_r.p, 1, e->qvals.p,
+                                      e->corr.p, e->mask.p, P, e->size, nullptr, e->weight_all.p, 0, 1);
+                }
             }
             HIP_TRY(hipGetLastError());
             e->weight_all_valid = true;
@@ -855,10 +1000,10 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
             // P^T K on matrix cores: P stored as fp16(P * 2^15) where P > 1e-6
             const double pscale = 32768.;
             emc::launch_posterior_fp16(s, e->R.p, N, r0, Cc, e->mp_ldd, pscale, e->p16.p);
-            if (emc::launch_mp_gemm(s, 1, 1, e->ktom.p, e->mp_ldd, e->p16.p, nullptr, e->mp_ldd,
-                                    (int)e->mp_ldd, (int)e->mp_rows_t, (int)round_up(Cc, emc::kGemmBN), P,
-                                    Cc, e->logw.p, (size_t)P * emc::kRTile, nullptr, e->psum_r.p + r0,
-                                    Rn - r0, 1. / pscale))
+            if (emc::launch_mp_gemm(s, 1, 1, e->p16.p, e->mp_ldd, e->ktom.p, nullptr, e->mp_ldd,
+                                    (int)e->mp_ldd, (int)round_up(Cc, emc::kGemmBM), (int)e->mp_rows_t, Cc,
+                                    P, e->logw.p, (size_t)P, nullptr, e->psum_r.p + r0, Rn - r0,
+                                    1. / pscale))
                 return fail(EMC_E_STATE, "tomogram product: shape does not fit the tiles");
         } else if (parts)
             emc::launch_tomogram_parts(s, e->R.p, N, r0, Cc, e->tom_off2.p, e->tom_parts,
@@ -872,13 +1017,20 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
             emc::launch_tomogram(s, e->R.p, N, r0, Cc, e->tom_off.p, e->tom_frame.p,
                                  e->tom_cnt.p, e->psum_r.p, Rn, P, e->logw.p, e->tomo_tiles);
         HIP_TRY(hipEventRecord(e->ev[1], s));
-        if (parts)
+        if (parts && bricks)
+            emc::launch_tomogram_reduce_rmajor(s, e->tom_partial.p, e->tom_parts, r0, Cc, e->psum_r.p,
+                                               Rn, P, e->logw.p);
+        else if (parts)
             emc::launch_tomogram_reduce(s, e->tom_partial.p, e->tom_parts, r0, Cc, e->psum_r.p,
                                         Rn, P, e->logw.p);
         HIP_TRY(hipEventRecord(e->ev[2], s));
-        emc::launch_merge(s, e->rots.p, r0, Cc, Rn, e->logw.p, e->psum_r.p, 0, e->qvals.p,
-                          e->corr.p, e->mask.p, P, e->size, e->model2.p, e->weight.p, 1,
-                          cached_weight ? 0 : 1);
+        if (bricks) {
+            if ((rc = merge_chunk(e, r0 / C, e->logw.p, false, true, !cached_weight))) return rc;
+        } else {
+            emc::launch_merge(s, e->rots.p, r0, Cc, Rn, e->logw.p, e->psum_r.p, 0, e->qvals.p,
+                              e->corr.p, e->mask.p, P, e->size, e->model2.p, e->weight.p, 1,
+                              cached_weight ? 0 : 1);
+        }
         HIP_TRY(hipEventRecord(e->ev[8], s));
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventSynchronize(e->ev[8]));

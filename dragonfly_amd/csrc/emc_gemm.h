@@ -1,0 +1,172 @@
+// Matrix-core product tile shared by the mixed-precision EMC path and the orientation
+// proposal network: C[m][n] = sum_k A[m][k] * (B0[n][k] (+ B1[n][k])) with 16-bit operands
+// (FP16 or BF16), FP32 accumulation, on gfx950 v_mfma_f32_32x32x16_{f16,bf16}.
+//
+// Block tile kGemmBM x kGemmBN, 8 waves as 4 (m) x 2 (n), wave tile 64 x 128 = 2 x 4 MFMA
+// tiles of 32 x 32. K advances kGemmBK elements per step; the next step's operands are
+// loaded into registers while the current step's MFMAs read LDS. LDS rows are padded by 8
+// elements (80-byte stride) so the 16-byte fragment reads of 32 consecutive rows hit
+// distinct banks.
+//
+// MFMA operand layout (checked on the GPU): lane l supplies A[l % 32][8 (l / 32) + i] and
+// B[l % 32][8 (l / 32) + i], i = 0..7; accumulator element v of lane l is
+// C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+//
+// Shapes: rows of A and B padded to the block tile, K to kGemmBK, leading dimensions
+// multiples of 8 elements; the padding holds finite values (zeros).
+#pragma once
+
+#include "emc_common.h"
+
+namespace emc {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<_Float16> {
+    typedef _Float16 vec8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ floatx16 run(vec8 a, vec8 b, floatx16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Mfma<__bf16> {
+    typedef __bf16 vec8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ floatx16 run(vec8 a, vec8 b, floatx16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+constexpr int kGemmLdsPad = 8;
+constexpr int kGemmRow = kGemmBK + kGemmLdsPad;
+
+template <typename T>
+constexpr size_t gemm_lds_elems(int nb) {
+    return (size_t)(kGemmBM + nb * kGemmBN) * kGemmRow;
+}
+
+// Block -> output tile. Consecutive blocks go round-robin to the 8 XCDs; each XCD gets a
+// contiguous range of tiles, walked in panels of kGemmGroupM m-tiles per n-tile, so the
+// blocks resident on one XCD share A and B k-slices in its L2. Launch with
+// gemm_grid(m_tiles, n_tiles) blocks.
+struct GemmTile {
+    size_t m0, n0;
+    bool valid;
+};
+
+__host__ __device__ inline int gemm_grid(int m_tiles, int n_tiles) {
+    const int tiles = m_tiles * n_tiles;
+    return (tiles + kNumXcd - 1) / kNumXcd * kNumXcd;
+}
+
+__device__ __forceinline__ GemmTile gemm_block_tile(int m_tiles, int n_tiles) {
+    const int tiles = m_tiles * n_tiles;
+    const int per_xcd = (tiles + kNumXcd - 1) / kNumXcd;
+    const int xcd = blockIdx.x % kNumXcd, seq = blockIdx.x / kNumXcd;
+    const int tile = xcd * per_xcd + seq;
+    GemmTile t{0, 0, seq < per_xcd && tile < tiles};
+    if (!t.valid) return t;
+    const int panel_tiles = kGemmGroupM * n_tiles;
+    const int panel = tile / panel_tiles, in_panel = tile % panel_tiles;
+    const int group_m = min(kGemmGroupM, m_tiles - panel * kGemmGroupM);
+    t.m0 = (size_t)(panel * kGemmGroupM + in_panel % group_m) * kGemmBM;
+    t.n0 = (size_t)(in_panel / group_m) * kGemmBN;
+    return t;
+}
+
+// Accumulates the block's tile into acc (zeroed here). smem holds gemm_lds_elems<T>(NB).
+template <typename T, int NB>
+__device__ __forceinline__ void gemm_mainloop(const T *__restrict__ A, size_t lda,
+                                              const T *__restrict__ B0, const T *__restrict__ B1,
+                                              size_t ldb, int K, size_t m0, size_t n0, T *smem,
+                                              floatx16 (&acc)[2][4]) {
+    typedef typename Mfma<T>::vec8 vec8;
+    T *sA = smem;
+    T *sB = smem + kGemmBM * kGemmRow;  // NB consecutive [kGemmBN][kGemmRow] tiles
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 3, wn = wave >> 2;
+    // global -> register staging: each thread moves kLoads 16-byte chunks per operand per
+    // step (rows tid / 4 + 128 j, chunk tid % 4 of the 64-byte row segment)
+    constexpr int kLoads = kGemmBM * kGemmBK / 8 / kGemmThreads;
+    static_assert(kGemmBM == kGemmBN, "square block tile");
+    const int lrow = tid >> 2, lchunk = (tid & 3) * 8;
+    vec8 ra[kLoads], rb[NB][kLoads];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < kLoads; ++j) {
+            const size_t row = lrow + (kGemmThreads / 4) * j;
+            ra[j] = *reinterpret_cast<const vec8 *>(A + (m0 + row) * lda + k0 + lchunk);
+            rb[0][j] = *reinterpret_cast<const vec8 *>(B0 + (n0 + row) * ldb + k0 + lchunk);
+            if (NB == 2)
+                rb[NB - 1][j] = *reinterpret_cast<const vec8 *>(B1 + (n0 + row) * ldb + k0 + lchunk);
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int j = 0; j < kLoads; ++j) {
+            const int row = lrow + (kGemmThreads / 4) * j;
+            *reinterpret_cast<vec8 *>(sA + row * kGemmRow + lchunk) = ra[j];
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+                *reinterpret_cast<vec8 *>(sB + (b * kGemmBN + row) * kGemmRow + lchunk) = rb[b][j];
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+
+    const int frag_row = lane & 31, frag_k = (lane >> 5) * 8;
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += kGemmBK) {
+        __syncthreads();  // previous step's fragment reads are done
+        store();
+        __syncthreads();
+        if (k0 + kGemmBK < K) load(k0 + kGemmBK);
+#pragma unroll
+        for (int kk = 0; kk < kGemmBK; kk += 16) {
+            vec8 fa[2], fb[NB][4];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                fa[i] = *reinterpret_cast<const vec8 *>(sA + (wm * 64 + i * 32 + frag_row) * kGemmRow + kk + frag_k);
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    fb[b][j] = *reinterpret_cast<const vec8 *>(
+                        sB + (b * kGemmBN + wn * 128 + j * 32 + frag_row) * kGemmRow + kk + frag_k);
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::run(fa[i], fb[b][j], acc[i][j]);
+        }
+    }
+}
+
+// Calls f(m, n, value) for every accumulator element of the block's tile.
+template <typename F>
+__device__ __forceinline__ void gemm_for_each(size_t m0, size_t n0, const floatx16 (&acc)[2][4], F f) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 3, wn = wave >> 2;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t n = n0 + wn * 128 + j * 32 + (lane & 31);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const size_t m = m0 + wm * 64 + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                f(m, n, acc[i][j][v]);
+            }
+        }
+}
+
+}  // namespace emc

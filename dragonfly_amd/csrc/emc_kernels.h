@@ -21,12 +21,13 @@ void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, co
 void launch_frame_max(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *logwr, const double *u, const double *scale, int use_scale,
                       const double *scalars, const uint8_t *blacklist, const uint8_t *dense,
-                      const int *rbad, double *max_exp, int *rmax);
+                      const int *rbad, const int *qmap, const uint32_t *relbits, int words,
+                      double *max_exp, int *rmax);
 void launch_frame_norm(hipStream_t s, const double *R, int num_rot, int num_data,
                        const double *beta, const uint8_t *blacklist, const double *max_exp,
                        const int *rmax, double *p_norm);
 void launch_posterior(hipStream_t s, double *R, int num_rot, int num_data,
-                      const double *quats, const double *u, const double *beta,
+                      const double *logwr, const double *u, const double *beta,
                       const double *sum_fact, const uint8_t *blacklist, const double *max_exp,
                       const int *rmax, const double *p_norm, const double *scalars,
                       int want_psum_d, double *likelihood, double *info, double *psum_d);
@@ -46,6 +47,17 @@ void launch_tomogram_parts(hipStream_t s, const double *P, int num_data, int r0,
                            double *view, int width32, int unroll, int seq);
 void launch_tomogram_reduce(hipStream_t s, const double *partial, int nparts, int r0, int C,
                             const double *psum_r, int num_rot, int num_pix, double *view);
+void launch_tomogram_reduce_rmajor(hipStream_t s, const double *partial, int nparts, int r0, int C,
+                                   const double *psum_r, int num_rot, int num_pix, double *view);
+// ---- brick merge (emc_merge.hip) ----
+void launch_brick_items(hipStream_t s, const double *rots, int r0, int Cc, int num_rot,
+                        const double *qvals, const uint8_t *mask, int num_pix, long size, int npatch,
+                        int nbx, int pass, int *counts, const long *off, int *fill, int *list);
+void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const int *list,
+                         const double *rots, int r0, int num_rot, const double *view,
+                         const double *psum_r, int all_on, const double *qvals, const double *corr,
+                         const uint8_t *mask, int num_pix, long size, int npatch, int nbx,
+                         double *model2, double *weight, int do_model, int do_weight);
 void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot, const double *view,
                   const double *psum_r, int all_on, const double *qvals, const double *corr,
                   const uint8_t *mask, int num_pix, long size, double *model2, double *weight,
@@ -71,7 +83,7 @@ void launch_normalize_scale(hipStream_t s, double *scale, const uint8_t *blackli
 size_t gemm_lds_bytes(int nb);
 int launch_mp_gemm(hipStream_t s, int nb, int epi, const _Float16 *A, size_t lda, const _Float16 *B0,
                    const _Float16 *B1, size_t ldb, int K, int Mp, int Np, int M_valid, int N_valid,
-                   double *out, size_t ts, const double *row_add, const double *col_div,
+                   double *out, size_t ts, const double *row_add, const double *row_div,
                    int div_valid, double scale);
 void launch_build_klik(hipStream_t s, const long *off, const int *pix, const double *cnt, int num_data,
                        size_t ldp, _Float16 *klik);

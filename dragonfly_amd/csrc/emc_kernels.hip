@@ -198,10 +198,15 @@ __global__ __launch_bounds__(kBlock) void frame_max_kernel(
     const double *__restrict__ u, const double *__restrict__ scale, int use_scale,
     const double *__restrict__ scalars, const uint8_t *__restrict__ blacklist,
     const uint8_t *__restrict__ dense, const int *__restrict__ rbad,
+    const int *__restrict__ qmap, const uint32_t *__restrict__ relbits, int words,
     double *__restrict__ max_exp, int *__restrict__ rmax) {
     __shared__ double s_v[kBlock];
     __shared__ int s_i[kBlock];
     const int d = blockIdx.x;
+    // refinement (legacy src/max_emc.c:264-274): frame d only takes orientations whose coarse
+    // cell qmap[r] is set in its bit row; the others are not evaluated (R = -inf: never the
+    // maximum, never kept)
+    const uint32_t *bits = relbits ? relbits + (size_t)d * words : nullptr;
     // dense frames sum K log W over every mask < 1 pixel (max_prob.c:173-177): an orientation
     // with log W = -inf or NaN on such a pixel gives 0 * -inf = NaN and is never kept
     const bool dense_frame = dense != nullptr && dense[d];
@@ -218,7 +223,12 @@ __global__ __launch_bounds__(kBlock) void frame_max_kernel(
     int bi = -1;
     for (int r = threadIdx.x; r < num_rot; r += kBlock) {
         double &x = row[(size_t)(r >> 6) * tsR + (r & 63)];
-        const double v = (dense_frame && rbad[r]) ? -INFINITY : (logwr[r] - u[r] * phi) + x;
+        bool skip = dense_frame && rbad[r];
+        if (bits) {
+            const int c = qmap[r];
+            skip |= !((bits[c >> 5] >> (c & 31)) & 1u);
+        }
+        const double v = skip ? -INFINITY : (logwr[r] - u[r] * phi) + x;
         x = v;
         if (v > best) {
             best = v;
@@ -279,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void frame_norm_kernel(
 // P = exp(beta (R - M)) / p_norm on the kept set, 0 elsewhere (written in place);
 // likelihood += P (R - sum_fact); psum_d += P u_r rescale; info += P log(P / w_r) for P >= 1e-6.
 __global__ __launch_bounds__(kBlock) void posterior_kernel(
-    double *__restrict__ R, size_t tsR, int num_rot, const double *__restrict__ quats,
+    double *__restrict__ R, size_t tsR, int num_rot, const double *__restrict__ logwr,
     const double *__restrict__ u, const double *__restrict__ beta,
     const double *__restrict__ sum_fact, const uint8_t *__restrict__ blacklist,
     const double *__restrict__ max_exp, const int *__restrict__ rmax,
@@ -299,17 +309,20 @@ __global__ __launch_bounds__(kBlock) void posterior_kernel(
         return;
     }
     const double b = beta[d], M = max_exp[d], c = kPdiffThresh / b, pn = p_norm[d];
-    const double sf = sum_fact[d], rescale = scalars[0];
+    const double sf = sum_fact[d], rescale = scalars[0], log_pn = log(pn);
     double lk = 0., inf = 0., ps = 0.;
     for (int r = threadIdx.x; r < num_rot; r += kBlock) {
         double &x = row[(size_t)(r >> 6) * tsR + (r & 63)];
         const double v = x;
         double prob = 0.;
         if (kept(v, r, rf, M, c)) {
-            prob = exp(b * (v - M)) / pn;
+            const double e = b * (v - M);
+            prob = exp(e) / pn;
             lk += prob * (v - sf);
             ps += prob * u[r] * rescale;
-            if (!(prob < kProbMin)) inf += prob * log(prob / quats[5 * (size_t)r + 4] * 1);
+            // log(P / w_r) (max_update.c:127-130) taken as e - log p_norm - log w_r: the same
+            // value to rounding, without a logarithm per pair
+            if (!(prob < kProbMin)) inf += prob * ((e - log_pn) - logwr[r]);
         }
         x = prob;
     }
@@ -498,6 +511,32 @@ __global__ __launch_bounds__(kBlock) void tomogram_part32_kernel(
         const size_t tsW = (size_t)num_pix * kRTile, tsPart = tsW * (C / kRTile);
         partial[part * tsPart + (size_t)(c0 >> 6) * tsW + (size_t)t * kRTile + (c0 & 63) + lane] = acc;
     }
+}
+
+// Same sum, written orientation-major (view[c][t], c chunk-local) through an LDS transpose:
+// block = 64 pixels x 64 orientations.
+__global__ __launch_bounds__(kBlock) void tomogram_reduce_rmajor_kernel(
+    const double *__restrict__ partial, int nparts, int r0, int C, int num_pix,
+    const double *__restrict__ psum_r, int num_rot, double *__restrict__ view) {
+    __shared__ double s[64][kRTile + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t0 = blockIdx.x * 64, c0 = blockIdx.y * kRTile;
+    const size_t tsW = (size_t)num_pix * kRTile, tsPart = tsW * (C / kRTile);
+    const int r = r0 + c0 + lane;
+    const double ps = (r < num_rot) ? psum_r[r] : 0.;
+    for (int k = wave; k < 64; k += kWavesPerBlock) {
+        const int t = t0 + k;
+        if (t >= num_pix) break;
+        const size_t i = (size_t)(c0 >> 6) * tsW + (size_t)t * kRTile + lane;
+        double acc = 0.;
+        for (int x = 0; x < nparts; ++x) acc += partial[x * tsPart + i];
+        s[k][lane] = (ps > 0.) ? acc / ps : acc;
+    }
+    __syncthreads();
+    const int t = t0 + lane;
+    if (t < num_pix)
+        for (int k = wave; k < kRTile; k += kWavesPerBlock)
+            view[(size_t)(c0 + k) * num_pix + t] = s[lane][k];
 }
 
 __global__ void tomogram_reduce_kernel(const double *__restrict__ partial, int nparts, int r0,
@@ -925,9 +964,11 @@ void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, co
 void launch_frame_max(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *logwr, const double *u, const double *scale, int use_scale,
                       const double *scalars, const uint8_t *blacklist, const uint8_t *dense,
-                      const int *rbad, double *max_exp, int *rmax) {
+                      const int *rbad, const int *qmap, const uint32_t *relbits, int words,
+                      double *max_exp, int *rmax) {
     frame_max_kernel<<<num_data, kBlock, 0, s>>>(R, (size_t)num_data * kRTile, num_rot, logwr, u, scale, use_scale,
-                                                 scalars, blacklist, dense, rbad, max_exp, rmax);
+                                                 scalars, blacklist, dense, rbad, qmap, relbits, words,
+                                                 max_exp, rmax);
 }
 
 void launch_frame_norm(hipStream_t s, const double *R, int num_rot, int num_data,
@@ -938,11 +979,11 @@ void launch_frame_norm(hipStream_t s, const double *R, int num_rot, int num_data
 }
 
 void launch_posterior(hipStream_t s, double *R, int num_rot, int num_data,
-                      const double *quats, const double *u, const double *beta,
+                      const double *logwr, const double *u, const double *beta,
                       const double *sum_fact, const uint8_t *blacklist, const double *max_exp,
                       const int *rmax, const double *p_norm, const double *scalars,
                       int want_psum_d, double *likelihood, double *info, double *psum_d) {
-    posterior_kernel<<<num_data, kBlock, 0, s>>>(R, (size_t)num_data * kRTile, num_rot, quats, u, beta, sum_fact,
+    posterior_kernel<<<num_data, kBlock, 0, s>>>(R, (size_t)num_data * kRTile, num_rot, logwr, u, beta, sum_fact,
                                                  blacklist, max_exp, rmax, p_norm, scalars,
                                                  want_psum_d, likelihood, info, psum_d);
 }
@@ -1003,6 +1044,13 @@ void launch_tomogram_parts(hipStream_t s, const double *P, int num_data, int r0,
         }
 #undef EMC_TP
     }
+}
+
+void launch_tomogram_reduce_rmajor(hipStream_t s, const double *partial, int nparts, int r0, int C,
+                                   const double *psum_r, int num_rot, int num_pix, double *view) {
+    dim3 grid(cdiv(num_pix, 64), C / kRTile);
+    tomogram_reduce_rmajor_kernel<<<grid, kBlock, 0, s>>>(partial, nparts, r0, C, num_pix, psum_r, num_rot,
+                                                          view);
 }
 
 void launch_tomogram_reduce(hipStream_t s, const double *partial, int nparts, int r0, int C,

@@ -1,0 +1,166 @@
+// Brick merge: slice_merge3d of every orientation of a chunk (merge_tomogram,
+// dragonfly/src/max_update.c:18-40; model.c:171-244) accumulated on chip.
+//
+// The volume is cut into bricks of kBrick^3 voxels. A work item is (orientation c of the
+// chunk, patch p of 64 consecutive engine pixels); since engine pixels follow a Morton curve,
+// a patch is a compact detector tile and its trilinear cells for one orientation fill a small
+// box that meets 1-8 bricks. Which bricks an item meets depends only on the rotations and the
+// detector, so the per-brick item lists are built once (brick_items_kernel, two passes:
+// count, then fill) and reused by every iteration. Each block then owns one brick (or a
+// range of a crowded brick's items), adds every corner contribution that falls inside the
+// brick into an LDS copy of it, and adds the brick to HBM once: one global atomic per touched
+// voxel per block instead of one per (orientation, pixel, corner).
+//
+// Values are those of the reference's merge (same cell, same corner weights, w = view / corr
+// for mask <= 1 pixels of orientations with psum_r > 0); only the summation order differs.
+// The view is orientation-major: view[c][t], c the chunk-local orientation.
+#include <climits>
+
+#include "emc_common.h"
+#include "emc_geometry.h"
+#include "emc_kernels.h"
+
+namespace emc {
+
+namespace {
+
+__device__ __forceinline__ int wmin(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ __forceinline__ int wmax(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+}  // namespace
+
+// Pass 0 counts items per brick (counts[b] += 1); pass 1 writes item ids into
+// list[off[b] + fill[b]++]. One wavefront per item; the wave's lanes are the patch's pixels.
+__global__ __launch_bounds__(kBlock) void brick_items_kernel(
+    const double *__restrict__ rots, int r0, int Cc, int num_rot, const double *__restrict__ qvals,
+    const uint8_t *__restrict__ mask, int num_pix, long size, int npatch, int nbx, int pass,
+    int *__restrict__ counts, const long *__restrict__ off, int *__restrict__ fill,
+    int *__restrict__ list) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long item = (long)blockIdx.x * kWavesPerBlock + wave;
+    if (item >= (long)Cc * npatch) return;
+    const int c = (int)(item / npatch), p = (int)(item % npatch);
+    const int r = r0 + c;
+    if (r >= num_rot) return;
+    const int t = p * 64 + lane;
+    Cell cell;
+    bool ok = false;
+    if (t < num_pix && mask[t] <= 1)
+        ok = locate(rots + 9 * (size_t)r, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1],
+                    qvals[3 * (size_t)t + 2], (double)(size / 2), size, cell);
+    const int xmax = wmax(ok ? (int)cell.x : -1);
+    if (xmax < 0) return;
+    const int xmin = wmin(ok ? (int)cell.x : INT_MAX);
+    const int ymin = wmin(ok ? (int)cell.y : INT_MAX), ymax = wmax(ok ? (int)cell.y : -1);
+    const int zmin = wmin(ok ? (int)cell.z : INT_MAX), zmax = wmax(ok ? (int)cell.z : -1);
+    // corners reach cell + 1 (never past size - 1: cells stop at size - 2)
+    const int bx0 = xmin / kBrick, bx1 = (xmax + 1) / kBrick;
+    const int by0 = ymin / kBrick, by1 = (ymax + 1) / kBrick;
+    const int bz0 = zmin / kBrick, bz1 = (zmax + 1) / kBrick;
+    const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1;
+    const int nb = (bx1 - bx0 + 1) * ny * nz;
+    for (int i = lane; i < nb; i += 64) {
+        const int b = ((bx0 + i / (ny * nz)) * nbx + by0 + (i / nz) % ny) * nbx + bz0 + i % nz;
+        if (pass == 0) {
+            atomicAdd(&counts[b], 1);
+        } else {
+            const int at = atomicAdd(&fill[b], 1);
+            list[off[b] + at] = (int)item;
+        }
+    }
+}
+
+// One block per work unit (brick, item range). kModel adds w * f into model2, kWeight adds f
+// into weight.
+template <bool kModel, bool kWeight>
+__global__ __launch_bounds__(kBlock) void merge_brick_kernel(
+    const int4 *__restrict__ units, const int *__restrict__ list, const double *__restrict__ rots,
+    int r0, int num_rot, const double *__restrict__ view, const double *__restrict__ psum_r,
+    int all_on, const double *__restrict__ qvals, const double *__restrict__ corr,
+    const uint8_t *__restrict__ mask, int num_pix, long size, int npatch, int nbx,
+    double *__restrict__ model2, double *__restrict__ weight) {
+    constexpr int kVox = kBrick * kBrick * kBrick;
+    __shared__ double s_m[kModel ? kVox : 1];
+    __shared__ double s_w[kWeight ? kVox : 1];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int4 u = units[blockIdx.x];
+    const int b = u.x;
+    const int X0 = (b / (nbx * nbx)) * kBrick, Y0 = ((b / nbx) % nbx) * kBrick, Z0 = (b % nbx) * kBrick;
+    for (int i = tid; i < kVox; i += kBlock) {
+        if (kModel) s_m[i] = 0.;
+        if (kWeight) s_w[i] = 0.;
+    }
+    __syncthreads();
+    const double center = (double)(size / 2);
+    for (int k = u.y + wave; k < u.z; k += kWavesPerBlock) {
+        const int item = list[k];
+        const int c = item / npatch, p = item % npatch;
+        const int r = r0 + c;
+        if (!all_on && !(psum_r[r] > 0.)) continue;
+        const int t = p * 64 + lane;
+        if (t >= num_pix || mask[t] > 1) continue;
+        Cell cell;
+        if (!locate(rots + 9 * (size_t)r, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1],
+                    qvals[3 * (size_t)t + 2], center, size, cell))
+            continue;
+        const int lx = (int)cell.x - X0, ly = (int)cell.y - Y0, lz = (int)cell.z - Z0;
+        if (lx < -1 || lx >= kBrick || ly < -1 || ly >= kBrick || lz < -1 || lz >= kBrick) continue;
+        double w = 0.;
+        if (kModel) w = view[(size_t)c * num_pix + t] / corr[t];
+        const double cx = 1. - cell.fx, cy = 1. - cell.fy, cz = 1. - cell.fz;
+        const double fx = cell.fx, fy = cell.fy, fz = cell.fz;
+        const double f[8] = {cx * cy * cz, cx * cy * fz, cx * fy * cz, cx * fy * fz,
+                             fx * cy * cz, fx * cy * fz, fx * fy * cz, fx * fy * fz};
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int ix = lx + (q >> 2), iy = ly + ((q >> 1) & 1), iz = lz + (q & 1);
+            if (ix < 0 || ix >= kBrick || iy < 0 || iy >= kBrick || iz < 0 || iz >= kBrick) continue;
+            const int li = (ix * kBrick + iy) * kBrick + iz;
+            if (kWeight) unsafeAtomicAdd(&s_w[li], f[q]);
+            if (kModel) unsafeAtomicAdd(&s_m[li], f[q] * w);
+        }
+    }
+    __syncthreads();
+    const long s2 = size * size;
+    for (int i = tid; i < kVox; i += kBlock) {
+        const int x = X0 + i / (kBrick * kBrick), y = Y0 + (i / kBrick) % kBrick, z = Z0 + i % kBrick;
+        if (x >= size || y >= size || z >= size) continue;
+        const long g = x * s2 + (long)y * size + z;
+        if (kModel && s_m[i] != 0.) unsafeAtomicAdd(&model2[g], s_m[i]);
+        if (kWeight && s_w[i] != 0.) unsafeAtomicAdd(&weight[g], s_w[i]);
+    }
+}
+
+void launch_brick_items(hipStream_t s, const double *rots, int r0, int Cc, int num_rot,
+                        const double *qvals, const uint8_t *mask, int num_pix, long size, int npatch,
+                        int nbx, int pass, int *counts, const long *off, int *fill, int *list) {
+    const long items = (long)Cc * npatch;
+    const unsigned grid = (unsigned)((items + kWavesPerBlock - 1) / kWavesPerBlock);
+    brick_items_kernel<<<grid, kBlock, 0, s>>>(rots, r0, Cc, num_rot, qvals, mask, num_pix, size, npatch,
+                                               nbx, pass, counts, off, fill, list);
+}
+
+void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const int *list,
+                         const double *rots, int r0, int num_rot, const double *view,
+                         const double *psum_r, int all_on, const double *qvals, const double *corr,
+                         const uint8_t *mask, int num_pix, long size, int npatch, int nbx,
+                         double *model2, double *weight, int do_model, int do_weight) {
+    if (num_units <= 0) return;
+#define EMC_MB(M, W)                                                                             \
+    merge_brick_kernel<M, W><<<num_units, kBlock, 0, s>>>(units, list, rots, r0, num_rot, view, psum_r, \
+                                                          all_on, qvals, corr, mask, num_pix, size, \
+                                                          npatch, nbx, model2, weight)
+    if (do_model && do_weight) EMC_MB(true, true);
+    else if (do_model) EMC_MB(true, false);
+    else if (do_weight) EMC_MB(false, true);
+#undef EMC_MB
+}
+
+}  // namespace emc

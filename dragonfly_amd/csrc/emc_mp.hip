@@ -2,7 +2,7 @@
 //
 // For dense frames the two photon contractions of an iteration are dense products:
 //   likelihood  G[d][r]    = sum_t K[d][t] log W[r][t]         (mask < 1 pixels)
-//   tomogram    view[t][r] = sum_d K[d][t] P[d][r]              (mask < 2, P > 1e-6)
+//   tomogram    view[r][t] = sum_d P[d][r] K[d][t]              (mask < 2, P > 1e-6)
 // They run here as FP16 x FP16 products on MFMA (v_mfma_f32_32x32x16_f16) with FP32
 // accumulation; everything else of the iteration (per-frame maximum, keep rule, p_norm,
 // posterior, likelihood and information sums, psum_r / psum_d, merge, model update) stays in
@@ -24,155 +24,48 @@
 //   lw_hi, lw_lo [Np_c][ldp]  centred log W of one orientation chunk
 //   p16   [Np_c][ldd]   scaled posterior of one orientation chunk
 // with ldp = num_pix rounded up to kGemmBK and ldd = num_data rounded up to kGemmBK; row
-// counts are rounded up to the GEMM tile. Outputs are the FP64 tile-major buffers of the
-// plain path (R and view), so the FP64 kernels read them unchanged.
+// counts are rounded up to the GEMM tile. Outputs are the FP64 buffers of the plain path: R
+// tile-major (frame_max_kernel and the later per-frame passes read it unchanged) and the view
+// orientation-major, as the brick merge reads it.
 #include <cfloat>
 
 #include "emc_common.h"
+#include "emc_gemm.h"
 #include "emc_geometry.h"
 #include "emc_kernels.h"
 
 namespace emc {
 
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // ---- product kernel ----------------------------------------------------------------------
-// C[m][n] = sum_k A[m][k] * (B0[n][k] + B1[n][k])   (B1 only when NB == 2)
-// Block tile kGemmBM x kGemmBN, 8 waves as 4 (m) x 2 (n), wave tile 64 x 128 = 2 x 4 MFMA
-// tiles of 32 x 32. K advances kGemmBK halves per step; the next step's operands are loaded
-// into registers while the current step's MFMAs read LDS. LDS rows are padded by 8 halves
-// (80-byte stride) so the 16-byte fragment reads of 32 consecutive rows hit distinct banks.
-//
-// MFMA operand layout (v_mfma_f32_32x32x16_f16): lane l supplies A[l % 32][8 (l / 32) + i]
-// and B[l % 32][8 (l / 32) + i], i = 0..7; accumulator element v of lane l is
-// C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
-//
-// Epilogue: element (m, n) with m < M_valid, n < N_valid goes to out[(n / 64) * ts + m * 64 +
-// n % 64] (the tile-major FP64 layout), as
-//   EPI 0: acc + row_add[m]                       (likelihood: + sum_t K_dt c_t)
-//   EPI 1: acc * scale / col_div[n] if n < div_valid and col_div[n] > 0, else acc * scale
-//          (tomogram; columns at or past div_valid are padding orientations)
-constexpr int kGemmLdsPad = 8;
-constexpr int kGemmRow = kGemmBK + kGemmLdsPad;
-
+// C[m][n] = sum_k A[m][k] * (B0[n][k] + B1[n][k]) (B1 only when NB == 2) with the shared
+// matrix-core tile (emc_gemm.h). Epilogue, for elements (m, n) with m < M_valid, n < N_valid:
+//   EPI 0: out[(n / 64) * ts + m * 64 + n % 64] = acc + row_add[m]   (tile-major; likelihood
+//          of frame m, orientation n, plus sum_t K_mt c_t)
+//   EPI 1: out[m * ts + n] = acc * scale / row_div[m] if m < div_valid and row_div[m] > 0,
+//          else acc * scale   (row-major; tomogram of orientation m at pixel n; rows at or past
+//          div_valid are padding orientations)
 template <int NB, int EPI>
 __global__ __launch_bounds__(kGemmThreads) void mp_gemm_kernel(
     const _Float16 *__restrict__ A, size_t lda, const _Float16 *__restrict__ B0,
     const _Float16 *__restrict__ B1, size_t ldb, int K, int m_tiles, int n_tiles, int M_valid,
     int N_valid, double *__restrict__ out, size_t ts, const double *__restrict__ row_add,
-    const double *__restrict__ col_div, int div_valid, double scale) {
+    const double *__restrict__ row_div, int div_valid, double scale) {
     extern __shared__ _Float16 smem[];
-    _Float16 *sA = smem;
-    _Float16 *sB = smem + kGemmBM * kGemmRow;  // NB consecutive [kGemmBN][kGemmRow] tiles
-
-    // block -> tile: consecutive blocks go round-robin to the 8 XCDs; give each XCD a
-    // contiguous range of tiles, walked in panels of kGemmGroupM m-tiles per n-tile so that
-    // the blocks resident on one XCD share A and B k-slices in its L2.
-    const int tiles = m_tiles * n_tiles;
-    const int per_xcd = (tiles + kNumXcd - 1) / kNumXcd;
-    const int xcd = blockIdx.x % kNumXcd, seq = blockIdx.x / kNumXcd;
-    const int tile = xcd * per_xcd + seq;
-    if (seq >= per_xcd || tile >= tiles) return;
-    const int panel_tiles = kGemmGroupM * n_tiles;
-    const int panel = tile / panel_tiles, in_panel = tile % panel_tiles;
-    const int group_m = min(kGemmGroupM, m_tiles - panel * kGemmGroupM);
-    const int mt = panel * kGemmGroupM + in_panel % group_m;
-    const int nt = in_panel / group_m;
-    const size_t m0 = (size_t)mt * kGemmBM, n0 = (size_t)nt * kGemmBN;
-
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 3, wn = wave >> 2;
-
-    // global -> register staging: each thread moves kGemmLoads 16-byte chunks per operand
-    // per step (rows tid / 4 + 128 j, chunk tid % 4 of the 64-byte row segment)
-    constexpr int kLoads = kGemmBM * kGemmBK / 8 / kGemmThreads;
-    static_assert(kGemmBM == kGemmBN, "square block tile");
-    const int lrow = tid >> 2, lchunk = (tid & 3) * 8;
-    half8 ra[kLoads], rb[NB][kLoads];
-    auto load = [&](int k0) {
-#pragma unroll
-        for (int j = 0; j < kLoads; ++j) {
-            const size_t row = lrow + (kGemmThreads / 4) * j;
-            ra[j] = *reinterpret_cast<const half8 *>(A + (m0 + row) * lda + k0 + lchunk);
-            rb[0][j] = *reinterpret_cast<const half8 *>(B0 + (n0 + row) * ldb + k0 + lchunk);
-            if (NB == 2)
-                rb[NB - 1][j] = *reinterpret_cast<const half8 *>(B1 + (n0 + row) * ldb + k0 + lchunk);
-        }
-    };
-    auto store = [&]() {
-#pragma unroll
-        for (int j = 0; j < kLoads; ++j) {
-            const int row = lrow + (kGemmThreads / 4) * j;
-            *reinterpret_cast<half8 *>(sA + row * kGemmRow + lchunk) = ra[j];
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-                *reinterpret_cast<half8 *>(sB + (b * kGemmBN + row) * kGemmRow + lchunk) = rb[b][j];
-        }
-    };
-
+    const GemmTile tile = gemm_block_tile(m_tiles, n_tiles);
+    if (!tile.valid) return;
     floatx16 acc[2][4];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
-
-    const int frag_row = lane & 31, frag_k = (lane >> 5) * 8;
-    load(0);
-    for (int k0 = 0; k0 < K; k0 += kGemmBK) {
-        __syncthreads();  // previous step's fragment reads are done
-        store();
-        __syncthreads();
-        if (k0 + kGemmBK < K) load(k0 + kGemmBK);
-#pragma unroll
-        for (int kk = 0; kk < kGemmBK; kk += 16) {
-            half8 fa[2], fb[NB][4];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-                fa[i] = *reinterpret_cast<const half8 *>(
-                    sA + (wm * 64 + i * 32 + frag_row) * kGemmRow + kk + frag_k);
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    fb[b][j] = *reinterpret_cast<const half8 *>(
-                        sB + (b * kGemmBN + wn * 128 + j * 32 + frag_row) * kGemmRow + kk + frag_k);
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[i], fb[b][j], acc[i][j], 0, 0, 0);
+    gemm_mainloop<_Float16, NB>(A, lda, B0, B1, ldb, K, tile.m0, tile.n0, smem, acc);
+    gemm_for_each(tile.m0, tile.n0, acc, [&](size_t m, size_t n, float a) {
+        if (m >= (size_t)M_valid || n >= (size_t)N_valid) return;
+        const double x = (double)a;
+        if (EPI == 0) {
+            out[(n >> 6) * ts + m * kRTile + (n & 63)] = x + row_add[m];
+        } else {
+            const double ps = (m < (size_t)div_valid) ? row_div[m] : 0.;
+            out[m * ts + n] = ps > 0. ? x * scale / ps : x * scale;
         }
-    }
-
-    // epilogue
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int n = (int)n0 + wn * 128 + j * 32 + (lane & 31);
-            if (n >= N_valid) continue;
-            double cd = 1.;
-            if (EPI == 1 && n < div_valid) {
-                const double ps = col_div[n];
-                cd = ps > 0. ? ps : 1.;
-            }
-            double *__restrict__ col = out + (size_t)(n >> 6) * ts + (n & 63);
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int m = (int)m0 + wm * 64 + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                if (m >= M_valid) continue;
-                double x = (double)acc[i][j][v];
-                if (EPI == 0) x += row_add[m];
-                else x = x * scale / cd;
-                col[(size_t)m * kRTile] = x;
-            }
-        }
-    }
+    });
 }
 
 // ---- operand builders ---------------------------------------------------------------------
@@ -329,23 +222,21 @@ __global__ __launch_bounds__(kBlock) void posterior_fp16_kernel(const double *__
 
 // ---- launchers -----------------------------------------------------------------------------
 
-size_t gemm_lds_bytes(int nb) { return (size_t)(kGemmBM + nb * kGemmBN) * kGemmRow * sizeof(_Float16); }
+size_t gemm_lds_bytes(int nb) { return gemm_lds_elems<_Float16>(nb) * sizeof(_Float16); }
 
 int launch_mp_gemm(hipStream_t s, int nb, int epi, const _Float16 *A, size_t lda, const _Float16 *B0,
                    const _Float16 *B1, size_t ldb, int K, int Mp, int Np, int M_valid, int N_valid,
-                   double *out, size_t ts, const double *row_add, const double *col_div,
+                   double *out, size_t ts, const double *row_add, const double *row_div,
                    int div_valid, double scale) {
     if (Mp % kGemmBM || Np % kGemmBN || K % kGemmBK || lda % 8 || ldb % 8 || (size_t)K > lda ||
         (size_t)K > ldb || M_valid > Mp || N_valid > Np)
         return -1;
     const int m_tiles = Mp / kGemmBM, n_tiles = Np / kGemmBN;
-    const int tiles = m_tiles * n_tiles;
-    const int per_xcd = (tiles + kNumXcd - 1) / kNumXcd;
-    const dim3 grid(per_xcd * kNumXcd);
+    const dim3 grid(gemm_grid(m_tiles, n_tiles));
     const size_t lds = gemm_lds_bytes(nb);
 #define EMC_GEMM(NB, EPI)                                                                         \
     mp_gemm_kernel<NB, EPI><<<grid, kGemmThreads, lds, s>>>(A, lda, B0, B1, ldb, K, m_tiles, n_tiles, \
-                                                            M_valid, N_valid, out, ts, row_add, col_div, div_valid, scale)
+                                                            M_valid, N_valid, out, ts, row_add, row_div, div_valid, scale)
     if (nb == 2 && epi == 0) EMC_GEMM(2, 0);
     else if (nb == 1 && epi == 0) EMC_GEMM(1, 0);
     else if (nb == 1 && epi == 1) EMC_GEMM(1, 1);

@@ -254,3 +254,27 @@ extern "C" int emc_quat_gen(int num, double *quats) {
     for (int r = 0; r < nrot; ++r) quats[r * 5 + 4] *= total;
     return nrot;
 }
+
+// voronoi_subset (dragonfly/src/quaternion.c:844-862): nearest coarse orientation of every
+// fine one, by 1 - (q_f . q_c)^2, the first minimum kept.
+extern "C" int emc_voronoi_subset(int num_coarse, const double *qcoarse, int num_fine,
+                                  const double *qfine, int *nearest) {
+    if (num_coarse <= 0 || num_fine < 0 || !qcoarse || !qfine || !nearest) return -1;
+    for (int i = 0; i < num_fine; ++i) {
+        const double *f = qfine + 5 * (size_t)i;
+        double dmin = 2.;
+        int best = 0;
+        for (int j = 0; j < num_coarse; ++j) {
+            const double *c = qcoarse + 5 * (size_t)j;
+            double dot = 0.;
+            for (int k = 0; k < 4; ++k) dot += f[k] * c[k];
+            const double dist = 1. - dot * dot;
+            if (dist < dmin) {
+                dmin = dist;
+                best = j;
+            }
+        }
+        nearest[i] = best;
+    }
+    return 0;
+}

@@ -73,6 +73,9 @@ class Iterate:
         self.mean_count = None
         self.rescale = None
         self.likelihood = self.mutual_info = self.rms_change = 0.
+        # refinement: coarse cell of every orientation and per-frame coarse sets (CSR)
+        self.quat_mapping = self.rel_off = self.rel_quat = None
+        self.num_coarse = 0
         if config_fname != '':
             self.from_config(config_fname, section_name, resume=resume)
 
@@ -146,7 +149,69 @@ class Iterate:
         self.parse_blacklist(blacklist_file, sel_string)
         beta_str = config.get(section_name, 'beta', fallback='auto')
         self.calc_beta(None if beta_str == 'auto' else float(beta_str))
+        if self.params.refine:
+            try:
+                probs = config.get_filename(section_name, 'probs_file')
+            except configparser.NoOptionError:
+                raise ValueError('Need coarse probabilities in probs_file to do refinement')
+            self.setup_refine(self.params.coarse_div, self.parse_rel_quat(probs, self.params.coarse_div))
         return self
+
+    # ---- refinement (legacy src/iterate.c:103-130, :364-525) ----------------------------
+    def setup_refine(self, coarse_div, rel_quat):
+        """Map every (fine) orientation to its nearest coarse one (voronoi_subset) and attach
+        the per-frame coarse sets: rel_quat is a list of int arrays, one per frame."""
+        from . import _lib
+        if len(rel_quat) != self.tot_num_data:
+            raise ValueError('rel_quat has %d frames, data has %d' % (len(rel_quat), self.tot_num_data))
+        qcoarse = _lib.quat_gen(coarse_div)
+        self.quat_mapping = _lib.voronoi_subset(qcoarse, self.quat.quats)
+        self.num_coarse = qcoarse.shape[0]
+        counts = np.array([len(r) for r in rel_quat], dtype='i8')
+        self.rel_off = np.zeros(len(rel_quat) + 1, dtype='i8')
+        np.cumsum(counts, out=self.rel_off[1:])
+        self.rel_quat = np.concatenate([np.asarray(r, 'i4') for r in rel_quat]) if len(rel_quat) \
+            else np.zeros(0, 'i4')
+        if self.rel_quat.size and (self.rel_quat.min() < 0 or self.rel_quat.max() >= self.num_coarse):
+            raise ValueError('rel_quat entries must index the %d coarse orientations' % self.num_coarse)
+
+    def parse_rel_quat(self, fname, coarse_div):
+        """Per-frame coarse orientation sets from a previous coarse run: the binary probs
+        format (int num_data, int num_rot, 1024-byte header, int num_rel_quat[num_data], then
+        the lists), or an output file with 'probabilities/num_prob' + 'probabilities/place'
+        (sparse posteriors) or else 'orientations' (one per frame)."""
+        from . import _lib
+        num_rot_coarse = _lib.num_rot(coarse_div)
+        with open(fname, 'rb') as f:
+            head = f.read(8)
+        if head == b'\x89HDF\r\n\x1a\n':
+            try:
+                import h5py
+            except ImportError:
+                raise TypeError('HDF5 probability files need h5py, which is not available here')
+            with h5py.File(fname, 'r') as f:
+                data = {k: f[k][()] for k in ('probabilities/num_prob', 'probabilities/place',
+                                              'orientations') if k in f}
+        elif fname.endswith('.npz'):
+            with np.load(fname) as f:
+                data = {k: f[k] for k in f.files}
+        else:
+            with open(fname, 'rb') as f:
+                num_data, num_rot = (int(x) for x in np.fromfile(f, dtype='i4', count=2))
+                if num_data != self.tot_num_data:
+                    raise ValueError('num_data of rel_quat (%d) does not match' % num_data)
+                if num_rot != num_rot_coarse:
+                    raise ValueError('Rotation sampling in probs file does not match num_rot_coarse '
+                                     '(%d vs %d)' % (num_rot, num_rot_coarse))
+                f.seek(1024)
+                counts = np.fromfile(f, dtype='i4', count=num_data)
+                flat = np.fromfile(f, dtype='i4', count=int(counts.sum()))
+            return np.split(flat, np.cumsum(counts)[:-1])
+        if 'probabilities/num_prob' in data:
+            counts = np.asarray(data['probabilities/num_prob'], 'i8')
+            return np.split(np.asarray(data['probabilities/place'], 'i4'), np.cumsum(counts)[:-1])
+        # frames without an orientation (rmax -1) match no coarse cell
+        return [np.array([o] if o >= 0 else [], 'i4') for o in np.asarray(data['orientations']).ravel()]
 
     def set_params(self, param):
         self.params = param

@@ -1,0 +1,205 @@
+"""Learned orientation proposal: a BF16 multilayer perceptron that maps a polar photon
+histogram of a frame to logits over a coarse orientation set, trained on the GPU on
+(pattern, EMC posterior) pairs that earlier iterations produce.
+
+Every arithmetic step runs in this package's own HIP kernels (csrc/emc_nn.hip: matrix-core
+products with FP32 accumulation, softmax cross entropy, column sums, Adam, histogram,
+top-k) through the C ABI; torch only allocates the device buffers and provides the stream.
+
+The proposals feed refinement: the top-k coarse cells of each frame become its coarse set
+(Iterate.setup_refine / Engine.set_refine), and the fine search is restricted to the Voronoi
+cells of those orientations.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+
+TILE = 256      # product tile (rows of both operands, output columns)
+KSTEP = 32      # product K step
+
+
+def _round(x, m):
+    return (x + m - 1) // m * m
+
+
+def polar_bins(qvals, mask, num_r=64, num_a=64):
+    """Bin id of every pixel: radial bin of |q| over [0, max |q|] times angular bin of
+    atan2(qy, qx); -1 for pixels with mask >= 1."""
+    q = np.asarray(qvals, dtype='f8')
+    r = np.sqrt((q ** 2).sum(1))
+    rb = np.minimum((r / max(r.max(), 1e-30) * num_r).astype(int), num_r - 1)
+    ab = np.minimum(((np.arctan2(q[:, 1], q[:, 0]) + math.pi) / (2 * math.pi) * num_a).astype(int), num_a - 1)
+    ids = (rb * num_a + ab).astype('i4')
+    ids[np.asarray(mask) >= 1] = -1
+    return ids
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ok(rc, what):
+    if rc != 0:
+        raise _lib.EMCError('%s: bad shape or argument (%d)' % (what, rc))
+
+
+class OrientationProposal:
+    """MLP nbins -> width (x depth, ReLU) -> num_out logits, BF16 weights and activations,
+    FP32 master weights with Adam."""
+
+    def __init__(self, num_out, nbins=4096, width=1024, depth=3, device=0, seed=0):
+        import torch
+        self.torch = torch
+        self.dev = torch.device('cuda', device)
+        self.num_out = int(num_out)
+        self.nbins = int(nbins)
+        dims = [self.nbins] + [int(width)] * int(depth) + [self.num_out]
+        self.dims = dims
+        self.pdims = [_round(d, TILE) for d in dims]
+        rng = np.random.default_rng(seed)
+        self.W32, self.Wb, self.Wt, self.b = [], [], [], []
+        self.mW, self.vW, self.mb, self.vb, self.gW, self.gb = [], [], [], [], [], []
+        for l in range(len(dims) - 1):
+            fin, fout = dims[l], dims[l + 1]
+            pin, pout = self.pdims[l], self.pdims[l + 1]
+            w = np.zeros((pout, pin), 'f4')
+            w[:fout, :fin] = rng.normal(0., math.sqrt(2. / fin), (fout, fin))
+            W32 = torch.tensor(w, device=self.dev)
+            self.W32.append(W32)
+            self.Wb.append(W32.to(torch.bfloat16))
+            self.Wt.append(W32.t().contiguous().to(torch.bfloat16))
+            self.b.append(torch.zeros(pout, dtype=torch.float32, device=self.dev))
+            for lst, shape in ((self.mW, (pout, pin)), (self.vW, (pout, pin)), (self.gW, (pout, pin)),
+                               (self.mb, (1, pout)), (self.vb, (1, pout)), (self.gb, (1, pout))):
+                lst.append(torch.zeros(shape, dtype=torch.float32, device=self.dev))
+        self.step = 0
+        self._acts = None
+
+    # ---- features ------------------------------------------------------------------------
+    def features(self, off, pix, cnt, bin_of):
+        """Polar histograms of all frames: off (num_frames + 1) int64, pix int32, cnt float64
+        photon lists (mask < 1 photons), bin_of int32 per pixel (polar_bins). Returns a BF16
+        (num_frames rounded up to the tile, nbins) device tensor, rows past num_frames zero."""
+        torch = self.torch
+        n = len(off) - 1
+        X = torch.zeros((_round(max(n, 1), TILE), self.pdims[0]), dtype=torch.bfloat16, device=self.dev)
+        t_off = torch.as_tensor(np.asarray(off, 'i8'), device=self.dev)
+        t_pix = torch.as_tensor(np.asarray(pix, 'i4') if len(pix) else np.zeros(1, 'i4'), device=self.dev)
+        t_cnt = torch.as_tensor(np.asarray(cnt, 'f8') if len(cnt) else np.zeros(1), device=self.dev)
+        t_bin = torch.as_tensor(np.asarray(bin_of, 'i4'), device=self.dev)
+        _ok(_lib.lib().emc_nn_polar_hist(_stream(), _ptr(t_off), _ptr(t_pix), _ptr(t_cnt), _ptr(t_bin),
+                                         self.nbins, n, _ptr(X), X.shape[1]), 'polar histogram')
+        return X
+
+    # ---- forward / backward ------------------------------------------------------------------
+    def forward(self, X, rows):
+        """Logits (FP32, padded columns) of the first `rows` rows of X (rows padded to TILE)."""
+        torch = self.torch
+        L = _lib.lib()
+        B = X.shape[0]
+        acts = [X]
+        h = X
+        for l in range(len(self.dims) - 1):
+            last = l == len(self.dims) - 2
+            out = torch.zeros((B, self.pdims[l + 1]), device=self.dev,
+                              dtype=torch.float32 if last else torch.bfloat16)
+            _ok(L.emc_nn_forward(_stream(), _ptr(h), h.shape[1], _ptr(self.Wb[l]), self.Wb[l].shape[1], B,
+                                 self.pdims[l + 1], self.pdims[l], rows, self.pdims[l + 1], _ptr(self.b[l]),
+                                 0 if last else 1, 1 if last else 0, _ptr(out), out.shape[1]), 'forward')
+            acts.append(out)
+            h = out
+        self._acts = acts
+        return h
+
+    def loss_and_grads(self, X, Y, rows):
+        """Forward, soft-target cross entropy against Y (FP32 (B, num_out)), and the gradients
+        of the mean loss into gW / gb. Returns the mean loss."""
+        torch = self.torch
+        L = _lib.lib()
+        B = X.shape[0]
+        Z = self.forward(X, rows)
+        dZ = torch.zeros((B, self.pdims[-1]), dtype=torch.bfloat16, device=self.dev)
+        loss = torch.zeros(max(rows, 1), dtype=torch.float32, device=self.dev)
+        Yc = Y.contiguous()
+        _ok(L.emc_nn_softmax_xent(_stream(), _ptr(Z), Z.shape[1], _ptr(Yc), Yc.shape[1], rows, self.num_out,
+                                  C.c_float(1. / rows), _ptr(loss), _ptr(dZ), dZ.shape[1]), 'softmax')
+        g = dZ
+        for l in range(len(self.dims) - 2, -1, -1):
+            A = self._acts[l]                         # input of layer l, (B, pin)
+            pin, pout = self.pdims[l], self.pdims[l + 1]
+            gT = torch.empty((pout, B), dtype=torch.bfloat16, device=self.dev)
+            AT = torch.empty((pin, B), dtype=torch.bfloat16, device=self.dev)
+            _ok(L.emc_nn_transpose(_stream(), _ptr(g), g.shape[1], B, pout, _ptr(gT), B), 'transpose')
+            _ok(L.emc_nn_transpose(_stream(), _ptr(A), A.shape[1], B, pin, _ptr(AT), B), 'transpose')
+            _ok(L.emc_nn_backward_weight(_stream(), _ptr(gT), B, _ptr(AT), B, pout, pin, B, pout, pin,
+                                         _ptr(self.gW[l]), pin), 'weight gradient')
+            _ok(L.emc_nn_colsum(_stream(), _ptr(g), g.shape[1], B, pout, _ptr(self.gb[l])), 'bias gradient')
+            if l > 0:
+                dX = torch.zeros((B, pin), dtype=torch.bfloat16, device=self.dev)
+                _ok(L.emc_nn_backward_data(_stream(), _ptr(g), g.shape[1], _ptr(self.Wt[l]), self.Wt[l].shape[1],
+                                           B, pin, pout, rows, pin, _ptr(A), A.shape[1], _ptr(dX), pin),
+                    'data gradient')
+                g = dX
+        return float(loss.cpu().numpy()[:rows].sum()) / max(rows, 1)
+
+    def apply_adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, wd=0.):
+        L = _lib.lib()
+        self.step += 1
+        for l in range(len(self.dims) - 1):
+            W = self.W32[l]
+            _ok(L.emc_nn_adam(_stream(), _ptr(W), _ptr(self.gW[l]), _ptr(self.mW[l]), _ptr(self.vW[l]),
+                              W.shape[0], W.shape[1], W.shape[1], C.c_float(lr), C.c_float(b1), C.c_float(b2),
+                              C.c_float(eps), C.c_float(wd), self.step, _ptr(self.Wb[l]), _ptr(self.Wt[l]),
+                              self.Wt[l].shape[1]), 'adam')
+            bvec = self.b[l]
+            _ok(L.emc_nn_adam(_stream(), _ptr(bvec), _ptr(self.gb[l]), _ptr(self.mb[l]), _ptr(self.vb[l]), 1,
+                              bvec.shape[0], bvec.shape[0], C.c_float(lr), C.c_float(b1), C.c_float(b2),
+                              C.c_float(eps), C.c_float(0.), self.step, None, None, 0), 'adam')
+
+    # ---- training and inference ----------------------------------------------------------------
+    def fit(self, X, Y, num, epochs=5, batch=1024, lr=1e-3, seed=0, log=None):
+        """Minibatch Adam on the first `num` rows of X (features) and Y (targets, rows summing
+        to 1). Returns the mean loss of every epoch."""
+        torch = self.torch
+        batch = _round(batch, TILE)
+        gen = torch.Generator(device='cpu')
+        gen.manual_seed(seed)
+        Xb = torch.zeros((batch, X.shape[1]), dtype=X.dtype, device=self.dev)
+        Yb = torch.zeros((batch, Y.shape[1]), dtype=Y.dtype, device=self.dev)
+        history = []
+        for ep in range(epochs):
+            perm = torch.randperm(num, generator=gen).to(self.dev)
+            tot, cnt = 0., 0
+            for b0 in range(0, num, batch):
+                idx = perm[b0:b0 + batch]
+                rows = idx.numel()
+                Xb.zero_()
+                Yb.zero_()
+                Xb[:rows] = X[idx]
+                Yb[:rows] = Y[idx]
+                tot += self.loss_and_grads(Xb, Yb, rows) * rows
+                cnt += rows
+                self.apply_adam(lr)
+            history.append(tot / max(cnt, 1))
+            if log:
+                log('epoch %d loss %.4f' % (ep + 1, history[-1]))
+        return history
+
+    def logits(self, X, num):
+        return self.forward(X, num)[:num, :self.num_out]
+
+    def propose(self, X, num, k):
+        """Top-k output indices (descending logit) of the first `num` rows: (num, k) int32."""
+        torch = self.torch
+        Z = self.forward(X, num)
+        idx = torch.empty((max(num, 1), k), dtype=torch.int32, device=self.dev)
+        _ok(_lib.lib().emc_nn_topk(_stream(), _ptr(Z), Z.shape[1], num, self.num_out, k, _ptr(idx)), 'top-k')
+        return idx[:num].cpu().numpy()

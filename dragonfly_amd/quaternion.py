@@ -35,9 +35,10 @@ class Quaternion:
         if rtype != '3d':
             raise NotImplementedError('only 3d orientation sampling is implemented')
         num_div = config.get(section_name, 'num_div', fallback='0').split()
+        # "num_div = fine coarse" selects refinement (params.pyx:121-126): the fine sampling
+        # is the orientation set, the coarse one indexes the per-frame subsets
         self.generate_3d(int(num_div[0]))
-        if len(num_div) == 2:
-            raise NotImplementedError('refinement (two num_div values) is not implemented')
+        self.coarse_div = int(num_div[1]) if len(num_div) > 1 else 0
         point_group = config.get(section_name, 'point_group', fallback='1')
         if point_group != '1':
             raise NotImplementedError('point group %s is not implemented' % point_group)

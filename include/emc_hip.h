@@ -88,6 +88,10 @@ int emc_synchronize(emc_engine *e);
  * quats_out receives emc_num_rot(num_div) rows of (q0, q1, q2, q3, weight).            */
 int emc_num_rot(int num_div);
 int emc_quat_gen(int num_div, double *quats_out);
+/* voronoi_subset(qcoarse, qfine, nearest)  dragonfly/src/quaternion.c:844-862: the coarse
+ * orientation nearest to every fine one (used for refinement).                          */
+int emc_voronoi_subset(int num_coarse, const double *qcoarse, int num_fine,
+                       const double *qfine, int *nearest);
 
 /* ---- inputs ----------------------------------------------------------------------------
  * detector: struct detector qvals/corr/raw_mask  (dragonfly/src/detector.h:6-16,
@@ -110,6 +114,14 @@ int emc_set_frames_sparse(emc_engine *e, int num_data, const int *ones, const in
  * orientation with log W = -inf on any such pixel is dropped for these frames; the tomogram
  * adds every pixel (max_update.c:211-214).                                             */
 int emc_set_frames_dense(emc_engine *e, int num_data, int num_pix, const int *frames);
+/* refinement: orientation subsets per frame (legacy engine: quat_mapping from
+ * voronoi_subset, rel_quat/num_rel_quat from parse_rel_quat, src/iterate.c:103-130,
+ * :364-525; applied in src/max_emc.c:264-274). quat_mapping[r] is the coarse cell of
+ * orientation r (num_rot entries, values < num_coarse); frame d is evaluated only at the
+ * orientations whose cell is listed in rel_quat[rel_off[d] .. rel_off[d + 1]). A null
+ * quat_mapping switches refinement off; setting quaternions or frames also does.        */
+int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const long *rel_off,
+                   const int *rel_quat);
 /* arithmetic of the two photon contractions. No counterpart in the reference (which
  * computes everything in FP64): 0 = plain FP64 (default, reproduces the reference);
  * 1 = dense frames only: likelihood and tomogram as FP16 products on matrix cores with
@@ -154,6 +166,43 @@ int emc_slice_merge(emc_engine *e, const double *quat, const double *view);
 int emc_get_field(emc_engine *e, int field, void *dst, size_t count);
 /* Zero model2 and inter_weight (max_data allocate_memory, maximize.c:93-94).            */
 int emc_clear_merge(emc_engine *e);
+
+/* ---- orientation proposal network --------------------------------------------------------
+ * No counterpart in the reference: kernels of the learned orientation proposal (a BF16
+ * multilayer perceptron trained on (pattern, posterior) pairs whose top-k coarse cells feed
+ * emc_set_refine). All pointers are device pointers; work is ordered on `stream` (a
+ * hipStream_t). Products need M, N multiples of 256, K of 32, leading dimensions (in
+ * elements) multiples of 8 and >= K; BF16 operands, FP32 accumulation. Return 0, or -1 on a
+ * bad shape or pointer (nothing launched).                                                */
+/* out[m][n] = act(X[m] . W[n] + bias[n]), m < M_valid, n < N_valid; relu: max(0, .);
+ * out_f32: FP32 output, else BF16                                                          */
+int emc_nn_forward(void *stream, const void *X, size_t ldx, const void *W, size_t ldw, int M, int N,
+                   int K, int M_valid, int N_valid, const float *bias, int relu, int out_f32, void *out,
+                   size_t ldo);
+/* dX[m][n] = (dY[m] . Wt[n]) * (H[m][n] > 0) (H may be null), BF16                         */
+int emc_nn_backward_data(void *stream, const void *dY, size_t ldy, const void *Wt, size_t ldwt, int M,
+                         int N, int K, int M_valid, int N_valid, const void *H, size_t ldh, void *dX,
+                         size_t ldx);
+/* dW[m][n] = dYt[m] . Xt[n], FP32                                                           */
+int emc_nn_backward_weight(void *stream, const void *dYt, size_t lda, const void *Xt, size_t ldb, int M,
+                           int N, int K, int M_valid, int N_valid, float *dW, size_t ldo);
+/* out[c][r] = in[r][c], BF16                                                                */
+int emc_nn_transpose(void *stream, const void *in, size_t ldi, int rows, int cols, void *out, size_t ldo);
+/* per row: p = softmax(Z[0..n)), loss = -sum Y log p, dZ = (p - Y) * grad_scale (BF16)      */
+int emc_nn_softmax_xent(void *stream, const float *Z, size_t ldz, const float *Y, size_t ldy, int rows,
+                        int n, float grad_scale, float *loss, void *dZ, size_t ldd);
+/* db[c] = sum over rows of dY[r][c] (BF16 in, FP32 out)                                     */
+int emc_nn_colsum(void *stream, const void *dY, size_t ldy, int rows, int cols, float *db);
+/* Adam step on FP32 master weights; refreshes the BF16 copy wb and transposed copy wt       */
+int emc_nn_adam(void *stream, float *w, const float *g, float *m, float *v, int rows, int cols, size_t ld,
+                float lr, float b1, float b2, float eps, float wd, int step, void *wb, void *wt,
+                size_t ldt);
+/* polar photon histogram per frame: photon lists (off, pix, cnt), bin_of[pixel] (-1 none),
+ * written as log(1 + 1000 * fraction) to out[frame][0..nbins), BF16                        */
+int emc_nn_polar_hist(void *stream, const long *off, const int *pix, const double *cnt,
+                      const int *bin_of, int nbins, int num_frames, void *out, size_t ldo);
+/* indices of the k (<= 64) largest of Z[row][0..n), descending, ties to the lower index      */
+int emc_nn_topk(void *stream, const float *Z, size_t ldz, int rows, int n, int k, int *idx);
 
 #ifdef __cplusplus
 }

@@ -75,8 +75,10 @@ def dense_counts(frames, num_pix):
     return K
 
 
-def iterate(det, quats, model1, frames, fparams, flags):
-    """One iteration; returns the same fields as oracle.core.maximize."""
+def iterate(det, quats, model1, frames, fparams, flags, refine=None):
+    """One iteration; returns the same fields as oracle.core.maximize. refine (optional):
+    dict quat_mapping, rel_off, rel_quat -- frame d only takes orientations whose coarse cell
+    is in its set (legacy src/max_emc.c:264-274)."""
     qvals, corr, mask = det['qvals'], det['corr'], det['mask']
     Rn, P = quats.shape[0], len(corr)
     size = model1.shape[0]
@@ -106,6 +108,9 @@ def iterate(det, quats, model1, frames, fparams, flags):
         t = np.nonzero(good)[0] if dense else np.nonzero((K[d] > 0) & good)[0]
         with np.errstate(invalid='ignore'):
             R[d] = np.log(w_r) - u * phi + (K[d, t][None, :] * logW[:, t]).sum(1)
+        if refine is not None:
+            cells = refine['rel_quat'][refine['rel_off'][d]:refine['rel_off'][d + 1]]
+            R[d][~np.isin(refine['quat_mapping'], cells)] = -np.inf
     M = np.full(D, -1.e100)
     rmax = np.full(D, -1, dtype=np.int64)
     Pm = np.zeros((D, Rn))

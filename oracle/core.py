@@ -24,6 +24,7 @@ class Problem(C.Structure):
         ('count_multi', _ip), ('int_frames', _ip),
         ('beta', _dp), ('scale', _dp), ('blacklist', _u8p), ('fcounts', _ip), ('sum_fact', _dp),
         ('mean_count', C.c_double), ('num_blacklist', C.c_int),
+        ('quat_mapping', _ip), ('rel_off', _lp), ('rel_quat', _ip),
         ('need_scaling', C.c_int), ('update_scale', C.c_int), ('iteration', C.c_int),
         ('known_scale', C.c_int), ('num_threads', C.c_int),
     ]
@@ -61,6 +62,7 @@ def lib():
         L.orc_maximize.restype = C.c_double
         L.orc_update_model.argtypes = [_dp, _dp, _dp, C.c_long, C.c_double]
         L.orc_update_model.restype = C.c_double
+        L.orc_voronoi_subset.argtypes = [C.c_int, _dp, C.c_int, _dp, _ip]
         _lib = L
     return _lib
 
@@ -116,7 +118,17 @@ def update_model(model1, model2, inter_weight, alpha=0.):
                                   float(alpha))
 
 
-def maximize(det, quats, model1, frames, fparams, flags, num_threads=1, dense_prob=False):
+def voronoi_subset(qcoarse, qfine):
+    """Nearest coarse orientation of every fine orientation (quaternion.c:844-862)."""
+    qc = np.ascontiguousarray(qcoarse, dtype='f8')
+    qf = np.ascontiguousarray(qfine, dtype='f8')
+    out = np.zeros(len(qf), dtype='i4')
+    lib().orc_voronoi_subset(len(qc), _p(qc), len(qf), _p(qf), _p(out, _ip))
+    return out
+
+
+def maximize(det, quats, model1, frames, fparams, flags, num_threads=1, dense_prob=False,
+             refine=None):
     """Run one oracle iteration.
 
     det      : dict qvals (N,3) f8, corr (N,) f8, mask (N,) u1
@@ -126,6 +138,8 @@ def maximize(det, quats, model1, frames, fparams, flags, num_threads=1, dense_pr
                place_multi, count_multi, or 'dense' key int_frames (D, N) i4
     fparams  : dict beta, scale, blacklist, fcounts, sum_fact, mean_count
     flags    : dict need_scaling, update_scale, iteration, known_scale
+    refine   : None, or dict quat_mapping (R,) i4, rel_off (D+1,) i8, rel_quat i4 (frame d is
+               evaluated only where quat_mapping[r] is in rel_quat[rel_off[d]:rel_off[d+1]])
     Returns a dict of outputs; fparams['scale'] is updated in place when scaling is on.
     """
     qvals = np.ascontiguousarray(det['qvals'], dtype='f8')
@@ -180,6 +194,10 @@ def maximize(det, quats, model1, frames, fparams, flags, num_threads=1, dense_pr
     prob.iteration = int(flags.get('iteration', 1))
     prob.known_scale = int(flags.get('known_scale', 0))
     prob.num_threads = int(num_threads)
+    if refine is not None:
+        prob.quat_mapping = _p(arr(refine['quat_mapping'], 'i4'), _ip)
+        prob.rel_off = _p(arr(refine['rel_off'], 'i8'), _lp)
+        prob.rel_quat = _p(arr(refine['rel_quat'], 'i4'), _ip)
 
     out = dict(u=np.zeros(R), max_exp=np.zeros(D), p_norm=np.zeros(D), rmax=np.zeros(D, 'i4'),
                likelihood=np.zeros(D), info=np.zeros(D), psum_d=np.zeros(D))

@@ -312,6 +312,24 @@ int orc_quat_gen(int num, double *quats) {
     return nrot;
 }
 
+void orc_voronoi_subset(int num_coarse, const double *qcoarse, int num_fine, const double *qfine,
+                        int *nearest) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < num_fine; ++i) {
+        double dmin = 2.;
+        nearest[i] = 0;
+        for (int j = 0; j < num_coarse; ++j) {
+            double dot = 0.;
+            for (int c = 0; c < 4; ++c) dot += qfine[i * 5 + c] * qcoarse[j * 5 + c];
+            double dist = 1. - dot * dot;
+            if (dist < dmin) {
+                dmin = dist;
+                nearest[i] = j;
+            }
+        }
+    }
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* Geometry: dragonfly/src/model.c                                                       */
 /* ------------------------------------------------------------------------------------ */
@@ -527,6 +545,12 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
             double logw = log(p->quats[r * 5 + 4]);
             for (int d = 0; d < N; ++d) {
                 if (p->blacklist[d]) continue;
+                if (p->quat_mapping) {
+                    int found = 0;
+                    for (long k = p->rel_off[d]; k < p->rel_off[d + 1] && !found; ++k)
+                        found = p->rel_quat[k] == p->quat_mapping[r];
+                    if (!found) continue;
+                }
                 double pval;
                 if (p->need_scaling && (p->iteration > 1 || p->known_scale))
                     pval = logw - res->u[r] * p->scale[d];

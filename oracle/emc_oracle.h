@@ -46,6 +46,12 @@ typedef struct orc_problem {
     const double *sum_fact;
     double mean_count;
     int num_blacklist;
+    /* refinement (legacy src/iterate.c:103-130, src/max_emc.c:264-274): when quat_mapping is
+     * set, frame d is evaluated only at orientations r whose coarse cell quat_mapping[r] is
+     * one of rel_quat[rel_off[d] .. rel_off[d + 1]) */
+    const int *quat_mapping;  /* [num_rot] or NULL */
+    const long *rel_off;      /* [num_data + 1] */
+    const int *rel_quat;
     /* flags (dragonfly/src/params.h) */
     int need_scaling, update_scale, iteration, known_scale;
     int num_threads;          /* OpenMP threads; 1 gives the canonical single-thread keep rule */
@@ -67,6 +73,10 @@ typedef struct orc_result {
 } orc_result;
 
 int orc_num_rot(int num_div);
+/* voronoi_subset (dragonfly/src/quaternion.c:844-862): nearest coarse orientation of every
+ * fine one by 1 - (q_f . q_c)^2, first minimum kept */
+void orc_voronoi_subset(int num_coarse, const double *qcoarse, int num_fine, const double *qfine,
+                        int *nearest);
 int orc_quat_gen(int num_div, double *quats);
 void orc_make_rot_quat(const double *q, double *rot9);
 void orc_slice_gen3d(const double *quat, const double *model, long size,

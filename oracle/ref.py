@@ -52,6 +52,7 @@ def lib():
         L.slice_gen3d.argtypes = [_dp, C.c_int, _dp, C.POINTER(CDet), C.POINTER(CModel)]
         L.slice_merge3d.argtypes = [_dp, C.c_int, _dp, _dp, _dp, C.c_long, C.POINTER(CDet)]
         L.symmetrize_friedel.argtypes = [_dp, _dp, C.c_int]
+        L.voronoi_subset.argtypes = [C.POINTER(CQuat), C.POINTER(CQuat), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -61,6 +62,18 @@ def quat_gen(num_div):
     q.num_div = num_div
     n = lib().quat_gen(num_div, C.byref(q))
     return np.ctypeslib.as_array(q.quats, shape=(n * 5,)).copy().reshape(n, 5)
+
+
+def voronoi_subset(qcoarse, qfine):
+    """voronoi_subset (quaternion.c:844-862) on the given quaternion arrays."""
+    qc = np.ascontiguousarray(qcoarse, dtype='f8')
+    qf = np.ascontiguousarray(qfine, dtype='f8')
+    a, b = CQuat(), CQuat()
+    a.num_rot, a.quats = len(qc), qc.ctypes.data_as(_dp)
+    b.num_rot, b.quats = len(qf), qf.ctypes.data_as(_dp)
+    out = np.zeros(len(qf), dtype='i4')
+    lib().voronoi_subset(C.byref(a), C.byref(b), out.ctypes.data_as(C.POINTER(C.c_int)))
+    return out
 
 
 def make_rot_quat(quat):

@@ -8,6 +8,10 @@ runs only where they are mounted). Writes, next to this script:
                     the row count and a few rows in full
   geometry.npz      inputs and outputs of slice_gen3d, slice_merge3d, make_rot_quat and
                     symmetrize_friedel on small seeded cases (odd and even grid sizes)
+  voronoi_sha256.json  sha256 of the int32 output of voronoi_subset(quat_gen(c), quat_gen(f))
+                    for the (coarse, fine) pairs in VORONOI, plus its first entries
+
+`python make_golden.py voronoi` writes only the last file.
 """
 import hashlib
 import json
@@ -21,12 +25,26 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 NUM_DIVS = (1, 2, 3, 4, 5, 6, 7, 10, 12, 16)
+VORONOI = ((1, 3), (2, 5), (4, 8))
+
+
+def write_voronoi(ref):
+    out = {}
+    for c, f in VORONOI:
+        m = ref.voronoi_subset(ref.quat_gen(c), ref.quat_gen(f)).astype('<i4')
+        out['%d_%d' % (c, f)] = dict(num_fine=int(m.size), sha256=hashlib.sha256(m.tobytes()).hexdigest(),
+                                     head=m[:16].tolist())
+    with open(os.path.join(HERE, 'voronoi_sha256.json'), 'w') as fh:
+        json.dump(out, fh, indent=1)
 
 
 def main():
     from oracle import ref
     if not ref.available():
         raise SystemExit('oracle/_ref/libdfref.so missing: run `make -C oracle ref` first')
+    write_voronoi(ref)
+    if sys.argv[1:] == ['voronoi']:
+        return
     quats = {}
     for n in NUM_DIVS:
         q = ref.quat_gen(n)

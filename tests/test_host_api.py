@@ -165,3 +165,42 @@ def test_calculate_size():
     assert df.Iterate.calculate_size(60.95) == 125
     assert df.Iterate.calculate_size(123.9) == 251
     assert df.Iterate.calculate_size(61.0) == 125
+
+
+def test_parse_rel_quat_formats(tmp_path):
+    """Per-frame coarse sets for refinement (legacy parse_rel_quat, src/iterate.c:364-525):
+    the binary probs layout, an output file with sparse probabilities, and one with only the
+    most likely orientations (rmax -1 gives an empty set); then the Voronoi map."""
+    itr, _ = _iterate(0)
+    itr.set_quat(df.Quaternion(2))
+    D = itr.tot_num_data
+    rng = np.random.default_rng(4)
+    sets = [np.sort(rng.choice(60, rng.integers(0, 5), replace=False)).astype('i4') for _ in range(D)]
+    hdr = np.zeros(256, 'i4')
+    hdr[0], hdr[1] = D, 60
+    fn = tmp_path / 'probs.bin'
+    with open(fn, 'wb') as f:
+        hdr.tofile(f)
+        np.array([len(s) for s in sets], 'i4').tofile(f)
+        for s in sets:
+            s.tofile(f)
+    got = itr.parse_rel_quat(str(fn), 1)
+    assert all(np.array_equal(a, b) for a, b in zip(got, sets))
+    hdr[1] = 61
+    with open(fn, 'r+b') as f:
+        hdr[:2].tofile(f)
+    with pytest.raises(ValueError, match='num_rot_coarse'):
+        itr.parse_rel_quat(str(fn), 1)
+    npz = tmp_path / 'out.npz'
+    np.savez(npz, **{'probabilities/num_prob': np.array([len(s) for s in sets], 'i4'),
+                     'probabilities/place': np.concatenate(sets), 'orientations': np.arange(D)})
+    got = itr.parse_rel_quat(str(npz), 1)
+    assert all(np.array_equal(a, b) for a, b in zip(got, sets))
+    orient = np.arange(D) - 1
+    np.savez(npz, orientations=orient)
+    got = itr.parse_rel_quat(str(npz), 1)
+    assert len(got[0]) == 0 and all(list(g) == [o] for g, o in zip(got[1:], orient[1:]))
+    itr.setup_refine(1, sets)
+    assert itr.num_coarse == 60 and len(itr.quat_mapping) == itr.quat.num_rot
+    assert itr.rel_off[-1] == sum(len(s) for s in sets)
+    assert np.array_equal(itr.quat_mapping, df._lib.voronoi_subset(df._lib.quat_gen(1), itr.quat.quats))

@@ -162,3 +162,52 @@ def test_dense_frames_vs_brute(tiny, zero_block):
     if zero_block:
         sparse = P.run_oracle(dict(tiny, model1=p['model1']), flags)
         assert (o['prob'] == 0).sum() > (sparse['prob'] == 0).sum()
+
+
+def _refine_sets(p, coarse_div, per_frame, seed):
+    """Voronoi map of the problem's orientations onto quat_gen(coarse_div) and random
+    per-frame coarse sets of `per_frame` cells."""
+    from oracle import core
+    qc = core.quat_gen(coarse_div)
+    qmap = core.voronoi_subset(qc, p['quats'])
+    rng = np.random.default_rng(seed)
+    D = len(p['fparams']['beta'])
+    sets = [np.sort(rng.choice(len(qc), per_frame, replace=False)).astype('i4') for _ in range(D)]
+    sets[3] = np.zeros(0, 'i4')          # a frame with no orientation at all
+    off = np.zeros(D + 1, 'i8')
+    np.cumsum([len(x) for x in sets], out=off[1:])
+    return dict(quat_mapping=qmap, rel_off=off, rel_quat=np.concatenate(sets)), len(qc)
+
+
+def test_refinement_vs_brute(tiny):
+    """Refinement (legacy src/max_emc.c:264-274): frames evaluated only on their coarse cells'
+    orientations; oracle vs brute force, and the frame with an empty set gets no orientation."""
+    ref, _ = _refine_sets(tiny, 1, 12, 5)
+    flags = dict(iteration=1, need_scaling=0)
+    from oracle import core
+    o = core.maximize(tiny['det'], tiny['quats'], tiny['model1'], tiny['frames'],
+                      dict(tiny['fparams'], scale=np.ones(len(tiny['fparams']['beta']))), flags,
+                      dense_prob=True, refine=ref)
+    b = brute.iterate(tiny['det'], tiny['quats'], tiny['model1'], tiny['frames'], tiny['fparams'],
+                      flags, refine=ref)
+    _check(o, b)
+    assert o['rmax'][3] == -1 and (o['prob'][3] == 0).all()
+    allowed = np.isin(ref['quat_mapping'], ref['rel_quat'][ref['rel_off'][0]:ref['rel_off'][1]])
+    assert (o['prob'][0][~allowed] == 0).all() and o['prob'][0][allowed].sum() > 0.999999
+
+
+def test_refinement_with_every_cell_is_the_full_search(tiny):
+    from oracle import core
+    qc = core.quat_gen(1)
+    D = len(tiny['fparams']['beta'])
+    ref = dict(quat_mapping=core.voronoi_subset(qc, tiny['quats']),
+               rel_off=np.arange(D + 1, dtype='i8') * len(qc),
+               rel_quat=np.tile(np.arange(len(qc), dtype='i4'), D))
+    flags = dict(iteration=1, need_scaling=0)
+    fp = dict(tiny['fparams'], scale=np.ones(D))
+    a = core.maximize(tiny['det'], tiny['quats'], tiny['model1'], tiny['frames'], fp, flags,
+                      dense_prob=True, refine=ref)
+    b = core.maximize(tiny['det'], tiny['quats'], tiny['model1'], tiny['frames'], dict(fp), flags,
+                      dense_prob=True)
+    for k in KEYS:
+        assert np.array_equal(a[k], b[k]), k

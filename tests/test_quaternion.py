@@ -81,3 +81,25 @@ def test_divide_uses_integer_division():
     assert isinstance(q.divide(0, 1), int)
     q2 = Quaternion(1)
     assert [q2.divide(r, 7) for r in range(7)] == [9, 9, 9, 9, 8, 8, 8]
+
+
+with open(os.path.join(HERE, 'golden', 'voronoi_sha256.json')) as _f:
+    VORONOI = json.load(_f)
+
+
+@pytest.mark.parametrize('key', list(VORONOI))
+def test_voronoi_subset_matches_golden(oracle_lib, key):
+    """Nearest coarse orientation of every fine one (quaternion.c:844-862): the product's and
+    the oracle's mapping equal the compiled reference's bytes, and every fine orientation's
+    cell maximises |q_f . q_c| (brute force over all coarse orientations)."""
+    c, f = (int(x) for x in key.split('_'))
+    qc, qf = _lib.quat_gen(c), _lib.quat_gen(f)
+    m = _lib.voronoi_subset(qc, qf)
+    assert m.size == VORONOI[key]['num_fine']
+    assert hashlib.sha256(m.astype('<i4').tobytes()).hexdigest() == VORONOI[key]['sha256']
+    assert np.array_equal(oracle_lib.voronoi_subset(qc, qf), m)
+    rows = np.random.default_rng(3).choice(len(qf), min(len(qf), 1500), replace=False)
+    dots = np.abs(sum(qf[rows, k][:, None] * qc[None, :, k] for k in range(4)))
+    assert np.allclose(dots[np.arange(len(rows)), m[rows]], dots.max(1), rtol=0, atol=1e-15)
+    # every coarse orientation is its own cell's member when it is also a fine one
+    assert np.bincount(m, minlength=len(qc)).min() >= 1
