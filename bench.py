@@ -223,6 +223,108 @@ def bench_dense(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_learned(args):
+    """Learned orientation proposal at fine num_div 16 against the plain FP64 full search.
+
+    The current model is the true intensity (the situation of later iterations, where the
+    proposal pays). (1) One plain FP64 iteration at the coarse sampling gives each frame's
+    posterior over the coarse orientations: the training targets. (2) The network (polar
+    photon histogram -> coarse logits) is trained on 90% of the frames. (3) Timed: proposal
+    for every frame (histogram, forward, top-k with the probability it holds), frames holding
+    less than the threshold fall back to every coarse cell, and one refinement iteration over
+    the chosen Voronoi cells of the fine sampling, plus the model update. (4) Timed: the plain
+    FP64 iteration over every fine orientation from the same model. Reported: full-search
+    equivalent pairs/s of (3), its speedup over (4), the fallback rate, the posterior mass the
+    proposals capture on the held-out frames, and the relative error of the updated model."""
+    import torch
+    import dragonfly_amd as df
+    from dragonfly_amd import synth, proposal
+    cfg0 = synth.CONFIGS[args.config]
+    log('generating %s workload ...' % args.config)
+    prob = synth.make_problem(args.config, device='cuda', num_data=args.num_data, num_div=args.num_div,
+                              batch=1024)
+    cfg = prob['cfg']
+    det, mask = prob['det'], prob['det']['mask']
+    true_model = np.ascontiguousarray(prob['intens'])
+
+    def recon_for(num_div):
+        c = dict(cfg, num_div=num_div)
+        recon, itr = build_recon(prob, c, start=true_model)
+        return recon, itr
+
+    # (1) coarse posteriors
+    t0 = time.perf_counter()
+    recon, itr = recon_for(cfg['coarse_div'])
+    itr.params.iteration = 1
+    recon.run_iteration()
+    post = recon.prob_dense
+    D = itr.tot_num_data
+    recon.free()
+    log('coarse iteration + posteriors: %.1f s' % (time.perf_counter() - t0))
+    # (2) training
+    num_coarse = post.shape[1]
+    net = proposal.OrientationProposal(num_coarse, nbins=4096, width=1024, depth=3, seed=0)
+    off, pix, cnt = proposal.photon_lists(prob['frames'], mask)
+    bins = proposal.polar_bins(det['qvals'], mask)
+    X = net.features(off, pix, cnt, bins)
+    Y = torch.tensor(post, dtype=torch.float32, device='cuda')
+    ntr = int(0.9 * D)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = net.fit(X, Y, ntr, epochs=args.epochs, batch=1024, lr=1e-3, log=log)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del Y
+    # (3) learned iteration
+    recon, itr = recon_for(cfg['num_div'])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    top, mass = net.propose(X, D, cfg['top_k'], with_mass=True)
+    sets, fallback = proposal.refine_sets(top, mass, num_coarse, cfg['mass_threshold'])
+    itr.setup_refine(cfg['coarse_div'], sets)
+    recon.engine.set_refine(itr.num_coarse, itr.quat_mapping, itr.rel_off, itr.rel_quat)
+    itr.params.iteration = 1
+    recon.run_iteration()
+    torch.cuda.synchronize()
+    learned_s = time.perf_counter() - t0
+    st_learned = recon.last_stats['stage_ms']
+    model_learned = itr.model.model1.copy()
+    recon.free()
+    held = post[ntr:]
+    captured = float(np.take_along_axis(held, top[ntr:], 1).sum(1).mean())
+    # (4) plain FP64 full search
+    recon, itr = recon_for(cfg['num_div'])
+    itr.params.iteration = 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recon.run_iteration()
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    st_plain = recon.last_stats['stage_ms']
+    model_plain = itr.model.model1.copy()
+    recon.free()
+    R = itr.quat.num_rot
+    model_err = float(np.abs(model_learned - model_plain).max() / np.abs(model_plain).max())
+    out = {
+        'metric': METRIC, 'value': round(D * R / learned_s, 1), 'unit': 'frame×orientation pairs/s',
+        'n_gpus': 1, 'steps': 1, 'warmup': 0, 'ms_per_step': round(learned_s * 1e3, 1),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64 EMC / bf16 proposal network', 'data': 'synthetic',
+        'config': {'workload': '%s: %d frames, fine num_div %d (%d orientations), coarse num_div %d (%d), '
+                               'top-%d proposals, fallback below %.2f' % (args.config, D, cfg['num_div'], R,
+                                                                         cfg['coarse_div'], num_coarse,
+                                                                         cfg['top_k'], cfg['mass_threshold']),
+                   'path': 'learned proposal + sparse refinement', 'frames': D, 'num_rot': R},
+        'speedup_vs_fp64': round(plain_s / learned_s, 2), 'fp64_ms_per_step': round(plain_s * 1e3, 1),
+        'fallback_rate': round(float(fallback.mean()), 4), 'heldout_captured_mass': round(captured, 4),
+        'model_rel_err': float('%.3g' % model_err), 'train_s': round(train_s, 2),
+        'train_loss': [round(h, 4) for h in hist], 'stage_ms_learned': {k: round(v, 1) for k, v in st_learned.items()},
+        'stage_ms_fp64': {k: round(v, 1) for k, v in st_plain.items()},
+        'roofline': None, 'cpu_baseline': None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -235,6 +337,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--precision', type=int, default=2,
                     help='dense-frame configurations: 1 or 2 FP16 terms of log W')
+    ap.add_argument('--epochs', type=int, default=4, help='learned-proposal workload: training epochs')
     ap.add_argument('--no-fp64', action='store_true',
                     help='dense-frame configurations: skip the FP64 comparison run')
     ap.add_argument('--copy-calibration-mib', type=int, default=0,
@@ -257,6 +360,9 @@ def main():
         del a, b
     if synth.CONFIGS.get(args.config, {}).get('dense'):
         bench_dense(args)
+        return
+    if synth.CONFIGS.get(args.config, {}).get('learned'):
+        bench_learned(args)
         return
     t0 = time.perf_counter()
     log('generating %s workload ...' % args.config)

@@ -74,7 +74,7 @@ def lib():
             'emc_nn_colsum': (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_void_p]),
             'emc_nn_adam': (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_size_t, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t]),
             'emc_nn_polar_hist': (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t]),
-            'emc_nn_topk': (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+            'emc_nn_topk': (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
             'emc_set_precision': (C.c_int, [C.c_void_p, C.c_int]),
             'emc_set_refine': (C.c_int, [C.c_void_p, C.c_int, _ip, _lp, _ip]),
             'emc_set_frame_params': (C.c_int, [C.c_void_p, _dp, _dp, _u8p, _ip, _dp, C.c_double, C.c_int]),

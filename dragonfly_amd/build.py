@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libemc_hip.so')
 INCLUDE = os.path.join(os.path.dirname(HERE), 'include')
 
-SOURCES = ['emc_engine.hip', 'emc_kernels.hip', 'emc_mp.hip', 'emc_merge.hip', 'emc_nn.hip', 'quat_gen.cpp']
+SOURCES = ['emc_engine.hip', 'emc_kernels.hip', 'emc_mp.hip', 'emc_merge.hip', 'emc_nn.hip', 'emc_subset.hip', 'quat_gen.cpp']
 
 # -ffp-contract=off: no fused multiply-add anywhere, so every FP64 value keeps the
 # reference's rounding sequence. Packed FP32 ops are disabled for device code.
@@ -36,6 +36,12 @@ def build(verbose=False):
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError('hipcc failed building libemc_hip.so')
+    # every symbol must resolve now, not at the first call on a GPU machine
+    nm = subprocess.run(['nm', '-D', '--undefined-only', tmp], capture_output=True, text=True)
+    missing = [l.split()[-1] for l in nm.stdout.splitlines() if '_ZN3emc' in l]
+    if missing:
+        os.remove(tmp)
+        raise RuntimeError('libemc_hip.so has unresolved engine symbols: %s' % ', '.join(missing))
     os.replace(tmp, LIB)
     return LIB
 

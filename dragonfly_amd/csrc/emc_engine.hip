@@ -132,6 +132,7 @@ struct emc_engine {
     DevBuf<int> d_perm;
     // orientations (rots padded to Rp with identity)
     int num_rot = 0, Rp = 0;
+    std::vector<double> h_quats;
     DevBuf<double> quats, rots, logwr;
     // model
     long size = 0;
@@ -193,12 +194,6 @@ struct emc_engine {
     DevBuf<_Float16> klik, ktom, lw_hi, lw_lo, p16;
     DevBuf<double> center_t, center_base;
 
-    // refinement: coarse cell of every orientation and per-frame bit rows of allowed cells
-    bool refine = false;
-    int rel_words = 0;
-    DevBuf<int> qmap;
-    DevBuf<uint32_t> relbits;
-
     // brick merge (1, default) or per-wave voxel boxes (0); the brick form needs an
     // orientation-major view, which the frame-partitioned tomogram and the mixed path write
     int merge_mode = 1;
@@ -211,6 +206,30 @@ struct emc_engine {
     bool bricks_valid = false;
     DevBuf<int> brick_counts, brick_fill;
     DevBuf<long> brick_off;
+    // refinement: coarse cell of every orientation and per-frame bit rows of allowed cells
+    bool refine = false;
+    int rel_words = 0;
+    DevBuf<int> qmap;
+    DevBuf<uint32_t> relbits;
+    // sparse evaluation of the subsets (default with refinement; EMC_SUBSET=0 evaluates every
+    // pair and masks instead): orientations ordered by coarse cell in slots, cells padded to
+    // tiles of 64, (frame, tile) pairs
+    int subset_mode = 1;
+    bool subset_ready = false, subset_ran = false;
+    int num_slots = 0, num_tiles = 0, slot_C = 0;
+    long num_pairs = 0;
+    DevBuf<int> slot_rot, rot_slot, pair_frame, pair_tile, tile_pair, rbad_slot;
+    DevBuf<long> pair_off, tile_off;
+    DevBuf<double> rots_slot, quats_slot, logwr_slot, u_slot, psum_slot, Rsub;
+    std::vector<BrickChunk> slot_bricks;
+    bool slot_bricks_valid = false;
+    // frame photon lists of mask < 2 pixels sorted by engine pixel, with per-range offsets
+    bool fsort_valid = false;
+    int nranges = 0;
+    DevBuf<long> fsort_off;
+    DevBuf<int> fsort_pix, fsort_roff;
+    DevBuf<double> fsort_cnt;
+
 };
 
 namespace {
@@ -362,16 +381,19 @@ int build_lists(emc_engine *e) {
     e->max_count = max_count;
     e->lists_dirty = false;
     e->mp_built = false;
+    e->fsort_valid = false;
     return 0;
 }
 
 size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
 
-// Per-chunk brick item lists and work units of the brick merge (geometry only: rebuilt when
-// rotations, detector, model size or chunk width change).
-int build_bricks(emc_engine *e) {
+// Per-chunk brick item lists and work units of the brick merge (geometry only). `rots` holds
+// `total` rotations (a multiple of 64) walked in chunks of C; rotations at or past num_valid,
+// and slots with slot_rot < 0 when slot_rot is given, make no items.
+int build_bricks(emc_engine *e, std::vector<emc_engine::BrickChunk> &out, const double *rots, int total,
+                 int C, int num_valid, const int *slot_rot) {
     hipStream_t s = e->stream;
-    const int P = e->num_pix, C = e->C, Rn = e->num_rot;
+    const int P = e->num_pix;
     const int nbx = (int)((e->size + emc::kBrick - 1) / emc::kBrick);
     const size_t nb = (size_t)nbx * nbx * nbx;
     const int npatch = (P + 63) / 64;
@@ -381,52 +403,184 @@ int build_bricks(emc_engine *e) {
     if ((rc = e->brick_off.alloc(nb))) return rc;
     std::vector<int> counts(nb);
     std::vector<long> off(nb);
-    e->bricks.clear();
-    for (int r0 = 0; r0 < e->Rp; r0 += C) {
-        const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
-        e->bricks.emplace_back();
-        emc_engine::BrickChunk &bc = e->bricks.back();
+    out.clear();
+    for (int r0 = 0; r0 < total; r0 += C) {
+        const int Cc = (total - r0 < C) ? (total - r0) : C;
+        out.emplace_back();
+        emc_engine::BrickChunk &bc = out.back();
         bc.r0 = r0;
         bc.Cc = Cc;
         HIP_TRY(hipMemsetAsync(e->brick_counts.p, 0, nb * sizeof(int), s));
-        emc::launch_brick_items(s, e->rots.p, r0, Cc, Rn, e->qvals.p, e->mask.p, P, e->size, npatch, nbx, 0,
-                                e->brick_counts.p, nullptr, nullptr, nullptr);
+        emc::launch_brick_items(s, rots, r0, Cc, num_valid, e->qvals.p, e->mask.p, P, e->size, npatch, nbx, 0,
+                                e->brick_counts.p, nullptr, nullptr, nullptr, slot_rot);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(counts.data(), e->brick_counts.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        long total = 0;
+        long tot = 0;
         std::vector<int4> units;
         for (size_t b = 0; b < nb; ++b) {
-            off[b] = total;
+            off[b] = tot;
             for (int k = 0; k < counts[b]; k += emc::kUnitItems)
-                units.push_back(make_int4((int)b, (int)(total + k),
-                                          (int)(total + std::min(counts[b], k + emc::kUnitItems)), 0));
-            total += counts[b];
+                units.push_back(make_int4((int)b, (int)(tot + k),
+                                          (int)(tot + std::min(counts[b], k + emc::kUnitItems)), 0));
+            tot += counts[b];
         }
-        if (total >= INT32_MAX) return fail(EMC_E_ARG, "brick merge: too many work items");
-        if ((rc = bc.list.alloc(total > 0 ? total : 1))) return rc;
+        if (tot >= INT32_MAX) return fail(EMC_E_ARG, "brick merge: too many work items");
+        if ((rc = bc.list.alloc(tot > 0 ? tot : 1))) return rc;
         if ((rc = e->brick_off.upload(off.data(), nb, s))) return rc;
         HIP_TRY(hipMemsetAsync(e->brick_fill.p, 0, nb * sizeof(int), s));
-        emc::launch_brick_items(s, e->rots.p, r0, Cc, Rn, e->qvals.p, e->mask.p, P, e->size, npatch, nbx, 1,
-                                nullptr, e->brick_off.p, e->brick_fill.p, bc.list.p);
+        emc::launch_brick_items(s, rots, r0, Cc, num_valid, e->qvals.p, e->mask.p, P, e->size, npatch, nbx, 1,
+                                nullptr, e->brick_off.p, e->brick_fill.p, bc.list.p, slot_rot);
         HIP_TRY(hipGetLastError());
         bc.num_units = (int)units.size();
         if (!units.empty() && (rc = bc.units.upload(units.data(), units.size(), s))) return rc;
         HIP_TRY(hipStreamSynchronize(s));
     }
-    e->bricks_valid = true;
     return 0;
 }
 
-int merge_chunk(emc_engine *e, int chunk, const double *view, bool all_on, bool do_model,
-                bool do_weight) {
-    const emc_engine::BrickChunk &bc = e->bricks[chunk];
+int build_bricks(emc_engine *e) {
+    int rc = build_bricks(e, e->bricks, e->rots.p, e->Rp, e->C, e->num_rot, nullptr);
+    if (!rc) e->bricks_valid = true;
+    return rc;
+}
+
+int merge_chunk(const emc_engine *e, const emc_engine::BrickChunk &bc, const double *rots, int num_valid,
+                const double *view, const double *psum, bool all_on, bool do_model, bool do_weight) {
     const int nbx = (int)((e->size + emc::kBrick - 1) / emc::kBrick);
-    emc::launch_merge_bricks(e->stream, bc.units.p, bc.num_units, bc.list.p, e->rots.p, bc.r0, e->num_rot,
-                             view, e->psum_r.p, all_on, e->qvals.p, e->corr.p, e->mask.p, e->num_pix,
-                             e->size, (e->num_pix + 63) / 64, nbx, e->model2.p, e->weight.p, do_model,
-                             do_weight);
+    emc::launch_merge_bricks(e->stream, bc.units.p, bc.num_units, bc.list.p, rots, bc.r0, num_valid, view, psum,
+                             all_on, e->qvals.p, e->corr.p, e->mask.p, e->num_pix, e->size,
+                             (e->num_pix + 63) / 64, nbx, e->model2.p, e->weight.p, do_model, do_weight);
     return 0;
+}
+
+// Slot order and (frame, tile) pairs of the refinement subsets: orientations grouped by coarse
+// cell (ascending index inside a cell), every cell padded to whole tiles of 64; a frame's
+// pairs are the tiles of its (distinct) cells in ascending tile order; the tile-major copy of
+// the pair list (tile_off / tile_pair) lists each tile's pairs by ascending frame.
+int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel_off, const int *rel_quat) {
+    const int Rn = e->num_rot, N = e->num_data;
+    std::vector<int> cnt(num_coarse, 0), tile0(num_coarse + 1, 0);
+    for (int r = 0; r < Rn; ++r) ++cnt[qmap[r]];
+    for (int c = 0; c < num_coarse; ++c) tile0[c + 1] = tile0[c] + (cnt[c] + emc::kRTile - 1) / emc::kRTile;
+    const int T = tile0[num_coarse], S = T * emc::kRTile;
+    if (S <= 0) return fail(EMC_E_ARG, "emc_set_refine: no orientations");
+    std::vector<int> slot_rot(S, -1), rot_slot(Rn), fill(num_coarse, 0);
+    for (int r = 0; r < Rn; ++r) {
+        const int c = qmap[r];
+        const int sl = tile0[c] * emc::kRTile + fill[c]++;
+        slot_rot[sl] = r;
+        rot_slot[r] = sl;
+    }
+    std::vector<double> rots((size_t)S * 9, 0.), quats((size_t)S * 5, 0.), lw(S, 0.);
+    for (int sl = 0; sl < S; ++sl) {
+        const int r = slot_rot[sl];
+        if (r >= 0) {
+            make_rot(&e->h_quats[5 * (size_t)r], &rots[9 * (size_t)sl]);
+            for (int k = 0; k < 5; ++k) quats[5 * (size_t)sl + k] = e->h_quats[5 * (size_t)r + k];
+            lw[sl] = std::log(e->h_quats[5 * (size_t)r + 4]);
+        } else {
+            rots[9 * (size_t)sl] = rots[9 * (size_t)sl + 4] = rots[9 * (size_t)sl + 8] = 1.;
+        }
+    }
+    std::vector<long> poff(N + 1, 0);
+    std::vector<int> pframe, ptile, cells;
+    for (int d = 0; d < N; ++d) {
+        cells.assign(rel_quat + rel_off[d], rel_quat + rel_off[d + 1]);
+        std::sort(cells.begin(), cells.end());
+        cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+        for (int c : cells)
+            for (int t = tile0[c]; t < tile0[c + 1]; ++t) {
+                pframe.push_back(d);
+                ptile.push_back(t);
+            }
+        poff[d + 1] = (long)pframe.size();
+    }
+    const long K = (long)pframe.size();
+    if (K >= INT32_MAX) return fail(EMC_E_ARG, "emc_set_refine: too many (frame, tile) pairs");
+    std::vector<long> toff(T + 1, 0);
+    for (long k = 0; k < K; ++k) ++toff[ptile[k] + 1];
+    for (int t = 0; t < T; ++t) toff[t + 1] += toff[t];
+    std::vector<int> tpair(K > 0 ? K : 1);
+    std::vector<long> tfill(toff.begin(), toff.end() - 1);
+    for (long k = 0; k < K; ++k) tpair[tfill[ptile[k]]++] = (int)k;
+    if (pframe.empty()) {
+        pframe.push_back(0);
+        ptile.push_back(0);
+    }
+    int rc;
+    hipStream_t s = e->stream;
+    if ((rc = e->slot_rot.upload(slot_rot.data(), S, s))) return rc;
+    if ((rc = e->rot_slot.upload(rot_slot.data(), Rn, s))) return rc;
+    if ((rc = e->rots_slot.upload(rots.data(), rots.size(), s))) return rc;
+    if ((rc = e->quats_slot.upload(quats.data(), quats.size(), s))) return rc;
+    if ((rc = e->logwr_slot.upload(lw.data(), S, s))) return rc;
+    if ((rc = e->pair_off.upload(poff.data(), poff.size(), s))) return rc;
+    if ((rc = e->pair_frame.upload(pframe.data(), pframe.size(), s))) return rc;
+    if ((rc = e->pair_tile.upload(ptile.data(), ptile.size(), s))) return rc;
+    if ((rc = e->tile_off.upload(toff.data(), toff.size(), s))) return rc;
+    if ((rc = e->tile_pair.upload(tpair.data(), tpair.size(), s))) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    e->num_slots = S;
+    e->num_tiles = T;
+    e->num_pairs = K;
+    e->slot_bricks_valid = false;
+    e->subset_ready = true;
+    return 0;
+}
+
+// Photons of mask < 2 pixels per frame, sorted by engine pixel, with the offset (relative to
+// the frame's first photon) of the first photon at or past every range start p * kPairRange.
+int build_fsort(emc_engine *e) {
+    const int N = e->num_data, P = e->num_pix;
+    const int nr = (P + emc::kPairRange - 1) / emc::kPairRange;
+    const uint8_t *mask = e->h_mask.data();
+    std::vector<long> off(N + 1, 0);
+    std::vector<int> pix, roff((size_t)N * (nr + 1));
+    std::vector<double> cnt;
+    std::vector<std::pair<int, double>> fr;
+    for (int d = 0; d < N; ++d) {
+        fr.clear();
+        for (int k = 0; k < e->h_ones[d]; ++k) {
+            const int p = e->h_place_ones[e->h_ones_accum[d] + k];
+            if (mask[p] < 2) fr.emplace_back(e->inv[p], 1.);
+        }
+        for (int k = 0; k < e->h_multi[d]; ++k) {
+            const long j = e->h_multi_accum[d] + k;
+            const int p = e->h_place_multi[j];
+            if (mask[p] < 2) fr.emplace_back(e->inv[p], (double)e->h_count_multi[j]);
+        }
+        std::stable_sort(fr.begin(), fr.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        int q = 0;
+        for (int rg = 0; rg <= nr; ++rg) {
+            while (q < (int)fr.size() && fr[q].first < rg * emc::kPairRange) ++q;
+            roff[(size_t)d * (nr + 1) + rg] = q;
+        }
+        for (const auto &x : fr) {
+            pix.push_back(x.first);
+            cnt.push_back(x.second);
+        }
+        off[d + 1] = (long)pix.size();
+    }
+    if (pix.empty()) {
+        pix.push_back(0);
+        cnt.push_back(0.);
+    }
+    int rc;
+    hipStream_t s = e->stream;
+    if ((rc = e->fsort_off.upload(off.data(), off.size(), s))) return rc;
+    if ((rc = e->fsort_pix.upload(pix.data(), pix.size(), s))) return rc;
+    if ((rc = e->fsort_cnt.upload(cnt.data(), cnt.size(), s))) return rc;
+    if ((rc = e->fsort_roff.upload(roff.data(), roff.size(), s))) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    e->nranges = nr;
+    e->fsort_valid = true;
+    return 0;
+}
+
+int merge_chunk(emc_engine *e, int chunk, const double *view, bool all_on, bool do_model, bool do_weight) {
+    return merge_chunk(e, e->bricks[chunk], e->rots.p, e->num_rot, view, e->psum_r.p, all_on, do_model,
+                       do_weight);
 }
 
 // Work buffers. The posterior matrix R/P holds all frames x all orientations (the keep
@@ -528,6 +682,183 @@ int ensure_work(emc_engine *e) {
     return 0;
 }
 
+// One iteration over the refinement subsets only (pairs of frames and slot tiles). Same
+// outputs as emc_maximize's plain path; per-orientation outputs are mapped back to the
+// caller's orientation order.
+int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats) {
+    int rc;
+    hipStream_t s = e->stream;
+    const int N = e->num_data, Rn = e->num_rot, P = e->num_pix, S = e->num_slots;
+    const size_t vol = (size_t)e->size * e->size * e->size;
+    if (!e->fsort_valid && (rc = build_fsort(e))) return rc;
+    // memory: pair rows, one slot chunk of log W / view, per-slot and per-frame arrays
+    if ((rc = e->Rsub.alloc((size_t)std::max<long>(e->num_pairs, 1) * emc::kRTile))) return rc;
+    size_t freeb = 0, totalb = 0;
+    HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+    size_t budget = (freeb > ((size_t)4 << 30) ? freeb - ((size_t)4 << 30) : 0) + e->logw.n * sizeof(double);
+    budget = std::min(budget, (size_t)120 << 30);
+    size_t cmax = budget / ((size_t)P * sizeof(double));
+    if (e->max_chunk_cols > 0) cmax = std::min(cmax, (size_t)e->max_chunk_cols);
+    cmax = cmax / emc::kRTile * emc::kRTile;
+    if (cmax < (size_t)emc::kRTile) return fail(EMC_E_NOMEM, "no device memory for an orientation chunk");
+    const int C = (int)std::min(cmax, (size_t)S);
+    if (C != e->slot_C) e->slot_bricks_valid = false;
+    e->slot_C = C;
+    if (e->logw.n < (size_t)P * C) {
+        e->logw.release();
+        e->C = 0;
+        e->bricks_valid = false;
+        if ((rc = e->logw.alloc((size_t)P * C))) return rc;
+    }
+    const int ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
+    if ((rc = e->upart.alloc((size_t)ngroups * C))) return rc;
+    if ((rc = e->ubad.alloc((size_t)ngroups * C))) return rc;
+    if ((rc = e->u_slot.alloc(S))) return rc;
+    if ((rc = e->rbad_slot.alloc(S))) return rc;
+    if ((rc = e->psum_slot.alloc(S))) return rc;
+    if ((rc = e->u.alloc(e->Rp))) return rc;
+    if ((rc = e->rbad.alloc(e->Rp))) return rc;
+    if ((rc = e->psum_r.alloc(e->Rp))) return rc;
+    if ((rc = e->scalars.alloc(8))) return rc;
+    if ((rc = e->model_part.alloc(kModelParts))) return rc;
+    if ((rc = e->max_exp.alloc(N))) return rc;
+    if ((rc = e->p_norm.alloc(N))) return rc;
+    if ((rc = e->likelihood.alloc(N))) return rc;
+    if ((rc = e->info.alloc(N))) return rc;
+    if ((rc = e->psum_d.alloc(N))) return rc;
+    if ((rc = e->rmax.alloc(N))) return rc;
+    if ((rc = e->d_count.alloc(1))) return rc;
+
+    const int use_scale = opts->need_scaling && (opts->iteration > 1 || opts->known_scale);
+    const int want_psum_d = opts->need_scaling && opts->update_scale;
+    float ms[EMC_NUM_STAGES] = {};
+    auto timed = [&](int stage, auto &&fn) -> int {
+        HIP_TRY(hipEventRecord(e->ev[0], s));
+        int r = fn();
+        if (r) return r;
+        HIP_TRY(hipEventRecord(e->ev[1], s));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventSynchronize(e->ev[1]));
+        float a = 0.f;
+        HIP_TRY(hipEventElapsedTime(&a, e->ev[0], e->ev[1]));
+        ms[stage] += a;
+        return 0;
+    };
+    emc::launch_fill(s, e->model2.p, vol, 0.);
+    emc::launch_fill(s, e->weight.p, vol, 0.);
+    const uint8_t *dense = e->any_dense ? e->dense.p : nullptr;
+    for (int s0 = 0; s0 < S; s0 += C) {
+        const int Cc = std::min(C, S - s0);
+        if ((rc = timed(0, [&] {
+                 emc::launch_expand(s, e->rots_slot.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p, P,
+                                    s0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p);
+                 emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, s0, S, e->u_slot.p, e->rbad_slot.p);
+                 return 0;
+             })))
+            return rc;
+        if ((rc = timed(1, [&] {
+                 emc::launch_pair_likelihood(s, e->logw.p, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
+                                             e->pair_frame.p, e->pair_tile.p, e->num_pairs, s0 / emc::kRTile,
+                                             (s0 + Cc) / emc::kRTile, e->Rsub.p);
+                 return 0;
+             })))
+            return rc;
+    }
+    if ((rc = timed(2, [&] {
+             emc::launch_rescale(s, e->quats_slot.p, e->u_slot.p, S, e->mean_count, e->scalars.p);
+             return 0;
+         })))
+        return rc;
+    if ((rc = timed(3, [&] {
+             emc::launch_pair_max(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, e->slot_rot.p, e->logwr_slot.p,
+                                  e->u_slot.p, e->rbad_slot.p, e->scale.p, use_scale, e->scalars.p,
+                                  e->blacklist.p, dense, N, e->max_exp.p, e->rmax.p);
+             return 0;
+         })))
+        return rc;
+    if ((rc = timed(4, [&] {
+             emc::launch_pair_norm(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, e->slot_rot.p, e->beta.p,
+                                   e->blacklist.p, e->max_exp.p, e->rmax.p, N, e->p_norm.p);
+             return 0;
+         })))
+        return rc;
+    if ((rc = timed(5, [&] {
+             emc::launch_pair_posterior(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, e->slot_rot.p,
+                                        e->logwr_slot.p, e->u_slot.p, e->beta.p, e->sum_fact.p, e->blacklist.p,
+                                        e->max_exp.p, e->rmax.p, e->p_norm.p, e->scalars.p, want_psum_d, N,
+                                        e->likelihood.p, e->info.p, e->psum_d.p);
+             emc::launch_pair_psum(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p, e->num_tiles,
+                                   e->scale.p, opts->need_scaling, e->psum_slot.p);
+             emc::launch_slot_gather(s, e->psum_slot.p, e->rot_slot.p, Rn, e->psum_r.p);
+             emc::launch_slot_gather(s, e->u_slot.p, e->rot_slot.p, Rn, e->u.p);
+             return 0;
+         })))
+        return rc;
+
+    // merge: bricks over the slot order (padding slots make no items)
+    if (!e->slot_bricks_valid) {
+        if ((rc = timed(8, [&] {
+                 return build_bricks(e, e->slot_bricks, e->rots_slot.p, S, C, S, e->slot_rot.p);
+             })))
+            return rc;
+        e->slot_bricks_valid = true;
+    }
+    int unmerged = 0;
+    emc::launch_count_unmerged(s, e->psum_r.p, Rn, e->d_count.p);
+    HIP_TRY(hipMemcpyAsync(&unmerged, e->d_count.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    e->last_unmerged = unmerged;
+    const bool cached_weight = unmerged == 0;
+    if (cached_weight && (rc = timed(8, [&] {
+                              if (!e->weight_all_valid) {
+                                  int r2;
+                                  if ((r2 = e->weight_all.alloc(vol))) return r2;
+                                  for (const auto &bc : e->slot_bricks)
+                                      merge_chunk(e, bc, e->rots_slot.p, S, nullptr, e->psum_slot.p, true, false, true);
+                                  HIP_TRY(hipMemcpyAsync(e->weight_all.p, e->weight.p, vol * sizeof(double),
+                                                         hipMemcpyDeviceToDevice, s));
+                                  e->weight_all_valid = true;
+                              } else {
+                                  HIP_TRY(hipMemcpyAsync(e->weight.p, e->weight_all.p, vol * sizeof(double),
+                                                         hipMemcpyDeviceToDevice, s));
+                              }
+                              return 0;
+                          })))
+        return rc;
+    for (size_t k = 0; k < e->slot_bricks.size(); ++k) {
+        const auto &bc = e->slot_bricks[k];
+        if ((rc = timed(6, [&] {
+                 emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p,
+                                           e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
+                                           e->fsort_cnt.p, e->psum_slot.p, bc.r0 / emc::kRTile,
+                                           bc.Cc / emc::kRTile, P, e->logw.p);
+                 return 0;
+             })))
+            return rc;
+        if ((rc = timed(8, [&] {
+                 return merge_chunk(e, bc, e->rots_slot.p, S, e->logw.p, e->psum_slot.p, false, true,
+                                    !cached_weight);
+             })))
+            return rc;
+    }
+    emc::launch_frame_stats(s, e->likelihood.p, e->info.p, e->blacklist.p, N, N - e->num_blacklist, e->scalars.p);
+    if (want_psum_d) emc::launch_scale(s, e->fcounts.p, e->psum_d.p, e->blacklist.p, N, e->scale.p);
+    HIP_TRY(hipGetLastError());
+    double host_scalars[3];
+    HIP_TRY(hipMemcpyAsync(host_scalars, e->scalars.p, sizeof host_scalars, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    e->have_results = true;
+    e->subset_ran = true;
+    if (stats) {
+        stats->rescale = host_scalars[0];
+        stats->avg_likelihood = host_scalars[1];
+        stats->mutual_info = host_scalars[2];
+        for (int i = 0; i < EMC_NUM_STAGES; ++i) stats->stage_ms[i] = ms[i];
+        stats->num_chunks = (int)e->slot_bricks.size();
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -558,6 +889,7 @@ int emc_create(int device, emc_engine **out) {
     if (const char *v = getenv("EMC_TOMO_UNROLL")) e->tomo_unroll = atoi(v);
     if (const char *v = getenv("EMC_TOMO_SEQ")) e->tomo_seq = atoi(v);
     if (const char *v = getenv("EMC_MERGE_MODE")) e->merge_mode = atoi(v);
+    if (const char *v = getenv("EMC_SUBSET")) e->subset_mode = atoi(v);
 
     *out = e;
     return 0;
@@ -655,6 +987,7 @@ int emc_set_quats(emc_engine *e, int num_rot, const double *quats) {
         }
     }
     int rc;
+    e->h_quats.assign(quats, quats + 5 * (size_t)num_rot);
     if ((rc = e->quats.upload(quats, 5 * (size_t)num_rot, e->stream))) return rc;
     if ((rc = e->rots.upload(rot.data(), rot.size(), e->stream))) return rc;
     if ((rc = e->logwr.upload(lw.data(), lw.size(), e->stream))) return rc;
@@ -776,7 +1109,7 @@ int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->rel_words = words;
     e->refine = true;
-    return 0;
+    return build_subset(e, num_coarse, quat_mapping, rel_off, rel_quat);
 }
 
 int emc_set_precision(emc_engine *e, int mode) {
@@ -845,6 +1178,8 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     HIP_TRY(hipSetDevice(e->device));
     if (e->lists_dirty && (rc = build_lists(e))) return rc;
     const bool mp = e->precision > 0;
+    if (e->refine && e->subset_mode && e->subset_ready && !mp) return maximize_subset(e, opts, stats);
+    e->subset_ran = false;
     if (mp) {
         for (uint8_t v : e->h_dense)
             if (!v) return fail(EMC_E_STATE, "emc_maximize: the mixed-precision path needs dense frames");
@@ -1150,6 +1485,18 @@ int emc_get_field(emc_engine *e, int field, void *dst, size_t count) {
         case EMC_FIELD_PROB: {
             if (!e->have_results) return fail(EMC_E_STATE, "emc_get_field: no iteration has run");
             if (count != N * Rn) return fail(EMC_E_ARG, "emc_get_field: count %zu != %zu", count, N * Rn);
+            if (e->subset_ran) {
+                DevBuf<double> dense;
+                int rc;
+                if ((rc = dense.alloc(N * Rn))) return rc;
+                HIP_TRY(hipMemsetAsync(dense.p, 0, N * Rn * sizeof(double), e->stream));
+                emc::launch_pair_scatter(e->stream, e->Rsub.p, e->pair_frame.p, e->pair_tile.p, e->slot_rot.p,
+                                         e->num_pairs, (int)Rn, dense.p);
+                HIP_TRY(hipMemcpyAsync(dst, dense.p, N * Rn * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+                HIP_TRY(hipStreamSynchronize(e->stream));
+                dense.release();
+                return 0;
+            }
             // tile-major [Rp/64][N][64] to the caller's [N][num_rot]
             for (size_t j = 0; j * emc::kRTile < Rn; ++j) {
                 const size_t w = std::min((size_t)emc::kRTile, Rn - j * emc::kRTile);

@@ -52,7 +52,8 @@ void launch_tomogram_reduce_rmajor(hipStream_t s, const double *partial, int npa
 // ---- brick merge (emc_merge.hip) ----
 void launch_brick_items(hipStream_t s, const double *rots, int r0, int Cc, int num_rot,
                         const double *qvals, const uint8_t *mask, int num_pix, long size, int npatch,
-                        int nbx, int pass, int *counts, const long *off, int *fill, int *list);
+                        int nbx, int pass, int *counts, const long *off, int *fill, int *list,
+                        const int *slot_rot);
 void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const int *list,
                          const double *rots, int r0, int num_rot, const double *view,
                          const double *psum_r, int all_on, const double *qvals, const double *corr,
@@ -101,5 +102,32 @@ void launch_expand_mp(hipStream_t s, const double *rots, const double *model, lo
                       int *ubad);
 void launch_posterior_fp16(hipStream_t s, const double *P, int num_data, int r0, int C, size_t ldd,
                            double pscale, _Float16 *p16);
+
+// ---- sparse orientation subsets (emc_subset.hip) ----
+void launch_pair_likelihood(hipStream_t s, const double *logw, int num_pix, const long *off, const int *pix,
+                            const double *cnt, const int *pair_frame, const int *pair_tile, long num_pairs,
+                            int t0, int t1, double *G);
+void launch_pair_max(hipStream_t s, double *R, const long *pair_off, const int *pair_tile, const int *slot_rot,
+                     const double *logwr_slot, const double *u_slot, const int *rbad_slot, const double *scale,
+                     int use_scale, const double *scalars, const uint8_t *blacklist, const uint8_t *dense,
+                     int num_data, double *max_exp, int *rmax);
+void launch_pair_norm(hipStream_t s, const double *R, const long *pair_off, const int *pair_tile,
+                      const int *slot_rot, const double *beta, const uint8_t *blacklist, const double *max_exp,
+                      const int *rmax, int num_data, double *p_norm);
+void launch_pair_posterior(hipStream_t s, double *R, const long *pair_off, const int *pair_tile,
+                           const int *slot_rot, const double *logwr_slot, const double *u_slot,
+                           const double *beta, const double *sum_fact, const uint8_t *blacklist,
+                           const double *max_exp, const int *rmax, const double *p_norm, const double *scalars,
+                           int want_psum_d, int num_data, double *likelihood, double *info, double *psum_d);
+void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
+                      const int *pair_frame, int num_tiles, const double *scale, int need_scaling,
+                      double *psum_slot);
+void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
+                          const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
+                          const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
+                          double *view);
+void launch_pair_scatter(hipStream_t s, const double *P, const int *pair_frame, const int *pair_tile,
+                         const int *slot_rot, long num_pairs, int num_rot, double *out);
+void launch_slot_gather(hipStream_t s, const double *in, const int *rot_slot, int num_rot, double *out);
 
 }  // namespace emc

@@ -42,13 +42,14 @@ __global__ __launch_bounds__(kBlock) void brick_items_kernel(
     const double *__restrict__ rots, int r0, int Cc, int num_rot, const double *__restrict__ qvals,
     const uint8_t *__restrict__ mask, int num_pix, long size, int npatch, int nbx, int pass,
     int *__restrict__ counts, const long *__restrict__ off, int *__restrict__ fill,
-    int *__restrict__ list) {
+    int *__restrict__ list, const int *__restrict__ slot_rot) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long item = (long)blockIdx.x * kWavesPerBlock + wave;
     if (item >= (long)Cc * npatch) return;
     const int c = (int)(item / npatch), p = (int)(item % npatch);
     const int r = r0 + c;
     if (r >= num_rot) return;
+    if (slot_rot && slot_rot[r] < 0) return;  // padding slot of the sparse-subset order
     const int t = p * 64 + lane;
     Cell cell;
     bool ok = false;
@@ -140,11 +141,12 @@ __global__ __launch_bounds__(kBlock) void merge_brick_kernel(
 
 void launch_brick_items(hipStream_t s, const double *rots, int r0, int Cc, int num_rot,
                         const double *qvals, const uint8_t *mask, int num_pix, long size, int npatch,
-                        int nbx, int pass, int *counts, const long *off, int *fill, int *list) {
+                        int nbx, int pass, int *counts, const long *off, int *fill, int *list,
+                        const int *slot_rot) {
     const long items = (long)Cc * npatch;
     const unsigned grid = (unsigned)((items + kWavesPerBlock - 1) / kWavesPerBlock);
     brick_items_kernel<<<grid, kBlock, 0, s>>>(rots, r0, Cc, num_rot, qvals, mask, num_pix, size, npatch,
-                                               nbx, pass, counts, off, fill, list);
+                                               nbx, pass, counts, off, fill, list, slot_rot);
 }
 
 void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const int *list,

@@ -205,13 +205,22 @@ __global__ __launch_bounds__(kBlock) void nn_polar_hist_kernel(
 }
 
 // Indices of the k largest of z[row][0..n) in descending order (k <= 64), one wave per row:
-// k rounds of a wave-wide arg-max over the lane-strided row (ties: lower index).
+// k rounds of a wave-wide arg-max over the lane-strided row (ties: lower index). When mass is
+// given, mass[row] = the softmax probability of the selected entries.
 __global__ __launch_bounds__(kBlock) void nn_topk_kernel(const float *__restrict__ Z, size_t ldz, int rows,
-                                                         int n, int k, int *__restrict__ idx) {
+                                                         int n, int k, int *__restrict__ idx,
+                                                         float *__restrict__ mass) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = blockIdx.x * kWavesPerBlock + wave;
     if (row >= rows) return;
     const float *z = Z + (size_t)row * ldz;
+    float zmax = -FLT_MAX;
+    for (int j = lane; j < n; j += 64) zmax = fmaxf(zmax, z[j]);
+    for (int o = 32; o > 0; o >>= 1) zmax = fmaxf(zmax, __shfl_xor(zmax, o));
+    float zsum = 0.f;
+    for (int j = lane; j < n; j += 64) zsum += __expf(z[j] - zmax);
+    for (int o = 32; o > 0; o >>= 1) zsum += __shfl_xor(zsum, o);
+    float captured = 0.f;
     unsigned long long taken_lo = 0;  // lane-local bitmap over this lane's (up to 64) columns
     unsigned long long taken_hi = 0;
     for (int round = 0; round < k; ++round) {
@@ -241,7 +250,9 @@ __global__ __launch_bounds__(kBlock) void nn_topk_kernel(const float *__restrict
             else taken_hi |= 1ull << (slot - 64);
         }
         if (lane == 0) idx[(size_t)row * k + round] = (wi == INT_MAX) ? -1 : wi;
+        if (wi != INT_MAX) captured += __expf(wb - zmax);
     }
+    if (mass && lane == 0) mass[row] = captured / zsum;
 }
 
 // ---- launchers --------------------------------------------------------------------------------
@@ -312,8 +323,8 @@ void launch_nn_polar_hist(hipStream_t s, const long *off, const int *pix, const 
                                                                            num_frames, out, ldo);
 }
 
-void launch_nn_topk(hipStream_t s, const float *Z, size_t ldz, int rows, int n, int k, int *idx) {
-    nn_topk_kernel<<<(rows + kWavesPerBlock - 1) / kWavesPerBlock, kBlock, 0, s>>>(Z, ldz, rows, n, k, idx);
+void launch_nn_topk(hipStream_t s, const float *Z, size_t ldz, int rows, int n, int k, int *idx, float *mass) {
+    nn_topk_kernel<<<(rows + kWavesPerBlock - 1) / kWavesPerBlock, kBlock, 0, s>>>(Z, ldz, rows, n, k, idx, mass);
 }
 
 }  // namespace emc
@@ -379,9 +390,9 @@ int emc_nn_polar_hist(void *stream, const long *off, const int *pix, const doubl
     return 0;
 }
 
-int emc_nn_topk(void *stream, const float *Z, size_t ldz, int rows, int n, int k, int *idx) {
+int emc_nn_topk(void *stream, const float *Z, size_t ldz, int rows, int n, int k, int *idx, float *mass) {
     if (!Z || !idx || k < 1 || k > 64 || k > n || n > 128 * 64) return -1;
-    emc::launch_nn_topk((hipStream_t)stream, Z, ldz, rows, n, k, idx);
+    emc::launch_nn_topk((hipStream_t)stream, Z, ldz, rows, n, k, idx, mass);
     return 0;
 }
 

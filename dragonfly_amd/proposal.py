@@ -37,6 +37,22 @@ def polar_bins(qvals, mask, num_r=64, num_a=64):
     return ids
 
 
+def photon_lists(frames, mask):
+    """Per-frame photon lists of mask < 1 pixels from sparse frames (ones / multi arrays):
+    (off int64 (num_frames + 1), pix int32, cnt float64), frames in order."""
+    ones, multi = np.asarray(frames['ones']), np.asarray(frames['multi'])
+    D = len(ones)
+    fr = np.concatenate([np.repeat(np.arange(D), ones), np.repeat(np.arange(D), multi)])
+    pix = np.concatenate([frames['place_ones'], frames['place_multi']]).astype('i4')
+    cnt = np.concatenate([np.ones(len(frames['place_ones'])), np.asarray(frames['count_multi'], 'f8')])
+    keep = np.asarray(mask)[pix] < 1
+    fr, pix, cnt = fr[keep], pix[keep], cnt[keep]
+    order = np.argsort(fr, kind='stable')
+    off = np.zeros(D + 1, 'i8')
+    np.cumsum(np.bincount(fr, minlength=D), out=off[1:])
+    return off, pix[order], cnt[order]
+
+
 def _ptr(t):
     return C.c_void_p(t.data_ptr())
 
@@ -104,6 +120,10 @@ class OrientationProposal:
         """Logits (FP32, padded columns) of the first `rows` rows of X (rows padded to TILE)."""
         torch = self.torch
         L = _lib.lib()
+        if X.shape[0] % TILE or X.shape[1] != self.pdims[0]:
+            Xp = torch.zeros((_round(max(X.shape[0], 1), TILE), self.pdims[0]), dtype=X.dtype, device=self.dev)
+            Xp[:X.shape[0], :X.shape[1]] = X
+            X = Xp
         B = X.shape[0]
         acts = [X]
         h = X
@@ -196,10 +216,25 @@ class OrientationProposal:
     def logits(self, X, num):
         return self.forward(X, num)[:num, :self.num_out]
 
-    def propose(self, X, num, k):
-        """Top-k output indices (descending logit) of the first `num` rows: (num, k) int32."""
+    def propose(self, X, num, k, with_mass=False):
+        """Top-k output indices (descending logit) of the first `num` rows: (num, k) int32;
+        with_mass also returns the softmax probability each row's k outputs hold."""
         torch = self.torch
         Z = self.forward(X, num)
         idx = torch.empty((max(num, 1), k), dtype=torch.int32, device=self.dev)
-        _ok(_lib.lib().emc_nn_topk(_stream(), _ptr(Z), Z.shape[1], num, self.num_out, k, _ptr(idx)), 'top-k')
+        mass = torch.empty(max(num, 1), dtype=torch.float32, device=self.dev)
+        _ok(_lib.lib().emc_nn_topk(_stream(), _ptr(Z), Z.shape[1], num, self.num_out, k, _ptr(idx), _ptr(mass)),
+            'top-k')
+        if with_mass:
+            return idx[:num].cpu().numpy(), mass[:num].cpu().numpy()
         return idx[:num].cpu().numpy()
+
+
+def refine_sets(top, mass, num_coarse, threshold=0.99):
+    """Per-frame coarse sets for refinement from proposals: the frame's top-k cells, or every
+    cell (fall back to the full search) where the proposal holds less than `threshold` of the
+    network's probability. Returns (list of int32 arrays, fallback mask)."""
+    fallback = np.asarray(mass) < threshold
+    every = np.arange(num_coarse, dtype='i4')
+    sets = [every if fb else np.sort(np.asarray(t, 'i4')) for t, fb in zip(top, fallback)]
+    return sets, fallback

@@ -33,6 +33,11 @@ CONFIGS = {
     'cfg3': dict(size=251, atoms=400, atom_radius=30., sigma=1.5, nx=260, ny=257, gap=10,
                  stoprad=5., num_data=20000, mean_count=5000., phi=None, num_div=10,
                  need_scaling=0, beta='auto', iters=5, dense=True, alpha=0.5),
+    # learned orientation proposal: the 100k-frame data set, coarse num_div 4, fine num_div 16
+    'cfg5': dict(size=251, atoms=400, atom_radius=30., sigma=1.5, nx=260, ny=257, gap=10,
+                 stoprad=5., num_data=100000, mean_count=200., phi=(0.5, 1.5), num_div=16,
+                 coarse_div=4, top_k=16, mass_threshold=0.99, need_scaling=1, beta='auto',
+                 iters=5, learned=True),
 }
 
 

@@ -201,8 +201,9 @@ int emc_nn_adam(void *stream, float *w, const float *g, float *m, float *v, int 
  * written as log(1 + 1000 * fraction) to out[frame][0..nbins), BF16                        */
 int emc_nn_polar_hist(void *stream, const long *off, const int *pix, const double *cnt,
                       const int *bin_of, int nbins, int num_frames, void *out, size_t ldo);
-/* indices of the k (<= 64) largest of Z[row][0..n), descending, ties to the lower index      */
-int emc_nn_topk(void *stream, const float *Z, size_t ldz, int rows, int n, int k, int *idx);
+/* indices of the k (<= 64) largest of Z[row][0..n), descending, ties to the lower index;
+ * mass (may be null) receives the softmax probability those k entries hold                */
+int emc_nn_topk(void *stream, const float *Z, size_t ldz, int rows, int n, int k, int *idx, float *mass);
 
 #ifdef __cplusplus
 }

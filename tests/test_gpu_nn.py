@@ -82,10 +82,17 @@ def test_topk_and_polar_histogram():
     import ctypes as C
     from dragonfly_amd import _lib
     idx = torch.empty((rows, 7), dtype=torch.int32, device='cuda')
+    mass = torch.empty(rows, dtype=torch.float32, device='cuda')
+    Zs = Z / 3000.
+    Zp[:rows, :300] = Zs
     assert _lib.lib().emc_nn_topk(proposal._stream(), proposal._ptr(Zp), 512, rows, 300, 7,
-                                  proposal._ptr(idx)) == 0
-    want = np.argsort(-Z.cpu().numpy(), axis=1)[:, :7]
+                                  proposal._ptr(idx), proposal._ptr(mass)) == 0
+    zn = Zs.cpu().numpy().astype('f8')
+    want = np.argsort(-zn, axis=1)[:, :7]
     assert np.array_equal(idx.cpu().numpy(), want)
+    pz = np.exp(zn - zn.max(1, keepdims=True))
+    pz /= pz.sum(1, keepdims=True)
+    assert np.allclose(mass.cpu().numpy(), np.take_along_axis(pz, want, 1).sum(1), rtol=1e-4)
     # histogram: 3 frames over 10 pixels in 4 bins (pixel 9 masked)
     bin_of = np.array([0, 0, 1, 1, 2, 2, 3, 3, 3, -1], 'i4')
     off = np.array([0, 3, 3, 6], 'i8')

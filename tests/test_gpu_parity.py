@@ -234,9 +234,15 @@ def test_dense_integer_frames(tiny, zero_block):
     _compare(o, g, 'dense zero_block=%s' % zero_block)
 
 
-def test_refinement_subsets(tiny):
+@pytest.mark.parametrize('form', ['subset', 'subset-chunked', 'masked'])
+def test_refinement_subsets(tiny, monkeypatch, form):
     """emc_set_refine: per-frame coarse cells (Voronoi of quat_gen(1)), one frame with an
-    empty set, against the oracle's refinement branch."""
+    empty set, against the oracle's refinement branch; evaluated sparsely over (frame, tile)
+    pairs (in one or in 64-slot chunks), or over every pair with the excluded ones masked."""
+    if form == 'subset-chunked':
+        monkeypatch.setenv('EMC_MAX_CHUNK_COLS', '64')
+    if form == 'masked':
+        monkeypatch.setenv('EMC_SUBSET', '0')
     from oracle import core
     from dragonfly_amd import _lib
     qc = _lib.quat_gen(1)
@@ -257,5 +263,37 @@ def test_refinement_subsets(tiny):
     eng.set_refine(len(qc), qmap, off, ref['rel_quat'])
     g = P.run_engine(eng, flags)
     eng.close()
-    _compare(o, g, 'refinement')
+    _compare(o, g, 'refinement ' + form)
     assert g['rmax'][2] == -1
+    if form == 'subset-chunked':
+        assert g['num_chunks'] > 1
+
+
+def test_refinement_subsets_scaled_blacklisted(tiny_scaled):
+    """Sparse subsets with scale factors (iterations 1 and 2) and blacklisted frames."""
+    from oracle import core
+    from dragonfly_amd import _lib
+    p = tiny_scaled
+    qc = _lib.quat_gen(1)
+    qmap = _lib.voronoi_subset(qc, p['quats'])
+    rng = np.random.default_rng(12)
+    D = len(p['fparams']['beta'])
+    sets = [np.sort(rng.choice(len(qc), 20, replace=False)).astype('i4') for _ in range(D)]
+    off = np.zeros(D + 1, 'i8')
+    np.cumsum([len(x) for x in sets], out=off[1:])
+    ref = dict(quat_mapping=qmap, rel_off=off, rel_quat=np.concatenate(sets))
+    bl = np.zeros(D, 'u1')
+    bl[[1, 7]] = 1
+    fp = dict(p['fparams'], blacklist=bl, num_blacklist=2, scale=np.asarray(p['true_scale'], 'f8').copy())
+    eng = P.engine_for(dict(p, fparams=fp))
+    eng.set_refine(len(qc), qmap, off, ref['rel_quat'])
+    for it in (1, 2):
+        flags = dict(iteration=it, need_scaling=1, update_scale=1)
+        fo = dict(fp, scale=fp['scale'].copy())
+        o = core.maximize(p['det'], p['quats'], p['model1'], p['frames'], fo, flags, dense_prob=True,
+                          refine=ref)
+        o['scale'] = fo['scale']
+        eng.set_scale(fp['scale'])
+        g = P.run_engine(eng, flags)
+        _compare(o, g, 'refinement scaled it=%d' % it, scaling=True)
+    eng.close()
