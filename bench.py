@@ -164,9 +164,10 @@ def run_steps(recon, itr, warmup, steps, keep_models=False):
 
 
 def bench_dense(args):
-    """Dense integer frames: the mixed-precision path (FP16 products on matrix cores) timed,
-    then the plain FP64 path on the same frames from the same start model, with the model and
-    log-likelihood relative errors of every iteration."""
+    """A reduced-precision path timed, then the plain FP64 path on the same frames from the
+    same start model, with the model and log-likelihood relative errors of every iteration:
+    dense integer frames with FP16 products on matrix cores (precision 1 or 2), or any frames
+    with FP32 log W and posterior gathers (precision 3)."""
     import torch
     from dragonfly_amd import synth
     log('generating %s workload ...' % args.config)
@@ -174,7 +175,7 @@ def bench_dense(args):
                               num_div=args.num_div, batch=512)
     cfg = prob['cfg']
     mask = prob['det']['mask']
-    K = synth.dense_frames(prob['frames'], len(mask))
+    K = synth.dense_frames(prob['frames'], len(mask)) if cfg.get('dense') else None
     recon, itr = build_recon(prob, cfg, dense=K, precision=args.precision)
     start = itr.model.model1.copy()
     D, R = itr.tot_num_data, itr.quat.num_rot
@@ -200,21 +201,32 @@ def bench_dense(args):
     terms = 2 if args.precision == 2 else 1
     P = len(mask)
     launch_ms = st_mp['likelihood'] / args.steps / nch_mp
-    flops = 2. * D * (R / nch_mp) * P * terms
-    achieved = flops / (launch_ms * 1e-3) / 1e12
-    roof = dict(bound='mfma', achieved=round(achieved, 1), peak=2500., unit='TFLOP/s',
-                frac=round(achieved / 2500., 4), traffic=None, kernel='mp_gemm_kernel (likelihood)',
-                launch_ms=round(launch_ms, 3), algorithmic_flops=flops)
+    if args.precision in (1, 2):
+        flops = 2. * D * (R / nch_mp) * P * terms
+        achieved = flops / (launch_ms * 1e-3) / 1e12
+        roof = dict(bound='mfma', achieved=round(achieved, 1), peak=2500., unit='TFLOP/s',
+                    frac=round(achieved / 2500., 4), traffic=None, kernel='mp_gemm_kernel (likelihood)',
+                    launch_ms=round(launch_ms, 3), algorithmic_flops=flops)
+        path = 'mixed precision, %d-term FP16 log W' % terms
+        dtype = 'f16 products / f32 accumulate / f64 rest'
+    else:
+        roof = roofline(prob, itr, st_mp, args.steps, nch_mp, pairs, None)
+        for k in ('achieved', 'algorithmic_bytes'):
+            roof[k] = round(roof[k] / 2.)    # 4-byte log W / P gathers instead of 8
+        roof['frac'] = round(roof['achieved'] / HBM_PEAK_GBS, 4)
+        path = 'FP32 log W and posterior gathers, FP64 sums'
+        dtype = 'f32 gathers / f64 rest'
+    
     out = {
         'metric': METRIC, 'value': round(value, 1), 'unit': 'frame×orientation pairs/s',
         'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': round(el_mp / args.steps * 1e3, 2), 'higher_is_better': True,
-        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f16 products / f32 accumulate / f64 rest',
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': dtype,
         'data': 'synthetic',
-        'config': {'workload': '%s: %d dense frames, %d orientations (num_div %d), %d^3 model, %d pixels'
-                               % (args.config, D, R, cfg['num_div'], cfg['size'], P),
-                   'path': 'mixed precision, %d-term FP16 log W' % terms, 'frames': D, 'num_rot': R,
-                   'num_pix': P, 'mean_photons': round(float(K.sum() / D), 1)},
+        'config': {'workload': '%s: %d %s frames, %d orientations (num_div %d), %d^3 model, %d pixels'
+                               % (args.config, D, 'dense' if K is not None else 'sparse', R, cfg['num_div'],
+                                  cfg['size'], P),
+                   'path': path, 'frames': D, 'num_rot': R, 'num_pix': P},
         'stage_ms_mean': {k: round(v / args.steps, 2) for k, v in st_mp.items()},
         'roofline': roof, 'cpu_baseline': None,
     }
@@ -289,6 +301,8 @@ def bench_learned(args):
     learned_s = time.perf_counter() - t0
     st_learned = recon.last_stats['stage_ms']
     model_learned = itr.model.model1.copy()
+    diag_l = dict(loglik=itr.likelihood, rescale=float(itr.rescale[0]), rmax=recon.rmax.copy(),
+                  psum_r=recon.psum_r.copy(), u=recon.u[0].copy())
     recon.free()
     held = post[ntr:]
     captured = float(np.take_along_axis(held, top[ntr:], 1).sum(1).mean())
@@ -302,7 +316,23 @@ def bench_learned(args):
     plain_s = time.perf_counter() - t0
     st_plain = recon.last_stats['stage_ms']
     model_plain = itr.model.model1.copy()
+    diag_p = dict(loglik=itr.likelihood, rescale=float(itr.rescale[0]), rmax=recon.rmax.copy(),
+                  psum_r=recon.psum_r.copy(), u=recon.u[0].copy())
     recon.free()
+    # median angle between each frame's most likely orientation and its true orientation
+    qf = itr.quat.quats[:, :4]
+    qt = prob['frames']['true_quats'][:D, :4]
+
+    def med_angle(rmax):
+        ok = rmax >= 0
+        dots = np.abs((qf[rmax[ok]] * qt[ok]).sum(1)).clip(0., 1.)
+        return float(np.degrees(2. * np.median(np.arccos(dots))))
+    diag = dict(loglik_rel_err=abs(diag_l['loglik'] - diag_p['loglik']) / abs(diag_p['loglik']),
+                rmax_angle_learned_deg=med_angle(diag_l['rmax']), rmax_angle_fp64_deg=med_angle(diag_p['rmax']),
+                rescale_rel_err=abs(diag_l['rescale'] - diag_p['rescale']) / diag_p['rescale'],
+                rmax_agree=float(np.mean(diag_l['rmax'] == diag_p['rmax'])),
+                psum_r_rel_err=float(np.abs(diag_l['psum_r'] - diag_p['psum_r']).max() / np.abs(diag_p['psum_r']).max()),
+                u_rel_err=float(np.abs(diag_l['u'] - diag_p['u']).max() / np.abs(diag_p['u']).max()))
     R = itr.quat.num_rot
     model_err = float(np.abs(model_learned - model_plain).max() / np.abs(model_plain).max())
     out = {
@@ -318,6 +348,7 @@ def bench_learned(args):
         'speedup_vs_fp64': round(plain_s / learned_s, 2), 'fp64_ms_per_step': round(plain_s * 1e3, 1),
         'fallback_rate': round(float(fallback.mean()), 4), 'heldout_captured_mass': round(captured, 4),
         'model_rel_err': float('%.3g' % model_err), 'train_s': round(train_s, 2),
+        'agreement_vs_fp64': {k: float('%.3g' % v) for k, v in diag.items()},
         'train_loss': [round(h, 4) for h in hist], 'stage_ms_learned': {k: round(v, 1) for k, v in st_learned.items()},
         'stage_ms_fp64': {k: round(v, 1) for k, v in st_plain.items()},
         'roofline': None, 'cpu_baseline': None,
@@ -335,8 +366,9 @@ def main():
     ap.add_argument('--num-div', type=int, default=None)
     ap.add_argument('--cpu-rot', type=int, default=3072, help='orientations in the CPU sample')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--precision', type=int, default=2,
-                    help='dense-frame configurations: 1 or 2 FP16 terms of log W')
+    ap.add_argument('--precision', type=int, default=None,
+                    help='dense-frame configurations: 1 (default) or 2 FP16 terms of log W; 3 (any '
+                    'configuration): FP32 log W and posterior gathers, compared with FP64')
     ap.add_argument('--epochs', type=int, default=4, help='learned-proposal workload: training epochs')
     ap.add_argument('--no-fp64', action='store_true',
                     help='dense-frame configurations: skip the FP64 comparison run')
@@ -358,7 +390,9 @@ def main():
         torch.cuda.synchronize()
         log('copy calibration: %d MiB read + %d MiB written' % (args.copy_calibration_mib, args.copy_calibration_mib))
         del a, b
-    if synth.CONFIGS.get(args.config, {}).get('dense'):
+    if args.precision is None:
+        args.precision = 1 if synth.CONFIGS.get(args.config, {}).get('dense') else 0
+    if synth.CONFIGS.get(args.config, {}).get('dense') or args.precision == 3:
         bench_dense(args)
         return
     if synth.CONFIGS.get(args.config, {}).get('learned'):

@@ -206,8 +206,10 @@ class Engine:
         _check(lib().emc_set_refine(self._h, int(num_coarse), _ptr(qm, _ip), _ptr(ro, _lp), _ptr(rq, _ip)))
 
     def set_precision(self, mode):
-        """0: plain FP64; 1: FP16 log W on matrix cores (dense frames); 2: FP16 hi + lo."""
+        """0: plain FP64; 1: FP16 log W on matrix cores (dense frames); 2: FP16 hi + lo;
+        3: FP32 log W and posterior gathers (any frames)."""
         _check(lib().emc_set_precision(self._h, int(mode)))
+
     def set_frame_params(self, scale, beta, blacklist, fcounts, sum_fact, mean_count, num_blacklist=0):
         a = [np.ascontiguousarray(x, dtype=t) for x, t in (
             (scale, 'f8'), (beta, 'f8'), (blacklist, 'u1'), (fcounts, 'i4'), (sum_fact, 'f8'))]

@@ -37,4 +37,12 @@ constexpr double kProbMin = 1.e-6;
 constexpr double kPdiffThresh = 14.;
 constexpr double kMaxExpStart = -1.e100;
 
+// A dispatch's grid size is counted in work-items in 32 bits: a 1-D launch of blocks x kBlock
+// work-items past 2^32 wraps silently and drops the tail. Kernels whose work can grow past
+// that (pairs x 64, items x 64) take a capped grid and stride over their work.
+constexpr long kMaxStrideBlocks = 1L << 20;
+inline unsigned stride_grid(long blocks) {
+    return (unsigned)(blocks < kMaxStrideBlocks ? blocks : kMaxStrideBlocks);
+}
+
 }  // namespace emc

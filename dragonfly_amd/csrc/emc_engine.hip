@@ -192,6 +192,8 @@ struct emc_engine {
     size_t mp_rows_d = 0, mp_rows_t = 0, mp_rows_c = 0;
     bool mp_built = false;
     DevBuf<_Float16> klik, ktom, lw_hi, lw_lo, p16;
+    // FP32 gathers (precision 3): log W of one chunk and the chunk's posterior in FP32
+    DevBuf<float> logw32, p32;
     DevBuf<double> center_t, center_base;
 
     // brick merge (1, default) or per-wave voxel boxes (0); the brick form needs an
@@ -215,7 +217,12 @@ struct emc_engine {
     // pair and masks instead): orientations ordered by coarse cell in slots, cells padded to
     // tiles of 64, (frame, tile) pairs
     int subset_mode = 1;
+    double dense_tile_frac = 0.3;     // tiles taken by more than this fraction of frames are gathered per pixel
     bool subset_ready = false, subset_ran = false;
+    std::vector<int> h_dense_tiles;
+    DevBuf<int> dense_tiles, tile_cell, tile_sub, pair_of;
+    DevBuf<uint8_t> tile_dense;
+    int num_cells = 0;
     int num_slots = 0, num_tiles = 0, slot_C = 0;
     long num_pairs = 0;
     DevBuf<int> slot_rot, rot_slot, pair_frame, pair_tile, tile_pair, rbad_slot;
@@ -508,8 +515,37 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
         pframe.push_back(0);
         ptile.push_back(0);
     }
+    // tiles most frames take are gathered per pixel: per-tile cell and index in the cell, and
+    // the first pair of every (frame, cell)
+    std::vector<int> tcell(T), tsub(T), dense;
+    std::vector<uint8_t> tdense(T, 0);
+    for (int c = 0; c < num_coarse; ++c)
+        for (int t = tile0[c]; t < tile0[c + 1]; ++t) {
+            tcell[t] = c;
+            tsub[t] = t - tile0[c];
+        }
+    for (int t = 0; t < T; ++t)
+        if ((double)(toff[t + 1] - toff[t]) > e->dense_tile_frac * N) {
+            tdense[t] = 1;
+            dense.push_back(t);
+        }
+    std::vector<int> pof;
+    if (!dense.empty()) {
+        pof.assign((size_t)N * num_coarse, -1);
+        for (int d = 0; d < N; ++d)
+            for (long k = poff[d + 1] - 1; k >= poff[d]; --k) pof[(size_t)d * num_coarse + tcell[ptile[k]]] = (int)k;
+    } else {
+        pof.assign(1, -1);
+    }
     int rc;
     hipStream_t s = e->stream;
+    if ((rc = e->tile_cell.upload(tcell.data(), T, s))) return rc;
+    if ((rc = e->tile_sub.upload(tsub.data(), T, s))) return rc;
+    if ((rc = e->tile_dense.upload(tdense.data(), T, s))) return rc;
+    if ((rc = e->pair_of.upload(pof.data(), pof.size(), s))) return rc;
+    if (!dense.empty() && (rc = e->dense_tiles.upload(dense.data(), dense.size(), s))) return rc;
+    e->h_dense_tiles = dense;
+    e->num_cells = num_coarse;
     if ((rc = e->slot_rot.upload(slot_rot.data(), S, s))) return rc;
     if ((rc = e->rot_slot.upload(rot_slot.data(), Rn, s))) return rc;
     if ((rc = e->rots_slot.upload(rots.data(), rots.size(), s))) return rc;
@@ -598,7 +634,9 @@ int ensure_work(emc_engine *e) {
     if (e->logw.n) budget += e->logw.n * sizeof(double);
     if (e->tom_partial.n) budget += e->tom_partial.n * sizeof(double);
     budget += (e->lw_hi.n + e->lw_lo.n + e->p16.n) * sizeof(_Float16);
-    const bool mp = e->precision > 0;
+    const bool mp = e->precision == 1 || e->precision == 2;
+    const bool g32 = e->precision == 3;
+    budget += (e->logw32.n + e->p32.n) * sizeof(float);
     if (mp) {
         e->mp_ldp = round_up(P, emc::kGemmBK);
         e->mp_ldd = round_up(N, emc::kGemmBK);
@@ -613,7 +651,8 @@ int ensure_work(emc_engine *e) {
     // tomogram (plain path) or the FP16 log W rows and posterior column (mixed path)
     const size_t per_col = mp ? P * sizeof(double) + (e->precision == 2 ? 2 : 1) * e->mp_ldp * sizeof(_Float16) +
                                     e->mp_ldd * sizeof(_Float16)
-                              : P * sizeof(double) * (1 + (e->tomo_tile_mode == 2 ? e->tom_parts : 0));
+                              : P * sizeof(double) * (1 + (e->tomo_tile_mode == 2 ? e->tom_parts : 0)) +
+                                    (g32 ? P * sizeof(float) + N * sizeof(float) : 0);
     size_t cmax = budget / per_col;
     if (e->max_chunk_cols > 0 && cmax > (size_t)e->max_chunk_cols) cmax = e->max_chunk_cols;
     cmax = cmax / emc::kRTile * emc::kRTile;
@@ -662,6 +701,15 @@ int ensure_work(emc_engine *e) {
         e->mp_built = false;
         e->mp_rows_c = 0;
         if (e->tomo_tile_mode == 2 && (rc = e->tom_partial.alloc((size_t)e->tom_parts * P * (size_t)C))) return rc;
+    }
+    if (g32) {
+        // 128-wide tiles: room for the chunk rounded up to 128 orientations
+        const size_t C128 = round_up((size_t)e->C, 128);
+        if ((rc = e->logw32.alloc(P * C128))) return rc;
+        if ((rc = e->p32.alloc(N * C128))) return rc;
+    } else {
+        e->logw32.release();
+        e->p32.release();
     }
     const size_t ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
     if ((rc = e->upart.alloc(ngroups * (size_t)e->C))) return rc;
@@ -828,10 +876,19 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     for (size_t k = 0; k < e->slot_bricks.size(); ++k) {
         const auto &bc = e->slot_bricks[k];
         if ((rc = timed(6, [&] {
+                 const int t0 = bc.r0 / emc::kRTile, t1 = (bc.r0 + bc.Cc) / emc::kRTile;
                  emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p,
                                            e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
-                                           e->fsort_cnt.p, e->psum_slot.p, bc.r0 / emc::kRTile,
-                                           bc.Cc / emc::kRTile, P, e->logw.p);
+                                           e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P, e->tile_dense.p,
+                                           e->logw.p);
+                 const auto &dt = e->h_dense_tiles;
+                 const long lo = std::lower_bound(dt.begin(), dt.end(), t0) - dt.begin();
+                 const long hi = std::lower_bound(dt.begin(), dt.end(), t1) - dt.begin();
+                 if (hi > lo)
+                     emc::launch_pair_tomogram_dense(s, e->Rsub.p, e->dense_tiles.p + lo, (int)(hi - lo),
+                                                     e->tile_cell.p, e->tile_sub.p, e->pair_of.p, e->num_cells,
+                                                     e->tom_off.p, e->tom_frame.p, e->tom_cnt.p, e->psum_slot.p,
+                                                     t0, P, e->logw.p);
                  return 0;
              })))
             return rc;
@@ -890,6 +947,7 @@ int emc_create(int device, emc_engine **out) {
     if (const char *v = getenv("EMC_TOMO_SEQ")) e->tomo_seq = atoi(v);
     if (const char *v = getenv("EMC_MERGE_MODE")) e->merge_mode = atoi(v);
     if (const char *v = getenv("EMC_SUBSET")) e->subset_mode = atoi(v);
+    if (const char *v = getenv("EMC_DENSE_TILE_FRAC")) e->dense_tile_frac = atof(v);
 
     *out = e;
     return 0;
@@ -1114,7 +1172,7 @@ int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const
 
 int emc_set_precision(emc_engine *e, int mode) {
     if (!e) return fail(EMC_E_ARG, "null engine");
-    if (mode < 0 || mode > 2) return fail(EMC_E_ARG, "emc_set_precision: mode %d (0, 1 or 2)", mode);
+    if (mode < 0 || mode > 3) return fail(EMC_E_ARG, "emc_set_precision: mode %d (0 to 3)", mode);
     if (mode != e->precision) {
         e->precision = mode;
         e->mp_built = false;
@@ -1177,8 +1235,10 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     if (!opts) return fail(EMC_E_ARG, "emc_maximize: null options");
     HIP_TRY(hipSetDevice(e->device));
     if (e->lists_dirty && (rc = build_lists(e))) return rc;
-    const bool mp = e->precision > 0;
-    if (e->refine && e->subset_mode && e->subset_ready && !mp) return maximize_subset(e, opts, stats);
+    const bool mp = e->precision == 1 || e->precision == 2;
+    const bool g32 = e->precision == 3;
+    if (g32) e->tomo_tile_mode = 2;  // the FP32 posterior feeds the frame-partitioned tomogram
+    if (e->refine && e->subset_mode && e->subset_ready && !mp && !g32) return maximize_subset(e, opts, stats);
     e->subset_ran = false;
     if (mp) {
         for (uint8_t v : e->h_dense)
@@ -1224,6 +1284,9 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
             emc::launch_expand_mp(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
                                   e->center_t.p, P, r0, Cc, kPixPerGroup, e->mp_ldp, e->precision == 2,
                                   e->lw_hi.p, e->lw_lo.p, e->upart.p, e->ubad.p);
+        else if (g32)
+            emc::launch_expand(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
+                               P, r0, Cc, kPixPerGroup, e->logw32.p, e->upart.p, e->ubad.p);
         else
             emc::launch_expand(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
                                P, r0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p);
@@ -1237,6 +1300,9 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
                                     e->R.p + (size_t)(r0 / emc::kRTile) * ts, ts, e->center_base.p,
                                     nullptr, 0, 1.))
                 return fail(EMC_E_STATE, "likelihood product: shape does not fit the tiles");
+        } else if (g32) {
+            emc::launch_likelihood(s, e->logw32.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
+                                   e->blacklist.p, N, e->R.p, r0, e->lik_tiles, e->lik_unroll);
         } else {
             emc::launch_likelihood(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
                                    e->blacklist.p, N, e->R.p, r0, e->lik_tiles, e->lik_unroll);
@@ -1340,6 +1406,10 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
                                     P, e->logw.p, (size_t)P, nullptr, e->psum_r.p + r0, Rn - r0,
                                     1. / pscale))
                 return fail(EMC_E_STATE, "tomogram product: shape does not fit the tiles");
+        } else if (g32) {
+            emc::launch_to_f32(s, e->R.p, N, r0, Cc, e->p32.p);
+            emc::launch_tomogram_parts_f32(s, e->p32.p, N, Cc, e->tom_off2.p, e->tom_parts, e->tom_frame.p,
+                                           e->tom_cnt.p, P, e->tom_partial.p);
         } else if (parts)
             emc::launch_tomogram_parts(s, e->R.p, N, r0, Cc, e->tom_off2.p, e->tom_parts,
                                        e->tom_frame.p, e->tom_cnt.p, e->psum_r.p, Rn, P,

@@ -11,6 +11,16 @@ namespace emc {
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad);
+void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
+                   const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
+                   int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad);
+void launch_to_f32(hipStream_t s, const double *P, int num_data, int r0, int C, float *out);
+void launch_likelihood(hipStream_t s, const float *logw, int C, int num_pix, const long *off,
+                       const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
+                       double *G, int r0, int tiles, int unroll);
+void launch_tomogram_parts_f32(hipStream_t s, const float *P32, int num_data, int C, const long *toff2,
+                               int nparts, const int *tframe, const double *tcnt, int num_pix,
+                               double *partial);
 void launch_u_reduce(hipStream_t s, const double *upart, const int *ubad, int ngroups, int C,
                      int r0, int num_rot, double *u, int *rbad);
 void launch_rescale(hipStream_t s, const double *quats, const double *u, int num_rot,
@@ -125,7 +135,11 @@ void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, cons
 void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                           const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
                           const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
-                          double *view);
+                          const uint8_t *tile_dense, double *view);
+void launch_pair_tomogram_dense(hipStream_t s, const double *P, const int *dense_tiles, int ndense,
+                                const int *tile_cell, const int *tile_sub, const int *pair_of, int num_cells,
+                                const long *toff, const int *tframe, const double *tcnt, const double *psum_slot,
+                                int t0, int num_pix, double *view);
 void launch_pair_scatter(hipStream_t s, const double *P, const int *pair_frame, const int *pair_tile,
                          const int *slot_rot, long num_pairs, int num_rot, double *out);
 void launch_slot_gather(hipStream_t s, const double *in, const int *rot_slot, int num_rot, double *out);

@@ -45,11 +45,12 @@ __device__ __forceinline__ void merge_value(double *__restrict__ model2, double 
 // Block = 64 orientations x one pixel group. Compute with lanes along pixels (neighbouring
 // pixels hit neighbouring voxels), transpose through LDS, store with lanes along
 // orientations (coalesced rows of logW). u_r (max_prob.c:36-38) sums W over mask < 1.
+template <typename T, int kW>
 __global__ __launch_bounds__(kBlock) void expand_kernel(
     const double *__restrict__ rots, const double *__restrict__ model, long size,
     const double *__restrict__ qvals, const double *__restrict__ corr,
     const uint8_t *__restrict__ mask, int num_pix, int r0, int C, int pix_per_group,
-    double *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad) {
+    T *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad) {
     __shared__ double s_rot[kRTile * 9];
     __shared__ double s_tile[64][kRTile + 1];
     __shared__ double s_u[kWavesPerBlock][kRTile];
@@ -85,7 +86,9 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
                     uacc += W;
                     bad |= !(W > 0.);
                 }
-                logw[(size_t)(c0 >> 6) * ((size_t)num_pix * kRTile) + (size_t)tt * kRTile + lane] = log(W);
+                // tile-major with kW-wide tiles: (c / kW) * num_pix * kW + t * kW + c % kW
+                const int c = c0 + lane;
+                logw[(size_t)(c / kW) * ((size_t)num_pix * kW) + (size_t)tt * kW + (c % kW)] = (T)log(W);
             }
         }
         __syncthreads();
@@ -136,9 +139,9 @@ __global__ __launch_bounds__(kBlock) void rescale_kernel(const double *__restric
 // One wavefront = one frame x NT*64 orientations; the frame's (pixel, count) list is read
 // with scalar loads (wave-uniform) and each photon costs NT 512-byte row reads of logW.
 // Row loads of several photons are issued before the additions, which stay in list order.
-template <int NT, int U>
+template <int NT, int U, typename T>
 __global__ __launch_bounds__(kBlock) void likelihood_kernel(
-    const double *__restrict__ logw, int C, size_t tsW, const long *__restrict__ off,
+    const T *__restrict__ logw, int C, size_t tsW, const long *__restrict__ off,
     const int *__restrict__ pix, const double *__restrict__ cnt,
     const uint8_t *__restrict__ blacklist, int num_data, double *__restrict__ G, size_t tsG,
     int r0) {
@@ -148,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void likelihood_kernel(
     if (d >= num_data || blacklist[d]) return;
     const int c0 = blockIdx.y * (kRTile * NT);
     const int nt = min(NT, (C - c0) / kRTile);
-    const double *__restrict__ col = logw + (size_t)(c0 >> 6) * tsW + lane;
+    const T *__restrict__ col = logw + (size_t)(c0 >> 6) * tsW + lane;
     const long k1 = off[d + 1];
     long k = off[d];
     double acc[NT];
@@ -160,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void likelihood_kernel(
         for (int u = 0; u < U; ++u) {
             const size_t row = (size_t)pix[k + u] * kRTile;
 #pragma unroll
-            for (int j = 0; j < NT; ++j) v[u][j] = (j < nt) ? col[row + tsW * j] : 0.;
+            for (int j = 0; j < NT; ++j) v[u][j] = (j < nt) ? (double)col[row + tsW * j] : 0.;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -171,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void likelihood_kernel(
         const size_t row = (size_t)pix[k] * kRTile;
 #pragma unroll
         for (int j = 0; j < NT; ++j)
-            if (j < nt) acc[j] += cnt[k] * col[row + tsW * j];
+            if (j < nt) acc[j] += cnt[k] * (double)col[row + tsW * j];
     }
 #pragma unroll
     for (int j = 0; j < NT; ++j)
@@ -426,9 +429,9 @@ __global__ __launch_bounds__(kBlock) void tomogram_kernel(
 // kSeq orders the blocks so that each XCD walks its ranges one after another (all pixels of
 // range x, then of range x + 8, ...) instead of interleaving them, which keeps one range's
 // P slab in that XCD's L2 at a time; nparts is then a multiple of 8.
-template <int U, bool kSeq>
+template <int U, bool kSeq, typename T>
 __global__ __launch_bounds__(kBlock) void tomogram_part_kernel(
-    const double *__restrict__ P, size_t tsP, int r0, int C, const long *__restrict__ toff2,
+    const T *__restrict__ P, size_t tsP, int r0, int C, const long *__restrict__ toff2,
     int nparts, const int *__restrict__ tframe, const double *__restrict__ tcnt, int num_pix,
     int ngroups, double *__restrict__ partial) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -445,7 +448,7 @@ __global__ __launch_bounds__(kBlock) void tomogram_part_kernel(
     const int t = group * kWavesPerBlock + wave;
     if (t >= num_pix) return;
     const int c0 = blockIdx.y * kRTile;
-    const double *__restrict__ col = P + (size_t)((r0 + c0) >> 6) * tsP + lane;
+    const T *__restrict__ col = P + (size_t)((r0 + c0) >> 6) * tsP + lane;
     const size_t at = (size_t)t * nparts + part;
     long k = toff2[at];
     const long k1 = toff2[at + 1];
@@ -453,13 +456,13 @@ __global__ __launch_bounds__(kBlock) void tomogram_part_kernel(
     for (; k + U <= k1; k += U) {
         double v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = col[(size_t)tframe[k + u] * kRTile];
+        for (int u = 0; u < U; ++u) v[u] = (double)col[(size_t)tframe[k + u] * kRTile];
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (v[u] > kProbMin) acc += tcnt[k + u] * v[u];
     }
     for (; k < k1; ++k) {
-        const double p = col[(size_t)tframe[k] * kRTile];
+        const double p = (double)col[(size_t)tframe[k] * kRTile];
         if (p > kProbMin) acc += tcnt[k] * p;
     }
     const size_t tsW = (size_t)num_pix * kRTile, tsPart = tsW * (C / kRTile);
@@ -931,8 +934,119 @@ void launch_expand(hipStream_t s, const double *rots, const double *model, long 
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad) {
     dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
-    expand_kernel<<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0, C,
-                                          pix_per_group, logw, upart, ubad);
+    expand_kernel<double, kRTile><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0, C,
+                                                          pix_per_group, logw, upart, ubad);
+}
+
+void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
+                   const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
+                   int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad) {
+    dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
+    expand_kernel<float, 2 * kRTile><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0,
+                                                             C, pix_per_group, logw, upart, ubad);
+}
+
+// FP32 copy of one orientation chunk of the tile-major posterior in 128-wide tiles:
+// out[(c / 128) * N * 128 + d * 128 + c % 128] = P[((r0 + c) / 64) * N * 64 + d * 64 + c % 64]
+// for chunk orientations c < C (r0 a multiple of 64; columns of the last tile past C are 0).
+__global__ void to_f32_w128_kernel(const double *__restrict__ P, int num_data, int r0, int C,
+                                   float *__restrict__ out) {
+    const size_t n = (size_t)num_data * ((C + 127) / 128) * 128;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int cw = (int)(i % 128);
+        const size_t rest = i / 128;
+        const int d = (int)(rest % num_data);
+        const int c = (int)(rest / num_data) * 128 + cw;
+        out[i] = c < C ? (float)P[(size_t)((r0 + c) / kRTile) * num_data * kRTile + (size_t)d * kRTile + (c % kRTile)]
+                       : 0.f;
+    }
+}
+
+void launch_to_f32(hipStream_t s, const double *P, int num_data, int r0, int C, float *out) {
+    const size_t n = (size_t)num_data * ((C + 127) / 128) * 128;
+    to_f32_w128_kernel<<<grid_cap(cdiv(n, 256)), 256, 0, s>>>(P, num_data, r0, C, out);
+}
+
+// Likelihood with FP32 log W in 128-wide tiles: one wavefront = one frame x 128 orientations,
+// every lane reads two neighbouring orientations (one 8-byte load, 512 bytes per row) and
+// keeps two FP64 sums; writes the 64-wide tile-major G of orientations < C.
+template <int U>
+__global__ __launch_bounds__(kBlock) void likelihood_w128_kernel(
+    const float *__restrict__ logw, int C, size_t tsW, const long *__restrict__ off,
+    const int *__restrict__ pix, const double *__restrict__ cnt, const uint8_t *__restrict__ blacklist,
+    int num_data, double *__restrict__ G, size_t tsG, int r0) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int d = blockIdx.x * kWavesPerBlock + wave;
+    if (d >= num_data || blacklist[d]) return;
+    const int tile = blockIdx.y;
+    const float2 *__restrict__ col = reinterpret_cast<const float2 *>(logw + (size_t)tile * tsW) + lane;
+    const long k1 = off[d + 1];
+    long k = off[d];
+    double a0 = 0., a1 = 0.;
+    for (; k + U <= k1; k += U) {
+        float2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = col[(size_t)pix[k + u] * 64];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            a0 += cnt[k + u] * (double)v[u].x;
+            a1 += cnt[k + u] * (double)v[u].y;
+        }
+    }
+    for (; k < k1; ++k) {
+        const float2 v = col[(size_t)pix[k] * 64];
+        a0 += cnt[k] * (double)v.x;
+        a1 += cnt[k] * (double)v.y;
+    }
+    const int c = tile * 128 + 2 * lane;
+    if (c < C) {
+        double2 *dst = reinterpret_cast<double2 *>(G + (size_t)((r0 + c) / kRTile) * tsG + (size_t)d * kRTile + (c % kRTile));
+        *dst = make_double2(a0, a1);
+    }
+}
+
+// Frame-partitioned tomogram over an FP32 posterior in 128-wide tiles (see to_f32_w128_kernel):
+// each lane gathers two orientations per photon; partial sums in FP64, written in the 64-wide
+// tile-major partial layout of tomogram_part_kernel.
+template <int U>
+__global__ __launch_bounds__(kBlock) void tomogram_part_w128_kernel(
+    const float *__restrict__ P32, size_t tsP, int C, const long *__restrict__ toff2, int nparts,
+    const int *__restrict__ tframe, const double *__restrict__ tcnt, int num_pix, double *__restrict__ partial) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int part = blockIdx.x % nparts;
+    const int t = (blockIdx.x / nparts) * kWavesPerBlock + wave;
+    if (t >= num_pix) return;
+    const int tile = blockIdx.y;
+    const float2 *__restrict__ col = reinterpret_cast<const float2 *>(P32 + (size_t)tile * tsP) + lane;
+    const size_t at = (size_t)t * nparts + part;
+    long k = toff2[at];
+    const long k1 = toff2[at + 1];
+    double a0 = 0., a1 = 0.;
+    for (; k + U <= k1; k += U) {
+        float2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = col[(size_t)tframe[k + u] * 64];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double c = tcnt[k + u];
+            if ((double)v[u].x > kProbMin) a0 += c * (double)v[u].x;
+            if ((double)v[u].y > kProbMin) a1 += c * (double)v[u].y;
+        }
+    }
+    for (; k < k1; ++k) {
+        const float2 v = col[(size_t)tframe[k] * 64];
+        if ((double)v.x > kProbMin) a0 += tcnt[k] * (double)v.x;
+        if ((double)v.y > kProbMin) a1 += tcnt[k] * (double)v.y;
+    }
+    const int c = tile * 128 + 2 * lane;
+    if (c < C) {
+        const size_t tsW = (size_t)num_pix * kRTile, tsPart = tsW * (C / kRTile);
+        double2 *dst = reinterpret_cast<double2 *>(partial + part * tsPart + (size_t)(c / kRTile) * tsW +
+                                                   (size_t)t * kRTile + (c % kRTile));
+        *dst = make_double2(a0, a1);
+    }
 }
 
 void launch_u_reduce(hipStream_t s, const double *upart, const int *ubad, int ngroups, int C,
@@ -945,20 +1059,40 @@ void launch_rescale(hipStream_t s, const double *quats, const double *u, int num
     rescale_kernel<<<1, kBlock, 0, s>>>(quats, u, num_rot, mean_count, scalars);
 }
 
-void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
-                       const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
-                       double *G, int r0, int tiles, int unroll) {
+template <typename T>
+static void likelihood_launch(hipStream_t s, const T *logw, int C, int num_pix, const long *off,
+                              const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
+                              double *G, int r0, int tiles, int unroll) {
     const unsigned gx = cdiv(num_data, kWavesPerBlock);
     const size_t tsW = (size_t)num_pix * kRTile, tsG = (size_t)num_data * kRTile;
 #define EMC_LIK(NT, U)                                                                           \
-    likelihood_kernel<NT, U><<<dim3(gx, cdiv(C, kRTile * NT)), kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, \
-                                                                             blacklist, num_data, G, tsG, r0)
+    likelihood_kernel<NT, U, T><<<dim3(gx, cdiv(C, kRTile * NT)), kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, \
+                                                                                blacklist, num_data, G, tsG, r0)
     if (tiles >= 4) EMC_LIK(4, 2);
     else if (tiles == 2) EMC_LIK(2, 4);
     else if (unroll >= 16) EMC_LIK(1, 16);
     else if (unroll == 4) EMC_LIK(1, 4);
     else EMC_LIK(1, 8);
 #undef EMC_LIK
+}
+
+void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
+                       const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
+                       double *G, int r0, int tiles, int unroll) {
+    likelihood_launch(s, logw, C, num_pix, off, pix, cnt, blacklist, num_data, G, r0, tiles, unroll);
+}
+
+void launch_likelihood(hipStream_t s, const float *logw, int C, int num_pix, const long *off,
+                       const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
+                       double *G, int r0, int tiles, int unroll) {
+    (void)tiles;
+    const unsigned gx = cdiv(num_data, kWavesPerBlock);
+    const size_t tsW = (size_t)num_pix * 128, tsG = (size_t)num_data * kRTile;
+    const dim3 grid(gx, cdiv(C, 128));
+    if (unroll >= 16)
+        likelihood_w128_kernel<16><<<grid, kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, blacklist, num_data, G, tsG, r0);
+    else
+        likelihood_w128_kernel<8><<<grid, kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, blacklist, num_data, G, tsG, r0);
 }
 
 void launch_frame_max(hipStream_t s, double *R, int num_rot, int num_data,
@@ -1035,8 +1169,8 @@ void launch_tomogram_parts(hipStream_t s, const double *P, int num_data, int r0,
         dim3 grid(ngroups * nparts, C / kRTile);
         const bool sq = seq && nparts % kNumXcd == 0;
 #define EMC_TP(U, SQ)                                                                            \
-    tomogram_part_kernel<U, SQ><<<grid, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, r0, C, toff2, \
-                                                        nparts, tframe, tcnt, num_pix, ngroups, partial)
+    tomogram_part_kernel<U, SQ, double><<<grid, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, r0, C, toff2, \
+                                                                nparts, tframe, tcnt, num_pix, ngroups, partial)
         if (sq) {
             if (unroll >= 16) EMC_TP(16, true); else EMC_TP(8, true);
         } else {
@@ -1044,6 +1178,16 @@ void launch_tomogram_parts(hipStream_t s, const double *P, int num_data, int r0,
         }
 #undef EMC_TP
     }
+}
+
+// FP32 posterior of one orientation chunk (128-wide tiles, to_f32_w128_kernel), FP64 sums.
+void launch_tomogram_parts_f32(hipStream_t s, const float *P32, int num_data, int C, const long *toff2,
+                               int nparts, const int *tframe, const double *tcnt, int num_pix,
+                               double *partial) {
+    const int ngroups = (int)cdiv(num_pix, kWavesPerBlock);
+    dim3 grid(ngroups * nparts, cdiv(C, 128));
+    tomogram_part_w128_kernel<16><<<grid, kBlock, 0, s>>>(P32, (size_t)num_data * 128, C, toff2, nparts, tframe,
+                                                          tcnt, num_pix, partial);
 }
 
 void launch_tomogram_reduce_rmajor(hipStream_t s, const double *partial, int nparts, int r0, int C,

@@ -37,43 +37,45 @@ __device__ __forceinline__ int wmax(int v) {
 }  // namespace
 
 // Pass 0 counts items per brick (counts[b] += 1); pass 1 writes item ids into
-// list[off[b] + fill[b]++]. One wavefront per item; the wave's lanes are the patch's pixels.
+// list[off[b] + fill[b]++]. One wavefront per item (waves stride over the items); the wave's
+// lanes are the patch's pixels.
 __global__ __launch_bounds__(kBlock) void brick_items_kernel(
     const double *__restrict__ rots, int r0, int Cc, int num_rot, const double *__restrict__ qvals,
     const uint8_t *__restrict__ mask, int num_pix, long size, int npatch, int nbx, int pass,
     int *__restrict__ counts, const long *__restrict__ off, int *__restrict__ fill,
     int *__restrict__ list, const int *__restrict__ slot_rot) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const long item = (long)blockIdx.x * kWavesPerBlock + wave;
-    if (item >= (long)Cc * npatch) return;
-    const int c = (int)(item / npatch), p = (int)(item % npatch);
-    const int r = r0 + c;
-    if (r >= num_rot) return;
-    if (slot_rot && slot_rot[r] < 0) return;  // padding slot of the sparse-subset order
-    const int t = p * 64 + lane;
-    Cell cell;
-    bool ok = false;
-    if (t < num_pix && mask[t] <= 1)
-        ok = locate(rots + 9 * (size_t)r, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1],
-                    qvals[3 * (size_t)t + 2], (double)(size / 2), size, cell);
-    const int xmax = wmax(ok ? (int)cell.x : -1);
-    if (xmax < 0) return;
-    const int xmin = wmin(ok ? (int)cell.x : INT_MAX);
-    const int ymin = wmin(ok ? (int)cell.y : INT_MAX), ymax = wmax(ok ? (int)cell.y : -1);
-    const int zmin = wmin(ok ? (int)cell.z : INT_MAX), zmax = wmax(ok ? (int)cell.z : -1);
-    // corners reach cell + 1 (never past size - 1: cells stop at size - 2)
-    const int bx0 = xmin / kBrick, bx1 = (xmax + 1) / kBrick;
-    const int by0 = ymin / kBrick, by1 = (ymax + 1) / kBrick;
-    const int bz0 = zmin / kBrick, bz1 = (zmax + 1) / kBrick;
-    const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1;
-    const int nb = (bx1 - bx0 + 1) * ny * nz;
-    for (int i = lane; i < nb; i += 64) {
-        const int b = ((bx0 + i / (ny * nz)) * nbx + by0 + (i / nz) % ny) * nbx + bz0 + i % nz;
-        if (pass == 0) {
-            atomicAdd(&counts[b], 1);
-        } else {
-            const int at = atomicAdd(&fill[b], 1);
-            list[off[b] + at] = (int)item;
+    const long items = (long)Cc * npatch, stride = (long)gridDim.x * kWavesPerBlock;
+    for (long item = (long)blockIdx.x * kWavesPerBlock + wave; item < items; item += stride) {
+        const int c = (int)(item / npatch), p = (int)(item % npatch);
+        const int r = r0 + c;
+        if (r >= num_rot) continue;
+        if (slot_rot && slot_rot[r] < 0) continue;  // padding slot of the sparse-subset order
+        const int t = p * 64 + lane;
+        Cell cell;
+        bool ok = false;
+        if (t < num_pix && mask[t] <= 1)
+            ok = locate(rots + 9 * (size_t)r, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1],
+                        qvals[3 * (size_t)t + 2], (double)(size / 2), size, cell);
+        const int xmax = wmax(ok ? (int)cell.x : -1);
+        if (xmax < 0) continue;
+        const int xmin = wmin(ok ? (int)cell.x : INT_MAX);
+        const int ymin = wmin(ok ? (int)cell.y : INT_MAX), ymax = wmax(ok ? (int)cell.y : -1);
+        const int zmin = wmin(ok ? (int)cell.z : INT_MAX), zmax = wmax(ok ? (int)cell.z : -1);
+        // corners reach cell + 1 (never past size - 1: cells stop at size - 2)
+        const int bx0 = xmin / kBrick, bx1 = (xmax + 1) / kBrick;
+        const int by0 = ymin / kBrick, by1 = (ymax + 1) / kBrick;
+        const int bz0 = zmin / kBrick, bz1 = (zmax + 1) / kBrick;
+        const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1;
+        const int nb = (bx1 - bx0 + 1) * ny * nz;
+        for (int i = lane; i < nb; i += 64) {
+            const int b = ((bx0 + i / (ny * nz)) * nbx + by0 + (i / nz) % ny) * nbx + bz0 + i % nz;
+            if (pass == 0) {
+                atomicAdd(&counts[b], 1);
+            } else {
+                const int at = atomicAdd(&fill[b], 1);
+                list[off[b] + at] = (int)item;
+            }
         }
     }
 }
@@ -144,7 +146,7 @@ void launch_brick_items(hipStream_t s, const double *rots, int r0, int Cc, int n
                         int nbx, int pass, int *counts, const long *off, int *fill, int *list,
                         const int *slot_rot) {
     const long items = (long)Cc * npatch;
-    const unsigned grid = (unsigned)((items + kWavesPerBlock - 1) / kWavesPerBlock);
+    const unsigned grid = stride_grid((items + kWavesPerBlock - 1) / kWavesPerBlock);
     brick_items_kernel<<<grid, kBlock, 0, s>>>(rots, r0, Cc, num_rot, qvals, mask, num_pix, size, npatch,
                                                nbx, pass, counts, off, fill, list, slot_rot);
 }

@@ -28,24 +28,25 @@ __global__ __launch_bounds__(kBlock) void pair_likelihood_kernel(
     const int *__restrict__ pair_tile, long num_pairs, int t0, int t1, double *__restrict__ G) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const long k = (long)blockIdx.x * kWavesPerBlock + wave;
-    if (k >= num_pairs) return;
-    const int T = pair_tile[k];
-    if (T < t0 || T >= t1) return;
-    const int d = pair_frame[k];
-    const double *__restrict__ col = logw + (size_t)(T - t0) * tsW + lane;
-    long j = off[d];
-    const long j1 = off[d + 1];
-    double acc = 0.;
-    for (; j + U <= j1; j += U) {
-        double v[U];
+    const long stride = (long)gridDim.x * kWavesPerBlock;
+    for (long k = (long)blockIdx.x * kWavesPerBlock + wave; k < num_pairs; k += stride) {
+        const int T = pair_tile[k];
+        if (T < t0 || T >= t1) continue;
+        const int d = pair_frame[k];
+        const double *__restrict__ col = logw + (size_t)(T - t0) * tsW + lane;
+        long j = off[d];
+        const long j1 = off[d + 1];
+        double acc = 0.;
+        for (; j + U <= j1; j += U) {
+            double v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = col[(size_t)pix[j + u] * kRTile];
+            for (int u = 0; u < U; ++u) v[u] = col[(size_t)pix[j + u] * kRTile];
 #pragma unroll
-        for (int u = 0; u < U; ++u) acc += cnt[j + u] * v[u];
+            for (int u = 0; u < U; ++u) acc += cnt[j + u] * v[u];
+        }
+        for (; j < j1; ++j) acc += cnt[j] * col[(size_t)pix[j] * kRTile];
+        G[(size_t)k * kRTile + lane] = acc;
     }
-    for (; j < j1; ++j) acc += cnt[j] * col[(size_t)pix[j] * kRTile];
-    G[(size_t)k * kRTile + lane] = acc;
 }
 
 // Per frame over its pairs: R = log w - u phi + G for real, allowed orientations (-inf for
@@ -227,11 +228,13 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     const double *__restrict__ P, const long *__restrict__ tile_off, const int *__restrict__ tile_pair,
     const int *__restrict__ pair_frame, const long *__restrict__ foff, const int *__restrict__ roff,
     int nranges, const int *__restrict__ fpix, const double *__restrict__ fcnt,
-    const double *__restrict__ psum_slot, int t0, int num_pix, double *__restrict__ view) {
+    const double *__restrict__ psum_slot, int t0, int num_pix, const uint8_t *__restrict__ tile_dense,
+    double *__restrict__ view) {
     __shared__ double acc[kPairRange][kRTile + 1];
     const int T = t0 + blockIdx.y, rg = blockIdx.x;
     const long j0 = tile_off[T], j1 = tile_off[T + 1];
     if (j0 == j1) return;  // no frame takes this tile: psum 0, the merge skips it
+    if (tile_dense && tile_dense[T]) return;  // gathered per pixel (pair_tomogram_dense_kernel)
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int i = threadIdx.x; i < kPairRange * (kRTile + 1); i += kBlock) (&acc[0][0])[i] = 0.;
     __syncthreads();
@@ -259,17 +262,56 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
         }
 }
 
+// Tomogram of tiles that most frames take (dense tiles), gathered per pixel like the plain
+// path: block = 64 pixels x one dense tile; for each pixel, the frames with a photon there
+// (tomogram photon list, frames ascending) that have the tile find their pair through
+// pair_of[d][cell] (first pair of the frame's cell, -1 if none) + the tile's index in its cell.
+__global__ __launch_bounds__(kBlock) void pair_tomogram_dense_kernel(
+    const double *__restrict__ P, const int *__restrict__ dense_tiles, const int *__restrict__ tile_cell,
+    const int *__restrict__ tile_sub, const int *__restrict__ pair_of, int num_cells,
+    const long *__restrict__ toff, const int *__restrict__ tframe, const double *__restrict__ tcnt,
+    const double *__restrict__ psum_slot, int t0, int num_pix, double *__restrict__ view) {
+    __shared__ double s[64][kRTile + 1];
+    const int T = dense_tiles[blockIdx.y];
+    const int cell = tile_cell[T], sub = tile_sub[T];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int p0 = blockIdx.x * 64;
+    for (int i = wave; i < 64; i += kWavesPerBlock) {
+        const int t = p0 + i;
+        double acc = 0.;
+        if (t < num_pix) {
+            for (long q = toff[t]; q < toff[t + 1]; ++q) {
+                const int d = tframe[q];
+                const int k0 = pair_of[(size_t)d * num_cells + cell];
+                if (k0 < 0) continue;
+                const double p = P[((size_t)k0 + sub) * kRTile + lane];
+                if (p > kProbMin) acc += tcnt[q] * p;
+            }
+        }
+        s[i][lane] = acc;
+    }
+    __syncthreads();
+    const double ps = psum_slot[(size_t)T * kRTile + lane];
+    for (int i = wave; i < 64; i += kWavesPerBlock) s[i][lane] = ps > 0. ? s[i][lane] / ps : s[i][lane];
+    __syncthreads();
+    const int t = p0 + lane;
+    if (t < num_pix)
+        for (int j = wave; j < kRTile; j += kWavesPerBlock)
+            view[((size_t)(T - t0) * kRTile + j) * num_pix + t] = s[lane][j];
+}
+
 // Scatter of pair values to a dense (num_data x num_rot) matrix (zeroed by the caller).
 __global__ __launch_bounds__(kBlock) void pair_scatter_kernel(const double *__restrict__ P,
                                                               const int *__restrict__ pair_frame,
                                                               const int *__restrict__ pair_tile,
                                                               const int *__restrict__ slot_rot, long num_pairs,
                                                               int num_rot, double *__restrict__ out) {
-    const long e = blockIdx.x * (long)kBlock + threadIdx.x;
-    if (e >= num_pairs * kRTile) return;
-    const long k = e / kRTile;
-    const int r = slot_rot[pair_tile[k] * kRTile + (int)(e % kRTile)];
-    if (r >= 0) out[(size_t)pair_frame[k] * num_rot + r] = P[e];
+    const long n = num_pairs * kRTile;
+    for (long e = blockIdx.x * (long)kBlock + threadIdx.x; e < n; e += (long)gridDim.x * kBlock) {
+        const long k = e / kRTile;
+        const int r = slot_rot[pair_tile[k] * kRTile + (int)(e % kRTile)];
+        if (r >= 0) out[(size_t)pair_frame[k] * num_rot + r] = P[e];
+    }
 }
 
 // Gather of slot values into the original orientation order: out[r] = in[rot_slot[r]].
@@ -284,7 +326,7 @@ __global__ void slot_gather_kernel(const double *__restrict__ in, const int *__r
 void launch_pair_likelihood(hipStream_t s, const double *logw, int num_pix, const long *off, const int *pix,
                             const double *cnt, const int *pair_frame, const int *pair_tile, long num_pairs,
                             int t0, int t1, double *G) {
-    const unsigned grid = (unsigned)((num_pairs + kWavesPerBlock - 1) / kWavesPerBlock);
+    const unsigned grid = stride_grid((num_pairs + kWavesPerBlock - 1) / kWavesPerBlock);
     if (grid == 0) return;
     pair_likelihood_kernel<8><<<grid, kBlock, 0, s>>>(logw, (size_t)num_pix * kRTile, off, pix, cnt, pair_frame,
                                                       pair_tile, num_pairs, t0, t1, G);
@@ -325,18 +367,28 @@ void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, cons
 void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                           const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
                           const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
-                          double *view) {
+                          const uint8_t *tile_dense, double *view) {
     if (ntiles <= 0) return;
     dim3 grid(nranges, ntiles);
     pair_tomogram_kernel<<<grid, kBlock, 0, s>>>(P, tile_off, tile_pair, pair_frame, foff, roff, nranges, fpix,
-                                                 fcnt, psum_slot, t0, num_pix, view);
+                                                 fcnt, psum_slot, t0, num_pix, tile_dense, view);
+}
+
+void launch_pair_tomogram_dense(hipStream_t s, const double *P, const int *dense_tiles, int ndense,
+                                const int *tile_cell, const int *tile_sub, const int *pair_of, int num_cells,
+                                const long *toff, const int *tframe, const double *tcnt, const double *psum_slot,
+                                int t0, int num_pix, double *view) {
+    if (ndense <= 0) return;
+    dim3 grid((num_pix + 63) / 64, ndense);
+    pair_tomogram_dense_kernel<<<grid, kBlock, 0, s>>>(P, dense_tiles, tile_cell, tile_sub, pair_of, num_cells,
+                                                       toff, tframe, tcnt, psum_slot, t0, num_pix, view);
 }
 
 void launch_pair_scatter(hipStream_t s, const double *P, const int *pair_frame, const int *pair_tile,
                          const int *slot_rot, long num_pairs, int num_rot, double *out) {
     const long n = num_pairs * kRTile;
     if (n == 0) return;
-    pair_scatter_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(P, pair_frame, pair_tile, slot_rot,
+    pair_scatter_kernel<<<stride_grid((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(P, pair_frame, pair_tile, slot_rot,
                                                                                 num_pairs, num_rot, out);
 }
 

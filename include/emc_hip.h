@@ -126,8 +126,9 @@ int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const
  * computes everything in FP64): 0 = plain FP64 (default, reproduces the reference);
  * 1 = dense frames only: likelihood and tomogram as FP16 products on matrix cores with
  * FP32 accumulation, log W centred per pixel and rounded once to FP16; 2 = as 1 with
- * log W carried as an FP16 hi + lo pair. Every other step stays FP64. Modes 1 and 2 need
- * frames set with emc_set_frames_dense and counts <= 2048.                              */
+ * log W carried as an FP16 hi + lo pair; 3 = any frames: log W and the posterior read by
+ * the photon gathers in FP32 (half the bytes), sums in FP64. Every other step stays FP64.
+ * Modes 1 and 2 need frames set with emc_set_frames_dense and counts <= 2048.           */
 int emc_set_precision(emc_engine *e, int mode);
 /* per-frame state of struct iterate: scale, beta, blacklist, fcounts, sum_fact,
  *        mean_count[0], num_blacklist (iterate.h:17-24,33; iterate.c:3-117)          */

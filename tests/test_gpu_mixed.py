@@ -117,3 +117,32 @@ def test_mixed_path_refuses_sparse_frames():
     eng.set_precision(0)
     eng.maximize(1, 0, 1, 0)
     eng.close()
+
+
+@pytest.mark.parametrize('scaled', [False, True])
+def test_fp32_gathers_vs_oracle(scaled):
+    """Precision 3 (FP32 log W and posterior gathers, FP64 sums) on sparse frames, unscaled and
+    scaled (iteration 2 uses the per-frame scales), in 64-orientation chunks."""
+    import os
+    cfg = P.tiny_config(num_data=200, mean_count=200., phi=(0.5, 1.5) if scaled else None)
+    p = P.build(cfg)
+    fp = dict(p['fparams'])
+    if scaled:
+        fp['scale'] = np.asarray(p['true_scale'], 'f8').copy()
+    p = dict(p, fparams=fp)
+    flags = dict(iteration=2 if scaled else 1, need_scaling=int(scaled), update_scale=1)
+    o = P.run_oracle(p, flags)
+    os.environ['EMC_MAX_CHUNK_COLS'] = '64'
+    try:
+        eng = P.engine_for(p)
+        eng.set_precision(3)
+        g = P.run_engine(eng, flags)
+        eng.close()
+    finally:
+        del os.environ['EMC_MAX_CHUNK_COLS']
+    e = _errs(g, o)
+    print('fp32 gathers scaled=%s' % scaled, {k: '%.2e' % v for k, v in e.items()}, flush=True)
+    assert g['num_chunks'] > 1
+    assert e['u'] <= 1e-12 and e['inter_weight'] <= 1e-12
+    assert e['model2'] <= 1e-4
+    assert e['avg_likelihood'] <= 1e-6 and e['mutual_info'] <= 1e-5

@@ -297,3 +297,80 @@ def test_refinement_subsets_scaled_blacklisted(tiny_scaled):
         g = P.run_engine(eng, flags)
         _compare(o, g, 'refinement scaled it=%d' % it, scaling=True)
     eng.close()
+
+
+@pytest.mark.parametrize('cells', ['all', 'random'])
+def test_refinement_subsets_at_scale_vs_engine(cells):
+    """151x151 detector, 125^3 model, fine num_div 4 (3240 orientations), coarse num_div 2
+    (420 cells), 1500 frames: the sparse subset path against the plain engine with every cell
+    (identical to the full search), and against the masked form with 24 random cells per frame;
+    several slot chunks (1024 slots each)."""
+    import os
+    from dragonfly_amd import _lib, synth
+    cfg = dict(synth.CONFIGS['cfg1'], num_data=1500, mean_count=300.)
+    p = P.build(cfg, device='cuda', num_ave=1000)
+    qc = _lib.quat_gen(2)
+    qmap = _lib.voronoi_subset(qc, p['quats'])
+    D = len(p['fparams']['beta'])
+    if cells == 'all':
+        sets = [np.arange(len(qc), dtype='i4')] * D
+    else:
+        rng = np.random.default_rng(3)
+        sets = [np.sort(rng.choice(len(qc), 24, replace=False)).astype('i4') for _ in range(D)]
+    off = np.zeros(D + 1, 'i8')
+    np.cumsum([len(x) for x in sets], out=off[1:])
+    rq = np.concatenate(sets)
+    flags = dict(iteration=1, need_scaling=0)
+    ref = P.engine_for(p)
+    if cells == 'random':
+        os.environ['EMC_SUBSET'] = '0'
+        try:
+            ref.close()
+            ref = P.engine_for(p)
+        finally:
+            del os.environ['EMC_SUBSET']
+        ref.set_refine(len(qc), qmap, off, rq)
+    o = P.run_engine(ref, flags)
+    ref.close()
+    os.environ['EMC_MAX_CHUNK_COLS'] = '1024'
+    try:
+        eng = P.engine_for(p)
+    finally:
+        del os.environ['EMC_MAX_CHUNK_COLS']
+    eng.set_refine(len(qc), qmap, off, rq)
+    g = P.run_engine(eng, flags)
+    eng.close()
+    assert g['num_chunks'] > 1
+    _compare(o, g, 'refinement at scale, %s cells' % cells)
+
+
+def test_refinement_subsets_past_32bit_grid():
+    """More (frame, tile) pairs than a 32-bit count of work-items holds: 16000 frames on a
+    22x20 detector, fine num_div 16 (204960 orientations), every one of the 3240 coarse cells
+    (4571 slot tiles) per frame gives 7.3e7 pairs, 4.7e9 lanes of the pair likelihood. The
+    subset path must agree with the plain engine for every frame (a wrapped launch grid drops
+    the last frames' pairs)."""
+    from dragonfly_amd import _lib
+    cfg = P.tiny_config(num_data=16000, mean_count=40., num_div=16)
+    p = P.build(cfg, device='cuda')
+    qc = _lib.quat_gen(4)
+    qmap = _lib.voronoi_subset(qc, p['quats'])
+    D = len(p['fparams']['beta'])
+    off = np.arange(D + 1, dtype='i8') * len(qc)
+    rq = np.tile(np.arange(len(qc), dtype='i4'), D)
+    keys = ('u', 'max_exp', 'p_norm', 'rmax', 'likelihood', 'info', 'psum_r', 'model2',
+            'inter_weight')
+    out = []
+    for refine in (False, True):
+        eng = P.engine_for(p)
+        if refine:
+            eng.set_refine(len(qc), qmap, off, rq)
+        eng.maximize(1, 0, 1, 0)
+        out.append({k: eng.get(k) for k in keys})
+        eng.close()
+    o, g = out
+    errs = {k: P.rel_err(g[k], o[k]) for k in keys if k != 'rmax'}
+    print('refinement past 2^32 lanes', {k: '%.1e' % v for k, v in errs.items()}, flush=True)
+    bad = {k: v for k, v in errs.items() if not v <= TOL}
+    assert not bad, bad
+    assert np.array_equal(g['rmax'], o['rmax'])
