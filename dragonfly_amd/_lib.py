@@ -87,6 +87,7 @@ def lib():
             'emc_slice_merge': (C.c_int, [C.c_void_p, _dp, _dp]),
             'emc_get_field': (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
             'emc_clear_merge': (C.c_int, [C.c_void_p]),
+            'emc_cell_posterior': (C.c_int, [C.c_void_p, C.c_int, _ip, C.POINTER(C.c_float)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -274,4 +275,15 @@ class Engine:
                  'vol': (self.size,) * 3, 'data_rot': (self.num_data, self.num_rot)}[kind]
         out = np.empty(shape, dtype=dt)
         _check(lib().emc_get_field(self._h, fid, out.ctypes.data_as(C.c_void_p), out.size))
+        return out
+
+    def cell_posterior(self, num_coarse, quat_mapping):
+        """(num_data, num_coarse) float32: each frame's posterior mass in each coarse cell
+        (quat_mapping[r] = cell of orientation r) after the last iteration."""
+        qm = np.ascontiguousarray(quat_mapping, dtype='i4')
+        if qm.shape != (self.num_rot,):
+            raise ValueError('quat_mapping must have num_rot entries')
+        out = np.empty((self.num_data, int(num_coarse)), dtype='f4')
+        _check(lib().emc_cell_posterior(self._h, int(num_coarse), _ptr(qm, _ip),
+                                        out.ctypes.data_as(C.POINTER(C.c_float))))
         return out

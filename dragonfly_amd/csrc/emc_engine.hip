@@ -42,6 +42,24 @@ template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     int alloc(size_t count) {
         if (count == n && p) return 0;
         release();
@@ -150,6 +168,10 @@ struct emc_engine {
     DevBuf<double> lik_cnt, tom_cnt;
     // per (pixel, frame range) offsets into the tomogram list for the frame-partitioned form
     DevBuf<long> tom_off2;
+    // host copies of the per-pixel photon lists (tom_off / tom_frame / tom_cnt)
+    std::vector<long> h_tom_off;
+    std::vector<int> h_tom_frame;
+    std::vector<double> h_tom_cnt;
     int tom_parts = 8, tomo_width32 = 0;
     // upper bound on the orientation chunk width (0: memory decides)
     int max_chunk_cols = 0;
@@ -217,13 +239,25 @@ struct emc_engine {
     // pair and masks instead): orientations ordered by coarse cell in slots, cells padded to
     // tiles of 64, (frame, tile) pairs
     int subset_mode = 1;
-    double dense_tile_frac = 0.3;     // tiles taken by more than this fraction of frames are gathered per pixel
     bool subset_ready = false, subset_ran = false;
-    std::vector<int> h_dense_tiles;
-    DevBuf<int> dense_tiles, tile_cell, tile_sub, pair_of;
-    DevBuf<uint8_t> tile_dense;
-    int num_cells = 0;
     int num_slots = 0, num_tiles = 0, slot_C = 0;
+    // slot order of the last coarse mapping (kept while the mapping does not change)
+    std::vector<int> h_slot_qmap, h_tile0;
+    int slot_num_coarse = 0;
+    // full frames (every coarse cell): rows tile-major in RF; photon lists of mask < 2 pixels
+    // of the full frames per pixel (frame index f), split into tom_parts frame ranges
+    int num_full = 0;
+    std::vector<int> h_fidx;
+    DevBuf<int> fidx, ffull, ftom_frame;
+    DevBuf<long> ftom_off2;
+    DevBuf<double> ftom_cnt, RF, scale_full, psum_full, psum_full_part;
+    DevBuf<uint8_t> blacklist_full;
+    bool full_lists_valid = false;
+    // host scratch of the per-iteration subset set-up, kept so that its pages stay mapped
+    std::vector<uint32_t> h_relbits;
+    std::vector<long> h_poff, h_toff, h_ftom_off2;
+    std::vector<int> h_pframe, h_ptile, h_tpair, h_ffull, h_fof, h_ftom_frame;
+    std::vector<double> h_ftom_cnt;
     long num_pairs = 0;
     DevBuf<int> slot_rot, rot_slot, pair_frame, pair_tile, tile_pair, rbad_slot;
     DevBuf<long> pair_off, tile_off;
@@ -385,10 +419,14 @@ int build_lists(emc_engine *e) {
     if (!e->tom_frame.p && (rc = e->tom_frame.alloc(1))) return rc;
     if (!e->tom_cnt.p && (rc = e->tom_cnt.alloc(1))) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
+    e->h_tom_off = std::move(tcount);
+    e->h_tom_frame = std::move(tframe);
+    e->h_tom_cnt = std::move(tcnt);
     e->max_count = max_count;
     e->lists_dirty = false;
     e->mp_built = false;
     e->fsort_valid = false;
+    e->full_lists_valid = false;
     return 0;
 }
 
@@ -462,106 +500,150 @@ int merge_chunk(const emc_engine *e, const emc_engine::BrickChunk &bc, const dou
 }
 
 // Slot order and (frame, tile) pairs of the refinement subsets: orientations grouped by coarse
-// cell (ascending index inside a cell), every cell padded to whole tiles of 64; a frame's
-// pairs are the tiles of its (distinct) cells in ascending tile order; the tile-major copy of
-// the pair list (tile_off / tile_pair) lists each tile's pairs by ascending frame.
+// cell (ascending index inside a cell), every cell padded to whole tiles of 64. Frames whose
+// set holds every cell are full frames (rows in RF, tile-major); the pairs of the others are
+// the tiles of their (distinct) cells in ascending tile order, and the tile-major copy of
+// that pair list (tile_off / tile_pair) lists each tile's pairs by ascending frame.
 int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel_off, const int *rel_quat) {
     const int Rn = e->num_rot, N = e->num_data;
-    std::vector<int> cnt(num_coarse, 0), tile0(num_coarse + 1, 0);
-    for (int r = 0; r < Rn; ++r) ++cnt[qmap[r]];
-    for (int c = 0; c < num_coarse; ++c) tile0[c + 1] = tile0[c] + (cnt[c] + emc::kRTile - 1) / emc::kRTile;
-    const int T = tile0[num_coarse], S = T * emc::kRTile;
-    if (S <= 0) return fail(EMC_E_ARG, "emc_set_refine: no orientations");
-    std::vector<int> slot_rot(S, -1), rot_slot(Rn), fill(num_coarse, 0);
-    for (int r = 0; r < Rn; ++r) {
-        const int c = qmap[r];
-        const int sl = tile0[c] * emc::kRTile + fill[c]++;
-        slot_rot[sl] = r;
-        rot_slot[r] = sl;
-    }
-    std::vector<double> rots((size_t)S * 9, 0.), quats((size_t)S * 5, 0.), lw(S, 0.);
-    for (int sl = 0; sl < S; ++sl) {
-        const int r = slot_rot[sl];
-        if (r >= 0) {
-            make_rot(&e->h_quats[5 * (size_t)r], &rots[9 * (size_t)sl]);
-            for (int k = 0; k < 5; ++k) quats[5 * (size_t)sl + k] = e->h_quats[5 * (size_t)r + k];
-            lw[sl] = std::log(e->h_quats[5 * (size_t)r + 4]);
-        } else {
-            rots[9 * (size_t)sl] = rots[9 * (size_t)sl + 4] = rots[9 * (size_t)sl + 8] = 1.;
+    hipStream_t s = e->stream;
+    int rc;
+    const bool same_slots = e->slot_num_coarse == num_coarse && (int)e->h_slot_qmap.size() == Rn &&
+                            std::equal(e->h_slot_qmap.begin(), e->h_slot_qmap.end(), qmap);
+    if (!same_slots) {
+        std::vector<int> cnt(num_coarse, 0), tile0(num_coarse + 1, 0);
+        for (int r = 0; r < Rn; ++r) ++cnt[qmap[r]];
+        for (int c = 0; c < num_coarse; ++c) tile0[c + 1] = tile0[c] + (cnt[c] + emc::kRTile - 1) / emc::kRTile;
+        const int T = tile0[num_coarse], S = T * emc::kRTile;
+        if (S <= 0) return fail(EMC_E_ARG, "emc_set_refine: no orientations");
+        std::vector<int> slot_rot(S, -1), rot_slot(Rn), fill(num_coarse, 0);
+        for (int r = 0; r < Rn; ++r) {
+            const int c = qmap[r];
+            const int sl = tile0[c] * emc::kRTile + fill[c]++;
+            slot_rot[sl] = r;
+            rot_slot[r] = sl;
         }
-    }
-    std::vector<long> poff(N + 1, 0);
-    std::vector<int> pframe, ptile, cells;
-    for (int d = 0; d < N; ++d) {
-        cells.assign(rel_quat + rel_off[d], rel_quat + rel_off[d + 1]);
-        std::sort(cells.begin(), cells.end());
-        cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
-        for (int c : cells)
-            for (int t = tile0[c]; t < tile0[c + 1]; ++t) {
-                pframe.push_back(d);
-                ptile.push_back(t);
+        std::vector<double> rots((size_t)S * 9, 0.), quats((size_t)S * 5, 0.), lw(S, 0.);
+        for (int sl = 0; sl < S; ++sl) {
+            const int r = slot_rot[sl];
+            if (r >= 0) {
+                make_rot(&e->h_quats[5 * (size_t)r], &rots[9 * (size_t)sl]);
+                for (int k = 0; k < 5; ++k) quats[5 * (size_t)sl + k] = e->h_quats[5 * (size_t)r + k];
+                lw[sl] = std::log(e->h_quats[5 * (size_t)r + 4]);
+            } else {
+                rots[9 * (size_t)sl] = rots[9 * (size_t)sl + 4] = rots[9 * (size_t)sl + 8] = 1.;
             }
+        }
+        if ((rc = e->slot_rot.upload(slot_rot.data(), S, s))) return rc;
+        if ((rc = e->rot_slot.upload(rot_slot.data(), Rn, s))) return rc;
+        if ((rc = e->rots_slot.upload(rots.data(), rots.size(), s))) return rc;
+        if ((rc = e->quats_slot.upload(quats.data(), quats.size(), s))) return rc;
+        if ((rc = e->logwr_slot.upload(lw.data(), S, s))) return rc;
+        HIP_TRY(hipStreamSynchronize(s));
+        e->h_slot_qmap.assign(qmap, qmap + Rn);
+        e->h_tile0 = tile0;
+        e->slot_num_coarse = num_coarse;
+        e->num_slots = S;
+        e->num_tiles = T;
+        e->slot_bricks_valid = false;
+    }
+    const std::vector<int> &tile0 = e->h_tile0;
+    const int T = e->num_tiles;
+    std::vector<long> &poff = e->h_poff;
+    std::vector<int> &pframe = e->h_pframe, &ptile = e->h_ptile, &ffull = e->h_ffull;
+    std::vector<int> fidx;
+    poff.assign(N + 1, 0);
+    pframe.clear();
+    ptile.clear();
+    ffull.assign(N, -1);
+    std::vector<uint8_t> seen(num_coarse, 0);
+    std::vector<int> cells;
+    for (int d = 0; d < N; ++d) {
+        cells.clear();
+        for (long k = rel_off[d]; k < rel_off[d + 1]; ++k)
+            if (!seen[rel_quat[k]]) {
+                seen[rel_quat[k]] = 1;
+                cells.push_back(rel_quat[k]);
+            }
+        for (int c : cells) seen[c] = 0;
+        if ((int)cells.size() == num_coarse) {
+            ffull[d] = (int)fidx.size();
+            fidx.push_back(d);
+        } else {
+            std::sort(cells.begin(), cells.end());
+            for (int c : cells)
+                for (int t = tile0[c]; t < tile0[c + 1]; ++t) {
+                    pframe.push_back(d);
+                    ptile.push_back(t);
+                }
+        }
         poff[d + 1] = (long)pframe.size();
     }
     const long K = (long)pframe.size();
     if (K >= INT32_MAX) return fail(EMC_E_ARG, "emc_set_refine: too many (frame, tile) pairs");
-    std::vector<long> toff(T + 1, 0);
+    std::vector<long> &toff = e->h_toff;
+    toff.assign(T + 1, 0);
     for (long k = 0; k < K; ++k) ++toff[ptile[k] + 1];
     for (int t = 0; t < T; ++t) toff[t + 1] += toff[t];
-    std::vector<int> tpair(K > 0 ? K : 1);
+    std::vector<int> &tpair = e->h_tpair;
+    tpair.resize(K > 0 ? K : 1);
     std::vector<long> tfill(toff.begin(), toff.end() - 1);
     for (long k = 0; k < K; ++k) tpair[tfill[ptile[k]]++] = (int)k;
     if (pframe.empty()) {
         pframe.push_back(0);
         ptile.push_back(0);
     }
-    // tiles most frames take are gathered per pixel: per-tile cell and index in the cell, and
-    // the first pair of every (frame, cell)
-    std::vector<int> tcell(T), tsub(T), dense;
-    std::vector<uint8_t> tdense(T, 0);
-    for (int c = 0; c < num_coarse; ++c)
-        for (int t = tile0[c]; t < tile0[c + 1]; ++t) {
-            tcell[t] = c;
-            tsub[t] = t - tile0[c];
-        }
-    for (int t = 0; t < T; ++t)
-        if ((double)(toff[t + 1] - toff[t]) > e->dense_tile_frac * N) {
-            tdense[t] = 1;
-            dense.push_back(t);
-        }
-    std::vector<int> pof;
-    if (!dense.empty()) {
-        pof.assign((size_t)N * num_coarse, -1);
-        for (int d = 0; d < N; ++d)
-            for (long k = poff[d + 1] - 1; k >= poff[d]; --k) pof[(size_t)d * num_coarse + tcell[ptile[k]]] = (int)k;
-    } else {
-        pof.assign(1, -1);
-    }
-    int rc;
-    hipStream_t s = e->stream;
-    if ((rc = e->tile_cell.upload(tcell.data(), T, s))) return rc;
-    if ((rc = e->tile_sub.upload(tsub.data(), T, s))) return rc;
-    if ((rc = e->tile_dense.upload(tdense.data(), T, s))) return rc;
-    if ((rc = e->pair_of.upload(pof.data(), pof.size(), s))) return rc;
-    if (!dense.empty() && (rc = e->dense_tiles.upload(dense.data(), dense.size(), s))) return rc;
-    e->h_dense_tiles = dense;
-    e->num_cells = num_coarse;
-    if ((rc = e->slot_rot.upload(slot_rot.data(), S, s))) return rc;
-    if ((rc = e->rot_slot.upload(rot_slot.data(), Rn, s))) return rc;
-    if ((rc = e->rots_slot.upload(rots.data(), rots.size(), s))) return rc;
-    if ((rc = e->quats_slot.upload(quats.data(), quats.size(), s))) return rc;
-    if ((rc = e->logwr_slot.upload(lw.data(), S, s))) return rc;
     if ((rc = e->pair_off.upload(poff.data(), poff.size(), s))) return rc;
     if ((rc = e->pair_frame.upload(pframe.data(), pframe.size(), s))) return rc;
     if ((rc = e->pair_tile.upload(ptile.data(), ptile.size(), s))) return rc;
     if ((rc = e->tile_off.upload(toff.data(), toff.size(), s))) return rc;
     if ((rc = e->tile_pair.upload(tpair.data(), tpair.size(), s))) return rc;
+    if ((rc = e->ffull.upload(ffull.data(), ffull.size(), s))) return rc;
+    if (!fidx.empty() && (rc = e->fidx.upload(fidx.data(), fidx.size(), s))) return rc;
     HIP_TRY(hipStreamSynchronize(s));
-    e->num_slots = S;
-    e->num_tiles = T;
     e->num_pairs = K;
-    e->slot_bricks_valid = false;
+    e->num_full = (int)fidx.size();
+    e->h_fidx = std::move(fidx);
+    e->full_lists_valid = false;
     e->subset_ready = true;
+    return 0;
+}
+
+// Per-pixel photon lists (mask < 2) restricted to the full frames, frame index f, with the
+// offsets of tom_parts ranges of f: the plain path's partitioned tomogram lists for RF.
+int build_full_lists(emc_engine *e) {
+    const int P = e->num_pix, nF = e->num_full, nparts = e->tom_parts;
+    std::vector<int> &fof = e->h_fof;
+    fof.assign(e->num_data, -1);
+    for (int f = 0; f < nF; ++f) fof[e->h_fidx[f]] = f;
+    std::vector<long> &off2 = e->h_ftom_off2;
+    off2.resize((size_t)P * nparts + 1);
+    std::vector<int> &fr = e->h_ftom_frame;
+    std::vector<double> &cn = e->h_ftom_cnt;
+    fr.clear();
+    cn.clear();
+    for (int t = 0; t < P; ++t) {
+        int x = 0;
+        for (long k = e->h_tom_off[t]; k < e->h_tom_off[t + 1]; ++k) {
+            const int f = fof[e->h_tom_frame[k]];
+            if (f < 0) continue;
+            while (x < nparts && (long)nF * x / nparts <= f) off2[(size_t)t * nparts + x++] = (long)fr.size();
+            fr.push_back(f);
+            cn.push_back(e->h_tom_cnt[k]);
+        }
+        while (x < nparts) off2[(size_t)t * nparts + x++] = (long)fr.size();
+    }
+    off2[(size_t)P * nparts] = (long)fr.size();
+    if (fr.empty()) {
+        fr.push_back(0);
+        cn.push_back(0.);
+    }
+    int rc;
+    if ((rc = e->ftom_off2.upload(off2.data(), off2.size(), e->stream))) return rc;
+    if ((rc = e->ftom_frame.upload(fr.data(), fr.size(), e->stream))) return rc;
+    if ((rc = e->ftom_cnt.upload(cn.data(), cn.size(), e->stream))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->full_lists_valid = true;
     return 0;
 }
 
@@ -625,6 +707,9 @@ int merge_chunk(emc_engine *e, int chunk, const double *view, bool all_on, bool 
 int ensure_work(emc_engine *e) {
     int rc;
     const size_t N = e->num_data, Rp = e->Rp, P = e->num_pix;
+    // the sparse-subset rows are not needed by the plain path
+    e->Rsub.release();
+    e->RF.release();
     if ((rc = e->R.alloc(N * Rp))) return rc;
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
@@ -730,22 +815,37 @@ int ensure_work(emc_engine *e) {
     return 0;
 }
 
-// One iteration over the refinement subsets only (pairs of frames and slot tiles). Same
-// outputs as emc_maximize's plain path; per-orientation outputs are mapped back to the
-// caller's orientation order.
+// One iteration over the refinement subsets only (pairs of partial frames and slot tiles, and
+// every slot tile of the full frames). Same outputs as emc_maximize's plain path;
+// per-orientation outputs are mapped back to the caller's orientation order.
 int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats) {
     int rc;
     hipStream_t s = e->stream;
-    const int N = e->num_data, Rn = e->num_rot, P = e->num_pix, S = e->num_slots;
+    const int N = e->num_data, Rn = e->num_rot, P = e->num_pix, S = e->num_slots, T = e->num_tiles;
+    const int nF = e->num_full, nparts = e->tom_parts;
     const size_t vol = (size_t)e->size * e->size * e->size;
     if (!e->fsort_valid && (rc = build_fsort(e))) return rc;
-    // memory: pair rows, one slot chunk of log W / view, per-slot and per-frame arrays
+    if (nF > 0 && !e->full_lists_valid && (rc = build_full_lists(e))) return rc;
+    // memory: the plain path's rows are not needed here
+    e->R.release();
+    e->tom_partial.release();
     if ((rc = e->Rsub.alloc((size_t)std::max<long>(e->num_pairs, 1) * emc::kRTile))) return rc;
+    if (nF > 0) {
+        if ((rc = e->RF.alloc((size_t)nF * T * emc::kRTile))) return rc;
+        const size_t pp = (nF + kFramesPerPsumPart - 1) / kFramesPerPsumPart;
+        if ((rc = e->psum_full_part.alloc(pp * (size_t)S))) return rc;
+        if ((rc = e->psum_full.alloc(S))) return rc;
+        if ((rc = e->scale_full.alloc(nF))) return rc;
+        if ((rc = e->blacklist_full.alloc(nF))) return rc;
+    } else {
+        e->RF.release();
+    }
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     size_t budget = (freeb > ((size_t)4 << 30) ? freeb - ((size_t)4 << 30) : 0) + e->logw.n * sizeof(double);
     budget = std::min(budget, (size_t)120 << 30);
-    size_t cmax = budget / ((size_t)P * sizeof(double));
+    // per slot column: log W / view, plus the full frames' partial views
+    size_t cmax = budget / ((size_t)P * sizeof(double) * (1 + (nF > 0 ? nparts : 0)));
     if (e->max_chunk_cols > 0) cmax = std::min(cmax, (size_t)e->max_chunk_cols);
     cmax = cmax / emc::kRTile * emc::kRTile;
     if (cmax < (size_t)emc::kRTile) return fail(EMC_E_NOMEM, "no device memory for an orientation chunk");
@@ -758,6 +858,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
         e->bricks_valid = false;
         if ((rc = e->logw.alloc((size_t)P * C))) return rc;
     }
+    if (nF > 0 && (rc = e->tom_partial.alloc((size_t)nparts * P * (size_t)C))) return rc;
     const int ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
     if ((rc = e->upart.alloc((size_t)ngroups * C))) return rc;
     if ((rc = e->ubad.alloc((size_t)ngroups * C))) return rc;
@@ -776,6 +877,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     if ((rc = e->psum_d.alloc(N))) return rc;
     if ((rc = e->rmax.alloc(N))) return rc;
     if ((rc = e->d_count.alloc(1))) return rc;
+    const emc::FullRows full{nF > 0 ? e->RF.p : nullptr, nF > 0 ? e->ffull.p : nullptr, nF, T};
 
     const int use_scale = opts->need_scaling && (opts->iteration > 1 || opts->known_scale);
     const int want_psum_d = opts->need_scaling && opts->update_scale;
@@ -799,15 +901,18 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
         const int Cc = std::min(C, S - s0);
         if ((rc = timed(0, [&] {
                  emc::launch_expand(s, e->rots_slot.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p, P,
-                                    s0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p);
+                                    s0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p, e->slot_rot.p);
                  emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, s0, S, e->u_slot.p, e->rbad_slot.p);
                  return 0;
              })))
             return rc;
         if ((rc = timed(1, [&] {
                  emc::launch_pair_likelihood(s, e->logw.p, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
-                                             e->pair_frame.p, e->pair_tile.p, e->num_pairs, s0 / emc::kRTile,
-                                             (s0 + Cc) / emc::kRTile, e->Rsub.p);
+                                             e->pair_frame.p, e->pair_tile.p, e->tile_off.p, e->tile_pair.p,
+                                             e->num_pairs, s0 / emc::kRTile, (s0 + Cc) / emc::kRTile, e->Rsub.p);
+                 if (nF > 0)
+                     emc::launch_likelihood_mapped(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
+                                                   e->blacklist.p, e->fidx.p, nF, e->RF.p, s0);
                  return 0;
              })))
             return rc;
@@ -818,25 +923,31 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
          })))
         return rc;
     if ((rc = timed(3, [&] {
-             emc::launch_pair_max(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, e->slot_rot.p, e->logwr_slot.p,
-                                  e->u_slot.p, e->rbad_slot.p, e->scale.p, use_scale, e->scalars.p,
+             emc::launch_pair_max(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, full, e->slot_rot.p,
+                                  e->logwr_slot.p, e->u_slot.p, e->rbad_slot.p, e->scale.p, use_scale, e->scalars.p,
                                   e->blacklist.p, dense, N, e->max_exp.p, e->rmax.p);
              return 0;
          })))
         return rc;
     if ((rc = timed(4, [&] {
-             emc::launch_pair_norm(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, e->slot_rot.p, e->beta.p,
+             emc::launch_pair_norm(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, full, e->slot_rot.p, e->beta.p,
                                    e->blacklist.p, e->max_exp.p, e->rmax.p, N, e->p_norm.p);
              return 0;
          })))
         return rc;
     if ((rc = timed(5, [&] {
-             emc::launch_pair_posterior(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, e->slot_rot.p,
+             emc::launch_pair_posterior(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, full, e->slot_rot.p,
                                         e->logwr_slot.p, e->u_slot.p, e->beta.p, e->sum_fact.p, e->blacklist.p,
                                         e->max_exp.p, e->rmax.p, e->p_norm.p, e->scalars.p, want_psum_d, N,
                                         e->likelihood.p, e->info.p, e->psum_d.p);
-             emc::launch_pair_psum(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p, e->num_tiles,
-                                   e->scale.p, opts->need_scaling, e->psum_slot.p);
+             if (nF > 0) {
+                 emc::launch_frame_gather(s, e->scale.p, e->fidx.p, nF, e->scale_full.p);
+                 emc::launch_frame_gather(s, e->blacklist.p, e->fidx.p, nF, e->blacklist_full.p);
+                 emc::launch_psum_r(s, e->RF.p, S, nF, kFramesPerPsumPart, e->scale_full.p, opts->need_scaling,
+                                    e->blacklist_full.p, e->psum_full_part.p, e->psum_full.p);
+             }
+             emc::launch_pair_psum(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p, T, e->scale.p,
+                                   opts->need_scaling, nF > 0 ? e->psum_full.p : nullptr, e->psum_slot.p);
              emc::launch_slot_gather(s, e->psum_slot.p, e->rot_slot.p, Rn, e->psum_r.p);
              emc::launch_slot_gather(s, e->u_slot.p, e->rot_slot.p, Rn, e->u.p);
              return 0;
@@ -875,20 +986,24 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
         return rc;
     for (size_t k = 0; k < e->slot_bricks.size(); ++k) {
         const auto &bc = e->slot_bricks[k];
+        const int t0 = bc.r0 / emc::kRTile, t1 = (bc.r0 + bc.Cc) / emc::kRTile;
+        if (nF > 0 && (rc = timed(6, [&] {
+                           emc::launch_tomogram_parts(s, e->RF.p, nF, bc.r0, bc.Cc, e->ftom_off2.p, nparts,
+                                                      e->ftom_frame.p, e->ftom_cnt.p, nullptr, S, P,
+                                                      e->tom_partial.p, e->logw.p, 0, 16, 0);
+                           return 0;
+                       })))
+            return rc;
+        if (nF > 0 && (rc = timed(7, [&] {
+                           emc::launch_tomogram_reduce_rmajor(s, e->tom_partial.p, nparts, bc.r0, bc.Cc, nullptr, S,
+                                                              P, e->logw.p);
+                           return 0;
+                       })))
+            return rc;
         if ((rc = timed(6, [&] {
-                 const int t0 = bc.r0 / emc::kRTile, t1 = (bc.r0 + bc.Cc) / emc::kRTile;
                  emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p,
                                            e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
-                                           e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P, e->tile_dense.p,
-                                           e->logw.p);
-                 const auto &dt = e->h_dense_tiles;
-                 const long lo = std::lower_bound(dt.begin(), dt.end(), t0) - dt.begin();
-                 const long hi = std::lower_bound(dt.begin(), dt.end(), t1) - dt.begin();
-                 if (hi > lo)
-                     emc::launch_pair_tomogram_dense(s, e->Rsub.p, e->dense_tiles.p + lo, (int)(hi - lo),
-                                                     e->tile_cell.p, e->tile_sub.p, e->pair_of.p, e->num_cells,
-                                                     e->tom_off.p, e->tom_frame.p, e->tom_cnt.p, e->psum_slot.p,
-                                                     t0, P, e->logw.p);
+                                           e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P, nF > 0, e->logw.p);
                  return 0;
              })))
             return rc;
@@ -947,7 +1062,6 @@ int emc_create(int device, emc_engine **out) {
     if (const char *v = getenv("EMC_TOMO_SEQ")) e->tomo_seq = atoi(v);
     if (const char *v = getenv("EMC_MERGE_MODE")) e->merge_mode = atoi(v);
     if (const char *v = getenv("EMC_SUBSET")) e->subset_mode = atoi(v);
-    if (const char *v = getenv("EMC_DENSE_TILE_FRAC")) e->dense_tile_frac = atof(v);
 
     *out = e;
     return 0;
@@ -1056,6 +1170,8 @@ int emc_set_quats(emc_engine *e, int num_rot, const double *quats) {
     e->weight_all_valid = false;
     e->bricks_valid = false;
     e->refine = false;  // the orientation -> coarse cell map no longer applies
+    e->h_slot_qmap.clear();
+    e->slot_num_coarse = 0;
     return 0;
 }
 
@@ -1152,7 +1268,8 @@ int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const
         qm[r] = quat_mapping[r];
     }
     if (rel_off[0] != 0) return fail(EMC_E_ARG, "emc_set_refine: rel_off[0] must be 0");
-    std::vector<uint32_t> bits((size_t)N * words, 0u);
+    std::vector<uint32_t> &bits = e->h_relbits;
+    bits.assign((size_t)N * words, 0u);
     for (int d = 0; d < N; ++d) {
         if (rel_off[d + 1] < rel_off[d]) return fail(EMC_E_ARG, "emc_set_refine: rel_off not ascending");
         for (long k = rel_off[d]; k < rel_off[d + 1]; ++k) {
@@ -1168,6 +1285,32 @@ int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const
     e->rel_words = words;
     e->refine = true;
     return build_subset(e, num_coarse, quat_mapping, rel_off, rel_quat);
+}
+
+int emc_cell_posterior(emc_engine *e, int num_coarse, const int *quat_mapping, float *out) {
+    if (!e || !quat_mapping || !out) return fail(EMC_E_ARG, "emc_cell_posterior: null argument");
+    if (num_coarse <= 0 || num_coarse > 6144)
+        return fail(EMC_E_ARG, "emc_cell_posterior: num_coarse %d (1 to 6144)", num_coarse);
+    if (!e->have_results) return fail(EMC_E_STATE, "emc_cell_posterior: no iteration has run");
+    HIP_TRY(hipSetDevice(e->device));
+    const int N = e->num_data, Rn = e->num_rot;
+    for (int r = 0; r < Rn; ++r)
+        if (quat_mapping[r] < 0 || quat_mapping[r] >= num_coarse)
+            return fail(EMC_E_ARG, "emc_cell_posterior: quat_mapping[%d] = %d out of range", r, quat_mapping[r]);
+    DevBuf<int> qm;
+    DevBuf<float> dout;
+    int rc;
+    if ((rc = qm.upload(quat_mapping, Rn, e->stream))) return rc;
+    if ((rc = dout.alloc((size_t)N * num_coarse))) return rc;
+    const bool plain = !e->subset_ran;
+    const emc::FullRows full{e->num_full > 0 ? e->RF.p : nullptr, e->num_full > 0 ? e->ffull.p : nullptr,
+                             e->num_full, e->num_tiles};
+    emc::launch_cell_posterior(e->stream, plain ? e->R.p : e->Rsub.p, plain, N, Rn, e->pair_off.p, e->pair_tile.p,
+                               full, e->slot_rot.p, qm.p, num_coarse, dout.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)N * num_coarse * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
 }
 
 int emc_set_precision(emc_engine *e, int mode) {
@@ -1562,6 +1705,9 @@ int emc_get_field(emc_engine *e, int field, void *dst, size_t count) {
                 HIP_TRY(hipMemsetAsync(dense.p, 0, N * Rn * sizeof(double), e->stream));
                 emc::launch_pair_scatter(e->stream, e->Rsub.p, e->pair_frame.p, e->pair_tile.p, e->slot_rot.p,
                                          e->num_pairs, (int)Rn, dense.p);
+                if (e->num_full > 0)
+                    emc::launch_full_scatter(e->stream, e->RF.p, e->num_full, e->num_tiles, e->fidx.p,
+                                             e->slot_rot.p, (int)Rn, dense.p);
                 HIP_TRY(hipMemcpyAsync(dst, dense.p, N * Rn * sizeof(double), hipMemcpyDeviceToHost, e->stream));
                 HIP_TRY(hipStreamSynchronize(e->stream));
                 dense.release();

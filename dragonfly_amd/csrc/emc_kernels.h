@@ -10,7 +10,8 @@ namespace emc {
 
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
-                   int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad);
+                   int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad,
+                   const int *slot_rot = nullptr);
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad);
@@ -18,6 +19,10 @@ void launch_to_f32(hipStream_t s, const double *P, int num_data, int r0, int C, 
 void launch_likelihood(hipStream_t s, const float *logw, int C, int num_pix, const long *off,
                        const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
                        double *G, int r0, int tiles, int unroll);
+// fmap: row d of G holds frame fmap[d] (photon lists and blacklist indexed by fmap[d]).
+void launch_likelihood_mapped(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
+                              const int *pix, const double *cnt, const uint8_t *blacklist, const int *fmap,
+                              int num_rows, double *G, int r0);
 void launch_tomogram_parts_f32(hipStream_t s, const float *P32, int num_data, int C, const long *toff2,
                                int nparts, const int *tframe, const double *tcnt, int num_pix,
                                double *partial);
@@ -114,34 +119,46 @@ void launch_posterior_fp16(hipStream_t s, const double *P, int num_data, int r0,
                            double pscale, _Float16 *p16);
 
 // ---- sparse orientation subsets (emc_subset.hip) ----
+// Rows of full frames (every coarse cell), tile-major like the plain path:
+// RF[(T * nF + f) * 64 + j], f = ffull[d] (-1 for partial frames).
+struct FullRows {
+    double *RF;
+    const int *ffull;
+    int nF;
+    int num_tiles;
+};
 void launch_pair_likelihood(hipStream_t s, const double *logw, int num_pix, const long *off, const int *pix,
-                            const double *cnt, const int *pair_frame, const int *pair_tile, long num_pairs,
-                            int t0, int t1, double *G);
-void launch_pair_max(hipStream_t s, double *R, const long *pair_off, const int *pair_tile, const int *slot_rot,
-                     const double *logwr_slot, const double *u_slot, const int *rbad_slot, const double *scale,
-                     int use_scale, const double *scalars, const uint8_t *blacklist, const uint8_t *dense,
-                     int num_data, double *max_exp, int *rmax);
-void launch_pair_norm(hipStream_t s, const double *R, const long *pair_off, const int *pair_tile,
+                            const double *cnt, const int *pair_frame, const int *pair_tile, const long *tile_off,
+                            const int *tile_pair, long num_pairs, int t0, int t1, double *G);
+void launch_pair_max(hipStream_t s, double *R, const long *pair_off, const int *pair_tile, const FullRows &full,
+                     const int *slot_rot, const double *logwr_slot, const double *u_slot, const int *rbad_slot,
+                     const double *scale, int use_scale, const double *scalars, const uint8_t *blacklist,
+                     const uint8_t *dense, int num_data, double *max_exp, int *rmax);
+void launch_pair_norm(hipStream_t s, double *R, const long *pair_off, const int *pair_tile, const FullRows &full,
                       const int *slot_rot, const double *beta, const uint8_t *blacklist, const double *max_exp,
                       const int *rmax, int num_data, double *p_norm);
 void launch_pair_posterior(hipStream_t s, double *R, const long *pair_off, const int *pair_tile,
-                           const int *slot_rot, const double *logwr_slot, const double *u_slot,
-                           const double *beta, const double *sum_fact, const uint8_t *blacklist,
-                           const double *max_exp, const int *rmax, const double *p_norm, const double *scalars,
-                           int want_psum_d, int num_data, double *likelihood, double *info, double *psum_d);
+                           const FullRows &full, const int *slot_rot, const double *logwr_slot,
+                           const double *u_slot, const double *beta, const double *sum_fact,
+                           const uint8_t *blacklist, const double *max_exp, const int *rmax, const double *p_norm,
+                           const double *scalars, int want_psum_d, int num_data, double *likelihood, double *info,
+                           double *psum_d);
 void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                       const int *pair_frame, int num_tiles, const double *scale, int need_scaling,
-                      double *psum_slot);
+                      const double *psum_full, double *psum_slot);
 void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                           const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
                           const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
-                          const uint8_t *tile_dense, double *view);
-void launch_pair_tomogram_dense(hipStream_t s, const double *P, const int *dense_tiles, int ndense,
-                                const int *tile_cell, const int *tile_sub, const int *pair_of, int num_cells,
-                                const long *toff, const int *tframe, const double *tcnt, const double *psum_slot,
-                                int t0, int num_pix, double *view);
+                          int has_full, double *view);
 void launch_pair_scatter(hipStream_t s, const double *P, const int *pair_frame, const int *pair_tile,
                          const int *slot_rot, long num_pairs, int num_rot, double *out);
+void launch_full_scatter(hipStream_t s, const double *RF, int nF, int num_tiles, const int *fidx,
+                         const int *slot_rot, int num_rot, double *out);
 void launch_slot_gather(hipStream_t s, const double *in, const int *rot_slot, int num_rot, double *out);
+void launch_cell_posterior(hipStream_t s, double *R, int plain, int num_data, int num_rot, const long *pair_off,
+                           const int *pair_tile, const FullRows &full, const int *slot_rot, const int *qmap,
+                           int num_coarse, float *out);
+void launch_frame_gather(hipStream_t s, const double *in, const int *fidx, int nF, double *out);
+void launch_frame_gather(hipStream_t s, const uint8_t *in, const int *fidx, int nF, uint8_t *out);
 
 }  // namespace emc

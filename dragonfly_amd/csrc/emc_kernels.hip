@@ -50,15 +50,19 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
     const double *__restrict__ rots, const double *__restrict__ model, long size,
     const double *__restrict__ qvals, const double *__restrict__ corr,
     const uint8_t *__restrict__ mask, int num_pix, int r0, int C, int pix_per_group,
-    T *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad) {
+    T *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad,
+    const int *__restrict__ slot_rot) {
     __shared__ double s_rot[kRTile * 9];
     __shared__ double s_tile[64][kRTile + 1];
     __shared__ double s_u[kWavesPerBlock][kRTile];
     __shared__ int s_bad[kWavesPerBlock][kRTile];
+    __shared__ int s_pad[kRTile];
     const int tid = threadIdx.x;
     const int c0 = blockIdx.x * kRTile;
     const int g = blockIdx.y;
     for (int i = tid; i < kRTile * 9; i += kBlock) s_rot[i] = rots[(size_t)(r0 + c0) * 9 + i];
+    // padding slots of the subset order (slot_rot < 0) get W = 1 without interpolation
+    if (tid < kRTile) s_pad[tid] = slot_rot != nullptr && slot_rot[r0 + c0 + tid] < 0;
     __syncthreads();
     const double center = (double)(size / 2);
     const int lane = tid & 63, wave = tid >> 6;
@@ -73,7 +77,7 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
                          q2 = qvals[3 * (size_t)t + 2], cr = corr[t];
             for (int k = 0; k < kRTile / kWavesPerBlock; ++k) {
                 const int rr = wave + kWavesPerBlock * k;
-                s_tile[lane][rr] = slice_value(model, size, &s_rot[rr * 9], q0, q1, q2, cr, center);
+                s_tile[lane][rr] = s_pad[rr] ? 1. : slice_value(model, size, &s_rot[rr * 9], q0, q1, q2, cr, center);
             }
         }
         __syncthreads();
@@ -144,16 +148,18 @@ __global__ __launch_bounds__(kBlock) void likelihood_kernel(
     const T *__restrict__ logw, int C, size_t tsW, const long *__restrict__ off,
     const int *__restrict__ pix, const double *__restrict__ cnt,
     const uint8_t *__restrict__ blacklist, int num_data, double *__restrict__ G, size_t tsG,
-    int r0) {
+    int r0, const int *__restrict__ fmap) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int d = blockIdx.x * kWavesPerBlock + wave;
-    if (d >= num_data || blacklist[d]) return;
+    if (d >= num_data) return;
+    const int fd = fmap ? fmap[d] : d;  // frame whose photons row d holds
+    if (blacklist[fd]) return;
     const int c0 = blockIdx.y * (kRTile * NT);
     const int nt = min(NT, (C - c0) / kRTile);
     const T *__restrict__ col = logw + (size_t)(c0 >> 6) * tsW + lane;
-    const long k1 = off[d + 1];
-    long k = off[d];
+    const long k1 = off[fd + 1];
+    long k = off[fd];
     double acc[NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[j] = 0.;
@@ -526,7 +532,7 @@ __global__ __launch_bounds__(kBlock) void tomogram_reduce_rmajor_kernel(
     const int t0 = blockIdx.x * 64, c0 = blockIdx.y * kRTile;
     const size_t tsW = (size_t)num_pix * kRTile, tsPart = tsW * (C / kRTile);
     const int r = r0 + c0 + lane;
-    const double ps = (r < num_rot) ? psum_r[r] : 0.;
+    const double ps = (psum_r && r < num_rot) ? psum_r[r] : 0.;  // no psum: sums only
     for (int k = wave; k < 64; k += kWavesPerBlock) {
         const int t = t0 + k;
         if (t >= num_pix) break;
@@ -932,10 +938,11 @@ static inline unsigned grid_cap(size_t blocks) { return (unsigned)(blocks < 6553
 
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
-                   int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad) {
+                   int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad,
+                   const int *slot_rot) {
     dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
     expand_kernel<double, kRTile><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0, C,
-                                                          pix_per_group, logw, upart, ubad);
+                                                          pix_per_group, logw, upart, ubad, slot_rot);
 }
 
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
@@ -943,7 +950,7 @@ void launch_expand(hipStream_t s, const double *rots, const double *model, long 
                    int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad) {
     dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
     expand_kernel<float, 2 * kRTile><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0,
-                                                             C, pix_per_group, logw, upart, ubad);
+                                                             C, pix_per_group, logw, upart, ubad, nullptr);
 }
 
 // FP32 copy of one orientation chunk of the tile-major posterior in 128-wide tiles:
@@ -1062,12 +1069,12 @@ void launch_rescale(hipStream_t s, const double *quats, const double *u, int num
 template <typename T>
 static void likelihood_launch(hipStream_t s, const T *logw, int C, int num_pix, const long *off,
                               const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
-                              double *G, int r0, int tiles, int unroll) {
+                              double *G, int r0, int tiles, int unroll, const int *fmap = nullptr) {
     const unsigned gx = cdiv(num_data, kWavesPerBlock);
     const size_t tsW = (size_t)num_pix * kRTile, tsG = (size_t)num_data * kRTile;
 #define EMC_LIK(NT, U)                                                                           \
     likelihood_kernel<NT, U, T><<<dim3(gx, cdiv(C, kRTile * NT)), kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, \
-                                                                                blacklist, num_data, G, tsG, r0)
+                                                                                blacklist, num_data, G, tsG, r0, fmap)
     if (tiles >= 4) EMC_LIK(4, 2);
     else if (tiles == 2) EMC_LIK(2, 4);
     else if (unroll >= 16) EMC_LIK(1, 16);
@@ -1080,6 +1087,12 @@ void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, co
                        const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
                        double *G, int r0, int tiles, int unroll) {
     likelihood_launch(s, logw, C, num_pix, off, pix, cnt, blacklist, num_data, G, r0, tiles, unroll);
+}
+
+void launch_likelihood_mapped(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
+                              const int *pix, const double *cnt, const uint8_t *blacklist, const int *fmap,
+                              int num_rows, double *G, int r0) {
+    if (num_rows > 0) likelihood_launch(s, logw, C, num_pix, off, pix, cnt, blacklist, num_rows, G, r0, 1, 8, fmap);
 }
 
 void launch_likelihood(hipStream_t s, const float *logw, int C, int num_pix, const long *off,

@@ -109,14 +109,18 @@ __global__ __launch_bounds__(kBlock) void merge_brick_kernel(
         if (!all_on && !(psum_r[r] > 0.)) continue;
         const int t = p * 64 + lane;
         if (t >= num_pix || mask[t] > 1) continue;
+        double w = 0.;
+        if (kModel) {
+            w = view[(size_t)c * num_pix + t] / corr[t];
+            // model-only merge: a zero view adds exactly nothing
+            if (!kWeight && w == 0.) continue;
+        }
         Cell cell;
         if (!locate(rots + 9 * (size_t)r, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1],
                     qvals[3 * (size_t)t + 2], center, size, cell))
             continue;
         const int lx = (int)cell.x - X0, ly = (int)cell.y - Y0, lz = (int)cell.z - Z0;
         if (lx < -1 || lx >= kBrick || ly < -1 || ly >= kBrick || lz < -1 || lz >= kBrick) continue;
-        double w = 0.;
-        if (kModel) w = view[(size_t)c * num_pix + t] / corr[t];
         const double cx = 1. - cell.fx, cy = 1. - cell.fy, cz = 1. - cell.fz;
         const double fx = cell.fx, fy = cell.fy, fz = cell.fz;
         const double f[8] = {cx * cy * cz, cx * cy * fz, cx * fy * cz, cx * fy * fz,

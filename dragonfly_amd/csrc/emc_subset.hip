@@ -1,12 +1,18 @@
 // Per-frame orientation subsets evaluated sparsely (refinement, legacy src/max_emc.c:264-274).
 //
 // With refinement every frame is evaluated only at the orientations of its coarse cells. The
-// engine then orders orientations by coarse cell and pads every cell to whole tiles of 64
+// engine orders orientations by coarse cell and pads every cell to whole tiles of 64
 // ("slots": slot s holds orientation slot_rot[s], or -1 for padding), so the orientations a
-// frame may take are a union of whole slot tiles. A (frame, tile) pair is one row of 64
-// values; the pairs of a frame are contiguous (pair_off[d] .. pair_off[d + 1]) and every
-// kernel below works on pairs only, so the work and memory scale with the subsets' size, not
-// with frames x orientations.
+// frame may take are a union of whole slot tiles.
+//
+// Two kinds of frames:
+//  * partial frames (some cells): a (frame, tile) pair is one row of 64 values; the pairs of a
+//    frame are contiguous in R (pair_off[d] .. pair_off[d + 1]) and a tile-major copy of the
+//    pair list (tile_off / tile_pair, frames ascending) drives the per-tile passes;
+//  * full frames (every cell: the fall-back to the plain search): their rows are kept
+//    tile-major like the plain path, RF[(T * nF + f) * 64 + j] with f = ffull[d], so the plain
+//    path's likelihood, psum and partitioned tomogram kernels run on them unchanged.
+// Work and memory scale with the pairs of partial frames plus nF x tiles.
 //
 // The values are the plain path's: the same likelihood sums in the same photon order, the
 // same first arg-max (ties broken by the original orientation index), the same keep rule
@@ -19,42 +25,83 @@
 
 namespace emc {
 
-// G[k][lane] = sum over the frame's photons of count * logW[tile - t0][pix][lane] for pairs
-// whose tile lies in [t0, t1) (one orientation chunk); other pairs are left alone.
+namespace {
+
+// The rows of frame d: element e (e0 <= e < e1) has address at(e) and slot `slot`.
+struct FrameRows {
+    double *base;
+    long e0, e1;
+    int f, nF;
+    const int *pair_tile;
+
+    __device__ __forceinline__ double *at(long e, int &slot) const {
+        if (f >= 0) {
+            slot = (int)e;
+            return base + ((size_t)(e / kRTile) * nF + f) * kRTile + (e % kRTile);
+        }
+        slot = pair_tile[e / kRTile] * kRTile + (int)(e % kRTile);
+        return base + e;
+    }
+};
+
+__device__ __forceinline__ FrameRows frame_rows(double *R, const long *pair_off, const int *pair_tile,
+                                                const FullRows &full, int d) {
+    FrameRows fr;
+    fr.f = full.ffull ? full.ffull[d] : -1;
+    fr.nF = full.nF;
+    fr.pair_tile = pair_tile;
+    if (fr.f >= 0) {
+        fr.base = full.RF;
+        fr.e0 = 0;
+        fr.e1 = (long)full.num_tiles * kRTile;
+    } else {
+        fr.base = R;
+        fr.e0 = pair_off[d] * kRTile;
+        fr.e1 = pair_off[d + 1] * kRTile;
+    }
+    return fr;
+}
+
+}  // namespace
+
+// G[k][lane] = sum over the frame's photons of count * logW[tile - t0][pix][lane] for the
+// partial-frame pairs of tiles [t0, t1). Pairs are walked tile-major (tile_off / tile_pair), so
+// the waves resident at one time read rows of the same log W tile.
 template <int U>
 __global__ __launch_bounds__(kBlock) void pair_likelihood_kernel(
     const double *__restrict__ logw, size_t tsW, const long *__restrict__ off,
     const int *__restrict__ pix, const double *__restrict__ cnt, const int *__restrict__ pair_frame,
-    const int *__restrict__ pair_tile, long num_pairs, int t0, int t1, double *__restrict__ G) {
+    const int *__restrict__ pair_tile, const long *__restrict__ tile_off, const int *__restrict__ tile_pair,
+    int t0, int t1, double *__restrict__ G) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
+    const long j1 = tile_off[t1];
     const long stride = (long)gridDim.x * kWavesPerBlock;
-    for (long k = (long)blockIdx.x * kWavesPerBlock + wave; k < num_pairs; k += stride) {
-        const int T = pair_tile[k];
-        if (T < t0 || T >= t1) continue;
+    for (long j = tile_off[t0] + (long)blockIdx.x * kWavesPerBlock + wave; j < j1; j += stride) {
+        const int k = tile_pair[j];
         const int d = pair_frame[k];
-        const double *__restrict__ col = logw + (size_t)(T - t0) * tsW + lane;
-        long j = off[d];
-        const long j1 = off[d + 1];
+        const double *__restrict__ col = logw + (size_t)(pair_tile[k] - t0) * tsW + lane;
+        long q = off[d];
+        const long q1 = off[d + 1];
         double acc = 0.;
-        for (; j + U <= j1; j += U) {
+        for (; q + U <= q1; q += U) {
             double v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = col[(size_t)pix[j + u] * kRTile];
+            for (int u = 0; u < U; ++u) v[u] = col[(size_t)pix[q + u] * kRTile];
 #pragma unroll
-            for (int u = 0; u < U; ++u) acc += cnt[j + u] * v[u];
+            for (int u = 0; u < U; ++u) acc += cnt[q + u] * v[u];
         }
-        for (; j < j1; ++j) acc += cnt[j] * col[(size_t)pix[j] * kRTile];
+        for (; q < q1; ++q) acc += cnt[q] * col[(size_t)pix[q] * kRTile];
         G[(size_t)k * kRTile + lane] = acc;
     }
 }
 
-// Per frame over its pairs: R = log w - u phi + G for real, allowed orientations (-inf for
+// Per frame over its rows: R = log w - u phi + G for real, allowed orientations (-inf for
 // padding slots and, for dense frames, orientations with non-finite log W), maximum and first
 // arg-max by original orientation index.
 __global__ __launch_bounds__(kBlock) void pair_max_kernel(
     double *__restrict__ R, const long *__restrict__ pair_off, const int *__restrict__ pair_tile,
-    const int *__restrict__ slot_rot, const double *__restrict__ logwr_slot,
+    FullRows full, const int *__restrict__ slot_rot, const double *__restrict__ logwr_slot,
     const double *__restrict__ u_slot, const int *__restrict__ rbad_slot,
     const double *__restrict__ scale, int use_scale, const double *__restrict__ scalars,
     const uint8_t *__restrict__ blacklist, const uint8_t *__restrict__ dense,
@@ -71,17 +118,16 @@ __global__ __launch_bounds__(kBlock) void pair_max_kernel(
     }
     const bool dense_frame = dense != nullptr && dense[d];
     const double phi = use_scale ? scale[d] : scalars[0];
-    const long k0 = pair_off[d], k1 = pair_off[d + 1];
+    const FrameRows fr = frame_rows(R, pair_off, pair_tile, full, d);
     double best = kMaxExpStart;
     int bi = -1;
-    for (long e = k0 * kRTile + threadIdx.x; e < k1 * kRTile; e += kBlock) {
-        const long k = e / kRTile;
-        const int s = pair_tile[k] * kRTile + (int)(e % kRTile);
+    for (long e = fr.e0 + threadIdx.x; e < fr.e1; e += kBlock) {
+        int s;
+        double *x = fr.at(e, s);
         const int r = slot_rot[s];
-        double &x = R[e];
         double v = -INFINITY;
-        if (r >= 0 && !(dense_frame && rbad_slot[s])) v = (logwr_slot[s] - u_slot[s] * phi) + x;
-        x = v;
+        if (r >= 0 && !(dense_frame && rbad_slot[s])) v = (logwr_slot[s] - u_slot[s] * phi) + *x;
+        *x = v;
         if (v > best || (v == best && r >= 0 && bi >= 0 && r < bi)) {
             best = v;
             bi = r;
@@ -126,10 +172,10 @@ __device__ __forceinline__ double block_sum_pairs(double v, double *s) {
     return r;
 }
 
-// p_norm over the frame's kept pairs (normalize_prob, max_prob.c:245-250).
+// p_norm over the frame's kept orientations (normalize_prob, max_prob.c:245-250).
 __global__ __launch_bounds__(kBlock) void pair_norm_kernel(
-    const double *__restrict__ R, const long *__restrict__ pair_off, const int *__restrict__ pair_tile,
-    const int *__restrict__ slot_rot, const double *__restrict__ beta,
+    double *__restrict__ R, const long *__restrict__ pair_off, const int *__restrict__ pair_tile,
+    FullRows full, const int *__restrict__ slot_rot, const double *__restrict__ beta,
     const uint8_t *__restrict__ blacklist, const double *__restrict__ max_exp,
     const int *__restrict__ rmax, double *__restrict__ p_norm) {
     __shared__ double s[kBlock];
@@ -140,10 +186,12 @@ __global__ __launch_bounds__(kBlock) void pair_norm_kernel(
         return;
     }
     const double b = beta[d], M = max_exp[d], c = kPdiffThresh / b;
+    const FrameRows fr = frame_rows(R, pair_off, pair_tile, full, d);
     double acc = 0.;
-    for (long e = pair_off[d] * kRTile + threadIdx.x; e < pair_off[d + 1] * kRTile; e += kBlock) {
-        const int r = slot_rot[pair_tile[e / kRTile] * kRTile + (int)(e % kRTile)];
-        const double v = R[e];
+    for (long e = fr.e0 + threadIdx.x; e < fr.e1; e += kBlock) {
+        int sl;
+        const double v = *fr.at(e, sl);
+        const int r = slot_rot[sl];
         if (r >= 0 && pair_kept(v, r, rf, M, c)) acc += exp(b * (v - M));
     }
     const double t = block_sum_pairs(acc, s);
@@ -154,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void pair_norm_kernel(
 // max_update.c:42-140).
 __global__ __launch_bounds__(kBlock) void pair_posterior_kernel(
     double *__restrict__ R, const long *__restrict__ pair_off, const int *__restrict__ pair_tile,
-    const int *__restrict__ slot_rot, const double *__restrict__ logwr_slot,
+    FullRows full, const int *__restrict__ slot_rot, const double *__restrict__ logwr_slot,
     const double *__restrict__ u_slot, const double *__restrict__ beta,
     const double *__restrict__ sum_fact, const uint8_t *__restrict__ blacklist,
     const double *__restrict__ max_exp, const int *__restrict__ rmax, const double *__restrict__ p_norm,
@@ -163,9 +211,12 @@ __global__ __launch_bounds__(kBlock) void pair_posterior_kernel(
     __shared__ double s[kBlock];
     const int d = blockIdx.x;
     const int rf = rmax[d];
-    const long e0 = pair_off[d] * kRTile, e1 = pair_off[d + 1] * kRTile;
+    const FrameRows fr = frame_rows(R, pair_off, pair_tile, full, d);
     if (blacklist[d] || rf < 0) {
-        for (long e = e0 + threadIdx.x; e < e1; e += kBlock) R[e] = 0.;
+        for (long e = fr.e0 + threadIdx.x; e < fr.e1; e += kBlock) {
+            int sl;
+            *fr.at(e, sl) = 0.;
+        }
         if (threadIdx.x == 0) {
             likelihood[d] = 0.;
             info[d] = 0.;
@@ -176,10 +227,11 @@ __global__ __launch_bounds__(kBlock) void pair_posterior_kernel(
     const double b = beta[d], M = max_exp[d], c = kPdiffThresh / b, pn = p_norm[d];
     const double sf = sum_fact[d], rescale = scalars[0], log_pn = log(pn);
     double lk = 0., inf = 0., ps = 0.;
-    for (long e = e0 + threadIdx.x; e < e1; e += kBlock) {
-        const int sl = pair_tile[e / kRTile] * kRTile + (int)(e % kRTile);
+    for (long e = fr.e0 + threadIdx.x; e < fr.e1; e += kBlock) {
+        int sl;
+        double *x = fr.at(e, sl);
         const int r = slot_rot[sl];
-        const double v = R[e];
+        const double v = *x;
         double prob = 0.;
         if (r >= 0 && pair_kept(v, r, rf, M, c)) {
             const double ex = b * (v - M);
@@ -188,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void pair_posterior_kernel(
             ps += prob * u_slot[sl] * rescale;
             if (!(prob < kProbMin)) inf += prob * ((ex - log_pn) - logwr_slot[sl]);
         }
-        R[e] = prob;
+        *x = prob;
     }
     const double tl = block_sum_pairs(lk, s);
     const double ti = block_sum_pairs(inf, s);
@@ -200,18 +252,20 @@ __global__ __launch_bounds__(kBlock) void pair_posterior_kernel(
     }
 }
 
-// psum of every slot of a tile: sum over the tile's pairs (frames ascending) of P * phi'
-// (phi' = scale if need_scaling else 1; blacklisted frames have P = 0). One wave per tile.
+// psum of every slot of a tile: the full frames' sum (psum_full, or 0) plus the sum over the
+// tile's partial-frame pairs (frames ascending) of P * phi' (phi' = scale if need_scaling
+// else 1; blacklisted frames have P = 0). One wave per tile.
 __global__ __launch_bounds__(kBlock) void pair_psum_kernel(const double *__restrict__ P,
                                                            const long *__restrict__ tile_off,
                                                            const int *__restrict__ tile_pair,
                                                            const int *__restrict__ pair_frame, int num_tiles,
                                                            const double *__restrict__ scale, int need_scaling,
+                                                           const double *__restrict__ psum_full,
                                                            double *__restrict__ psum_slot) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int T = blockIdx.x * kWavesPerBlock + wave;
     if (T >= num_tiles) return;
-    double acc = 0.;
+    double acc = psum_full ? psum_full[(size_t)T * kRTile + lane] : 0.;
     for (long j = tile_off[T]; j < tile_off[T + 1]; ++j) {
         const int k = tile_pair[j];
         const double p = P[(size_t)k * kRTile + lane];
@@ -221,32 +275,42 @@ __global__ __launch_bounds__(kBlock) void pair_psum_kernel(const double *__restr
 }
 
 // Tomogram of the chunk's tiles [t0, t0 + ntiles) over pixel ranges of kPairRange pixels:
-// block (tile, range) accumulates K * P (P > 1e-6) of the tile's pairs into LDS rows of the
+// block (tile, range) starts from the full frames' sums already in the view (has_full) or
+// from zero, adds K * P (P > 1e-6) of the tile's partial-frame pairs into LDS rows of the
 // range's pixels (frame photon lists sorted by pixel, range offsets roff[d][range]), divides
 // by psum and writes the orientation-major view[(T - t0) * 64 + j][t].
 __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     const double *__restrict__ P, const long *__restrict__ tile_off, const int *__restrict__ tile_pair,
     const int *__restrict__ pair_frame, const long *__restrict__ foff, const int *__restrict__ roff,
     int nranges, const int *__restrict__ fpix, const double *__restrict__ fcnt,
-    const double *__restrict__ psum_slot, int t0, int num_pix, const uint8_t *__restrict__ tile_dense,
-    double *__restrict__ view) {
+    const double *__restrict__ psum_slot, int t0, int num_pix, int has_full, double *__restrict__ view) {
     __shared__ double acc[kPairRange][kRTile + 1];
     const int T = t0 + blockIdx.y, rg = blockIdx.x;
     const long j0 = tile_off[T], j1 = tile_off[T + 1];
-    if (j0 == j1) return;  // no frame takes this tile: psum 0, the merge skips it
-    if (tile_dense && tile_dense[T]) return;  // gathered per pixel (pair_tomogram_dense_kernel)
+    if (j0 == j1 && !has_full) return;  // no frame takes this tile: psum 0, the merge skips it
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int i = threadIdx.x; i < kPairRange * (kRTile + 1); i += kBlock) (&acc[0][0])[i] = 0.;
-    __syncthreads();
     const int p0 = rg * kPairRange;
+    if (has_full) {
+        // lanes along pixels: coalesced reads of the view rows
+        for (int jj = wave; jj < kRTile; jj += kWavesPerBlock)
+            for (int i = lane; i < kPairRange; i += 64) {
+                const int t = p0 + i;
+                acc[i][jj] = t < num_pix ? view[((size_t)blockIdx.y * kRTile + jj) * num_pix + t] : 0.;
+            }
+    } else {
+        for (int i = threadIdx.x; i < kPairRange * (kRTile + 1); i += kBlock) (&acc[0][0])[i] = 0.;
+    }
+    __syncthreads();
     for (long j = j0 + wave; j < j1; j += kWavesPerBlock) {
         const int k = tile_pair[j];
-        const double p = P[(size_t)k * kRTile + lane];
-        if (!(p > kProbMin)) continue;
         const int d = pair_frame[k];
         const int *__restrict__ rr = roff + (size_t)d * (nranges + 1);
+        const int q0 = rr[rg], q1 = rr[rg + 1];
+        if (q0 == q1) continue;  // no photon of the frame in this range: P row not needed
+        const double p = P[(size_t)k * kRTile + lane];
+        if (!(p > kProbMin)) continue;
         const long base = foff[d];
-        for (long q = base + rr[rg]; q < base + rr[rg + 1]; ++q)
+        for (long q = base + q0; q < base + q1; ++q)
             unsafeAtomicAdd(&acc[fpix[q] - p0][lane], fcnt[q] * p);
     }
     __syncthreads();
@@ -262,45 +326,8 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
         }
 }
 
-// Tomogram of tiles that most frames take (dense tiles), gathered per pixel like the plain
-// path: block = 64 pixels x one dense tile; for each pixel, the frames with a photon there
-// (tomogram photon list, frames ascending) that have the tile find their pair through
-// pair_of[d][cell] (first pair of the frame's cell, -1 if none) + the tile's index in its cell.
-__global__ __launch_bounds__(kBlock) void pair_tomogram_dense_kernel(
-    const double *__restrict__ P, const int *__restrict__ dense_tiles, const int *__restrict__ tile_cell,
-    const int *__restrict__ tile_sub, const int *__restrict__ pair_of, int num_cells,
-    const long *__restrict__ toff, const int *__restrict__ tframe, const double *__restrict__ tcnt,
-    const double *__restrict__ psum_slot, int t0, int num_pix, double *__restrict__ view) {
-    __shared__ double s[64][kRTile + 1];
-    const int T = dense_tiles[blockIdx.y];
-    const int cell = tile_cell[T], sub = tile_sub[T];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int p0 = blockIdx.x * 64;
-    for (int i = wave; i < 64; i += kWavesPerBlock) {
-        const int t = p0 + i;
-        double acc = 0.;
-        if (t < num_pix) {
-            for (long q = toff[t]; q < toff[t + 1]; ++q) {
-                const int d = tframe[q];
-                const int k0 = pair_of[(size_t)d * num_cells + cell];
-                if (k0 < 0) continue;
-                const double p = P[((size_t)k0 + sub) * kRTile + lane];
-                if (p > kProbMin) acc += tcnt[q] * p;
-            }
-        }
-        s[i][lane] = acc;
-    }
-    __syncthreads();
-    const double ps = psum_slot[(size_t)T * kRTile + lane];
-    for (int i = wave; i < 64; i += kWavesPerBlock) s[i][lane] = ps > 0. ? s[i][lane] / ps : s[i][lane];
-    __syncthreads();
-    const int t = p0 + lane;
-    if (t < num_pix)
-        for (int j = wave; j < kRTile; j += kWavesPerBlock)
-            view[((size_t)(T - t0) * kRTile + j) * num_pix + t] = s[lane][j];
-}
-
-// Scatter of pair values to a dense (num_data x num_rot) matrix (zeroed by the caller).
+// Scatter of partial-frame pair values to a dense (num_data x num_rot) matrix (zeroed by the
+// caller).
 __global__ __launch_bounds__(kBlock) void pair_scatter_kernel(const double *__restrict__ P,
                                                               const int *__restrict__ pair_frame,
                                                               const int *__restrict__ pair_tile,
@@ -314,6 +341,20 @@ __global__ __launch_bounds__(kBlock) void pair_scatter_kernel(const double *__re
     }
 }
 
+// The same for full frames: RF[(T * nF + f) * 64 + j] -> out[fidx[f]][slot_rot[T * 64 + j]].
+__global__ __launch_bounds__(kBlock) void full_scatter_kernel(const double *__restrict__ RF, int nF,
+                                                              int num_tiles, const int *__restrict__ fidx,
+                                                              const int *__restrict__ slot_rot, int num_rot,
+                                                              double *__restrict__ out) {
+    const long n = (long)nF * num_tiles * kRTile;
+    for (long e = blockIdx.x * (long)kBlock + threadIdx.x; e < n; e += (long)gridDim.x * kBlock) {
+        const long row = e / kRTile;
+        const int T = (int)(row / nF), f = (int)(row % nF);
+        const int r = slot_rot[T * kRTile + (int)(e % kRTile)];
+        if (r >= 0) out[(size_t)fidx[f] * num_rot + r] = RF[e];
+    }
+}
+
 // Gather of slot values into the original orientation order: out[r] = in[rot_slot[r]].
 __global__ void slot_gather_kernel(const double *__restrict__ in, const int *__restrict__ rot_slot,
                                    int num_rot, double *__restrict__ out) {
@@ -321,67 +362,105 @@ __global__ void slot_gather_kernel(const double *__restrict__ in, const int *__r
     if (r < num_rot) out[r] = in[rot_slot[r]];
 }
 
+// Per-frame values of the full frames: out[f] = in[fidx[f]].
+template <typename V>
+__global__ void frame_gather_kernel(const V *__restrict__ in, const int *__restrict__ fidx, int nF,
+                                    V *__restrict__ out) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < nF) out[f] = in[fidx[f]];
+}
+
+// out[d][c] = sum of frame d's posterior over the orientations of coarse cell c (f32), from
+// the plain path's rows (plain: R[T][num_data][64], orientation T * 64 + j) or from the
+// subset rows (partial-frame pairs and RF). Sums in LDS (num_coarse doubles).
+__global__ __launch_bounds__(kBlock) void cell_posterior_kernel(
+    double *__restrict__ R, int plain, int num_data, int num_rot, const long *__restrict__ pair_off,
+    const int *__restrict__ pair_tile, FullRows full, const int *__restrict__ slot_rot,
+    const int *__restrict__ qmap, int num_coarse, float *__restrict__ out) {
+    extern __shared__ double s_cell[];
+    const int d = blockIdx.x;
+    for (int c = threadIdx.x; c < num_coarse; c += kBlock) s_cell[c] = 0.;
+    __syncthreads();
+    if (plain) {
+        const size_t tsR = (size_t)num_data * kRTile;
+        for (int r = threadIdx.x; r < num_rot; r += kBlock) {
+            const double p = R[(size_t)(r >> 6) * tsR + (size_t)d * kRTile + (r & 63)];
+            if (p != 0.) unsafeAtomicAdd(&s_cell[qmap[r]], p);
+        }
+    } else {
+        const FrameRows fr = frame_rows(R, pair_off, pair_tile, full, d);
+        for (long e = fr.e0 + threadIdx.x; e < fr.e1; e += kBlock) {
+            int sl;
+            const double p = *fr.at(e, sl);
+            const int r = slot_rot[sl];
+            if (r >= 0 && p != 0.) unsafeAtomicAdd(&s_cell[qmap[r]], p);
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < num_coarse; c += kBlock) out[(size_t)d * num_coarse + c] = (float)s_cell[c];
+}
+
 // ---- launchers -----------------------------------------------------------------------------
 
+void launch_cell_posterior(hipStream_t s, double *R, int plain, int num_data, int num_rot, const long *pair_off,
+                           const int *pair_tile, const FullRows &full, const int *slot_rot, const int *qmap,
+                           int num_coarse, float *out) {
+    if (num_data <= 0) return;
+    cell_posterior_kernel<<<num_data, kBlock, num_coarse * sizeof(double), s>>>(
+        R, plain, num_data, num_rot, pair_off, pair_tile, full, slot_rot, qmap, num_coarse, out);
+}
+
 void launch_pair_likelihood(hipStream_t s, const double *logw, int num_pix, const long *off, const int *pix,
-                            const double *cnt, const int *pair_frame, const int *pair_tile, long num_pairs,
-                            int t0, int t1, double *G) {
+                            const double *cnt, const int *pair_frame, const int *pair_tile, const long *tile_off,
+                            const int *tile_pair, long num_pairs, int t0, int t1, double *G) {
     const unsigned grid = stride_grid((num_pairs + kWavesPerBlock - 1) / kWavesPerBlock);
     if (grid == 0) return;
     pair_likelihood_kernel<8><<<grid, kBlock, 0, s>>>(logw, (size_t)num_pix * kRTile, off, pix, cnt, pair_frame,
-                                                      pair_tile, num_pairs, t0, t1, G);
+                                                      pair_tile, tile_off, tile_pair, t0, t1, G);
 }
 
-void launch_pair_max(hipStream_t s, double *R, const long *pair_off, const int *pair_tile, const int *slot_rot,
-                     const double *logwr_slot, const double *u_slot, const int *rbad_slot, const double *scale,
-                     int use_scale, const double *scalars, const uint8_t *blacklist, const uint8_t *dense,
-                     int num_data, double *max_exp, int *rmax) {
-    pair_max_kernel<<<num_data, kBlock, 0, s>>>(R, pair_off, pair_tile, slot_rot, logwr_slot, u_slot, rbad_slot,
-                                                scale, use_scale, scalars, blacklist, dense, max_exp, rmax);
+void launch_pair_max(hipStream_t s, double *R, const long *pair_off, const int *pair_tile, const FullRows &full,
+                     const int *slot_rot, const double *logwr_slot, const double *u_slot, const int *rbad_slot,
+                     const double *scale, int use_scale, const double *scalars, const uint8_t *blacklist,
+                     const uint8_t *dense, int num_data, double *max_exp, int *rmax) {
+    pair_max_kernel<<<num_data, kBlock, 0, s>>>(R, pair_off, pair_tile, full, slot_rot, logwr_slot, u_slot,
+                                                rbad_slot, scale, use_scale, scalars, blacklist, dense, max_exp,
+                                                rmax);
 }
 
-void launch_pair_norm(hipStream_t s, const double *R, const long *pair_off, const int *pair_tile,
+void launch_pair_norm(hipStream_t s, double *R, const long *pair_off, const int *pair_tile, const FullRows &full,
                       const int *slot_rot, const double *beta, const uint8_t *blacklist, const double *max_exp,
                       const int *rmax, int num_data, double *p_norm) {
-    pair_norm_kernel<<<num_data, kBlock, 0, s>>>(R, pair_off, pair_tile, slot_rot, beta, blacklist, max_exp, rmax,
-                                                 p_norm);
+    pair_norm_kernel<<<num_data, kBlock, 0, s>>>(R, pair_off, pair_tile, full, slot_rot, beta, blacklist, max_exp,
+                                                 rmax, p_norm);
 }
 
 void launch_pair_posterior(hipStream_t s, double *R, const long *pair_off, const int *pair_tile,
-                           const int *slot_rot, const double *logwr_slot, const double *u_slot,
-                           const double *beta, const double *sum_fact, const uint8_t *blacklist,
-                           const double *max_exp, const int *rmax, const double *p_norm, const double *scalars,
-                           int want_psum_d, int num_data, double *likelihood, double *info, double *psum_d) {
-    pair_posterior_kernel<<<num_data, kBlock, 0, s>>>(R, pair_off, pair_tile, slot_rot, logwr_slot, u_slot, beta,
-                                                      sum_fact, blacklist, max_exp, rmax, p_norm, scalars,
+                           const FullRows &full, const int *slot_rot, const double *logwr_slot,
+                           const double *u_slot, const double *beta, const double *sum_fact,
+                           const uint8_t *blacklist, const double *max_exp, const int *rmax, const double *p_norm,
+                           const double *scalars, int want_psum_d, int num_data, double *likelihood, double *info,
+                           double *psum_d) {
+    pair_posterior_kernel<<<num_data, kBlock, 0, s>>>(R, pair_off, pair_tile, full, slot_rot, logwr_slot, u_slot,
+                                                      beta, sum_fact, blacklist, max_exp, rmax, p_norm, scalars,
                                                       want_psum_d, likelihood, info, psum_d);
 }
 
 void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                       const int *pair_frame, int num_tiles, const double *scale, int need_scaling,
-                      double *psum_slot) {
+                      const double *psum_full, double *psum_slot) {
     pair_psum_kernel<<<(num_tiles + kWavesPerBlock - 1) / kWavesPerBlock, kBlock, 0, s>>>(
-        P, tile_off, tile_pair, pair_frame, num_tiles, scale, need_scaling, psum_slot);
+        P, tile_off, tile_pair, pair_frame, num_tiles, scale, need_scaling, psum_full, psum_slot);
 }
 
 void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                           const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
                           const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
-                          const uint8_t *tile_dense, double *view) {
+                          int has_full, double *view) {
     if (ntiles <= 0) return;
     dim3 grid(nranges, ntiles);
     pair_tomogram_kernel<<<grid, kBlock, 0, s>>>(P, tile_off, tile_pair, pair_frame, foff, roff, nranges, fpix,
-                                                 fcnt, psum_slot, t0, num_pix, tile_dense, view);
-}
-
-void launch_pair_tomogram_dense(hipStream_t s, const double *P, const int *dense_tiles, int ndense,
-                                const int *tile_cell, const int *tile_sub, const int *pair_of, int num_cells,
-                                const long *toff, const int *tframe, const double *tcnt, const double *psum_slot,
-                                int t0, int num_pix, double *view) {
-    if (ndense <= 0) return;
-    dim3 grid((num_pix + 63) / 64, ndense);
-    pair_tomogram_dense_kernel<<<grid, kBlock, 0, s>>>(P, dense_tiles, tile_cell, tile_sub, pair_of, num_cells,
-                                                       toff, tframe, tcnt, psum_slot, t0, num_pix, view);
+                                                 fcnt, psum_slot, t0, num_pix, has_full, view);
 }
 
 void launch_pair_scatter(hipStream_t s, const double *P, const int *pair_frame, const int *pair_tile,
@@ -389,11 +468,27 @@ void launch_pair_scatter(hipStream_t s, const double *P, const int *pair_frame, 
     const long n = num_pairs * kRTile;
     if (n == 0) return;
     pair_scatter_kernel<<<stride_grid((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(P, pair_frame, pair_tile, slot_rot,
-                                                                                num_pairs, num_rot, out);
+                                                                                  num_pairs, num_rot, out);
+}
+
+void launch_full_scatter(hipStream_t s, const double *RF, int nF, int num_tiles, const int *fidx,
+                         const int *slot_rot, int num_rot, double *out) {
+    const long n = (long)nF * num_tiles * kRTile;
+    if (n == 0) return;
+    full_scatter_kernel<<<stride_grid((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(RF, nF, num_tiles, fidx, slot_rot,
+                                                                                  num_rot, out);
 }
 
 void launch_slot_gather(hipStream_t s, const double *in, const int *rot_slot, int num_rot, double *out) {
     slot_gather_kernel<<<(num_rot + 255) / 256, 256, 0, s>>>(in, rot_slot, num_rot, out);
+}
+
+void launch_frame_gather(hipStream_t s, const double *in, const int *fidx, int nF, double *out) {
+    if (nF > 0) frame_gather_kernel<double><<<(nF + 255) / 256, 256, 0, s>>>(in, fidx, nF, out);
+}
+
+void launch_frame_gather(hipStream_t s, const uint8_t *in, const int *fidx, int nF, uint8_t *out) {
+    if (nF > 0) frame_gather_kernel<uint8_t><<<(nF + 255) / 256, 256, 0, s>>>(in, fidx, nF, out);
 }
 
 }  // namespace emc

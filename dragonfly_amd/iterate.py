@@ -161,17 +161,32 @@ class Iterate:
     def setup_refine(self, coarse_div, rel_quat):
         """Map every (fine) orientation to its nearest coarse one (voronoi_subset) and attach
         the per-frame coarse sets: rel_quat is a list of int arrays, one per frame."""
-        from . import _lib
         if len(rel_quat) != self.tot_num_data:
             raise ValueError('rel_quat has %d frames, data has %d' % (len(rel_quat), self.tot_num_data))
-        qcoarse = _lib.quat_gen(coarse_div)
-        self.quat_mapping = _lib.voronoi_subset(qcoarse, self.quat.quats)
-        self.num_coarse = qcoarse.shape[0]
         counts = np.array([len(r) for r in rel_quat], dtype='i8')
-        self.rel_off = np.zeros(len(rel_quat) + 1, dtype='i8')
-        np.cumsum(counts, out=self.rel_off[1:])
-        self.rel_quat = np.concatenate([np.asarray(r, 'i4') for r in rel_quat]) if len(rel_quat) \
+        rel_off = np.zeros(len(rel_quat) + 1, dtype='i8')
+        np.cumsum(counts, out=rel_off[1:])
+        flat = np.concatenate([np.asarray(r, 'i4') for r in rel_quat]) if len(rel_quat) \
             else np.zeros(0, 'i4')
+        self.set_refine_lists(coarse_div, rel_off, flat)
+
+    def set_refine_lists(self, coarse_div, rel_off, rel_quat):
+        """As setup_refine with the sets already flattened: frame d's coarse orientations are
+        rel_quat[rel_off[d]:rel_off[d + 1]]. The orientation -> cell map is kept while the
+        orientations and coarse_div do not change."""
+        from . import _lib
+        rel_off = np.ascontiguousarray(rel_off, dtype='i8')
+        rel_quat = np.ascontiguousarray(rel_quat, dtype='i4')
+        if rel_off.shape != (self.tot_num_data + 1,) or rel_off[0] != 0 or rel_off[-1] != rel_quat.size:
+            raise ValueError('rel_off must hold num_data + 1 ascending offsets into rel_quat')
+        key = (int(coarse_div), id(self.quat), self.quat.num_rot)
+        if getattr(self, '_qmap_key', None) != key:
+            qcoarse = _lib.quat_gen(coarse_div)
+            self.quat_mapping = _lib.voronoi_subset(qcoarse, self.quat.quats)
+            self.num_coarse = qcoarse.shape[0]
+            self._qmap_key = key
+        self.rel_off = rel_off
+        self.rel_quat = rel_quat
         if self.rel_quat.size and (self.rel_quat.min() < 0 or self.rel_quat.max() >= self.num_coarse):
             raise ValueError('rel_quat entries must index the %d coarse orientations' % self.num_coarse)
 

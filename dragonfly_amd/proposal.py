@@ -230,6 +230,29 @@ class OrientationProposal:
         return idx[:num].cpu().numpy()
 
 
+def refine_lists(top, mass, num_coarse, threshold=0.99, buf=None):
+    """refine_sets flattened for Iterate.set_refine_lists: (rel_off, rel_quat, fallback, buf).
+    rel_quat is a view of `buf` (reused when large enough, so repeated calls do not allocate
+    and fault in a new list of every cell for every fall-back frame)."""
+    top = np.asarray(top)
+    D, k = top.shape
+    fallback = np.asarray(mass) < threshold
+    rel_off = np.zeros(D + 1, dtype='i8')
+    np.cumsum(np.where(fallback, num_coarse, k), out=rel_off[1:])
+    total = int(rel_off[-1])
+    if buf is None or buf.size < total:
+        buf = np.empty(max(total, 1), dtype='i4')
+    rel_quat = buf[:total]
+    keep = ~fallback
+    if keep.any():
+        idx = rel_off[:-1][keep][:, None] + np.arange(k)[None, :]
+        rel_quat[idx.ravel()] = np.sort(top[keep], axis=1).astype('i4').ravel()
+    every = np.arange(num_coarse, dtype='i4')
+    for s in rel_off[:-1][fallback]:
+        rel_quat[s:s + num_coarse] = every
+    return rel_off, rel_quat, fallback, buf
+
+
 def refine_sets(top, mass, num_coarse, threshold=0.99):
     """Per-frame coarse sets for refinement from proposals: the frame's top-k cells, or every
     cell (fall back to the full search) where the proposal holds less than `threshold` of the

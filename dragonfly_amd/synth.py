@@ -33,11 +33,13 @@ CONFIGS = {
     'cfg3': dict(size=251, atoms=400, atom_radius=30., sigma=1.5, nx=260, ny=257, gap=10,
                  stoprad=5., num_data=20000, mean_count=5000., phi=None, num_div=10,
                  need_scaling=0, beta='auto', iters=5, dense=True, alpha=0.5),
-    # learned orientation proposal: the 100k-frame data set, coarse num_div 4, fine num_div 16
+    # learned orientation proposal: the 100k-frame data set, coarse num_div 4, fine num_div 16;
+    # beta_factor 4 is the reference schedule's beta after 40 iterations (jump sqrt(2) every
+    # 10), the late iterations in which proposals are used
     'cfg5': dict(size=251, atoms=400, atom_radius=30., sigma=1.5, nx=260, ny=257, gap=10,
                  stoprad=5., num_data=100000, mean_count=200., phi=(0.5, 1.5), num_div=16,
                  coarse_div=4, top_k=16, mass_threshold=0.99, need_scaling=1, beta='auto',
-                 iters=5, learned=True),
+                 beta_factor=4., iters=5, learned=True),
 }
 
 

@@ -165,6 +165,12 @@ int emc_slice_merge(emc_engine *e, const double *quat, const double *view);
 /* Copy field `field` (enum emc_field) into dst; count is the element count of dst and
  * must equal the field's size.                                                          */
 int emc_get_field(emc_engine *e, int field, void *dst, size_t count);
+/* Posterior mass of every frame in every coarse cell after an iteration:
+ * out[d][c] = sum over orientations r with quat_mapping[r] == c of P_dr (f32,
+ * num_data x num_coarse, host memory; num_coarse <= 6144). No counterpart in the reference:
+ * it gives an orientation-proposal model its training targets (the cell posteriors an
+ * iteration produced) without moving the dense num_data x num_rot posterior.            */
+int emc_cell_posterior(emc_engine *e, int num_coarse, const int *quat_mapping, float *out);
 /* Zero model2 and inter_weight (max_data allocate_memory, maximize.c:93-94).            */
 int emc_clear_merge(emc_engine *e);
 

@@ -237,8 +237,10 @@ def test_dense_integer_frames(tiny, zero_block):
 @pytest.mark.parametrize('form', ['subset', 'subset-chunked', 'masked'])
 def test_refinement_subsets(tiny, monkeypatch, form):
     """emc_set_refine: per-frame coarse cells (Voronoi of quat_gen(1)), one frame with an
-    empty set, against the oracle's refinement branch; evaluated sparsely over (frame, tile)
-    pairs (in one or in 64-slot chunks), or over every pair with the excluded ones masked."""
+    empty set and every fourth frame with every cell (a full frame: the fall-back to the plain
+    search, with repeated cells in its list), against the oracle's refinement branch;
+    evaluated sparsely over (frame, tile) pairs (in one or in 64-slot chunks), or over every
+    pair with the excluded ones masked."""
     if form == 'subset-chunked':
         monkeypatch.setenv('EMC_MAX_CHUNK_COLS', '64')
     if form == 'masked':
@@ -250,6 +252,8 @@ def test_refinement_subsets(tiny, monkeypatch, form):
     rng = np.random.default_rng(11)
     D = len(tiny['fparams']['beta'])
     sets = [np.sort(rng.choice(len(qc), 10, replace=False)).astype('i4') for _ in range(D)]
+    for d in range(1, D, 4):
+        sets[d] = np.concatenate([rng.permutation(len(qc)), [3, 3]]).astype('i4')
     sets[2] = np.zeros(0, 'i4')
     off = np.zeros(D + 1, 'i8')
     np.cumsum([len(x) for x in sets], out=off[1:])
@@ -270,7 +274,8 @@ def test_refinement_subsets(tiny, monkeypatch, form):
 
 
 def test_refinement_subsets_scaled_blacklisted(tiny_scaled):
-    """Sparse subsets with scale factors (iterations 1 and 2) and blacklisted frames."""
+    """Sparse subsets with scale factors (iterations 1 and 2) and blacklisted frames; every
+    third frame is a full frame (frame 7, blacklisted, among them)."""
     from oracle import core
     from dragonfly_amd import _lib
     p = tiny_scaled
@@ -279,6 +284,8 @@ def test_refinement_subsets_scaled_blacklisted(tiny_scaled):
     rng = np.random.default_rng(12)
     D = len(p['fparams']['beta'])
     sets = [np.sort(rng.choice(len(qc), 20, replace=False)).astype('i4') for _ in range(D)]
+    for d in range(1, D, 3):
+        sets[d] = np.arange(len(qc), dtype='i4')
     off = np.zeros(D + 1, 'i8')
     np.cumsum([len(x) for x in sets], out=off[1:])
     ref = dict(quat_mapping=qmap, rel_off=off, rel_quat=np.concatenate(sets))
@@ -345,26 +352,35 @@ def test_refinement_subsets_at_scale_vs_engine(cells):
 
 
 def test_refinement_subsets_past_32bit_grid():
-    """More (frame, tile) pairs than a 32-bit count of work-items holds: 16000 frames on a
-    22x20 detector, fine num_div 16 (204960 orientations), every one of the 3240 coarse cells
-    (4571 slot tiles) per frame gives 7.3e7 pairs, 4.7e9 lanes of the pair likelihood. The
-    subset path must agree with the plain engine for every frame (a wrapped launch grid drops
-    the last frames' pairs)."""
+    """More (frame, tile) pairs than a 32-bit count of work-items holds: 17000 frames on a
+    22x20 detector, fine num_div 16 (204960 orientations); 16000 frames take all but one of
+    the 3240 coarse cells (4571 slot tiles), 7.3e7 pairs or 4.7e9 lanes of the pair
+    likelihood, and 1000 frames take every cell (full frames). The subset path must agree
+    with the plain engine for every frame (a wrapped launch grid drops the last pairs)."""
     from dragonfly_amd import _lib
-    cfg = P.tiny_config(num_data=16000, mean_count=40., num_div=16)
+    cfg = P.tiny_config(num_data=17000, mean_count=40., num_div=16)
     p = P.build(cfg, device='cuda')
     qc = _lib.quat_gen(4)
     qmap = _lib.voronoi_subset(qc, p['quats'])
     D = len(p['fparams']['beta'])
-    off = np.arange(D + 1, dtype='i8') * len(qc)
-    rq = np.tile(np.arange(len(qc), dtype='i4'), D)
+    every = np.arange(len(qc), dtype='i4')
+    sets = [every if d % 17 == 5 else np.delete(every, (7 * d) % len(qc)) for d in range(D)]
+    off = np.zeros(D + 1, 'i8')
+    np.cumsum([len(x) for x in sets], out=off[1:])
+    rq = np.concatenate(sets)
     keys = ('u', 'max_exp', 'p_norm', 'rmax', 'likelihood', 'info', 'psum_r', 'model2',
             'inter_weight')
     out = []
-    for refine in (False, True):
-        eng = P.engine_for(p)
-        if refine:
-            eng.set_refine(len(qc), qmap, off, rq)
+    import os
+    for sparse in (False, True):
+        # reference: the same subsets evaluated over every pair and masked
+        if not sparse:
+            os.environ['EMC_SUBSET'] = '0'
+        try:
+            eng = P.engine_for(p)
+        finally:
+            os.environ.pop('EMC_SUBSET', None)
+        eng.set_refine(len(qc), qmap, off, rq)
         eng.maximize(1, 0, 1, 0)
         out.append({k: eng.get(k) for k in keys})
         eng.close()
@@ -374,3 +390,32 @@ def test_refinement_subsets_past_32bit_grid():
     bad = {k: v for k, v in errs.items() if not v <= TOL}
     assert not bad, bad
     assert np.array_equal(g['rmax'], o['rmax'])
+
+
+def test_cell_posterior(tiny):
+    """emc_cell_posterior: per-frame posterior mass of every coarse cell, from the plain
+    path's rows and from the subset rows (partial and full frames), against the dense
+    posterior summed per cell (float32 output)."""
+    from dragonfly_amd import _lib
+    qc = _lib.quat_gen(1)
+    qmap = _lib.voronoi_subset(qc, tiny['quats'])
+    D = len(tiny['fparams']['beta'])
+    rng = np.random.default_rng(5)
+    sets = [np.arange(len(qc), dtype='i4') if d % 3 == 0 else
+            np.sort(rng.choice(len(qc), 12, replace=False)).astype('i4') for d in range(D)]
+    off = np.zeros(D + 1, 'i8')
+    np.cumsum([len(x) for x in sets], out=off[1:])
+    for refine in (False, True):
+        eng = P.engine_for(tiny)
+        if refine:
+            eng.set_refine(len(qc), qmap, off, np.concatenate(sets))
+        g = P.run_engine(eng, dict(iteration=1, need_scaling=0))
+        cp = eng.cell_posterior(len(qc), qmap)
+        eng.close()
+        want = np.zeros((D, len(qc)))
+        for r in range(len(qmap)):
+            want[:, qmap[r]] += g['prob'][:, r]
+        assert cp.shape == (D, len(qc))
+        assert np.abs(cp - want).max() <= 1e-6, refine
+        ok = g['rmax'] >= 0
+        assert np.allclose(cp[ok].sum(1), 1., atol=1e-5)
