@@ -339,10 +339,13 @@ def bench_learned(args):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     fallback = propose()
+    torch.cuda.synchronize()
+    propose_s = time.perf_counter() - t0
     recon.run_iteration()
     torch.cuda.synchronize()
     learned_s = time.perf_counter() - t0
-    log('iteration k + 1 with proposals: %.3f s, fallback %.4f' % (learned_s, fallback.mean()))
+    log('iteration k + 1 with proposals: %.3f s (proposals and subset set-up %.3f s), fallback %.4f'
+        % (learned_s, propose_s, fallback.mean()))
     st_learned = recon.last_stats['stage_ms']
     diag_l = diagnostics(recon, itr)
     recon.free()
@@ -374,6 +377,7 @@ def bench_learned(args):
                                   cfg['top_k'], cfg['mass_threshold'], beta_factor),
                    'path': 'learned proposal + sparse refinement', 'frames': D, 'num_rot': R},
         'speedup_vs_fp64': round(plain_s / learned_s, 2), 'fp64_ms_per_step': round(plain_s * 1e3, 1),
+        'propose_ms': round(propose_s * 1e3, 1),
         'fallback_rate': round(float(fallback.mean()), 4),
         'heldout_fallback_rate': round(float(held_fallback.mean()), 4),
         'heldout_captured_mass': round(float(captured.mean()), 4),
@@ -404,7 +408,7 @@ def main():
     ap.add_argument('--precision', type=int, default=None,
                     help='dense-frame configurations: 1 (default) or 2 FP16 terms of log W; 3 (any '
                     'configuration): FP32 log W and posterior gathers, compared with FP64')
-    ap.add_argument('--epochs', type=int, default=40, help='learned-proposal workload: training epochs')
+    ap.add_argument('--epochs', type=int, default=80, help='learned-proposal workload: training epochs')
     ap.add_argument('--no-fp64', action='store_true',
                     help='dense-frame configurations: skip the FP64 comparison run')
     ap.add_argument('--copy-calibration-mib', type=int, default=0,

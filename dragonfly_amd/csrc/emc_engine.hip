@@ -41,27 +41,32 @@ int fail(int code, const char *fmt, ...) {
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
-    size_t n = 0;
+    size_t n = 0;    // elements in use
+    size_t cap = 0;  // elements allocated: a smaller request reuses the allocation
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) {
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n), cap(o.cap) {
         o.p = nullptr;
-        o.n = 0;
+        o.n = o.cap = 0;
     }
     DevBuf &operator=(DevBuf &&o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             n = o.n;
+            cap = o.cap;
             o.p = nullptr;
-            o.n = 0;
+            o.n = o.cap = 0;
         }
         return *this;
     }
     ~DevBuf() { release(); }
     int alloc(size_t count) {
-        if (count == n && p) return 0;
+        if (p && count <= cap && count > 0) {
+            n = count;
+            return 0;
+        }
         release();
         if (count == 0) return 0;
         hipError_t err = hipMalloc((void **)&p, count * sizeof(T));
@@ -70,7 +75,7 @@ struct DevBuf {
             return fail(EMC_E_NOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
                         hipGetErrorString(err));
         }
-        n = count;
+        n = cap = count;
         return 0;
     }
     int upload(const T *src, size_t count, hipStream_t s) {
@@ -82,7 +87,7 @@ struct DevBuf {
     void release() {
         if (p) (void)hipFree(p);
         p = nullptr;
-        n = 0;
+        n = cap = 0;
     }
 };
 
@@ -132,6 +137,7 @@ std::vector<int> morton_order(int num_pix, const double *qvals) {
 
 constexpr int kPixPerGroup = 1024;
 constexpr int kFramesPerPsumPart = 512;
+constexpr int kFramesPerTomoPart = 12500;
 constexpr int kModelParts = 1024;
 
 }  // namespace
@@ -246,7 +252,7 @@ struct emc_engine {
     int slot_num_coarse = 0;
     // full frames (every coarse cell): rows tile-major in RF; photon lists of mask < 2 pixels
     // of the full frames per pixel (frame index f), split into tom_parts frame ranges
-    int num_full = 0;
+    int num_full = 0, full_parts = 1;
     std::vector<int> h_fidx;
     DevBuf<int> fidx, ffull, ftom_frame;
     DevBuf<long> ftom_off2;
@@ -612,7 +618,11 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
 // Per-pixel photon lists (mask < 2) restricted to the full frames, frame index f, with the
 // offsets of tom_parts ranges of f: the plain path's partitioned tomogram lists for RF.
 int build_full_lists(emc_engine *e) {
-    const int P = e->num_pix, nF = e->num_full, nparts = e->tom_parts;
+    // frame ranges of about kFramesPerTomoPart frames (at most tom_parts): fewer full frames
+    // share fewer partial views, whose writes and reads would otherwise dominate
+    const int P = e->num_pix, nF = e->num_full;
+    const int nparts = std::max(1, std::min(e->tom_parts, (nF + kFramesPerTomoPart - 1) / kFramesPerTomoPart));
+    e->full_parts = nparts;
     std::vector<int> &fof = e->h_fof;
     fof.assign(e->num_data, -1);
     for (int f = 0; f < nF; ++f) fof[e->h_fidx[f]] = f;
@@ -716,19 +726,19 @@ int ensure_work(emc_engine *e) {
     const size_t reserve = (size_t)4 << 30;
     size_t budget = freeb > reserve ? freeb - reserve : 0;
     const size_t cap = (size_t)120 << 30;
-    if (e->logw.n) budget += e->logw.n * sizeof(double);
-    if (e->tom_partial.n) budget += e->tom_partial.n * sizeof(double);
-    budget += (e->lw_hi.n + e->lw_lo.n + e->p16.n) * sizeof(_Float16);
+    budget += e->logw.cap * sizeof(double);
+    budget += e->tom_partial.cap * sizeof(double);
+    budget += (e->lw_hi.cap + e->lw_lo.cap + e->p16.cap) * sizeof(_Float16);
     const bool mp = e->precision == 1 || e->precision == 2;
     const bool g32 = e->precision == 3;
-    budget += (e->logw32.n + e->p32.n) * sizeof(float);
+    budget += (e->logw32.cap + e->p32.cap) * sizeof(float);
     if (mp) {
         e->mp_ldp = round_up(P, emc::kGemmBK);
         e->mp_ldd = round_up(N, emc::kGemmBK);
         e->mp_rows_d = round_up(N, emc::kGemmBM);
         e->mp_rows_t = round_up(P, emc::kGemmBM);
         const size_t kbytes = (e->mp_rows_d * e->mp_ldp + e->mp_rows_t * e->mp_ldd) * sizeof(_Float16);
-        const size_t have = (e->klik.n + e->ktom.n) * sizeof(_Float16);
+        const size_t have = (e->klik.cap + e->ktom.cap) * sizeof(_Float16);
         budget = budget + have > kbytes ? budget + have - kbytes : 0;
     }
     if (budget > cap) budget = cap;
@@ -822,13 +832,13 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     int rc;
     hipStream_t s = e->stream;
     const int N = e->num_data, Rn = e->num_rot, P = e->num_pix, S = e->num_slots, T = e->num_tiles;
-    const int nF = e->num_full, nparts = e->tom_parts;
+    const int nF = e->num_full;
     const size_t vol = (size_t)e->size * e->size * e->size;
     if (!e->fsort_valid && (rc = build_fsort(e))) return rc;
     if (nF > 0 && !e->full_lists_valid && (rc = build_full_lists(e))) return rc;
+    const int nparts = e->full_parts;
     // memory: the plain path's rows are not needed here
     e->R.release();
-    e->tom_partial.release();
     if ((rc = e->Rsub.alloc((size_t)std::max<long>(e->num_pairs, 1) * emc::kRTile))) return rc;
     if (nF > 0) {
         if ((rc = e->RF.alloc((size_t)nF * T * emc::kRTile))) return rc;
@@ -842,7 +852,8 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     }
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
-    size_t budget = (freeb > ((size_t)4 << 30) ? freeb - ((size_t)4 << 30) : 0) + e->logw.n * sizeof(double);
+    size_t budget = (freeb > ((size_t)4 << 30) ? freeb - ((size_t)4 << 30) : 0) + e->logw.cap * sizeof(double) +
+                    e->tom_partial.cap * sizeof(double);
     budget = std::min(budget, (size_t)120 << 30);
     // per slot column: log W / view, plus the full frames' partial views
     size_t cmax = budget / ((size_t)P * sizeof(double) * (1 + (nF > 0 ? nparts : 0)));
@@ -852,7 +863,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     const int C = (int)std::min(cmax, (size_t)S);
     if (C != e->slot_C) e->slot_bricks_valid = false;
     e->slot_C = C;
-    if (e->logw.n < (size_t)P * C) {
+    if (e->logw.cap < (size_t)P * C) {
         e->logw.release();
         e->C = 0;
         e->bricks_valid = false;
@@ -994,16 +1005,12 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                            return 0;
                        })))
             return rc;
-        if (nF > 0 && (rc = timed(7, [&] {
-                           emc::launch_tomogram_reduce_rmajor(s, e->tom_partial.p, nparts, bc.r0, bc.Cc, nullptr, S,
-                                                              P, e->logw.p);
-                           return 0;
-                       })))
-            return rc;
-        if ((rc = timed(6, [&] {
+        // partial frames' pairs added to the full frames' partial sums, divided by psum
+        if ((rc = timed(7, [&] {
                  emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p,
                                            e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
-                                           e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P, nF > 0, e->logw.p);
+                                           e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
+                                           nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p);
                  return 0;
              })))
             return rc;
@@ -1297,19 +1304,25 @@ int emc_cell_posterior(emc_engine *e, int num_coarse, const int *quat_mapping, f
     for (int r = 0; r < Rn; ++r)
         if (quat_mapping[r] < 0 || quat_mapping[r] >= num_coarse)
             return fail(EMC_E_ARG, "emc_cell_posterior: quat_mapping[%d] = %d out of range", r, quat_mapping[r]);
+    // frames in batches, so that the device copy stays small next to the engine's rows
+    const int batch = std::max(1, std::min(N, (int)((size_t)64 << 20) / (num_coarse * (int)sizeof(float))));
     DevBuf<int> qm;
     DevBuf<float> dout;
     int rc;
     if ((rc = qm.upload(quat_mapping, Rn, e->stream))) return rc;
-    if ((rc = dout.alloc((size_t)N * num_coarse))) return rc;
+    if ((rc = dout.alloc((size_t)batch * num_coarse))) return rc;
     const bool plain = !e->subset_ran;
     const emc::FullRows full{e->num_full > 0 ? e->RF.p : nullptr, e->num_full > 0 ? e->ffull.p : nullptr,
                              e->num_full, e->num_tiles};
-    emc::launch_cell_posterior(e->stream, plain ? e->R.p : e->Rsub.p, plain, N, Rn, e->pair_off.p, e->pair_tile.p,
-                               full, e->slot_rot.p, qm.p, num_coarse, dout.p);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)N * num_coarse * sizeof(float), hipMemcpyDeviceToHost, e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (int d0 = 0; d0 < N; d0 += batch) {
+        const int cnt = std::min(batch, N - d0);
+        emc::launch_cell_posterior(e->stream, plain ? e->R.p : e->Rsub.p, plain, N, Rn, e->pair_off.p,
+                                   e->pair_tile.p, full, e->slot_rot.p, qm.p, num_coarse, d0, cnt, dout.p);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out + (size_t)d0 * num_coarse, dout.p, (size_t)cnt * num_coarse * sizeof(float),
+                               hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
     return 0;
 }
 

@@ -149,7 +149,7 @@ void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, cons
 void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                           const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
                           const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
-                          int has_full, double *view);
+                          const double *partial, int nparts, double *view);
 void launch_pair_scatter(hipStream_t s, const double *P, const int *pair_frame, const int *pair_tile,
                          const int *slot_rot, long num_pairs, int num_rot, double *out);
 void launch_full_scatter(hipStream_t s, const double *RF, int nF, int num_tiles, const int *fidx,
@@ -157,7 +157,7 @@ void launch_full_scatter(hipStream_t s, const double *RF, int nF, int num_tiles,
 void launch_slot_gather(hipStream_t s, const double *in, const int *rot_slot, int num_rot, double *out);
 void launch_cell_posterior(hipStream_t s, double *R, int plain, int num_data, int num_rot, const long *pair_off,
                            const int *pair_tile, const FullRows &full, const int *slot_rot, const int *qmap,
-                           int num_coarse, float *out);
+                           int num_coarse, int d0, int count, float *out);
 void launch_frame_gather(hipStream_t s, const double *in, const int *fidx, int nF, double *out);
 void launch_frame_gather(hipStream_t s, const uint8_t *in, const int *fidx, int nF, uint8_t *out);
 

@@ -275,43 +275,65 @@ __global__ __launch_bounds__(kBlock) void pair_psum_kernel(const double *__restr
 }
 
 // Tomogram of the chunk's tiles [t0, t0 + ntiles) over pixel ranges of kPairRange pixels:
-// block (tile, range) starts from the full frames' sums already in the view (has_full) or
-// from zero, adds K * P (P > 1e-6) of the tile's partial-frame pairs into LDS rows of the
-// range's pixels (frame photon lists sorted by pixel, range offsets roff[d][range]), divides
-// by psum and writes the orientation-major view[(T - t0) * 64 + j][t].
+// block (tile, range) starts from the full frames' partial sums (tomogram_part_kernel
+// layout partial[x][tile][t][64], added over the nparts frame ranges in order) or from zero,
+// adds K * P (P > 1e-6) of the tile's partial-frame pairs into LDS rows of the range's pixels
+// (frame photon lists sorted by pixel, range offsets roff[d][range]), divides by psum and
+// writes the orientation-major view[(T - t0) * 64 + j][t]. The tile's pairs are examined
+// kBlock at a time, one per thread, and only those with photons in the range are queued for
+// the waves, so a pair costs one row of P per range it has photons in.
 __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     const double *__restrict__ P, const long *__restrict__ tile_off, const int *__restrict__ tile_pair,
     const int *__restrict__ pair_frame, const long *__restrict__ foff, const int *__restrict__ roff,
     int nranges, const int *__restrict__ fpix, const double *__restrict__ fcnt,
-    const double *__restrict__ psum_slot, int t0, int num_pix, int has_full, double *__restrict__ view) {
+    const double *__restrict__ psum_slot, int t0, int ntiles, int num_pix,
+    const double *__restrict__ partial, int nparts, double *__restrict__ view) {
     __shared__ double acc[kPairRange][kRTile + 1];
+    __shared__ int s_k[kBlock];
+    __shared__ long s_q0[kBlock], s_q1[kBlock];
+    __shared__ int s_n;
     const int T = t0 + blockIdx.y, rg = blockIdx.x;
     const long j0 = tile_off[T], j1 = tile_off[T + 1];
-    if (j0 == j1 && !has_full) return;  // no frame takes this tile: psum 0, the merge skips it
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (j0 == j1 && !partial) return;  // no frame takes this tile: psum 0, the merge skips it
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int p0 = rg * kPairRange;
-    if (has_full) {
-        // lanes along pixels: coalesced reads of the view rows
-        for (int jj = wave; jj < kRTile; jj += kWavesPerBlock)
-            for (int i = lane; i < kPairRange; i += 64) {
-                const int t = p0 + i;
-                acc[i][jj] = t < num_pix ? view[((size_t)blockIdx.y * kRTile + jj) * num_pix + t] : 0.;
+    if (partial) {
+        const size_t tsW = (size_t)num_pix * kRTile, tsPart = tsW * ntiles;
+        for (int i = wave; i < kPairRange; i += kWavesPerBlock) {
+            const int t = p0 + i;
+            double a = 0.;
+            if (t < num_pix) {
+                const size_t at = (size_t)blockIdx.y * tsW + (size_t)t * kRTile + lane;
+                for (int x = 0; x < nparts; ++x) a += partial[x * tsPart + at];
             }
+            acc[i][lane] = a;
+        }
     } else {
-        for (int i = threadIdx.x; i < kPairRange * (kRTile + 1); i += kBlock) (&acc[0][0])[i] = 0.;
+        for (int i = tid; i < kPairRange * (kRTile + 1); i += kBlock) (&acc[0][0])[i] = 0.;
     }
-    __syncthreads();
-    for (long j = j0 + wave; j < j1; j += kWavesPerBlock) {
-        const int k = tile_pair[j];
-        const int d = pair_frame[k];
-        const int *__restrict__ rr = roff + (size_t)d * (nranges + 1);
-        const int q0 = rr[rg], q1 = rr[rg + 1];
-        if (q0 == q1) continue;  // no photon of the frame in this range: P row not needed
-        const double p = P[(size_t)k * kRTile + lane];
-        if (!(p > kProbMin)) continue;
-        const long base = foff[d];
-        for (long q = base + q0; q < base + q1; ++q)
-            unsafeAtomicAdd(&acc[fpix[q] - p0][lane], fcnt[q] * p);
+    for (long jb = j0; jb < j1; jb += kBlock) {
+        if (tid == 0) s_n = 0;
+        __syncthreads();
+        const long j = jb + tid;
+        if (j < j1) {
+            const int k = tile_pair[j];
+            const int d = pair_frame[k];
+            const int *__restrict__ rr = roff + (size_t)d * (nranges + 1);
+            const int q0 = rr[rg], q1 = rr[rg + 1];
+            if (q0 < q1) {
+                const int at = atomicAdd(&s_n, 1);
+                s_k[at] = k;
+                s_q0[at] = foff[d] + q0;
+                s_q1[at] = foff[d] + q1;
+            }
+        }
+        __syncthreads();
+        const int n = s_n;
+        for (int i = wave; i < n; i += kWavesPerBlock) {
+            const double p = P[(size_t)s_k[i] * kRTile + lane];
+            if (!(p > kProbMin)) continue;
+            for (long q = s_q0[i]; q < s_q1[i]; ++q) unsafeAtomicAdd(&acc[fpix[q] - p0][lane], fcnt[q] * p);
+        }
     }
     __syncthreads();
     const double ps = psum_slot[(size_t)T * kRTile + lane];
@@ -370,15 +392,15 @@ __global__ void frame_gather_kernel(const V *__restrict__ in, const int *__restr
     if (f < nF) out[f] = in[fidx[f]];
 }
 
-// out[d][c] = sum of frame d's posterior over the orientations of coarse cell c (f32), from
+// out[d - d0][c] = sum of frame d's posterior over the orientations of coarse cell c (f32), from
 // the plain path's rows (plain: R[T][num_data][64], orientation T * 64 + j) or from the
 // subset rows (partial-frame pairs and RF). Sums in LDS (num_coarse doubles).
 __global__ __launch_bounds__(kBlock) void cell_posterior_kernel(
     double *__restrict__ R, int plain, int num_data, int num_rot, const long *__restrict__ pair_off,
     const int *__restrict__ pair_tile, FullRows full, const int *__restrict__ slot_rot,
-    const int *__restrict__ qmap, int num_coarse, float *__restrict__ out) {
+    const int *__restrict__ qmap, int num_coarse, int d0, float *__restrict__ out) {
     extern __shared__ double s_cell[];
-    const int d = blockIdx.x;
+    const int d = d0 + blockIdx.x;
     for (int c = threadIdx.x; c < num_coarse; c += kBlock) s_cell[c] = 0.;
     __syncthreads();
     if (plain) {
@@ -397,17 +419,17 @@ __global__ __launch_bounds__(kBlock) void cell_posterior_kernel(
         }
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < num_coarse; c += kBlock) out[(size_t)d * num_coarse + c] = (float)s_cell[c];
+    for (int c = threadIdx.x; c < num_coarse; c += kBlock) out[(size_t)blockIdx.x * num_coarse + c] = (float)s_cell[c];
 }
 
 // ---- launchers -----------------------------------------------------------------------------
 
 void launch_cell_posterior(hipStream_t s, double *R, int plain, int num_data, int num_rot, const long *pair_off,
                            const int *pair_tile, const FullRows &full, const int *slot_rot, const int *qmap,
-                           int num_coarse, float *out) {
-    if (num_data <= 0) return;
-    cell_posterior_kernel<<<num_data, kBlock, num_coarse * sizeof(double), s>>>(
-        R, plain, num_data, num_rot, pair_off, pair_tile, full, slot_rot, qmap, num_coarse, out);
+                           int num_coarse, int d0, int count, float *out) {
+    if (count <= 0) return;
+    cell_posterior_kernel<<<count, kBlock, num_coarse * sizeof(double), s>>>(
+        R, plain, num_data, num_rot, pair_off, pair_tile, full, slot_rot, qmap, num_coarse, d0, out);
 }
 
 void launch_pair_likelihood(hipStream_t s, const double *logw, int num_pix, const long *off, const int *pix,
@@ -456,11 +478,11 @@ void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, cons
 void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                           const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
                           const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
-                          int has_full, double *view) {
+                          const double *partial, int nparts, double *view) {
     if (ntiles <= 0) return;
     dim3 grid(nranges, ntiles);
     pair_tomogram_kernel<<<grid, kBlock, 0, s>>>(P, tile_off, tile_pair, pair_frame, foff, roff, nranges, fpix,
-                                                 fcnt, psum_slot, t0, num_pix, has_full, view);
+                                                 fcnt, psum_slot, t0, ntiles, num_pix, partial, nparts, view);
 }
 
 void launch_pair_scatter(hipStream_t s, const double *P, const int *pair_frame, const int *pair_tile,
