@@ -263,8 +263,8 @@ def bench_learned(args):
     true_model = np.ascontiguousarray(prob['intens'] * prob['frames']['rescale'])
     beta_factor = float(cfg.get('beta_factor', 1.))
 
-    def recon_for(num_div):
-        recon, itr = build_recon(prob, dict(cfg, num_div=num_div), start=true_model)
+    def recon_for(num_div, precision=0):
+        recon, itr = build_recon(prob, dict(cfg, num_div=num_div), start=true_model, precision=precision)
         itr.params.beta_factor = beta_factor
         itr.params.iteration = 1
         return recon, itr
@@ -316,7 +316,7 @@ def bench_learned(args):
     held_fallback = mass_h < cfg['mass_threshold']
 
     # (4) iteration k + 1 with proposals
-    recon, itr = recon_for(cfg['num_div'])
+    recon, itr = recon_for(cfg['num_div'], precision=args.precision)
     itr.params.iteration = 2
     lists = {}
 
@@ -331,6 +331,7 @@ def bench_learned(args):
                                                                  lists.get('buf'))
         itr.set_refine_lists(cfg['coarse_div'], off, rq)
         recon.engine.set_refine(itr.num_coarse, itr.quat_mapping, itr.rel_off, itr.rel_quat)
+        lists['top'] = top
         return fallback
     start_k1()
     propose()
@@ -357,6 +358,9 @@ def bench_learned(args):
         ok = rmax >= 0
         dots = np.abs((qf[rmax[ok]] * qt[ok]).sum(1)).clip(0., 1.)
         return float(np.degrees(2. * np.median(np.arccos(dots))))
+    # iteration k's posterior mass in the accepted proposals (every frame; held-out included)
+    acc = ~fallback
+    cap_all = np.take_along_axis(post, lists['top'], 1).sum(1)[acc] if acc.any() else np.zeros(1)
     mp, ml = diag_p['model'], diag_l['model']
     model_err = float(np.abs(ml - mp).max() / np.abs(mp).max())
     diag = dict(loglik_rel_err=abs(diag_l['loglik'] - diag_p['loglik']) / abs(diag_p['loglik']),
@@ -370,12 +374,14 @@ def bench_learned(args):
         'metric': METRIC, 'value': round(D * R / learned_s, 1), 'unit': 'frame×orientation pairs/s',
         'n_gpus': 1, 'steps': 1, 'warmup': 1, 'ms_per_step': round(learned_s * 1e3, 1),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'f64 EMC / bf16 proposal network', 'data': 'synthetic',
+        'dtype': ('f32 gathers, f64 sums' if args.precision == 3 else 'f64') + ' EMC / bf16 proposal network',
+        'data': 'synthetic',
         'config': {'workload': '%s: %d frames, fine num_div %d (%d orientations), coarse num_div %d (%d), '
                                'top-%d proposals, fallback below %.2f, beta_start x %g'
                                % (args.config, D, cfg['num_div'], R, cfg['coarse_div'], num_coarse,
                                   cfg['top_k'], cfg['mass_threshold'], beta_factor),
-                   'path': 'learned proposal + sparse refinement', 'frames': D, 'num_rot': R},
+                   'path': 'learned proposal + sparse refinement', 'precision': args.precision,
+                   'frames': D, 'num_rot': R},
         'speedup_vs_fp64': round(plain_s / learned_s, 2), 'fp64_ms_per_step': round(plain_s * 1e3, 1),
         'propose_ms': round(propose_s * 1e3, 1),
         'fallback_rate': round(float(fallback.mean()), 4),
@@ -383,6 +389,8 @@ def bench_learned(args):
         'heldout_captured_mass': round(float(captured.mean()), 4),
         'heldout_captured_mass_accepted': round(float(captured[~held_fallback].mean()), 4)
         if (~held_fallback).any() else None,
+        'accepted_captured_mass_mean': round(float(cap_all.mean()), 5),
+        'accepted_captured_below_threshold': round(float((cap_all < cfg['mass_threshold']).mean()), 4),
         'heldout_best_topk_mass': round(float(best.mean()), 4),
         'heldout_best_topk_ge_threshold': round(float((best >= cfg['mass_threshold']).mean()), 4),
         'model_rel_err': float('%.3g' % model_err), 'train_s': round(train_s, 2), 'epochs': args.epochs,
@@ -407,7 +415,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--precision', type=int, default=None,
                     help='dense-frame configurations: 1 (default) or 2 FP16 terms of log W; 3 (any '
-                    'configuration): FP32 log W and posterior gathers, compared with FP64')
+                    'configuration): FP32 log W and posterior gathers, compared with FP64 (the '
+                    'default of the learned-proposal iteration; 0 there: FP64)')
     ap.add_argument('--epochs', type=int, default=80, help='learned-proposal workload: training epochs')
     ap.add_argument('--no-fp64', action='store_true',
                     help='dense-frame configurations: skip the FP64 comparison run')
@@ -429,13 +438,17 @@ def main():
         torch.cuda.synchronize()
         log('copy calibration: %d MiB read + %d MiB written' % (args.copy_calibration_mib, args.copy_calibration_mib))
         del a, b
+    if synth.CONFIGS.get(args.config, {}).get('learned'):
+        # the learned-proposal iteration gathers in FP32 unless told otherwise (0: FP64)
+        args.precision = 3 if args.precision is None else args.precision
+        if args.precision not in (0, 3):
+            sys.exit('the learned-proposal workload has sparse frames: --precision 0 or 3')
+        bench_learned(args)
+        return
     if args.precision is None:
         args.precision = 1 if synth.CONFIGS.get(args.config, {}).get('dense') else 0
     if synth.CONFIGS.get(args.config, {}).get('dense') or args.precision == 3:
         bench_dense(args)
-        return
-    if synth.CONFIGS.get(args.config, {}).get('learned'):
-        bench_learned(args)
         return
     t0 = time.perf_counter()
     log('generating %s workload ...' % args.config)

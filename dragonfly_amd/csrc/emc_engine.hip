@@ -850,13 +850,17 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     } else {
         e->RF.release();
     }
+    const bool g32 = e->precision == 3;
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     size_t budget = (freeb > ((size_t)4 << 30) ? freeb - ((size_t)4 << 30) : 0) + e->logw.cap * sizeof(double) +
-                    e->tom_partial.cap * sizeof(double);
+                    e->tom_partial.cap * sizeof(double) + (e->logw32.cap + e->p32.cap) * sizeof(float);
     budget = std::min(budget, (size_t)120 << 30);
-    // per slot column: log W / view, plus the full frames' partial views
-    size_t cmax = budget / ((size_t)P * sizeof(double) * (1 + (nF > 0 ? nparts : 0)));
+    // per slot column: log W / view, the full frames' partial views, and with FP32 gathers the
+    // FP32 log W and full-frame posterior columns
+    const size_t per_col = (size_t)P * sizeof(double) * (1 + (nF > 0 ? nparts : 0)) +
+                           (g32 ? ((size_t)P + nF) * sizeof(float) : 0);
+    size_t cmax = budget / per_col;
     if (e->max_chunk_cols > 0) cmax = std::min(cmax, (size_t)e->max_chunk_cols);
     cmax = cmax / emc::kRTile * emc::kRTile;
     if (cmax < (size_t)emc::kRTile) return fail(EMC_E_NOMEM, "no device memory for an orientation chunk");
@@ -870,6 +874,12 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
         if ((rc = e->logw.alloc((size_t)P * C))) return rc;
     }
     if (nF > 0 && (rc = e->tom_partial.alloc((size_t)nparts * P * (size_t)C))) return rc;
+    if (g32) {
+        // 128-wide tiles: room for the chunk rounded up to 128 slots
+        const size_t C128 = round_up((size_t)C, 128);
+        if ((rc = e->logw32.alloc((size_t)P * C128))) return rc;
+        if (nF > 0 && (rc = e->p32.alloc((size_t)nF * C128))) return rc;
+    }
     const int ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
     if ((rc = e->upart.alloc((size_t)ngroups * C))) return rc;
     if ((rc = e->ubad.alloc((size_t)ngroups * C))) return rc;
@@ -911,19 +921,33 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     for (int s0 = 0; s0 < S; s0 += C) {
         const int Cc = std::min(C, S - s0);
         if ((rc = timed(0, [&] {
-                 emc::launch_expand(s, e->rots_slot.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p, P,
-                                    s0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p, e->slot_rot.p);
+                 if (g32)
+                     emc::launch_expand(s, e->rots_slot.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
+                                        P, s0, Cc, kPixPerGroup, e->logw32.p, e->upart.p, e->ubad.p, e->slot_rot.p);
+                 else
+                     emc::launch_expand(s, e->rots_slot.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
+                                        P, s0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p, e->slot_rot.p);
                  emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, s0, S, e->u_slot.p, e->rbad_slot.p);
                  return 0;
              })))
             return rc;
         if ((rc = timed(1, [&] {
-                 emc::launch_pair_likelihood(s, e->logw.p, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
-                                             e->pair_frame.p, e->pair_tile.p, e->tile_off.p, e->tile_pair.p,
-                                             e->num_pairs, s0 / emc::kRTile, (s0 + Cc) / emc::kRTile, e->Rsub.p);
-                 if (nF > 0)
-                     emc::launch_likelihood_mapped(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
-                                                   e->blacklist.p, e->fidx.p, nF, e->RF.p, s0);
+                 const int t0 = s0 / emc::kRTile, t1 = (s0 + Cc) / emc::kRTile;
+                 if (g32) {
+                     emc::launch_pair_likelihood(s, e->logw32.p, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
+                                                 e->pair_frame.p, e->pair_tile.p, e->tile_off.p, e->tile_pair.p,
+                                                 e->num_pairs, t0, t1, e->Rsub.p);
+                     if (nF > 0)
+                         emc::launch_likelihood_mapped(s, e->logw32.p, Cc, P, e->lik_off.p, e->lik_pix.p,
+                                                       e->lik_cnt.p, e->blacklist.p, e->fidx.p, nF, e->RF.p, s0);
+                 } else {
+                     emc::launch_pair_likelihood(s, e->logw.p, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
+                                                 e->pair_frame.p, e->pair_tile.p, e->tile_off.p, e->tile_pair.p,
+                                                 e->num_pairs, t0, t1, e->Rsub.p);
+                     if (nF > 0)
+                         emc::launch_likelihood_mapped(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p,
+                                                       e->lik_cnt.p, e->blacklist.p, e->fidx.p, nF, e->RF.p, s0);
+                 }
                  return 0;
              })))
             return rc;
@@ -999,9 +1023,15 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
         const auto &bc = e->slot_bricks[k];
         const int t0 = bc.r0 / emc::kRTile, t1 = (bc.r0 + bc.Cc) / emc::kRTile;
         if (nF > 0 && (rc = timed(6, [&] {
-                           emc::launch_tomogram_parts(s, e->RF.p, nF, bc.r0, bc.Cc, e->ftom_off2.p, nparts,
-                                                      e->ftom_frame.p, e->ftom_cnt.p, nullptr, S, P,
-                                                      e->tom_partial.p, e->logw.p, 0, 16, 0);
+                           if (g32) {
+                               emc::launch_to_f32(s, e->RF.p, nF, bc.r0, bc.Cc, e->p32.p);
+                               emc::launch_tomogram_parts_f32(s, e->p32.p, nF, bc.Cc, e->ftom_off2.p, nparts,
+                                                              e->ftom_frame.p, e->ftom_cnt.p, P, e->tom_partial.p);
+                           } else {
+                               emc::launch_tomogram_parts(s, e->RF.p, nF, bc.r0, bc.Cc, e->ftom_off2.p, nparts,
+                                                          e->ftom_frame.p, e->ftom_cnt.p, nullptr, S, P,
+                                                          e->tom_partial.p, e->logw.p, 0, 16, 0);
+                           }
                            return 0;
                        })))
             return rc;
@@ -1394,7 +1424,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     const bool mp = e->precision == 1 || e->precision == 2;
     const bool g32 = e->precision == 3;
     if (g32) e->tomo_tile_mode = 2;  // the FP32 posterior feeds the frame-partitioned tomogram
-    if (e->refine && e->subset_mode && e->subset_ready && !mp && !g32) return maximize_subset(e, opts, stats);
+    if (e->refine && e->subset_mode && e->subset_ready && !mp) return maximize_subset(e, opts, stats);
     e->subset_ran = false;
     if (mp) {
         for (uint8_t v : e->h_dense)

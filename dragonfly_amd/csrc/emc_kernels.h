@@ -14,13 +14,17 @@ void launch_expand(hipStream_t s, const double *rots, const double *model, long 
                    const int *slot_rot = nullptr);
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
-                   int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad);
+                   int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad,
+                   const int *slot_rot = nullptr);
 void launch_to_f32(hipStream_t s, const double *P, int num_data, int r0, int C, float *out);
 void launch_likelihood(hipStream_t s, const float *logw, int C, int num_pix, const long *off,
                        const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
                        double *G, int r0, int tiles, int unroll);
 // fmap: row d of G holds frame fmap[d] (photon lists and blacklist indexed by fmap[d]).
 void launch_likelihood_mapped(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
+                              const int *pix, const double *cnt, const uint8_t *blacklist, const int *fmap,
+                              int num_rows, double *G, int r0);
+void launch_likelihood_mapped(hipStream_t s, const float *logw, int C, int num_pix, const long *off,
                               const int *pix, const double *cnt, const uint8_t *blacklist, const int *fmap,
                               int num_rows, double *G, int r0);
 void launch_tomogram_parts_f32(hipStream_t s, const float *P32, int num_data, int C, const long *toff2,
@@ -128,6 +132,10 @@ struct FullRows {
     int num_tiles;
 };
 void launch_pair_likelihood(hipStream_t s, const double *logw, int num_pix, const long *off, const int *pix,
+                            const double *cnt, const int *pair_frame, const int *pair_tile, const long *tile_off,
+                            const int *tile_pair, long num_pairs, int t0, int t1, double *G);
+// FP32 log W in 128-wide tiles (expand_kernel<float, 128>)
+void launch_pair_likelihood(hipStream_t s, const float *logw, int num_pix, const long *off, const int *pix,
                             const double *cnt, const int *pair_frame, const int *pair_tile, const long *tile_off,
                             const int *tile_pair, long num_pairs, int t0, int t1, double *G);
 void launch_pair_max(hipStream_t s, double *R, const long *pair_off, const int *pair_tile, const FullRows &full,

@@ -947,10 +947,11 @@ void launch_expand(hipStream_t s, const double *rots, const double *model, long 
 
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
-                   int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad) {
+                   int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad,
+                   const int *slot_rot) {
     dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
     expand_kernel<float, 2 * kRTile><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0,
-                                                             C, pix_per_group, logw, upart, ubad, nullptr);
+                                                             C, pix_per_group, logw, upart, ubad, slot_rot);
 }
 
 // FP32 copy of one orientation chunk of the tile-major posterior in 128-wide tiles:
@@ -981,15 +982,17 @@ template <int U>
 __global__ __launch_bounds__(kBlock) void likelihood_w128_kernel(
     const float *__restrict__ logw, int C, size_t tsW, const long *__restrict__ off,
     const int *__restrict__ pix, const double *__restrict__ cnt, const uint8_t *__restrict__ blacklist,
-    int num_data, double *__restrict__ G, size_t tsG, int r0) {
+    int num_data, double *__restrict__ G, size_t tsG, int r0, const int *__restrict__ fmap) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int d = blockIdx.x * kWavesPerBlock + wave;
-    if (d >= num_data || blacklist[d]) return;
+    if (d >= num_data) return;
+    const int fd = fmap ? fmap[d] : d;  // frame whose photons row d holds
+    if (blacklist[fd]) return;
     const int tile = blockIdx.y;
     const float2 *__restrict__ col = reinterpret_cast<const float2 *>(logw + (size_t)tile * tsW) + lane;
-    const long k1 = off[d + 1];
-    long k = off[d];
+    const long k1 = off[fd + 1];
+    long k = off[fd];
     double a0 = 0., a1 = 0.;
     for (; k + U <= k1; k += U) {
         float2 v[U];
@@ -1095,17 +1098,31 @@ void launch_likelihood_mapped(hipStream_t s, const double *logw, int C, int num_
     if (num_rows > 0) likelihood_launch(s, logw, C, num_pix, off, pix, cnt, blacklist, num_rows, G, r0, 1, 8, fmap);
 }
 
-void launch_likelihood(hipStream_t s, const float *logw, int C, int num_pix, const long *off,
-                       const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
-                       double *G, int r0, int tiles, int unroll) {
-    (void)tiles;
+static void likelihood_w128_launch(hipStream_t s, const float *logw, int C, int num_pix, const long *off,
+                                   const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
+                                   double *G, int r0, int unroll, const int *fmap) {
     const unsigned gx = cdiv(num_data, kWavesPerBlock);
     const size_t tsW = (size_t)num_pix * 128, tsG = (size_t)num_data * kRTile;
     const dim3 grid(gx, cdiv(C, 128));
     if (unroll >= 16)
-        likelihood_w128_kernel<16><<<grid, kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, blacklist, num_data, G, tsG, r0);
+        likelihood_w128_kernel<16><<<grid, kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, blacklist, num_data, G, tsG, r0,
+                                                           fmap);
     else
-        likelihood_w128_kernel<8><<<grid, kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, blacklist, num_data, G, tsG, r0);
+        likelihood_w128_kernel<8><<<grid, kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, blacklist, num_data, G, tsG, r0,
+                                                          fmap);
+}
+
+void launch_likelihood(hipStream_t s, const float *logw, int C, int num_pix, const long *off,
+                       const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
+                       double *G, int r0, int tiles, int unroll) {
+    (void)tiles;
+    likelihood_w128_launch(s, logw, C, num_pix, off, pix, cnt, blacklist, num_data, G, r0, unroll, nullptr);
+}
+
+void launch_likelihood_mapped(hipStream_t s, const float *logw, int C, int num_pix, const long *off,
+                              const int *pix, const double *cnt, const uint8_t *blacklist, const int *fmap,
+                              int num_rows, double *G, int r0) {
+    if (num_rows > 0) likelihood_w128_launch(s, logw, C, num_pix, off, pix, cnt, blacklist, num_rows, G, r0, 16, fmap);
 }
 
 void launch_frame_max(hipStream_t s, double *R, int num_rot, int num_data,

@@ -66,13 +66,15 @@ __device__ __forceinline__ FrameRows frame_rows(double *R, const long *pair_off,
 
 // G[k][lane] = sum over the frame's photons of count * logW[tile - t0][pix][lane] for the
 // partial-frame pairs of tiles [t0, t1). Pairs are walked tile-major (tile_off / tile_pair), so
-// the waves resident at one time read rows of the same log W tile.
-template <int U>
+// the waves resident at one time read rows of the same log W tile. log W is chunk-local and
+// tile-major with kW-wide tiles of type TW (64 x f64, or 128 x f32: two 64-slot tiles per row).
+template <int U, typename TW, int kW>
 __global__ __launch_bounds__(kBlock) void pair_likelihood_kernel(
-    const double *__restrict__ logw, size_t tsW, const long *__restrict__ off,
+    const TW *__restrict__ logw, size_t tsW, const long *__restrict__ off,
     const int *__restrict__ pix, const double *__restrict__ cnt, const int *__restrict__ pair_frame,
     const int *__restrict__ pair_tile, const long *__restrict__ tile_off, const int *__restrict__ tile_pair,
     int t0, int t1, double *__restrict__ G) {
+    constexpr int kSub = kW / kRTile;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const long j1 = tile_off[t1];
@@ -80,18 +82,19 @@ __global__ __launch_bounds__(kBlock) void pair_likelihood_kernel(
     for (long j = tile_off[t0] + (long)blockIdx.x * kWavesPerBlock + wave; j < j1; j += stride) {
         const int k = tile_pair[j];
         const int d = pair_frame[k];
-        const double *__restrict__ col = logw + (size_t)(pair_tile[k] - t0) * tsW + lane;
+        const int lt = pair_tile[k] - t0;
+        const TW *__restrict__ col = logw + (size_t)(lt / kSub) * tsW + (lt % kSub) * kRTile + lane;
         long q = off[d];
         const long q1 = off[d + 1];
         double acc = 0.;
         for (; q + U <= q1; q += U) {
             double v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = col[(size_t)pix[q + u] * kRTile];
+            for (int u = 0; u < U; ++u) v[u] = (double)col[(size_t)pix[q + u] * kW];
 #pragma unroll
             for (int u = 0; u < U; ++u) acc += cnt[q + u] * v[u];
         }
-        for (; q < q1; ++q) acc += cnt[q] * col[(size_t)pix[q] * kRTile];
+        for (; q < q1; ++q) acc += cnt[q] * (double)col[(size_t)pix[q] * kW];
         G[(size_t)k * kRTile + lane] = acc;
     }
 }
@@ -437,8 +440,19 @@ void launch_pair_likelihood(hipStream_t s, const double *logw, int num_pix, cons
                             const int *tile_pair, long num_pairs, int t0, int t1, double *G) {
     const unsigned grid = stride_grid((num_pairs + kWavesPerBlock - 1) / kWavesPerBlock);
     if (grid == 0) return;
-    pair_likelihood_kernel<8><<<grid, kBlock, 0, s>>>(logw, (size_t)num_pix * kRTile, off, pix, cnt, pair_frame,
-                                                      pair_tile, tile_off, tile_pair, t0, t1, G);
+    pair_likelihood_kernel<8, double, kRTile><<<grid, kBlock, 0, s>>>(logw, (size_t)num_pix * kRTile, off, pix, cnt,
+                                                                      pair_frame, pair_tile, tile_off, tile_pair, t0,
+                                                                      t1, G);
+}
+
+void launch_pair_likelihood(hipStream_t s, const float *logw, int num_pix, const long *off, const int *pix,
+                            const double *cnt, const int *pair_frame, const int *pair_tile, const long *tile_off,
+                            const int *tile_pair, long num_pairs, int t0, int t1, double *G) {
+    const unsigned grid = stride_grid((num_pairs + kWavesPerBlock - 1) / kWavesPerBlock);
+    if (grid == 0) return;
+    pair_likelihood_kernel<8, float, 2 * kRTile><<<grid, kBlock, 0, s>>>(logw, (size_t)num_pix * 2 * kRTile, off, pix,
+                                                                         cnt, pair_frame, pair_tile, tile_off,
+                                                                         tile_pair, t0, t1, G);
 }
 
 void launch_pair_max(hipStream_t s, double *R, const long *pair_off, const int *pair_tile, const FullRows &full,

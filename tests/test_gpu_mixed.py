@@ -146,3 +146,44 @@ def test_fp32_gathers_vs_oracle(scaled):
     assert e['u'] <= 1e-12 and e['inter_weight'] <= 1e-12
     assert e['model2'] <= 1e-4
     assert e['avg_likelihood'] <= 1e-6 and e['mutual_info'] <= 1e-5
+
+
+def test_fp32_gathers_refinement_subsets():
+    """Precision 3 on the sparse subset path: partial frames (10 of 60 cells), full frames
+    (every cell) and an empty set, in 128-slot chunks, against the oracle's refinement branch
+    (tolerances of the FP32 gathers above)."""
+    import os
+    from oracle import core
+    from dragonfly_amd import _lib
+    cfg = P.tiny_config(num_data=120, mean_count=200.)
+    p = P.build(cfg)
+    qc = _lib.quat_gen(1)
+    qmap = _lib.voronoi_subset(qc, p['quats'])
+    rng = np.random.default_rng(21)
+    D = len(p['fparams']['beta'])
+    sets = [np.arange(len(qc), dtype='i4') if d % 3 == 0 else
+            np.sort(rng.choice(len(qc), 10, replace=False)).astype('i4') for d in range(D)]
+    sets[4] = np.zeros(0, 'i4')
+    off = np.zeros(D + 1, 'i8')
+    np.cumsum([len(x) for x in sets], out=off[1:])
+    ref = dict(quat_mapping=qmap, rel_off=off, rel_quat=np.concatenate(sets))
+    flags = dict(iteration=1, need_scaling=0)
+    fp = dict(p['fparams'], scale=np.ones(D))
+    o = core.maximize(p['det'], p['quats'], p['model1'], p['frames'], fp, flags, dense_prob=True, refine=ref)
+    o['scale'] = fp['scale']
+    os.environ['EMC_MAX_CHUNK_COLS'] = '128'
+    try:
+        eng = P.engine_for(p)
+    finally:
+        del os.environ['EMC_MAX_CHUNK_COLS']
+    eng.set_precision(3)
+    eng.set_refine(len(qc), qmap, off, ref['rel_quat'])
+    g = P.run_engine(eng, flags)
+    eng.close()
+    e = _errs(g, o)
+    print('fp32 gathers, refinement subsets', {k: '%.2e' % v for k, v in e.items()}, flush=True)
+    assert g['num_chunks'] > 1
+    assert g['rmax'][4] == -1
+    assert e['u'] <= 1e-12 and e['inter_weight'] <= 1e-12
+    assert e['model2'] <= 1e-4
+    assert e['avg_likelihood'] <= 1e-6 and e['mutual_info'] <= 1e-5
