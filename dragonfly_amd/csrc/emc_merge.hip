@@ -80,6 +80,90 @@ __global__ __launch_bounds__(kBlock) void brick_items_kernel(
     }
 }
 
+// Adds the eight corner contributions of one (orientation, pixel) that fall inside the brick
+// at (X0, Y0, Z0) into the LDS copies.
+template <bool kModel, bool kWeight>
+__device__ __forceinline__ void brick_add(const double *rot, double q0, double q1, double q2, double center,
+                                          long size, int X0, int Y0, int Z0, double w, double *s_m,
+                                          double *s_w) {
+    Cell cell;
+    if (!locate(rot, q0, q1, q2, center, size, cell)) return;
+    const int lx = (int)cell.x - X0, ly = (int)cell.y - Y0, lz = (int)cell.z - Z0;
+    if (lx < -1 || lx >= kBrick || ly < -1 || ly >= kBrick || lz < -1 || lz >= kBrick) return;
+    const double cx = 1. - cell.fx, cy = 1. - cell.fy, cz = 1. - cell.fz;
+    const double fx = cell.fx, fy = cell.fy, fz = cell.fz;
+    const double f[8] = {cx * cy * cz, cx * cy * fz, cx * fy * cz, cx * fy * fz,
+                         fx * cy * cz, fx * cy * fz, fx * fy * cz, fx * fy * fz};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int ix = lx + (q >> 2), iy = ly + ((q >> 1) & 1), iz = lz + (q & 1);
+        if (ix < 0 || ix >= kBrick || iy < 0 || iy >= kBrick || iz < 0 || iz >= kBrick) continue;
+        const int li = (ix * kBrick + iy) * kBrick + iz;
+        if (kWeight) unsafeAtomicAdd(&s_w[li], f[q]);
+        if (kModel) unsafeAtomicAdd(&s_m[li], f[q] * w);
+    }
+}
+
+// Model-only merge (the weights are cached): most views of a sparse posterior are zero, so a
+// wave compacts the nonzero (orientation, pixel) entries of its items into a 128-entry ring in
+// LDS and interpolates and adds them 64 at a time, one entry per lane.
+__global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
+    const int4 *__restrict__ units, const int *__restrict__ list, const double *__restrict__ rots,
+    int r0, int num_rot, const double *__restrict__ view, const double *__restrict__ psum_r, int all_on,
+    const double *__restrict__ qvals, const double *__restrict__ corr, const uint8_t *__restrict__ mask,
+    int num_pix, long size, int npatch, int nbx, double *__restrict__ model2) {
+    constexpr int kVox = kBrick * kBrick * kBrick;
+    constexpr int kRing = 128;
+    __shared__ double s_m[kVox];
+    __shared__ int q_r[kWavesPerBlock][kRing], q_t[kWavesPerBlock][kRing];
+    __shared__ double q_w[kWavesPerBlock][kRing];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int4 u = units[blockIdx.x];
+    const int b = u.x;
+    const int X0 = (b / (nbx * nbx)) * kBrick, Y0 = ((b / nbx) % nbx) * kBrick, Z0 = (b % nbx) * kBrick;
+    for (int i = tid; i < kVox; i += kBlock) s_m[i] = 0.;
+    __syncthreads();
+    const double center = (double)(size / 2);
+    int head = 0, tail = 0;  // wave-uniform ring positions
+    auto drain = [&](int count) {
+        if (lane < count) {
+            const int at = (head + lane) & (kRing - 1);
+            const int r = q_r[wave][at], t = q_t[wave][at];
+            brick_add<true, false>(rots + 9 * (size_t)r, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1],
+                                   qvals[3 * (size_t)t + 2], center, size, X0, Y0, Z0, q_w[wave][at], s_m,
+                                   nullptr);
+        }
+        head += count;
+    };
+    for (int k = u.y + wave; k < u.z; k += kWavesPerBlock) {
+        const int item = list[k];
+        const int c = item / npatch, p = item % npatch;
+        const int r = r0 + c;
+        if (!all_on && !(psum_r[r] > 0.)) continue;
+        const int t = p * 64 + lane;
+        double w = 0.;
+        if (t < num_pix && mask[t] <= 1) w = view[(size_t)c * num_pix + t] / corr[t];
+        const bool nz = w != 0.;  // a zero view adds exactly nothing
+        const unsigned long long bal = __ballot(nz);
+        if (nz) {
+            const int at = (tail + __popcll(bal & ((1ull << lane) - 1))) & (kRing - 1);
+            q_r[wave][at] = r;
+            q_t[wave][at] = t;
+            q_w[wave][at] = w;
+        }
+        tail += __popcll(bal);
+        if (tail - head >= 64) drain(64);
+    }
+    if (tail > head) drain(tail - head);
+    __syncthreads();
+    const long s2 = size * size;
+    for (int i = tid; i < kVox; i += kBlock) {
+        const int x = X0 + i / (kBrick * kBrick), y = Y0 + (i / kBrick) % kBrick, z = Z0 + i % kBrick;
+        if (x >= size || y >= size || z >= size) continue;
+        if (s_m[i] != 0.) unsafeAtomicAdd(&model2[x * s2 + (long)y * size + z], s_m[i]);
+    }
+}
+
 // One block per work unit (brick, item range). kModel adds w * f into model2, kWeight adds f
 // into weight.
 template <bool kModel, bool kWeight>
@@ -166,7 +250,9 @@ void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const 
                                                           all_on, qvals, corr, mask, num_pix, size, \
                                                           npatch, nbx, model2, weight)
     if (do_model && do_weight) EMC_MB(true, true);
-    else if (do_model) EMC_MB(true, false);
+    else if (do_model)
+        merge_brick_model_kernel<<<num_units, kBlock, 0, s>>>(units, list, rots, r0, num_rot, view, psum_r, all_on,
+                                                              qvals, corr, mask, num_pix, size, npatch, nbx, model2);
     else if (do_weight) EMC_MB(false, true);
 #undef EMC_MB
 }
