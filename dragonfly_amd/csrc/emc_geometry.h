@@ -12,8 +12,10 @@ namespace emc {
 
 // Rotated voxel position of a pixel and its trilinear cell, as slice_gen3d / slice_merge3d
 // (dragonfly/src/model.c:19-45 and :184-207): p = R q + center, cell = trunc(p).
+// Cell coordinates are 32-bit: truncation toward zero (v_cvt_i32_f64, which saturates far
+// outside the grid, where the range test fails either way).
 struct Cell {
-    long x, y, z;
+    int x, y, z;
     double fx, fy, fz;
 };
 
@@ -22,10 +24,11 @@ __device__ __forceinline__ bool locate(const double *rot, double q0, double q1, 
     double p0 = rot[0] * q0 + rot[1] * q1 + rot[2] * q2 + center;
     double p1 = rot[3] * q0 + rot[4] * q1 + rot[5] * q2 + center;
     double p2 = rot[6] * q0 + rot[7] * q1 + rot[8] * q2 + center;
-    c.x = (long)p0;
-    c.y = (long)p1;
-    c.z = (long)p2;
-    if (c.x < 0 || c.x > size - 2 || c.y < 0 || c.y > size - 2 || c.z < 0 || c.z > size - 2)
+    c.x = __double2int_rz(p0);
+    c.y = __double2int_rz(p1);
+    c.z = __double2int_rz(p2);
+    const int hi = (int)size - 2;
+    if (c.x < 0 || c.x > hi || c.y < 0 || c.y > hi || c.z < 0 || c.z > hi)
         return false;
     c.fx = p0 - c.x;
     c.fy = p1 - c.y;
@@ -41,15 +44,16 @@ __device__ __forceinline__ double slice_value(const double *__restrict__ model, 
     if (!locate(rot, q0, q1, q2, center, size, c)) return DBL_MIN;
     const long s2 = size * size;
     const long x1 = (c.x + 1) % size, y1 = (c.y + 1) % size, z1 = (c.z + 1) % size;
+    const long cx0 = c.x, cy0 = c.y, cz0 = c.z;
     const double cx = 1. - c.fx, cy = 1. - c.fy, cz = 1. - c.fz;
     const double fx = c.fx, fy = c.fy, fz = c.fz;
-    double v = cx * cy * cz * model[c.x * s2 + c.y * size + c.z] +
-               cx * cy * fz * model[c.x * s2 + c.y * size + z1] +
-               cx * fy * cz * model[c.x * s2 + y1 * size + c.z] +
-               cx * fy * fz * model[c.x * s2 + y1 * size + z1] +
-               fx * cy * cz * model[x1 * s2 + c.y * size + c.z] +
-               fx * cy * fz * model[x1 * s2 + c.y * size + z1] +
-               fx * fy * cz * model[x1 * s2 + y1 * size + c.z] +
+    double v = cx * cy * cz * model[cx0 * s2 + cy0 * size + cz0] +
+               cx * cy * fz * model[cx0 * s2 + cy0 * size + z1] +
+               cx * fy * cz * model[cx0 * s2 + y1 * size + cz0] +
+               cx * fy * fz * model[cx0 * s2 + y1 * size + z1] +
+               fx * cy * cz * model[x1 * s2 + cy0 * size + cz0] +
+               fx * cy * fz * model[x1 * s2 + cy0 * size + z1] +
+               fx * fy * cz * model[x1 * s2 + y1 * size + cz0] +
                fx * fy * fz * model[x1 * s2 + y1 * size + z1];
     return v * corr;
 }

@@ -94,11 +94,22 @@ __device__ __forceinline__ void brick_add(const double *rot, double q0, double q
     const double fx = cell.fx, fy = cell.fy, fz = cell.fz;
     const double f[8] = {cx * cy * cz, cx * cy * fz, cx * fy * cz, cx * fy * fz,
                          fx * cy * cz, fx * cy * fz, fx * fy * cz, fx * fy * fz};
+    const int l0 = (lx * kBrick + ly) * kBrick + lz;
+    if (lx >= 0 && lx < kBrick - 1 && ly >= 0 && ly < kBrick - 1 && lz >= 0 && lz < kBrick - 1) {
+        // all eight corners inside the brick (the common case)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int li = l0 + ((q >> 2) * kBrick + ((q >> 1) & 1)) * kBrick + (q & 1);
+            if (kWeight) unsafeAtomicAdd(&s_w[li], f[q]);
+            if (kModel) unsafeAtomicAdd(&s_m[li], f[q] * w);
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int ix = lx + (q >> 2), iy = ly + ((q >> 1) & 1), iz = lz + (q & 1);
         if (ix < 0 || ix >= kBrick || iy < 0 || iy >= kBrick || iz < 0 || iz >= kBrick) continue;
-        const int li = (ix * kBrick + iy) * kBrick + iz;
+        const int li = l0 + ((q >> 2) * kBrick + ((q >> 1) & 1)) * kBrick + (q & 1);
         if (kWeight) unsafeAtomicAdd(&s_w[li], f[q]);
         if (kModel) unsafeAtomicAdd(&s_m[li], f[q] * w);
     }
@@ -130,8 +141,8 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
             const int at = (head + lane) & (kRing - 1);
             const int r = q_r[wave][at], t = q_t[wave][at];
             brick_add<true, false>(rots + 9 * (size_t)r, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1],
-                                   qvals[3 * (size_t)t + 2], center, size, X0, Y0, Z0, q_w[wave][at], s_m,
-                                   nullptr);
+                                   qvals[3 * (size_t)t + 2], center, size, X0, Y0, Z0, q_w[wave][at] / corr[t],
+                                   s_m, nullptr);
         }
         head += count;
     };
@@ -141,8 +152,8 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
         const int r = r0 + c;
         if (!all_on && !(psum_r[r] > 0.)) continue;
         const int t = p * 64 + lane;
-        double w = 0.;
-        if (t < num_pix && mask[t] <= 1) w = view[(size_t)c * num_pix + t] / corr[t];
+        double w = 0.;  // the view; divided by corr when its entry is drained
+        if (t < num_pix && mask[t] <= 1) w = view[(size_t)c * num_pix + t];
         const bool nz = w != 0.;  // a zero view adds exactly nothing
         const unsigned long long bal = __ballot(nz);
         if (nz) {

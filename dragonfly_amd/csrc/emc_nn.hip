@@ -141,14 +141,34 @@ __global__ __launch_bounds__(kBlock) void nn_softmax_xent_kernel(
     if (tid == 0 && loss) loss[row] = s[0];
 }
 
-// db[n] = sum over rows of dY[row][n] (BF16 in, FP32 out); one thread per column.
-__global__ void nn_colsum_kernel(const bf16 *__restrict__ dY, size_t ldy, int rows, int cols,
-                                 float *__restrict__ db) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+// db[n] = sum over rows of dY[row][n] (BF16 in, FP32 out). Block = 64 columns x 16 waves: wave
+// w sums rows w, w + 16, ... of its lane's column (eight loads in flight), then the 16 partial
+// sums are added in wave order.
+constexpr int kColsumWaves = 16;
+__global__ __launch_bounds__(64 * kColsumWaves) void nn_colsum_kernel(const bf16 *__restrict__ dY, size_t ldy,
+                                                                      int rows, int cols, float *__restrict__ db) {
+    __shared__ float part[kColsumWaves][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + lane;
     float acc = 0.f;
-    for (int r = 0; r < rows; ++r) acc += (float)dY[(size_t)r * ldy + c];
-    db[c] = acc;
+    if (c < cols) {
+        int r = wave;
+        for (; r + 7 * kColsumWaves < rows; r += 8 * kColsumWaves) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = (float)dY[(size_t)(r + u * kColsumWaves) * ldy + c];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += v[u];
+        }
+        for (; r < rows; r += kColsumWaves) acc += (float)dY[(size_t)r * ldy + c];
+    }
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && c < cols) {
+        float t = 0.f;
+        for (int w = 0; w < kColsumWaves; ++w) t += part[w][lane];
+        db[c] = t;
+    }
 }
 
 // Adam step on an FP32 master tensor [rows][cols] (leading dimension ld) with FP32 gradient;
@@ -307,7 +327,7 @@ void launch_nn_softmax_xent(hipStream_t s, const float *Z, size_t ldz, const flo
 }
 
 void launch_nn_colsum(hipStream_t s, const bf16 *dY, size_t ldy, int rows, int cols, float *db) {
-    nn_colsum_kernel<<<(cols + 255) / 256, 256, 0, s>>>(dY, ldy, rows, cols, db);
+    nn_colsum_kernel<<<(cols + 63) / 64, 64 * kColsumWaves, 0, s>>>(dY, ldy, rows, cols, db);
 }
 
 void launch_nn_adam(hipStream_t s, float *w, const float *g, float *m, float *v, int rows, int cols, size_t ld,
