@@ -403,6 +403,102 @@ def bench_learned(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_batched(args):
+    """Uneven batching: 1M frames with lognormal photon counts (median 150, 20-20000) and
+    per-frame orientation subsets of 50-5000 of the num_div 12 orientations, whole Voronoi
+    cells of num_div 3 around a prior 3 degrees from the true orientation, from a model at the
+    truth (photon scale), beta_start. Every frame's subset is evaluated sparsely: its (frame,
+    64-orientation tile) pairs in one likelihood launch per orientation chunk, pairs walked
+    tile-major, per-frame reductions over the pair rows. A step is one iteration; the metric
+    counts the evaluated frame x orientation pairs (sum of the subset sizes).
+
+    Beside it, on the first `--compare-frames` frames (a slice whose dense posterior fits the
+    GPU): the same iteration over every pair with the excluded ones masked (the plain FP64
+    search restricted to the subsets), for the speedup and the relative model difference."""
+    import torch
+    from dragonfly_amd import _lib, synth
+    log('generating %s workload ...' % args.config)
+    t0 = time.perf_counter()
+    prob = synth.make_problem(args.config, device='cuda', num_data=args.num_data, num_div=args.num_div,
+                              batch=2048)
+    cfg = prob['cfg']
+    log('generated in %.1f s' % (time.perf_counter() - t0))
+    true_model = np.ascontiguousarray(prob['intens'] * prob['frames']['rescale'])
+    qcoarse = _lib.quat_gen(cfg['prior_div'])
+    nfine = _lib.num_rot(cfg['num_div'])
+    rel_off, rel_quat = synth.prior_subsets(prob['frames']['true_quats'], qcoarse, nfine, cfg['subset_range'],
+                                            cfg['prior_error_deg'])
+    prec = 0 if args.precision is None else args.precision
+
+    def setup(p, n):
+        recon, itr = build_recon(p, cfg, start=true_model, precision=prec)
+        itr.set_refine_lists(cfg['prior_div'], rel_off[:n + 1], rel_quat[:rel_off[n]])
+        recon.engine.set_refine(itr.num_coarse, itr.quat_mapping, itr.rel_off, itr.rel_quat)
+        cell_size = np.bincount(itr.quat_mapping, minlength=itr.num_coarse)
+        pairs = int(cell_size[itr.rel_quat].sum())
+        return recon, itr, pairs
+
+    recon, itr, pairs = setup(prob, len(prob['frames']['ones']))
+    D = itr.tot_num_data
+    counts = itr.fcounts.astype('f8')
+    log('%d frames, photons per frame median %.0f, mean %.0f, max %d; %.3g evaluated pairs (subset sizes '
+        '%d-%d)' % (D, np.median(counts), counts.mean(), counts.max(), pairs,
+                    int(np.diff(rel_off).min() * nfine / len(qcoarse)), int(np.diff(rel_off).max() * nfine / len(qcoarse))))
+    for it in range(1, args.warmup + 1):
+        itr.params.iteration = it
+        recon.run_iteration()
+        log('warmup iteration %d: %s' % (it, json.dumps({k: round(v, 1) for k, v in recon.last_stats['stage_ms'].items()})))
+    times, stages = [], []
+    for it in range(args.warmup + 1, args.warmup + args.steps + 1):
+        itr.params.iteration = it
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        recon.run_iteration()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        stages.append(recon.last_stats['stage_ms'])
+        log('iteration %d: %.3f s %s' % (it, times[-1], json.dumps({k: round(v, 1) for k, v in stages[-1].items()})))
+    recon.free()
+    step_s = float(np.mean(times))
+
+    # sparse vs masked on a slice of frames
+    n = min(D, args.compare_frames)
+    sub = dict(prob, frames=synth.take_frames(prob['frames'], n))
+    res = {}
+    for mode in ('sparse', 'masked'):
+        r2, i2, p2 = setup(sub, n)
+        r2.engine.set_option('subset', 1 if mode == 'sparse' else 0)
+        i2.params.iteration = 1
+        r2.run_iteration()
+        i2.model.model1 = true_model
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r2.run_iteration()
+        torch.cuda.synchronize()
+        res[mode] = dict(s=time.perf_counter() - t0, model=i2.model.model1.copy(), pairs=p2)
+        log('%d-frame slice, %s: %.3f s' % (n, mode, res[mode]['s']))
+        r2.free()
+    mdiff = float(np.abs(res['sparse']['model'] - res['masked']['model']).max() / np.abs(res['masked']['model']).max())
+    out = {
+        'metric': METRIC, 'value': round(pairs / step_s, 1), 'unit': 'frame×orientation pairs/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(step_s * 1e3, 1),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32 gathers, f64 sums' if prec == 3 else 'f64', 'data': 'synthetic',
+        'config': {'workload': '%s: %d frames, lognormal photons (median %.0f, 20-20000), num_div %d (%d '
+                               'orientations), subsets of %d-%d orientations (num_div %d cells around a prior)'
+                               % (args.config, D, np.median(counts), cfg['num_div'], nfine, cfg['subset_range'][0],
+                                  cfg['subset_range'][1], cfg['prior_div']),
+                   'path': 'sparse refinement subsets', 'frames': D, 'evaluated_pairs': pairs,
+                   'mean_photons': round(float(counts.mean()), 1), 'max_photons': int(counts.max())},
+        'stage_ms_mean': {k: round(float(np.mean([s[k] for s in stages])), 1) for k in stages[0]},
+        'slice_compare': {'frames': n, 'pairs': res['sparse']['pairs'],
+                          'sparse_ms': round(res['sparse']['s'] * 1e3, 1), 'masked_ms': round(res['masked']['s'] * 1e3, 1),
+                          'speedup': round(res['masked']['s'] / res['sparse']['s'], 2), 'model_rel_diff': float('%.3g' % mdiff)},
+        'roofline': None, 'cpu_baseline': None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -418,6 +514,10 @@ def main():
                     'configuration): FP32 log W and posterior gathers, compared with FP64 (the '
                     'default of the learned-proposal iteration; 0 there: FP64)')
     ap.add_argument('--epochs', type=int, default=80, help='learned-proposal workload: training epochs')
+    ap.add_argument('--compare-frames', type=int, default=50000,
+                    help='uneven-batching workload: frames of the sparse vs masked comparison')
+    ap.add_argument('--autotune', action='store_true',
+                    help='plain workloads: pick kernel variants on the live workload before timing')
     ap.add_argument('--no-fp64', action='store_true',
                     help='dense-frame configurations: skip the FP64 comparison run')
     ap.add_argument('--copy-calibration-mib', type=int, default=0,
@@ -438,6 +538,9 @@ def main():
         torch.cuda.synchronize()
         log('copy calibration: %d MiB read + %d MiB written' % (args.copy_calibration_mib, args.copy_calibration_mib))
         del a, b
+    if synth.CONFIGS.get(args.config, {}).get('batched'):
+        bench_batched(args)
+        return
     if synth.CONFIGS.get(args.config, {}).get('learned'):
         # the learned-proposal iteration gathers in FP32 unless told otherwise (0: FP64)
         args.precision = 3 if args.precision is None else args.precision
@@ -459,6 +562,12 @@ def main():
     # host copy of the start model for the CPU baseline before the GPU changes it
     recon, itr = build_recon(prob, cfg)
     start_model = itr.model.model1.copy()
+    tuned = None
+    if args.autotune:
+        from dragonfly_amd import autotune
+        t0 = time.perf_counter()
+        tuned, _ = autotune.tune(recon.engine, iteration=1, need_scaling=cfg['need_scaling'], log=log)
+        log('autotune: %s in %.1f s' % (tuned, time.perf_counter() - t0))
     mask = prob['det']['mask']
     D = itr.tot_num_data
     nbar = photon_stats(prob, itr)['lik'][0]
@@ -510,6 +619,8 @@ def main():
         'roofline': roof,
         'cpu_baseline': cpu,
     }
+    if tuned is not None:
+        out['autotune'] = tuned
     print(json.dumps(out), flush=True)
 
 

@@ -88,6 +88,8 @@ def lib():
             'emc_get_field': (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
             'emc_clear_merge': (C.c_int, [C.c_void_p]),
             'emc_cell_posterior': (C.c_int, [C.c_void_p, C.c_int, _ip, C.POINTER(C.c_float)]),
+            'emc_set_option': (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+            'emc_get_option': (C.c_int, [C.c_void_p, C.c_char_p, _ip]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -276,6 +278,17 @@ class Engine:
         out = np.empty(shape, dtype=dt)
         _check(lib().emc_get_field(self._h, fid, out.ctypes.data_as(C.c_void_p), out.size))
         return out
+
+    def set_option(self, name, value):
+        """Kernel-variant option (emc_set_option): lik_tiles, lik_unroll, tomo_tiles,
+        tomo_tile_mode, tomo_parts, tomo_width32, tomo_unroll, tomo_seq, max_chunk_cols,
+        merge_mode, subset."""
+        _check(lib().emc_set_option(self._h, name.encode(), int(value)))
+
+    def get_option(self, name):
+        v = C.c_int(0)
+        _check(lib().emc_get_option(self._h, name.encode(), C.byref(v)))
+        return v.value
 
     def cell_posterior(self, num_coarse, quat_mapping):
         """(num_data, num_coarse) float32: each frame's posterior mass in each coarse cell

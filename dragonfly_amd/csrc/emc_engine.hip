@@ -8,6 +8,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <cctype>
+#include <climits>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1068,6 +1070,39 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     return 0;
 }
 
+// Kernel-variant options (emc_set_option / EMC_<NAME>): the field, its allowed range, and
+// whether a change invalidates the per-pixel lists (built for a number of frame ranges).
+struct EngineOption {
+    const char *name;
+    int emc_engine::*field;
+    int lo, hi;
+    bool lists;
+};
+const EngineOption kOptions[] = {
+    {"lik_tiles", &emc_engine::lik_tiles, 1, 4, false},
+    {"lik_unroll", &emc_engine::lik_unroll, 4, 16, false},
+    {"tomo_tiles", &emc_engine::tomo_tiles, 1, 4, false},
+    {"tomo_tile_mode", &emc_engine::tomo_tile_mode, 0, 2, false},
+    {"tomo_parts", &emc_engine::tom_parts, 1, 64, true},
+    {"tomo_width32", &emc_engine::tomo_width32, 0, 1, false},
+    {"tomo_unroll", &emc_engine::tomo_unroll, 8, 16, false},
+    {"tomo_seq", &emc_engine::tomo_seq, 0, 1, false},
+    {"max_chunk_cols", &emc_engine::max_chunk_cols, 0, INT_MAX, false},
+    {"merge_mode", &emc_engine::merge_mode, 0, 1, false},
+    {"subset", &emc_engine::subset_mode, 0, 1, false},
+};
+
+int set_option(emc_engine *e, const EngineOption &o, int value) {
+    if (value < o.lo || value > o.hi)
+        return fail(EMC_E_ARG, "option %s = %d outside [%d, %d]", o.name, value, o.lo, o.hi);
+    if (e->*(o.field) != value && o.lists) {
+        e->lists_dirty = true;
+        e->full_lists_valid = false;
+    }
+    e->*(o.field) = value;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1088,20 +1123,32 @@ int emc_create(int device, emc_engine **out) {
         return fail(EMC_E_HIP, "hipStreamCreate failed: %s", hipGetErrorString(err));
     }
     for (auto &ev : e->ev) (void)hipEventCreate(&ev);
-    if (const char *v = getenv("EMC_LIK_TILES")) e->lik_tiles = atoi(v);
-    if (const char *v = getenv("EMC_TOMO_TILES")) e->tomo_tiles = atoi(v);
-    if (const char *v = getenv("EMC_TOMO_TILE_MODE")) e->tomo_tile_mode = atoi(v);
-    if (const char *v = getenv("EMC_TOMO_PARTS")) e->tom_parts = std::max(1, atoi(v));
-    if (const char *v = getenv("EMC_TOMO_WIDTH32")) e->tomo_width32 = atoi(v);
-    if (const char *v = getenv("EMC_MAX_CHUNK_COLS")) e->max_chunk_cols = atoi(v);
-    if (const char *v = getenv("EMC_LIK_UNROLL")) e->lik_unroll = atoi(v);
-    if (const char *v = getenv("EMC_TOMO_UNROLL")) e->tomo_unroll = atoi(v);
-    if (const char *v = getenv("EMC_TOMO_SEQ")) e->tomo_seq = atoi(v);
-    if (const char *v = getenv("EMC_MERGE_MODE")) e->merge_mode = atoi(v);
-    if (const char *v = getenv("EMC_SUBSET")) e->subset_mode = atoi(v);
+    // start values of the kernel-variant options from EMC_<NAME> (upper case)
+    for (const EngineOption &o : kOptions) {
+        std::string env = "EMC_";
+        for (const char *c = o.name; *c; ++c) env += (char)toupper(*c);
+        if (const char *v = getenv(env.c_str())) (void)set_option(e, o, atoi(v));
+    }
 
     *out = e;
     return 0;
+}
+
+int emc_set_option(emc_engine *e, const char *name, int value) {
+    if (!e || !name) return fail(EMC_E_ARG, "emc_set_option: null argument");
+    for (const EngineOption &o : kOptions)
+        if (!strcmp(o.name, name)) return set_option(e, o, value);
+    return fail(EMC_E_ARG, "emc_set_option: unknown option '%s'", name);
+}
+
+int emc_get_option(emc_engine *e, const char *name, int *value) {
+    if (!e || !name || !value) return fail(EMC_E_ARG, "emc_get_option: null argument");
+    for (const EngineOption &o : kOptions)
+        if (!strcmp(o.name, name)) {
+            *value = e->*(o.field);
+            return 0;
+        }
+    return fail(EMC_E_ARG, "emc_get_option: unknown option '%s'", name);
 }
 
 void emc_destroy(emc_engine *e) {

@@ -33,6 +33,14 @@ CONFIGS = {
     'cfg3': dict(size=251, atoms=400, atom_radius=30., sigma=1.5, nx=260, ny=257, gap=10,
                  stoprad=5., num_data=20000, mean_count=5000., phi=None, num_div=10,
                  need_scaling=0, beta='auto', iters=5, dense=True, alpha=0.5),
+    # uneven batching: 1M frames with lognormal photon counts (median 150, clipped to
+    # 20-20000), per-frame orientation subsets of 50-5000 of the num_div 12 orientations around
+    # a prior (whole Voronoi cells of num_div 3, about 63 orientations each)
+    'cfg4': dict(size=251, atoms=400, atom_radius=30., sigma=1.5, nx=260, ny=257, gap=10,
+                 stoprad=5., num_data=1000000, mean_count=150.,
+                 phi=dict(dist='lognormal', sigma=1.0, range=(20., 20000.)), num_div=12,
+                 prior_div=3, subset_range=(50, 5000), prior_error_deg=3., need_scaling=1,
+                 beta='auto', iters=5, batched=True),
     # learned orientation proposal: the 100k-frame data set, coarse num_div 4, fine num_div 16;
     # beta_factor 4 is the reference schedule's beta after 40 iterations (jump sqrt(2) every
     # 10), the late iterations in which proposals are used
@@ -198,7 +206,12 @@ def simulate_frames(intens, det, num_data, mean_count, phi=None, device='cpu', b
         v = _slices(torch, model_flat, size, rots, qv, corr)
         tot += float((v * keep[None, :]).sum())
     rescale = mean_count / (tot / num_ave)
-    if phi is not None:
+    if isinstance(phi, dict) and phi.get('dist') == 'lognormal':
+        # per-frame photon count ~ lognormal with the given median, clipped to [lo, hi]
+        lo, hi = phi['range']
+        counts = np.random.default_rng(seed_phi).lognormal(np.log(mean_count), phi['sigma'], num_data)
+        scales = np.clip(counts, lo, hi) / mean_count
+    elif phi is not None:
         scales = np.random.default_rng(seed_phi).uniform(phi[0], phi[1], num_data)
     else:
         scales = np.ones(num_data)
@@ -271,3 +284,53 @@ def make_problem(name='cfg1', device='cpu', num_data=None, num_div=None, batch=1
     frames = simulate_frames(intens, det, cfg['num_data'], cfg['mean_count'], phi=cfg['phi'],
                              device=device, batch=batch)
     return dict(cfg=cfg, det=det, intens=intens, frames=frames)
+
+
+def prior_subsets(true_quats, qcoarse, num_fine, subset_range, prior_error_deg, seed=5, device='cuda'):
+    """Per-frame refinement sets around a prior orientation: the prior is the true orientation
+    turned by prior_error_deg about a random axis; a frame takes the k Voronoi cells of
+    qcoarse nearest its prior, k chosen so that the set holds a log-uniform target number of
+    fine orientations in subset_range (num_fine / len(qcoarse) per cell on average).
+    Returns (rel_off int64[N + 1], rel_quat int32[...])."""
+    import torch
+    rng = np.random.default_rng(seed)
+    N = len(true_quats)
+    axis = rng.normal(size=(N, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    half = np.radians(prior_error_deg) / 2.
+    turn = np.concatenate([np.full((N, 1), np.cos(half)), axis * np.sin(half)], axis=1)
+    q = np.asarray(true_quats, 'f8')[:, :4]
+    # Hamilton product q * turn
+    w1, x1, y1, z1 = q.T
+    w2, x2, y2, z2 = turn.T
+    prior = np.stack([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                      w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2], axis=1)
+    per_cell = num_fine / len(qcoarse)
+    lo, hi = subset_range
+    target = np.exp(rng.uniform(np.log(lo), np.log(hi), N))
+    k = np.clip(np.rint(target / per_cell), 1, len(qcoarse)).astype('i8')
+    kmax = int(k.max())
+    qc = torch.as_tensor(np.asarray(qcoarse, 'f8')[:, :4], device=device)
+    idx = np.empty((N, kmax), 'i4')
+    for b0 in range(0, N, 65536):
+        pr = torch.as_tensor(prior[b0:b0 + 65536], device=device)
+        top = torch.topk((pr @ qc.T).abs(), kmax, dim=1).indices
+        idx[b0:b0 + len(pr)] = top.to(torch.int32).cpu().numpy()
+    rel_off = np.zeros(N + 1, 'i8')
+    np.cumsum(k, out=rel_off[1:])
+    rel_quat = idx[np.arange(kmax)[None, :] < k[:, None]]
+    return rel_off, rel_quat.astype('i4')
+
+
+def take_frames(frames, n):
+    """The first n frames of a sparse frame set (same layout)."""
+    out = dict(frames)
+    no = int(frames['ones'][:n].sum())
+    nm = int(frames['multi'][:n].sum())
+    for k in ('ones', 'multi', 'ones_accum', 'multi_accum', 'true_quats', 'true_scale'):
+        if k in frames:
+            out[k] = frames[k][:n]
+    out['place_ones'] = frames['place_ones'][:no]
+    out['place_multi'] = frames['place_multi'][:nm]
+    out['count_multi'] = frames['count_multi'][:nm]
+    return out

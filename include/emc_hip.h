@@ -130,6 +130,14 @@ int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const
  * the photon gathers in FP32 (half the bytes), sums in FP64. Every other step stays FP64.
  * Modes 1 and 2 need frames set with emc_set_frames_dense and counts <= 2048.           */
 int emc_set_precision(emc_engine *e, int mode);
+/* kernel-variant options, no counterpart in the reference (results are the same to the
+ * parity tolerance; only speed changes): lik_tiles (1-4 orientation tiles per likelihood
+ * wave), lik_unroll (4/8/16 rows in flight), tomo_tiles, tomo_tile_mode (0 per-pixel gather,
+ * 1 LDS tiles, 2 frame-partitioned, default), tomo_parts (frame ranges, 1-64), tomo_width32,
+ * tomo_unroll (8/16), tomo_seq, max_chunk_cols (0 = by memory), merge_mode (0 box, 1 brick),
+ * subset (1 sparse refinement subsets, 0 masked). Start values come from EMC_<NAME>.      */
+int emc_set_option(emc_engine *e, const char *name, int value);
+int emc_get_option(emc_engine *e, const char *name, int *value);
 /* per-frame state of struct iterate: scale, beta, blacklist, fcounts, sum_fact,
  *        mean_count[0], num_blacklist (iterate.h:17-24,33; iterate.c:3-117)          */
 int emc_set_frame_params(emc_engine *e, const double *scale, const double *beta,

@@ -419,3 +419,25 @@ def test_cell_posterior(tiny):
         assert np.abs(cp - want).max() <= 1e-6, refine
         ok = g['rmax'] >= 0
         assert np.allclose(cp[ok].sum(1), 1., atol=1e-5)
+
+
+def test_options_roundtrip_and_variants_agree(tiny):
+    """emc_set_option: values read back, unknown names and out-of-range values are refused,
+    and variant choices give the same iteration (to the parity tolerance)."""
+    from dragonfly_amd import _lib
+    eng = P.engine_for(tiny)
+    eng.set_option('lik_unroll', 16)
+    assert eng.get_option('lik_unroll') == 16
+    with pytest.raises(_lib.EMCError):
+        eng.set_option('no_such_option', 1)
+    with pytest.raises(_lib.EMCError):
+        eng.set_option('tomo_parts', 0)
+    flags = dict(iteration=1, need_scaling=0)
+    g1 = P.run_engine(eng, flags)
+    eng.set_option('tomo_parts', 3)
+    eng.set_option('lik_tiles', 2)
+    g2 = P.run_engine(eng, flags)
+    eng.close()
+    for k in FIELDS_F:
+        assert P.rel_err(g2[k], g1[k]) <= TOL, k
+    assert np.array_equal(g1['rmax'], g2['rmax'])

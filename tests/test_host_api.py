@@ -204,3 +204,68 @@ def test_parse_rel_quat_formats(tmp_path):
     assert itr.num_coarse == 60 and len(itr.quat_mapping) == itr.quat.num_rot
     assert itr.rel_off[-1] == sum(len(s) for s in sets)
     assert np.array_equal(itr.quat_mapping, df._lib.voronoi_subset(df._lib.quat_gen(1), itr.quat.quats))
+
+
+def test_autotune_greedy_choice():
+    """The tuner keeps, option by option, the value whose stages ran fastest, and leaves the
+    engine set to it; scale factors are never updated while tuning."""
+    from dragonfly_amd import autotune
+
+    class FakeEngine:
+        def __init__(self):
+            self.opt = {'lik_unroll': 8, 'tomo_parts': 8}
+            self.calls = []
+
+        def set_option(self, name, value):
+            self.opt[name] = value
+
+        def maximize(self, iteration, need_scaling, update_scale, known_scale):
+            self.calls.append(update_scale)
+            lik = {4: 30., 8: 20., 16: 10.}[self.opt['lik_unroll']]
+            tomo = {4: 9., 8: 7., 16: 8.}[self.opt['tomo_parts']]
+            return {'stage_ms': {'likelihood': lik, 'tomogram': tomo, 'reduce': 1.}}
+
+    eng = FakeEngine()
+    space = (('lik_unroll', (8, 16, 4), ('likelihood',)), ('tomo_parts', (8, 16, 4), ('tomogram', 'reduce')))
+    chosen, times = autotune.tune(eng, space=space)
+    assert chosen == {'lik_unroll': 16, 'tomo_parts': 8}
+    assert eng.opt == chosen
+    assert times['tomo_parts'] == {8: 8., 16: 9., 4: 10.}
+    assert not any(eng.calls)
+
+
+def test_refine_lists_match_refine_sets():
+    """The flattened proposal lists equal the per-frame sets (top-k sorted, or every cell for
+    a fall-back frame), and a reused buffer gives the same lists."""
+    from dragonfly_amd import proposal
+    rng = np.random.default_rng(4)
+    top = np.stack([rng.choice(50, 5, replace=False) for _ in range(40)]).astype('i4')
+    mass = rng.random(40)
+    sets, fb = proposal.refine_sets(top, mass, 50, 0.5)
+    off, rq, fb2, buf = proposal.refine_lists(top, mass, 50, 0.5)
+    assert np.array_equal(fb, fb2)
+    assert np.array_equal(off, np.concatenate([[0], np.cumsum([len(x) for x in sets])]))
+    assert np.array_equal(rq, np.concatenate(sets))
+    off3, rq3, _, buf3 = proposal.refine_lists(top, mass, 50, 0.5, buf)
+    assert buf3 is buf and np.array_equal(rq3, rq)
+
+
+def test_prior_subsets_sizes_and_nearest_cell():
+    """Refinement sets around a prior: sizes follow the requested range of fine orientations
+    (whole cells), the cells are distinct, and with no prior error the nearest cell to the
+    true orientation is always in the set."""
+    from dragonfly_amd import _lib, synth
+    qc = _lib.quat_gen(2)
+    nfine = _lib.num_rot(6)
+    rng = np.random.default_rng(2)
+    tq = rng.normal(size=(300, 4))
+    tq /= np.linalg.norm(tq, axis=1, keepdims=True)
+    off, rq = synth.prior_subsets(tq, qc, nfine, (50, 2000), 0., device='cpu')
+    per_cell = nfine / len(qc)
+    k = np.diff(off)
+    assert k.min() >= 1 and (k * per_cell).max() <= 2000 + per_cell
+    near = np.argmax(np.abs(tq @ qc[:, :4].T), axis=1)
+    for d in range(300):
+        cells = rq[off[d]:off[d + 1]]
+        assert len(set(cells.tolist())) == len(cells)
+        assert near[d] in cells
