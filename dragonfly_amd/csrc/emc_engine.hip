@@ -270,6 +270,7 @@ struct emc_engine {
     DevBuf<int> slot_rot, rot_slot, pair_frame, pair_tile, tile_pair, rbad_slot;
     DevBuf<long> pair_off, tile_off;
     DevBuf<double> rots_slot, quats_slot, logwr_slot, u_slot, psum_slot, Rsub;
+    DevBuf<float> Rsub32;  // FP32 copy of the pair rows' posterior (FP32 gathers)
     std::vector<BrickChunk> slot_bricks;
     bool slot_bricks_valid = false;
     // frame photon lists of mask < 2 pixels sorted by engine pixel, with per-range offsets
@@ -853,6 +854,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
         e->RF.release();
     }
     const bool g32 = e->precision == 3;
+    if (g32 && (rc = e->Rsub32.alloc((size_t)std::max<long>(e->num_pairs, 1) * emc::kRTile))) return rc;
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     size_t budget = (freeb > ((size_t)4 << 30) ? freeb - ((size_t)4 << 30) : 0) + e->logw.cap * sizeof(double) +
@@ -987,6 +989,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                                    opts->need_scaling, nF > 0 ? e->psum_full.p : nullptr, e->psum_slot.p);
              emc::launch_slot_gather(s, e->psum_slot.p, e->rot_slot.p, Rn, e->psum_r.p);
              emc::launch_slot_gather(s, e->u_slot.p, e->rot_slot.p, Rn, e->u.p);
+             if (g32) emc::launch_pair_to_f32(s, e->Rsub.p, e->num_pairs * emc::kRTile, e->Rsub32.p);
              return 0;
          })))
         return rc;
@@ -1039,10 +1042,16 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
             return rc;
         // partial frames' pairs added to the full frames' partial sums, divided by psum
         if ((rc = timed(7, [&] {
-                 emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p,
-                                           e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
-                                           e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
-                                           nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p);
+                 if (g32)
+                     emc::launch_pair_tomogram(s, e->Rsub32.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p,
+                                               e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
+                                               e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
+                                               nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p);
+                 else
+                     emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p,
+                                               e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
+                                               e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
+                                               nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p);
                  return 0;
              })))
             return rc;

@@ -158,6 +158,11 @@ void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, 
                           const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
                           const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
                           const double *partial, int nparts, double *view);
+void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *tile_pair,
+                          const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
+                          const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
+                          const double *partial, int nparts, double *view);
+void launch_pair_to_f32(hipStream_t s, const double *P, long n, float *out);
 void launch_pair_scatter(hipStream_t s, const double *P, const int *pair_frame, const int *pair_tile,
                          const int *slot_rot, long num_pairs, int num_rot, double *out);
 void launch_full_scatter(hipStream_t s, const double *RF, int nF, int num_tiles, const int *fidx,

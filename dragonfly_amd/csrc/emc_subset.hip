@@ -285,8 +285,9 @@ __global__ __launch_bounds__(kBlock) void pair_psum_kernel(const double *__restr
 // writes the orientation-major view[(T - t0) * 64 + j][t]. The tile's pairs are examined
 // kBlock at a time, one per thread, and only those with photons in the range are queued for
 // the waves, so a pair costs one row of P per range it has photons in.
+template <typename TP>
 __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
-    const double *__restrict__ P, const long *__restrict__ tile_off, const int *__restrict__ tile_pair,
+    const TP *__restrict__ P, const long *__restrict__ tile_off, const int *__restrict__ tile_pair,
     const int *__restrict__ pair_frame, const long *__restrict__ foff, const int *__restrict__ roff,
     int nranges, const int *__restrict__ fpix, const double *__restrict__ fcnt,
     const double *__restrict__ psum_slot, int t0, int ntiles, int num_pix,
@@ -295,7 +296,13 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     __shared__ int s_k[kBlock];
     __shared__ long s_q0[kBlock], s_q1[kBlock];
     __shared__ int s_n;
-    const int T = t0 + blockIdx.y, rg = blockIdx.x;
+    // XCD-aware order: workgroups go round-robin to the kNumXcd XCDs, so XCD x takes tiles
+    // x, x + kNumXcd, ... one at a time, all ranges of a tile together, and its L2 keeps the
+    // tile's P rows while the ranges read them
+    const int xcd = blockIdx.x % kNumXcd, j = blockIdx.x / kNumXcd;
+    const int lt = xcd + kNumXcd * (j / nranges), rg = j % nranges;
+    if (lt >= ntiles) return;
+    const int T = t0 + lt;
     const long j0 = tile_off[T], j1 = tile_off[T + 1];
     if (j0 == j1 && !partial) return;  // no frame takes this tile: psum 0, the merge skips it
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -306,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
             const int t = p0 + i;
             double a = 0.;
             if (t < num_pix) {
-                const size_t at = (size_t)blockIdx.y * tsW + (size_t)t * kRTile + lane;
+                const size_t at = (size_t)lt * tsW + (size_t)t * kRTile + lane;
                 for (int x = 0; x < nparts; ++x) a += partial[x * tsPart + at];
             }
             acc[i][lane] = a;
@@ -333,10 +340,11 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
         __syncthreads();
         const int n = s_n;
         for (int i = wave; i < n; i += kWavesPerBlock) {
-            const double p = P[(size_t)s_k[i] * kRTile + lane];
+            const double p = (double)P[(size_t)s_k[i] * kRTile + lane];
             if (!(p > kProbMin)) continue;
             for (long q = s_q0[i]; q < s_q1[i]; ++q) unsafeAtomicAdd(&acc[fpix[q] - p0][lane], fcnt[q] * p);
         }
+        __syncthreads();  // every wave has read the queue before it is refilled
     }
     __syncthreads();
     const double ps = psum_slot[(size_t)T * kRTile + lane];
@@ -347,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     for (int jj = wave; jj < kRTile; jj += kWavesPerBlock)
         for (int i = lane; i < kPairRange; i += 64) {
             const int t = p0 + i;
-            if (t < num_pix) view[((size_t)blockIdx.y * kRTile + jj) * num_pix + t] = acc[i][jj];
+            if (t < num_pix) view[((size_t)lt * kRTile + jj) * num_pix + t] = acc[i][jj];
         }
 }
 
@@ -489,14 +497,41 @@ void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, cons
         P, tile_off, tile_pair, pair_frame, num_tiles, scale, need_scaling, psum_full, psum_slot);
 }
 
+template <typename TP>
+static void pair_tomogram_launch(hipStream_t s, const TP *P, const long *tile_off, const int *tile_pair,
+                                 const int *pair_frame, const long *foff, const int *roff, int nranges,
+                                 const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
+                                 int num_pix, const double *partial, int nparts, double *view) {
+    if (ntiles <= 0) return;
+    const unsigned grid = (unsigned)((ntiles + kNumXcd - 1) / kNumXcd * kNumXcd) * (unsigned)nranges;
+    pair_tomogram_kernel<TP><<<grid, kBlock, 0, s>>>(P, tile_off, tile_pair, pair_frame, foff, roff, nranges, fpix,
+                                                     fcnt, psum_slot, t0, ntiles, num_pix, partial, nparts, view);
+}
+
 void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                           const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
                           const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
                           const double *partial, int nparts, double *view) {
-    if (ntiles <= 0) return;
-    dim3 grid(nranges, ntiles);
-    pair_tomogram_kernel<<<grid, kBlock, 0, s>>>(P, tile_off, tile_pair, pair_frame, foff, roff, nranges, fpix,
-                                                 fcnt, psum_slot, t0, ntiles, num_pix, partial, nparts, view);
+    pair_tomogram_launch(s, P, tile_off, tile_pair, pair_frame, foff, roff, nranges, fpix, fcnt, psum_slot, t0,
+                         ntiles, num_pix, partial, nparts, view);
+}
+
+void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *tile_pair,
+                          const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
+                          const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
+                          const double *partial, int nparts, double *view) {
+    pair_tomogram_launch(s, P, tile_off, tile_pair, pair_frame, foff, roff, nranges, fpix, fcnt, psum_slot, t0,
+                         ntiles, num_pix, partial, nparts, view);
+}
+
+// FP32 copy of the pair rows (the posterior read by the FP32 pair tomogram).
+__global__ void pair_to_f32_kernel(const double *__restrict__ P, long n, float *__restrict__ out) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = (float)P[i];
+}
+
+void launch_pair_to_f32(hipStream_t s, const double *P, long n, float *out) {
+    if (n > 0) pair_to_f32_kernel<<<stride_grid((n + 255) / 256), 256, 0, s>>>(P, n, out);
 }
 
 void launch_pair_scatter(hipStream_t s, const double *P, const int *pair_frame, const int *pair_tile,
