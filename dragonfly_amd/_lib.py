@@ -280,7 +280,7 @@ class Engine:
         return out
 
     def set_option(self, name, value):
-        """Kernel-variant option (emc_set_option): lik_tiles, lik_unroll, tomo_tiles,
+        """Kernel-variant option (emc_set_option): lik_tiles, lik_unroll, lik_xcd, tomo_tiles,
         tomo_tile_mode, tomo_parts, tomo_width32, tomo_unroll, tomo_seq, max_chunk_cols,
         merge_mode, subset."""
         _check(lib().emc_set_option(self._h, name.encode(), int(value)))

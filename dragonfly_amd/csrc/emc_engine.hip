@@ -180,6 +180,15 @@ struct emc_engine {
     std::vector<long> h_tom_off;
     std::vector<int> h_tom_frame;
     std::vector<double> h_tom_cnt;
+    // host copies of the per-frame likelihood lists, and the same lists split into lik_split
+    // pixel ranges of about equal photon count (lik_split > 1: one range per XCD group)
+    std::vector<long> h_lik_off;
+    std::vector<int> h_lik_pix;
+    std::vector<double> h_lik_cnt;
+    int lik_split = 1, split_built = 0;
+    DevBuf<long> lik8_off;
+    DevBuf<int> lik8_pix;
+    DevBuf<double> lik8_cnt, lik_part;
     int tom_parts = 8, tomo_width32 = 0;
     // upper bound on the orientation chunk width (0: memory decides)
     int max_chunk_cols = 0;
@@ -213,7 +222,7 @@ struct emc_engine {
     // orientation tiles (of 64) per wavefront in the two photon-gather kernels; tomogram
     // form (2: frame-partitioned gather, 1: pixel-tile LDS accumulation, 0: per-pixel gather)
     int lik_tiles = 1, tomo_tiles = 1, tomo_tile_mode = 2;
-    int lik_unroll = 8, tomo_unroll = 16, tomo_seq = 0;
+    int lik_unroll = 8, tomo_unroll = 16, tomo_seq = 0, lik_xcd = 1;
 
     // mixed-precision dense-frame path (0: plain FP64; 1: FP16 log W; 2: FP16 hi + lo log W)
     int precision = 0;
@@ -428,6 +437,10 @@ int build_lists(emc_engine *e) {
     if (!e->tom_frame.p && (rc = e->tom_frame.alloc(1))) return rc;
     if (!e->tom_cnt.p && (rc = e->tom_cnt.alloc(1))) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
+    e->h_lik_off = std::move(loff);
+    e->h_lik_pix = std::move(lpix);
+    e->h_lik_cnt = std::move(lcnt);
+    e->split_built = 0;
     e->h_tom_off = std::move(tcount);
     e->h_tom_frame = std::move(tframe);
     e->h_tom_cnt = std::move(tcnt);
@@ -440,6 +453,54 @@ int build_lists(emc_engine *e) {
 }
 
 size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+
+// Likelihood lists split into X pixel ranges of about equal photon count: frame d's photons
+// of range x are lik8_pix[lik8_off[d * (X + 1) + x] .. lik8_off[d * (X + 1) + x + 1]), in
+// their list order. Engine pixels follow a Morton curve, so a range is a compact detector
+// region and its rows of a log W tile are a compact slab.
+int build_lik_split(emc_engine *e, int X) {
+    const int N = e->num_data, P = e->num_pix;
+    const std::vector<long> &off = e->h_lik_off;
+    const std::vector<int> &pix = e->h_lik_pix;
+    std::vector<long> hist(P + 1, 0);
+    for (int q : pix) ++hist[q + 1];
+    for (int t = 0; t < P; ++t) hist[t + 1] += hist[t];
+    const long total = hist[P];
+    std::vector<uint8_t> range_of(P);
+    int x = 0;
+    for (int t = 0; t < P; ++t) {
+        while (x < X - 1 && hist[t] >= (long)((double)total * (x + 1) / X)) ++x;
+        range_of[t] = (uint8_t)x;
+    }
+    std::vector<long> off8((size_t)N * (X + 1));
+    std::vector<int> pix8(pix.size() > 0 ? pix.size() : 1);
+    std::vector<double> cnt8(pix.size() > 0 ? pix.size() : 1);
+    std::vector<long> fill(X);
+    for (int d = 0; d < N; ++d) {
+        std::fill(fill.begin(), fill.end(), 0);
+        for (long k = off[d]; k < off[d + 1]; ++k) ++fill[range_of[pix[k]]];
+        long at = off[d];
+        for (int r = 0; r < X; ++r) {
+            off8[(size_t)d * (X + 1) + r] = at;
+            const long n = fill[r];
+            fill[r] = at;
+            at += n;
+        }
+        off8[(size_t)d * (X + 1) + X] = at;
+        for (long k = off[d]; k < off[d + 1]; ++k) {
+            const long to = fill[range_of[pix[k]]]++;
+            pix8[to] = pix[k];
+            cnt8[to] = e->h_lik_cnt[k];
+        }
+    }
+    int rc;
+    if ((rc = e->lik8_off.upload(off8.data(), off8.size(), e->stream))) return rc;
+    if ((rc = e->lik8_pix.upload(pix8.data(), pix8.size(), e->stream))) return rc;
+    if ((rc = e->lik8_cnt.upload(cnt8.data(), cnt8.size(), e->stream))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->split_built = X;
+    return 0;
+}
 
 // Per-chunk brick item lists and work units of the brick merge (geometry only). `rots` holds
 // `total` rotations (a multiple of 64) walked in chunks of C; rotations at or past num_valid,
@@ -504,7 +565,8 @@ int merge_chunk(const emc_engine *e, const emc_engine::BrickChunk &bc, const dou
     const int nbx = (int)((e->size + emc::kBrick - 1) / emc::kBrick);
     emc::launch_merge_bricks(e->stream, bc.units.p, bc.num_units, bc.list.p, rots, bc.r0, num_valid, view, psum,
                              all_on, e->qvals.p, e->corr.p, e->mask.p, e->num_pix, e->size,
-                             (e->num_pix + 63) / 64, nbx, e->model2.p, e->weight.p, do_model, do_weight);
+                             (e->num_pix + 63) / 64, nbx, e->model2.p, e->weight.p, do_model, do_weight,
+                             e->precision != 0);  // reduced-precision modes sum bricks in FP32
     return 0;
 }
 
@@ -747,18 +809,27 @@ int ensure_work(emc_engine *e) {
     if (budget > cap) budget = cap;
     // per orientation column: log W / view, plus the partial views of the partitioned
     // tomogram (plain path) or the FP16 log W rows and posterior column (mixed path)
+    const bool split = !mp && !g32 && e->lik_split > 1;
+    budget += e->lik_part.cap * sizeof(double);
+    if (!split) e->lik_part.release();  // before the chunk buffers grow into its room
     const size_t per_col = mp ? P * sizeof(double) + (e->precision == 2 ? 2 : 1) * e->mp_ldp * sizeof(_Float16) +
                                     e->mp_ldd * sizeof(_Float16)
                               : P * sizeof(double) * (1 + (e->tomo_tile_mode == 2 ? e->tom_parts : 0)) +
-                                    (g32 ? P * sizeof(float) + N * sizeof(float) : 0);
+                                    (g32 ? P * sizeof(float) + N * sizeof(float) : 0) +
+                                    (split ? (size_t)e->lik_split * N * sizeof(double) : 0);
     size_t cmax = budget / per_col;
     if (e->max_chunk_cols > 0 && cmax > (size_t)e->max_chunk_cols) cmax = e->max_chunk_cols;
     cmax = cmax / emc::kRTile * emc::kRTile;
     if (cmax < (size_t)emc::kRTile) return fail(EMC_E_NOMEM, "no device memory for an orientation chunk");
     const int C = (int)(cmax < Rp ? cmax : Rp);
     if (C != e->C || !e->logw.p) {
+        // exact sizes: the budget above counted these buffers as free
         e->bricks_valid = false;
         e->tom_partial.release();
+        e->logw.release();
+        e->lik_part.release();
+        e->logw32.release();
+        e->p32.release();
         if ((rc = e->logw.alloc(P * (size_t)C))) return rc;
         e->C = C;
     }
@@ -799,6 +870,10 @@ int ensure_work(emc_engine *e) {
         e->mp_built = false;
         e->mp_rows_c = 0;
         if (e->tomo_tile_mode == 2 && (rc = e->tom_partial.alloc((size_t)e->tom_parts * P * (size_t)C))) return rc;
+    }
+    if (split) {
+        if (e->split_built != e->lik_split && (rc = build_lik_split(e, e->lik_split))) return rc;
+        if ((rc = e->lik_part.alloc((size_t)e->lik_split * N * e->C))) return rc;
     }
     if (g32) {
         // 128-wide tiles: room for the chunk rounded up to 128 orientations
@@ -1090,6 +1165,8 @@ struct EngineOption {
 const EngineOption kOptions[] = {
     {"lik_tiles", &emc_engine::lik_tiles, 1, 4, false},
     {"lik_unroll", &emc_engine::lik_unroll, 4, 16, false},
+    {"lik_xcd", &emc_engine::lik_xcd, 0, 1, false},
+    {"lik_split", &emc_engine::lik_split, 1, 8, false},
     {"tomo_tiles", &emc_engine::tomo_tiles, 1, 4, false},
     {"tomo_tile_mode", &emc_engine::tomo_tile_mode, 0, 2, false},
     {"tomo_parts", &emc_engine::tom_parts, 1, 64, true},
@@ -1546,8 +1623,12 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
             emc::launch_likelihood(s, e->logw32.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
                                    e->blacklist.p, N, e->R.p, r0, e->lik_tiles, e->lik_unroll);
         } else {
-            emc::launch_likelihood(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
-                                   e->blacklist.p, N, e->R.p, r0, e->lik_tiles, e->lik_unroll);
+            if (e->lik_part.p)
+                emc::launch_likelihood_split(s, e->logw.p, Cc, P, e->lik8_off.p, e->lik_split, e->lik8_pix.p,
+                                             e->lik8_cnt.p, e->blacklist.p, N, e->lik_part.p, e->R.p, r0);
+            else
+                emc::launch_likelihood(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
+                                       e->blacklist.p, N, e->R.p, r0, e->lik_tiles, e->lik_unroll, e->lik_xcd);
         }
         HIP_TRY(hipEventRecord(e->ev[2], s));
         HIP_TRY(hipGetLastError());

@@ -20,6 +20,11 @@ void launch_to_f32(hipStream_t s, const double *P, int num_data, int r0, int C, 
 void launch_likelihood(hipStream_t s, const float *logw, int C, int num_pix, const long *off,
                        const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
                        double *G, int r0, int tiles, int unroll);
+// Likelihood over X pixel ranges (lists split per range, off8[d * (X + 1) + x]) into partials
+// part[X][C / 64][num_data][64], then added in range order into the chunk's tiles of G.
+void launch_likelihood_split(hipStream_t s, const double *logw, int C, int num_pix, const long *off8, int X,
+                             const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
+                             double *part, double *G, int r0);
 // fmap: row d of G holds frame fmap[d] (photon lists and blacklist indexed by fmap[d]).
 void launch_likelihood_mapped(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
                               const int *pix, const double *cnt, const uint8_t *blacklist, const int *fmap,
@@ -36,7 +41,7 @@ void launch_rescale(hipStream_t s, const double *quats, const double *u, int num
                     double mean_count, double *scalars);
 void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
                        const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
-                       double *G, int r0, int tiles, int unroll);
+                       double *G, int r0, int tiles, int unroll, int xcd_order = 0);
 void launch_frame_max(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *logwr, const double *u, const double *scale, int use_scale,
                       const double *scalars, const uint8_t *blacklist, const uint8_t *dense,
@@ -77,7 +82,7 @@ void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const 
                          const double *rots, int r0, int num_rot, const double *view,
                          const double *psum_r, int all_on, const double *qvals, const double *corr,
                          const uint8_t *mask, int num_pix, long size, int npatch, int nbx,
-                         double *model2, double *weight, int do_model, int do_weight);
+                         double *model2, double *weight, int do_model, int do_weight, int f32_sums = 0);
 void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot, const double *view,
                   const double *psum_r, int all_on, const double *qvals, const double *corr,
                   const uint8_t *mask, int num_pix, long size, double *model2, double *weight,

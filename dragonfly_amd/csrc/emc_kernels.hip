@@ -148,14 +148,25 @@ __global__ __launch_bounds__(kBlock) void likelihood_kernel(
     const T *__restrict__ logw, int C, size_t tsW, const long *__restrict__ off,
     const int *__restrict__ pix, const double *__restrict__ cnt,
     const uint8_t *__restrict__ blacklist, int num_data, double *__restrict__ G, size_t tsG,
-    int r0, const int *__restrict__ fmap) {
+    int r0, const int *__restrict__ fmap, int xcd_order) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int d = blockIdx.x * kWavesPerBlock + wave;
+    int fg = blockIdx.x, tb = blockIdx.y;
+    if (xcd_order) {
+        // 1-D grid, XCD-aware: workgroups go round-robin to the kNumXcd XCDs, so XCD x takes
+        // orientation tiles x, x + kNumXcd, ... one at a time with all frames, and its L2 holds
+        // the rows of one log W tile instead of sharing one tile with every other XCD
+        const int nfg = (num_data + kWavesPerBlock - 1) / kWavesPerBlock;
+        const int x = blockIdx.x % kNumXcd, j = blockIdx.x / kNumXcd;
+        tb = x + kNumXcd * (j / nfg);
+        fg = j % nfg;
+        if (tb * (kRTile * NT) >= C) return;
+    }
+    const int d = fg * kWavesPerBlock + wave;
     if (d >= num_data) return;
     const int fd = fmap ? fmap[d] : d;  // frame whose photons row d holds
     if (blacklist[fd]) return;
-    const int c0 = blockIdx.y * (kRTile * NT);
+    const int c0 = tb * (kRTile * NT);
     const int nt = min(NT, (C - c0) / kRTile);
     const T *__restrict__ col = logw + (size_t)(c0 >> 6) * tsW + lane;
     const long k1 = off[fd + 1];
@@ -185,6 +196,48 @@ __global__ __launch_bounds__(kBlock) void likelihood_kernel(
 #pragma unroll
     for (int j = 0; j < NT; ++j)
         if (j < nt) G[(size_t)((r0 + c0) / kRTile + j) * tsG + (size_t)d * kRTile + lane] = acc[j];
+}
+
+// Likelihood over pixel ranges: block (range x, frame group, tile) sums the frame's photons of
+// range x (lists split by build_lik_split) against one log W tile into its own partial G_x.
+// Workgroups go round-robin to the XCDs and the range varies fastest, so with X = 8 XCD x only
+// ever reads the rows of range x: about 1/8 of a tile's rows, which its L2 can hold. The
+// partials are added in range order by likelihood_split_reduce_kernel.
+template <int U>
+__global__ __launch_bounds__(kBlock) void likelihood_split_kernel(
+    const double *__restrict__ logw, size_t tsW, const long *__restrict__ off8, int X,
+    const int *__restrict__ pix, const double *__restrict__ cnt, const uint8_t *__restrict__ blacklist,
+    int num_data, int nfg, double *__restrict__ part, size_t tsPart) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int x = blockIdx.x % X, fg = blockIdx.x / X;
+    if (fg >= nfg) return;
+    const int d = fg * kWavesPerBlock + wave;
+    if (d >= num_data || blacklist[d]) return;
+    const int tile = blockIdx.y;
+    const double *__restrict__ col = logw + (size_t)tile * tsW + lane;
+    long k = off8[(size_t)d * (X + 1) + x];
+    const long k1 = off8[(size_t)d * (X + 1) + x + 1];
+    double acc = 0.;
+    for (; k + U <= k1; k += U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = col[(size_t)pix[k + u] * kRTile];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc += cnt[k + u] * v[u];
+    }
+    for (; k < k1; ++k) acc += cnt[k] * col[(size_t)pix[k] * kRTile];
+    part[(size_t)x * tsPart + (size_t)tile * num_data * kRTile + (size_t)d * kRTile + lane] = acc;
+}
+
+// G[i] = sum over ranges x (in order) of part[x][i], for the chunk's tiles.
+__global__ void likelihood_split_reduce_kernel(const double *__restrict__ part, int X, size_t n,
+                                               double *__restrict__ G) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double acc = 0.;
+        for (int x = 0; x < X; ++x) acc += part[(size_t)x * n + i];
+        G[i] = acc;
+    }
 }
 
 // ---- block helpers -----------------------------------------------------------------------
@@ -935,6 +988,7 @@ namespace emc {
 
 static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 static inline unsigned grid_cap(size_t blocks) { return (unsigned)(blocks < 65536 ? blocks : 65536); }
+static inline int round_up_int(int a, int b) { return (a + b - 1) / b * b; }
 
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
@@ -1072,12 +1126,19 @@ void launch_rescale(hipStream_t s, const double *quats, const double *u, int num
 template <typename T>
 static void likelihood_launch(hipStream_t s, const T *logw, int C, int num_pix, const long *off,
                               const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
-                              double *G, int r0, int tiles, int unroll, const int *fmap = nullptr) {
+                              double *G, int r0, int tiles, int unroll, const int *fmap = nullptr,
+                              int xcd_order = 0) {
     const unsigned gx = cdiv(num_data, kWavesPerBlock);
     const size_t tsW = (size_t)num_pix * kRTile, tsG = (size_t)num_data * kRTile;
 #define EMC_LIK(NT, U)                                                                           \
-    likelihood_kernel<NT, U, T><<<dim3(gx, cdiv(C, kRTile * NT)), kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, \
-                                                                                blacklist, num_data, G, tsG, r0, fmap)
+    do {                                                                                         \
+        const size_t gy = cdiv(C, kRTile * NT), gy8 = (gy + kNumXcd - 1) / kNumXcd * kNumXcd;    \
+        /* the 1-D order needs its work-item count to fit the 32-bit grid size */                \
+        const int xo = xcd_order && (size_t)gx * gy8 * kBlock < ((size_t)1 << 32);               \
+        const dim3 grid = xo ? dim3((unsigned)(gx * gy8)) : dim3(gx, (unsigned)gy);              \
+        likelihood_kernel<NT, U, T><<<grid, kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, blacklist, num_data, G, \
+                                                            tsG, r0, fmap, xo);                 \
+    } while (0)
     if (tiles >= 4) EMC_LIK(4, 2);
     else if (tiles == 2) EMC_LIK(2, 4);
     else if (unroll >= 16) EMC_LIK(1, 16);
@@ -1088,8 +1149,25 @@ static void likelihood_launch(hipStream_t s, const T *logw, int C, int num_pix, 
 
 void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
                        const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
-                       double *G, int r0, int tiles, int unroll) {
-    likelihood_launch(s, logw, C, num_pix, off, pix, cnt, blacklist, num_data, G, r0, tiles, unroll);
+                       double *G, int r0, int tiles, int unroll, int xcd_order) {
+    likelihood_launch(s, logw, C, num_pix, off, pix, cnt, blacklist, num_data, G, r0, tiles, unroll, nullptr,
+                      xcd_order);
+}
+
+void launch_likelihood_split(hipStream_t s, const double *logw, int C, int num_pix, const long *off8, int X,
+                             const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
+                             double *part, double *G, int r0) {
+    if (num_data <= 0 || C <= 0) return;
+    const int nfg = (int)cdiv(num_data, kWavesPerBlock);
+    // the range index varies fastest and the row of blocks is a multiple of the XCD count, so
+    // block (x, fg) lands on an XCD of group x (x = XCD index when X = kNumXcd)
+    const unsigned gx = (unsigned)(X * round_up_int(nfg, kNumXcd));
+    const int ntiles = C / kRTile;
+    const size_t tsW = (size_t)num_pix * kRTile, n = (size_t)ntiles * num_data * kRTile;
+    likelihood_split_kernel<8><<<dim3(gx, ntiles), kBlock, 0, s>>>(logw, tsW, off8, X, pix, cnt, blacklist, num_data,
+                                                                  nfg, part, n);
+    likelihood_split_reduce_kernel<<<grid_cap(cdiv(n, 256)), 256, 0, s>>>(
+        part, X, n, G + (size_t)(r0 / kRTile) * num_data * kRTile);
 }
 
 void launch_likelihood_mapped(hipStream_t s, const double *logw, int C, int num_pix, const long *off,

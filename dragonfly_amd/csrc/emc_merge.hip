@@ -82,10 +82,9 @@ __global__ __launch_bounds__(kBlock) void brick_items_kernel(
 
 // Adds the eight corner contributions of one (orientation, pixel) that fall inside the brick
 // at (X0, Y0, Z0) into the LDS copies.
-template <bool kModel, bool kWeight>
+template <bool kModel, bool kWeight, typename A = double>
 __device__ __forceinline__ void brick_add(const double *rot, double q0, double q1, double q2, double center,
-                                          long size, int X0, int Y0, int Z0, double w, double *s_m,
-                                          double *s_w) {
+                                          long size, int X0, int Y0, int Z0, double w, A *s_m, A *s_w) {
     Cell cell;
     if (!locate(rot, q0, q1, q2, center, size, cell)) return;
     const int lx = (int)cell.x - X0, ly = (int)cell.y - Y0, lz = (int)cell.z - Z0;
@@ -100,8 +99,8 @@ __device__ __forceinline__ void brick_add(const double *rot, double q0, double q
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int li = l0 + ((q >> 2) * kBrick + ((q >> 1) & 1)) * kBrick + (q & 1);
-            if (kWeight) unsafeAtomicAdd(&s_w[li], f[q]);
-            if (kModel) unsafeAtomicAdd(&s_m[li], f[q] * w);
+            if (kWeight) unsafeAtomicAdd(&s_w[li], (A)f[q]);
+            if (kModel) unsafeAtomicAdd(&s_m[li], (A)(f[q] * w));
         }
         return;
     }
@@ -110,14 +109,17 @@ __device__ __forceinline__ void brick_add(const double *rot, double q0, double q
         const int ix = lx + (q >> 2), iy = ly + ((q >> 1) & 1), iz = lz + (q & 1);
         if (ix < 0 || ix >= kBrick || iy < 0 || iy >= kBrick || iz < 0 || iz >= kBrick) continue;
         const int li = l0 + ((q >> 2) * kBrick + ((q >> 1) & 1)) * kBrick + (q & 1);
-        if (kWeight) unsafeAtomicAdd(&s_w[li], f[q]);
-        if (kModel) unsafeAtomicAdd(&s_m[li], f[q] * w);
+        if (kWeight) unsafeAtomicAdd(&s_w[li], (A)f[q]);
+        if (kModel) unsafeAtomicAdd(&s_m[li], (A)(f[q] * w));
     }
 }
 
 // Model-only merge (the weights are cached): most views of a sparse posterior are zero, so a
 // wave compacts the nonzero (orientation, pixel) entries of its items into a 128-entry ring in
-// LDS and interpolates and adds them 64 at a time, one entry per lane.
+// LDS and interpolates and adds them 64 at a time, one entry per lane. A is the type of the
+// brick's LDS sums: double for the FP64 path, float for the reduced-precision modes (the
+// brick is added to the FP64 model once per work unit).
+template <typename A>
 __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
     const int4 *__restrict__ units, const int *__restrict__ list, const double *__restrict__ rots,
     int r0, int num_rot, const double *__restrict__ view, const double *__restrict__ psum_r, int all_on,
@@ -125,14 +127,14 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
     int num_pix, long size, int npatch, int nbx, double *__restrict__ model2) {
     constexpr int kVox = kBrick * kBrick * kBrick;
     constexpr int kRing = 128;
-    __shared__ double s_m[kVox];
+    __shared__ A s_m[kVox];
     __shared__ int q_r[kWavesPerBlock][kRing], q_t[kWavesPerBlock][kRing];
     __shared__ double q_w[kWavesPerBlock][kRing];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int4 u = units[blockIdx.x];
     const int b = u.x;
     const int X0 = (b / (nbx * nbx)) * kBrick, Y0 = ((b / nbx) % nbx) * kBrick, Z0 = (b % nbx) * kBrick;
-    for (int i = tid; i < kVox; i += kBlock) s_m[i] = 0.;
+    for (int i = tid; i < kVox; i += kBlock) s_m[i] = (A)0;
     __syncthreads();
     const double center = (double)(size / 2);
     int head = 0, tail = 0;  // wave-uniform ring positions
@@ -140,9 +142,9 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
         if (lane < count) {
             const int at = (head + lane) & (kRing - 1);
             const int r = q_r[wave][at], t = q_t[wave][at];
-            brick_add<true, false>(rots + 9 * (size_t)r, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1],
-                                   qvals[3 * (size_t)t + 2], center, size, X0, Y0, Z0, q_w[wave][at] / corr[t],
-                                   s_m, nullptr);
+            brick_add<true, false, A>(rots + 9 * (size_t)r, qvals[3 * (size_t)t], qvals[3 * (size_t)t + 1],
+                                      qvals[3 * (size_t)t + 2], center, size, X0, Y0, Z0, q_w[wave][at] / corr[t],
+                                      s_m, nullptr);
         }
         head += count;
     };
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
     for (int i = tid; i < kVox; i += kBlock) {
         const int x = X0 + i / (kBrick * kBrick), y = Y0 + (i / kBrick) % kBrick, z = Z0 + i % kBrick;
         if (x >= size || y >= size || z >= size) continue;
-        if (s_m[i] != 0.) unsafeAtomicAdd(&model2[x * s2 + (long)y * size + z], s_m[i]);
+        if (s_m[i] != (A)0) unsafeAtomicAdd(&model2[x * s2 + (long)y * size + z], (double)s_m[i]);
     }
 }
 
@@ -254,16 +256,21 @@ void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const 
                          const double *rots, int r0, int num_rot, const double *view,
                          const double *psum_r, int all_on, const double *qvals, const double *corr,
                          const uint8_t *mask, int num_pix, long size, int npatch, int nbx,
-                         double *model2, double *weight, int do_model, int do_weight) {
+                         double *model2, double *weight, int do_model, int do_weight, int f32_sums) {
     if (num_units <= 0) return;
 #define EMC_MB(M, W)                                                                             \
     merge_brick_kernel<M, W><<<num_units, kBlock, 0, s>>>(units, list, rots, r0, num_rot, view, psum_r, \
                                                           all_on, qvals, corr, mask, num_pix, size, \
                                                           npatch, nbx, model2, weight)
     if (do_model && do_weight) EMC_MB(true, true);
+    else if (do_model && f32_sums)
+        merge_brick_model_kernel<float><<<num_units, kBlock, 0, s>>>(units, list, rots, r0, num_rot, view, psum_r,
+                                                                     all_on, qvals, corr, mask, num_pix, size,
+                                                                     npatch, nbx, model2);
     else if (do_model)
-        merge_brick_model_kernel<<<num_units, kBlock, 0, s>>>(units, list, rots, r0, num_rot, view, psum_r, all_on,
-                                                              qvals, corr, mask, num_pix, size, npatch, nbx, model2);
+        merge_brick_model_kernel<double><<<num_units, kBlock, 0, s>>>(units, list, rots, r0, num_rot, view, psum_r,
+                                                                      all_on, qvals, corr, mask, num_pix, size,
+                                                                      npatch, nbx, model2);
     else if (do_weight) EMC_MB(false, true);
 #undef EMC_MB
 }

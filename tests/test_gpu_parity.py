@@ -436,8 +436,14 @@ def test_options_roundtrip_and_variants_agree(tiny):
     g1 = P.run_engine(eng, flags)
     eng.set_option('tomo_parts', 3)
     eng.set_option('lik_tiles', 2)
+    eng.set_option('lik_xcd', 0)
     g2 = P.run_engine(eng, flags)
+    eng.set_option('lik_split', 4)
+    eng.set_option('max_chunk_cols', 128)
+    g3 = P.run_engine(eng, flags)
     eng.close()
-    for k in FIELDS_F:
-        assert P.rel_err(g2[k], g1[k]) <= TOL, k
-    assert np.array_equal(g1['rmax'], g2['rmax'])
+    for g in (g2, g3):
+        for k in FIELDS_F:
+            assert P.rel_err(g[k], g1[k]) <= TOL, k
+        assert np.array_equal(g1['rmax'], g['rmax'])
+    assert g3['num_chunks'] > 1
