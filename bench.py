@@ -305,7 +305,7 @@ def bench_learned(args):
     ntr = int(0.9 * D)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    hist = net.fit(X, Y, ntr, epochs=args.epochs, batch=1024, lr=1e-3, log=log)
+    hist = net.fit(X, Y, ntr, epochs=args.epochs, batch=1024, lr=1e-3, log=log, wd=args.wd)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     del Y
@@ -394,6 +394,7 @@ def bench_learned(args):
         'heldout_best_topk_mass': round(float(best.mean()), 4),
         'heldout_best_topk_ge_threshold': round(float((best >= cfg['mass_threshold']).mean()), 4),
         'model_rel_err': float('%.3g' % model_err), 'train_s': round(train_s, 2), 'epochs': args.epochs,
+        'weight_decay': args.wd,
         'agreement_vs_fp64': {k: float('%.3g' % v) for k, v in diag.items()},
         'train_loss': [round(h, 4) for h in hist[::max(1, len(hist) // 10)]] + [round(hist[-1], 4)],
         'stage_ms_learned': {k: round(v, 1) for k, v in st_learned.items()},
@@ -514,6 +515,7 @@ def main():
                     'configuration): FP32 log W and posterior gathers, compared with FP64 (the '
                     'default of the learned-proposal iteration; 0 there: FP64)')
     ap.add_argument('--epochs', type=int, default=80, help='learned-proposal workload: training epochs')
+    ap.add_argument('--wd', type=float, default=0., help='learned-proposal workload: L2 weight decay')
     ap.add_argument('--compare-frames', type=int, default=50000,
                     help='uneven-batching workload: frames of the sparse vs masked comparison')
     ap.add_argument('--autotune', action='store_true',

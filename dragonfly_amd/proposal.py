@@ -185,9 +185,9 @@ class OrientationProposal:
                               C.c_float(eps), C.c_float(0.), self.step, None, None, 0), 'adam')
 
     # ---- training and inference ----------------------------------------------------------------
-    def fit(self, X, Y, num, epochs=5, batch=1024, lr=1e-3, seed=0, log=None):
-        """Minibatch Adam on the first `num` rows of X (features) and Y (targets, rows summing
-        to 1). Returns the mean loss of every epoch."""
+    def fit(self, X, Y, num, epochs=5, batch=1024, lr=1e-3, seed=0, log=None, wd=0.):
+        """Minibatch Adam (L2 weight decay wd on the weights) on the first `num` rows of X
+        (features) and Y (targets, rows summing to 1). Returns the mean loss of every epoch."""
         torch = self.torch
         batch = _round(batch, TILE)
         gen = torch.Generator(device='cpu')
@@ -207,7 +207,7 @@ class OrientationProposal:
                 Yb[:rows] = Y[idx]
                 tot += self.loss_and_grads(Xb, Yb, rows) * rows
                 cnt += rows
-                self.apply_adam(lr)
+                self.apply_adam(lr, wd=wd)
             history.append(tot / max(cnt, 1))
             if log:
                 log('epoch %d loss %.4f' % (ep + 1, history[-1]))
