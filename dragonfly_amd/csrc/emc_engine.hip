@@ -259,8 +259,14 @@ struct emc_engine {
     bool subset_ready = false, subset_ran = false;
     int num_slots = 0, num_tiles = 0, slot_C = 0;
     // slot order of the last coarse mapping (kept while the mapping does not change)
-    std::vector<int> h_slot_qmap, h_tile0;
+    std::vector<int> h_slot_qmap, h_tile0;  // h_tile0: first tile of each cell (per-cell padding)
     int slot_num_coarse = 0;
+    // slot_pack = 1: cells placed back to back along a nearest-neighbour chain of their mean
+    // orientations, no padding per cell; a tile may then hold orientations of two or three
+    // cells, and slots outside a partial frame's cells are masked (slot_cell, relbits)
+    int slot_pack = 1, slot_pack_built = -1;
+    std::vector<int> h_ctile0, h_ctile1;  // packed: first and last tile of each cell
+    DevBuf<int> slot_cell;
     // full frames (every coarse cell): rows tile-major in RF; photon lists of mask < 2 pixels
     // of the full frames per pixel (frame index f), split into tom_parts frame ranges
     int num_full = 0, full_parts = 1;
@@ -579,21 +585,103 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
     const int Rn = e->num_rot, N = e->num_data;
     hipStream_t s = e->stream;
     int rc;
+    const bool pack = e->slot_pack != 0;
     const bool same_slots = e->slot_num_coarse == num_coarse && (int)e->h_slot_qmap.size() == Rn &&
+                            e->slot_pack_built == (int)pack &&
                             std::equal(e->h_slot_qmap.begin(), e->h_slot_qmap.end(), qmap);
     if (!same_slots) {
         std::vector<int> cnt(num_coarse, 0), tile0(num_coarse + 1, 0);
         for (int r = 0; r < Rn; ++r) ++cnt[qmap[r]];
-        for (int c = 0; c < num_coarse; ++c) tile0[c + 1] = tile0[c] + (cnt[c] + emc::kRTile - 1) / emc::kRTile;
-        const int T = tile0[num_coarse], S = T * emc::kRTile;
-        if (S <= 0) return fail(EMC_E_ARG, "emc_set_refine: no orientations");
-        std::vector<int> slot_rot(S, -1), rot_slot(Rn), fill(num_coarse, 0);
-        for (int r = 0; r < Rn; ++r) {
-            const int c = qmap[r];
-            const int sl = tile0[c] * emc::kRTile + fill[c]++;
-            slot_rot[sl] = r;
-            rot_slot[r] = sl;
+        int T = 0;
+        std::vector<int> slot_rot, rot_slot(Rn), cell_slot;
+        std::vector<int> ctile0(num_coarse, 0), ctile1(num_coarse, -1);
+        if (pack) {
+            // cells along a greedy nearest-neighbour chain of their mean orientations (from the
+            // first cell, always on to the closest cell not yet placed), so that the cells
+            // around an orientation tend to sit next to each other in slot order; members in
+            // ascending orientation index
+            std::vector<double> mean((size_t)num_coarse * 4, 0.), first((size_t)num_coarse * 4, 0.);
+            std::vector<uint8_t> has(num_coarse, 0);
+            for (int r = 0; r < Rn; ++r) {
+                const int c = qmap[r];
+                const double *q = &e->h_quats[5 * (size_t)r];
+                if (!has[c]) {
+                    has[c] = 1;
+                    for (int k = 0; k < 4; ++k) first[4 * (size_t)c + k] = q[k];
+                }
+                double dot = 0.;
+                for (int k = 0; k < 4; ++k) dot += q[k] * first[4 * (size_t)c + k];
+                const double sg = dot < 0. ? -1. : 1.;
+                for (int k = 0; k < 4; ++k) mean[4 * (size_t)c + k] += sg * q[k];
+            }
+            for (int c = 0; c < num_coarse; ++c) {
+                double *m = &mean[4 * (size_t)c];
+                double nrm = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2] + m[3] * m[3]);
+                if (nrm == 0.) nrm = 1.;
+                for (int k = 0; k < 4; ++k) m[k] /= nrm;
+            }
+            std::vector<int> chain;
+            chain.reserve(num_coarse);
+            std::vector<uint8_t> placed(num_coarse, 0);
+            int cur = 0;
+            for (int n = 0; n < num_coarse; ++n) {
+                chain.push_back(cur);
+                placed[cur] = 1;
+                const double *a = &mean[4 * (size_t)cur];
+                int next = -1;
+                double best = -1.;
+                for (int c = 0; c < num_coarse; ++c) {
+                    if (placed[c]) continue;
+                    const double *b = &mean[4 * (size_t)c];
+                    const double dd = std::fabs(a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]);
+                    if (dd > best) {
+                        best = dd;
+                        next = c;
+                    }
+                }
+                if (next < 0) break;
+                cur = next;
+            }
+            std::vector<int> start(num_coarse, 0), fill(num_coarse, 0);
+            int at = 0;
+            for (int c : chain) {
+                start[c] = at;
+                at += cnt[c];
+            }
+            const int S = (Rn + emc::kRTile - 1) / emc::kRTile * emc::kRTile;
+            T = S / emc::kRTile;
+            slot_rot.assign(S, -1);
+            cell_slot.assign(S, -1);
+            for (int r = 0; r < Rn; ++r) {
+                const int c = qmap[r];
+                const int sl = start[c] + fill[c]++;
+                slot_rot[sl] = r;
+                rot_slot[r] = sl;
+                cell_slot[sl] = c;
+            }
+            for (int c = 0; c < num_coarse; ++c)
+                if (cnt[c] > 0) {
+                    ctile0[c] = start[c] / emc::kRTile;
+                    ctile1[c] = (start[c] + cnt[c] - 1) / emc::kRTile;
+                }
+        } else {
+            for (int c = 0; c < num_coarse; ++c) tile0[c + 1] = tile0[c] + (cnt[c] + emc::kRTile - 1) / emc::kRTile;
+            T = tile0[num_coarse];
+            slot_rot.assign((size_t)T * emc::kRTile, -1);
+            std::vector<int> fill(num_coarse, 0);
+            for (int r = 0; r < Rn; ++r) {
+                const int c = qmap[r];
+                const int sl = tile0[c] * emc::kRTile + fill[c]++;
+                slot_rot[sl] = r;
+                rot_slot[r] = sl;
+            }
+            for (int c = 0; c < num_coarse; ++c) {
+                ctile0[c] = tile0[c];
+                ctile1[c] = tile0[c + 1] - 1;
+            }
         }
+        const int S = T * emc::kRTile;
+        if (S <= 0) return fail(EMC_E_ARG, "emc_set_refine: no orientations");
         std::vector<double> rots((size_t)S * 9, 0.), quats((size_t)S * 5, 0.), lw(S, 0.);
         for (int sl = 0; sl < S; ++sl) {
             const int r = slot_rot[sl];
@@ -610,16 +698,25 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
         if ((rc = e->rots_slot.upload(rots.data(), rots.size(), s))) return rc;
         if ((rc = e->quats_slot.upload(quats.data(), quats.size(), s))) return rc;
         if ((rc = e->logwr_slot.upload(lw.data(), S, s))) return rc;
+        if (pack) {
+            if ((rc = e->slot_cell.upload(cell_slot.data(), S, s))) return rc;
+        } else {
+            e->slot_cell.release();
+        }
         HIP_TRY(hipStreamSynchronize(s));
         e->h_slot_qmap.assign(qmap, qmap + Rn);
         e->h_tile0 = tile0;
+        e->h_ctile0 = ctile0;
+        e->h_ctile1 = ctile1;
+        e->slot_pack_built = (int)pack;
         e->slot_num_coarse = num_coarse;
         e->num_slots = S;
         e->num_tiles = T;
         e->slot_bricks_valid = false;
     }
-    const std::vector<int> &tile0 = e->h_tile0;
+    const std::vector<int> &ct0 = e->h_ctile0, &ct1 = e->h_ctile1;
     const int T = e->num_tiles;
+    std::vector<int> tiles;
     std::vector<long> &poff = e->h_poff;
     std::vector<int> &pframe = e->h_pframe, &ptile = e->h_ptile, &ffull = e->h_ffull;
     std::vector<int> fidx;
@@ -641,12 +738,17 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
             ffull[d] = (int)fidx.size();
             fidx.push_back(d);
         } else {
-            std::sort(cells.begin(), cells.end());
+            // the tiles holding the frame's cells, ascending and distinct (with packed cells
+            // neighbouring cells share tiles)
+            tiles.clear();
             for (int c : cells)
-                for (int t = tile0[c]; t < tile0[c + 1]; ++t) {
-                    pframe.push_back(d);
-                    ptile.push_back(t);
-                }
+                for (int t = ct0[c]; t <= ct1[c]; ++t) tiles.push_back(t);
+            std::sort(tiles.begin(), tiles.end());
+            tiles.erase(std::unique(tiles.begin(), tiles.end()), tiles.end());
+            for (int t : tiles) {
+                pframe.push_back(d);
+                ptile.push_back(t);
+            }
         }
         poff[d + 1] = (long)pframe.size();
     }
@@ -1038,6 +1140,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
         return rc;
     if ((rc = timed(3, [&] {
              emc::launch_pair_max(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, full, e->slot_rot.p,
+                                  e->slot_pack_built == 1 ? e->slot_cell.p : nullptr, e->relbits.p, e->rel_words,
                                   e->logwr_slot.p, e->u_slot.p, e->rbad_slot.p, e->scale.p, use_scale, e->scalars.p,
                                   e->blacklist.p, dense, N, e->max_exp.p, e->rmax.p);
              return 0;
@@ -1176,6 +1279,7 @@ const EngineOption kOptions[] = {
     {"max_chunk_cols", &emc_engine::max_chunk_cols, 0, INT_MAX, false},
     {"merge_mode", &emc_engine::merge_mode, 0, 1, false},
     {"subset", &emc_engine::subset_mode, 0, 1, false},
+    {"slot_pack", &emc_engine::slot_pack, 0, 1, false},
 };
 
 int set_option(emc_engine *e, const EngineOption &o, int value) {

@@ -143,10 +143,13 @@ void launch_pair_likelihood(hipStream_t s, const double *logw, int num_pix, cons
 void launch_pair_likelihood(hipStream_t s, const float *logw, int num_pix, const long *off, const int *pix,
                             const double *cnt, const int *pair_frame, const int *pair_tile, const long *tile_off,
                             const int *tile_pair, long num_pairs, int t0, int t1, double *G);
+// slot_cell (packed slot order, else null) with the frames' cell bit rows (relbits, words per
+// frame) masks the orientations of a partial frame's tiles outside its cells.
 void launch_pair_max(hipStream_t s, double *R, const long *pair_off, const int *pair_tile, const FullRows &full,
-                     const int *slot_rot, const double *logwr_slot, const double *u_slot, const int *rbad_slot,
-                     const double *scale, int use_scale, const double *scalars, const uint8_t *blacklist,
-                     const uint8_t *dense, int num_data, double *max_exp, int *rmax);
+                     const int *slot_rot, const int *slot_cell, const uint32_t *relbits, int words,
+                     const double *logwr_slot, const double *u_slot, const int *rbad_slot, const double *scale,
+                     int use_scale, const double *scalars, const uint8_t *blacklist, const uint8_t *dense,
+                     int num_data, double *max_exp, int *rmax);
 void launch_pair_norm(hipStream_t s, double *R, const long *pair_off, const int *pair_tile, const FullRows &full,
                       const int *slot_rot, const double *beta, const uint8_t *blacklist, const double *max_exp,
                       const int *rmax, int num_data, double *p_norm);

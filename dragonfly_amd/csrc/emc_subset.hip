@@ -104,7 +104,8 @@ __global__ __launch_bounds__(kBlock) void pair_likelihood_kernel(
 // arg-max by original orientation index.
 __global__ __launch_bounds__(kBlock) void pair_max_kernel(
     double *__restrict__ R, const long *__restrict__ pair_off, const int *__restrict__ pair_tile,
-    FullRows full, const int *__restrict__ slot_rot, const double *__restrict__ logwr_slot,
+    FullRows full, const int *__restrict__ slot_rot, const int *__restrict__ slot_cell,
+    const uint32_t *__restrict__ relbits, int words, const double *__restrict__ logwr_slot,
     const double *__restrict__ u_slot, const int *__restrict__ rbad_slot,
     const double *__restrict__ scale, int use_scale, const double *__restrict__ scalars,
     const uint8_t *__restrict__ blacklist, const uint8_t *__restrict__ dense,
@@ -122,14 +123,21 @@ __global__ __launch_bounds__(kBlock) void pair_max_kernel(
     const bool dense_frame = dense != nullptr && dense[d];
     const double phi = use_scale ? scale[d] : scalars[0];
     const FrameRows fr = frame_rows(R, pair_off, pair_tile, full, d);
+    // packed slots: a partial frame's tiles may hold orientations of cells outside its set
+    const uint32_t *__restrict__ bits = (slot_cell && fr.f < 0) ? relbits + (size_t)d * words : nullptr;
     double best = kMaxExpStart;
     int bi = -1;
     for (long e = fr.e0 + threadIdx.x; e < fr.e1; e += kBlock) {
         int s;
         double *x = fr.at(e, s);
         const int r = slot_rot[s];
+        bool on = r >= 0 && !(dense_frame && rbad_slot[s]);
+        if (on && bits) {
+            const int c = slot_cell[s];
+            on = (bits[c >> 5] >> (c & 31)) & 1u;
+        }
         double v = -INFINITY;
-        if (r >= 0 && !(dense_frame && rbad_slot[s])) v = (logwr_slot[s] - u_slot[s] * phi) + *x;
+        if (on) v = (logwr_slot[s] - u_slot[s] * phi) + *x;
         *x = v;
         if (v > best || (v == best && r >= 0 && bi >= 0 && r < bi)) {
             best = v;
@@ -464,12 +472,13 @@ void launch_pair_likelihood(hipStream_t s, const float *logw, int num_pix, const
 }
 
 void launch_pair_max(hipStream_t s, double *R, const long *pair_off, const int *pair_tile, const FullRows &full,
-                     const int *slot_rot, const double *logwr_slot, const double *u_slot, const int *rbad_slot,
-                     const double *scale, int use_scale, const double *scalars, const uint8_t *blacklist,
-                     const uint8_t *dense, int num_data, double *max_exp, int *rmax) {
-    pair_max_kernel<<<num_data, kBlock, 0, s>>>(R, pair_off, pair_tile, full, slot_rot, logwr_slot, u_slot,
-                                                rbad_slot, scale, use_scale, scalars, blacklist, dense, max_exp,
-                                                rmax);
+                     const int *slot_rot, const int *slot_cell, const uint32_t *relbits, int words,
+                     const double *logwr_slot, const double *u_slot, const int *rbad_slot, const double *scale,
+                     int use_scale, const double *scalars, const uint8_t *blacklist, const uint8_t *dense,
+                     int num_data, double *max_exp, int *rmax) {
+    pair_max_kernel<<<num_data, kBlock, 0, s>>>(R, pair_off, pair_tile, full, slot_rot, slot_cell, relbits, words,
+                                                logwr_slot, u_slot, rbad_slot, scale, use_scale, scalars, blacklist,
+                                                dense, max_exp, rmax);
 }
 
 void launch_pair_norm(hipStream_t s, double *R, const long *pair_off, const int *pair_tile, const FullRows &full,

@@ -137,7 +137,9 @@ int emc_set_precision(emc_engine *e, int mode);
  * group, partial sums added in range order), tomo_tiles, tomo_tile_mode (0 per-pixel gather,
  * 1 LDS tiles, 2 frame-partitioned, default), tomo_parts (frame ranges, 1-64), tomo_width32,
  * tomo_unroll (8/16), tomo_seq, max_chunk_cols (0 = by memory), merge_mode (0 box, 1 brick),
- * subset (1 sparse refinement subsets, 0 masked). Start values come from EMC_<NAME>.      */
+ * subset (1 sparse refinement subsets, 0 masked), slot_pack (1: refinement cells packed
+ * back to back along a nearest-neighbour chain, 0: each cell padded to whole 64-orientation
+ * tiles; takes effect at the next emc_set_refine). Start values come from EMC_<NAME>.      */
 int emc_set_option(emc_engine *e, const char *name, int value);
 int emc_get_option(emc_engine *e, const char *name, int *value);
 /* per-frame state of struct iterate: scale, beta, blacklist, fcounts, sum_fact,

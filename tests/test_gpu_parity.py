@@ -234,15 +234,17 @@ def test_dense_integer_frames(tiny, zero_block):
     _compare(o, g, 'dense zero_block=%s' % zero_block)
 
 
-@pytest.mark.parametrize('form', ['subset', 'subset-chunked', 'masked'])
+@pytest.mark.parametrize('form', ['subset', 'subset-chunked', 'subset-padded', 'masked'])
 def test_refinement_subsets(tiny, monkeypatch, form):
     """emc_set_refine: per-frame coarse cells (Voronoi of quat_gen(1)), one frame with an
     empty set and every fourth frame with every cell (a full frame: the fall-back to the plain
     search, with repeated cells in its list), against the oracle's refinement branch;
-    evaluated sparsely over (frame, tile) pairs (in one or in 64-slot chunks), or over every
-    pair with the excluded ones masked."""
+    evaluated sparsely over (frame, tile) pairs (in one or in 64-slot chunks; cells packed back
+    to back, or each padded to whole tiles), or over every pair with the excluded ones masked."""
     if form == 'subset-chunked':
         monkeypatch.setenv('EMC_MAX_CHUNK_COLS', '64')
+    if form == 'subset-padded':
+        monkeypatch.setenv('EMC_SLOT_PACK', '0')
     if form == 'masked':
         monkeypatch.setenv('EMC_SUBSET', '0')
     from oracle import core
