@@ -724,17 +724,19 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
     pframe.clear();
     ptile.clear();
     ffull.assign(N, -1);
-    std::vector<uint8_t> seen(num_coarse, 0);
+    // the frames' distinct cells come from their bit rows (built by emc_set_refine)
+    const int words = e->rel_words;
+    const std::vector<uint32_t> &bits = e->h_relbits;
     std::vector<int> cells;
     for (int d = 0; d < N; ++d) {
         cells.clear();
-        for (long k = rel_off[d]; k < rel_off[d + 1]; ++k)
-            if (!seen[rel_quat[k]]) {
-                seen[rel_quat[k]] = 1;
-                cells.push_back(rel_quat[k]);
-            }
-        for (int c : cells) seen[c] = 0;
-        if ((int)cells.size() == num_coarse) {
+        const uint32_t *row = &bits[(size_t)d * words];
+        int n = 0;
+        for (int w = 0; w < words; ++w) n += __builtin_popcount(row[w]);
+        if (n < num_coarse)
+            for (int w = 0; w < words; ++w)
+                for (uint32_t x = row[w]; x; x &= x - 1) cells.push_back(w * 32 + __builtin_ctz(x));
+        if (n == num_coarse) {
             ffull[d] = (int)fidx.size();
             fidx.push_back(d);
         } else {

@@ -186,9 +186,7 @@ class Iterate:
             self.num_coarse = qcoarse.shape[0]
             self._qmap_key = key
         self.rel_off = rel_off
-        self.rel_quat = rel_quat
-        if self.rel_quat.size and (self.rel_quat.min() < 0 or self.rel_quat.max() >= self.num_coarse):
-            raise ValueError('rel_quat entries must index the %d coarse orientations' % self.num_coarse)
+        self.rel_quat = rel_quat  # entries are range-checked by the engine (emc_set_refine)
 
     def parse_rel_quat(self, fname, coarse_div):
         """Per-frame coarse orientation sets from a previous coarse run: the binary probs
