@@ -22,7 +22,7 @@ FIELDS = {
     'rmax': (3, 'i4', 'data'), 'likelihood': (4, 'f8', 'data'), 'info': (5, 'f8', 'data'),
     'psum_d': (6, 'f8', 'data'), 'scale': (7, 'f8', 'data'), 'model1': (8, 'f8', 'vol'),
     'model2': (9, 'f8', 'vol'), 'inter_weight': (10, 'f8', 'vol'), 'psum_r': (11, 'f8', 'rot'),
-    'prob': (12, 'f8', 'data_rot'),
+    'prob': (12, 'f8', 'data_rot'), 'quat_norm': (13, 'f8', 'data_mode'),
 }
 
 STAGES = ('expand', 'likelihood', 'rescale', 'max', 'norm', 'posterior', 'tomogram', 'reduce',
@@ -61,6 +61,7 @@ def lib():
             'emc_num_rot': (C.c_int, [C.c_int]),
             'emc_quat_gen': (C.c_int, [C.c_int, _dp]),
             'emc_set_detector': (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _u8p]),
+            'emc_set_modes': (C.c_int, [C.c_void_p, C.c_int]),
             'emc_set_quats': (C.c_int, [C.c_void_p, C.c_int, _dp]),
             'emc_set_model': (C.c_int, [C.c_void_p, C.c_long, _dp]),
             'emc_set_frames_sparse': (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _lp, _lp, _ip, _ip, _ip]),
@@ -141,6 +142,7 @@ class Engine:
         _check(lib().emc_create(int(device), C.byref(h)))
         self._h = h
         self.num_pix = self.num_rot = self.num_data = self.size = 0
+        self.num_rot_user, self.num_modes = 0, 1
 
     def close(self):
         if getattr(self, '_h', None):
@@ -162,19 +164,32 @@ class Engine:
         _check(lib().emc_set_detector(self._h, corr.shape[0], _ptr(qvals), _ptr(corr), _ptr(mask, _u8p)))
         self.num_pix = corr.shape[0]
 
+    def set_modes(self, num_modes):
+        """Number of model modes (params num_modes). The engine's orientations become the
+        combined r = rotation * num_modes + mode; the model must be set again afterwards."""
+        _check(lib().emc_set_modes(self._h, int(num_modes)))
+        if int(num_modes) != self.num_modes:
+            self.size = 0
+        self.num_modes = int(num_modes)
+        self.num_rot = self.num_rot_user * self.num_modes
+
     def set_quats(self, quats):
         quats = np.ascontiguousarray(quats, dtype='f8')
         if quats.ndim != 2 or quats.shape[1] != 5:
             raise ValueError('quats must be (num_rot, 5)')
         _check(lib().emc_set_quats(self._h, quats.shape[0], _ptr(quats)))
-        self.num_rot = quats.shape[0]
+        self.num_rot_user = quats.shape[0]
+        self.num_rot = self.num_rot_user * self.num_modes
 
     def set_model(self, model1):
+        """model1: (size, size, size) for one mode, (num_modes, size, size, size) for any."""
         model1 = np.ascontiguousarray(model1, dtype='f8')
-        if model1.ndim != 3 or len(set(model1.shape)) != 1:
-            raise ValueError('model1 must be a cube')
-        _check(lib().emc_set_model(self._h, model1.shape[0], _ptr(model1)))
-        self.size = model1.shape[0]
+        if model1.ndim == 3 and self.num_modes == 1:
+            model1 = model1[None]
+        if model1.ndim != 4 or model1.shape[0] != self.num_modes or len(set(model1.shape[1:])) != 1:
+            raise ValueError('model1 must be %d cube(s)' % self.num_modes)
+        _check(lib().emc_set_model(self._h, model1.shape[1], _ptr(model1)))
+        self.size = model1.shape[1]
 
     def set_frames(self, ones, multi, ones_accum, multi_accum, place_ones, place_multi, count_multi):
         a = [np.ascontiguousarray(x, dtype=t) for x, t in (
@@ -273,8 +288,10 @@ class Engine:
 
     def get(self, name):
         fid, dt, kind = FIELDS[name]
-        shape = {'rot': (self.num_rot,), 'data': (self.num_data,),
-                 'vol': (self.size,) * 3, 'data_rot': (self.num_data, self.num_rot)}[kind]
+        vol = (self.size,) * 3 if self.num_modes == 1 else (self.num_modes,) + (self.size,) * 3
+        shape = {'rot': (self.num_rot,), 'data': (self.num_data,), 'vol': vol,
+                 'data_rot': (self.num_data, self.num_rot),
+                 'data_mode': (self.num_data, self.num_modes)}[kind]
         out = np.empty(shape, dtype=dt)
         _check(lib().emc_get_field(self._h, fid, out.ctypes.data_as(C.c_void_p), out.size))
         return out

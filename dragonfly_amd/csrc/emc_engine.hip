@@ -156,13 +156,16 @@ struct emc_engine {
     DevBuf<double> qvals, corr;        // engine pixel order
     DevBuf<uint8_t> mask;
     DevBuf<int> d_perm;
-    // orientations (rots padded to Rp with identity)
-    int num_rot = 0, Rp = 0;
-    std::vector<double> h_quats;
+    // orientations (rots padded to Rp with identity). With num_modes > 1 the engine's
+    // orientations are the combined ones, r = rotation * num_modes + mode (max_prob.c:80-84):
+    // num_rot = num_rot_user * num_modes, and quats / rots / logwr hold rotation r / num_modes.
+    int num_rot = 0, Rp = 0, num_rot_user = 0, num_modes = 1;
+    std::vector<double> h_quats;  // the caller's [num_rot_user][5]
     DevBuf<double> quats, rots, logwr;
-    // model
+    // model: num_modes volumes of size^3 each
     long size = 0;
     DevBuf<double> model1, model2, weight;
+    DevBuf<double> quat_norm;  // [num_data][num_modes] occupancies when num_modes > 1
     // frames
     int num_data = 0;
     std::vector<int> h_ones, h_multi, h_place_ones, h_place_multi, h_count_multi;
@@ -512,11 +515,11 @@ int build_lik_split(emc_engine *e, int X) {
 // `total` rotations (a multiple of 64) walked in chunks of C; rotations at or past num_valid,
 // and slots with slot_rot < 0 when slot_rot is given, make no items.
 int build_bricks(emc_engine *e, std::vector<emc_engine::BrickChunk> &out, const double *rots, int total,
-                 int C, int num_valid, const int *slot_rot) {
+                 int C, int num_valid, const int *slot_rot, int modes) {
     hipStream_t s = e->stream;
     const int P = e->num_pix;
     const int nbx = (int)((e->size + emc::kBrick - 1) / emc::kBrick);
-    const size_t nb = (size_t)nbx * nbx * nbx;
+    const size_t nb = (size_t)nbx * nbx * nbx * modes;  // bricks of every mode's volume
     const int npatch = (P + 63) / 64;
     int rc;
     if ((rc = e->brick_counts.alloc(nb))) return rc;
@@ -533,7 +536,7 @@ int build_bricks(emc_engine *e, std::vector<emc_engine::BrickChunk> &out, const 
         bc.Cc = Cc;
         HIP_TRY(hipMemsetAsync(e->brick_counts.p, 0, nb * sizeof(int), s));
         emc::launch_brick_items(s, rots, r0, Cc, num_valid, e->qvals.p, e->mask.p, P, e->size, npatch, nbx, 0,
-                                e->brick_counts.p, nullptr, nullptr, nullptr, slot_rot);
+                                e->brick_counts.p, nullptr, nullptr, nullptr, slot_rot, modes);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(counts.data(), e->brick_counts.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
@@ -551,7 +554,7 @@ int build_bricks(emc_engine *e, std::vector<emc_engine::BrickChunk> &out, const 
         if ((rc = e->brick_off.upload(off.data(), nb, s))) return rc;
         HIP_TRY(hipMemsetAsync(e->brick_fill.p, 0, nb * sizeof(int), s));
         emc::launch_brick_items(s, rots, r0, Cc, num_valid, e->qvals.p, e->mask.p, P, e->size, npatch, nbx, 1,
-                                nullptr, e->brick_off.p, e->brick_fill.p, bc.list.p, slot_rot);
+                                nullptr, e->brick_off.p, e->brick_fill.p, bc.list.p, slot_rot, modes);
         HIP_TRY(hipGetLastError());
         bc.num_units = (int)units.size();
         if (!units.empty() && (rc = bc.units.upload(units.data(), units.size(), s))) return rc;
@@ -561,7 +564,7 @@ int build_bricks(emc_engine *e, std::vector<emc_engine::BrickChunk> &out, const 
 }
 
 int build_bricks(emc_engine *e) {
-    int rc = build_bricks(e, e->bricks, e->rots.p, e->Rp, e->C, e->num_rot, nullptr);
+    int rc = build_bricks(e, e->bricks, e->rots.p, e->Rp, e->C, e->num_rot, nullptr, e->num_modes);
     if (!rc) e->bricks_valid = true;
     return rc;
 }
@@ -1177,7 +1180,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     // merge: bricks over the slot order (padding slots make no items)
     if (!e->slot_bricks_valid) {
         if ((rc = timed(8, [&] {
-                 return build_bricks(e, e->slot_bricks, e->rots_slot.p, S, C, S, e->slot_rot.p);
+                 return build_bricks(e, e->slot_bricks, e->rots_slot.p, S, C, S, e->slot_rot.p, 1);
              })))
             return rc;
         e->slot_bricks_valid = true;
@@ -1421,27 +1424,58 @@ int emc_set_detector(emc_engine *e, int num_pix, const double *qvals, const doub
     return 0;
 }
 
-int emc_set_quats(emc_engine *e, int num_rot, const double *quats) {
-    if (!e || num_rot <= 0 || !quats) return fail(EMC_E_ARG, "emc_set_quats: bad argument");
-    HIP_TRY(hipSetDevice(e->device));
+// Rotation matrices, log weights and quaternions of the combined orientations
+// r = rotation * num_modes + mode from the caller's quaternions (h_quats).
+static int upload_quats(emc_engine *e) {
+    const int M = e->num_modes, nu = e->num_rot_user;
+    if ((long)nu * M > INT32_MAX - emc::kRTile) return fail(EMC_E_ARG, "emc_set_quats: too many orientations");
+    const int num_rot = nu * M;
     const int Rp = (num_rot + emc::kRTile - 1) / emc::kRTile * emc::kRTile;
-    std::vector<double> rot((size_t)Rp * 9, 0.), lw(Rp, 0.);
+    std::vector<double> rot((size_t)Rp * 9, 0.), lw(Rp, 0.), q((size_t)num_rot * 5);
     for (int r = 0; r < Rp; ++r) {
         if (r < num_rot) {
-            make_rot(&quats[5 * (size_t)r], &rot[9 * (size_t)r]);
-            lw[r] = std::log(quats[5 * (size_t)r + 4]);  // log(w_r), max_prob.c:125
+            const double *qu = &e->h_quats[5 * (size_t)(r / M)];
+            make_rot(qu, &rot[9 * (size_t)r]);
+            lw[r] = std::log(qu[4]);  // log(w_r) of the rotation, max_prob.c:125
+            std::copy(qu, qu + 5, &q[5 * (size_t)r]);
         } else {
             rot[9 * (size_t)r] = rot[9 * (size_t)r + 4] = rot[9 * (size_t)r + 8] = 1.;
         }
     }
     int rc;
-    e->h_quats.assign(quats, quats + 5 * (size_t)num_rot);
-    if ((rc = e->quats.upload(quats, 5 * (size_t)num_rot, e->stream))) return rc;
+    if ((rc = e->quats.upload(q.data(), q.size(), e->stream))) return rc;
     if ((rc = e->rots.upload(rot.data(), rot.size(), e->stream))) return rc;
     if ((rc = e->logwr.upload(lw.data(), lw.size(), e->stream))) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->num_rot = num_rot;
     e->Rp = Rp;
+    return 0;
+}
+
+int emc_set_modes(emc_engine *e, int num_modes) {
+    if (!e || num_modes < 1 || num_modes > EMC_MAX_MODES)
+        return fail(EMC_E_ARG, "emc_set_modes: num_modes must be 1..%d", EMC_MAX_MODES);
+    if (num_modes == e->num_modes) return 0;
+    HIP_TRY(hipSetDevice(e->device));
+    e->num_modes = num_modes;
+    e->size = 0;  // the model holds num_modes volumes: set it again
+    e->have_results = false;
+    e->weight_all_valid = false;
+    e->bricks_valid = false;
+    e->refine = false;
+    e->h_slot_qmap.clear();
+    e->slot_num_coarse = 0;
+    if (e->num_rot_user > 0) return upload_quats(e);
+    return 0;
+}
+
+int emc_set_quats(emc_engine *e, int num_rot, const double *quats) {
+    if (!e || num_rot <= 0 || !quats) return fail(EMC_E_ARG, "emc_set_quats: bad argument");
+    HIP_TRY(hipSetDevice(e->device));
+    e->h_quats.assign(quats, quats + 5 * (size_t)num_rot);
+    e->num_rot_user = num_rot;
+    int rc;
+    if ((rc = upload_quats(e))) return rc;
     e->have_results = false;
     e->weight_all_valid = false;
     e->bricks_valid = false;
@@ -1454,7 +1488,7 @@ int emc_set_quats(emc_engine *e, int num_rot, const double *quats) {
 int emc_set_model(emc_engine *e, long size, const double *model1) {
     if (!e || size < 3 || !model1) return fail(EMC_E_ARG, "emc_set_model: bad argument");
     HIP_TRY(hipSetDevice(e->device));
-    const size_t vol = (size_t)size * size * size;
+    const size_t vol = (size_t)size * size * size * e->num_modes;  // every mode's volume
     int rc;
     if ((rc = e->model1.upload(model1, vol, e->stream))) return rc;
     if ((rc = e->model2.alloc(vol))) return rc;
@@ -1535,6 +1569,9 @@ int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const
     if (!e->num_rot || !e->num_data)
         return fail(EMC_E_STATE, "emc_set_refine: set quaternions and frames first");
     if (num_coarse <= 0 || !rel_off || !rel_quat) return fail(EMC_E_ARG, "emc_set_refine: bad argument");
+    // the reference indexes quat_mapping (one entry per rotation) by the combined orientation
+    // index (max_prob.c:111), which runs past it for several modes
+    if (e->num_modes > 1) return fail(EMC_E_STATE, "emc_set_refine: refinement takes one mode");
     HIP_TRY(hipSetDevice(e->device));
     const int N = e->num_data, words = (num_coarse + 31) / 32;
     std::vector<int> qm(e->Rp, 0);
@@ -1647,7 +1684,7 @@ int emc_set_scale(emc_engine *e, const double *scale) {
 int emc_clear_merge(emc_engine *e) {
     if (!e || !e->size) return fail(EMC_E_STATE, "emc_clear_merge: model not set");
     HIP_TRY(hipSetDevice(e->device));
-    const size_t vol = (size_t)e->size * e->size * e->size;
+    const size_t vol = (size_t)e->size * e->size * e->size * e->num_modes;
     emc::launch_fill(e->stream, e->model2.p, vol, 0.);
     emc::launch_fill(e->stream, e->weight.p, vol, 0.);
     HIP_TRY(hipGetLastError());
@@ -1665,6 +1702,8 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     if (g32) e->tomo_tile_mode = 2;  // the FP32 posterior feeds the frame-partitioned tomogram
     if (e->refine && e->subset_mode && e->subset_ready && !mp) return maximize_subset(e, opts, stats);
     e->subset_ran = false;
+    if (mp && e->num_modes > 1)
+        return fail(EMC_E_STATE, "emc_maximize: the mixed-precision path takes one mode");
     if (mp) {
         for (uint8_t v : e->h_dense)
             if (!v) return fail(EMC_E_STATE, "emc_maximize: the mixed-precision path needs dense frames");
@@ -1674,8 +1713,8 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     if ((rc = ensure_work(e))) return rc;
 
     hipStream_t s = e->stream;
-    const int N = e->num_data, Rn = e->num_rot, P = e->num_pix, C = e->C;
-    const size_t vol = (size_t)e->size * e->size * e->size;
+    const int N = e->num_data, Rn = e->num_rot, P = e->num_pix, C = e->C, M = e->num_modes;
+    const size_t vol = (size_t)e->size * e->size * e->size * M;  // every mode's volume
     const int ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
     const int use_scale = opts->need_scaling && (opts->iteration > 1 || opts->known_scale);
     const int want_psum_d = opts->need_scaling && opts->update_scale;
@@ -1711,10 +1750,10 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
                                   e->lw_hi.p, e->lw_lo.p, e->upart.p, e->ubad.p);
         else if (g32)
             emc::launch_expand(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
-                               P, r0, Cc, kPixPerGroup, e->logw32.p, e->upart.p, e->ubad.p);
+                               P, r0, Cc, kPixPerGroup, e->logw32.p, e->upart.p, e->ubad.p, nullptr, M);
         else
             emc::launch_expand(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
-                               P, r0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p);
+                               P, r0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p, nullptr, M);
         emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, r0, Rn, e->u.p, e->rbad.p);
         HIP_TRY(hipEventRecord(e->ev[1], s));
         if (mp) {
@@ -1745,7 +1784,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         ms_lik += b;
     }
     HIP_TRY(hipEventRecord(e->ev[3], s));
-    emc::launch_rescale(s, e->quats.p, e->u.p, Rn, e->mean_count, e->scalars.p);
+    emc::launch_rescale(s, e->quats.p, e->u.p, Rn, e->mean_count, e->scalars.p, M);
     HIP_TRY(hipEventRecord(e->ev[4], s));
     emc::launch_frame_max(s, e->R.p, Rn, N, e->logwr.p, e->u.p, e->scale.p, use_scale,
                           e->scalars.p, e->blacklist.p, e->any_dense ? e->dense.p : nullptr,
@@ -1757,7 +1796,11 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     HIP_TRY(hipEventRecord(e->ev[6], s));
     emc::launch_posterior(s, e->R.p, Rn, N, e->logwr.p, e->u.p, e->beta.p, e->sum_fact.p,
                           e->blacklist.p, e->max_exp.p, e->rmax.p, e->p_norm.p, e->scalars.p,
-                          want_psum_d, e->likelihood.p, e->info.p, e->psum_d.p);
+                          want_psum_d, e->likelihood.p, e->info.p, e->psum_d.p, M);
+    if (M > 1) {
+        if ((rc = e->quat_norm.alloc((size_t)N * M))) return rc;
+        emc::launch_occupancy(s, e->R.p, Rn, N, M, e->quat_norm.p);
+    }
     emc::launch_psum_r(s, e->R.p, Rn, N, kFramesPerPsumPart, e->scale.p, opts->need_scaling,
                        e->blacklist.p, e->psum_part.p, e->psum_r.p);
     HIP_TRY(hipEventRecord(e->ev[7], s));
@@ -1805,7 +1848,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
                 for (int r0 = 0; r0 < e->Rp; r0 += C) {
                     const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
                     emc::launch_merge(s, e->rots.p, r0, Cc, Rn, nullptr, e->psum_r.p, 1, e->qvals.p,
-                                      e->corr.p, e->mask.p, P, e->size, nullptr, e->weight_all.p, 0, 1);
+                                      e->corr.p, e->mask.p, P, e->size, nullptr, e->weight_all.p, 0, 1, M);
                 }
             }
             HIP_TRY(hipGetLastError());
@@ -1863,7 +1906,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         } else {
             emc::launch_merge(s, e->rots.p, r0, Cc, Rn, e->logw.p, e->psum_r.p, 0, e->qvals.p,
                               e->corr.p, e->mask.p, P, e->size, e->model2.p, e->weight.p, 1,
-                              cached_weight ? 0 : 1);
+                              cached_weight ? 0 : 1, M);
         }
         HIP_TRY(hipEventRecord(e->ev[8], s));
         HIP_TRY(hipGetLastError());
@@ -1903,13 +1946,13 @@ int emc_update_model(emc_engine *e, double alpha, double *rms_change) {
     if ((rc = e->model_part.alloc(kModelParts))) return rc;
     if ((rc = e->scalars.alloc(8))) return rc;
     emc::launch_update_model(e->stream, e->model1.p, e->model2.p, e->weight.p, e->size, alpha,
-                             e->model_part.p, kModelParts, e->scalars.p + 3);
+                             e->model_part.p, kModelParts, e->scalars.p + 3, e->num_modes);
     HIP_TRY(hipGetLastError());
     double change = 0.;
     HIP_TRY(hipMemcpyAsync(&change, e->scalars.p + 3, sizeof change, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     const long vol = e->size * e->size * e->size;
-    if (rms_change) *rms_change = std::sqrt(change / 1 / vol);
+    if (rms_change) *rms_change = std::sqrt(change / e->num_modes / vol);  // recon.pyx:149
     return 0;
 }
 
@@ -1978,9 +2021,14 @@ int emc_get_field(emc_engine *e, int field, void *dst, size_t count) {
         case EMC_FIELD_PSUM_D: src = e->psum_d.p; n = N; break;
         case EMC_FIELD_PSUM_R: src = e->psum_r.p; n = Rn; break;
         case EMC_FIELD_SCALE: src = e->scale.p; n = N; needs_results = false; break;
-        case EMC_FIELD_MODEL1: src = e->model1.p; n = vol; needs_results = false; break;
-        case EMC_FIELD_MODEL2: src = e->model2.p; n = vol; needs_results = false; break;
-        case EMC_FIELD_INTER_WEIGHT: src = e->weight.p; n = vol; needs_results = false; break;
+        case EMC_FIELD_MODEL1: src = e->model1.p; n = vol * e->num_modes; needs_results = false; break;
+        case EMC_FIELD_MODEL2: src = e->model2.p; n = vol * e->num_modes; needs_results = false; break;
+        case EMC_FIELD_INTER_WEIGHT: src = e->weight.p; n = vol * e->num_modes; needs_results = false; break;
+        case EMC_FIELD_QUAT_NORM:
+            if (e->num_modes < 2) return fail(EMC_E_STATE, "emc_get_field: occupancies need num_modes > 1");
+            src = e->quat_norm.p;
+            n = N * e->num_modes;
+            break;
         case EMC_FIELD_PROB: {
             if (!e->have_results) return fail(EMC_E_STATE, "emc_get_field: no iteration has run");
             if (count != N * Rn) return fail(EMC_E_ARG, "emc_get_field: count %zu != %zu", count, N * Rn);

@@ -11,11 +11,11 @@ namespace emc {
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad,
-                   const int *slot_rot = nullptr);
+                   const int *slot_rot = nullptr, int modes = 1);
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad,
-                   const int *slot_rot = nullptr);
+                   const int *slot_rot = nullptr, int modes = 1);
 void launch_to_f32(hipStream_t s, const double *P, int num_data, int r0, int C, float *out);
 void launch_likelihood(hipStream_t s, const float *logw, int C, int num_pix, const long *off,
                        const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
@@ -38,7 +38,7 @@ void launch_tomogram_parts_f32(hipStream_t s, const float *P32, int num_data, in
 void launch_u_reduce(hipStream_t s, const double *upart, const int *ubad, int ngroups, int C,
                      int r0, int num_rot, double *u, int *rbad);
 void launch_rescale(hipStream_t s, const double *quats, const double *u, int num_rot,
-                    double mean_count, double *scalars);
+                    double mean_count, double *scalars, int modes = 1);
 void launch_likelihood(hipStream_t s, const double *logw, int C, int num_pix, const long *off,
                        const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
                        double *G, int r0, int tiles, int unroll, int xcd_order = 0);
@@ -54,7 +54,9 @@ void launch_posterior(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *logwr, const double *u, const double *beta,
                       const double *sum_fact, const uint8_t *blacklist, const double *max_exp,
                       const int *rmax, const double *p_norm, const double *scalars,
-                      int want_psum_d, double *likelihood, double *info, double *psum_d);
+                      int want_psum_d, double *likelihood, double *info, double *psum_d, int modes = 1);
+void launch_occupancy(hipStream_t s, const double *P, int num_rot, int num_data, int modes,
+                      double *quat_norm);
 void launch_psum_r(hipStream_t s, const double *P, int num_rot, int num_data,
                    int frames_per_part, const double *scale, int need_scaling,
                    const uint8_t *blacklist, double *part, double *psum_r);
@@ -77,7 +79,7 @@ void launch_tomogram_reduce_rmajor(hipStream_t s, const double *partial, int npa
 void launch_brick_items(hipStream_t s, const double *rots, int r0, int Cc, int num_rot,
                         const double *qvals, const uint8_t *mask, int num_pix, long size, int npatch,
                         int nbx, int pass, int *counts, const long *off, int *fill, int *list,
-                        const int *slot_rot);
+                        const int *slot_rot, int modes = 1);
 void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const int *list,
                          const double *rots, int r0, int num_rot, const double *view,
                          const double *psum_r, int all_on, const double *qvals, const double *corr,
@@ -86,7 +88,7 @@ void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const 
 void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot, const double *view,
                   const double *psum_r, int all_on, const double *qvals, const double *corr,
                   const uint8_t *mask, int num_pix, long size, double *model2, double *weight,
-                  int do_model, int do_weight);
+                  int do_model, int do_weight, int modes = 1);
 void launch_count_unmerged(hipStream_t s, const double *psum_r, int num_rot, int *count);
 void launch_slice_gen(hipStream_t s, const double *rot, const double *model, long size,
                       const double *qvals, const double *corr, const int *perm, int num_pix,
@@ -100,7 +102,7 @@ void launch_scale(hipStream_t s, const int *fcounts, const double *psum_d, const
                   int num_data, double *scale);
 void launch_fill(hipStream_t s, double *a, size_t n, double v);
 void launch_update_model(hipStream_t s, double *model1, double *model2, const double *w, long size,
-                         double alpha, double *part, int nparts, double *out_change);
+                         double alpha, double *part, int nparts, double *out_change, int modes = 1);
 void launch_normalize_scale(hipStream_t s, double *scale, const uint8_t *blacklist, int num_data,
                             double *model1, long vol, double *mean_out);
 

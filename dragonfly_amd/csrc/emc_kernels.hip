@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
     const double *__restrict__ qvals, const double *__restrict__ corr,
     const uint8_t *__restrict__ mask, int num_pix, int r0, int C, int pix_per_group,
     T *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad,
-    const int *__restrict__ slot_rot) {
+    const int *__restrict__ slot_rot, int modes) {
     __shared__ double s_rot[kRTile * 9];
     __shared__ double s_tile[64][kRTile + 1];
     __shared__ double s_u[kWavesPerBlock][kRTile];
@@ -77,7 +77,10 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
                          q2 = qvals[3 * (size_t)t + 2], cr = corr[t];
             for (int k = 0; k < kRTile / kWavesPerBlock; ++k) {
                 const int rr = wave + kWavesPerBlock * k;
-                s_tile[lane][rr] = s_pad[rr] ? 1. : slice_value(model, size, &s_rot[rr * 9], q0, q1, q2, cr, center);
+                // combined orientation index r = rotation * modes + mode (max_prob.c:80-84)
+                const double *mdl =
+                    modes > 1 ? model + (size_t)((r0 + c0 + rr) % modes) * (size_t)(size * size * size) : model;
+                s_tile[lane][rr] = s_pad[rr] ? 1. : slice_value(mdl, size, &s_rot[rr * 9], q0, q1, q2, cr, center);
             }
         }
         __syncthreads();
@@ -125,7 +128,7 @@ __global__ void u_reduce_kernel(const double *__restrict__ upart, const int *__r
 // rescale = mean_count / sum_r w_r u_r (max_prob.c:40, :59); one block, fixed order.
 __global__ __launch_bounds__(kBlock) void rescale_kernel(const double *__restrict__ quats,
                                                          const double *__restrict__ u, int num_rot,
-                                                         double mean_count,
+                                                         double mean_count, double modes,
                                                          double *__restrict__ scalars) {
     __shared__ double s[kBlock];
     double acc = 0.;
@@ -136,7 +139,8 @@ __global__ __launch_bounds__(kBlock) void rescale_kernel(const double *__restric
         if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) scalars[0] = mean_count / s[0] * 1;
+    // rescale = mean_count / total * num_modes (max_prob.c:59)
+    if (threadIdx.x == 0) scalars[0] = mean_count / s[0] * modes;
 }
 
 // ---- likelihood: G_dr = sum_t K_dt log W_rt over one frame's photons (mask < 1) ------------
@@ -349,14 +353,15 @@ __global__ __launch_bounds__(kBlock) void frame_norm_kernel(
 
 // Posterior and per-frame statistics (calc_psum_r, max_update.c:95-130):
 // P = exp(beta (R - M)) / p_norm on the kept set, 0 elsewhere (written in place);
-// likelihood += P (R - sum_fact); psum_d += P u_r rescale; info += P log(P / w_r) for P >= 1e-6.
+// likelihood += P (R - sum_fact); psum_d += P u_r rescale; info += P log(P / w_r * num_modes)
+// for P >= 1e-6.
 __global__ __launch_bounds__(kBlock) void posterior_kernel(
     double *__restrict__ R, size_t tsR, int num_rot, const double *__restrict__ logwr,
     const double *__restrict__ u, const double *__restrict__ beta,
     const double *__restrict__ sum_fact, const uint8_t *__restrict__ blacklist,
     const double *__restrict__ max_exp, const int *__restrict__ rmax,
     const double *__restrict__ p_norm, const double *__restrict__ scalars, int want_psum_d,
-    double *__restrict__ likelihood, double *__restrict__ info, double *__restrict__ psum_d) {
+    double log_modes, double *__restrict__ likelihood, double *__restrict__ info, double *__restrict__ psum_d) {
     __shared__ double s[kBlock];
     const int d = blockIdx.x;
     double *row = R + (size_t)d * kRTile;
@@ -382,9 +387,10 @@ __global__ __launch_bounds__(kBlock) void posterior_kernel(
             prob = exp(e) / pn;
             lk += prob * (v - sf);
             ps += prob * u[r] * rescale;
-            // log(P / w_r) (max_update.c:127-130) taken as e - log p_norm - log w_r: the same
-            // value to rounding, without a logarithm per pair
-            if (!(prob < kProbMin)) inf += prob * ((e - log_pn) - logwr[r]);
+            // log(P / w_r * num_modes) (max_update.c:127-130) taken as e - log p_norm - log w_r
+            // + log num_modes: the same value to rounding, without a logarithm per pair (one
+            // mode adds exactly 0)
+            if (!(prob < kProbMin)) inf += prob * ((e - log_pn) - logwr[r] + log_modes);
         }
         x = prob;
     }
@@ -395,6 +401,26 @@ __global__ __launch_bounds__(kBlock) void posterior_kernel(
         likelihood[d] = tl;
         info[d] = ti;
         if (want_psum_d) psum_d[d] = tp;
+    }
+}
+
+// Occupancy of every frame in every mode (max_update.c:122-124): quat_norm[d][m] = sum of the
+// posterior P_dr >= 1e-6 over the combined orientations r = rotation * modes + m. One block
+// per frame; P is the tile-major posterior (0 where not kept).
+__global__ __launch_bounds__(kBlock) void occupancy_kernel(const double *__restrict__ P, size_t tsR,
+                                                           int num_rot, int modes,
+                                                           double *__restrict__ quat_norm) {
+    __shared__ double s[kBlock];
+    const int d = blockIdx.x;
+    const double *row = P + (size_t)d * kRTile;
+    for (int m = 0; m < modes; ++m) {
+        double acc = 0.;
+        for (int r = m + modes * (int)threadIdx.x; r < num_rot; r += modes * kBlock) {
+            const double v = row[(size_t)(r >> 6) * tsR + (r & 63)];
+            if (!(v < kProbMin)) acc += v;
+        }
+        const double tot = block_sum(acc, s);
+        if (threadIdx.x == 0) quat_norm[(size_t)d * modes + m] = tot;
     }
 }
 
@@ -712,7 +738,7 @@ __global__ __launch_bounds__(kBlock) void merge_box_kernel(
     const double *__restrict__ rots, int r0, int C, int num_rot, const double *__restrict__ view,
     const double *__restrict__ psum_r, int all_on, const double *__restrict__ qvals,
     const double *__restrict__ corr, const uint8_t *__restrict__ mask, int num_pix, long size,
-    double *__restrict__ model2, double *__restrict__ weight) {
+    double *__restrict__ model2_all, double *__restrict__ weight_all, int modes) {
     constexpr int kPer = (kModel ? kBoxCap : 0) + (kWeight ? kBoxCap : 0);
     __shared__ double s_rot[kRTile * 9];
     __shared__ int s_on[kRTile];
@@ -741,6 +767,10 @@ __global__ __launch_bounds__(kBlock) void merge_box_kernel(
     const long s2 = size * size;
     for (int cc = 0; cc < kRTile; ++cc) {
         if (!s_on[cc]) continue;
+        // the volume of this orientation's mode (slice_merge3d, model.c:175-176)
+        const size_t moff = modes > 1 ? (size_t)((r0 + c0 + cc) % modes) * (size_t)(size * s2) : 0;
+        double *model2 = kModel ? model2_all + moff : nullptr;
+        double *weight = kWeight ? weight_all + moff : nullptr;
         Cell cell;
         const bool ok = pix_ok && locate(&s_rot[cc * 9], q0, q1, q2, center, size, cell);
         double w = 0.;
@@ -993,19 +1023,19 @@ static inline int round_up_int(int a, int b) { return (a + b - 1) / b * b; }
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad,
-                   const int *slot_rot) {
+                   const int *slot_rot, int modes) {
     dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
     expand_kernel<double, kRTile><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0, C,
-                                                          pix_per_group, logw, upart, ubad, slot_rot);
+                                                          pix_per_group, logw, upart, ubad, slot_rot, modes);
 }
 
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad,
-                   const int *slot_rot) {
+                   const int *slot_rot, int modes) {
     dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
     expand_kernel<float, 2 * kRTile><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0,
-                                                             C, pix_per_group, logw, upart, ubad, slot_rot);
+                                                             C, pix_per_group, logw, upart, ubad, slot_rot, modes);
 }
 
 // FP32 copy of one orientation chunk of the tile-major posterior in 128-wide tiles:
@@ -1119,8 +1149,8 @@ void launch_u_reduce(hipStream_t s, const double *upart, const int *ubad, int ng
 }
 
 void launch_rescale(hipStream_t s, const double *quats, const double *u, int num_rot,
-                    double mean_count, double *scalars) {
-    rescale_kernel<<<1, kBlock, 0, s>>>(quats, u, num_rot, mean_count, scalars);
+                    double mean_count, double *scalars, int modes) {
+    rescale_kernel<<<1, kBlock, 0, s>>>(quats, u, num_rot, mean_count, (double)modes, scalars);
 }
 
 template <typename T>
@@ -1224,10 +1254,15 @@ void launch_posterior(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *logwr, const double *u, const double *beta,
                       const double *sum_fact, const uint8_t *blacklist, const double *max_exp,
                       const int *rmax, const double *p_norm, const double *scalars,
-                      int want_psum_d, double *likelihood, double *info, double *psum_d) {
+                      int want_psum_d, double *likelihood, double *info, double *psum_d, int modes) {
     posterior_kernel<<<num_data, kBlock, 0, s>>>(R, (size_t)num_data * kRTile, num_rot, logwr, u, beta, sum_fact,
                                                  blacklist, max_exp, rmax, p_norm, scalars,
-                                                 want_psum_d, likelihood, info, psum_d);
+                                                 want_psum_d, log((double)modes), likelihood, info, psum_d);
+}
+
+void launch_occupancy(hipStream_t s, const double *P, int num_rot, int num_data, int modes,
+                      double *quat_norm) {
+    occupancy_kernel<<<num_data, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, num_rot, modes, quat_norm);
 }
 
 void launch_psum_r(hipStream_t s, const double *P, int num_rot, int num_data,
@@ -1315,11 +1350,11 @@ void launch_tomogram_reduce(hipStream_t s, const double *partial, int nparts, in
 void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot, const double *view,
                   const double *psum_r, int all_on, const double *qvals, const double *corr,
                   const uint8_t *mask, int num_pix, long size, double *model2, double *weight,
-                  int do_model, int do_weight) {
+                  int do_model, int do_weight, int modes) {
     dim3 grid(C / kRTile, cdiv(num_pix, 64 * kWavesPerBlock));
 #define EMC_MERGE(M, W)                                                                          \
     merge_box_kernel<M, W><<<grid, kBlock, 0, s>>>(rots, r0, C, num_rot, view, psum_r, all_on, qvals, \
-                                                   corr, mask, num_pix, size, model2, weight)
+                                                   corr, mask, num_pix, size, model2, weight, modes)
     if (do_model && do_weight) EMC_MERGE(true, true);
     else if (do_model) EMC_MERGE(true, false);
     else if (do_weight) EMC_MERGE(false, true);
@@ -1358,11 +1393,17 @@ void launch_fill(hipStream_t s, double *a, size_t n, double v) {
 }
 
 void launch_update_model(hipStream_t s, double *model1, double *model2, const double *w, long size,
-                         double alpha, double *part, int nparts, double *out_change) {
+                         double alpha, double *part, int nparts, double *out_change, int modes) {
     const long vol = size * size * size;
-    friedel_offplane_kernel<<<grid_cap(cdiv(vol / 2 + 1, 256)), 256, 0, s>>>(model2, w, (int)size);
-    friedel_plane_kernel<<<cdiv((size_t)size * size, 256), 256, 0, s>>>(model2, w, (int)size);
-    model_divide_blend_kernel<<<nparts, kBlock, 0, s>>>(model1, model2, w, vol, alpha, part);
+    // Friedel symmetry per mode, then divide, change and blend over every mode's volume
+    // (recon.pyx:125-149)
+    for (int m = 0; m < modes; ++m) {
+        double *m2 = model2 + (size_t)m * vol;
+        const double *mw = w + (size_t)m * vol;
+        friedel_offplane_kernel<<<grid_cap(cdiv(vol / 2 + 1, 256)), 256, 0, s>>>(m2, mw, (int)size);
+        friedel_plane_kernel<<<cdiv((size_t)size * size, 256), 256, 0, s>>>(m2, mw, (int)size);
+    }
+    model_divide_blend_kernel<<<nparts, kBlock, 0, s>>>(model1, model2, w, vol * modes, alpha, part);
     sum_parts_kernel<<<1, kBlock, 0, s>>>(part, nparts, out_change);
 }
 

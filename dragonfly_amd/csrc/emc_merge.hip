@@ -14,6 +14,10 @@
 // Values are those of the reference's merge (same cell, same corner weights, w = view / corr
 // for mask <= 1 pixels of orientations with psum_r > 0); only the summation order differs.
 // The view is orientation-major: view[c][t], c the chunk-local orientation.
+//
+// With several modes (combined orientation r = rotation * modes + mode, max_prob.c:80-84) each
+// mode's volume has its own bricks: brick id = mode * nbx^3 + brick, and a block adds into
+// the mode's volume at mode * size^3.
 #include <climits>
 
 #include "emc_common.h"
@@ -43,8 +47,9 @@ __global__ __launch_bounds__(kBlock) void brick_items_kernel(
     const double *__restrict__ rots, int r0, int Cc, int num_rot, const double *__restrict__ qvals,
     const uint8_t *__restrict__ mask, int num_pix, long size, int npatch, int nbx, int pass,
     int *__restrict__ counts, const long *__restrict__ off, int *__restrict__ fill,
-    int *__restrict__ list, const int *__restrict__ slot_rot) {
+    int *__restrict__ list, const int *__restrict__ slot_rot, int modes) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nb3 = nbx * nbx * nbx;
     const long items = (long)Cc * npatch, stride = (long)gridDim.x * kWavesPerBlock;
     for (long item = (long)blockIdx.x * kWavesPerBlock + wave; item < items; item += stride) {
         const int c = (int)(item / npatch), p = (int)(item % npatch);
@@ -68,8 +73,9 @@ __global__ __launch_bounds__(kBlock) void brick_items_kernel(
         const int bz0 = zmin / kBrick, bz1 = (zmax + 1) / kBrick;
         const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1;
         const int nb = (bx1 - bx0 + 1) * ny * nz;
+        const int mbase = modes > 1 ? (r % modes) * nb3 : 0;
         for (int i = lane; i < nb; i += 64) {
-            const int b = ((bx0 + i / (ny * nz)) * nbx + by0 + (i / nz) % ny) * nbx + bz0 + i % nz;
+            const int b = mbase + ((bx0 + i / (ny * nz)) * nbx + by0 + (i / nz) % ny) * nbx + bz0 + i % nz;
             if (pass == 0) {
                 atomicAdd(&counts[b], 1);
             } else {
@@ -132,7 +138,8 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
     __shared__ double q_w[kWavesPerBlock][kRing];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int4 u = units[blockIdx.x];
-    const int b = u.x;
+    const int nb3 = nbx * nbx * nbx, b = u.x % nb3;
+    model2 += (size_t)(u.x / nb3) * (size_t)(size * size * size);  // the brick's mode volume
     const int X0 = (b / (nbx * nbx)) * kBrick, Y0 = ((b / nbx) % nbx) * kBrick, Z0 = (b % nbx) * kBrick;
     for (int i = tid; i < kVox; i += kBlock) s_m[i] = (A)0;
     __syncthreads();
@@ -191,7 +198,10 @@ __global__ __launch_bounds__(kBlock) void merge_brick_kernel(
     __shared__ double s_w[kWeight ? kVox : 1];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int4 u = units[blockIdx.x];
-    const int b = u.x;
+    const int nb3 = nbx * nbx * nbx, b = u.x % nb3;
+    const size_t moff = (size_t)(u.x / nb3) * (size_t)(size * size * size);  // the brick's mode volume
+    if (kModel) model2 += moff;
+    if (kWeight) weight += moff;
     const int X0 = (b / (nbx * nbx)) * kBrick, Y0 = ((b / nbx) % nbx) * kBrick, Z0 = (b % nbx) * kBrick;
     for (int i = tid; i < kVox; i += kBlock) {
         if (kModel) s_m[i] = 0.;
@@ -245,11 +255,11 @@ __global__ __launch_bounds__(kBlock) void merge_brick_kernel(
 void launch_brick_items(hipStream_t s, const double *rots, int r0, int Cc, int num_rot,
                         const double *qvals, const uint8_t *mask, int num_pix, long size, int npatch,
                         int nbx, int pass, int *counts, const long *off, int *fill, int *list,
-                        const int *slot_rot) {
+                        const int *slot_rot, int modes) {
     const long items = (long)Cc * npatch;
     const unsigned grid = stride_grid((items + kWavesPerBlock - 1) / kWavesPerBlock);
     brick_items_kernel<<<grid, kBlock, 0, s>>>(rots, r0, Cc, num_rot, qvals, mask, num_pix, size, npatch,
-                                               nbx, pass, counts, off, fill, list, slot_rot);
+                                               nbx, pass, counts, off, fill, list, slot_rot, modes);
 }
 
 void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const int *list,

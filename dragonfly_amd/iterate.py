@@ -130,12 +130,13 @@ class Iterate:
             self.params.start_iter = last_iter + 1
             model_file = op.join(param.output_folder, 'output_%.3d.npz' % last_iter)
             scale_file = model_file
-        model = Model(self.calculate_size(max(d.qmax() for d in self.dets)), 1)
+        model = Model(self.calculate_size(max(d.qmax() for d in self.dets)), self.params.num_modes)
         model_mean = self.mean_count[0] / (self.dets[0].raw_mask == 0).sum() * 2.
         if model_file.endswith('.npz') and op.isfile(model_file):
             with np.load(model_file) as f:
                 model.allocate('', model_mean)
-                model.model1 = np.ascontiguousarray(f['intens'].reshape((model.size,) * 3))
+                shape = (model.size,) * 3 if model.num_modes == 1 else (model.num_modes,) + (model.size,) * 3
+                model.model1 = np.ascontiguousarray(f['intens'].reshape(shape))
         else:
             model.allocate(model_file, model_mean, fixed_seed=bool(param.fixed_seed))
         self.set_model(model)

@@ -1,4 +1,4 @@
-"""Model: the 3D intensity volume (mirror of dragonfly/model.pyx:18-313, 3D, one mode)."""
+"""Model: the 3D intensity volumes (mirror of dragonfly/model.pyx:18-313, 3D)."""
 import os
 
 import numpy as np
@@ -6,13 +6,14 @@ import numpy as np
 
 class Model:
     """model1 (current estimate), model2 (merged numerator) and inter_weight, each
-    (size, size, size) float64, index x*size*size + y*size + z as the reference."""
+    (size, size, size) float64 for one mode and (num_modes, size, size, size) for several
+    (model.pyx:58), index x*size*size + y*size + z within a mode as the reference."""
 
     def __init__(self, size=0, num_modes=1, model_type='3d'):
         if model_type.lower() != '3d':
             raise NotImplementedError('only 3d models are implemented')
-        if num_modes != 1:
-            raise NotImplementedError('num_modes > 1 is not implemented')
+        if num_modes < 1:
+            raise ValueError('num_modes must be at least 1')
         self.mtype = 'MODEL_3D'
         self.num_modes = num_modes
         self.size = int(size)
@@ -64,7 +65,7 @@ class Model:
     def allocate(self, fname='', model_mean=1., rank=0, fixed_seed=False):
         """Load model1 from a raw float64 file, or draw a random start (model.pyx:43-103):
         uniform [0, model_mean), numpy's legacy generator seeded 0x5EED when fixed_seed."""
-        shape = (self.size,) * 3
+        shape = (self.size,) * 3 if self.num_modes == 1 else (self.num_modes,) + (self.size,) * 3
         self.model2 = np.zeros(shape)
         self.inter_weight = np.zeros(shape)
         if fname and os.path.isfile(fname):

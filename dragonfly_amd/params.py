@@ -66,8 +66,13 @@ class EMCParams:
         self.num_modes = config.getint(section_name, 'num_modes', fallback=1)
         self.nonrot_modes = config.getint(section_name, 'num_nonrot_modes', fallback=0)
         self.rot_per_mode = config.getint(section_name, 'num_rot', fallback=0)
-        if self.num_modes != 1 or self.nonrot_modes != 0:
-            raise NotImplementedError('num_modes > 1 is not implemented on the GPU engine')
+        if self.num_modes < 1:
+            raise ValueError('num_modes must be at least 1')
+        if self.nonrot_modes != 0:
+            # the Cython engine's Model holds num_modes volumes (iterate.pyx:164), so a
+            # non-rotating mode (index num_modes and up, max_prob.c:82-84) has no volume
+            raise NotImplementedError('num_nonrot_modes > 0 is not supported (the model has no '
+                                      'volume for a non-rotating mode)')
         beta_schedule = config.get(section_name, 'beta_schedule', fallback='1. 100').split()
         self.beta_jump = float(beta_schedule[0])
         self.beta_period = int(beta_schedule[1])

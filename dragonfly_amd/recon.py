@@ -68,6 +68,7 @@ class EMCRecon:
         eng.set_precision(self.precision)
         det = itr.dets[0]
         eng.set_detector(det.qvals, det.corr, det.raw_mask)
+        eng.set_modes(par.num_modes)
         eng.set_quats(itr.quat.quats)
         eng.set_model(itr.model.model1)
         if getattr(itr, 'dense', False):
@@ -144,7 +145,7 @@ class EMCRecon:
             fp.write('System size:\n')
             fp.write('\tnum_rot = %d\n\tnum_pix = %d/%d\n\t' % (itr.quat.num_rot, (mask == 0).sum(), mask.size))
             s = itr.model.size
-            fp.write('system_volume = %d X %ld X %ld X %ld\n\n' % (1, s, s, s))
+            fp.write('system_volume = %d X %ld X %ld X %ld\n\n' % (itr.model.num_modes, s, s, s))
             fp.write('Reconstruction parameters:\n')
             fp.write('\tnum_threads = %d\n\tnum_proc = %d\n\talpha = %.6f\n\tbeta = %.6f\n\tneed_scaling = %s' % (
                 self.num_threads, par.num_proc, par.alpha, itr.beta_start[0],
@@ -162,9 +163,13 @@ class EMCRecon:
     def _output_dict(self):
         itr = self.iter
         itr.model.sync()
+        m1, w = itr.model.model1, itr.model.inter_weight
+        if itr.model.num_modes == 1:
+            m1, w = m1[None], w[None]
         out = {'orientations': self.rmax, 'mutual_info': self.info, 'likelihood': self.likelihood,
-               'intens': itr.model.model1[None], 'inter_weight': itr.model.inter_weight[None],
-               'quaternions': itr.quat.quats}
+               'intens': m1, 'inter_weight': w, 'quaternions': itr.quat.quats}
+        if itr.model.num_modes > 1:
+            out['occupancies'] = self.quat_norm
         if itr.params.need_scaling != 0:
             out['scale'] = itr.scale
         for k, v in vars(itr.params).items():
@@ -200,7 +205,8 @@ class EMCRecon:
     def save_initial_model(self):
         par = self.iter.params
         os.makedirs(par.output_folder, exist_ok=True)
-        out = {'intens': self.iter.model.model1[None]}
+        m1 = self.iter.model.model1
+        out = {'intens': m1[None] if self.iter.model.num_modes == 1 else m1}
         if par.need_scaling != 0:
             out['scale'] = self.iter.scale
         np.savez(os.path.join(par.output_folder, 'output_000.npz'), **out)
@@ -237,6 +243,12 @@ class EMCRecon:
     @property
     def psum_r(self):
         return self.engine.get('psum_r')
+
+    @property
+    def quat_norm(self):
+        """(num_data, num_modes) occupancy of each frame in each mode (max_update.c:122-124,
+        recon.pyx:259-260); num_modes > 1 only."""
+        return self.engine.get('quat_norm')
 
     @property
     def prob_dense(self):

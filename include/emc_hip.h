@@ -34,6 +34,8 @@ extern "C" {
 #define EMC_E_HIP (-3)      /* HIP runtime error (allocation, launch, copy) */
 #define EMC_E_NOMEM (-4)    /* device memory budget exceeded */
 
+#define EMC_MAX_MODES 64    /* largest num_modes emc_set_modes accepts */
+
 typedef struct emc_engine emc_engine;
 
 /* Options of one iteration; the fields of struct params that maximize() reads
@@ -68,12 +70,14 @@ enum emc_field {
     EMC_FIELD_INFO,           /* f64[num_data]  info         (max_update.c:128) recon.pyx:389 */
     EMC_FIELD_PSUM_D,         /* f64[num_data]  psum_d       (max_update.c:113) recon.pyx:358 */
     EMC_FIELD_SCALE,          /* f64[num_data]  iter->scale  (max_scale.c:15) */
-    EMC_FIELD_MODEL1,         /* f64[size^3]    mod->model1 */
-    EMC_FIELD_MODEL2,         /* f64[size^3]    mod->model2  (max_prob.c:281) */
-    EMC_FIELD_INTER_WEIGHT,   /* f64[size^3]    mod->inter_weight (max_prob.c:282) */
+    EMC_FIELD_MODEL1,         /* f64[num_modes][size^3] mod->model1 */
+    EMC_FIELD_MODEL2,         /* f64[num_modes][size^3] mod->model2 (max_prob.c:281) */
+    EMC_FIELD_INTER_WEIGHT,   /* f64[num_modes][size^3] mod->inter_weight (max_prob.c:282) */
     EMC_FIELD_PSUM_R,         /* f64[num_rot]   psum_r per orientation (max_update.c:109,116) */
     EMC_FIELD_PROB,           /* f64[num_data][num_rot] posterior P_dr, 0 where not kept
                                  (dense form of max_data.prob/place_prob, recon.pyx:409-415) */
+    EMC_FIELD_QUAT_NORM,      /* f64[num_data][num_modes] occupancy of each frame in each mode,
+                                 num_modes > 1 only (max_update.c:122-124, recon.pyx:259-260) */
     EMC_FIELD_COUNT
 };
 
@@ -98,9 +102,18 @@ int emc_voronoi_subset(int num_coarse, const double *qcoarse, int num_fine,
  *           filled by dragonfly/detector.pyx:31-74 / setters :161-206)               */
 int emc_set_detector(emc_engine *e, int num_pix, const double *qvals, const double *corr,
                      const uint8_t *raw_mask);
+/* modes: params.num_modes (dragonfly/params.pyx:104) and the model's mode count
+ *        (Model(size, num_modes), iterate.pyx:164). The engine's orientations become the
+ *        combined ones, r = rotation * num_modes + mode (max_prob.c:80-84): num_rot,
+ *        rmax, u, psum_r and the posterior's columns count num_rot * num_modes. Default 1.
+ *        Setting a new count re-expands the quaternions, switches refinement off and drops
+ *        the model (set it again with num_modes volumes). Non-rotating modes
+ *        (num_nonrot_modes) are not taken: the Cython engine's model holds num_modes
+ *        volumes only. The mixed-precision and refinement paths take one mode.          */
+int emc_set_modes(emc_engine *e, int num_modes);
 /* orientations: struct quaternion quats[num_rot][5] (dragonfly/src/quaternion.h:11-15) */
 int emc_set_quats(emc_engine *e, int num_rot, const double *quats);
-/* model: struct model model1[size^3], one mode (dragonfly/src/model.h:15-20,
+/* model: struct model model1[num_modes][size^3] (dragonfly/src/model.h:15-20,
  *        Model.allocate dragonfly/model.pyx:43-103)                                   */
 int emc_set_model(emc_engine *e, long size, const double *model1);
 /* sparse photon frames: struct dataset ones, multi, place_ones, place_multi, count_multi

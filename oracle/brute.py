@@ -78,10 +78,15 @@ def dense_counts(frames, num_pix):
 def iterate(det, quats, model1, frames, fparams, flags, refine=None):
     """One iteration; returns the same fields as oracle.core.maximize. refine (optional):
     dict quat_mapping, rel_off, rel_quat -- frame d only takes orientations whose coarse cell
-    is in its set (legacy src/max_emc.c:264-274)."""
+    is in its set (legacy src/max_emc.c:264-274). model1 (M, S, S, S) runs M modes over the
+    combined orientations r = rotation * M + mode (max_prob.c:80-84) and adds quat_norm."""
     qvals, corr, mask = det['qvals'], det['corr'], det['mask']
+    nm = model1.shape[0] if model1.ndim == 4 else 1
+    vols = model1 if model1.ndim == 4 else model1[None]
+    quats = np.repeat(quats, nm, axis=0)  # rotation of each combined orientation
+    mode_of = np.tile(np.arange(nm), quats.shape[0] // nm)
     Rn, P = quats.shape[0], len(corr)
-    size = model1.shape[0]
+    size = vols.shape[-1]
     beta, scale = fparams['beta'], fparams['scale']
     black = np.asarray(fparams['blacklist']).astype(bool)
     sum_fact = fparams['sum_fact']
@@ -91,11 +96,11 @@ def iterate(det, quats, model1, frames, fparams, flags, refine=None):
     K = dense_counts(frames, P)
     D = K.shape[0]
     dense = 'int_frames' in frames
-    W = np.array([slice_view(model1, q, qvals, corr) for q in quats])
+    W = np.array([slice_view(vols[m], q, qvals, corr) for q, m in zip(quats, mode_of)])
     good = mask < 1
     u = W[:, good].sum(1)
     w_r = quats[:, 4]
-    rescale = fparams['mean_count'] / (w_r * u).sum()
+    rescale = fparams['mean_count'] / (w_r * u).sum() * nm
     with np.errstate(divide='ignore'):
         logW = np.log(W)
     R = np.full((D, Rn), -np.inf)
@@ -129,7 +134,7 @@ def iterate(det, quats, model1, frames, fparams, flags, refine=None):
         Pm[d] = e / p_norm[d]
         like[d] = (Pm[d] * np.where(kept, R[d] - sum_fact[d], 0.)).sum()
         big = kept & (Pm[d] >= 1e-6)
-        info[d] = (Pm[d][big] * np.log(Pm[d][big] / w_r[big])).sum()
+        info[d] = (Pm[d][big] * np.log(Pm[d][big] / w_r[big] * nm)).sum()
         psum_d[d] = (Pm[d] * u * rescale).sum()
     phi_p = np.where(black, 0., scale if need_scaling else 1.)
     psum_r = (Pm * phi_p[:, None]).sum(0)
@@ -137,16 +142,22 @@ def iterate(det, quats, model1, frames, fparams, flags, refine=None):
     Psig[black] = 0.
     K2 = np.where((mask < 2)[None, :], K, 0)
     views = Psig.T @ K2
-    model2 = np.zeros((size,) * 3)
-    weight = np.zeros((size,) * 3)
+    model2 = np.zeros((nm,) + (size,) * 3)
+    weight = np.zeros((nm,) + (size,) * 3)
     for r in range(Rn):
         if psum_r[r] > 0:
-            merge_view(model2, weight, quats[r], views[r] / psum_r[r], qvals, corr, mask)
+            merge_view(model2[mode_of[r]], weight[mode_of[r]], quats[r], views[r] / psum_r[r], qvals, corr,
+                       mask)
+    if model1.ndim == 3:
+        model2, weight = model2[0], weight[0]
     nb = D - int(black.sum())
     out = dict(u=u, rescale=rescale, max_exp=M, rmax=rmax, p_norm=p_norm, likelihood=like,
                info=info, psum_d=psum_d, psum_r=psum_r, prob=Pm, model2=model2,
                inter_weight=weight, R=R, avg_likelihood=like[~black].sum() / nb,
                mutual_info=info[~black].sum() / nb)
+    if nm > 1:
+        out['quat_norm'] = np.stack([np.where((mode_of == m)[None, :] & (Pm >= 1e-6), Pm, 0.).sum(1)
+                                     for m in range(nm)], axis=1)
     if need_scaling and update_scale:
         new_scale = scale.copy()
         new_scale[~black] = fparams['fcounts'][~black] / psum_d[~black]

@@ -26,7 +26,7 @@ class Problem(C.Structure):
         ('mean_count', C.c_double), ('num_blacklist', C.c_int),
         ('quat_mapping', _ip), ('rel_off', _lp), ('rel_quat', _ip),
         ('need_scaling', C.c_int), ('update_scale', C.c_int), ('iteration', C.c_int),
-        ('known_scale', C.c_int), ('num_threads', C.c_int),
+        ('known_scale', C.c_int), ('num_threads', C.c_int), ('num_modes', C.c_int),
     ]
 
 
@@ -35,7 +35,7 @@ class Result(C.Structure):
         ('u', _dp), ('max_exp', _dp), ('p_norm', _dp), ('rmax', _ip), ('likelihood', _dp),
         ('info', _dp), ('psum_d', _dp), ('prob_dense', _dp),
         ('rescale', C.c_double), ('mutual_info', C.c_double), ('avg_likelihood', C.c_double),
-        ('seconds', C.c_double * 5),
+        ('seconds', C.c_double * 5), ('quat_norm', _dp),
     ]
 
 
@@ -62,6 +62,8 @@ def lib():
         L.orc_maximize.restype = C.c_double
         L.orc_update_model.argtypes = [_dp, _dp, _dp, C.c_long, C.c_double]
         L.orc_update_model.restype = C.c_double
+        L.orc_update_model_modes.argtypes = [_dp, _dp, _dp, C.c_long, C.c_double, C.c_int]
+        L.orc_update_model_modes.restype = C.c_double
         L.orc_voronoi_subset.argtypes = [C.c_int, _dp, C.c_int, _dp, _ip]
         _lib = L
     return _lib
@@ -114,6 +116,9 @@ def symmetrize_friedel(array, weights):
 
 
 def update_model(model1, model2, inter_weight, alpha=0.):
+    if model1.ndim == 4:
+        return lib().orc_update_model_modes(_p(model1), _p(model2), _p(inter_weight), model1.shape[1],
+                                            float(alpha), model1.shape[0])
     return lib().orc_update_model(_p(model1), _p(model2), _p(inter_weight), model1.shape[0],
                                   float(alpha))
 
@@ -133,7 +138,9 @@ def maximize(det, quats, model1, frames, fparams, flags, num_threads=1, dense_pr
 
     det      : dict qvals (N,3) f8, corr (N,) f8, mask (N,) u1
     quats    : (R,5) f8
-    model1   : (S,S,S) f8
+    model1   : (S,S,S) f8, or (M,S,S,S) for M modes: orientations are then the combined
+               r = rotation * M + mode and u, prob, rmax count R * M of them; the result holds
+               quat_norm (D, M)
     frames   : dict with ftype 'sparse' keys ones, multi, ones_accum, multi_accum, place_ones,
                place_multi, count_multi, or 'dense' key int_frames (D, N) i4
     fparams  : dict beta, scale, blacklist, fcounts, sum_fact, mean_count
@@ -147,12 +154,13 @@ def maximize(det, quats, model1, frames, fparams, flags, num_threads=1, dense_pr
     mask = np.ascontiguousarray(det['mask'], dtype='u1')
     quats = np.ascontiguousarray(quats, dtype='f8')
     model1 = np.ascontiguousarray(model1, dtype='f8')
-    S = model1.shape[0]
+    M = model1.shape[0] if model1.ndim == 4 else 1
+    S = model1.shape[-1]
     model2 = np.zeros_like(model1)
     inter_weight = np.zeros_like(model1)
     dense = 'int_frames' in frames
     D = frames['int_frames'].shape[0] if dense else len(frames['ones'])
-    R = quats.shape[0]
+    R = quats.shape[0] * M
     keep = []
 
     def arr(x, dt):
@@ -163,7 +171,7 @@ def maximize(det, quats, model1, frames, fparams, flags, num_threads=1, dense_pr
     prob = Problem()
     prob.num_pix = len(corr)
     prob.qvals, prob.corr, prob.mask = _p(qvals), _p(corr), _p(mask, _u8p)
-    prob.num_rot, prob.quats = R, _p(quats)
+    prob.num_rot, prob.quats, prob.num_modes = quats.shape[0], _p(quats), M
     prob.size, prob.model1, prob.model2, prob.inter_weight = S, _p(model1), _p(model2), _p(inter_weight)
     prob.num_data = D
     if dense:
@@ -208,6 +216,9 @@ def maximize(det, quats, model1, frames, fparams, flags, num_threads=1, dense_pr
     if dense_prob:
         out['prob'] = np.zeros((D, R))
         res.prob_dense = _p(out['prob'])
+    if M > 1:
+        out['quat_norm'] = np.zeros((D, M))
+        res.quat_norm = _p(out['quat_norm'])
     avg = lib().orc_maximize(C.byref(prob), C.byref(res))
     out.update(avg_likelihood=avg, rescale=res.rescale, mutual_info=res.mutual_info,
                model2=model2, inter_weight=inter_weight, seconds=list(res.seconds))

@@ -25,6 +25,12 @@
  *                       constant model) and by invariants (posterior sums to 1, rescale
  *                       identity, scale identity).
  *   orc_update_model    Friedel invariance, divide-by-weight and rms closed forms.
+ *   num_modes > 1       (combined orientations r = rotation * num_modes + mode) pinned by the
+ *                       identical-modes reduction to the one-mode oracle: every mode a copy of
+ *                       one model gives P / M per combined column, rmax = M * rmax_1, the same
+ *                       rescale, likelihood and info (the log M term cancels the split),
+ *                       model2 / M and the one-mode inter_weight in every mode, and
+ *                       occupancies 1 / M; with modes made distinct, by the dense brute force.
  */
 #include "emc_oracle.h"
 
@@ -478,8 +484,10 @@ static int orc_keep_find(const orc_keep *k, int r) {
 static double orc_now(void) { return omp_get_wtime(); }
 
 double orc_maximize(const orc_problem *p, orc_result *res) {
-    const int N = p->num_data, Rn = p->num_rot, P = p->num_pix;
-    const long vol = p->size * p->size * p->size;
+    /* combined orientations r = rotind * M + mode (max_prob.c:80-84) */
+    const int M = p->num_modes > 1 ? p->num_modes : 1;
+    const int N = p->num_data, Rn = p->num_rot * M, P = p->num_pix;
+    const long vol1 = p->size * p->size * p->size, vol = vol1 * M;
     const int want_psum_d = p->need_scaling && p->update_scale;
     double t0 = orc_now(), t1;
     int nthreads = p->num_threads > 0 ? p->num_threads : 1;
@@ -499,6 +507,7 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
         if (want_psum_d) res->psum_d[d] = 0.;
     }
     if (res->prob_dense) memset(res->prob_dense, 0, (size_t)N * Rn * sizeof(double));
+    if (res->quat_norm) memset(res->quat_norm, 0, (size_t)N * M * sizeof(double));
 
     /* calculate_rescale (max_prob.c:3-64) */
     double total = 0.;
@@ -508,16 +517,18 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
         double *view = malloc(P * sizeof(double));
 #pragma omp for schedule(static, 1)
         for (int r = 0; r < Rn; ++r) {
-            orc_slice_gen3d(&p->quats[r * 5], p->model1, p->size, P, p->qvals, p->corr, view);
+            const int rotind = r / M, mode = r % M;
+            orc_slice_gen3d(&p->quats[rotind * 5], p->model1 + mode * vol1, p->size, P, p->qvals, p->corr,
+                            view);
             for (int t = 0; t < P; ++t)
                 if (p->mask[t] < 1) res->u[r] += view[t];
-            priv_total += p->quats[r * 5 + 4] * res->u[r];
+            priv_total += p->quats[rotind * 5 + 4] * res->u[r];
         }
 #pragma omp critical(orc_total)
         total += priv_total;
         free(view);
     }
-    res->rescale = p->mean_count / total * 1;
+    res->rescale = p->mean_count / total * M;
     t1 = orc_now();
     res->seconds[0] = t1 - t0;
 
@@ -535,14 +546,17 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
         double *pinfo = calloc(N, sizeof(double));
         double *ppsum_d = calloc(N, sizeof(double));
         double *pnorm = calloc(N, sizeof(double));
+        double *pqnorm = calloc((size_t)N * M, sizeof(double));
         for (int d = 0; d < N; ++d) pmax[d] = ORC_MAX_EXP_START;
 
         /* calculate_prob (max_prob.c:66-213) */
 #pragma omp for schedule(static, 1)
         for (int r = 0; r < Rn; ++r) {
-            orc_slice_gen3d(&p->quats[r * 5], p->model1, p->size, P, p->qvals, p->corr, view);
+            const int rotind = r / M, mode = r % M;
+            orc_slice_gen3d(&p->quats[rotind * 5], p->model1 + mode * vol1, p->size, P, p->qvals, p->corr,
+                            view);
             for (int t = 0; t < P; ++t) view[t] = log(view[t]);
-            double logw = log(p->quats[r * 5 + 4]);
+            double logw = log(p->quats[rotind * 5 + 4]);
             for (int d = 0; d < N; ++d) {
                 if (p->blacklist[d]) continue;
                 if (p->quat_mapping) {
@@ -603,7 +617,8 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
         /* update_tomogram + merge_tomogram (max_update.c:3-230) */
 #pragma omp for schedule(static, 1)
         for (int r = 0; r < Rn; ++r) {
-            double w_r = p->quats[r * 5 + 4], psum_r = 0.;
+            const int rotind = r / M, mode = r % M;
+            double w_r = p->quats[rotind * 5 + 4], psum_r = 0.;
             for (int d = 0; d < N; ++d) {
                 if (p->blacklist[d]) continue;
                 int ind = orc_keep_find(&keep[d], r);
@@ -620,7 +635,8 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
                 }
                 if (res->prob_dense) res->prob_dense[(size_t)d * Rn + r] = prob;
                 if (prob < ORC_PROB_MIN) continue;
-                pinfo[d] += prob * log(prob / w_r * 1);
+                if (M > 1) pqnorm[(size_t)d * M + mode] += prob;
+                pinfo[d] += prob * log(prob / w_r * M);
             }
             memset(view, 0, P * sizeof(double));
             for (int d = 0; d < N; ++d) {
@@ -644,8 +660,8 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
             }
             if (psum_r > 0.) {
                 for (int t = 0; t < P; ++t) view[t] /= psum_r;
-                orc_slice_merge3d(&p->quats[r * 5], view, pmodel, pweight, p->size, P, p->qvals,
-                                  p->corr, p->mask);
+                orc_slice_merge3d(&p->quats[rotind * 5], view, pmodel + mode * vol1, pweight + mode * vol1,
+                                  p->size, P, p->qvals, p->corr, p->mask);
             }
         }
 
@@ -660,6 +676,10 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
             res->likelihood[d] += plike[d];
             res->info[d] += pinfo[d];
         }
+        if (M > 1 && res->quat_norm) {
+#pragma omp critical(orc_quat_norm)
+            for (size_t k = 0; k < (size_t)N * M; ++k) res->quat_norm[k] += pqnorm[k];
+        }
         if (want_psum_d) {
 #pragma omp critical(orc_scale)
             for (int d = 0; d < N; ++d)
@@ -670,7 +690,7 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
             free(keep[d].pos);
         }
         free(keep); free(pmax); free(prmax); free(view); free(pmodel); free(pweight);
-        free(plike); free(pinfo); free(ppsum_d); free(pnorm);
+        free(plike); free(pinfo); free(ppsum_d); free(pnorm); free(pqnorm);
     }
     double t2 = orc_now();
 
@@ -699,10 +719,12 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
     return avg;
 }
 
-/* EMCRecon.update_model, Cython order: symmetrize, divide, rms, blend (recon.pyx:112-149) */
-double orc_update_model(double *model1, double *model2, const double *w, long size, double alpha) {
-    long vol = size * size * size;
-    orc_symmetrize_friedel(model2, w, (int)size);
+/* EMCRecon.update_model, Cython order: symmetrize each mode, then divide, rms and blend over
+ * every mode's volume (recon.pyx:112-149) */
+double orc_update_model_modes(double *model1, double *model2, const double *w, long size, double alpha,
+                              int num_modes) {
+    const long vol1 = size * size * size, vol = vol1 * num_modes;
+    for (int m = 0; m < num_modes; ++m) orc_symmetrize_friedel(model2 + m * vol1, w + m * vol1, (int)size);
     for (long x = 0; x < vol; ++x)
         if (w[x] > 0.) model2[x] /= w[x];
     double change = 0.;
@@ -714,5 +736,9 @@ double orc_update_model(double *model1, double *model2, const double *w, long si
         else
             model1[x] = model2[x];
     }
-    return sqrt(change / 1 / vol);
+    return sqrt(change / num_modes / vol1);
+}
+
+double orc_update_model(double *model1, double *model2, const double *w, long size, double alpha) {
+    return orc_update_model_modes(model1, model2, w, size, alpha, 1);
 }

@@ -26,7 +26,7 @@ typedef struct orc_problem {
     /* orientations (dragonfly/src/quaternion.h:11-15), [num_rot][5] = q0..q3, weight */
     int num_rot;
     const double *quats;
-    /* model (dragonfly/src/model.h:15-20), one mode */
+    /* model (dragonfly/src/model.h:15-20): num_modes volumes of size^3 */
     long size;
     const double *model1;
     double *model2;           /* out: merged numerator */
@@ -55,6 +55,10 @@ typedef struct orc_problem {
     /* flags (dragonfly/src/params.h) */
     int need_scaling, update_scale, iteration, known_scale;
     int num_threads;          /* OpenMP threads; 1 gives the canonical single-thread keep rule */
+    /* params.num_modes (0 is taken as 1). Orientations are then the combined
+     * r = rotation * num_modes + mode, r < num_rot * num_modes (max_prob.c:80-84); u,
+     * prob_dense and rmax count combined orientations. */
+    int num_modes;
 } orc_problem;
 
 typedef struct orc_result {
@@ -70,6 +74,7 @@ typedef struct orc_result {
     double mutual_info;
     double avg_likelihood;
     double seconds[5];        /* rescale, prob, norm, update, total */
+    double *quat_norm;        /* optional [num_data][num_modes] occupancies (max_update.c:122-124) */
 } orc_result;
 
 int orc_num_rot(int num_div);
@@ -88,6 +93,10 @@ void orc_symmetrize_friedel(double *array, const double *weights, int size);
 double orc_maximize(const orc_problem *p, orc_result *res);
 double orc_update_model(double *model1, double *model2, const double *inter_weight,
                         long size, double alpha);
+/* update_model over num_modes volumes: Friedel symmetry per mode, then divide, rms change and
+ * blend over all of them, rms = sqrt(change / num_modes / size^3) (recon.pyx:112-149) */
+double orc_update_model_modes(double *model1, double *model2, const double *inter_weight, long size,
+                              double alpha, int num_modes);
 
 #ifdef __cplusplus
 }
