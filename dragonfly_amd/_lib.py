@@ -87,6 +87,7 @@ def lib():
             'emc_slice_gen': (C.c_int, [C.c_void_p, _dp, _dp]),
             'emc_slice_merge': (C.c_int, [C.c_void_p, _dp, _dp]),
             'emc_get_field': (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
+            'emc_get_sparse_prob': (C.c_int, [C.c_void_p, _ip, C.c_long, _ip, _dp, _lp]),
             'emc_clear_merge': (C.c_int, [C.c_void_p]),
             'emc_cell_posterior': (C.c_int, [C.c_void_p, C.c_int, _ip, C.POINTER(C.c_float)]),
             'emc_set_option': (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
@@ -295,6 +296,20 @@ class Engine:
         out = np.empty(shape, dtype=dt)
         _check(lib().emc_get_field(self._h, fid, out.ctypes.data_as(C.c_void_p), out.size))
         return out
+
+    def sparse_prob(self):
+        """Sparse posterior of the last iteration (max_prob.c:295-326): (num_prob, place, prob)
+        with num_prob (num_data,) i4 counts of P > 1e-6 and place (i4) / prob (f8) the
+        entries frame after frame, orientations ascending. Raises EMCError after an
+        iteration on refinement subsets."""
+        num = np.zeros(self.num_data, dtype='i4')
+        tot = C.c_long(0)
+        _check(lib().emc_get_sparse_prob(self._h, _ptr(num, _ip), 0, None, None, C.byref(tot)))
+        place = np.empty(max(tot.value, 1), dtype='i4')
+        prob = np.empty(max(tot.value, 1), dtype='f8')
+        _check(lib().emc_get_sparse_prob(self._h, _ptr(num, _ip), place.size, _ptr(place, _ip), _ptr(prob),
+                                         C.byref(tot)))
+        return num, place[:tot.value], prob[:tot.value]
 
     def set_option(self, name, value):
         """Kernel-variant option (emc_set_option): lik_tiles, lik_unroll, lik_xcd, tomo_tiles,

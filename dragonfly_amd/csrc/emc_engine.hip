@@ -2004,6 +2004,59 @@ int emc_slice_merge(emc_engine *e, const double *quat, const double *view) {
     return 0;
 }
 
+int emc_get_sparse_prob(emc_engine *e, int *num_prob, long capacity, int *place, double *prob,
+                        long *total) {
+    if (!e || !num_prob) return fail(EMC_E_ARG, "emc_get_sparse_prob: bad argument");
+    if (!e->have_results) return fail(EMC_E_STATE, "emc_get_sparse_prob: no iteration has run");
+    if (e->subset_ran)
+        return fail(EMC_E_STATE, "emc_get_sparse_prob: the last iteration ran on refinement subsets "
+                                 "(read EMC_FIELD_PROB)");
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t s = e->stream;
+    const int N = e->num_data, Rn = e->num_rot;
+    int rc;
+    DevBuf<int> cnt;
+    if ((rc = cnt.alloc(N))) return rc;
+    emc::launch_sparse_count(s, e->R.p, Rn, N, cnt.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(num_prob, cnt.p, (size_t)N * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    long tot = 0;
+    for (int d = 0; d < N; ++d) tot += num_prob[d];
+    if (total) *total = tot;
+    if (!place || !prob) return 0;
+    if (capacity < tot) return fail(EMC_E_ARG, "emc_get_sparse_prob: capacity %ld < %ld entries", capacity, tot);
+    // frames in batches of at most kBatch entries (a frame with more goes alone)
+    constexpr long kBatch = 32L << 20;
+    DevBuf<int> d_place;
+    DevBuf<double> d_prob;
+    DevBuf<long> d_off;
+    std::vector<long> off;
+    long done = 0;
+    for (int d0 = 0; d0 < N;) {
+        int d1 = d0;
+        long n = 0;
+        off.clear();
+        while (d1 < N && (d1 == d0 || n + num_prob[d1] <= kBatch)) {
+            off.push_back(n);
+            n += num_prob[d1++];
+        }
+        if (n > 0) {
+            if ((rc = d_place.alloc(n))) return rc;
+            if ((rc = d_prob.alloc(n))) return rc;
+            if ((rc = d_off.upload(off.data(), off.size(), s))) return rc;
+            emc::launch_sparse_fill(s, e->R.p, Rn, N, d0, d1 - d0, d_off.p, d_place.p, d_prob.p);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(place + done, d_place.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(prob + done, d_prob.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        done += n;
+        d0 = d1;
+    }
+    return 0;
+}
+
 int emc_get_field(emc_engine *e, int field, void *dst, size_t count) {
     if (!e || !dst) return fail(EMC_E_ARG, "emc_get_field: bad argument");
     HIP_TRY(hipSetDevice(e->device));

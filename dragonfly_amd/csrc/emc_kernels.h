@@ -57,6 +57,9 @@ void launch_posterior(hipStream_t s, double *R, int num_rot, int num_data,
                       int want_psum_d, double *likelihood, double *info, double *psum_d, int modes = 1);
 void launch_occupancy(hipStream_t s, const double *P, int num_rot, int num_data, int modes,
                       double *quat_norm);
+void launch_sparse_count(hipStream_t s, const double *P, int num_rot, int num_data, int *counts);
+void launch_sparse_fill(hipStream_t s, const double *P, int num_rot, int num_data, int d0, int nd,
+                        const long *off, int *place, double *prob);
 void launch_psum_r(hipStream_t s, const double *P, int num_rot, int num_data,
                    int frames_per_part, const double *scale, int need_scaling,
                    const uint8_t *blacklist, double *part, double *psum_r);

@@ -404,6 +404,52 @@ __global__ __launch_bounds__(kBlock) void posterior_kernel(
     }
 }
 
+// Sparse posterior (the saved probabilities, max_prob.c:295-326: entries P > 1e-6 of every
+// frame, resparsify's cut). Count pass: one block per frame.
+__global__ __launch_bounds__(kBlock) void sparse_count_kernel(const double *__restrict__ P, size_t tsR,
+                                                              int num_rot, int *__restrict__ counts) {
+    __shared__ int s_n[kWavesPerBlock];
+    const int d = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double *row = P + (size_t)d * kRTile;
+    int n = 0;
+    for (int r = threadIdx.x; r < num_rot; r += kBlock) n += row[(size_t)(r >> 6) * tsR + (r & 63)] > kProbMin;
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+    if (lane == 0) s_n[wave] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) counts[d] = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+}
+
+// Fill pass: frame d0 + blockIdx.x writes its entries, orientations ascending, at off[blockIdx.x]
+// (offsets relative to the batch's first entry): place = orientation, prob = P.
+__global__ __launch_bounds__(kBlock) void sparse_fill_kernel(const double *__restrict__ P, size_t tsR,
+                                                             int num_rot, int d0, const long *__restrict__ off,
+                                                             int *__restrict__ place, double *__restrict__ prob) {
+    __shared__ int s_n[kWavesPerBlock];
+    const int d = d0 + blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double *row = P + (size_t)d * kRTile;
+    long at = off[blockIdx.x];
+    for (int base = 0; base < num_rot; base += kBlock) {
+        const int r = base + threadIdx.x;
+        const double v = r < num_rot ? row[(size_t)(r >> 6) * tsR + (r & 63)] : 0.;
+        const bool keep = v > kProbMin;
+        const unsigned long long bal = __ballot(keep);
+        if (lane == 0) s_n[wave] = __popcll(bal);
+        __syncthreads();
+        int before = 0, total = 0;
+        for (int w = 0; w < kWavesPerBlock; ++w) {
+            before += w < wave ? s_n[w] : 0;
+            total += s_n[w];
+        }
+        if (keep) {
+            const long k = at + before + __popcll(bal & ((1ull << lane) - 1));
+            place[k] = r;
+            prob[k] = v;
+        }
+        at += total;
+        __syncthreads();  // s_n is rewritten by the next round
+    }
+}
+
 // Occupancy of every frame in every mode (max_update.c:122-124): quat_norm[d][m] = sum of the
 // posterior P_dr >= 1e-6 over the combined orientations r = rotation * modes + m. One block
 // per frame; P is the tile-major posterior (0 where not kept).
@@ -1263,6 +1309,16 @@ void launch_posterior(hipStream_t s, double *R, int num_rot, int num_data,
 void launch_occupancy(hipStream_t s, const double *P, int num_rot, int num_data, int modes,
                       double *quat_norm) {
     occupancy_kernel<<<num_data, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, num_rot, modes, quat_norm);
+}
+
+void launch_sparse_count(hipStream_t s, const double *P, int num_rot, int num_data, int *counts) {
+    sparse_count_kernel<<<num_data, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, num_rot, counts);
+}
+
+void launch_sparse_fill(hipStream_t s, const double *P, int num_rot, int num_data, int d0, int nd,
+                        const long *off, int *place, double *prob) {
+    if (nd <= 0) return;
+    sparse_fill_kernel<<<nd, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, num_rot, d0, off, place, prob);
 }
 
 void launch_psum_r(hipStream_t s, const double *P, int num_rot, int num_data,

@@ -177,11 +177,11 @@ class EMCRecon:
         out['params/num_rot'] = np.asarray(itr.quat.num_rot)
         out['params/num_div'] = np.asarray(itr.quat.num_div)
         if itr.params.save_prob:
-            place, prob = self.place_prob, self.prob
+            num, place, prob = self.sparse_prob()
             out['probabilities/num_rot'] = np.asarray([itr.quat.num_rot])
-            out['probabilities/num_prob'] = np.array([len(p) for p in place], dtype='i4')
-            out['probabilities/place'] = np.concatenate(place) if place else np.zeros(0, 'i4')
-            out['probabilities/prob'] = np.concatenate(prob) if prob else np.zeros(0)
+            out['probabilities/num_prob'] = num
+            out['probabilities/place'] = place
+            out['probabilities/prob'] = prob
         return out
 
     def save_output(self, dynamic=False):
@@ -255,21 +255,34 @@ class EMCRecon:
         """(num_data, num_rot) posterior, 0 where an orientation was not kept."""
         return self.engine.get('prob')
 
+    def sparse_prob(self):
+        """(num_prob, place, prob) of the saved sparse posterior (P > PROB_MIN,
+        max_prob.c:323-325), entries frame after frame: compacted on the GPU, or, after an
+        iteration on refinement subsets, cut from the dense posterior."""
+        try:
+            return self.engine.sparse_prob()
+        except _lib.EMCError:
+            if self.iter is None or self.iter.quat_mapping is None:
+                raise
+        P = self.prob_dense
+        keep = P > PROB_MIN
+        return keep.sum(1).astype('i4'), np.nonzero(keep)[1].astype('i4'), P[keep]
+
     @property
     def place_prob(self):
-        """Per frame, orientations with P > PROB_MIN (the saved sparse posterior,
-        max_prob.c:323-325)."""
-        P = self.prob_dense
-        return [np.nonzero(row > PROB_MIN)[0].astype('i4') for row in P]
+        """Per frame, orientations with P > PROB_MIN (recon.pyx:413-415)."""
+        num, place, _ = self.sparse_prob()
+        return np.split(place, np.cumsum(num)[:-1])
 
     @property
     def prob(self):
-        P = self.prob_dense
-        return [row[row > PROB_MIN] for row in P]
+        """Per frame, the posterior of those orientations (recon.pyx:405-411)."""
+        num, _, prob = self.sparse_prob()
+        return np.split(prob, np.cumsum(num)[:-1])
 
     @property
     def num_prob(self):
-        return np.array([len(p) for p in self.place_prob], dtype='i4')
+        return self.sparse_prob()[0]
 
     def free(self):
         if self.engine is not None:

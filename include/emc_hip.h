@@ -187,6 +187,15 @@ int emc_slice_gen(emc_engine *e, const double *quat, double *view_out);
 int emc_slice_merge(emc_engine *e, const double *quat, const double *view);
 
 /* ---- outputs ------------------------------------------------------------------------- */
+/* Sparse posterior of the last iteration: for every frame the orientations with P > 1e-6 and
+ * their P, orientations ascending (common->prob / place_prob after resparsify with PROB_MIN,
+ * max_prob.c:295-326; saved as probabilities/place and probabilities/prob, recon.pyx:269-279,
+ * the single-thread order). num_prob[num_data] receives the counts and *total their sum;
+ * with place and prob non-NULL (capacity >= total entries) the entries follow, frame after
+ * frame. Counted and compacted on the GPU. After an iteration on refinement subsets it
+ * returns EMC_E_STATE (read EMC_FIELD_PROB instead). */
+int emc_get_sparse_prob(emc_engine *e, int *num_prob, long capacity, int *place, double *prob,
+                        long *total);
 /* Copy field `field` (enum emc_field) into dst; count is the element count of dst and
  * must equal the field's size.                                                          */
 int emc_get_field(emc_engine *e, int field, void *dst, size_t count);

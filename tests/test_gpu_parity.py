@@ -449,3 +449,28 @@ def test_options_roundtrip_and_variants_agree(tiny):
             assert P.rel_err(g[k], g1[k]) <= TOL, k
         assert np.array_equal(g1['rmax'], g['rmax'])
     assert g3['num_chunks'] > 1
+
+
+def test_sparse_prob(tiny):
+    """emc_get_sparse_prob (the saved probabilities, max_prob.c:295-326): counts and entries
+    compacted on the GPU equal the cut P > 1e-6 of the engine's dense posterior bit for bit,
+    orientations ascending, blacklisted frames empty; against the oracle within 1e-9."""
+    D = len(tiny['fparams']['beta'])
+    bl = np.zeros(D, 'u1')
+    bl[[3, 17]] = 1
+    p = dict(tiny, fparams=dict(tiny['fparams'], blacklist=bl, num_blacklist=2))
+    flags = dict(iteration=1, need_scaling=0)
+    o = P.run_oracle(p, flags)
+    eng = P.engine_for(p)
+    P.run_engine(eng, flags)
+    num, place, prob = eng.sparse_prob()
+    dense = eng.get('prob')
+    eng.close()
+    keep = dense > 1e-6
+    assert np.array_equal(num, keep.sum(1))
+    assert num[3] == 0 and num[17] == 0 and num.sum() > D
+    assert np.array_equal(place, np.nonzero(keep)[1])
+    assert np.array_equal(prob, dense[keep])
+    okeep = o['prob'] > 1e-6
+    assert np.array_equal(keep, okeep)
+    assert P.rel_err(prob, o['prob'][okeep]) <= TOL

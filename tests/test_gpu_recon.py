@@ -1,0 +1,62 @@
+"""EMCRecon end to end on the GPU (dragonfly/recon.pyx:48-279 mirrored by
+dragonfly_amd/recon.py): one iteration with two model modes and save_prob, its output file
+checked against the oracle on the same inputs (1e-9 relative; orientation lists exact)."""
+import numpy as np
+import pytest
+
+import dragonfly_amd as df
+import problems as P
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+
+
+def test_recon_two_modes_save_prob(tmp_path):
+    from dragonfly_amd import synth
+    from oracle import core
+    p = P.build(P.tiny_config(num_data=40))
+    det = df.CDetector.from_arrays(p['det']['qvals'], p['det']['corr'], p['det']['mask'])
+    f = p['frames']
+    itr = df.Iterate()
+    itr.dets = [det]
+    itr.set_data(df.CDataset.from_arrays(det, f['ones'], f['multi'], f['place_ones'], f['place_multi'],
+                                         f['count_multi']))
+    itr.set_quat(df.Quaternion(2))
+    par = itr.params
+    par.num_modes, par.save_prob = 2, 1
+    par.beta_factor = 1.  # the config default; EMCParams() starts at -1 as the reference's
+    par.output_folder, par.log_fname = str(tmp_path), str(tmp_path / 'EMC.log')
+    S = p['model1'].shape[0]
+    vols = np.stack([p['model1'], synth.start_model(S, p['fparams']['mean_count'], p['det']['mask'], seed=9)])
+    model = df.Model(S, 2)
+    model.model1 = vols.copy()
+    model.model2 = np.zeros_like(vols)
+    model.inter_weight = np.zeros_like(vols)
+    itr.set_model(model)
+    itr.calc_beta(0.8)
+    rec = df.EMCRecon()
+    rec.set_iterate(itr)
+    par.iteration = 1
+    print('case: recon two modes save_prob', flush=True)
+    rec.run_iteration()
+    out = np.load(tmp_path / 'output_001.npz')
+    rec.free()
+
+    fp = dict(beta=itr.beta.copy(), scale=np.ones(40), blacklist=np.zeros(40, 'u1'), fcounts=itr.fcounts,
+              sum_fact=itr.sum_fact, mean_count=itr.mean_count[0], num_blacklist=0)
+    o = core.maximize(p['det'], itr.quat.quats, vols, f, fp, dict(iteration=1, need_scaling=0),
+                      dense_prob=True)
+    m1, m2 = vols.copy(), o['model2'].copy()
+    rms = core.update_model(m1, m2, o['inter_weight'], 0.)
+    assert out['intens'].shape == (2, S, S, S)
+    assert P.rel_err(out['intens'], m1) <= TOL
+    assert P.rel_err(out['inter_weight'], o['inter_weight']) <= TOL
+    assert P.rel_err(out['occupancies'], o['quat_norm']) <= TOL
+    assert np.array_equal(out['orientations'], o['rmax'])
+    assert abs(itr.rms_change - rms) <= TOL * rms
+    keep = o['prob'] > 1e-6
+    assert np.array_equal(out['probabilities/num_prob'], keep.sum(1))
+    assert np.array_equal(out['probabilities/place'], np.nonzero(keep)[1])
+    assert P.rel_err(out['probabilities/prob'], o['prob'][keep]) <= TOL
+    assert int(out['probabilities/num_rot'][0]) == itr.quat.num_rot
