@@ -62,6 +62,7 @@ def lib():
             'emc_quat_gen': (C.c_int, [C.c_int, _dp]),
             'emc_set_detector': (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _u8p]),
             'emc_set_modes': (C.c_int, [C.c_void_p, C.c_int]),
+            'emc_set_background': (C.c_int, [C.c_void_p, _dp, _dp]),
             'emc_set_quats': (C.c_int, [C.c_void_p, C.c_int, _dp]),
             'emc_set_model': (C.c_int, [C.c_void_p, C.c_long, _dp]),
             'emc_set_frames_sparse': (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _lp, _lp, _ip, _ip, _ip]),
@@ -211,6 +212,18 @@ class Engine:
         _check(lib().emc_set_frames_dense(self._h, frames.shape[0], frames.shape[1], _ptr(frames, _ip)))
         self.num_data = frames.shape[0]
 
+
+    def set_background(self, background, bgscale):
+        """Detector background (num_pix,) and per-frame background scales (num_data,); None
+        switches it off. Used by the iteration when need_scaling is set."""
+        if background is None:
+            _check(lib().emc_set_background(self._h, None, None))
+            return
+        bg = np.ascontiguousarray(background, dtype='f8')
+        bs = np.ascontiguousarray(bgscale, dtype='f8')
+        if bg.shape != (self.num_pix,) or bs.shape != (self.num_data,):
+            raise ValueError('background must be (num_pix,) and bgscale (num_data,)')
+        _check(lib().emc_set_background(self._h, _ptr(bg), _ptr(bs)))
 
     def set_refine(self, num_coarse, quat_mapping, rel_off, rel_quat):
         """Per-frame orientation subsets (refinement); quat_mapping None switches it off."""

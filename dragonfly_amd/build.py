@@ -11,7 +11,8 @@ CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libemc_hip.so')
 INCLUDE = os.path.join(os.path.dirname(HERE), 'include')
 
-SOURCES = ['emc_engine.hip', 'emc_kernels.hip', 'emc_mp.hip', 'emc_merge.hip', 'emc_nn.hip', 'emc_subset.hip', 'quat_gen.cpp']
+SOURCES = ['emc_engine.hip', 'emc_kernels.hip', 'emc_mp.hip', 'emc_merge.hip', 'emc_nn.hip', 'emc_subset.hip', 'emc_background.hip',
+           'quat_gen.cpp']
 
 # -ffp-contract=off: no fused multiply-add anywhere, so every FP64 value keeps the
 # reference's rounding sequence. Packed FP32 ops are disabled for device code.

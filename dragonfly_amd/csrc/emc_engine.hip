@@ -166,6 +166,12 @@ struct emc_engine {
     long size = 0;
     DevBuf<double> model1, model2, weight;
     DevBuf<double> quat_norm;  // [num_data][num_modes] occupancies when num_modes > 1
+    // detector background (background-aware updates when the scale factors are on)
+    bool with_bg = false;
+    DevBuf<double> bg, powder, bgscale;  // engine pixel order; per frame
+    DevBuf<double> bg_smax, bg_smax_part, bg_gpart, bg_gacc, bg_s;
+    DevBuf<uint8_t> bg_dmask;
+    DevBuf<int> bg_last;
     // frames
     int num_data = 0;
     std::vector<int> h_ones, h_multi, h_place_ones, h_place_multi, h_count_multi;
@@ -1413,6 +1419,7 @@ int emc_set_detector(emc_engine *e, int num_pix, const double *qvals, const doub
     if ((rc = e->d_perm.upload(perm.data(), perm.size(), e->stream))) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->h_mask.assign(raw_mask, raw_mask + num_pix);
+    e->with_bg = false;  // the background and powder follow the detector and frames
     e->perm.swap(perm);
     e->inv.swap(inv);
     e->num_pix = num_pix;
@@ -1502,6 +1509,45 @@ int emc_set_model(emc_engine *e, long size, const double *model1) {
     return 0;
 }
 
+int emc_set_background(emc_engine *e, const double *background, const double *bgscale) {
+    if (!e) return fail(EMC_E_ARG, "null engine");
+    if (!background) {
+        e->with_bg = false;
+        return 0;
+    }
+    if (!bgscale) return fail(EMC_E_ARG, "emc_set_background: bgscale is NULL");
+    if (!e->num_pix || !e->num_data) return fail(EMC_E_STATE, "emc_set_background: set the detector and frames first");
+    for (uint8_t v : e->h_dense)
+        if (v) return fail(EMC_E_STATE, "emc_set_background: the background-aware path takes sparse frames");
+    HIP_TRY(hipSetDevice(e->device));
+    const int P = e->num_pix, N = e->num_data;
+    // calc_powder (iterate.c:119-179): mean photons per mask < 1 pixel over all frames
+    std::vector<double> pw(P, 0.);
+    for (int d = 0; d < N; ++d) {
+        for (int k = 0; k < e->h_ones[d]; ++k) {
+            const int t = e->h_place_ones[e->h_ones_accum[d] + k];
+            if (e->h_mask[t] < 1) pw[t]++;
+        }
+        for (int k = 0; k < e->h_multi[d]; ++k) {
+            const long at = e->h_multi_accum[d] + k;
+            const int t = e->h_place_multi[at];
+            if (e->h_mask[t] < 1) pw[t] += e->h_count_multi[at];
+        }
+    }
+    std::vector<double> b(P), pe(P);
+    for (int t = 0; t < P; ++t) {
+        b[t] = background[e->perm[t]];
+        pe[t] = pw[e->perm[t]] / N;
+    }
+    int rc;
+    if ((rc = e->bg.upload(b.data(), P, e->stream))) return rc;
+    if ((rc = e->powder.upload(pe.data(), P, e->stream))) return rc;
+    if ((rc = e->bgscale.upload(bgscale, N, e->stream))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->with_bg = true;
+    return 0;
+}
+
 int emc_set_frames_sparse(emc_engine *e, int num_data, const int *ones, const int *multi,
                           const long *ones_accum, const long *multi_accum, const int *place_ones,
                           const int *place_multi, const int *count_multi) {
@@ -1526,6 +1572,7 @@ int emc_set_frames_sparse(emc_engine *e, int num_data, const int *ones, const in
     e->num_data = num_data;
     e->lists_dirty = true;
     e->have_params = false;
+    e->with_bg = false;  // the powder and per-frame background scales belong to the old frames
     e->have_results = false;
     e->refine = false;  // per-frame orientation sets belong to the previous frames
     return 0;
@@ -1691,6 +1738,148 @@ int emc_clear_merge(emc_engine *e) {
     return 0;
 }
 
+// update_scale_bg (max_scale.c:134-288): per frame, the root in its scale of gradient_d,
+// bracketed (stopping once 99% of the frames are bracketed) and then found by Ridder's method.
+// The per-frame bookkeeping runs here, as the reference's does; every gradient evaluation is
+// gradient_d_kernel over all orientation chunks (views regenerated unless they fit one chunk).
+static int update_scale_bg(emc_engine *e, const emc_iter_opts *opts) {
+    hipStream_t s = e->stream;
+    const int N = e->num_data, P = e->num_pix, C = e->C, Rn = e->num_rot, Rp = e->Rp;
+    int rc;
+    std::vector<double> scale(N), psum_d(N);
+    std::vector<uint8_t> black(N);
+    double rescale = 0.;
+    HIP_TRY(hipMemcpyAsync(scale.data(), e->scale.p, N * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(psum_d.data(), e->psum_d.p, N * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(black.data(), e->blacklist.p, N, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&rescale, e->scalars.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if ((rc = e->bg_gpart.alloc((size_t)(C / emc::kRTile) * N))) return rc;
+    if ((rc = e->bg_gacc.alloc(N))) return rc;
+    if ((rc = e->bg_s.alloc(N))) return rc;
+    if ((rc = e->bg_dmask.alloc(N))) return rc;
+    const bool one_chunk = C >= Rp;
+    bool views_ready = false;
+    std::vector<double> tmp(N);
+    std::vector<uint8_t> mask(N, 0);
+    auto gradient = [&](const std::vector<double> &sv, std::vector<double> &G) -> int {
+        int r2;
+        if ((r2 = e->bg_dmask.upload(mask.data(), N, s))) return r2;
+        if ((r2 = e->bg_s.upload(sv.data(), N, s))) return r2;
+        emc::launch_fill(s, e->bg_gacc.p, N, 0.);
+        for (int r0 = 0; r0 < Rp; r0 += C) {
+            const int Cc = (Rp - r0 < C) ? (Rp - r0) : C;
+            // the views of the current model (the tomogram wrote its own into this buffer)
+            if (!(one_chunk && views_ready))
+                emc::launch_expand(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p, P, r0, Cc,
+                                   kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p, nullptr, 1, 1);
+            emc::launch_gradient_d(s, e->logw.p, P, e->R.p, N, r0, Cc, Rn, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
+                                   e->bg_dmask.p, e->bg_s.p, e->bgscale.p, e->bg.p, rescale, e->bg_gpart.p,
+                                   e->bg_gacc.p);
+        }
+        views_ready = true;
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(tmp.data(), e->bg_gacc.p, N * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int d = 0; d < N; ++d)
+            if (mask[d] == 0) G[d] = tmp[d] - psum_d[d];
+        return 0;
+    };
+    std::vector<double> s_old(N, 0.), s_new(N, 0.), s_mid(N, 0.), s_lat(N, 0.);
+    std::vector<double> g_old(N, 0.), g_new(N, 0.), g_mid(N, 0.), g_lat(N, 0.);
+    for (int d = 0; d < N; ++d) {
+        if (black[d]) {
+            mask[d] = 255;
+            scale[d] = -1.;
+            continue;
+        }
+        s_new[d] = (opts->iteration > 1 || opts->known_scale) ? 4. * scale[d] : 32.;
+    }
+    if ((rc = gradient(s_old, g_old))) return rc;
+    for (int d = 0; d < N; ++d)
+        if ((mask[d] != 255) & (g_old[d] < 1.e-6)) {
+            scale[d] = 0.001;  // fewer photons than the background
+            mask[d] = 255;
+        }
+    int num_mask = 0;
+    for (int i = 0; i < 5; ++i) {
+        if ((rc = gradient(s_new, g_new))) return rc;
+        num_mask = 0;
+        for (int d = 0; d < N; ++d) {
+            if (mask[d] != 0) {
+                num_mask++;
+            } else if (g_new[d] < 0.) {
+                mask[d] = 192;
+                num_mask++;
+            } else {
+                s_new[d] *= 4.;
+            }
+        }
+        if (num_mask > ((double)0.99 * N)) break;
+    }
+    for (int i = 0; i < 50; ++i) {
+        num_mask = 0;
+        for (int d = 0; d < N; ++d) {
+            if (mask[d] == 255) {
+                num_mask++;
+            } else if (std::fabs(s_old[d] - s_new[d]) < 1.e-5) {
+                mask[d] = 255;
+                scale[d] = 0.5 * (s_old[d] + s_new[d]);
+                num_mask++;
+            } else if (mask[d] == 192) {
+                mask[d] = 0;
+                s_mid[d] = 0.5 * (s_old[d] + s_new[d]);
+            } else {
+                s_mid[d] = 0.5 * (s_old[d] + s_new[d]);
+            }
+        }
+        if (num_mask == N) break;
+        if ((rc = gradient(s_mid, g_mid))) return rc;
+        for (int d = 0; d < N; ++d) {
+            if (mask[d] == 255) continue;
+            if (std::fabs(g_mid[d]) < 1.e-3) {
+                scale[d] = s_mid[d];
+                mask[d] = 255;
+                num_mask++;
+                continue;
+            }
+            const double sq = std::sqrt(g_mid[d] * g_mid[d] - g_old[d] * g_new[d]);
+            if (sq < 1.e-15) {
+                s_lat[d] = s_mid[d];
+            } else {
+                s_lat[d] = s_mid[d] + (s_mid[d] - s_old[d]) * std::copysign(1.0, g_old[d]) * g_mid[d] / sq;
+                const double lo = std::fmin(s_old[d], s_new[d]), hi = std::fmax(s_old[d], s_new[d]);
+                if (s_lat[d] <= lo || s_lat[d] >= hi) s_lat[d] = s_mid[d];
+            }
+        }
+        if (num_mask == N) break;
+        if ((rc = gradient(s_lat, g_lat))) return rc;
+        for (int d = 0; d < N; ++d)
+            if (mask[d] < 255) {
+                if (std::fabs(g_lat[d]) < 1.e-3) {
+                    scale[d] = s_lat[d];
+                    mask[d] = 255;
+                    num_mask++;
+                } else if (g_mid[d] * g_lat[d] < 0.) {
+                    s_old[d] = s_mid[d];
+                    g_old[d] = g_mid[d];
+                    s_new[d] = s_lat[d];
+                    g_new[d] = g_lat[d];
+                } else if (g_old[d] * g_lat[d] < 0.) {
+                    s_new[d] = s_lat[d];
+                    g_new[d] = g_lat[d];
+                } else {
+                    s_old[d] = s_lat[d];
+                    g_old[d] = g_lat[d];
+                }
+            }
+        if (num_mask == N) break;
+    }
+    if ((rc = e->scale.upload(scale.data(), N, s))) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
 int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats) {
     int rc;
     if ((rc = check_ready(e))) return rc;
@@ -1699,6 +1888,10 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     if (e->lists_dirty && (rc = build_lists(e))) return rc;
     const bool mp = e->precision == 1 || e->precision == 2;
     const bool g32 = e->precision == 3;
+    // background-aware updates (max_prob.c:94, max_update.c:11, max_scale.c:9)
+    const bool bgp = e->with_bg && opts->need_scaling;
+    if (bgp && (e->precision != 0 || e->num_modes > 1 || e->refine))
+        return fail(EMC_E_STATE, "emc_maximize: the background-aware path runs in FP64 with one mode, no refinement");
     if (g32) e->tomo_tile_mode = 2;  // the FP32 posterior feeds the frame-partitioned tomogram
     if (e->refine && e->subset_mode && e->subset_ready && !mp) return maximize_subset(e, opts, stats);
     e->subset_ran = false;
@@ -1753,7 +1946,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
                                P, r0, Cc, kPixPerGroup, e->logw32.p, e->upart.p, e->ubad.p, nullptr, M);
         else
             emc::launch_expand(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
-                               P, r0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p, nullptr, M);
+                               P, r0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p, nullptr, M, bgp);
         emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, r0, Rn, e->u.p, e->rbad.p);
         HIP_TRY(hipEventRecord(e->ev[1], s));
         if (mp) {
@@ -1767,6 +1960,9 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         } else if (g32) {
             emc::launch_likelihood(s, e->logw32.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
                                    e->blacklist.p, N, e->R.p, r0, e->lik_tiles, e->lik_unroll);
+        } else if (bgp) {
+            emc::launch_likelihood_bg(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p, e->blacklist.p,
+                                      N, e->scale.p, e->bgscale.p, e->bg.p, e->R.p, r0);
         } else {
             if (e->lik_part.p)
                 emc::launch_likelihood_split(s, e->logw.p, Cc, P, e->lik8_off.p, e->lik_split, e->lik8_pix.p,
@@ -1803,6 +1999,14 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     }
     emc::launch_psum_r(s, e->R.p, Rn, N, kFramesPerPsumPart, e->scale.p, opts->need_scaling,
                        e->blacklist.p, e->psum_part.p, e->psum_r.p);
+    if (bgp) {
+        // calc_psum_r's scalemax per orientation, for the background-aware tomogram
+        const int nseg = N < 64 ? N : 64;
+        if ((rc = e->bg_smax_part.alloc((size_t)nseg * e->Rp))) return rc;
+        if ((rc = e->bg_smax.alloc(e->Rp))) return rc;
+        if ((rc = e->bg_last.alloc(e->Rp))) return rc;
+        emc::launch_scalemax(s, e->R.p, N, e->Rp, e->scale.p, e->bg_smax_part.p, nseg, e->bg_smax.p);
+    }
     HIP_TRY(hipEventRecord(e->ev[7], s));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventSynchronize(e->ev[7]));
@@ -1868,8 +2072,17 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         const int Cc = (e->Rp - r0 < C) ? (e->Rp - r0) : C;
         float a = 0.f, b = 0.f, c = 0.f;
         HIP_TRY(hipEventRecord(e->ev[0], s));
-        const bool parts = !mp && e->tomo_tile_mode == 2;
-        if (mp) {
+        const bool parts = !mp && !bgp && e->tomo_tile_mode == 2;
+        if (bgp) {
+            // update_tomogram_bg: first pass for each orientation's last bracket round, then
+            // the solve; the view goes where the merge reads it
+            HIP_TRY(hipMemsetAsync(e->bg_last.p + r0, 0, (size_t)Cc * sizeof(int), s));
+            for (int pass = 1; pass >= 0; --pass)
+                emc::launch_tomogram_bg(s, pass, e->R.p, N, r0, Cc, Rn, e->tom_off.p, e->tom_frame.p, e->tom_cnt.p,
+                                        e->scale.p, e->bgscale.p, e->bg.p, e->powder.p, e->mask.p, P,
+                                        e->mean_count, e->psum_r.p, e->bg_smax.p, e->bg_last.p, e->logw.p,
+                                        bricks ? 1 : 0);
+        } else if (mp) {
             // P^T K on matrix cores: P stored as fp16(P * 2^15) where P > 1e-6
             const double pscale = 32768.;
             emc::launch_posterior_fp16(s, e->R.p, N, r0, Cc, e->mp_ldd, pscale, e->p16.p);
@@ -1921,7 +2134,11 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     // combine_information_mpi statistics and update_scale
     emc::launch_frame_stats(s, e->likelihood.p, e->info.p, e->blacklist.p, N, N - e->num_blacklist,
                             e->scalars.p);
-    if (want_psum_d) emc::launch_scale(s, e->fcounts.p, e->psum_d.p, e->blacklist.p, N, e->scale.p);
+    if (want_psum_d && e->with_bg) {
+        if ((rc = update_scale_bg(e, opts))) return rc;
+    } else if (want_psum_d) {
+        emc::launch_scale(s, e->fcounts.p, e->psum_d.p, e->blacklist.p, N, e->scale.p);
+    }
     HIP_TRY(hipGetLastError());
     double host_scalars[3];
     HIP_TRY(hipMemcpyAsync(host_scalars, e->scalars.p, sizeof host_scalars, hipMemcpyDeviceToHost, s));

@@ -11,7 +11,7 @@ namespace emc {
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad,
-                   const int *slot_rot = nullptr, int modes = 1);
+                   const int *slot_rot = nullptr, int modes = 1, int raw = 0);
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad,
@@ -57,6 +57,21 @@ void launch_posterior(hipStream_t s, double *R, int num_rot, int num_data,
                       int want_psum_d, double *likelihood, double *info, double *psum_d, int modes = 1);
 void launch_occupancy(hipStream_t s, const double *P, int num_rot, int num_data, int modes,
                       double *quat_norm);
+// background-aware path (emc_background.hip)
+void launch_likelihood_bg(hipStream_t s, const double *W, int C, int num_pix, const long *off, const int *pix,
+                          const double *cnt, const uint8_t *blacklist, int num_data, const double *scale,
+                          const double *bgscale, const double *bg, double *G, int r0);
+void launch_scalemax(hipStream_t s, const double *P, int num_data, int Rp, const double *scale, double *part,
+                     int nseg, double *out);
+void launch_tomogram_bg(hipStream_t s, int first, const double *P, int num_data, int r0, int C, int num_rot,
+                        const long *tom_off, const int *tom_frame, const double *tom_cnt, const double *scale,
+                        const double *bgscale, const double *bg, const double *powder, const uint8_t *raw_mask,
+                        int num_pix, double mean_count, const double *psum_r, const double *scalemax,
+                        int *last_round, double *view, int rmajor);
+void launch_gradient_d(hipStream_t s, const double *W, int num_pix, const double *P, int num_data, int r0, int C,
+                       int num_rot, const long *off, const int *pix, const double *cnt, const uint8_t *dmask,
+                       const double *sc, const double *bgscale, const double *bg, double rescale, double *part,
+                       double *gacc);
 void launch_sparse_count(hipStream_t s, const double *P, int num_rot, int num_data, int *counts);
 void launch_sparse_fill(hipStream_t s, const double *P, int num_rot, int num_data, int d0, int nd,
                         const long *off, int *place, double *prob);

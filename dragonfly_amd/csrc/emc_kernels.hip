@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
     const double *__restrict__ qvals, const double *__restrict__ corr,
     const uint8_t *__restrict__ mask, int num_pix, int r0, int C, int pix_per_group,
     T *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad,
-    const int *__restrict__ slot_rot, int modes) {
+    const int *__restrict__ slot_rot, int modes, int raw) {
     __shared__ double s_rot[kRTile * 9];
     __shared__ double s_tile[64][kRTile + 1];
     __shared__ double s_u[kWavesPerBlock][kRTile];
@@ -95,7 +95,8 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
                 }
                 // tile-major with kW-wide tiles: (c / kW) * num_pix * kW + t * kW + c % kW
                 const int c = c0 + lane;
-                logw[(size_t)(c / kW) * ((size_t)num_pix * kW) + (size_t)tt * kW + (c % kW)] = (T)log(W);
+                // raw: W itself (the background-aware likelihood takes log(phi W + b B))
+                logw[(size_t)(c / kW) * ((size_t)num_pix * kW) + (size_t)tt * kW + (c % kW)] = (T)(raw ? W : log(W));
             }
         }
         __syncthreads();
@@ -1069,10 +1070,10 @@ static inline int round_up_int(int a, int b) { return (a + b - 1) / b * b; }
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad,
-                   const int *slot_rot, int modes) {
+                   const int *slot_rot, int modes, int raw) {
     dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
     expand_kernel<double, kRTile><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0, C,
-                                                          pix_per_group, logw, upart, ubad, slot_rot, modes);
+                                                          pix_per_group, logw, upart, ubad, slot_rot, modes, raw);
 }
 
 void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
@@ -1081,7 +1082,7 @@ void launch_expand(hipStream_t s, const double *rots, const double *model, long 
                    const int *slot_rot, int modes) {
     dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
     expand_kernel<float, 2 * kRTile><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0,
-                                                             C, pix_per_group, logw, upart, ubad, slot_rot, modes);
+                                                             C, pix_per_group, logw, upart, ubad, slot_rot, modes, 0);
 }
 
 // FP32 copy of one orientation chunk of the tile-major posterior in 128-wide tiles:

@@ -19,6 +19,7 @@ class CDetector:
         self.detd = 0.
         self.ewald_rad = 0.
         self.with_bg = 0
+        self._background = None
         if fname is not None:
             self.parse(fname, norm=norm, rtype=rtype)
 
@@ -107,6 +108,31 @@ class CDetector:
         if arr.shape[0] != n:
             raise ValueError('num_pix mismatch with other data (%d vs %d)' % (arr.shape[0], n))
         self._mask = arr.copy()
+
+    @property
+    def background(self):
+        """Per-pixel background (detector.pyx:207-220), None without one."""
+        return self._background
+
+    @background.setter
+    def background(self, arr):
+        if arr is None:
+            self._background, self.with_bg = None, 0
+            return
+        arr = np.ascontiguousarray(arr, dtype='f8').ravel()
+        if arr.shape[0] != self.num_pix:
+            raise ValueError('background must have num_pix entries')
+        self._background, self.with_bg = arr.copy(), 1
+
+    def parse_background(self, fname):
+        """The 'background' dataset of a .npz (or HDF5, with h5py) file (detector.pyx:101-104)."""
+        if fname.endswith('.npz'):
+            with np.load(fname) as f:
+                self.background = f['background']
+        else:
+            import h5py
+            with h5py.File(fname, 'r') as f:
+                self.background = f['background'][:]
 
     @classmethod
     def from_arrays(cls, qvals, corr, raw_mask, detd=0., ewald_rad=0., fname='in-memory'):

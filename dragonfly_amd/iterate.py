@@ -294,16 +294,16 @@ class Iterate:
         return cat
 
     def parse_scale(self, fname, bg=False):
-        """Scale factors from a file, or ones (iterate.pyx:284-315)."""
-        if bg:
-            raise NotImplementedError('background scale factors are not implemented')
+        """Scale factors (bg: background scale factors, bgscale) from a file, or ones
+        (iterate.pyx:284-315)."""
         if not fname or not op.isfile(fname):
             if self.tot_num_data == 0:
                 raise AttributeError('Need tot_num_data to initialize scale factors')
             scale = np.ones(self.tot_num_data)
         elif fname.endswith('.npz'):
             with np.load(fname) as f:
-                scale = np.asarray(f['scale'], dtype='f8')
+                key = 'bgscale' if bg and 'bgscale' in f.files else 'scale'
+                scale = np.asarray(f[key], dtype='f8')
         else:
             scale = np.loadtxt(fname, dtype='f8').ravel()
         if self.tot_num_data < scale.shape[0]:
@@ -312,7 +312,10 @@ class Iterate:
         elif self.tot_num_data > scale.shape[0]:
             print('Insufficient scale factors in file. Setting rest to unity')
             scale = np.append(scale, np.ones(self.tot_num_data - scale.shape[0]))
-        self.scale = np.ascontiguousarray(scale, dtype='f8').copy()
+        if bg:
+            self.bgscale = np.ascontiguousarray(scale, dtype='f8').copy()
+        else:
+            self.scale = np.ascontiguousarray(scale, dtype='f8').copy()
 
     def parse_blacklist(self, fname, sel_string=None, refresh=False):
         """Blacklist from a 0/1 text file and/or odd_only / even_only (iterate.pyx:317-359)."""

@@ -80,6 +80,10 @@ class EMCRecon:
         itr.num_blacklist = int((itr.blacklist != 0).sum())
         eng.set_frame_params(itr.scale, itr.beta_start, itr.blacklist, itr.fcounts,
                              itr.sum_fact, itr.mean_count[0], itr.num_blacklist)
+        if det.with_bg:
+            if itr.bgscale is None:
+                itr.parse_scale('', bg=True)
+            eng.set_background(det.background, itr.bgscale)
         if itr.quat_mapping is not None:
             eng.set_refine(itr.num_coarse, itr.quat_mapping, itr.rel_off, itr.rel_quat)
         itr.model._bind(eng)

@@ -111,6 +111,15 @@ int emc_set_detector(emc_engine *e, int num_pix, const double *qvals, const doub
  *        (num_nonrot_modes) are not taken: the Cython engine's model holds num_modes
  *        volumes only. The mixed-precision and refinement paths take one mode.          */
 int emc_set_modes(emc_engine *e, int num_modes);
+/* detector background: det->background[num_pix] (caller's pixel order, detector.h:11-12,
+ *        CDetector.parse_background detector.pyx:101-104) and iter->bgscale[num_data]
+ *        (iterate.h:21, parse_scale(bg=True) iterate.pyx:284-315); the powder sum
+ *        (calc_powder, iterate.c:119-179) is computed from the frames. Call after the
+ *        detector and frames (setting either switches the background off again); NULL
+ *        switches it off. With need_scaling the iteration then runs the background-aware
+ *        likelihood, tomogram and scale updates (max_prob.c:134-156, max_update.c:232-543,
+ *        max_scale.c:21-288), in FP64 for sparse frames with one mode and no refinement. */
+int emc_set_background(emc_engine *e, const double *background, const double *bgscale);
 /* orientations: struct quaternion quats[num_rot][5] (dragonfly/src/quaternion.h:11-15) */
 int emc_set_quats(emc_engine *e, int num_rot, const double *quats);
 /* model: struct model model1[num_modes][size^3] (dragonfly/src/model.h:15-20,

@@ -27,6 +27,7 @@ class Problem(C.Structure):
         ('quat_mapping', _ip), ('rel_off', _lp), ('rel_quat', _ip),
         ('need_scaling', C.c_int), ('update_scale', C.c_int), ('iteration', C.c_int),
         ('known_scale', C.c_int), ('num_threads', C.c_int), ('num_modes', C.c_int),
+        ('with_bg', C.c_int), ('background', _dp), ('powder', _dp), ('bgscale', _dp),
     ]
 
 
@@ -65,6 +66,8 @@ def lib():
         L.orc_update_model_modes.argtypes = [_dp, _dp, _dp, C.c_long, C.c_double, C.c_int]
         L.orc_update_model_modes.restype = C.c_double
         L.orc_voronoi_subset.argtypes = [C.c_int, _dp, C.c_int, _dp, _ip]
+        L.orc_tomogram_bg_view.argtypes = [C.POINTER(Problem), _dp, C.c_double, C.c_double, _dp]
+        L.orc_calc_powder.argtypes = [C.c_int, _u8p, C.c_int, _ip, _ip, _lp, _lp, _ip, _ip, _ip, _dp]
         _lib = L
     return _lib
 
@@ -132,8 +135,20 @@ def voronoi_subset(qcoarse, qfine):
     return out
 
 
+def calc_powder(mask, frames):
+    """calc_powder (iterate.c:119-179) for sparse frames: mean photons per mask < 1 pixel."""
+    mask = np.ascontiguousarray(mask, dtype='u1')
+    a = [np.ascontiguousarray(frames[k], dtype=t) for k, t in (
+        ('ones', 'i4'), ('multi', 'i4'), ('ones_accum', 'i8'), ('multi_accum', 'i8'),
+        ('place_ones', 'i4'), ('place_multi', 'i4'), ('count_multi', 'i4'))]
+    out = np.zeros(len(mask))
+    lib().orc_calc_powder(len(mask), _p(mask, _u8p), len(a[0]), _p(a[0], _ip), _p(a[1], _ip), _p(a[2], _lp),
+                          _p(a[3], _lp), _p(a[4], _ip), _p(a[5], _ip), _p(a[6], _ip), _p(out))
+    return out
+
+
 def maximize(det, quats, model1, frames, fparams, flags, num_threads=1, dense_prob=False,
-             refine=None):
+             refine=None, background=None, bg_view=None):
     """Run one oracle iteration.
 
     det      : dict qvals (N,3) f8, corr (N,) f8, mask (N,) u1
@@ -147,6 +162,11 @@ def maximize(det, quats, model1, frames, fparams, flags, num_threads=1, dense_pr
     flags    : dict need_scaling, update_scale, iteration, known_scale
     refine   : None, or dict quat_mapping (R,) i4, rel_off (D+1,) i8, rel_quat i4 (frame d is
                evaluated only where quat_mapping[r] is in rel_quat[rel_off[d]:rel_off[d+1]])
+    background : None, or dict background (N,) f8 and bgscale (D,) f8 (a detector with a
+               background; powder from calc_powder); with need_scaling the likelihood, the
+               tomogram and the scale update take it into account
+    bg_view  : None, or dict pr (D,), psum_r, scalemax: instead of an iteration, return the
+               view update_tomogram_bg finds for one orientation with these inputs
     Returns a dict of outputs; fparams['scale'] is updated in place when scaling is on.
     """
     qvals = np.ascontiguousarray(det['qvals'], dtype='f8')
@@ -206,6 +226,17 @@ def maximize(det, quats, model1, frames, fparams, flags, num_threads=1, dense_pr
         prob.quat_mapping = _p(arr(refine['quat_mapping'], 'i4'), _ip)
         prob.rel_off = _p(arr(refine['rel_off'], 'i8'), _lp)
         prob.rel_quat = _p(arr(refine['rel_quat'], 'i4'), _ip)
+
+    if background is not None:
+        prob.with_bg = 1
+        prob.background = _p(arr(background['background'], 'f8'))
+        prob.bgscale = _p(arr(background['bgscale'], 'f8'))
+        prob.powder = _p(arr(calc_powder(mask, frames), 'f8'))
+    if bg_view is not None:
+        view = np.zeros(len(corr))
+        lib().orc_tomogram_bg_view(C.byref(prob), _p(arr(bg_view['pr'], 'f8')), float(bg_view['psum_r']),
+                                   float(bg_view['scalemax']), _p(view))
+        return view
 
     out = dict(u=np.zeros(R), max_exp=np.zeros(D), p_norm=np.zeros(D), rmax=np.zeros(D, 'i4'),
                likelihood=np.zeros(D), info=np.zeros(D), psum_d=np.zeros(D))

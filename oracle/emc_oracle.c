@@ -483,12 +483,352 @@ static int orc_keep_find(const orc_keep *k, int r) {
 
 static double orc_now(void) { return omp_get_wtime(); }
 
+/* ---- background-aware updates (need_scaling with a detector background) ---------------- */
+
+void orc_calc_powder(int num_pix, const uint8_t *mask, int num_data, const int *ones, const int *multi,
+                     const long *ones_accum, const long *multi_accum, const int *place_ones,
+                     const int *place_multi, const int *count_multi, double *powder) {
+    memset(powder, 0, num_pix * sizeof(double));
+    for (int d = 0; d < num_data; ++d) {
+        for (int k = 0; k < ones[d]; ++k) {
+            const int t = place_ones[ones_accum[d] + k];
+            if (mask[t] < 1) powder[t]++;
+        }
+        for (int k = 0; k < multi[d]; ++k) {
+            const int t = place_multi[multi_accum[d] + k];
+            if (mask[t] < 1) powder[t] += count_multi[multi_accum[d] + k];
+        }
+    }
+    for (int t = 0; t < num_pix; ++t) powder[t] /= num_data;
+}
+
+/* gradient_rt (max_update.c:232-341) for one orientation; pr[d] is the frame's posterior at
+ * it (0 where not kept). Pixel states: 0 to optimise, 1 no photon seen, 128 negligible
+ * background, 160 find the lower bound, 192 bracketed, 255 done. */
+static void orc_gradient_rt(const orc_problem *p, const double *pr, double psum_r, uint8_t *mask,
+                            const double *W, double *grad) {
+    for (int t = 0; t < p->num_pix; ++t) {
+        if (mask[t] < 128) {
+            grad[t] = -psum_r;
+            mask[t] = 1;
+        } else if (mask[t] == 160) {
+            grad[t] = -1e100;
+        } else {
+            grad[t] = 0.;
+        }
+    }
+    for (int d = 0; d < p->num_data; ++d) {
+        if (p->blacklist[d]) continue;
+        const double pv = pr[d];
+        if (!(pv > ORC_PROB_MIN)) continue;
+        const double sc = p->scale[d], bs = p->bgscale[d];
+        for (int k = 0; k < p->ones[d]; ++k) {
+            const int t = p->place_ones[p->ones_accum[d] + k];
+            if (mask[t] < 128) {
+                grad[t] += pv * sc / (W[t] * sc + bs * p->background[t]);
+                mask[t] = 0;
+            } else if (mask[t] == 128) {
+                grad[t] += pv;
+            } else if (mask[t] == 160) {
+                grad[t] = fmax(grad[t], sc / bs);
+            }
+        }
+        for (int k = 0; k < p->multi[d]; ++k) {
+            const int t = p->place_multi[p->multi_accum[d] + k];
+            const int c = p->count_multi[p->multi_accum[d] + k];
+            if (mask[t] < 128) {
+                grad[t] += pv * c * sc / (W[t] * sc + bs * p->background[t]);
+                mask[t] = 0;
+            } else if (mask[t] == 128) {
+                grad[t] += pv * c;
+            } else if (mask[t] == 160) {
+                grad[t] = fmax(grad[t], sc / bs);
+            }
+        }
+    }
+}
+
+/* update_tomogram_bg (max_update.c:343-543): per pixel, the root in W of the gradient of
+ * the background-aware log-likelihood, bracketed then found by Ridder's method. */
+static void orc_update_tomogram_bg(const orc_problem *p, const double *pr, double psum_r, double scalemax,
+                                   double *view) {
+    const int P = p->num_pix;
+    const double grad_tol = 1.e-4;
+    uint8_t *mask = malloc(P);
+    double *w = calloc(8 * (size_t)P, sizeof(double));
+    double *W_old = w, *W_new = w + P, *W_mid = w + 2 * P, *W_latest = w + 3 * P;
+    double *G_old = w + 4 * P, *G_new = w + 5 * P, *G_mid = w + 6 * P, *G_latest = w + 7 * P;
+    memset(view, 0, P * sizeof(double));
+    for (int t = 0; t < P; ++t) {
+        if (p->mask[t] < 2) {
+            if (p->background[t] / scalemax < 1.e-2 * p->powder[t]) {
+                mask[t] = 128;
+            } else {
+                W_new[t] = 1.e-8 - p->background[t] / scalemax;
+                mask[t] = 0;
+            }
+        } else {
+            mask[t] = 255;
+        }
+    }
+    orc_gradient_rt(p, pr, psum_r, mask, W_old, G_old);
+    for (int t = 0; t < P; ++t) {
+        if (fabs(G_old[t]) < grad_tol) {
+            view[t] = 0.;
+            mask[t] = 255;
+        } else if (G_old[t] > 0.) {
+            const double val = p->mean_count / P;
+            W_new[t] = p->powder[t] > val ? p->powder[t] : val;
+            W_new[t] *= 2.;
+        }
+    }
+    int i, nmask = 0;
+    for (i = 0; i < 10; ++i) {
+        nmask = 0;
+        orc_gradient_rt(p, pr, psum_r, mask, W_new, G_new);
+        for (int t = 0; t < P; ++t) {
+            if (mask[t] == 0) {
+                if (G_old[t] < 0. && G_new[t] < 0.) {
+                    mask[t] = 160;
+                } else if (G_old[t] > 0. && G_new[t] > 0.) {
+                    W_new[t] *= 2;
+                } else {
+                    mask[t] = 192;
+                    nmask++;
+                }
+            } else if (mask[t] == 160) {
+                W_new[t] = 1.e-8 - p->background[t] / G_new[t];
+                mask[t] = 0;
+            } else {
+                nmask++;
+            }
+        }
+        if (nmask == P) break;
+    }
+    for (i = 0; i < 50; ++i) {
+        nmask = 0;
+        for (int t = 0; t < P; ++t) {
+            if (mask[t] == 255) {
+                nmask++;
+            } else if (mask[t] == 1) {
+                mask[t] = 255;
+                view[t] = 0.;
+                nmask++;
+            } else if (mask[t] == 128) {
+                view[t] = G_old[t] / psum_r;
+                mask[t] = 255;
+                nmask++;
+            } else if (mask[t] == 192) {
+                mask[t] = 0;
+                W_mid[t] = 0.5 * (W_old[t] + W_new[t]);
+            } else if (mask[t] == 0) {
+                W_mid[t] = 0.5 * (W_old[t] + W_new[t]);
+            }
+        }
+        if (nmask == P) break;
+        orc_gradient_rt(p, pr, psum_r, mask, W_mid, G_mid);
+        for (int t = 0; t < P; ++t) {
+            if (mask[t] == 1) {
+                mask[t] = 255;
+                view[t] = 0.;
+                nmask++;
+            } else if (mask[t] == 0) {
+                if (fabs(G_mid[t]) < grad_tol) {
+                    view[t] = W_mid[t];
+                    mask[t] = 255;
+                    nmask++;
+                } else {
+                    const double sq = sqrt(G_mid[t] * G_mid[t] - G_old[t] * G_new[t]);
+                    if (sq < 1.e-15) {
+                        W_latest[t] = W_mid[t];
+                    } else {
+                        W_latest[t] = W_mid[t] + (W_mid[t] - W_old[t]) * copysign(1.0, G_old[t]) * G_mid[t] / sq;
+                        const double lo = fmin(W_old[t], W_new[t]), hi = fmax(W_old[t], W_new[t]);
+                        if (W_latest[t] <= lo || W_latest[t] >= hi) W_latest[t] = W_mid[t];
+                    }
+                }
+            }
+        }
+        if (nmask == P) break;
+        orc_gradient_rt(p, pr, psum_r, mask, W_latest, G_latest);
+        for (int t = 0; t < P; ++t) {
+            if (mask[t] == 1) {
+                mask[t] = 255;
+                view[t] = 0.;
+                nmask++;
+            } else if (mask[t] == 0) {
+                if (fabs(G_latest[t]) < grad_tol) {
+                    view[t] = W_latest[t];
+                    mask[t] = 255;
+                    nmask++;
+                } else if (G_mid[t] * G_latest[t] < 0.) {
+                    W_old[t] = W_mid[t];
+                    G_old[t] = G_mid[t];
+                    W_new[t] = W_latest[t];
+                    G_new[t] = G_latest[t];
+                } else if (G_old[t] * G_latest[t] < 0.) {
+                    W_new[t] = W_latest[t];
+                    G_new[t] = G_latest[t];
+                } else {
+                    W_old[t] = W_latest[t];
+                    G_old[t] = G_latest[t];
+                }
+            }
+        }
+        if (nmask == P) break;
+    }
+    free(mask);
+    free(w);
+}
+
+/* The view update_tomogram_bg finds for one orientation (pr: every frame's posterior at it,
+ * psum_r and scalemax as calc_psum_r gives them); for the tests' root checks. */
+void orc_tomogram_bg_view(const orc_problem *p, const double *pr, double psum_r, double scalemax, double *view) {
+    orc_update_tomogram_bg(p, pr, psum_r, scalemax, view);
+}
+
+/* gradient_d (max_scale.c:21-132): for every frame with mask 0, the derivative in its scale
+ * of the background-aware log-likelihood, summed over its posterior (P > 1e-6) and photons
+ * (mask < 1), minus psum_d. views[r] = slice_gen * rescale of combined orientation r. */
+static void orc_gradient_d(const orc_problem *p, int Rn, const double *views, const double *prob,
+                           const double *psum_d, const uint8_t *mask, const double *scale, double *grad) {
+    const int N = p->num_data, P = p->num_pix;
+    for (int d = 0; d < N; ++d)
+        if (mask[d] == 0) grad[d] = 0.;
+    for (int d = 0; d < N; ++d) {
+        if (mask[d] > 0) continue;
+        double acc = 0.;
+        for (int r = 0; r < Rn; ++r) {
+            const double pv = prob[(size_t)d * Rn + r];
+            if (!(pv > ORC_PROB_MIN)) continue;
+            const double *view = views + (size_t)r * P;
+            for (int k = 0; k < p->ones[d]; ++k) {
+                const int t = p->place_ones[p->ones_accum[d] + k];
+                if (p->mask[t] < 1)
+                    acc += pv * view[t] / (view[t] * scale[d] + p->bgscale[d] * p->background[t]);
+            }
+            for (int k = 0; k < p->multi[d]; ++k) {
+                const int t = p->place_multi[p->multi_accum[d] + k];
+                if (p->mask[t] < 1)
+                    acc += pv * p->count_multi[p->multi_accum[d] + k] * view[t] /
+                           (view[t] * scale[d] + p->bgscale[d] * p->background[t]);
+            }
+        }
+        grad[d] += acc;
+    }
+    for (int d = 0; d < N; ++d)
+        if (mask[d] == 0) grad[d] -= psum_d[d];
+}
+
+/* update_scale_bg (max_scale.c:134-288): per frame, the root in the scale of gradient_d,
+ * bracketed (with the 99% early stop over all frames) then found by Ridder's method. */
+static void orc_update_scale_bg(const orc_problem *p, int Rn, const double *views, const double *prob,
+                                const double *psum_d, double *scale) {
+    const int N = p->num_data;
+    double *w = calloc(8 * (size_t)N, sizeof(double));
+    double *s_old = w, *s_new = w + N, *s_mid = w + 2 * N, *s_latest = w + 3 * N;
+    double *G_old = w + 4 * N, *G_new = w + 5 * N, *G_mid = w + 6 * N, *G_latest = w + 7 * N;
+    uint8_t *mask = calloc(N, 1);
+    for (int d = 0; d < N; ++d) {
+        if (p->blacklist[d]) {
+            mask[d] = 255;
+            scale[d] = -1.;
+            continue;
+        }
+        s_new[d] = (p->iteration > 1 || p->known_scale) ? 4. * scale[d] : 32.;
+    }
+    orc_gradient_d(p, Rn, views, prob, psum_d, mask, s_old, G_old);
+    for (int d = 0; d < N; ++d)
+        if ((mask[d] != 255) & (G_old[d] < 1.e-6)) {
+            scale[d] = 0.001;
+            mask[d] = 255;
+        }
+    int i, num_mask = 0;
+    for (i = 0; i < 5; ++i) {
+        orc_gradient_d(p, Rn, views, prob, psum_d, mask, s_new, G_new);
+        num_mask = 0;
+        for (int d = 0; d < N; ++d) {
+            if (mask[d] != 0) {
+                num_mask++;
+            } else if (G_new[d] < 0.) {
+                mask[d] = 192;
+                num_mask++;
+            } else {
+                s_new[d] *= 4.;
+            }
+        }
+        if (num_mask > ((double)0.99 * N)) break;
+    }
+    for (i = 0; i < 50; ++i) {
+        num_mask = 0;
+        for (int d = 0; d < N; ++d) {
+            if (mask[d] == 255) {
+                num_mask++;
+            } else if (fabs(s_old[d] - s_new[d]) < 1.e-5) {
+                mask[d] = 255;
+                scale[d] = 0.5 * (s_old[d] + s_new[d]);
+                num_mask++;
+            } else if (mask[d] == 192) {
+                mask[d] = 0;
+                s_mid[d] = 0.5 * (s_old[d] + s_new[d]);
+            } else {
+                s_mid[d] = 0.5 * (s_old[d] + s_new[d]);
+            }
+        }
+        if (num_mask == N) break;
+        orc_gradient_d(p, Rn, views, prob, psum_d, mask, s_mid, G_mid);
+        for (int d = 0; d < N; ++d) {
+            if (mask[d] == 255) continue;
+            if (fabs(G_mid[d]) < 1.e-3) {
+                scale[d] = s_mid[d];
+                mask[d] = 255;
+                num_mask++;
+                continue;
+            }
+            const double sq = sqrt(G_mid[d] * G_mid[d] - G_old[d] * G_new[d]);
+            if (sq < 1.e-15) {
+                s_latest[d] = s_mid[d];
+            } else {
+                s_latest[d] = s_mid[d] + (s_mid[d] - s_old[d]) * copysign(1.0, G_old[d]) * G_mid[d] / sq;
+                const double lo = fmin(s_old[d], s_new[d]), hi = fmax(s_old[d], s_new[d]);
+                if (s_latest[d] <= lo || s_latest[d] >= hi) s_latest[d] = s_mid[d];
+            }
+        }
+        if (num_mask == N) break;
+        orc_gradient_d(p, Rn, views, prob, psum_d, mask, s_latest, G_latest);
+        for (int d = 0; d < N; ++d)
+            if (mask[d] < 255) {
+                if (fabs(G_latest[d]) < 1.e-3) {
+                    scale[d] = s_latest[d];
+                    mask[d] = 255;
+                    num_mask++;
+                } else if (G_mid[d] * G_latest[d] < 0.) {
+                    s_old[d] = s_mid[d];
+                    G_old[d] = G_mid[d];
+                    s_new[d] = s_latest[d];
+                    G_new[d] = G_latest[d];
+                } else if (G_old[d] * G_latest[d] < 0.) {
+                    s_new[d] = s_latest[d];
+                    G_new[d] = G_latest[d];
+                } else {
+                    s_old[d] = s_latest[d];
+                    G_old[d] = G_latest[d];
+                }
+            }
+        if (num_mask == N) break;
+    }
+    free(w);
+    free(mask);
+}
+
 double orc_maximize(const orc_problem *p, orc_result *res) {
     /* combined orientations r = rotind * M + mode (max_prob.c:80-84) */
     const int M = p->num_modes > 1 ? p->num_modes : 1;
     const int N = p->num_data, Rn = p->num_rot * M, P = p->num_pix;
     const long vol1 = p->size * p->size * p->size, vol = vol1 * M;
     const int want_psum_d = p->need_scaling && p->update_scale;
+    /* background-aware updates only with need_scaling (max_prob.c:94, max_update.c:11) */
+    const int bg_on = p->with_bg && p->need_scaling;
     double t0 = orc_now(), t1;
     int nthreads = p->num_threads > 0 ? p->num_threads : 1;
 
@@ -506,7 +846,10 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
         res->max_exp[d] = 0.;
         if (want_psum_d) res->psum_d[d] = 0.;
     }
-    if (res->prob_dense) memset(res->prob_dense, 0, (size_t)N * Rn * sizeof(double));
+    /* the background-aware scale update reads every frame's posterior after the merge */
+    double *pdense = res->prob_dense;
+    if (!pdense && bg_on && want_psum_d) pdense = malloc((size_t)N * Rn * sizeof(double));
+    if (pdense) memset(pdense, 0, (size_t)N * Rn * sizeof(double));
     if (res->quat_norm) memset(res->quat_norm, 0, (size_t)N * M * sizeof(double));
 
     /* calculate_rescale (max_prob.c:3-64) */
@@ -547,6 +890,7 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
         double *ppsum_d = calloc(N, sizeof(double));
         double *pnorm = calloc(N, sizeof(double));
         double *pqnorm = calloc((size_t)N * M, sizeof(double));
+        double *pr = bg_on ? calloc(N, sizeof(double)) : NULL;
         for (int d = 0; d < N; ++d) pmax[d] = ORC_MAX_EXP_START;
 
         /* calculate_prob (max_prob.c:66-213) */
@@ -555,7 +899,8 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
             const int rotind = r / M, mode = r % M;
             orc_slice_gen3d(&p->quats[rotind * 5], p->model1 + mode * vol1, p->size, P, p->qvals, p->corr,
                             view);
-            for (int t = 0; t < P; ++t) view[t] = log(view[t]);
+            if (!bg_on)
+                for (int t = 0; t < P; ++t) view[t] = log(view[t]);
             double logw = log(p->quats[rotind * 5 + 4]);
             for (int d = 0; d < N; ++d) {
                 if (p->blacklist[d]) continue;
@@ -570,7 +915,24 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
                     pval = logw - res->u[r] * p->scale[d];
                 else
                     pval = logw - res->u[r] * res->rescale;
-                if (p->ftype == 0) {
+                if (p->ftype == 0 && bg_on) {
+                    /* max_prob.c:134-156: log of scale * W + bgscale * background */
+                    const int *po = &p->place_ones[p->ones_accum[d]];
+                    for (int k = 0; k < p->ones[d]; ++k)
+                        if (p->mask[po[k]] < 1) {
+                            double ip = view[po[k]] * p->scale[d] + p->bgscale[d] * p->background[po[k]];
+                            if (ip < 0) ip = DBL_MIN;
+                            pval += log(ip);
+                        }
+                    const int *pm = &p->place_multi[p->multi_accum[d]];
+                    const int *cm = &p->count_multi[p->multi_accum[d]];
+                    for (int k = 0; k < p->multi[d]; ++k)
+                        if (p->mask[pm[k]] < 1) {
+                            double ip = view[pm[k]] * p->scale[d] + p->bgscale[d] * p->background[pm[k]];
+                            if (ip < 0) ip = DBL_MIN;
+                            pval += cm[k] * log(ip);
+                        }
+                } else if (p->ftype == 0) {
                     const int *po = &p->place_ones[p->ones_accum[d]];
                     for (int k = 0; k < p->ones[d]; ++k)
                         if (p->mask[po[k]] < 1) pval += view[po[k]];
@@ -619,6 +981,8 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
         for (int r = 0; r < Rn; ++r) {
             const int rotind = r / M, mode = r % M;
             double w_r = p->quats[rotind * 5 + 4], psum_r = 0.;
+            double scalemax = -DBL_MAX;  /* calc_psum_r's return value (max_update.c:50,120-121) */
+            if (pr) memset(pr, 0, N * sizeof(double));
             for (int d = 0; d < N; ++d) {
                 if (p->blacklist[d]) continue;
                 int ind = orc_keep_find(&keep[d], r);
@@ -633,10 +997,19 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
                 } else {
                     psum_r += prob;
                 }
-                if (res->prob_dense) res->prob_dense[(size_t)d * Rn + r] = prob;
+                if (pdense) pdense[(size_t)d * Rn + r] = prob;
+                if (pr) pr[d] = prob;
                 if (prob < ORC_PROB_MIN) continue;
                 if (M > 1) pqnorm[(size_t)d * M + mode] += prob;
                 pinfo[d] += prob * log(prob / w_r * M);
+                if (p->need_scaling && p->scale[d] > scalemax) scalemax = p->scale[d];
+            }
+            if (bg_on) {
+                orc_update_tomogram_bg(p, pr, psum_r, scalemax, view);
+                if (psum_r > 0.)
+                    orc_slice_merge3d(&p->quats[rotind * 5], view, pmodel + mode * vol1, pweight + mode * vol1,
+                                      p->size, P, p->qvals, p->corr, p->mask);
+                continue;
             }
             memset(view, 0, P * sizeof(double));
             for (int d = 0; d < N; ++d) {
@@ -690,7 +1063,7 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
             free(keep[d].pos);
         }
         free(keep); free(pmax); free(prmax); free(view); free(pmodel); free(pweight);
-        free(plike); free(pinfo); free(ppsum_d); free(pnorm); free(pqnorm);
+        free(plike); free(pinfo); free(ppsum_d); free(pnorm); free(pqnorm); free(pr);
     }
     double t2 = orc_now();
 
@@ -707,9 +1080,21 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
     res->avg_likelihood = avg;
 
     /* update_scale (max_scale.c:3-19) */
-    if (want_psum_d)
+    if (want_psum_d && p->with_bg) {
+        /* update_scale_bg takes the background whenever the detector has one (max_scale.c:9) */
+        double *views = malloc((size_t)Rn * P * sizeof(double));
+        for (int r = 0; r < Rn; ++r) {
+            orc_slice_gen3d(&p->quats[(r / M) * 5], p->model1 + (r % M) * vol1, p->size, P, p->qvals, p->corr,
+                            views + (size_t)r * P);
+            for (int t = 0; t < P; ++t) views[(size_t)r * P + t] *= res->rescale;
+        }
+        orc_update_scale_bg(p, Rn, views, pdense, res->psum_d, p->scale);
+        free(views);
+    } else if (want_psum_d) {
         for (int d = 0; d < N; ++d)
             if (!p->blacklist[d]) p->scale[d] = p->fcounts[d] / res->psum_d[d];
+    }
+    if (pdense != res->prob_dense) free(pdense);
     free(max_exp_p);
 
     res->seconds[1] = t_prob - t1;

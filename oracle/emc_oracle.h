@@ -59,6 +59,14 @@ typedef struct orc_problem {
      * r = rotation * num_modes + mode, r < num_rot * num_modes (max_prob.c:80-84); u,
      * prob_dense and rmax count combined orientations. */
     int num_modes;
+    /* background (dragonfly/src/detector.h:11-12, iterate.h:21): with_bg switches on the
+     * background-aware likelihood, tomogram and scale updates when need_scaling is set
+     * (max_prob.c:94,134-156, max_update.c:11, max_scale.c:9). background and powder are
+     * [num_pix] (powder from orc_calc_powder), bgscale [num_data]. */
+    int with_bg;
+    const double *background;
+    const double *powder;
+    const double *bgscale;
 } orc_problem;
 
 typedef struct orc_result {
@@ -90,7 +98,14 @@ void orc_slice_merge3d(const double *quat, const double *slice, double *model2,
                        double *weight, long size, int num_pix, const double *qvals,
                        const double *corr, const uint8_t *mask);
 void orc_symmetrize_friedel(double *array, const double *weights, int size);
+/* calc_powder (dragonfly/src/iterate.c:119-179), sparse frames: mean photon count per mask < 1
+ * pixel over all frames */
+void orc_calc_powder(int num_pix, const uint8_t *mask, int num_data, const int *ones, const int *multi,
+                     const long *ones_accum, const long *multi_accum, const int *place_ones,
+                     const int *place_multi, const int *count_multi, double *powder);
 double orc_maximize(const orc_problem *p, orc_result *res);
+/* update_tomogram_bg's view for one orientation (max_update.c:343-543), exposed for tests */
+void orc_tomogram_bg_view(const orc_problem *p, const double *pr, double psum_r, double scalemax, double *view);
 double orc_update_model(double *model1, double *model2, const double *inter_weight,
                         long size, double alpha);
 /* update_model over num_modes volumes: Friedel symmetry per mode, then divide, rms change and
