@@ -29,12 +29,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_recon(prob, cfg, device=0, dense=None, precision=0, start=None):
+def build_recon(prob, cfg, device=0, dense=None, precision=0, start=None, background=None):
     import dragonfly_amd as df
     from dragonfly_amd.iterate import Iterate
     det = prob['det']
     f = prob['frames']
     cdet = df.CDetector.from_arrays(det['qvals'], det['corr'], det['mask'], det['detd'], det['ewald_rad'])
+    if background is not None:
+        cdet.background = background
     if dense is not None:
         ds = df.CDataset.from_dense(cdet, dense)
     else:
@@ -62,7 +64,7 @@ def build_recon(prob, cfg, device=0, dense=None, precision=0, start=None):
     return recon, itr
 
 
-def cpu_baseline(prob, itr, num_rot_sample, threads):
+def cpu_baseline(prob, itr, num_rot_sample, threads, background=None):
     """Oracle (OpenMP restatement of the reference engine) on all frames and the first
     `num_rot_sample` orientations; pairs/s scales linearly in the orientation count."""
     from oracle import core
@@ -74,7 +76,7 @@ def cpu_baseline(prob, itr, num_rot_sample, threads):
     t0 = time.perf_counter()
     core.maximize(prob['det'], quats, itr.model.model1 if itr.model.model1 is not None else None,
                   f, fp, dict(iteration=1, need_scaling=itr.params.need_scaling, update_scale=1),
-                  num_threads=threads)
+                  num_threads=threads, background=background)
     dt = time.perf_counter() - t0
     pairs = itr.tot_num_data * quats.shape[0]
     return pairs / dt, dt
@@ -522,6 +524,9 @@ def main():
                     help='plain workloads: pick kernel variants on the live workload before timing')
     ap.add_argument('--no-fp64', action='store_true',
                     help='dense-frame configurations: skip the FP64 comparison run')
+    ap.add_argument('--background', type=float, default=0.,
+                    help='detector background of this many photons per pixel (background-aware path, '
+                         'scale optimisation on)')
     ap.add_argument('--copy-calibration-mib', type=int, default=0,
                     help='run one device copy of this many MiB first (counter calibration)')
     args = ap.parse_args()
@@ -561,8 +566,17 @@ def main():
                               num_div=args.num_div, batch=1024)
     cfg = prob['cfg']
     log('generated in %.1f s' % (time.perf_counter() - t0))
+    bgd = None
+    if args.background > 0:
+        # a radial background profile of mean args.background photons per pixel, unit
+        # background scales; the background-aware updates run with the scale factors on
+        q = prob['det']['qvals']
+        rad = np.sqrt((q ** 2).sum(1))
+        prof = 1. + rad / rad.max()
+        bgd = args.background * prof / prof.mean()
+        cfg = dict(cfg, need_scaling=1)
     # host copy of the start model for the CPU baseline before the GPU changes it
-    recon, itr = build_recon(prob, cfg)
+    recon, itr = build_recon(prob, cfg, background=bgd)
     start_model = itr.model.model1.copy()
     tuned = None
     if args.autotune:
@@ -597,13 +611,14 @@ def main():
     value = pairs * args.steps / elapsed
 
     default_size = args.num_data is None and args.num_div is None
-    roof = roofline(prob, itr, stage_tot, args.steps, nchunks, pairs,
-                    args.config if default_size else None)
+    roof = None if bgd is not None else roofline(prob, itr, stage_tot, args.steps, nchunks, pairs,
+                                                 args.config if default_size else None)
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         itr.model.model1 = start_model
-        v_cpu, dt = cpu_baseline(prob, itr, args.cpu_rot, threads)
+        v_cpu, dt = cpu_baseline(prob, itr, args.cpu_rot, threads,
+                                 None if bgd is None else dict(background=bgd, bgscale=np.ones(D)))
         cpu = dict(value=round(v_cpu, 1), unit='frame×orientation pairs/s', cores=threads,
                    kind='port', sample='iteration 1 of the oracle engine (OpenMP restatement of the '
                    'reference) on all %d frames x first %d orientations: %.1f s' % (D, args.cpu_rot, dt))
@@ -615,7 +630,9 @@ def main():
         'config': {'workload': '%s: %d frames, %d orientations (num_div %d), %d^3 model, %d pixels, '
                                'need_scaling %d' % (args.config, D, R, cfg['num_div'], cfg['size'],
                                                     len(mask), cfg['need_scaling']),
-                   'path': 'plain FP64', 'frames': D, 'num_rot': R, 'num_pix': int(len(mask)),
+                   'path': 'plain FP64' if bgd is None else
+                           'background-aware FP64 (background %.3g photons/pixel)' % args.background,
+                   'frames': D, 'num_rot': R, 'num_pix': int(len(mask)),
                    'mean_photon_pixels': round(float(nbar), 2)},
         'stage_ms_mean': {k: round(v / args.steps, 2) for k, v in stage_tot.items()},
         'roofline': roof,
@@ -623,6 +640,9 @@ def main():
     }
     if tuned is not None:
         out['autotune'] = tuned
+    if bgd is not None:
+        out['roofline_note'] = ('background-aware path: the tomogram is a per-(pixel, orientation) root '
+                                'search whose gradient passes depend on the data; see the kernel trace')
     print(json.dumps(out), flush=True)
 
 

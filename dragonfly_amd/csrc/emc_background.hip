@@ -114,8 +114,8 @@ __device__ __forceinline__ double gradient_rt(const PixelPhotons &ph, const doub
 
 }  // namespace
 
-// kFirst: run the bracket search over all 10 rounds and record, per orientation, the last
-// round the reference runs (atomicMax of each pixel's first settled round into last_round).
+// kFirst: run the bracket search until the pixel is settled and record, per orientation, the
+// last round the reference runs (atomicMax of each pixel's first settled round into last_round).
 // Otherwise: solve with the recorded rounds and write the view, orientation-major
 // (view[c * num_pix + t]) when rmajor, else in 64-wide tiles.
 template <bool kFirst>
@@ -165,9 +165,13 @@ __global__ __launch_bounds__(kBlock) void tomogram_bg_kernel(
     // bracket search: rounds 0 .. last (the reference stops after the round in which every
     // pixel of the orientation was settled, or after 10)
     const int last = kFirst ? 9 : (on ? min(last_round[r], 9) : -1);
-    int settled = 10;  // first round in which this pixel counts as settled
+    // first round in which this pixel counts as settled; states other than 0 (and 160, not
+    // reachable yet) count from round 0. A settled pixel's later rounds change nothing it
+    // uses again except a bracketed pixel's gradient, which they reset to 0 (below), so
+    // they are skipped.
+    int settled = (mask == kOpt) ? 10 : 0;
     for (int i = 0; i < 10; ++i) {
-        const bool act = on && i <= last;
+        const bool act = on && i <= last && settled == 10;
         if (!__any(act)) break;
         if (!act) continue;
         g_new = gradient_rt(ph, prow, scale, bgscale, bgt, psum, w_new, mask);
@@ -193,6 +197,9 @@ __global__ __launch_bounds__(kBlock) void tomogram_bg_kernel(
         if (on) atomicMax(&last_round[r], settled);
         return;
     }
+    // gradient_rt gives 0 for a bracketed pixel: a round of its orientation after the one that
+    // bracketed it left that in g_new
+    if (on && mask == kBracketed && settled < last) g_new = 0.;
     // Ridder's method
     for (int i = 0; i < 50; ++i) {
         if (!__any(on && mask != kDone && mask != kLowBound)) break;
