@@ -14,7 +14,7 @@ and keeps the fastest value before moving to the next option (greedy, in the ord
 # (8 ranges 561.6 ms, 4 ranges 450.4 ms, off 401.3 ms) and each value re-chunks the
 # orientations; add ('lik_split', (1, 8, 4), ('likelihood',)) to try it elsewhere.
 PLAIN_SPACE = (
-    ('lik_xcd', (1, 0), ('likelihood',)),
+    ('lik_xcd', (1, 2, 0), ('likelihood',)),
     ('lik_unroll', (8, 16, 4), ('likelihood',)),
     ('lik_tiles', (1, 2), ('likelihood',)),
     ('tomo_unroll', (16, 8), ('tomogram',)),

@@ -1279,7 +1279,7 @@ struct EngineOption {
 const EngineOption kOptions[] = {
     {"lik_tiles", &emc_engine::lik_tiles, 1, 4, false},
     {"lik_unroll", &emc_engine::lik_unroll, 4, 16, false},
-    {"lik_xcd", &emc_engine::lik_xcd, 0, 1, false},
+    {"lik_xcd", &emc_engine::lik_xcd, 0, 8, false},
     {"lik_split", &emc_engine::lik_split, 1, 8, false},
     {"tomo_tiles", &emc_engine::tomo_tiles, 1, 4, false},
     {"tomo_tile_mode", &emc_engine::tomo_tile_mode, 0, 2, false},

@@ -158,14 +158,16 @@ __global__ __launch_bounds__(kBlock) void likelihood_kernel(
     const int lane = threadIdx.x & 63;
     int fg = blockIdx.x, tb = blockIdx.y;
     if (xcd_order) {
-        // 1-D grid, XCD-aware: workgroups go round-robin to the kNumXcd XCDs, so XCD x takes
-        // orientation tiles x, x + kNumXcd, ... one at a time with all frames, and its L2 holds
-        // the rows of one log W tile instead of sharing one tile with every other XCD
-        const int nfg = (num_data + kWavesPerBlock - 1) / kWavesPerBlock;
+        // 1-D grid, XCD-aware: workgroups go round-robin to the kNumXcd XCDs. The XCDs form
+        // groups of G = xcd_order (1, 2, 4 or 8); group g takes orientation tiles
+        // g, g + kNumXcd / G, ... one at a time, its G members splitting the frames, so each
+        // L2 sees the rows of one log W tile and kNumXcd / G tiles are live chip-wide
+        const int G = xcd_order, ngrp = kNumXcd / G;
+        const int nfg = (num_data + kWavesPerBlock - 1) / kWavesPerBlock, nfgG = (nfg + G - 1) / G;
         const int x = blockIdx.x % kNumXcd, j = blockIdx.x / kNumXcd;
-        tb = x + kNumXcd * (j / nfg);
-        fg = j % nfg;
-        if (tb * (kRTile * NT) >= C) return;
+        tb = x / G + ngrp * (j / nfgG);
+        fg = (j % nfgG) * G + x % G;
+        if (tb * (kRTile * NT) >= C || fg >= nfg) return;
     }
     const int d = fg * kWavesPerBlock + wave;
     if (d >= num_data) return;
@@ -1207,14 +1209,17 @@ static void likelihood_launch(hipStream_t s, const T *logw, int C, int num_pix, 
                               int xcd_order = 0) {
     const unsigned gx = cdiv(num_data, kWavesPerBlock);
     const size_t tsW = (size_t)num_pix * kRTile, tsG = (size_t)num_data * kRTile;
+    /* XCDs sharing a tile in the 1-D order: 1, 2, 4 or 8 (0: 2-D grid) */
+    const int xg = xcd_order >= 8 ? 8 : xcd_order >= 4 ? 4 : xcd_order >= 2 ? 2 : (xcd_order > 0 ? 1 : 0);
 #define EMC_LIK(NT, U)                                                                           \
     do {                                                                                         \
-        const size_t gy = cdiv(C, kRTile * NT), gy8 = (gy + kNumXcd - 1) / kNumXcd * kNumXcd;    \
+        const size_t gy = cdiv(C, kRTile * NT);                                                  \
+        const size_t nblk = xg ? (size_t)kNumXcd * cdiv(gx, xg) * cdiv(gy, kNumXcd / xg) : 0;    \
         /* the 1-D order needs its work-item count to fit the 32-bit grid size */                \
-        const int xo = xcd_order && (size_t)gx * gy8 * kBlock < ((size_t)1 << 32);               \
-        const dim3 grid = xo ? dim3((unsigned)(gx * gy8)) : dim3(gx, (unsigned)gy);              \
+        const int xo = xg && nblk * kBlock < ((size_t)1 << 32);                                  \
+        const dim3 grid = xo ? dim3((unsigned)nblk) : dim3(gx, (unsigned)gy);                    \
         likelihood_kernel<NT, U, T><<<grid, kBlock, 0, s>>>(logw, C, tsW, off, pix, cnt, blacklist, num_data, G, \
-                                                            tsG, r0, fmap, xo);                 \
+                                                            tsG, r0, fmap, xo ? xg : 0);        \
     } while (0)
     if (tiles >= 4) EMC_LIK(4, 2);
     else if (tiles == 2) EMC_LIK(2, 4);

@@ -154,8 +154,8 @@ int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const
 int emc_set_precision(emc_engine *e, int mode);
 /* kernel-variant options, no counterpart in the reference (results are the same to the
  * parity tolerance; only speed changes): lik_tiles (1-4 orientation tiles per likelihood
- * wave), lik_unroll (4/8/16 rows in flight), lik_xcd (1: XCD-aware tile order of the
- * likelihood launch), lik_split (2-8: photon lists split into pixel ranges, one per XCD
+ * wave), lik_unroll (4/8/16 rows in flight), lik_xcd (XCD-aware tile order of the
+ * likelihood launch: 1, 2, 4 or 8 XCDs share a log W tile; 0 off), lik_split (2-8: photon lists split into pixel ranges, one per XCD
  * group, partial sums added in range order), tomo_tiles, tomo_tile_mode (0 per-pixel gather,
  * 1 LDS tiles, 2 frame-partitioned, default), tomo_parts (frame ranges, 1-64), tomo_width32,
  * tomo_unroll (8/16), tomo_seq, max_chunk_cols (0 = by memory), merge_mode (0 box, 1 brick),

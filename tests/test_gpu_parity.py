@@ -443,8 +443,15 @@ def test_options_roundtrip_and_variants_agree(tiny):
     eng.set_option('lik_split', 4)
     eng.set_option('max_chunk_cols', 128)
     g3 = P.run_engine(eng, flags)
+    # XCDs sharing a log W tile in the likelihood's 1-D order (2, 4, 8), chunked
+    eng.set_option('lik_split', 1)
+    gx = []
+    for v, tiles in ((2, 1), (4, 2), (8, 1)):
+        eng.set_option('lik_xcd', v)
+        eng.set_option('lik_tiles', tiles)
+        gx.append(P.run_engine(eng, flags))
     eng.close()
-    for g in (g2, g3):
+    for g in [g2, g3] + gx:
         for k in FIELDS_F:
             assert P.rel_err(g[k], g1[k]) <= TOL, k
         assert np.array_equal(g1['rmax'], g['rmax'])
