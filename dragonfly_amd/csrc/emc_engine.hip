@@ -291,7 +291,7 @@ struct emc_engine {
     std::vector<int> h_pframe, h_ptile, h_tpair, h_ffull, h_fof, h_ftom_frame;
     std::vector<double> h_ftom_cnt;
     long num_pairs = 0;
-    DevBuf<int> slot_rot, rot_slot, pair_frame, pair_tile, tile_pair, rbad_slot;
+    DevBuf<int> slot_rot, rot_slot, pair_frame, pair_tile, tile_pair, tile_frame, rbad_slot;
     DevBuf<long> pair_off, tile_off;
     DevBuf<double> rots_slot, quats_slot, logwr_slot, u_slot, psum_slot, Rsub;
     DevBuf<float> Rsub32;  // FP32 copy of the pair rows' posterior (FP32 gathers)
@@ -782,6 +782,14 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
     if ((rc = e->pair_tile.upload(ptile.data(), ptile.size(), s))) return rc;
     if ((rc = e->tile_off.upload(toff.data(), toff.size(), s))) return rc;
     if ((rc = e->tile_pair.upload(tpair.data(), tpair.size(), s))) return rc;
+    {
+        // frame of every entry of the tile-major list, so the pair tomogram reads it with the
+        // pair index instead of after it
+        std::vector<int> tframe(tpair.size());
+        for (size_t j = 0; j < tpair.size(); ++j) tframe[j] = pframe[tpair[j]];
+        if ((rc = e->tile_frame.upload(tframe.data(), tframe.size(), s))) return rc;
+        HIP_TRY(hipStreamSynchronize(s));
+    }
     if ((rc = e->ffull.upload(ffull.data(), ffull.size(), s))) return rc;
     if (!fidx.empty() && (rc = e->fidx.upload(fidx.data(), fidx.size(), s))) return rc;
     HIP_TRY(hipStreamSynchronize(s));
@@ -1232,12 +1240,12 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
         // partial frames' pairs added to the full frames' partial sums, divided by psum
         if ((rc = timed(7, [&] {
                  if (g32)
-                     emc::launch_pair_tomogram(s, e->Rsub32.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p,
+                     emc::launch_pair_tomogram(s, e->Rsub32.p, e->tile_off.p, e->tile_pair.p, e->tile_frame.p,
                                                e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
                                                e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
                                                nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p);
                  else
-                     emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p,
+                     emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->tile_frame.p,
                                                e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
                                                e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
                                                nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p);

@@ -183,11 +183,11 @@ void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, cons
                       const int *pair_frame, int num_tiles, const double *scale, int need_scaling,
                       const double *psum_full, double *psum_slot);
 void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
-                          const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
+                          const int *tile_frame, const long *foff, const int *roff, int nranges, const int *fpix,
                           const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
                           const double *partial, int nparts, double *view);
 void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *tile_pair,
-                          const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
+                          const int *tile_frame, const long *foff, const int *roff, int nranges, const int *fpix,
                           const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
                           const double *partial, int nparts, double *view);
 void launch_pair_to_f32(hipStream_t s, const double *P, long n, float *out);

@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void pair_psum_kernel(const double *__restr
 template <typename TP>
 __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     const TP *__restrict__ P, const long *__restrict__ tile_off, const int *__restrict__ tile_pair,
-    const int *__restrict__ pair_frame, const long *__restrict__ foff, const int *__restrict__ roff,
+    const int *__restrict__ tile_frame, const long *__restrict__ foff, const int *__restrict__ roff,
     int nranges, const int *__restrict__ fpix, const double *__restrict__ fcnt,
     const double *__restrict__ psum_slot, int t0, int ntiles, int num_pix,
     const double *__restrict__ partial, int nparts, double *__restrict__ view) {
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
         const long j = jb + tid;
         if (j < j1) {
             const int k = tile_pair[j];
-            const int d = pair_frame[k];
+            const int d = tile_frame[j];
             const int *__restrict__ rr = roff + (size_t)d * (nranges + 1);
             const int q0 = rr[rg], q1 = rr[rg + 1];
             if (q0 < q1) {
@@ -347,10 +347,29 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
         }
         __syncthreads();
         const int n = s_n;
-        for (int i = wave; i < n; i += kWavesPerBlock) {
-            const double p = (double)P[(size_t)s_k[i] * kRTile + lane];
-            if (!(p > kProbMin)) continue;
-            for (long q = s_q0[i]; q < s_q1[i]; ++q) unsafeAtomicAdd(&acc[fpix[q] - p0][lane], fcnt[q] * p);
+        // the P rows and first photons (a frame has about one photon in a 128-pixel range) of
+        // kQueueRows queue entries are loaded before any is added, so a wave has that many
+        // reads in flight instead of one chain of dependent ones
+        constexpr int kQueueRows = 8;
+        for (int i0 = wave; i0 < n; i0 += kWavesPerBlock * kQueueRows) {
+            double p[kQueueRows], c0[kQueueRows];
+            int px0[kQueueRows];
+            long qa[kQueueRows], qb[kQueueRows];
+#pragma unroll
+            for (int u = 0; u < kQueueRows; ++u) {
+                const int i = i0 + u * kWavesPerBlock;
+                qa[u] = i < n ? s_q0[i] : 0;
+                qb[u] = i < n ? s_q1[i] : 0;
+                p[u] = i < n ? (double)P[(size_t)s_k[i] * kRTile + lane] : 0.;
+                px0[u] = i < n ? fpix[qa[u]] : 0;
+                c0[u] = i < n ? fcnt[qa[u]] : 0.;
+            }
+#pragma unroll
+            for (int u = 0; u < kQueueRows; ++u) {
+                if (!(p[u] > kProbMin)) continue;  // also the entries past the queue (p = 0)
+                unsafeAtomicAdd(&acc[px0[u] - p0][lane], c0[u] * p[u]);
+                for (long q = qa[u] + 1; q < qb[u]; ++q) unsafeAtomicAdd(&acc[fpix[q] - p0][lane], fcnt[q] * p[u]);
+            }
         }
         __syncthreads();  // every wave has read the queue before it is refilled
     }
@@ -508,28 +527,28 @@ void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, cons
 
 template <typename TP>
 static void pair_tomogram_launch(hipStream_t s, const TP *P, const long *tile_off, const int *tile_pair,
-                                 const int *pair_frame, const long *foff, const int *roff, int nranges,
+                                 const int *tile_frame, const long *foff, const int *roff, int nranges,
                                  const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
                                  int num_pix, const double *partial, int nparts, double *view) {
     if (ntiles <= 0) return;
     const unsigned grid = (unsigned)((ntiles + kNumXcd - 1) / kNumXcd * kNumXcd) * (unsigned)nranges;
-    pair_tomogram_kernel<TP><<<grid, kBlock, 0, s>>>(P, tile_off, tile_pair, pair_frame, foff, roff, nranges, fpix,
+    pair_tomogram_kernel<TP><<<grid, kBlock, 0, s>>>(P, tile_off, tile_pair, tile_frame, foff, roff, nranges, fpix,
                                                      fcnt, psum_slot, t0, ntiles, num_pix, partial, nparts, view);
 }
 
 void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
-                          const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
+                          const int *tile_frame, const long *foff, const int *roff, int nranges, const int *fpix,
                           const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
                           const double *partial, int nparts, double *view) {
-    pair_tomogram_launch(s, P, tile_off, tile_pair, pair_frame, foff, roff, nranges, fpix, fcnt, psum_slot, t0,
+    pair_tomogram_launch(s, P, tile_off, tile_pair, tile_frame, foff, roff, nranges, fpix, fcnt, psum_slot, t0,
                          ntiles, num_pix, partial, nparts, view);
 }
 
 void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *tile_pair,
-                          const int *pair_frame, const long *foff, const int *roff, int nranges, const int *fpix,
+                          const int *tile_frame, const long *foff, const int *roff, int nranges, const int *fpix,
                           const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
                           const double *partial, int nparts, double *view) {
-    pair_tomogram_launch(s, P, tile_off, tile_pair, pair_frame, foff, roff, nranges, fpix, fcnt, psum_slot, t0,
+    pair_tomogram_launch(s, P, tile_off, tile_pair, tile_frame, foff, roff, nranges, fpix, fcnt, psum_slot, t0,
                          ntiles, num_pix, partial, nparts, view);
 }
 
