@@ -155,24 +155,47 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
         }
         head += count;
     };
-    for (int k = u.y + wave; k < u.z; k += kWavesPerBlock) {
-        const int item = list[k];
-        const int c = item / npatch, p = item % npatch;
-        const int r = r0 + c;
-        if (!all_on && !(psum_r[r] > 0.)) continue;
-        const int t = p * 64 + lane;
-        double w = 0.;  // the view; divided by corr when its entry is drained
-        if (t < num_pix && mask[t] <= 1) w = view[(size_t)c * num_pix + t];
-        const bool nz = w != 0.;  // a zero view adds exactly nothing
-        const unsigned long long bal = __ballot(nz);
-        if (nz) {
-            const int at = (tail + __popcll(bal & ((1ull << lane) - 1))) & (kRing - 1);
-            q_r[wave][at] = r;
-            q_t[wave][at] = t;
-            q_w[wave][at] = w;
+    // items are taken kItemsAhead at a time: their ids, then their views and masks, are loaded
+    // before the first is queued, so a wave has that many independent reads in flight
+    constexpr int kItemsAhead = 4;
+    for (int k0 = u.y + wave; k0 < u.z; k0 += kWavesPerBlock * kItemsAhead) {
+        int it[kItemsAhead];
+#pragma unroll
+        for (int a = 0; a < kItemsAhead; ++a) {
+            const int k = k0 + a * kWavesPerBlock;
+            it[a] = k < u.z ? list[k] : -1;
         }
-        tail += __popcll(bal);
-        if (tail - head >= 64) drain(64);
+        double wv[kItemsAhead];
+        int rr[kItemsAhead];
+#pragma unroll
+        for (int a = 0; a < kItemsAhead; ++a) {
+            wv[a] = 0.;  // the view; divided by corr when its entry is drained
+            rr[a] = -1;
+            if (it[a] < 0) continue;
+            const int c = it[a] / npatch, p = it[a] % npatch;
+            const int r = r0 + c, t = p * 64 + lane;
+            if (!all_on && !(psum_r[r] > 0.)) continue;
+            rr[a] = r;
+            if (t < num_pix) {
+                const double v = view[(size_t)c * num_pix + t];
+                wv[a] = mask[t] <= 1 ? v : 0.;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < kItemsAhead; ++a) {
+            if (rr[a] < 0) continue;
+            const int t = (it[a] % npatch) * 64 + lane;
+            const bool nz = wv[a] != 0.;  // a zero view adds exactly nothing
+            const unsigned long long bal = __ballot(nz);
+            if (nz) {
+                const int at = (tail + __popcll(bal & ((1ull << lane) - 1))) & (kRing - 1);
+                q_r[wave][at] = rr[a];
+                q_t[wave][at] = t;
+                q_w[wave][at] = wv[a];
+            }
+            tail += __popcll(bal);
+            if (tail - head >= 64) drain(64);
+        }
     }
     if (tail > head) drain(tail - head);
     __syncthreads();
