@@ -30,7 +30,7 @@ constexpr int kGemmGroupM = 8;
 constexpr int kBrick = 16;
 constexpr int kUnitItems = 2048;
 // Sparse subsets: pixels per LDS range of the pair tomogram.
-constexpr int kPairRange = 128;
+constexpr int kPairRange = 64;
 
 // Reference constants (dragonfly/src/max_internal.h:6-8).
 constexpr double kProbMin = 1.e-6;
