@@ -157,7 +157,7 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
     };
     // items are taken kItemsAhead at a time: their ids, then their views and masks, are loaded
     // before the first is queued, so a wave has that many independent reads in flight
-    constexpr int kItemsAhead = 8;
+    constexpr int kItemsAhead = 4;
     for (int k0 = u.y + wave; k0 < u.z; k0 += kWavesPerBlock * kItemsAhead) {
         int it[kItemsAhead];
 #pragma unroll
