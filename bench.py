@@ -260,7 +260,11 @@ def bench_learned(args):
     log('generating %s workload ...' % args.config)
     prob = synth.make_problem(args.config, device='cuda', num_data=args.num_data, num_div=args.num_div,
                               batch=1024)
-    cfg = prob['cfg']
+    cfg = dict(prob['cfg'])
+    if args.top_k:
+        cfg['top_k'] = args.top_k
+    if args.mass_threshold:
+        cfg['mass_threshold'] = args.mass_threshold
     det, mask = prob['det'], prob['det']['mask']
     true_model = np.ascontiguousarray(prob['intens'] * prob['frames']['rescale'])
     beta_factor = float(cfg.get('beta_factor', 1.))
@@ -518,6 +522,9 @@ def main():
                     'default of the learned-proposal iteration; 0 there: FP64)')
     ap.add_argument('--epochs', type=int, default=80, help='learned-proposal workload: training epochs')
     ap.add_argument('--wd', type=float, default=0., help='learned-proposal workload: L2 weight decay')
+    ap.add_argument('--top-k', type=int, default=0, help='learned-proposal workload: cells proposed per frame')
+    ap.add_argument('--mass-threshold', type=float, default=0.,
+                    help='learned-proposal workload: predicted mass below which a frame falls back')
     ap.add_argument('--compare-frames', type=int, default=50000,
                     help='uneven-batching workload: frames of the sparse vs masked comparison')
     ap.add_argument('--autotune', action='store_true',
