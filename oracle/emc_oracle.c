@@ -31,6 +31,17 @@
  *                       rescale, likelihood and info (the log M term cancels the split),
  *                       model2 / M and the one-mode inter_weight in every mode, and
  *                       occupancies 1 / M; with modes made distinct, by the dense brute force.
+ *   background path     (with_bg and need_scaling) orc_calc_powder against a dense sum; the
+ *                       likelihood log(phi W + b B), posterior, likelihood and info against
+ *                       a dense numpy formulation; a zero background gives the background-free
+ *                       posterior and scales fcounts / psum_d; every view of
+ *                       orc_update_tomogram_bg is zero, the negligible-background ratio or a
+ *                       root (|G| < 1e-4) of the gradient, and every scale of
+ *                       orc_update_scale_bg a root (|G| < 1e-3) or a 1e-5 bracket of one
+ *                       (tests/test_oracle_background.py). Which root the bracketed search
+ *                       lands on is not pinned beyond that tolerance: the reference's own
+ *                       search moves by up to 4e-3 of the largest view under 1e-14
+ *                       perturbations of its inputs.
  */
 #include "emc_oracle.h"
 
