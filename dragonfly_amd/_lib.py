@@ -313,8 +313,7 @@ class Engine:
     def sparse_prob(self):
         """Sparse posterior of the last iteration (max_prob.c:295-326): (num_prob, place, prob)
         with num_prob (num_data,) i4 counts of P > 1e-6 and place (i4) / prob (f8) the
-        entries frame after frame, orientations ascending. Raises EMCError after an
-        iteration on refinement subsets."""
+        entries frame after frame, orientations ascending."""
         num = np.zeros(self.num_data, dtype='i4')
         tot = C.c_long(0)
         _check(lib().emc_get_sparse_prob(self._h, _ptr(num, _ip), 0, None, None, C.byref(tot)))
@@ -327,7 +326,7 @@ class Engine:
     def set_option(self, name, value):
         """Kernel-variant option (emc_set_option): lik_tiles, lik_unroll, lik_xcd, tomo_tiles,
         tomo_tile_mode, tomo_parts, tomo_width32, tomo_unroll, tomo_seq, max_chunk_cols,
-        merge_mode, subset, slot_pack, lik_split."""
+        merge_mode, subset, slot_pack, lik_split, sparse_rows."""
         _check(lib().emc_set_option(self._h, name.encode(), int(value)))
 
     def get_option(self, name):

@@ -274,6 +274,7 @@ struct emc_engine {
     // orientations, no padding per cell; a tile may then hold orientations of two or three
     // cells, and slots outside a partial frame's cells are masked (slot_cell, relbits)
     int slot_pack = 1, slot_pack_built = -1;
+    int sparse_rows = 0;  // frames per batch of the subset sparse posterior (0: 256 MB of rows)
     std::vector<int> h_ctile0, h_ctile1;  // packed: first and last tile of each cell
     DevBuf<int> slot_cell;
     // full frames (every coarse cell): rows tile-major in RF; photon lists of mask < 2 pixels
@@ -1299,6 +1300,7 @@ const EngineOption kOptions[] = {
     {"merge_mode", &emc_engine::merge_mode, 0, 1, false},
     {"subset", &emc_engine::subset_mode, 0, 1, false},
     {"slot_pack", &emc_engine::slot_pack, 0, 1, false},
+    {"sparse_rows", &emc_engine::sparse_rows, 0, 1 << 30, false},
 };
 
 int set_option(emc_engine *e, const EngineOption &o, int value) {
@@ -2229,14 +2231,79 @@ int emc_slice_merge(emc_engine *e, const double *quat, const double *view) {
     return 0;
 }
 
+// emc_get_sparse_prob after an iteration on refinement subsets: frames in batches of at most
+// 256 MB of dense rows, each scattered from its pair rows (and, for full frames, its RF rows)
+// into orientation order and compacted like the dense posterior.
+static int sparse_prob_subset(emc_engine *e, int *num_prob, long capacity, int *place, double *prob,
+                              long *total) {
+    hipStream_t s = e->stream;
+    const int N = e->num_data, Rn = e->num_rot;
+    const long rowmax = e->sparse_rows > 0 ? (long)e->sparse_rows
+                                           : (long)(((size_t)256 << 20) / ((size_t)Rn * sizeof(double)));
+    const int nb = (int)std::max(1L, std::min((long)N, rowmax));
+    int rc;
+    DevBuf<double> dense, d_prob;
+    DevBuf<int> cnt, d_place;
+    DevBuf<long> d_off;
+    if ((rc = dense.alloc((size_t)nb * Rn))) return rc;
+    if ((rc = cnt.alloc(nb))) return rc;
+    const std::vector<long> &poff = e->h_poff;
+    const std::vector<int> &fidx = e->h_fidx;
+    auto scatter = [&](int d0, int d1) {
+        emc::launch_fill(s, dense.p, (size_t)(d1 - d0) * Rn, 0.);
+        emc::launch_pair_scatter(s, e->Rsub.p, e->pair_frame.p, e->pair_tile.p, e->slot_rot.p, poff[d1] - poff[d0],
+                                 Rn, dense.p, poff[d0], d0);
+        const int f0 = (int)(std::lower_bound(fidx.begin(), fidx.end(), d0) - fidx.begin());
+        const int f1 = (int)(std::lower_bound(fidx.begin(), fidx.end(), d1) - fidx.begin());
+        if (f1 > f0)
+            emc::launch_full_scatter(s, e->RF.p, e->num_full, e->num_tiles, e->fidx.p, e->slot_rot.p, Rn, dense.p,
+                                     f0, f1, d0);
+    };
+    long tot = 0;
+    for (int d0 = 0; d0 < N; d0 += nb) {
+        const int d1 = std::min(N, d0 + nb);
+        scatter(d0, d1);
+        emc::launch_sparse_count_rows(s, dense.p, Rn, d1 - d0, cnt.p);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(num_prob + d0, cnt.p, (size_t)(d1 - d0) * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int d = d0; d < d1; ++d) tot += num_prob[d];
+    }
+    if (total) *total = tot;
+    if (!place || !prob) return 0;
+    if (capacity < tot) return fail(EMC_E_ARG, "emc_get_sparse_prob: capacity %ld < %ld entries", capacity, tot);
+    std::vector<long> off;
+    long done = 0;
+    for (int d0 = 0; d0 < N; d0 += nb) {
+        const int d1 = std::min(N, d0 + nb);
+        off.assign(d1 - d0, 0);
+        long n = 0;
+        for (int d = d0; d < d1; ++d) {
+            off[d - d0] = n;
+            n += num_prob[d];
+        }
+        if (n > 0) {
+            scatter(d0, d1);
+            if ((rc = d_off.upload(off.data(), off.size(), s))) return rc;
+            if ((rc = d_place.alloc(n))) return rc;
+            if ((rc = d_prob.alloc(n))) return rc;
+            emc::launch_sparse_fill_rows(s, dense.p, Rn, d1 - d0, d_off.p, d_place.p, d_prob.p);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(place + done, d_place.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(prob + done, d_prob.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        done += n;
+    }
+    return 0;
+}
+
 int emc_get_sparse_prob(emc_engine *e, int *num_prob, long capacity, int *place, double *prob,
                         long *total) {
     if (!e || !num_prob) return fail(EMC_E_ARG, "emc_get_sparse_prob: bad argument");
     if (!e->have_results) return fail(EMC_E_STATE, "emc_get_sparse_prob: no iteration has run");
-    if (e->subset_ran)
-        return fail(EMC_E_STATE, "emc_get_sparse_prob: the last iteration ran on refinement subsets "
-                                 "(read EMC_FIELD_PROB)");
     HIP_TRY(hipSetDevice(e->device));
+    if (e->subset_ran) return sparse_prob_subset(e, num_prob, capacity, place, prob, total);
     hipStream_t s = e->stream;
     const int N = e->num_data, Rn = e->num_rot;
     int rc;

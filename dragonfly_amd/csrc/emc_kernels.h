@@ -75,6 +75,10 @@ void launch_gradient_d(hipStream_t s, const double *W, int num_pix, const double
 void launch_sparse_count(hipStream_t s, const double *P, int num_rot, int num_data, int *counts);
 void launch_sparse_fill(hipStream_t s, const double *P, int num_rot, int num_data, int d0, int nd,
                         const long *off, int *place, double *prob);
+// the same over row-major rows P[row * num_rot + r]
+void launch_sparse_count_rows(hipStream_t s, const double *P, int num_rot, int nrows, int *counts);
+void launch_sparse_fill_rows(hipStream_t s, const double *P, int num_rot, int nrows, const long *off, int *place,
+                             double *prob);
 void launch_psum_r(hipStream_t s, const double *P, int num_rot, int num_data,
                    int frames_per_part, const double *scale, int need_scaling,
                    const uint8_t *blacklist, double *part, double *psum_r);
@@ -191,10 +195,11 @@ void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, c
                           const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
                           const double *partial, int nparts, double *view);
 void launch_pair_to_f32(hipStream_t s, const double *P, long n, float *out);
+// pairs k0 .. k0 + num_pairs / full frames f0 .. f1 (-1: all), output rows from frame d0
 void launch_pair_scatter(hipStream_t s, const double *P, const int *pair_frame, const int *pair_tile,
-                         const int *slot_rot, long num_pairs, int num_rot, double *out);
+                         const int *slot_rot, long num_pairs, int num_rot, double *out, long k0 = 0, int d0 = 0);
 void launch_full_scatter(hipStream_t s, const double *RF, int nF, int num_tiles, const int *fidx,
-                         const int *slot_rot, int num_rot, double *out);
+                         const int *slot_rot, int num_rot, double *out, int f0 = 0, int f1 = -1, int d0 = 0);
 void launch_slot_gather(hipStream_t s, const double *in, const int *rot_slot, int num_rot, double *out);
 void launch_cell_posterior(hipStream_t s, double *R, int plain, int num_data, int num_rot, const long *pair_off,
                            const int *pair_tile, const FullRows &full, const int *slot_rot, const int *qmap,

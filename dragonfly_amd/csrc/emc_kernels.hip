@@ -409,13 +409,19 @@ __global__ __launch_bounds__(kBlock) void posterior_kernel(
 
 // Sparse posterior (the saved probabilities, max_prob.c:295-326: entries P > 1e-6 of every
 // frame, resparsify's cut). Count pass: one block per frame.
+// kRows: P is row-major, P[d * num_rot + r], instead of tile-major.
+template <bool kRows>
+__device__ __forceinline__ double post_at(const double *__restrict__ P, size_t tsR, int num_rot, int d, int r) {
+    return kRows ? P[(size_t)d * num_rot + r] : P[(size_t)d * kRTile + (size_t)(r >> 6) * tsR + (r & 63)];
+}
+
+template <bool kRows>
 __global__ __launch_bounds__(kBlock) void sparse_count_kernel(const double *__restrict__ P, size_t tsR,
                                                               int num_rot, int *__restrict__ counts) {
     __shared__ int s_n[kWavesPerBlock];
     const int d = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const double *row = P + (size_t)d * kRTile;
     int n = 0;
-    for (int r = threadIdx.x; r < num_rot; r += kBlock) n += row[(size_t)(r >> 6) * tsR + (r & 63)] > kProbMin;
+    for (int r = threadIdx.x; r < num_rot; r += kBlock) n += post_at<kRows>(P, tsR, num_rot, d, r) > kProbMin;
     for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
     if (lane == 0) s_n[wave] = n;
     __syncthreads();
@@ -424,16 +430,16 @@ __global__ __launch_bounds__(kBlock) void sparse_count_kernel(const double *__re
 
 // Fill pass: frame d0 + blockIdx.x writes its entries, orientations ascending, at off[blockIdx.x]
 // (offsets relative to the batch's first entry): place = orientation, prob = P.
+template <bool kRows>
 __global__ __launch_bounds__(kBlock) void sparse_fill_kernel(const double *__restrict__ P, size_t tsR,
                                                              int num_rot, int d0, const long *__restrict__ off,
                                                              int *__restrict__ place, double *__restrict__ prob) {
     __shared__ int s_n[kWavesPerBlock];
     const int d = d0 + blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const double *row = P + (size_t)d * kRTile;
     long at = off[blockIdx.x];
     for (int base = 0; base < num_rot; base += kBlock) {
         const int r = base + threadIdx.x;
-        const double v = r < num_rot ? row[(size_t)(r >> 6) * tsR + (r & 63)] : 0.;
+        const double v = r < num_rot ? post_at<kRows>(P, tsR, num_rot, d, r) : 0.;
         const bool keep = v > kProbMin;
         const unsigned long long bal = __ballot(keep);
         if (lane == 0) s_n[wave] = __popcll(bal);
@@ -1318,13 +1324,22 @@ void launch_occupancy(hipStream_t s, const double *P, int num_rot, int num_data,
 }
 
 void launch_sparse_count(hipStream_t s, const double *P, int num_rot, int num_data, int *counts) {
-    sparse_count_kernel<<<num_data, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, num_rot, counts);
+    sparse_count_kernel<false><<<num_data, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, num_rot, counts);
 }
 
 void launch_sparse_fill(hipStream_t s, const double *P, int num_rot, int num_data, int d0, int nd,
                         const long *off, int *place, double *prob) {
     if (nd <= 0) return;
-    sparse_fill_kernel<<<nd, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, num_rot, d0, off, place, prob);
+    sparse_fill_kernel<false><<<nd, kBlock, 0, s>>>(P, (size_t)num_data * kRTile, num_rot, d0, off, place, prob);
+}
+
+void launch_sparse_count_rows(hipStream_t s, const double *P, int num_rot, int nrows, int *counts) {
+    if (nrows > 0) sparse_count_kernel<true><<<nrows, kBlock, 0, s>>>(P, 0, num_rot, counts);
+}
+
+void launch_sparse_fill_rows(hipStream_t s, const double *P, int num_rot, int nrows, const long *off, int *place,
+                             double *prob) {
+    if (nrows > 0) sparse_fill_kernel<true><<<nrows, kBlock, 0, s>>>(P, 0, num_rot, 0, off, place, prob);
 }
 
 void launch_psum_r(hipStream_t s, const double *P, int num_rot, int num_data,

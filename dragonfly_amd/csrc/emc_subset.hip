@@ -388,30 +388,33 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
 
 // Scatter of partial-frame pair values to a dense (num_data x num_rot) matrix (zeroed by the
 // caller).
+// (pairs k0 .. k0 + num_pairs; rows of out counted from frame d0)
 __global__ __launch_bounds__(kBlock) void pair_scatter_kernel(const double *__restrict__ P,
                                                               const int *__restrict__ pair_frame,
                                                               const int *__restrict__ pair_tile,
-                                                              const int *__restrict__ slot_rot, long num_pairs,
-                                                              int num_rot, double *__restrict__ out) {
+                                                              const int *__restrict__ slot_rot, long k0,
+                                                              long num_pairs, int d0, int num_rot,
+                                                              double *__restrict__ out) {
     const long n = num_pairs * kRTile;
     for (long e = blockIdx.x * (long)kBlock + threadIdx.x; e < n; e += (long)gridDim.x * kBlock) {
-        const long k = e / kRTile;
+        const long k = k0 + e / kRTile;
         const int r = slot_rot[pair_tile[k] * kRTile + (int)(e % kRTile)];
-        if (r >= 0) out[(size_t)pair_frame[k] * num_rot + r] = P[e];
+        if (r >= 0) out[(size_t)(pair_frame[k] - d0) * num_rot + r] = P[k * kRTile + e % kRTile];
     }
 }
 
-// The same for full frames: RF[(T * nF + f) * 64 + j] -> out[fidx[f]][slot_rot[T * 64 + j]].
-__global__ __launch_bounds__(kBlock) void full_scatter_kernel(const double *__restrict__ RF, int nF,
-                                                              int num_tiles, const int *__restrict__ fidx,
-                                                              const int *__restrict__ slot_rot, int num_rot,
-                                                              double *__restrict__ out) {
-    const long n = (long)nF * num_tiles * kRTile;
+// The same for full frames f0 .. f1: RF[(T * nF + f) * 64 + j] -> out[fidx[f] - d0][slot_rot[T * 64 + j]].
+__global__ __launch_bounds__(kBlock) void full_scatter_kernel(const double *__restrict__ RF, int nF, int f0,
+                                                              int f1, int num_tiles, const int *__restrict__ fidx,
+                                                              const int *__restrict__ slot_rot, int d0,
+                                                              int num_rot, double *__restrict__ out) {
+    const int nb = f1 - f0;
+    const long n = (long)nb * num_tiles * kRTile;
     for (long e = blockIdx.x * (long)kBlock + threadIdx.x; e < n; e += (long)gridDim.x * kBlock) {
         const long row = e / kRTile;
-        const int T = (int)(row / nF), f = (int)(row % nF);
-        const int r = slot_rot[T * kRTile + (int)(e % kRTile)];
-        if (r >= 0) out[(size_t)fidx[f] * num_rot + r] = RF[e];
+        const int T = (int)(row / nb), f = f0 + (int)(row % nb), j = (int)(e % kRTile);
+        const int r = slot_rot[T * kRTile + j];
+        if (r >= 0) out[(size_t)(fidx[f] - d0) * num_rot + r] = RF[((size_t)T * nF + f) * kRTile + j];
     }
 }
 
@@ -563,19 +566,20 @@ void launch_pair_to_f32(hipStream_t s, const double *P, long n, float *out) {
 }
 
 void launch_pair_scatter(hipStream_t s, const double *P, const int *pair_frame, const int *pair_tile,
-                         const int *slot_rot, long num_pairs, int num_rot, double *out) {
+                         const int *slot_rot, long num_pairs, int num_rot, double *out, long k0, int d0) {
     const long n = num_pairs * kRTile;
     if (n == 0) return;
     pair_scatter_kernel<<<stride_grid((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(P, pair_frame, pair_tile, slot_rot,
-                                                                                  num_pairs, num_rot, out);
+                                                                                  k0, num_pairs, d0, num_rot, out);
 }
 
 void launch_full_scatter(hipStream_t s, const double *RF, int nF, int num_tiles, const int *fidx,
-                         const int *slot_rot, int num_rot, double *out) {
-    const long n = (long)nF * num_tiles * kRTile;
-    if (n == 0) return;
-    full_scatter_kernel<<<stride_grid((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(RF, nF, num_tiles, fidx, slot_rot,
-                                                                                  num_rot, out);
+                         const int *slot_rot, int num_rot, double *out, int f0, int f1, int d0) {
+    if (f1 < 0) f1 = nF;
+    const long n = (long)(f1 - f0) * num_tiles * kRTile;
+    if (n <= 0) return;
+    full_scatter_kernel<<<stride_grid((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(RF, nF, f0, f1, num_tiles, fidx,
+                                                                                  slot_rot, d0, num_rot, out);
 }
 
 void launch_slot_gather(hipStream_t s, const double *in, const int *rot_slot, int num_rot, double *out) {

@@ -261,16 +261,8 @@ class EMCRecon:
 
     def sparse_prob(self):
         """(num_prob, place, prob) of the saved sparse posterior (P > PROB_MIN,
-        max_prob.c:323-325), entries frame after frame: compacted on the GPU, or, after an
-        iteration on refinement subsets, cut from the dense posterior."""
-        try:
-            return self.engine.sparse_prob()
-        except _lib.EMCError:
-            if self.iter is None or self.iter.quat_mapping is None:
-                raise
-        P = self.prob_dense
-        keep = P > PROB_MIN
-        return keep.sum(1).astype('i4'), np.nonzero(keep)[1].astype('i4'), P[keep]
+        max_prob.c:323-325), entries frame after frame, compacted on the GPU."""
+        return self.engine.sparse_prob()
 
     @property
     def place_prob(self):

@@ -161,7 +161,9 @@ int emc_set_precision(emc_engine *e, int mode);
  * tomo_unroll (8/16), tomo_seq, max_chunk_cols (0 = by memory), merge_mode (0 box, 1 brick),
  * subset (1 sparse refinement subsets, 0 masked), slot_pack (1: refinement cells packed
  * back to back along a nearest-neighbour chain, 0: each cell padded to whole 64-orientation
- * tiles; takes effect at the next emc_set_refine). Start values come from EMC_<NAME>.      */
+ * tiles; takes effect at the next emc_set_refine), sparse_rows (frames per batch of
+ * emc_get_sparse_prob after a subset iteration, 0: 256 MB of rows). Start values come from
+ * EMC_<NAME>.      */
 int emc_set_option(emc_engine *e, const char *name, int value);
 int emc_get_option(emc_engine *e, const char *name, int *value);
 /* per-frame state of struct iterate: scale, beta, blacklist, fcounts, sum_fact,
@@ -201,8 +203,8 @@ int emc_slice_merge(emc_engine *e, const double *quat, const double *view);
  * max_prob.c:295-326; saved as probabilities/place and probabilities/prob, recon.pyx:269-279,
  * the single-thread order). num_prob[num_data] receives the counts and *total their sum;
  * with place and prob non-NULL (capacity >= total entries) the entries follow, frame after
- * frame. Counted and compacted on the GPU. After an iteration on refinement subsets it
- * returns EMC_E_STATE (read EMC_FIELD_PROB instead). */
+ * frame. Counted and compacted on the GPU (after an iteration on refinement subsets, frame
+ * batches are first scattered from the subset rows into orientation order). */
 int emc_get_sparse_prob(emc_engine *e, int *num_prob, long capacity, int *place, double *prob,
                         long *total);
 /* Copy field `field` (enum emc_field) into dst; count is the element count of dst and

@@ -268,9 +268,18 @@ def test_refinement_subsets(tiny, monkeypatch, form):
     eng = P.engine_for(tiny)
     eng.set_refine(len(qc), qmap, off, ref['rel_quat'])
     g = P.run_engine(eng, flags)
+    # the saved sparse posterior (one batch, then batches of 7 frames) is the cut of the dense one
+    sparse = [eng.sparse_prob()]
+    eng.set_option('sparse_rows', 7)
+    sparse.append(eng.sparse_prob())
     eng.close()
     _compare(o, g, 'refinement ' + form)
     assert g['rmax'][2] == -1
+    keep = g['prob'] > 1e-6
+    for num, place, prob in sparse:
+        assert np.array_equal(num, keep.sum(1))
+        assert np.array_equal(place, np.nonzero(keep)[1])
+        assert np.array_equal(prob, g['prob'][keep])
     if form == 'subset-chunked':
         assert g['num_chunks'] > 1
 
