@@ -75,12 +75,14 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
         if (t < t_end) {
             const double q0 = qvals[3 * (size_t)t], q1 = qvals[3 * (size_t)t + 1],
                          q2 = qvals[3 * (size_t)t + 2], cr = corr[t];
+#pragma unroll 4
             for (int k = 0; k < kRTile / kWavesPerBlock; ++k) {
                 const int rr = wave + kWavesPerBlock * k;
                 // combined orientation index r = rotation * modes + mode (max_prob.c:80-84)
                 const double *mdl =
                     modes > 1 ? model + (size_t)((r0 + c0 + rr) % modes) * (size_t)(size * size * size) : model;
-                s_tile[lane][rr] = s_pad[rr] ? 1. : slice_value(mdl, size, &s_rot[rr * 9], q0, q1, q2, cr, center);
+                const double v = slice_value_flat(mdl, size, &s_rot[rr * 9], q0, q1, q2, cr, center);
+                s_tile[lane][rr] = s_pad[rr] ? 1. : v;
             }
         }
         __syncthreads();
