@@ -139,7 +139,7 @@ std::vector<int> morton_order(int num_pix, const double *qvals) {
 
 constexpr int kPixPerGroup = 1024;
 constexpr int kFramesPerPsumPart = 512;
-constexpr int kFramesPerTomoPart = 12500;
+constexpr int kFramesPerTomoPart = 12500;  // default of the full_part_frames option
 constexpr int kModelParts = 1024;
 
 }  // namespace
@@ -275,6 +275,7 @@ struct emc_engine {
     // cells, and slots outside a partial frame's cells are masked (slot_cell, relbits)
     int slot_pack = 1, slot_pack_built = -1;
     int sparse_rows = 0;  // frames per batch of the subset sparse posterior (0: 256 MB of rows)
+    int full_part_frames = kFramesPerTomoPart;  // frames per range of the full-frame tomogram
     std::vector<int> h_ctile0, h_ctile1;  // packed: first and last tile of each cell
     DevBuf<int> slot_cell;
     // full frames (every coarse cell): rows tile-major in RF; photon lists of mask < 2 pixels
@@ -808,7 +809,8 @@ int build_full_lists(emc_engine *e) {
     // frame ranges of about kFramesPerTomoPart frames (at most tom_parts): fewer full frames
     // share fewer partial views, whose writes and reads would otherwise dominate
     const int P = e->num_pix, nF = e->num_full;
-    const int nparts = std::max(1, std::min(e->tom_parts, (nF + kFramesPerTomoPart - 1) / kFramesPerTomoPart));
+    const int per = e->full_part_frames;
+    const int nparts = std::max(1, std::min(e->tom_parts, (nF + per - 1) / per));
     e->full_parts = nparts;
     std::vector<int> &fof = e->h_fof;
     fof.assign(e->num_data, -1);
@@ -1301,6 +1303,7 @@ const EngineOption kOptions[] = {
     {"subset", &emc_engine::subset_mode, 0, 1, false},
     {"slot_pack", &emc_engine::slot_pack, 0, 1, false},
     {"sparse_rows", &emc_engine::sparse_rows, 0, 1 << 30, false},
+    {"full_part_frames", &emc_engine::full_part_frames, 1, 1 << 30, true},
 };
 
 int set_option(emc_engine *e, const EngineOption &o, int value) {

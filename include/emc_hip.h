@@ -162,8 +162,9 @@ int emc_set_precision(emc_engine *e, int mode);
  * subset (1 sparse refinement subsets, 0 masked), slot_pack (1: refinement cells packed
  * back to back along a nearest-neighbour chain, 0: each cell padded to whole 64-orientation
  * tiles; takes effect at the next emc_set_refine), sparse_rows (frames per batch of
- * emc_get_sparse_prob after a subset iteration, 0: 256 MB of rows). Start values come from
- * EMC_<NAME>.      */
+ * emc_get_sparse_prob after a subset iteration, 0: 256 MB of rows), full_part_frames
+ * (frames per range of the subset path's full-frame tomogram, default 12500). Start values
+ * come from EMC_<NAME>.      */
 int emc_set_option(emc_engine *e, const char *name, int value);
 int emc_get_option(emc_engine *e, const char *name, int *value);
 /* per-frame state of struct iterate: scale, beta, blacklist, fcounts, sum_fact,
