@@ -269,3 +269,37 @@ def test_prior_subsets_sizes_and_nearest_cell():
         cells = rq[off[d]:off[d + 1]]
         assert len(set(cells.tolist())) == len(cells)
         assert near[d] in cells
+
+
+def test_modes_and_background_host_side(tmp_path):
+    """num_modes from the config (params.pyx:104) gives a Model of that many volumes
+    (model.pyx:58, random start drawn over the whole (modes, size^3) block); non-rotating
+    modes are refused; a detector background (detector.pyx:101-104, 207-220) and the
+    background scales (parse_scale(bg=True), iterate.pyx:284-315) are read back."""
+    cfg = tmp_path / 'config.ini'
+    cfg.write_text('[emc]\nnum_modes = 2\n')
+    p = df.EMCParams()
+    p.from_config(str(cfg))
+    assert p.num_modes == 2 and p.nonrot_modes == 0
+    cfg.write_text('[emc]\nnum_modes = 2\nnum_nonrot_modes = 1\n')
+    with pytest.raises(NotImplementedError):
+        df.EMCParams().from_config(str(cfg))
+    m = df.Model(7, 2)
+    m.allocate('', model_mean=3., fixed_seed=True)
+    assert m.model1.shape == (2, 7, 7, 7) and m.model2.shape == (2, 7, 7, 7)
+    want = np.random.RandomState(0x5EED).random_sample((2, 7, 7, 7)) * 3.
+    assert np.array_equal(m.model1, want)
+    itr, dense = _iterate(0)
+    det = itr.dets[0]
+    assert det.with_bg == 0 and det.background is None
+    bg = np.linspace(0.1, 0.2, det.num_pix)
+    np.savez(tmp_path / 'bg.npz', background=bg)
+    det.parse_background(str(tmp_path / 'bg.npz'))
+    assert det.with_bg == 1 and np.array_equal(det.background, bg)
+    with pytest.raises(ValueError):
+        det.background = bg[:-1]
+    np.savez(tmp_path / 'bgscale.npz', bgscale=np.arange(1., 4.))
+    itr.parse_scale(str(tmp_path / 'bgscale.npz'), bg=True)
+    assert itr.bgscale.shape == (itr.tot_num_data,)
+    assert np.array_equal(itr.bgscale[:3], [1., 2., 3.]) and (itr.bgscale[3:] == 1.).all()
+    assert (itr.scale is None) or itr.scale.shape == (itr.tot_num_data,)
