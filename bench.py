@@ -538,8 +538,8 @@ def main():
                     help='run one device copy of this many MiB first (counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py measures one GPU only: this build has no multi-GPU path (the work '
-              'would shard by frames; see DESIGN.md). Run with --gpus 1.', file=sys.stderr)
+        print('bench.py measures one GPU only: there is no multi-GPU path (the work would shard '
+              'by frames, each GPU taking every orientation). Run with --gpus 1.', file=sys.stderr)
         sys.exit(2)
 
     import torch
