@@ -24,6 +24,8 @@ constexpr int kGemmBN = 256;
 constexpr int kGemmBK = 32;
 constexpr int kGemmThreads = 512;
 constexpr int kGemmGroupM = 8;
+// K step of the mixed-precision EMC products (the proposal network's products use kGemmBK)
+constexpr int kMpGemmBK = 64;
 
 // Brick merge: bricks of kBrick^3 voxels accumulated in LDS; at most kUnitItems work items
 // (orientation, 64-pixel patch) per block.

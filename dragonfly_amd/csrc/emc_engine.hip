@@ -922,8 +922,8 @@ int ensure_work(emc_engine *e) {
     const bool g32 = e->precision == 3;
     budget += (e->logw32.cap + e->p32.cap) * sizeof(float);
     if (mp) {
-        e->mp_ldp = round_up(P, emc::kGemmBK);
-        e->mp_ldd = round_up(N, emc::kGemmBK);
+        e->mp_ldp = round_up(P, emc::kMpGemmBK);
+        e->mp_ldd = round_up(N, emc::kMpGemmBK);
         e->mp_rows_d = round_up(N, emc::kGemmBM);
         e->mp_rows_t = round_up(P, emc::kGemmBM);
         const size_t kbytes = (e->mp_rows_d * e->mp_ldp + e->mp_rows_t * e->mp_ldd) * sizeof(_Float16);

@@ -45,8 +45,8 @@ constexpr int kGemmLdsPad = 8;
 constexpr int kGemmRow = kGemmBK + kGemmLdsPad;
 
 template <typename T>
-constexpr size_t gemm_lds_elems(int nb) {
-    return (size_t)(kGemmBM + nb * kGemmBN) * kGemmRow;
+constexpr size_t gemm_lds_elems(int nb, int bk = kGemmBK) {
+    return (size_t)(kGemmBM + nb * kGemmBN) * (bk + kGemmLdsPad);
 }
 
 // Block -> output tile. Consecutive blocks go round-robin to the 8 XCDs; each XCD gets a
@@ -78,27 +78,31 @@ __device__ __forceinline__ GemmTile gemm_block_tile(int m_tiles, int n_tiles) {
     return t;
 }
 
-// Accumulates the block's tile into acc (zeroed here). smem holds gemm_lds_elems<T>(NB).
-template <typename T, int NB>
+// Accumulates the block's tile into acc (zeroed here). smem holds gemm_lds_elems<T>(NB, BK);
+// K advances BK elements per step.
+template <typename T, int NB, int BK = kGemmBK>
 __device__ __forceinline__ void gemm_mainloop(const T *__restrict__ A, size_t lda,
                                               const T *__restrict__ B0, const T *__restrict__ B1,
                                               size_t ldb, int K, size_t m0, size_t n0, T *smem,
                                               floatx16 (&acc)[2][4]) {
     typedef typename Mfma<T>::vec8 vec8;
+    constexpr int kRow = BK + kGemmLdsPad;
     T *sA = smem;
-    T *sB = smem + kGemmBM * kGemmRow;  // NB consecutive [kGemmBN][kGemmRow] tiles
+    T *sB = smem + kGemmBM * kRow;  // NB consecutive [kGemmBN][kRow] tiles
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 3, wn = wave >> 2;
     // global -> register staging: each thread moves kLoads 16-byte chunks per operand per
-    // step (rows tid / 4 + 128 j, chunk tid % 4 of the 64-byte row segment)
-    constexpr int kLoads = kGemmBM * kGemmBK / 8 / kGemmThreads;
+    // step (rows tid / kChunks + (kGemmThreads / kChunks) j, chunk tid % kChunks of the row)
+    constexpr int kChunks = BK / 8;
+    constexpr int kLoads = kGemmBM * BK / 8 / kGemmThreads;
     static_assert(kGemmBM == kGemmBN, "square block tile");
-    const int lrow = tid >> 2, lchunk = (tid & 3) * 8;
+    static_assert(BK % 16 == 0 && kLoads >= 1, "K step");
+    const int lrow = tid / kChunks, lchunk = (tid % kChunks) * 8;
     vec8 ra[kLoads], rb[NB][kLoads];
     auto load = [&](int k0) {
 #pragma unroll
         for (int j = 0; j < kLoads; ++j) {
-            const size_t row = lrow + (kGemmThreads / 4) * j;
+            const size_t row = lrow + (kGemmThreads / kChunks) * j;
             ra[j] = *reinterpret_cast<const vec8 *>(A + (m0 + row) * lda + k0 + lchunk);
             rb[0][j] = *reinterpret_cast<const vec8 *>(B0 + (n0 + row) * ldb + k0 + lchunk);
             if (NB == 2)
@@ -108,11 +112,11 @@ __device__ __forceinline__ void gemm_mainloop(const T *__restrict__ A, size_t ld
     auto store = [&]() {
 #pragma unroll
         for (int j = 0; j < kLoads; ++j) {
-            const int row = lrow + (kGemmThreads / 4) * j;
-            *reinterpret_cast<vec8 *>(sA + row * kGemmRow + lchunk) = ra[j];
+            const int row = lrow + (kGemmThreads / kChunks) * j;
+            *reinterpret_cast<vec8 *>(sA + row * kRow + lchunk) = ra[j];
 #pragma unroll
             for (int b = 0; b < NB; ++b)
-                *reinterpret_cast<vec8 *>(sB + (b * kGemmBN + row) * kGemmRow + lchunk) = rb[b][j];
+                *reinterpret_cast<vec8 *>(sB + (b * kGemmBN + row) * kRow + lchunk) = rb[b][j];
         }
     };
 #pragma unroll
@@ -124,23 +128,23 @@ __device__ __forceinline__ void gemm_mainloop(const T *__restrict__ A, size_t ld
 
     const int frag_row = lane & 31, frag_k = (lane >> 5) * 8;
     load(0);
-    for (int k0 = 0; k0 < K; k0 += kGemmBK) {
+    for (int k0 = 0; k0 < K; k0 += BK) {
         __syncthreads();  // previous step's fragment reads are done
         store();
         __syncthreads();
-        if (k0 + kGemmBK < K) load(k0 + kGemmBK);
+        if (k0 + BK < K) load(k0 + BK);
 #pragma unroll
-        for (int kk = 0; kk < kGemmBK; kk += 16) {
+        for (int kk = 0; kk < BK; kk += 16) {
             vec8 fa[2], fb[NB][4];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
-                fa[i] = *reinterpret_cast<const vec8 *>(sA + (wm * 64 + i * 32 + frag_row) * kGemmRow + kk + frag_k);
+                fa[i] = *reinterpret_cast<const vec8 *>(sA + (wm * 64 + i * 32 + frag_row) * kRow + kk + frag_k);
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                     fb[b][j] = *reinterpret_cast<const vec8 *>(
-                        sB + (b * kGemmBN + wn * 128 + j * 32 + frag_row) * kGemmRow + kk + frag_k);
+                        sB + (b * kGemmBN + wn * 128 + j * 32 + frag_row) * kRow + kk + frag_k);
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll

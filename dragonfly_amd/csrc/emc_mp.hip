@@ -23,7 +23,7 @@
 //   ktom  [Mp_t][ldd]   FP16 counts, engine pixels x frames (mask < 2, else 0)
 //   lw_hi, lw_lo [Np_c][ldp]  centred log W of one orientation chunk
 //   p16   [Np_c][ldd]   scaled posterior of one orientation chunk
-// with ldp = num_pix rounded up to kGemmBK and ldd = num_data rounded up to kGemmBK; row
+// with ldp = num_pix rounded up to kMpGemmBK and ldd = num_data rounded up to kMpGemmBK; row
 // counts are rounded up to the GEMM tile. Outputs are the FP64 buffers of the plain path: R
 // tile-major (frame_max_kernel and the later per-frame passes read it unchanged) and the view
 // orientation-major, as the brick merge reads it.
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kGemmThreads) void mp_gemm_kernel(
     const GemmTile tile = gemm_block_tile(m_tiles, n_tiles);
     if (!tile.valid) return;
     floatx16 acc[2][4];
-    gemm_mainloop<_Float16, NB>(A, lda, B0, B1, ldb, K, tile.m0, tile.n0, smem, acc);
+    gemm_mainloop<_Float16, NB, kMpGemmBK>(A, lda, B0, B1, ldb, K, tile.m0, tile.n0, smem, acc);
     gemm_for_each(tile.m0, tile.n0, acc, [&](size_t m, size_t n, float a) {
         if (m >= (size_t)M_valid || n >= (size_t)N_valid) return;
         const double x = (double)a;
@@ -222,13 +222,13 @@ __global__ __launch_bounds__(kBlock) void posterior_fp16_kernel(const double *__
 
 // ---- launchers -----------------------------------------------------------------------------
 
-size_t gemm_lds_bytes(int nb) { return gemm_lds_elems<_Float16>(nb) * sizeof(_Float16); }
+size_t gemm_lds_bytes(int nb) { return gemm_lds_elems<_Float16>(nb, kMpGemmBK) * sizeof(_Float16); }
 
 int launch_mp_gemm(hipStream_t s, int nb, int epi, const _Float16 *A, size_t lda, const _Float16 *B0,
                    const _Float16 *B1, size_t ldb, int K, int Mp, int Np, int M_valid, int N_valid,
                    double *out, size_t ts, const double *row_add, const double *row_div,
                    int div_valid, double scale) {
-    if (Mp % kGemmBM || Np % kGemmBN || K % kGemmBK || lda % 8 || ldb % 8 || (size_t)K > lda ||
+    if (Mp % kGemmBM || Np % kGemmBN || K % kMpGemmBK || lda % 8 || ldb % 8 || (size_t)K > lda ||
         (size_t)K > ldb || M_valid > Mp || N_valid > Np)
         return -1;
     const int m_tiles = Mp / kGemmBM, n_tiles = Np / kGemmBN;
