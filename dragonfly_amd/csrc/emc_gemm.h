@@ -45,8 +45,8 @@ constexpr int kGemmLdsPad = 8;
 constexpr int kGemmRow = kGemmBK + kGemmLdsPad;
 
 template <typename T>
-constexpr size_t gemm_lds_elems(int nb, int bk = kGemmBK) {
-    return (size_t)(kGemmBM + nb * kGemmBN) * (bk + kGemmLdsPad);
+constexpr size_t gemm_lds_elems(int nb, int bk = kGemmBK, int stages = 1) {
+    return (size_t)stages * (kGemmBM + nb * kGemmBN) * (bk + kGemmLdsPad);
 }
 
 // Block -> output tile. Consecutive blocks go round-robin to the 8 XCDs; each XCD gets a
@@ -78,17 +78,19 @@ __device__ __forceinline__ GemmTile gemm_block_tile(int m_tiles, int n_tiles) {
     return t;
 }
 
-// Accumulates the block's tile into acc (zeroed here). smem holds gemm_lds_elems<T>(NB, BK);
-// K advances BK elements per step.
-template <typename T, int NB, int BK = kGemmBK>
+// Accumulates the block's tile into acc (zeroed here). smem holds
+// gemm_lds_elems<T>(NB, BK, DB ? 2 : 1); K advances BK elements per step. With DB the steps
+// alternate between two LDS stages: step k's fragments are read from one while step k + 1's
+// operands are stored to the other, one barrier per step.
+template <typename T, int NB, int BK = kGemmBK, bool DB = false>
 __device__ __forceinline__ void gemm_mainloop(const T *__restrict__ A, size_t lda,
                                               const T *__restrict__ B0, const T *__restrict__ B1,
                                               size_t ldb, int K, size_t m0, size_t n0, T *smem,
                                               floatx16 (&acc)[2][4]) {
     typedef typename Mfma<T>::vec8 vec8;
     constexpr int kRow = BK + kGemmLdsPad;
-    T *sA = smem;
-    T *sB = smem + kGemmBM * kRow;  // NB consecutive [kGemmBN][kRow] tiles
+    // one stage: the A tile [kGemmBM][kRow], then NB consecutive [kGemmBN][kRow] B tiles
+    constexpr int kStage = (kGemmBM + NB * kGemmBN) * kRow;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 3, wn = wave >> 2;
     // global -> register staging: each thread moves kLoads 16-byte chunks per operand per
@@ -109,7 +111,8 @@ __device__ __forceinline__ void gemm_mainloop(const T *__restrict__ A, size_t ld
                 rb[NB - 1][j] = *reinterpret_cast<const vec8 *>(B1 + (n0 + row) * ldb + k0 + lchunk);
         }
     };
-    auto store = [&]() {
+    auto store = [&](T *sA) {
+        T *sB = sA + kGemmBM * kRow;
 #pragma unroll
         for (int j = 0; j < kLoads; ++j) {
             const int row = lrow + (kGemmThreads / kChunks) * j;
@@ -127,12 +130,8 @@ __device__ __forceinline__ void gemm_mainloop(const T *__restrict__ A, size_t ld
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
 
     const int frag_row = lane & 31, frag_k = (lane >> 5) * 8;
-    load(0);
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        __syncthreads();  // previous step's fragment reads are done
-        store();
-        __syncthreads();
-        if (k0 + BK < K) load(k0 + BK);
+    auto compute = [&](const T *sA) {
+        const T *sB = sA + kGemmBM * kRow;
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
             vec8 fa[2], fb[NB][4];
@@ -151,6 +150,30 @@ __device__ __forceinline__ void gemm_mainloop(const T *__restrict__ A, size_t ld
                 for (int i = 0; i < 2; ++i)
 #pragma unroll
                     for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::run(fa[i], fb[b][j], acc[i][j]);
+        }
+    };
+    load(0);
+    if (!DB) {
+        for (int k0 = 0; k0 < K; k0 += BK) {
+            __syncthreads();  // previous step's fragment reads are done
+            store(smem);
+            __syncthreads();
+            if (k0 + BK < K) load(k0 + BK);
+            compute(smem);
+        }
+    } else {
+        store(smem);
+        __syncthreads();
+        int stage = 0;
+        for (int k0 = 0; k0 < K; k0 += BK) {
+            const bool more = k0 + BK < K;
+            if (more) load(k0 + BK);
+            compute(smem + stage * kStage);
+            // the other stage was last read by the previous step, which every wave finished
+            // before the previous barrier
+            if (more) store(smem + (stage ^ 1) * kStage);
+            __syncthreads();
+            stage ^= 1;
         }
     }
 }

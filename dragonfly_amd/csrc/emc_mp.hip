@@ -55,7 +55,8 @@ __global__ __launch_bounds__(kGemmThreads) void mp_gemm_kernel(
     const GemmTile tile = gemm_block_tile(m_tiles, n_tiles);
     if (!tile.valid) return;
     floatx16 acc[2][4];
-    gemm_mainloop<_Float16, NB, kMpGemmBK>(A, lda, B0, B1, ldb, K, tile.m0, tile.n0, smem, acc);
+    // one-term products double-buffer LDS (2 x 72 KiB); two-term ones do not fit twice
+    gemm_mainloop<_Float16, NB, kMpGemmBK, NB == 1>(A, lda, B0, B1, ldb, K, tile.m0, tile.n0, smem, acc);
     gemm_for_each(tile.m0, tile.n0, acc, [&](size_t m, size_t n, float a) {
         if (m >= (size_t)M_valid || n >= (size_t)N_valid) return;
         const double x = (double)a;
@@ -222,7 +223,9 @@ __global__ __launch_bounds__(kBlock) void posterior_fp16_kernel(const double *__
 
 // ---- launchers -----------------------------------------------------------------------------
 
-size_t gemm_lds_bytes(int nb) { return gemm_lds_elems<_Float16>(nb, kMpGemmBK) * sizeof(_Float16); }
+size_t gemm_lds_bytes(int nb) {
+    return gemm_lds_elems<_Float16>(nb, kMpGemmBK, nb == 1 ? 2 : 1) * sizeof(_Float16);
+}
 
 int launch_mp_gemm(hipStream_t s, int nb, int epi, const _Float16 *A, size_t lda, const _Float16 *B0,
                    const _Float16 *B1, size_t ldb, int K, int Mp, int Np, int M_valid, int N_valid,
