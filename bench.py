@@ -206,9 +206,15 @@ def bench_dense(args):
     if args.precision in (1, 2):
         flops = 2. * D * (R / nch_mp) * P * terms
         achieved = flops / (launch_ms * 1e-3) / 1e12
+        # measured HBM-side bytes of the one-term product at the default sizes
+        default_size = args.num_data is None and args.num_div is None and terms == 1
+        traffic = load_traffic(args.config, 'mp_gemm_kernel_likelihood') if default_size else None
         roof = dict(bound='mfma', achieved=round(achieved, 1), peak=2500., unit='TFLOP/s',
-                    frac=round(achieved / 2500., 4), traffic=None, kernel='mp_gemm_kernel (likelihood)',
-                    launch_ms=round(launch_ms, 3), algorithmic_flops=flops)
+                    frac=round(achieved / 2500., 4), traffic=None if traffic is None else round(traffic),
+                    kernel='mp_gemm_kernel (likelihood)', launch_ms=round(launch_ms, 3),
+                    algorithmic_flops=flops)
+        if traffic is not None:
+            roof['hbm_frac_measured'] = round(traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         path = 'mixed precision, %d-term FP16 log W' % terms
         dtype = 'f16 products / f32 accumulate / f64 rest'
     else:
