@@ -181,7 +181,7 @@ def bench_dense(args):
     recon, itr = build_recon(prob, cfg, dense=K, precision=args.precision)
     start = itr.model.model1.copy()
     D, R = itr.tot_num_data, itr.quat.num_rot
-    pairs = (D - itr.num_blacklist) * R
+    pairs = (D - int((itr.blacklist != 0).sum())) * R
     el_mp, st_mp, nch_mp, tr_mp = run_steps(recon, itr, args.warmup, args.steps, keep_models=True)
     recon.free()
     del recon
@@ -601,7 +601,7 @@ def main():
     D = itr.tot_num_data
     nbar = photon_stats(prob, itr)['lik'][0]
     R = itr.quat.num_rot
-    pairs = (D - itr.num_blacklist) * R
+    pairs = (D - int((itr.blacklist != 0).sum())) * R
 
     for it in range(1, args.warmup + 1):
         itr.params.iteration = it

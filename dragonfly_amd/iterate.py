@@ -114,6 +114,21 @@ class Iterate:
         else:
             raise ValueError('Need either in_photons_file or in_photons_list.')
         self.dets = [det]
+        # detector background (iterate.pyx:106-125); one detector here, so a list must
+        # name exactly one file, relative to the list's folder
+        background_file = config.get_filename(section_name, 'background_file', fallback=None)
+        background_list = config.get_filename(section_name, 'background_list', fallback=None)
+        if background_file is not None and background_list is not None:
+            raise ValueError('Both background_file and background_list specified. Pick one.')
+        if background_list is not None:
+            with open(background_list) as f:
+                bg_lines = [line.strip() for line in f if line.strip()]
+            if len(bg_lines) != len(self.dets):
+                raise ValueError('Background list has %d entries but there are %d unique detectors'
+                                 % (len(bg_lines), len(self.dets)))
+            det.parse_background(op.join(op.dirname(background_list), bg_lines[0]))
+        elif background_file is not None:
+            det.parse_background(background_file)
         self.set_data(frames)
         self.set_quat(Quaternion().from_config(config_fname, section_name))
         try:
@@ -124,24 +139,24 @@ class Iterate:
             scale_file = config.get_filename(section_name, 'scale_file')
         except configparser.NoOptionError:
             scale_file = ''
+        try:
+            bgscale_file = config.get_filename(section_name, 'bgscale_file')
+        except configparser.NoOptionError:
+            bgscale_file = ''
         if resume:
             with open(param.log_fname) as fp:
                 last_iter = int(fp.readlines()[-1].split()[0])
             self.params.start_iter = last_iter + 1
-            model_file = op.join(param.output_folder, 'output_%.3d.npz' % last_iter)
+            model_file = self.output_file(param.output_folder, last_iter)
             scale_file = model_file
+            print('Resuming from iteration', self.params.start_iter)
         model = Model(self.calculate_size(max(d.qmax() for d in self.dets)), self.params.num_modes)
         model_mean = self.mean_count[0] / (self.dets[0].raw_mask == 0).sum() * 2.
-        if model_file.endswith('.npz') and op.isfile(model_file):
-            with np.load(model_file) as f:
-                model.allocate('', model_mean)
-                shape = (model.size,) * 3 if model.num_modes == 1 else (model.num_modes,) + (model.size,) * 3
-                model.model1 = np.ascontiguousarray(f['intens'].reshape(shape))
-        else:
-            model.allocate(model_file, model_mean, fixed_seed=bool(param.fixed_seed))
+        model.allocate(model_file, model_mean, fixed_seed=bool(param.fixed_seed))
         self.set_model(model)
         if self.params.need_scaling == 1:
             self.parse_scale(scale_file)
+            self.parse_scale(bgscale_file, bg=True)
         sel_string = config.get(section_name, 'selection', fallback=None)
         try:
             blacklist_file = config.get_filename(section_name, 'blacklist_file')
@@ -301,11 +316,21 @@ class Iterate:
                 raise AttributeError('Need tot_num_data to initialize scale factors')
             scale = np.ones(self.tot_num_data)
         elif fname.endswith('.npz'):
+            # the 'scale' dataset, for background scales too (iterate.pyx:295-297)
             with np.load(fname) as f:
-                key = 'bgscale' if bg and 'bgscale' in f.files else 'scale'
-                scale = np.asarray(f[key], dtype='f8')
+                scale = np.asarray(f['scale'], dtype='f8')
         else:
-            scale = np.loadtxt(fname, dtype='f8').ravel()
+            with open(fname, 'rb') as f:
+                is_hdf5 = f.read(8) == b'\x89HDF\r\n\x1a\n'
+            if is_hdf5:
+                try:
+                    import h5py
+                except ImportError:
+                    raise TypeError('%s is an HDF5 file and h5py is not available' % fname)
+                with h5py.File(fname, 'r') as f:
+                    scale = np.asarray(f['scale'][:], dtype='f8')
+            else:
+                scale = np.loadtxt(fname, dtype='f8').ravel()
         if self.tot_num_data < scale.shape[0]:
             print('More scale factors than required. Taking only first', self.tot_num_data)
             scale = scale[:self.tot_num_data]
@@ -364,6 +389,16 @@ class Iterate:
             self.beta_start = np.array([math.exp(-6.5 * math.pow(f * 1.e-5, 0.15))
                                         for f in self.fcounts.tolist()])
         self.beta = self.beta_start.copy()
+
+    @staticmethod
+    def output_file(output_folder, iteration):
+        """The output file of an iteration: output_NNN.h5 as the reference writes it
+        (recon.pyx:230-279), or output_NNN.npz where h5py is absent."""
+        stem = op.join(output_folder, 'output_%.3d' % iteration)
+        for ext in ('.h5', '.npz'):
+            if op.isfile(stem + ext):
+                return stem + ext
+        raise FileNotFoundError('No output file %s.h5 or %s.npz to resume from' % (stem, stem))
 
     @staticmethod
     def calculate_size(qmax):

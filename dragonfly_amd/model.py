@@ -3,6 +3,25 @@ import os
 
 import numpy as np
 
+_HDF5_SIGNATURE = b'\x89HDF\r\n\x1a\n'
+
+
+def _read_intens(fname):
+    """The 'intens' dataset of an output file (.npz written here, or HDF5 as the reference
+    writes, recon.pyx:230-279), or None when fname is a raw float64 volume."""
+    if fname.endswith('.npz'):
+        with np.load(fname) as f:
+            return np.asarray(f['intens'], dtype='f8')
+    with open(fname, 'rb') as f:
+        if f.read(8) != _HDF5_SIGNATURE:
+            return None
+    try:
+        import h5py
+    except ImportError:
+        raise TypeError('%s is an HDF5 file and h5py is not available' % fname)
+    with h5py.File(fname, 'r') as f:
+        return np.asarray(f['intens'][()], dtype='f8')
+
 
 class Model:
     """model1 (current estimate), model2 (merged numerator) and inter_weight, each
@@ -69,6 +88,19 @@ class Model:
         self.model2 = np.zeros(shape)
         self.inter_weight = np.zeros(shape)
         if fname and os.path.isfile(fname):
+            intens = _read_intens(fname)
+            if intens is not None:
+                # an output file's 'intens' (num_modes, size, size, size): extra modes are
+                # dropped, missing ones drawn uniform [0, model_mean) (model.pyx:67-82)
+                mshape = (self.num_modes,) + (self.size,) * 3
+                intens = intens.reshape((-1,) + intens.shape[-3:])
+                if intens.shape[1:] != mshape[1:]:
+                    raise ValueError('Input model has wrong size (%d)' % intens.shape[1], intens.shape[1])
+                if intens.shape[0] < self.num_modes:
+                    extra = np.random.random((self.num_modes - intens.shape[0],) + mshape[1:]) * model_mean
+                    intens = np.concatenate([intens, extra])
+                self.model1 = np.ascontiguousarray(intens[:self.num_modes], dtype='f8').reshape(shape).copy()
+                return self
             if fname.endswith('.npy'):
                 arr = np.load(fname)
             else:

@@ -83,6 +83,10 @@ class EMCParams:
         self.sigmasq = config.getfloat(section_name, 'gaussian_sigma', fallback=0.) ** 2
         self.friedel_sym = config.getint(section_name, 'friedel_sym', fallback=0)
         self.axial_sym = config.getint(section_name, 'axial_sym', fallback=1)
+        if self.axial_sym > 1:
+            # recon.pyx:135-137 symmetrises every mode about the axis; the engine's model
+            # update applies Friedel symmetry only
+            raise NotImplementedError('axial_sym > 1 is not supported by the GPU engine')
         self.save_prob = config.getint(section_name, 'save_prob', fallback=0)
         self.update_scale = config.getint(section_name, 'update_scale', fallback=1)
         num_divs = config.get(section_name, 'num_div', fallback='0').split()

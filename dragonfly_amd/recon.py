@@ -77,7 +77,11 @@ class EMCRecon:
             sp = itr.sparse_arrays()
             eng.set_frames(sp['ones'], sp['multi'], sp['ones_accum'], sp['multi_accum'],
                            sp['place_ones'], sp['place_multi'], sp['count_multi'])
-        itr.num_blacklist = int((itr.blacklist != 0).sum())
+        # iter->num_blacklist is never set on the reference's Cython path (iterate.pyx:35
+        # callocs it and nothing writes it), so combine_information divides the summed
+        # likelihood and information by tot_num_data (max_prob.c:373-374) and update_beta
+        # its beta sum likewise (recon.pyx:172); blacklisted frames still contribute nothing.
+        itr.num_blacklist = 0
         eng.set_frame_params(itr.scale, itr.beta_start, itr.blacklist, itr.fcounts,
                              itr.sum_fact, itr.mean_count[0], itr.num_blacklist)
         if det.with_bg:

@@ -150,7 +150,9 @@ def iterate(det, quats, model1, frames, fparams, flags, refine=None):
                        mask)
     if model1.ndim == 3:
         model2, weight = model2[0], weight[0]
-    nb = D - int(black.sum())
+    # combine_information divides by tot_num_data - iter->num_blacklist (max_prob.c:373-374);
+    # the Cython path never sets that field (iterate.pyx:35 callocs it), so callers pass 0
+    nb = D - int(fparams.get('num_blacklist', 0))
     out = dict(u=u, rescale=rescale, max_exp=M, rmax=rmax, p_norm=p_norm, likelihood=like,
                info=info, psum_d=psum_d, psum_r=psum_r, prob=Pm, model2=model2,
                inter_weight=weight, R=R, avg_likelihood=like[~black].sum() / nb,
@@ -166,16 +168,20 @@ def iterate(det, quats, model1, frames, fparams, flags, refine=None):
 
 
 def update_model(model1, model2, weight, alpha=0.):
-    """EMCRecon.update_model in Cython order with a vectorised Friedel average."""
+    """EMCRecon.update_model in Cython order (recon.pyx:112-149): Friedel average of the
+    weighted sums, then divide by the weight, rms change, alpha blend. The Friedel partner of
+    voxel i is 2c - i with c = size // 2 on every axis; for even sizes index 0 has no partner
+    and is left alone (model.c:470-490 starts its loops at 1 then), which is the same
+    vectorised flip applied to the [1:, 1:, 1:] block."""
     size = model1.shape[0]
-    flipped_a = model2[::-1, ::-1, ::-1]
-    flipped_w = weight[::-1, ::-1, ::-1]
-    if size % 2 == 0:
-        raise ValueError('vectorised Friedel average is written for odd sizes')
-    num = 0.5 * (model2 * weight + flipped_a * flipped_w)
-    den = 0.5 * (weight + flipped_w)
-    sym = np.where(den > 0, num / np.where(den > 0, den, 1.), 0.)
-    m2 = np.where(weight > 0, sym / np.where(weight > 0, weight, 1.), sym)
+    lo = 1 if size % 2 == 0 else 0
+    m2 = np.array(model2, dtype='f8', copy=True)
+    a = m2[lo:, lo:, lo:]
+    w = weight[lo:, lo:, lo:]
+    num = 0.5 * (a * w + a[::-1, ::-1, ::-1] * w[::-1, ::-1, ::-1])
+    den = 0.5 * (w + w[::-1, ::-1, ::-1])
+    m2[lo:, lo:, lo:] = np.where(den > 0, num / np.where(den > 0, den, 1.), 0.)
+    m2 = np.where(weight > 0, m2 / np.where(weight > 0, weight, 1.), m2)
     rms = np.sqrt(((m2 - model1) ** 2).mean())
     new1 = alpha * model1 + (1 - alpha) * m2 if alpha > 0 else m2
     return new1, m2, rms

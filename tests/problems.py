@@ -107,3 +107,28 @@ def engine_for_dense(p):
     eng.set_frame_params(fp['scale'], fp['beta'], fp['blacklist'], fp['fcounts'], fp['sum_fact'],
                          fp['mean_count'], fp['num_blacklist'])
     return eng
+
+
+def write_config(p, folder, **keys):
+    """Write problem p as the files a reference run reads -- detector (.npz with the HDF5
+    dataset names), photons (.emc), start model (raw float64) -- and a config.ini whose
+    [emc] section holds these plus `keys`. Returns the config path."""
+    import os
+    from dragonfly_amd import CDetector, EMCWriter
+    os.makedirs(folder, exist_ok=True)
+    det = p['det']
+    CDetector.from_arrays(det['qvals'], det['corr'], det['mask']).write(os.path.join(folder, 'det.npz'))
+    f = p['frames']
+    with EMCWriter(os.path.join(folder, 'photons.emc'), len(det['corr'])) as w:
+        for d in range(len(f['ones'])):
+            o, m = f['ones_accum'][d], f['multi_accum'][d]
+            w.write_sparse_frame(f['place_ones'][o:o + f['ones'][d]], f['place_multi'][m:m + f['multi'][d]],
+                                 f['count_multi'][m:m + f['multi'][d]])
+    p['model1'].astype('f8').tofile(os.path.join(folder, 'start.bin'))
+    emc = dict(in_detector_file='det.npz', in_photons_file='photons.emc', start_model_file='start.bin',
+               num_div=str(p.get('num_div', 2)), output_folder='out/', log_file='EMC.log')
+    emc.update({k: str(v) for k, v in keys.items()})
+    path = os.path.join(folder, 'config.ini')
+    with open(path, 'w') as fp:
+        fp.write('[emc]\n' + ''.join('%s = %s\n' % kv for kv in emc.items()))
+    return path

@@ -142,7 +142,10 @@ def test_zero_model_region_gives_minus_inf(tiny):
     _compare(o, g, 'zero block')
 
 
-def test_three_iterations_with_model_update(tiny_scaled):
+@pytest.mark.parametrize('alpha', [0., 0.5])
+def test_three_iterations_with_model_update(tiny_scaled, alpha):
+    """Three scaled iterations chained through update_model (with alpha > 0 the blend
+    model1 = alpha model1 + (1 - alpha) model2, recon.pyx:140-147) and normalize_scale."""
     from oracle import core
     p = tiny_scaled
     eng = P.engine_for(p)
@@ -153,11 +156,11 @@ def test_three_iterations_with_model_update(tiny_scaled):
         p2 = dict(p, fparams=dict(p['fparams'], scale=scale.copy()))
         o = P.run_oracle(p2, flags, model1=model1)
         g = P.run_engine(eng, flags)
-        _compare(o, g, 'chain it=%d' % it, scaling=True)
+        _compare(o, g, 'chain alpha=%g it=%d' % (alpha, it), scaling=True)
         # host-side reference sequence: update_model then normalize_scale
         m2 = o['model2'].copy()
-        rms_o = core.update_model(model1, m2, o['inter_weight'], 0.)
-        rms_g = eng.update_model(0.)
+        rms_o = core.update_model(model1, m2, o['inter_weight'], alpha)
+        rms_g = eng.update_model(alpha)
         scale = o['scale'].copy()
         mean = scale.mean()
         model1 *= mean
@@ -171,11 +174,13 @@ def test_three_iterations_with_model_update(tiny_scaled):
 
 
 def test_cfg1_geometry_two_iterations():
-    """The 125^3 / 151x151-pixel geometry of the CPU-runnable configuration at num_div 4
-    (3240 orientations), 1000 frames, beta 1, chained through the model update."""
+    """The CPU-runnable configuration at its full size: 125^3 model, 151x151 pixels,
+    num_div 4 (3240 orientations), all 3000 frames, beta 1, two iterations chained through
+    the model update (the oracle takes ~20 s per iteration on one host thread)."""
     from dragonfly_amd import synth
     from oracle import core
-    cfg = dict(synth.CONFIGS['cfg1'], num_data=1000)
+    cfg = dict(synth.CONFIGS['cfg1'])
+    assert cfg['num_data'] == 3000
     p = P.build(cfg, device='cuda', num_ave=1000)
     eng = P.engine_for(p)
     model1 = p['model1'].copy()

@@ -60,3 +60,49 @@ def test_recon_two_modes_save_prob(tmp_path):
     assert np.array_equal(out['probabilities/place'], np.nonzero(keep)[1])
     assert P.rel_err(out['probabilities/prob'], o['prob'][keep]) <= TOL
     assert int(out['probabilities/num_rot'][0]) == itr.quat.num_rot
+
+
+def test_recon_from_config_odd_only(tmp_path):
+    """Iterate('config.ini') -> EMCRecon.run_iteration on files written to disk, with
+    selection = odd_only: every other frame blacklisted (iterate.pyx:317-359). The reference's
+    Cython path never sets iter->num_blacklist, so the logged log-likelihood and mutual
+    information are sums over the kept frames divided by all N_d frames
+    (max_prob.c:373-374); the oracle is run with the same rule and the log line is checked
+    against it."""
+    from oracle import core
+    p = P.build(P.tiny_config(num_data=50))
+    cfg = P.write_config(p, str(tmp_path), selection='odd_only', beta=0.7, num_div=2)
+    itr = df.Iterate(cfg)
+    assert list(itr.blacklist[:4]) == [0, 1, 0, 1] and itr.num_blacklist == 0
+    rec = df.EMCRecon()
+    rec.set_iterate(itr)
+    itr.params.iteration = 1
+    print('case: recon from config, odd_only', flush=True)
+    rec.run_iteration()
+    out = np.load(tmp_path / 'out' / 'output_001.npz')
+    lines = (tmp_path / 'EMC.log').read_text().splitlines()
+    rec.free()
+    det = itr.dets[0]
+    d = dict(qvals=det.qvals, corr=det.corr, mask=det.raw_mask)
+    D = itr.tot_num_data
+    fp = dict(beta=np.full(D, 0.7), scale=np.ones(D), blacklist=itr.blacklist.copy(), fcounts=itr.fcounts,
+              sum_fact=itr.sum_fact, mean_count=itr.mean_count[0], num_blacklist=0)
+    o = core.maximize(d, itr.quat.quats, p['model1'], p['frames'], fp, dict(iteration=1, need_scaling=0),
+                      dense_prob=True)
+    keep = itr.blacklist == 0
+    assert abs(o['avg_likelihood'] - o['likelihood'][keep].sum() / D) <= 1e-12 * abs(o['avg_likelihood'])
+    assert abs(itr.likelihood - o['avg_likelihood']) <= TOL * abs(o['avg_likelihood'])
+    assert abs(itr.mutual_info - o['mutual_info']) <= TOL * abs(o['mutual_info'])
+    assert np.array_equal(out['orientations'], o['rmax'])
+    assert (out['orientations'][~keep] == -1).all()
+    m1, m2 = p['model1'].copy(), o['model2'].copy()
+    core.update_model(m1, m2, o['inter_weight'], 0.)
+    assert P.rel_err(out['intens'][0], m1) <= TOL
+    # EMC.log: header without a blacklist count (num_blacklist == 0), then the iteration line
+    # with the mutual information, the average log-likelihood and the mean beta over N_d
+    assert any(line.strip() == 'num_data = %d' % D for line in lines)
+    it_line = lines[-1].split('\t')
+    assert int(it_line[0]) == 1
+    assert abs(float(it_line[3]) - o['mutual_info']) <= 1e-6  # written with %f
+    assert abs(float(it_line[4]) - o['avg_likelihood']) <= 1e-6 * abs(o['avg_likelihood'])
+    assert abs(float(it_line[6]) - 0.7 * keep.sum() / D) <= 1e-6

@@ -298,8 +298,77 @@ def test_modes_and_background_host_side(tmp_path):
     assert det.with_bg == 1 and np.array_equal(det.background, bg)
     with pytest.raises(ValueError):
         det.background = bg[:-1]
-    np.savez(tmp_path / 'bgscale.npz', bgscale=np.arange(1., 4.))
+    np.savez(tmp_path / 'bgscale.npz', scale=np.arange(1., 4.))
     itr.parse_scale(str(tmp_path / 'bgscale.npz'), bg=True)
     assert itr.bgscale.shape == (itr.tot_num_data,)
     assert np.array_equal(itr.bgscale[:3], [1., 2., 3.]) and (itr.bgscale[3:] == 1.).all()
     assert (itr.scale is None) or itr.scale.shape == (itr.tot_num_data,)
+
+
+def test_from_config_background_selection_resume(tmp_path):
+    """The config pipeline (iterate.pyx:40-182) on files written to disk: detector
+    background from background_file or background_list (iterate.pyx:106-125), background
+    scales from bgscale_file when need_scaling (:167-169, the file's 'scale' dataset),
+    odd_only selection with num_blacklist left at 0 as the Cython path leaves it, the start
+    model, refusal of axial_sym > 1, and resume from whichever of output_NNN.h5 / .npz
+    exists (:141-151)."""
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    import problems as P
+    p = P.build(P.tiny_config(num_data=30))
+    P_ = len(p['det']['corr'])
+    bg = np.linspace(0.01, 0.02, P_)
+    np.savez(tmp_path / 'bg.npz', background=bg)
+    np.savez(tmp_path / 'bgs.npz', scale=np.linspace(0.5, 1.5, 30))
+    cfg = P.write_config(p, str(tmp_path), need_scaling=1, background_file='bg.npz',
+                         bgscale_file='bgs.npz', selection='odd_only', beta=0.5)
+    itr = df.Iterate(cfg)
+    det = itr.dets[0]
+    assert det.with_bg == 1 and np.array_equal(det.background, bg)
+    assert np.array_equal(itr.bgscale, np.linspace(0.5, 1.5, 30))
+    assert np.array_equal(itr.scale, np.ones(30))
+    assert list(itr.blacklist) == [0, 1] * 15 and itr.num_blacklist == 0
+    assert np.array_equal(itr.model.model1, p['model1']) and (itr.beta_start == 0.5).all()
+    # background_list: one entry per detector, relative to the list's folder
+    (tmp_path / 'bglist.txt').write_text('bg.npz\n')
+    cfg = P.write_config(p, str(tmp_path), background_list='bglist.txt')
+    assert np.array_equal(df.Iterate(cfg).dets[0].background, bg)
+    cfg = P.write_config(p, str(tmp_path), background_list='bglist.txt', background_file='bg.npz')
+    with pytest.raises(ValueError, match='Pick one'):
+        df.Iterate(cfg)
+    cfg = P.write_config(p, str(tmp_path), axial_sym=4)
+    with pytest.raises(NotImplementedError):
+        df.Iterate(cfg)
+    # resume: the last logged iteration's output file gives model1 and the scales
+    cfg = P.write_config(p, str(tmp_path), need_scaling=1)
+    (tmp_path / 'EMC.log').write_text('header\n\nIter\ttime\n1\t0.1\n3\t0.1\n')
+    with pytest.raises(FileNotFoundError):
+        df.Iterate(cfg, resume=True)
+    os.makedirs(tmp_path / 'out', exist_ok=True)
+    intens = p['model1'][None] * 2.
+    np.savez(tmp_path / 'out' / 'output_003.npz', intens=intens, scale=np.full(30, 1.25))
+    itr = df.Iterate(cfg, resume=True)
+    assert itr.params.start_iter == 4
+    assert np.array_equal(itr.model.model1, intens[0]) and (itr.scale == 1.25).all()
+
+
+def test_update_beta_mean_over_all_frames():
+    """update_beta sums beta over frames that are not blacklisted and divides by
+    tot_num_data - num_blacklist with num_blacklist never set, i.e. by every frame
+    (recon.pyx:162-172)."""
+    itr, _ = _iterate(0)
+    itr.calc_beta(0.4)
+    itr.parse_blacklist('', 'even_only')
+    itr.params.beta_factor = 1.
+
+    class FakeEngine:
+        def set_beta(self, b):
+            pass
+    rec = df.EMCRecon()
+    rec.iter = itr
+    rec.engine = FakeEngine()
+    itr.params.iteration = 1
+    mean = rec.update_beta()
+    kept = int((itr.blacklist == 0).sum())
+    assert 0 < kept < itr.tot_num_data
+    assert abs(mean - 0.4 * kept / itr.tot_num_data) < 1e-15

@@ -72,6 +72,50 @@ def test_blacklist_and_beta_vs_brute(tiny):
     assert (o['prob'][[0, 5, 9]] == 0).all()
 
 
+def test_blacklist_denominator_is_every_frame(tiny):
+    """The reference's Cython path never sets iter->num_blacklist, so combine_information
+    divides the summed likelihood and information of the frames that are not blacklisted
+    by tot_num_data (max_prob.c:373-374). Both oracles follow the field; with it at 0 the
+    averages are the per-frame sums over N_d, not over the frames kept."""
+    D = len(tiny['frames']['ones'])
+    bl = (np.arange(D) % 2).astype('u1')
+    fp = dict(tiny['fparams'], blacklist=bl, num_blacklist=0)
+    flags = dict(iteration=1, need_scaling=0)
+    o = P.run_oracle(dict(tiny, fparams=fp), flags)
+    _check(o, brute.iterate(tiny['det'], tiny['quats'], tiny['model1'], tiny['frames'], fp, flags))
+    keep = bl == 0
+    assert abs(o['avg_likelihood'] - o['likelihood'][keep].sum() / D) <= 1e-13 * abs(o['avg_likelihood'])
+    assert abs(o['mutual_info'] - o['info'][keep].sum() / D) <= 1e-13 * abs(o['mutual_info'])
+    # with the field set to the true count the average is over the kept frames instead
+    o2 = P.run_oracle(dict(tiny, fparams=dict(fp, num_blacklist=int(bl.sum()))), flags)
+    assert abs(o2['avg_likelihood'] * keep.sum() - o['avg_likelihood'] * D) <= 1e-12 * abs(o['avg_likelihood'] * D)
+
+
+@pytest.mark.parametrize('size', [9, 10])
+@pytest.mark.parametrize('alpha', [0., 0.5])
+def test_update_model_vs_brute(size, alpha):
+    """orc_update_model (recon.pyx:112-149 with symmetrize_friedel, model.c:470-490) against
+    the vectorised numpy restatement: Friedel average of the weighted sums before the
+    divide, zero weights, rms change and the alpha blend, odd and even grids."""
+    from oracle import core
+    rng = np.random.default_rng(size)
+    m1 = rng.random((size,) * 3)
+    m2 = rng.random((size,) * 3) * 3.
+    w = rng.random((size,) * 3) * 2.
+    w[rng.random(w.shape) < 0.2] = 0.
+    w[0, 0, 0] = w[-1, -1, -1] = 0.  # a Friedel pair with no weight on either side
+    b1, b2, brms = brute.update_model(m1.copy(), m2.copy(), w, alpha)
+    o1, o2 = m1.copy(), m2.copy()
+    rms = core.update_model(o1, o2, w.copy(), alpha)
+    assert P.rel_err(o2, b2) <= 1e-14 and P.rel_err(o1, b1) <= 1e-14
+    assert abs(rms - brms) <= 1e-14 * brms
+    # a planted mistake in the oracle's inputs (dividing before the Friedel average, the
+    # legacy order) is detected
+    d1, d2 = m1.copy(), np.where(w > 0, m2 / np.where(w > 0, w, 1.), m2)
+    core.update_model(d1, d2, (w > 0).astype('f8'), alpha)
+    assert P.rel_err(d2, b2) > 1e-6
+
+
 def test_single_orientation_closed_form(tiny):
     """One orientation of weight 1: every frame has P = 1, info = 0 and
     likelihood = R - sum_fact with R = -u rescale + sum_t K log W."""
