@@ -64,22 +64,52 @@ def build_recon(prob, cfg, device=0, dense=None, precision=0, start=None, backgr
     return recon, itr
 
 
-def cpu_baseline(prob, itr, num_rot_sample, threads, background=None):
-    """Oracle (OpenMP restatement of the reference engine) on all frames and the first
-    `num_rot_sample` orientations; pairs/s scales linearly in the orientation count."""
+def cpu_model_name():
+    try:
+        with open('/proc/cpuinfo') as fh:
+            for line in fh:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpu_baseline(prob, itr, frame_counts, threads, background=None):
+    """Iteration 1 of the oracle engine (OpenMP restatement of the reference engine) on the
+    first n frames at the FULL orientation set, for each n in `frame_counts`, and a
+    least-squares fit T(n) = a + b n. The fit separates the per-orientation cost a (expand,
+    merge) from the per-frame cost b, which grows with the orientation count because the
+    reference looks every (frame, orientation) up by a linear search of the frame's
+    per-thread keep list (max_update.c:86-93), so a sample over fewer orientations would
+    overstate the rate. Returns pairs/s at the workload's frame count and the sample."""
     from oracle import core
+    from dragonfly_amd import synth
     f = prob['frames']
-    quats = itr.quat.quats[:num_rot_sample].copy()
-    fp = dict(scale=np.ones(itr.tot_num_data), beta=itr.beta_start.copy(),
-              blacklist=itr.blacklist, fcounts=itr.fcounts, sum_fact=itr.sum_fact,
-              mean_count=itr.mean_count[0])
-    t0 = time.perf_counter()
-    core.maximize(prob['det'], quats, itr.model.model1 if itr.model.model1 is not None else None,
-                  f, fp, dict(iteration=1, need_scaling=itr.params.need_scaling, update_scale=1),
-                  num_threads=threads, background=background)
-    dt = time.perf_counter() - t0
-    pairs = itr.tot_num_data * quats.shape[0]
-    return pairs / dt, dt
+    R = itr.quat.num_rot
+    D = itr.tot_num_data
+    times = []
+    for n in frame_counts:
+        sub = synth.take_frames(f, n)
+        fp = dict(scale=np.ones(n), beta=itr.beta_start[:n].copy(), blacklist=np.zeros(n, 'u1'),
+                  fcounts=itr.fcounts[:n], sum_fact=itr.sum_fact[:n], mean_count=itr.mean_count[0])
+        bg = None if background is None else dict(background=background['background'],
+                                                  bgscale=np.ones(n))
+        t0 = time.perf_counter()
+        core.maximize(prob['det'], itr.quat.quats, itr.model.model1, sub, fp,
+                      dict(iteration=1, need_scaling=itr.params.need_scaling, update_scale=1),
+                      num_threads=threads, background=bg)
+        times.append(time.perf_counter() - t0)
+        log('cpu baseline: %d frames x %d orientations, %d threads: %.2f s' % (n, R, threads, times[-1]))
+    b, a = np.polyfit(np.asarray(frame_counts, 'f8'), np.asarray(times), 1)
+    t_full = a + b * D
+    sample = ('iteration 1 of the oracle engine (OpenMP restatement of the reference) at all %d '
+              'orientations on the first %s frames: %s s; fit T = a + b N_d with a = %.2f s, '
+              'b = %.4g s/frame, extrapolated to %d frames: %.0f s'
+              % (R, '/'.join(str(n) for n in frame_counts), '/'.join('%.1f' % t for t in times), a, b, D,
+                 t_full))
+    return D * R / t_full, sample, dict(frames=list(frame_counts), seconds=[round(t, 3) for t in times],
+                                        a_s=round(float(a), 3), b_s_per_frame=float('%.4g' % b))
 
 
 def photon_stats(prob, itr):
@@ -96,44 +126,77 @@ def photon_stats(prob, itr):
     return out
 
 
+def kernel_source_hash():
+    """sha256 (16 hex digits) of the HIP sources the library is built from: a measured
+    traffic figure is used only while the kernels it was measured on are unchanged."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, 'dragonfly_amd', 'csrc')
+    for name in sorted(os.listdir(csrc)):
+        if name.endswith(('.hip', '.h', '.cpp')):
+            with open(os.path.join(csrc, name), 'rb') as fh:
+                h.update(name.encode() + b'\0' + fh.read())
+    return h.hexdigest()[:16]
+
+
 def load_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` on this workload from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy), or None."""
+    """(HBM-side bytes per launch of `kernel` on this workload, profile note) from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy),
+    or (None, reason) when there is none or it was measured on other kernel sources."""
     path = os.path.join(ROOT, 'profiles', 'pmc_traffic.json')
     try:
         with open(path) as fh:
             tab = json.load(fh)
     except (OSError, ValueError):
-        return None
+        return None, 'no profiles/pmc_traffic.json'
     ent = tab.get(workload, {}).get(kernel)
-    return None if ent is None else float(ent['bytes_per_launch'])
+    if ent is None:
+        return None, 'kernel not in profiles/pmc_traffic.json for this workload'
+    src = tab.get(workload, {}).get('_src_sha16')
+    if src != kernel_source_hash():
+        return None, 'profiles/pmc_traffic.json measured on other kernel sources (%s)' % src
+    return float(ent['bytes_per_launch']), tab[workload].get('_profile', 'profiles/pmc_traffic.json')
 
 
-def roofline(prob, itr, stage_tot, steps, nchunks, pairs, workload):
-    """Roofline of the dominant gather kernel (likelihood or tomogram gather, whichever takes
-    longer), timed with HIP events on the engine's stream.
+def roofline(prob, itr, stage_tot, steps, nchunks, pairs, workload, value_bytes=8):
+    """HBM roofline of the dominant photon kernel (likelihood or tomogram gather, whichever
+    takes longer), timed with HIP events on the engine's stream.
 
-    Algorithmic bytes per (frame, orientation) pair: 12 per photon pixel (int32 index + f64
-    gather of log W resp. P) + 4 per multi-photon pixel (int32 count), i.e. 12 n + 4 m, over
-    mask < 1 pixels for the likelihood and mask < 2 pixels for the tomogram. Every gather is
-    counted as if it came from HBM; the kernels serve most of them from L2 and the Infinity
-    Cache, so `frac` can exceed 1 and `traffic` (measured HBM-side bytes) is the figure that
-    shows what actually crossed the memory interface."""
+    `achieved` counts compulsory bytes: every operand of one launch moved once -- for the
+    likelihood the log W chunk (8 B per orientation x pixel), the frames' photon lists
+    (int32 pixel + f64 count) and R written (8 B per pair); for the tomogram the posterior
+    chunk (8 B per pair), the pixels' photon lists (int32 frame + f64 count) and the view
+    written (8 B per orientation x pixel). `gather_bytes` is what the kernel requests through
+    the caches instead (one 8-byte log W or P value per photon and orientation, 12 n + 4 m
+    bytes per pair); `traffic` is the measured HBM-side bytes per launch of the same kernel on
+    the same workload (rocprofv3 FETCH_SIZE + WRITE_SIZE), when a profile of the current
+    kernel sources exists. value_bytes 4: the gathered operand (log W, resp. P) is FP32."""
     ps = photon_stats(prob, itr)
+    D = itr.tot_num_data - int((itr.blacklist != 0).sum())
+    npix = len(prob['det']['mask'])
+    C = itr.quat.num_rot / nchunks
     kern = {'likelihood': ('likelihood_kernel', ps['lik']),
             'tomogram': ('tomogram_part_kernel', ps['tomo'])}
     stage = max(kern, key=lambda k: stage_tot.get(k, 0.))
     name, (n, m) = kern[stage]
     launch_ms = stage_tot[stage] / steps / nchunks
-    bytes_per_launch = pairs / nchunks * (12. * n + 4. * m)
-    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
-    traffic = load_traffic(workload, name)
+    nnz = n * itr.tot_num_data
+    vb = float(value_bytes)
+    if stage == 'likelihood':
+        compulsory = vb * C * npix + 12. * nnz + 8. * D * C
+    else:
+        compulsory = vb * D * C + 12. * nnz + 8. * C * npix
+    gather = pairs / nchunks * ((vb + 4.) * n + 4. * m)
+    achieved = compulsory / (launch_ms * 1e-3) / 1e9
+    traffic, note = load_traffic(workload, name) if workload else (None, 'not the default workload size')
     out = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
                frac=round(achieved / HBM_PEAK_GBS, 4),
                traffic=None if traffic is None else round(traffic),
-               kernel=name, launch_ms=round(launch_ms, 3),
-               algorithmic_bytes=round(bytes_per_launch))
+               kernel=name, launch_ms=round(launch_ms, 3), compulsory_bytes=round(compulsory),
+               gather_bytes=round(gather), gather_GBs=round(gather / (launch_ms * 1e-3) / 1e9, 1),
+               traffic_source=note)
     if traffic is not None:
+        out['traffic_over_compulsory'] = round(traffic / compulsory, 2)
         out['hbm_frac_measured'] = round(traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     return out
 
@@ -208,20 +271,18 @@ def bench_dense(args):
         achieved = flops / (launch_ms * 1e-3) / 1e12
         # measured HBM-side bytes of the one-term product at the default sizes
         default_size = args.num_data is None and args.num_div is None and terms == 1
-        traffic = load_traffic(args.config, 'mp_gemm_kernel_likelihood') if default_size else None
+        traffic, note = load_traffic(args.config, 'mp_gemm_kernel_likelihood') if default_size else \
+            (None, 'not the default workload size')
         roof = dict(bound='mfma', achieved=round(achieved, 1), peak=2500., unit='TFLOP/s',
                     frac=round(achieved / 2500., 4), traffic=None if traffic is None else round(traffic),
                     kernel='mp_gemm_kernel (likelihood)', launch_ms=round(launch_ms, 3),
-                    algorithmic_flops=flops)
+                    algorithmic_flops=flops, traffic_source=note)
         if traffic is not None:
             roof['hbm_frac_measured'] = round(traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         path = 'mixed precision, %d-term FP16 log W' % terms
         dtype = 'f16 products / f32 accumulate / f64 rest'
     else:
-        roof = roofline(prob, itr, st_mp, args.steps, nch_mp, pairs, None)
-        for k in ('achieved', 'algorithmic_bytes'):
-            roof[k] = round(roof[k] / 2.)    # 4-byte log W / P gathers instead of 8
-        roof['frac'] = round(roof['achieved'] / HBM_PEAK_GBS, 4)
+        roof = roofline(prob, itr, st_mp, args.steps, nch_mp, pairs, None, value_bytes=4)
         path = 'FP32 log W and posterior gathers, FP64 sums'
         dtype = 'f32 gathers / f64 rest'
     
@@ -520,7 +581,8 @@ def main():
     ap.add_argument('--config', default='cfg2')
     ap.add_argument('--num-data', type=int, default=None)
     ap.add_argument('--num-div', type=int, default=None)
-    ap.add_argument('--cpu-rot', type=int, default=3072, help='orientations in the CPU sample')
+    ap.add_argument('--cpu-frames', default='64,512',
+                    help='frame counts of the CPU baseline runs (every orientation, fit a + b N_d)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--precision', type=int, default=None,
                     help='dense-frame configurations: 1 (default) or 2 FP16 terms of log W; 3 (any '
@@ -629,12 +691,14 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
+        recon.free()
+        itr.model._engine = None
         itr.model.model1 = start_model
-        v_cpu, dt = cpu_baseline(prob, itr, args.cpu_rot, threads,
-                                 None if bgd is None else dict(background=bgd, bgscale=np.ones(D)))
+        frames = [int(x) for x in args.cpu_frames.split(',')]
+        v_cpu, sample, fit = cpu_baseline(prob, itr, frames, threads,
+                                          None if bgd is None else dict(background=bgd))
         cpu = dict(value=round(v_cpu, 1), unit='frame×orientation pairs/s', cores=threads,
-                   kind='port', sample='iteration 1 of the oracle engine (OpenMP restatement of the '
-                   'reference) on all %d frames x first %d orientations: %.1f s' % (D, args.cpu_rot, dt))
+                   kind='port', sample=sample, fit=fit, nproc=os.cpu_count(), cpu_model=cpu_model_name())
     out = {
         'metric': METRIC, 'value': round(value, 1), 'unit': 'frame×orientation pairs/s',
         'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(ms_per_step, 2),

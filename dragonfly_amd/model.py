@@ -132,13 +132,43 @@ class Model:
             self._weight = self._engine.get('inter_weight')
         return self
 
-    def slice_gen(self, quat, det, engine):
-        """View of model1 at one orientation (Model.slice_gen, model.pyx:105-131), on the GPU."""
-        engine.set_detector(det.qvals, det.corr, det.raw_mask)
-        engine.set_model(self.model1)
-        return engine.slice_gen(quat)
+    def _slice_engine(self, det, vol):
+        """A private engine holding `det` and one mode's volume `vol` (never the engine of a
+        running reconstruction, whose state these calls would disturb)."""
+        from ._lib import Engine
+        eng = getattr(self, '_slicer', None)
+        if eng is None:
+            eng = self._slicer = Engine(0)
+        eng.set_detector(det.qvals, det.corr, det.raw_mask)
+        eng.set_model(np.ascontiguousarray(vol, dtype='f8'))
+        return eng
 
-    def slice_merge(self, quat, view, det, engine):
-        """Merge one view into the engine's model2/inter_weight (model.pyx:133-150)."""
-        engine.set_detector(det.qvals, det.corr, det.raw_mask)
-        engine.slice_merge(quat, view)
+    def _mode_view(self, arr, mode):
+        if not 0 <= mode < self.num_modes:
+            raise ValueError('mode %d out of range (num_modes %d)' % (mode, self.num_modes))
+        return arr if self.num_modes == 1 else arr[mode]
+
+    def slice_gen(self, quat, det, mode=0, view=None):
+        """View of model1 (mode `mode`) at one orientation on `det` (Model.slice_gen,
+        model.pyx:105-131 -> slice_gen3d, model.c:11-59), on the GPU; written into `view`
+        when given."""
+        if self.model1 is None:
+            raise AttributeError('Allocate model1 first')
+        eng = self._slice_engine(det, self._mode_view(self.model1, mode))
+        out = eng.slice_gen(np.ascontiguousarray(quat, dtype='f8')[:4])
+        if view is None:
+            return out
+        view[:] = out
+        return view
+
+    def slice_merge(self, quat, view, det, mode=0):
+        """Merge one view into model2 / inter_weight of mode `mode` (Model.slice_merge,
+        model.pyx:133-150 -> slice_merge3d, model.c:171-244). The view is merged on the GPU
+        into zeroed volumes, which are then added to the host arrays."""
+        if self.model2 is None:
+            raise AttributeError('Allocate model2 first')
+        eng = self._slice_engine(det, self._mode_view(self.model2, mode))
+        eng.clear_merge()
+        eng.slice_merge(np.ascontiguousarray(quat, dtype='f8')[:4], view)
+        self._mode_view(self.model2, mode)[...] += eng.get('model2').reshape((self.size,) * 3)
+        self._mode_view(self.inter_weight, mode)[...] += eng.get('inter_weight').reshape((self.size,) * 3)

@@ -106,3 +106,32 @@ def test_recon_from_config_odd_only(tmp_path):
     assert abs(float(it_line[3]) - o['mutual_info']) <= 1e-6  # written with %f
     assert abs(float(it_line[4]) - o['avg_likelihood']) <= 1e-6 * abs(o['avg_likelihood'])
     assert abs(float(it_line[6]) - 0.7 * keep.sum() / D) <= 1e-6
+
+
+def test_model_slice_gen_merge_reference_signature():
+    """Model.slice_gen(quat, det, mode=0, view=None) and Model.slice_merge(quat, view, det,
+    mode=0) with the reference's signatures (model.pyx:105-150): the view of a chosen mode
+    bit for bit as the oracle's slice_gen3d, written into `view` when given, and a merge
+    that adds to that mode's model2 / inter_weight only."""
+    from oracle import core
+    p = P.build(P.tiny_config())
+    det = df.CDetector.from_arrays(p['det']['qvals'], p['det']['corr'], p['det']['mask'])
+    S = p['model1'].shape[0]
+    m = df.Model(S, 2)
+    m.allocate('')
+    m.model1 = np.stack([p['model1'], p['model1'][::-1].copy()])
+    q = p['quats'][17]
+    for mode in (0, 1):
+        want = core.slice_gen3d(q, m.model1[mode], det.qvals, det.corr)
+        got = m.slice_gen(q, det, mode=mode)
+        assert np.array_equal(got.view('u8'), want.view('u8')), mode
+    buf = np.zeros(det.num_pix)
+    assert m.slice_gen(q, det, 1, buf) is buf and np.array_equal(buf, got)
+    view = np.random.default_rng(5).random(det.num_pix)
+    m2, w2 = np.zeros((S,) * 3), np.zeros((S,) * 3)
+    core.slice_merge3d(q, view, m2, w2, det.qvals, det.corr, det.raw_mask)
+    m.slice_merge(q, view, det, mode=1)
+    assert P.rel_err(m.model2[1], m2) < 1e-14 and P.rel_err(m.inter_weight[1], w2) < 1e-14
+    assert not m.model2[0].any() and not m.inter_weight[0].any()
+    with pytest.raises(ValueError):
+        m.slice_gen(q, det, mode=2)

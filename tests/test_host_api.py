@@ -372,3 +372,29 @@ def test_update_beta_mean_over_all_frames():
     kept = int((itr.blacklist == 0).sum())
     assert 0 < kept < itr.tot_num_data
     assert abs(mean - 0.4 * kept / itr.tot_num_data) < 1e-15
+
+
+def test_reference_import_name_and_command():
+    """The reference's import name and console command (pyproject.toml:69 and
+    dragonfly/__init__.py there): `dragonfly` exposes the same objects, its submodules
+    import, and the `dragonfly.emc` script resolves to the reconstruction's main()."""
+    import configparser
+    import subprocess
+    import sys
+    import dragonfly
+    import dragonfly.iterate
+    import dragonfly.recon
+    from dragonfly.recon import EMCRecon, main
+    assert dragonfly.EMCRecon is EMCRecon is df.EMCRecon
+    assert dragonfly.iterate.Iterate is df.Iterate and main is df.recon.main
+    for name in ('CDetector', 'CDataset', 'EMCWriter', 'Quaternion', 'Model', 'EMCParams', 'Iterate'):
+        assert getattr(dragonfly, name) is getattr(df, name)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cp = configparser.ConfigParser()
+    cp.read(os.path.join(root, 'pyproject.toml'))
+    target = cp.get('project.scripts', "'dragonfly.emc'").strip("'")
+    mod, fn = target.split(':')
+    assert (mod, fn) == ('dragonfly.recon', 'main')
+    out = subprocess.run([sys.executable, '-m', 'dragonfly.recon', '--help'], cwd=root, capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0 and 'niter' in out.stdout
