@@ -75,41 +75,56 @@ def cpu_model_name():
     return 'unknown'
 
 
-def cpu_baseline(prob, itr, frame_counts, threads, background=None):
-    """Iteration 1 of the oracle engine (OpenMP restatement of the reference engine) on the
-    first n frames at the FULL orientation set, for each n in `frame_counts`, and a
-    least-squares fit T(n) = a + b n. The fit separates the per-orientation cost a (expand,
-    merge) from the per-frame cost b, which grows with the orientation count because the
-    reference looks every (frame, orientation) up by a linear search of the frame's
-    per-thread keep list (max_update.c:86-93), so a sample over fewer orientations would
-    overstate the rate. Returns pairs/s at the workload's frame count and the sample."""
+def cpu_baseline(prob, itr, threads, background=None, target_rot=(12800, 25600), frames=(64, 512)):
+    """Iteration 1 of the oracle engine (OpenMP restatement of the reference engine) timed on
+    orientation subsets (every k-th orientation, so that about target_rot orientations
+    remain) and frame subsets, fitted to the reference engine's cost structure and
+    extrapolated to the workload:
+
+        T(N, n) = alpha N + n (beta N + gamma N^2)
+
+    for N orientations and n frames: alpha N is the per-orientation expand and merge,
+    beta N the per-frame photon sums, and gamma N^2 the lookup of every (frame, orientation)
+    by a linear search of the frame's per-thread keep list of ~N / threads entries
+    (max_update.c:86-93). Three runs: (N1, n1), (N1, n2), (N2, n2). Returns pairs/s at the
+    workload's size, the sample text and the fit."""
     from oracle import core
     from dragonfly_amd import synth
     f = prob['frames']
     R = itr.quat.num_rot
-    D = itr.tot_num_data
-    times = []
-    for n in frame_counts:
+    D = itr.tot_num_data - int((itr.blacklist != 0).sum())
+    ks = [max(1, int(round(R / t))) for t in target_rot]
+    runs = [(ks[0], frames[0]), (ks[0], frames[1]), (ks[1], frames[1])]
+    pts = []
+    for k, n in runs:
         sub = synth.take_frames(f, n)
+        quats = itr.quat.quats[::k].copy()
         fp = dict(scale=np.ones(n), beta=itr.beta_start[:n].copy(), blacklist=np.zeros(n, 'u1'),
                   fcounts=itr.fcounts[:n], sum_fact=itr.sum_fact[:n], mean_count=itr.mean_count[0])
-        bg = None if background is None else dict(background=background['background'],
-                                                  bgscale=np.ones(n))
+        bg = None if background is None else dict(background=background['background'], bgscale=np.ones(n))
         t0 = time.perf_counter()
-        core.maximize(prob['det'], itr.quat.quats, itr.model.model1, sub, fp,
+        core.maximize(prob['det'], quats, itr.model.model1, sub, fp,
                       dict(iteration=1, need_scaling=itr.params.need_scaling, update_scale=1),
                       num_threads=threads, background=bg)
-        times.append(time.perf_counter() - t0)
-        log('cpu baseline: %d frames x %d orientations, %d threads: %.2f s' % (n, R, threads, times[-1]))
-    b, a = np.polyfit(np.asarray(frame_counts, 'f8'), np.asarray(times), 1)
-    t_full = a + b * D
-    sample = ('iteration 1 of the oracle engine (OpenMP restatement of the reference) at all %d '
-              'orientations on the first %s frames: %s s; fit T = a + b N_d with a = %.2f s, '
-              'b = %.4g s/frame, extrapolated to %d frames: %.0f s'
-              % (R, '/'.join(str(n) for n in frame_counts), '/'.join('%.1f' % t for t in times), a, b, D,
-                 t_full))
-    return D * R / t_full, sample, dict(frames=list(frame_counts), seconds=[round(t, 3) for t in times],
-                                        a_s=round(float(a), 3), b_s_per_frame=float('%.4g' % b))
+        pts.append((len(quats), n, time.perf_counter() - t0))
+        log('cpu baseline: %d frames x %d orientations, %d threads: %.2f s' % (n, len(quats), threads, pts[-1][2]))
+    (N1, n1, t1), (_, n2, t2), (N2, _, t3) = pts
+    b1 = (t2 - t1) / (n2 - n1)              # per-frame cost at N1
+    alpha = (t1 - n1 * b1) / N1
+    b2 = (t3 - alpha * N2) / n2             # per-frame cost at N2
+    # b = beta N + gamma N^2 through (N1, b1), (N2, b2)
+    gamma = (b2 / N2 - b1 / N1) / (N2 - N1)
+    beta = b1 / N1 - gamma * N1
+    t_full = alpha * R + D * (beta * R + gamma * R * R)
+    sample = ('iteration 1 of the oracle engine (OpenMP restatement of the reference) on %s; fit T = alpha N + '
+              'n (beta N + gamma N^2) (alpha: per-orientation expand and merge; gamma: keep-list linear '
+              'search) gives alpha = %.3g s, beta = %.3g s, gamma = %.3g s; extrapolated to %d frames x %d '
+              'orientations: %.0f s'
+              % (', '.join('%d frames x %d orientations %.1f s' % p for p in pts), alpha, beta, gamma, D, R, t_full))
+    fit = dict(points=[dict(orientations=p[0], frames=p[1], seconds=round(p[2], 3)) for p in pts],
+               alpha_s=float('%.4g' % alpha), beta_s=float('%.4g' % beta), gamma_s=float('%.4g' % gamma),
+               extrapolated_s=round(t_full, 1))
+    return D * R / t_full, sample, fit
 
 
 def photon_stats(prob, itr):
@@ -199,6 +214,25 @@ def roofline(prob, itr, stage_tot, steps, nchunks, pairs, workload, value_bytes=
         out['traffic_over_compulsory'] = round(traffic / compulsory, 2)
         out['hbm_frac_measured'] = round(traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     return out
+
+
+def roofline_fixed_stages(stage_ms, num_chunks, slots, num_pix, size, value_bytes):
+    """HBM roofline of the longer of the two per-orientation stages of an iteration, expand
+    and merge, per launch (one launch per orientation chunk), on compulsory bytes: expand
+    reads the model once (8 B per voxel) and writes log W (value_bytes per orientation x
+    pixel); the merge reads the view (8 B per orientation x pixel) and reads and writes the
+    merged volume once (16 B per voxel)."""
+    vol = float(size) ** 3
+    C = slots / num_chunks
+    comp = {'expand': 8. * vol + value_bytes * C * num_pix, 'merge': 8. * C * num_pix + 16. * vol}
+    stage = max(comp, key=lambda k: stage_ms.get(k, 0.))
+    launch_ms = stage_ms[stage] / num_chunks
+    achieved = comp[stage] / (launch_ms * 1e-3) / 1e9
+    return dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                traffic_source='no counter profile of this workload', stage=stage,
+                kernel='merge_brick_model_kernel' if stage == 'merge' else 'expand_kernel',
+                launch_ms=round(launch_ms, 3), compulsory_bytes=round(comp[stage]))
 
 
 def run_steps(recon, itr, warmup, steps, keep_models=False):
@@ -304,47 +338,34 @@ def bench_dense(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_learned(args):
-    """Learned orientation proposal at fine num_div 16 against the plain FP64 full search.
+def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
+    """One beta setting of the learned-proposal workload (fine num_div 16, coarse num_div 4).
 
-    The situation of late iterations: the model is near the truth (iteration k starts from
-    the true intensity at photon scale) and beta is the reference schedule's value after 40
-    iterations (beta_start x 4: jump sqrt(2) every 10 iterations), when posteriors are sharp
-    enough for a proposal to pay. (1) Plain FP64 iteration k over every fine orientation; its
-    posteriors summed over the Voronoi cells of the coarse sampling are the training targets
-    (the method's own output of an earlier iteration), its updated model and scales the input
-    of iteration k + 1. (2) Timed: plain FP64 iteration k + 1. (3) The network (polar photon
-    histogram -> coarse-cell logits) is trained on 90% of the frames. (4) Timed: iteration
-    k + 1 from the same model and scales with proposals: top-k cells per frame with the
-    probability they hold, frames whose k cells hold less than the threshold fall back to
-    every cell (the plain search), one refinement iteration over the chosen cells, plus the
-    model update (after one untimed iteration for the one-time set-up). Reported:
-    full-search equivalent pairs/s of (4), its speedup over (2), the fallback rate (all
-    frames and the held-out 10%), the fine posterior mass the proposals capture on held-out
-    frames, and the relative error of the model (4) gives against (2)."""
+    (1) Plain FP64 iteration k over every fine orientation from the true intensity at photon
+    scale; its posterior summed over the Voronoi cells of the coarse sampling gives the
+    training targets (the method's own output of an earlier iteration) and its updated model
+    and scales the input of iteration k + 1. (2) Plain FP64 iteration k + 1, timed
+    (`fp64_steps` times from the same input): the denominator and the reference model.
+    (3) The network (polar photon histogram -> coarse-cell logits) is trained on 90% of the
+    frames. (4) Iteration k + 1 from the same input with proposals, timed: a full search over
+    the 3240 coarse orientations (expectation step only) judges every proposal -- a frame keeps
+    its proposed cells (top-k of the network, joined with the coarse search's most likely cell)
+    when the coarse posterior puts at least `judge_threshold` of its mass inside them, and
+    falls back to the full fine search otherwise; then one refinement iteration over the kept
+    cells (FP32 gathers) and the model update. Each timed step includes the coarse pass, the
+    proposals, the lists and the iteration. (5) The same with the no-network proposals: each
+    frame's top-k cells of iteration k's cell posterior."""
     import torch
-    from dragonfly_amd import _lib, synth, proposal
-    log('generating %s workload ...' % args.config)
-    prob = synth.make_problem(args.config, device='cuda', num_data=args.num_data, num_div=args.num_div,
-                              batch=1024)
-    cfg = dict(prob['cfg'])
-    if args.top_k:
-        cfg['top_k'] = args.top_k
-    if args.mass_threshold:
-        cfg['mass_threshold'] = args.mass_threshold
+    from dragonfly_amd import _lib, proposal
     det, mask = prob['det'], prob['det']['mask']
     true_model = np.ascontiguousarray(prob['intens'] * prob['frames']['rescale'])
-    beta_factor = float(cfg.get('beta_factor', 1.))
+    k_top, tau = cfg['top_k'], args.judge_threshold
 
     def recon_for(num_div, precision=0):
         recon, itr = build_recon(prob, dict(cfg, num_div=num_div), start=true_model, precision=precision)
         itr.params.beta_factor = beta_factor
         itr.params.iteration = 1
         return recon, itr
-
-    def diagnostics(recon, itr):
-        return dict(loglik=itr.likelihood, rescale=float(itr.rescale[0]), rmax=recon.rmax.copy(),
-                    psum_r=recon.psum_r.copy(), u=recon.u[0].copy(), model=itr.model.model1.copy())
 
     # (1) iteration k, plain: targets and the input of iteration k + 1
     recon, itr = recon_for(cfg['num_div'])
@@ -354,125 +375,178 @@ def bench_learned(args):
     qmap = _lib.voronoi_subset(qcoarse, itr.quat.quats)
     t0 = time.perf_counter()
     recon.run_iteration()
-    log('iteration k (plain FP64, with set-up): %.3f s' % (time.perf_counter() - t0))
+    log('beta x %g: iteration k (plain FP64, with set-up): %.3f s' % (beta_factor, time.perf_counter() - t0))
     post = recon.engine.cell_posterior(num_coarse, qmap)
     model_k1 = itr.model.model1.copy()
     scale_k1 = np.array(itr.scale, dtype='f8')
-    # (2) iteration k + 1, plain
-    itr.params.iteration = 2
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    recon.run_iteration()
-    torch.cuda.synchronize()
-    plain_s = time.perf_counter() - t0
-    log('iteration k + 1, plain FP64: %.3f s' % plain_s)
+
+    def start_k1(rec, it):
+        it.model.model1 = model_k1
+        it.scale[:] = scale_k1
+        rec.engine.set_scale(scale_k1)
+        it.params.iteration = 2
+
+    # (2) iteration k + 1, plain, timed
+    plain_times = []
+    for _ in range(args.fp64_steps):
+        start_k1(recon, itr)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        recon.run_iteration()
+        torch.cuda.synchronize()
+        plain_times.append(time.perf_counter() - t0)
+    plain_s = float(np.median(plain_times))
+    log('beta x %g: iteration k + 1, plain FP64: %s s' % (beta_factor, ['%.3f' % t for t in plain_times]))
     st_plain = recon.last_stats['stage_ms']
-    diag_p = diagnostics(recon, itr)
+    model_plain = itr.model.model1.copy()
+    rmax_plain = recon.rmax.copy()
+    beta_k1 = itr.beta.copy()
     recon.free()
     # (3) training
     net = proposal.OrientationProposal(num_coarse, nbins=4096, width=1024, depth=3, seed=0)
     off, pix, cnt = proposal.photon_lists(prob['frames'], mask)
-    bins = proposal.polar_bins(det['qvals'], mask)
-    X = net.features(off, pix, cnt, bins)
+    X = net.features(off, pix, cnt, proposal.polar_bins(det['qvals'], mask))
     Y = torch.tensor(post, dtype=torch.float32, device='cuda')
     ntr = int(0.9 * D)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    hist = net.fit(X, Y, ntr, epochs=args.epochs, batch=1024, lr=1e-3, log=log, wd=args.wd)
+    hist = net.fit(X, Y, ntr, epochs=args.epochs, batch=1024, lr=1e-3, wd=args.wd)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
+    log('beta x %g: trained %d epochs in %.1f s, loss %.4f' % (beta_factor, args.epochs, train_s, hist[-1]))
+    # previous iteration's cell posterior, kept on the device for the no-network proposals
+    logpost = torch.log(Y.clamp_min(1e-30))
     del Y
-    held = post[ntr:]
-    top_h, mass_h = net.propose(X[ntr:].contiguous(), D - ntr, cfg['top_k'], with_mass=True)
-    captured = np.take_along_axis(held, top_h, 1).sum(1)
-    best = np.sort(held, axis=1)[:, -cfg['top_k']:].sum(1)
-    held_fallback = mass_h < cfg['mass_threshold']
 
-    # (4) iteration k + 1 with proposals
+    # (4), (5) iteration k + 1 with judged proposals
+    judge_itr = itr
+    judge_itr.model.model1 = model_k1
+    judge = proposal.CoarseJudge(judge_itr, cfg['coarse_div'], precision=args.judge_precision)
     recon, itr = recon_for(cfg['num_div'], precision=args.precision)
-    itr.params.iteration = 2
-    lists = {}
+    start_k1(recon, itr)
 
-    def start_k1():
-        itr.model.model1 = model_k1
-        itr.scale[:] = scale_k1
-        recon.engine.set_scale(scale_k1)
+    def sources():
+        return {'network': lambda: net.propose(X, D, k_top),
+                'previous_posterior': lambda: proposal.topk_device(logpost, D, num_coarse, k_top)[0]}
 
-    def propose():
-        top, mass = net.propose(X, D, cfg['top_k'], with_mass=True)
-        off, rq, fallback, lists['buf'] = proposal.refine_lists(top, mass, num_coarse, cfg['mass_threshold'],
-                                                                 lists.get('buf'))
-        itr.set_refine_lists(cfg['coarse_div'], off, rq)
+    def one_step(source):
+        """Judge, proposals, lists and the refinement iteration; returns (seconds, info)."""
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        judge.estep(model_k1, scale_k1, beta_k1, 2, cfg['need_scaling'])
+        top = sources()[source]()
+        offs, cells = proposal.union_cells(top, judge.argmax_cells())
+        mass = judge.mass(offs, cells)
+        accept = mass >= tau
+        rel_off, rel_quat = proposal.refine_lists_csr(offs, cells, accept, num_coarse)
+        itr.set_refine_lists(cfg['coarse_div'], rel_off, rel_quat)
         recon.engine.set_refine(itr.num_coarse, itr.quat_mapping, itr.rel_off, itr.rel_quat)
-        lists['top'] = top
-        return fallback
-    start_k1()
-    propose()
-    recon.run_iteration()
-    start_k1()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    fallback = propose()
-    torch.cuda.synchronize()
-    propose_s = time.perf_counter() - t0
-    recon.run_iteration()
-    torch.cuda.synchronize()
-    learned_s = time.perf_counter() - t0
-    log('iteration k + 1 with proposals: %.3f s (proposals and subset set-up %.3f s), fallback %.4f'
-        % (learned_s, propose_s, fallback.mean()))
-    st_learned = recon.last_stats['stage_ms']
-    diag_l = diagnostics(recon, itr)
-    recon.free()
-    # median angle between each frame's most likely orientation and its true orientation
-    qf = itr.quat.quats[:, :4]
-    qt = prob['frames']['true_quats'][:D, :4]
+        t1 = time.perf_counter()
+        recon.run_iteration()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return t2 - t0, dict(setup_s=t1 - t0, accept=accept, top=top, mass=mass)
 
-    def med_angle(rmax):
-        ok = rmax >= 0
-        dots = np.abs((qf[rmax[ok]] * qt[ok]).sum(1)).clip(0., 1.)
-        return float(np.degrees(2. * np.median(np.arccos(dots))))
-    # iteration k's posterior mass in the accepted proposals (every frame; held-out included)
-    acc = ~fallback
-    cap_all = np.take_along_axis(post, lists['top'], 1).sum(1)[acc] if acc.any() else np.zeros(1)
-    mp, ml = diag_p['model'], diag_l['model']
-    model_err = float(np.abs(ml - mp).max() / np.abs(mp).max())
-    diag = dict(loglik_rel_err=abs(diag_l['loglik'] - diag_p['loglik']) / abs(diag_p['loglik']),
-                rmax_angle_learned_deg=med_angle(diag_l['rmax']), rmax_angle_fp64_deg=med_angle(diag_p['rmax']),
-                rescale_rel_err=abs(diag_l['rescale'] - diag_p['rescale']) / diag_p['rescale'],
-                rmax_agree=float(np.mean(diag_l['rmax'] == diag_p['rmax'])),
-                psum_r_rel_err=float(np.abs(diag_l['psum_r'] - diag_p['psum_r']).max() / np.abs(diag_p['psum_r']).max()),
-                u_rel_err=float(np.abs(diag_l['u'] - diag_p['u']).max() / np.abs(diag_p['u']).max()),
-                model_rms_rel_err=float(np.sqrt(np.mean((ml - mp) ** 2) / np.mean(mp ** 2))))
+    results = {}
+    for source, nsteps, nwarm in (('network', timed_steps, warmup), ('previous_posterior', args.baseline_steps, 1)):
+        for _ in range(max(nwarm, 1)):
+            start_k1(recon, itr)
+            one_step(source)
+        times = []
+        for _ in range(nsteps):
+            start_k1(recon, itr)
+            dt, info = one_step(source)
+            times.append(dt)
+        ml = itr.model.model1.copy()
+        step_s = float(np.median(times))
+        acc = info['accept']
+        held = np.arange(D) >= ntr
+        # iteration k's fine posterior mass inside the proposals that were kept
+        captured = np.take_along_axis(post, info['top'], 1).sum(1)
+        res = dict(step_ms=round(step_s * 1e3, 1), steps_ms=[round(t * 1e3, 1) for t in times],
+                   num_chunks=recon.last_stats['num_chunks'],
+                   setup_ms=round(info['setup_s'] * 1e3, 1), speedup_vs_fp64=round(plain_s / step_s, 2),
+                   fallback_rate=round(float(1. - acc.mean()), 4),
+                   heldout_fallback_rate=round(float(1. - acc[held].mean()), 4),
+                   judge_mass_accepted_mean=round(float(info['mass'][acc].mean()), 5) if acc.any() else None,
+                   prev_mass_in_topk_mean=round(float(captured.mean()), 5),
+                   model_rel_err=float('%.3g' % (np.abs(ml - model_plain).max() / np.abs(model_plain).max())),
+                   model_rms_rel_err=float('%.3g' % np.sqrt(np.mean((ml - model_plain) ** 2) / np.mean(model_plain ** 2))),
+                   rmax_agree=round(float(np.mean(recon.rmax == rmax_plain)), 5),
+                   stage_ms={k: round(v, 1) for k, v in recon.last_stats['stage_ms'].items()})
+        results[source] = res
+        log('beta x %g, %s: %s' % (beta_factor, source, json.dumps({k: v for k, v in res.items() if k != 'stage_ms'})))
+    judge_ms = judge.stats['stage_ms'] if judge.stats else {}
+    judge.close()
+    recon.free()
+    return dict(D=D, R=R, num_coarse=num_coarse, plain_s=plain_s, plain_times=plain_times, st_plain=st_plain,
+                train_s=train_s, epochs=args.epochs, loss=hist[-1], results=results,
+                judge_stage_ms={k: round(v, 1) for k, v in judge_ms.items()}, itr=itr)
+
+
+def bench_learned(args):
+    """Learned orientation proposal at fine num_div 16 against the plain FP64 full search,
+    at beta_start x 4 (the reference schedule's beta after 40 iterations at sqrt(2) per 10,
+    the late iterations in which proposals are meant to be used; the timed headline) and at
+    the configuration's own beta_start (reported beside it)."""
+    from dragonfly_amd import synth
+    log('generating %s workload ...' % args.config)
+    t0 = time.perf_counter()
+    prob = synth.make_problem(args.config, device='cuda', num_data=args.num_data, num_div=args.num_div,
+                              batch=1024)
+    log('generated in %.1f s' % (time.perf_counter() - t0))
+    cfg = dict(prob['cfg'])
+    if args.top_k:
+        cfg['top_k'] = args.top_k
+    late = float(cfg.get('beta_factor', 1.)) if args.beta_factor is None else args.beta_factor
+    head = learned_case(prob, cfg, late, args, args.steps, args.warmup)
+    other = None
+    if not args.single_beta and late != 1.:
+        other = learned_case(prob, cfg, 1., args, max(1, min(args.steps, 3)), 1)
+    D, R = head['D'], head['R']
+    net = head['results']['network']
+    step_s = net['step_ms'] * 1e-3
+    cpu = None
+    if not args.no_cpu:
+        itr = head['itr']
+        itr.model.model1 = np.ascontiguousarray(prob['intens'] * prob['frames']['rescale'])
+        threads = min(16, os.cpu_count() or 1)
+        v_cpu, sample, fit = cpu_baseline(prob, itr, threads)
+        cpu = dict(value=round(v_cpu, 1), unit='frame×orientation pairs/s', cores=threads, kind='port',
+                   sample=sample + ' (full search at fine num_div %d, the reference engine has no proposals)'
+                   % cfg['num_div'], fit=fit, nproc=os.cpu_count(), cpu_model=cpu_model_name())
+
+    def summary(case):
+        return dict(beta_factor=case.get('beta_factor'), fp64_ms_per_step=round(case['plain_s'] * 1e3, 1),
+                    fp64_steps_ms=[round(t * 1e3, 1) for t in case['plain_times']],
+                    train_s=round(case['train_s'], 2), train_loss=round(case['loss'], 4),
+                    judge_stage_ms=case['judge_stage_ms'], methods=case['results'])
+    head['beta_factor'] = late
     out = {
-        'metric': METRIC, 'value': round(D * R / learned_s, 1), 'unit': 'frame×orientation pairs/s',
-        'n_gpus': 1, 'steps': 1, 'warmup': 1, 'ms_per_step': round(learned_s * 1e3, 1),
-        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'metric': METRIC, 'value': round(D * R / step_s, 1), 'unit': 'frame×orientation pairs/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': net['step_ms'],
+        'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': net['speedup_vs_fp64'],
+        'vs_baseline_def': 'speedup over the plain FP64 GPU full search on the same iteration (the denominator '
+                           'the metric names); model_rel_err is the max-abs error of the updated model relative '
+                           'to its maximum',
+        'model_rel_err': net['model_rel_err'],
         'dtype': ('f32 gathers, f64 sums' if args.precision == 3 else 'f64') + ' EMC / bf16 proposal network',
         'data': 'synthetic',
-        'config': {'workload': '%s: %d frames, fine num_div %d (%d orientations), coarse num_div %d (%d), '
-                               'top-%d proposals, fallback below %.2f, beta_start x %g'
-                               % (args.config, D, cfg['num_div'], R, cfg['coarse_div'], num_coarse,
-                                  cfg['top_k'], cfg['mass_threshold'], beta_factor),
-                   'path': 'learned proposal + sparse refinement', 'precision': args.precision,
+        'config': {'workload': '%s: %d frames (mean 200 photons, scale U(0.5, 1.5)), 251^3 model, 260x257 '
+                               'detector, fine num_div %d (%d orientations), coarse num_div %d (%d), top-%d '
+                               'proposals + coarse arg-max, judged by a coarse full search (fall back below '
+                               '%.4g of its mass), beta_start x %g (timed); beta_start x 1 beside it'
+                               % (args.config, D, cfg['num_div'], R, cfg['coarse_div'], head['num_coarse'],
+                                  cfg['top_k'], args.judge_threshold, late),
+                   'path': 'learned proposal + likelihood judge + sparse refinement', 'precision': args.precision,
                    'frames': D, 'num_rot': R},
-        'speedup_vs_fp64': round(plain_s / learned_s, 2), 'fp64_ms_per_step': round(plain_s * 1e3, 1),
-        'propose_ms': round(propose_s * 1e3, 1),
-        'fallback_rate': round(float(fallback.mean()), 4),
-        'heldout_fallback_rate': round(float(held_fallback.mean()), 4),
-        'heldout_captured_mass': round(float(captured.mean()), 4),
-        'heldout_captured_mass_accepted': round(float(captured[~held_fallback].mean()), 4)
-        if (~held_fallback).any() else None,
-        'accepted_captured_mass_mean': round(float(cap_all.mean()), 5),
-        'accepted_captured_below_threshold': round(float((cap_all < cfg['mass_threshold']).mean()), 4),
-        'heldout_best_topk_mass': round(float(best.mean()), 4),
-        'heldout_best_topk_ge_threshold': round(float((best >= cfg['mass_threshold']).mean()), 4),
-        'model_rel_err': float('%.3g' % model_err), 'train_s': round(train_s, 2), 'epochs': args.epochs,
-        'weight_decay': args.wd,
-        'agreement_vs_fp64': {k: float('%.3g' % v) for k, v in diag.items()},
-        'train_loss': [round(h, 4) for h in hist[::max(1, len(hist) // 10)]] + [round(hist[-1], 4)],
-        'stage_ms_learned': {k: round(v, 1) for k, v in st_learned.items()},
-        'stage_ms_fp64': {k: round(v, 1) for k, v in st_plain.items()},
-        'roofline': None, 'cpu_baseline': None,
+        'speedup_vs_fp64': net['speedup_vs_fp64'], 'fallback_rate': net['fallback_rate'],
+        'late_beta': summary(head),
+        'config_beta': summary(dict(other, beta_factor=1.)) if other else None,
+        'roofline': roofline_fixed_stages(net['stage_ms'], net['num_chunks'], R, len(prob['det']['mask']),
+                                          cfg['size'], 4 if args.precision == 3 else 8),
+        'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -578,11 +652,12 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--config', default='cfg2')
+    ap.add_argument('--config', default='cfg5',
+                    help='workload: cfg5 (default: learned proposals at num_div 16, the metric\'s headline), '
+                         'cfg1, cfg2 (plain FP64), cfg3 (dense frames, mixed precision), cfg4 (uneven batching)')
     ap.add_argument('--num-data', type=int, default=None)
     ap.add_argument('--num-div', type=int, default=None)
-    ap.add_argument('--cpu-frames', default='64,512',
-                    help='frame counts of the CPU baseline runs (every orientation, fit a + b N_d)')
+
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--precision', type=int, default=None,
                     help='dense-frame configurations: 1 (default) or 2 FP16 terms of log W; 3 (any '
@@ -591,8 +666,18 @@ def main():
     ap.add_argument('--epochs', type=int, default=80, help='learned-proposal workload: training epochs')
     ap.add_argument('--wd', type=float, default=0., help='learned-proposal workload: L2 weight decay')
     ap.add_argument('--top-k', type=int, default=0, help='learned-proposal workload: cells proposed per frame')
-    ap.add_argument('--mass-threshold', type=float, default=0.,
-                    help='learned-proposal workload: predicted mass below which a frame falls back')
+    ap.add_argument('--judge-threshold', type=float, default=0.999,
+                    help='learned-proposal workload: coarse-search posterior mass a proposal must hold')
+    ap.add_argument('--judge-precision', type=int, default=3,
+                    help='learned-proposal workload: arithmetic of the coarse judging search (0 FP64, 3 FP32 gathers)')
+    ap.add_argument('--beta-factor', type=float, default=None,
+                    help='learned-proposal workload: beta_start factor of the timed case (default: the config\'s)')
+    ap.add_argument('--single-beta', action='store_true',
+                    help='learned-proposal workload: skip the case at the configuration\'s own beta')
+    ap.add_argument('--fp64-steps', type=int, default=2,
+                    help='learned-proposal workload: timed plain FP64 iterations (the denominator)')
+    ap.add_argument('--baseline-steps', type=int, default=2,
+                    help='learned-proposal workload: timed steps of the no-network proposals')
     ap.add_argument('--compare-frames', type=int, default=50000,
                     help='uneven-batching workload: frames of the sparse vs masked comparison')
     ap.add_argument('--autotune', action='store_true',
@@ -692,11 +777,8 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         recon.free()
-        itr.model._engine = None
         itr.model.model1 = start_model
-        frames = [int(x) for x in args.cpu_frames.split(',')]
-        v_cpu, sample, fit = cpu_baseline(prob, itr, frames, threads,
-                                          None if bgd is None else dict(background=bgd))
+        v_cpu, sample, fit = cpu_baseline(prob, itr, threads, None if bgd is None else dict(background=bgd))
         cpu = dict(value=round(v_cpu, 1), unit='frame×orientation pairs/s', cores=threads,
                    kind='port', sample=sample, fit=fit, nproc=os.cpu_count(), cpu_model=cpu_model_name())
     out = {

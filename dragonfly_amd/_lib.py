@@ -86,6 +86,7 @@ def lib():
             'emc_update_model': (C.c_int, [C.c_void_p, C.c_double, _dp]),
             'emc_normalize_scale': (C.c_int, [C.c_void_p, _dp]),
             'emc_slice_gen': (C.c_int, [C.c_void_p, _dp, _dp]),
+            'emc_subset_mass': (C.c_int, [C.c_void_p, _lp, _ip, _dp]),
             'emc_slice_merge': (C.c_int, [C.c_void_p, _dp, _dp]),
             'emc_get_field': (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
             'emc_get_sparse_prob': (C.c_int, [C.c_void_p, _ip, C.c_long, _ip, _dp, _lp]),
@@ -239,7 +240,8 @@ class Engine:
 
     def set_precision(self, mode):
         """0: plain FP64; 1: FP16 log W on matrix cores (dense frames); 2: FP16 hi + lo;
-        3: FP32 log W and posterior gathers (any frames)."""
+        3: FP32 log W and posterior gathers (any frames); 4, 5: as 1, 2 with BF16 log W and
+        counts in the likelihood product (counts <= 256)."""
         _check(lib().emc_set_precision(self._h, int(mode)))
 
     def set_frame_params(self, scale, beta, blacklist, fcounts, sum_fact, mean_count, num_blacklist=0):
@@ -288,6 +290,18 @@ class Engine:
         quat = np.ascontiguousarray(quat, dtype='f8')
         out = np.empty(self.num_pix)
         _check(lib().emc_slice_gen(self._h, _ptr(quat), _ptr(out)))
+        return out
+
+    def subset_mass(self, rel_off, rel_quat):
+        """Per frame, the posterior mass of the last (plain) iteration on the orientations
+        rel_quat[rel_off[d]:rel_off[d + 1]]."""
+        rel_off = np.ascontiguousarray(rel_off, dtype='i8')
+        rel_quat = np.ascontiguousarray(rel_quat, dtype='i4')
+        if rel_off.shape != (self.num_data + 1,):
+            raise ValueError('rel_off must have num_data + 1 entries')
+        out = np.empty(self.num_data)
+        _check(lib().emc_subset_mass(self._h, _ptr(rel_off, _lp), _ptr(rel_quat, _ip) if rel_quat.size else None,
+                                     _ptr(out)))
         return out
 
     def slice_merge(self, quat, view):

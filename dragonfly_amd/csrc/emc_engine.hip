@@ -276,6 +276,12 @@ struct emc_engine {
     int slot_pack = 1, slot_pack_built = -1;
     int sparse_rows = 0;  // frames per batch of the subset sparse posterior (0: 256 MB of rows)
     int full_part_frames = kFramesPerTomoPart;  // frames per range of the full-frame tomogram
+    int estep_only = 0;   // stop after the posterior: no tomogram, merge or scale update
+    DevBuf<float> model32;        // FP32 copy of model1 for the FP32-gather expand
+    bool model32_valid = false;   // cleared whenever model1 changes
+    DevBuf<long> mass_off;
+    DevBuf<int> mass_idx;
+    DevBuf<double> mass_out;
     std::vector<int> h_ctile0, h_ctile1;  // packed: first and last tile of each cell
     DevBuf<int> slot_cell;
     // full frames (every coarse cell): rows tile-major in RF; photon lists of mask < 2 pixels
@@ -285,13 +291,13 @@ struct emc_engine {
     DevBuf<int> fidx, ffull, ftom_frame;
     DevBuf<long> ftom_off2;
     DevBuf<double> ftom_cnt, RF, scale_full, psum_full, psum_full_part;
+    DevBuf<int> d_fof, d_fcounts;   // full index of every frame (-1: partial) and list counts
     DevBuf<uint8_t> blacklist_full;
     bool full_lists_valid = false;
     // host scratch of the per-iteration subset set-up, kept so that its pages stay mapped
     std::vector<uint32_t> h_relbits;
     std::vector<long> h_poff, h_toff, h_ftom_off2;
-    std::vector<int> h_pframe, h_ptile, h_tpair, h_ffull, h_fof, h_ftom_frame;
-    std::vector<double> h_ftom_cnt;
+    std::vector<int> h_pframe, h_ptile, h_tpair, h_ffull, h_fof;
     long num_pairs = 0;
     DevBuf<int> slot_rot, rot_slot, pair_frame, pair_tile, tile_pair, tile_frame, rbad_slot;
     DevBuf<long> pair_off, tile_off;
@@ -309,6 +315,26 @@ struct emc_engine {
 };
 
 namespace {
+
+// FP32 copy of model1 (every mode) for the FP32-gather expand, refreshed after model1 changed.
+int ensure_model32(emc_engine *e) {
+    if (e->model32_valid) return 0;
+    const size_t vol = (size_t)e->size * e->size * e->size * e->num_modes;
+    int rc;
+    if ((rc = e->model32.alloc(vol))) return rc;
+    emc::launch_copy_f32(e->stream, e->model1.p, vol, e->model32.p);
+    HIP_TRY(hipGetLastError());
+    e->model32_valid = true;
+    return 0;
+}
+
+// precision modes of the dense-frame products: 1, 2 FP16 log W (one term, hi + lo); 4, 5 the
+// same in BF16 (the tomogram product stays FP16)
+inline bool mp_mode(const emc_engine *e) {
+    return e->precision == 1 || e->precision == 2 || e->precision == 4 || e->precision == 5;
+}
+inline bool mp_two(const emc_engine *e) { return e->precision == 2 || e->precision == 5; }
+inline bool mp_bf16(const emc_engine *e) { return e->precision == 4 || e->precision == 5; }
 
 int check_ready(emc_engine *e) {
     if (!e) return fail(EMC_E_ARG, "null engine");
@@ -808,40 +834,40 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
 int build_full_lists(emc_engine *e) {
     // frame ranges of about kFramesPerTomoPart frames (at most tom_parts): fewer full frames
     // share fewer partial views, whose writes and reads would otherwise dominate
-    const int P = e->num_pix, nF = e->num_full;
+    const int P = e->num_pix, nF = e->num_full, N = e->num_data;
     const int per = e->full_part_frames;
-    const int nparts = std::max(1, std::min(e->tom_parts, (nF + per - 1) / per));
+    const int nparts = std::max(1, std::min(std::min(e->tom_parts, 64), (nF + per - 1) / per));
     e->full_parts = nparts;
+    hipStream_t s = e->stream;
     std::vector<int> &fof = e->h_fof;
-    fof.assign(e->num_data, -1);
+    fof.assign(N, -1);
     for (int f = 0; f < nF; ++f) fof[e->h_fidx[f]] = f;
-    std::vector<long> &off2 = e->h_ftom_off2;
-    off2.resize((size_t)P * nparts + 1);
-    std::vector<int> &fr = e->h_ftom_frame;
-    std::vector<double> &cn = e->h_ftom_cnt;
-    fr.clear();
-    cn.clear();
-    for (int t = 0; t < P; ++t) {
-        int x = 0;
-        for (long k = e->h_tom_off[t]; k < e->h_tom_off[t + 1]; ++k) {
-            const int f = fof[e->h_tom_frame[k]];
-            if (f < 0) continue;
-            while (x < nparts && (long)nF * x / nparts <= f) off2[(size_t)t * nparts + x++] = (long)fr.size();
-            fr.push_back(f);
-            cn.push_back(e->h_tom_cnt[k]);
-        }
-        while (x < nparts) off2[(size_t)t * nparts + x++] = (long)fr.size();
-    }
-    off2[(size_t)P * nparts] = (long)fr.size();
-    if (fr.empty()) {
-        fr.push_back(0);
-        cn.push_back(0.);
-    }
     int rc;
-    if ((rc = e->ftom_off2.upload(off2.data(), off2.size(), e->stream))) return rc;
-    if ((rc = e->ftom_frame.upload(fr.data(), fr.size(), e->stream))) return rc;
-    if ((rc = e->ftom_cnt.upload(cn.data(), cn.size(), e->stream))) return rc;
-    HIP_TRY(hipStreamSynchronize(e->stream));
+    const size_t ncell = (size_t)P * nparts;
+    if ((rc = e->d_fof.upload(fof.data(), fof.size(), s))) return rc;
+    if ((rc = e->d_fcounts.alloc(ncell))) return rc;
+    // counts per (pixel, range) on the GPU, the scan here, then the fill on the GPU
+    emc::launch_full_lists(s, 0, e->tom_off.p, e->tom_frame.p, e->tom_cnt.p, e->d_fof.p, P, std::max(nF, 1),
+                           nparts, e->d_fcounts.p, nullptr, nullptr, nullptr);
+    HIP_TRY(hipGetLastError());
+    std::vector<int> cnt(ncell);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), e->d_fcounts.p, ncell * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<long> &off2 = e->h_ftom_off2;
+    off2.resize(ncell + 1);
+    long total = 0;
+    for (size_t i = 0; i < ncell; ++i) {
+        off2[i] = total;
+        total += cnt[i];
+    }
+    off2[ncell] = total;
+    if ((rc = e->ftom_off2.upload(off2.data(), off2.size(), s))) return rc;
+    if ((rc = e->ftom_frame.alloc((size_t)std::max(total, 1L)))) return rc;
+    if ((rc = e->ftom_cnt.alloc((size_t)std::max(total, 1L)))) return rc;
+    emc::launch_full_lists(s, 1, e->tom_off.p, e->tom_frame.p, e->tom_cnt.p, e->d_fof.p, P, std::max(nF, 1), nparts,
+                           nullptr, e->ftom_off2.p, e->ftom_frame.p, e->ftom_cnt.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
     e->full_lists_valid = true;
     return 0;
 }
@@ -918,7 +944,7 @@ int ensure_work(emc_engine *e) {
     budget += e->logw.cap * sizeof(double);
     budget += e->tom_partial.cap * sizeof(double);
     budget += (e->lw_hi.cap + e->lw_lo.cap + e->p16.cap) * sizeof(_Float16);
-    const bool mp = e->precision == 1 || e->precision == 2;
+    const bool mp = mp_mode(e);
     const bool g32 = e->precision == 3;
     budget += (e->logw32.cap + e->p32.cap) * sizeof(float);
     if (mp) {
@@ -936,7 +962,7 @@ int ensure_work(emc_engine *e) {
     const bool split = !mp && !g32 && e->lik_split > 1;
     budget += e->lik_part.cap * sizeof(double);
     if (!split) e->lik_part.release();  // before the chunk buffers grow into its room
-    const size_t per_col = mp ? P * sizeof(double) + (e->precision == 2 ? 2 : 1) * e->mp_ldp * sizeof(_Float16) +
+    const size_t per_col = mp ? P * sizeof(double) + (mp_two(e) ? 2 : 1) * e->mp_ldp * sizeof(_Float16) +
                                     e->mp_ldd * sizeof(_Float16)
                               : P * sizeof(double) * (1 + (e->tomo_tile_mode == 2 ? e->tom_parts : 0)) +
                                     (g32 ? P * sizeof(float) + N * sizeof(float) : 0) +
@@ -960,12 +986,12 @@ int ensure_work(emc_engine *e) {
     if (mp) {
         e->tom_partial.release();
         const size_t rows_c = round_up((size_t)e->C, emc::kGemmBN);
-        if (rows_c != e->mp_rows_c || !e->lw_hi.p || (e->precision == 2) != (e->lw_lo.p != nullptr)) {
+        if (rows_c != e->mp_rows_c || !e->lw_hi.p || mp_two(e) != (e->lw_lo.p != nullptr)) {
             e->lw_hi.release();
             e->lw_lo.release();
             e->p16.release();
             if ((rc = e->lw_hi.alloc(rows_c * e->mp_ldp))) return rc;
-            if (e->precision == 2 && (rc = e->lw_lo.alloc(rows_c * e->mp_ldp))) return rc;
+            if (mp_two(e) && (rc = e->lw_lo.alloc(rows_c * e->mp_ldp))) return rc;
             if ((rc = e->p16.alloc(rows_c * e->mp_ldd))) return rc;
             // padding rows and columns stay zero: the kernels only write valid entries
             HIP_TRY(hipMemsetAsync(e->lw_hi.p, 0, e->lw_hi.n * sizeof(_Float16), e->stream));
@@ -978,7 +1004,8 @@ int ensure_work(emc_engine *e) {
             if ((rc = e->ktom.alloc(e->mp_rows_t * e->mp_ldd))) return rc;
             HIP_TRY(hipMemsetAsync(e->klik.p, 0, e->klik.n * sizeof(_Float16), e->stream));
             HIP_TRY(hipMemsetAsync(e->ktom.p, 0, e->ktom.n * sizeof(_Float16), e->stream));
-            emc::launch_build_klik(e->stream, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p, (int)N, e->mp_ldp, e->klik.p);
+            emc::launch_build_klik(e->stream, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p, (int)N, e->mp_ldp, e->klik.p,
+                                   mp_bf16(e));
             emc::launch_build_ktom(e->stream, e->tom_off.p, e->tom_frame.p, e->tom_cnt.p, (int)P, e->mp_ldd, e->ktom.p);
             HIP_TRY(hipGetLastError());
             e->mp_built = true;
@@ -1054,6 +1081,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     }
     const bool g32 = e->precision == 3;
     if (g32 && (rc = e->Rsub32.alloc((size_t)std::max<long>(e->num_pairs, 1) * emc::kRTile))) return rc;
+    if (g32 && (rc = ensure_model32(e))) return rc;
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     size_t budget = (freeb > ((size_t)4 << 30) ? freeb - ((size_t)4 << 30) : 0) + e->logw.cap * sizeof(double) +
@@ -1125,7 +1153,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
         const int Cc = std::min(C, S - s0);
         if ((rc = timed(0, [&] {
                  if (g32)
-                     emc::launch_expand(s, e->rots_slot.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
+                     emc::launch_expand(s, e->rots_slot.p, e->model32.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
                                         P, s0, Cc, kPixPerGroup, e->logw32.p, e->upart.p, e->ubad.p, e->slot_rot.p);
                  else
                      emc::launch_expand(s, e->rots_slot.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
@@ -1304,6 +1332,7 @@ const EngineOption kOptions[] = {
     {"slot_pack", &emc_engine::slot_pack, 0, 1, false},
     {"sparse_rows", &emc_engine::sparse_rows, 0, 1 << 30, false},
     {"full_part_frames", &emc_engine::full_part_frames, 1, 1 << 30, true},
+    {"estep_only", &emc_engine::estep_only, 0, 1, false},
 };
 
 int set_option(emc_engine *e, const EngineOption &o, int value) {
@@ -1397,6 +1426,9 @@ void emc_destroy(emc_engine *e) {
     e->tom_frame.release();
     e->fcounts.release();
     e->rmax.release();
+    e->mass_off.release();
+    e->mass_idx.release();
+    e->mass_out.release();
     for (auto &ev : e->ev) (void)hipEventDestroy(ev);
     (void)hipStreamDestroy(e->stream);
     delete e;
@@ -1473,6 +1505,7 @@ static int upload_quats(emc_engine *e) {
 }
 
 int emc_set_modes(emc_engine *e, int num_modes) {
+    if (e) e->model32_valid = false;  // model1 changes below
     if (!e || num_modes < 1 || num_modes > EMC_MAX_MODES)
         return fail(EMC_E_ARG, "emc_set_modes: num_modes must be 1..%d", EMC_MAX_MODES);
     if (num_modes == e->num_modes) return 0;
@@ -1506,6 +1539,7 @@ int emc_set_quats(emc_engine *e, int num_rot, const double *quats) {
 }
 
 int emc_set_model(emc_engine *e, long size, const double *model1) {
+    if (e) e->model32_valid = false;  // model1 changes below
     if (!e || size < 3 || !model1) return fail(EMC_E_ARG, "emc_set_model: bad argument");
     HIP_TRY(hipSetDevice(e->device));
     const size_t vol = (size_t)size * size * size * e->num_modes;  // every mode's volume
@@ -1645,6 +1679,13 @@ int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const
     bits.assign((size_t)N * words, 0u);
     for (int d = 0; d < N; ++d) {
         if (rel_off[d + 1] < rel_off[d]) return fail(EMC_E_ARG, "emc_set_refine: rel_off not ascending");
+        if (rel_off[d + 1] - rel_off[d] == 1 && rel_quat[rel_off[d]] == -1) {
+            // the single entry -1: every coarse cell (the full search)
+            uint32_t *row = &bits[(size_t)d * words];
+            for (int w = 0; w < words; ++w) row[w] = ~0u;
+            if (num_coarse & 31) row[words - 1] = (1u << (num_coarse & 31)) - 1u;
+            continue;
+        }
         for (long k = rel_off[d]; k < rel_off[d + 1]; ++k) {
             const int c = rel_quat[k];
             if (c < 0 || c >= num_coarse) return fail(EMC_E_ARG, "emc_set_refine: rel_quat %d out of range", c);
@@ -1694,7 +1735,7 @@ int emc_cell_posterior(emc_engine *e, int num_coarse, const int *quat_mapping, f
 
 int emc_set_precision(emc_engine *e, int mode) {
     if (!e) return fail(EMC_E_ARG, "null engine");
-    if (mode < 0 || mode > 3) return fail(EMC_E_ARG, "emc_set_precision: mode %d (0 to 3)", mode);
+    if (mode < 0 || mode > 5) return fail(EMC_E_ARG, "emc_set_precision: mode %d (0 to 5)", mode);
     if (mode != e->precision) {
         e->precision = mode;
         e->mp_built = false;
@@ -1899,13 +1940,15 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     if (!opts) return fail(EMC_E_ARG, "emc_maximize: null options");
     HIP_TRY(hipSetDevice(e->device));
     if (e->lists_dirty && (rc = build_lists(e))) return rc;
-    const bool mp = e->precision == 1 || e->precision == 2;
+    const bool mp = mp_mode(e);
     const bool g32 = e->precision == 3;
     // background-aware updates (max_prob.c:94, max_update.c:11, max_scale.c:9)
     const bool bgp = e->with_bg && opts->need_scaling;
     if (bgp && (e->precision != 0 || e->num_modes > 1 || e->refine))
         return fail(EMC_E_STATE, "emc_maximize: the background-aware path runs in FP64 with one mode, no refinement");
     if (g32) e->tomo_tile_mode = 2;  // the FP32 posterior feeds the frame-partitioned tomogram
+    if (e->estep_only && (bgp || (e->refine && e->subset_mode && e->subset_ready && !mp)))
+        return fail(EMC_E_STATE, "emc_maximize: estep_only runs on the plain and mixed paths only");
     if (e->refine && e->subset_mode && e->subset_ready && !mp) return maximize_subset(e, opts, stats);
     e->subset_ran = false;
     if (mp && e->num_modes > 1)
@@ -1915,8 +1958,11 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
             if (!v) return fail(EMC_E_STATE, "emc_maximize: the mixed-precision path needs dense frames");
         if (e->max_count > 2048)
             return fail(EMC_E_STATE, "emc_maximize: photon count %d is not exact in FP16 (limit 2048)", e->max_count);
+        if (mp_bf16(e) && e->max_count > 256)
+            return fail(EMC_E_STATE, "emc_maximize: photon count %d is not exact in BF16 (limit 256)", e->max_count);
     }
     if ((rc = ensure_work(e))) return rc;
+    if (g32 && (rc = ensure_model32(e))) return rc;
 
     hipStream_t s = e->stream;
     const int N = e->num_data, Rn = e->num_rot, P = e->num_pix, C = e->C, M = e->num_modes;
@@ -1952,10 +1998,10 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         HIP_TRY(hipEventRecord(e->ev[0], s));
         if (mp)
             emc::launch_expand_mp(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
-                                  e->center_t.p, P, r0, Cc, kPixPerGroup, e->mp_ldp, e->precision == 2,
-                                  e->lw_hi.p, e->lw_lo.p, e->upart.p, e->ubad.p);
+                                  e->center_t.p, P, r0, Cc, kPixPerGroup, e->mp_ldp, mp_two(e),
+                                  e->lw_hi.p, e->lw_lo.p, e->upart.p, e->ubad.p, mp_bf16(e));
         else if (g32)
-            emc::launch_expand(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
+            emc::launch_expand(s, e->rots.p, e->model32.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
                                P, r0, Cc, kPixPerGroup, e->logw32.p, e->upart.p, e->ubad.p, nullptr, M);
         else
             emc::launch_expand(s, e->rots.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
@@ -1964,11 +2010,11 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         HIP_TRY(hipEventRecord(e->ev[1], s));
         if (mp) {
             const size_t ts = (size_t)N * emc::kRTile;
-            if (emc::launch_mp_gemm(s, e->precision == 2 ? 2 : 1, 0, e->klik.p, e->mp_ldp, e->lw_hi.p,
+            if (emc::launch_mp_gemm(s, mp_two(e) ? 2 : 1, 0, e->klik.p, e->mp_ldp, e->lw_hi.p,
                                     e->lw_lo.p, e->mp_ldp, (int)e->mp_ldp, (int)e->mp_rows_d,
                                     (int)round_up(Cc, emc::kGemmBN), N, Cc,
                                     e->R.p + (size_t)(r0 / emc::kRTile) * ts, ts, e->center_base.p,
-                                    nullptr, 0, 1.))
+                                    nullptr, 0, 1., mp_bf16(e)))
                 return fail(EMC_E_STATE, "likelihood product: shape does not fit the tiles");
         } else if (g32) {
             emc::launch_likelihood(s, e->logw32.p, Cc, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
@@ -2028,6 +2074,27 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     HIP_TRY(hipEventElapsedTime(&ms_max, e->ev[4], e->ev[5]));
     HIP_TRY(hipEventElapsedTime(&ms_norm, e->ev[5], e->ev[6]));
     HIP_TRY(hipEventElapsedTime(&ms_post, e->ev[6], e->ev[7]));
+
+    if (e->estep_only) {
+        // expectation step only: per-frame statistics of the posterior, model2 and
+        // inter_weight left zero, scale factors unchanged
+        emc::launch_frame_stats(s, e->likelihood.p, e->info.p, e->blacklist.p, N, N - e->num_blacklist,
+                                e->scalars.p);
+        HIP_TRY(hipGetLastError());
+        double host_scalars[3];
+        HIP_TRY(hipMemcpyAsync(host_scalars, e->scalars.p, sizeof host_scalars, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        e->have_results = true;
+        if (stats) {
+            stats->rescale = host_scalars[0];
+            stats->avg_likelihood = host_scalars[1];
+            stats->mutual_info = host_scalars[2];
+            const float ms[EMC_NUM_STAGES] = {ms_expand, ms_lik, ms_rescale, ms_max, ms_norm, ms_post, 0.f, 0.f, 0.f};
+            for (int i = 0; i < EMC_NUM_STAGES; ++i) stats->stage_ms[i] = ms[i];
+            stats->num_chunks = (e->Rp + C - 1) / C;
+        }
+        return 0;
+    }
 
     // inter_weight does not depend on the data: when every orientation merges (all
     // psum_r > 0) it is the all-orientation sum, computed once and reused
@@ -2170,6 +2237,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
 }
 
 int emc_update_model(emc_engine *e, double alpha, double *rms_change) {
+    if (e) e->model32_valid = false;  // model1 changes below
     if (!e || !e->size) return fail(EMC_E_STATE, "emc_update_model: model not set");
     HIP_TRY(hipSetDevice(e->device));
     int rc;
@@ -2187,6 +2255,7 @@ int emc_update_model(emc_engine *e, double alpha, double *rms_change) {
 }
 
 int emc_normalize_scale(emc_engine *e, double *mean_scale) {
+    if (e) e->model32_valid = false;  // model1 changes below
     if (!e || !e->have_params || !e->size) return fail(EMC_E_STATE, "emc_normalize_scale: state not set");
     HIP_TRY(hipSetDevice(e->device));
     int rc;
@@ -2349,6 +2418,32 @@ int emc_get_sparse_prob(emc_engine *e, int *num_prob, long capacity, int *place,
         done += n;
         d0 = d1;
     }
+    return 0;
+}
+
+int emc_subset_mass(emc_engine *e, const long *rel_off, const int *rel_quat, double *mass) {
+    if (!e || !rel_off || !mass) return fail(EMC_E_ARG, "emc_subset_mass: bad argument");
+    if (!e->have_results || e->subset_ran)
+        return fail(EMC_E_STATE, "emc_subset_mass: needs the dense posterior of a plain iteration");
+    const int N = e->num_data;
+    const long total = rel_off[N];
+    if (rel_off[0] != 0 || total < 0 || (total > 0 && !rel_quat))
+        return fail(EMC_E_ARG, "emc_subset_mass: rel_off must start at 0 and ascend");
+    for (int d = 0; d < N; ++d)
+        if (rel_off[d + 1] < rel_off[d]) return fail(EMC_E_ARG, "emc_subset_mass: rel_off descends at frame %d", d);
+    for (long k = 0; k < total; ++k)
+        if (rel_quat[k] < 0 || rel_quat[k] >= e->num_rot)
+            return fail(EMC_E_ARG, "emc_subset_mass: orientation %d at entry %ld outside [0, %d)", rel_quat[k], k,
+                        e->num_rot);
+    HIP_TRY(hipSetDevice(e->device));
+    int rc;
+    if ((rc = e->mass_off.upload(rel_off, (size_t)N + 1, e->stream))) return rc;
+    if (total > 0 && (rc = e->mass_idx.upload(rel_quat, (size_t)total, e->stream))) return rc;
+    if ((rc = e->mass_out.alloc((size_t)N))) return rc;
+    emc::launch_subset_mass(e->stream, e->R.p, N, e->mass_off.p, total > 0 ? e->mass_idx.p : nullptr, e->mass_out.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(mass, e->mass_out.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
     return 0;
 }
 

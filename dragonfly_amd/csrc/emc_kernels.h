@@ -12,10 +12,13 @@ void launch_expand(hipStream_t s, const double *rots, const double *model, long 
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad,
                    const int *slot_rot = nullptr, int modes = 1, int raw = 0);
-void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
+// FP32 log W from an FP32 copy of the model (FP32-gather path)
+void launch_expand(hipStream_t s, const double *rots, const float *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad,
                    const int *slot_rot = nullptr, int modes = 1);
+// out[i] = (float)in[i], i < n
+void launch_copy_f32(hipStream_t s, const double *in, size_t n, float *out);
 void launch_to_f32(hipStream_t s, const double *P, int num_data, int r0, int C, float *out);
 void launch_likelihood(hipStream_t s, const float *logw, int C, int num_pix, const long *off,
                        const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,
@@ -50,6 +53,9 @@ void launch_frame_max(hipStream_t s, double *R, int num_rot, int num_data,
 void launch_frame_norm(hipStream_t s, const double *R, int num_rot, int num_data,
                        const double *beta, const uint8_t *blacklist, const double *max_exp,
                        const int *rmax, double *p_norm);
+// mass[d] = sum of the posterior P_dr (tile-major R) over r in idx[off[d] .. off[d + 1])
+void launch_subset_mass(hipStream_t s, const double *R, int num_data, const long *off, const int *idx,
+                        double *mass);
 void launch_posterior(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *logwr, const double *u, const double *beta,
                       const double *sum_fact, const uint8_t *blacklist, const double *max_exp,
@@ -130,12 +136,13 @@ void launch_normalize_scale(hipStream_t s, double *scale, const uint8_t *blackli
 
 // ---- mixed-precision path (emc_mp.hip) ----
 size_t gemm_lds_bytes(int nb);
+// bf16: the operands hold BF16 values (the buffers are 16-bit either way), else FP16
 int launch_mp_gemm(hipStream_t s, int nb, int epi, const _Float16 *A, size_t lda, const _Float16 *B0,
                    const _Float16 *B1, size_t ldb, int K, int Mp, int Np, int M_valid, int N_valid,
                    double *out, size_t ts, const double *row_add, const double *row_div,
-                   int div_valid, double scale);
+                   int div_valid, double scale, int bf16 = 0);
 void launch_build_klik(hipStream_t s, const long *off, const int *pix, const double *cnt, int num_data,
-                       size_t ldp, _Float16 *klik);
+                       size_t ldp, _Float16 *klik, int bf16 = 0);
 void launch_build_ktom(hipStream_t s, const long *toff, const int *tframe, const double *tcnt,
                        int num_pix, size_t ldd, _Float16 *ktom);
 void launch_center(hipStream_t s, const double *rots, int num_rot, const double *model, long size,
@@ -147,7 +154,7 @@ void launch_expand_mp(hipStream_t s, const double *rots, const double *model, lo
                       const double *qvals, const double *corr, const uint8_t *mask,
                       const double *center_t, int num_pix, int r0, int C, int pix_per_group,
                       size_t ldp, int two_terms, _Float16 *lw_hi, _Float16 *lw_lo, double *upart,
-                      int *ubad);
+                      int *ubad, int bf16 = 0);
 void launch_posterior_fp16(hipStream_t s, const double *P, int num_data, int r0, int C, size_t ldd,
                            double pscale, _Float16 *p16);
 
@@ -201,6 +208,12 @@ void launch_pair_scatter(hipStream_t s, const double *P, const int *pair_frame, 
 void launch_full_scatter(hipStream_t s, const double *RF, int nF, int num_tiles, const int *fidx,
                          const int *slot_rot, int num_rot, double *out, int f0 = 0, int f1 = -1, int d0 = 0);
 void launch_slot_gather(hipStream_t s, const double *in, const int *rot_slot, int num_rot, double *out);
+// Full-frame photon lists per engine pixel in nparts ranges of full-frame index (nparts <= 64):
+// pass 0 writes counts[t * nparts + x]; pass 1 fills out_frame / out_cnt from off2 (the
+// exclusive scan of the counts). fof[d] is frame d's full index or -1.
+void launch_full_lists(hipStream_t s, int pass, const long *tom_off, const int *tom_frame, const double *tom_cnt,
+                       const int *fof, int num_pix, int nF, int nparts, int *counts, const long *off2,
+                       int *out_frame, double *out_cnt);
 void launch_cell_posterior(hipStream_t s, double *R, int plain, int num_data, int num_rot, const long *pair_off,
                            const int *pair_tile, const FullRows &full, const int *slot_rot, const int *qmap,
                            int num_coarse, int d0, int count, float *out);

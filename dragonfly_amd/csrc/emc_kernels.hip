@@ -45,9 +45,26 @@ __device__ __forceinline__ void merge_value(double *__restrict__ model2, double 
 // Block = 64 orientations x one pixel group. Compute with lanes along pixels (neighbouring
 // pixels hit neighbouring voxels), transpose through LDS, store with lanes along
 // orientations (coalesced rows of logW). u_r (max_prob.c:36-38) sums W over mask < 1.
-template <typename T, int kW>
+__device__ __forceinline__ double slice_value_model(const double *__restrict__ model, long size, const double *rot,
+                                                    double q0, double q1, double q2, double corr, double center) {
+    return slice_value_flat(model, size, rot, q0, q1, q2, corr, center);
+}
+__device__ __forceinline__ double slice_value_model(const float *__restrict__ model, long size, const double *rot,
+                                                    double q0, double q1, double q2, double corr, double center) {
+    return slice_value_f32m(model, size, rot, q0, q1, q2, corr, center);
+}
+// log W as stored: FP64 log for an FP64 model; FP32 log of the FP32 interpolation otherwise
+// (values below FLT_MIN, i.e. DBL_MIN outside the grid, keep the FP64 log)
+template <typename T, typename MT>
+__device__ __forceinline__ T log_stored(double W) {
+    if (sizeof(MT) == sizeof(float)) return (T)(W >= (double)FLT_MIN ? logf((float)W) : (float)log(W));
+    return (T)log(W);
+}
+
+// MT: element type of the model read (double, or float for the FP32-gather path's copy)
+template <typename T, int kW, typename MT = double>
 __global__ __launch_bounds__(kBlock) void expand_kernel(
-    const double *__restrict__ rots, const double *__restrict__ model, long size,
+    const double *__restrict__ rots, const MT *__restrict__ model, long size,
     const double *__restrict__ qvals, const double *__restrict__ corr,
     const uint8_t *__restrict__ mask, int num_pix, int r0, int C, int pix_per_group,
     T *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad,
@@ -79,9 +96,9 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
             for (int k = 0; k < kRTile / kWavesPerBlock; ++k) {
                 const int rr = wave + kWavesPerBlock * k;
                 // combined orientation index r = rotation * modes + mode (max_prob.c:80-84)
-                const double *mdl =
+                const MT *mdl =
                     modes > 1 ? model + (size_t)((r0 + c0 + rr) % modes) * (size_t)(size * size * size) : model;
-                const double v = slice_value_flat(mdl, size, &s_rot[rr * 9], q0, q1, q2, cr, center);
+                const double v = slice_value_model(mdl, size, &s_rot[rr * 9], q0, q1, q2, cr, center);
                 s_tile[lane][rr] = s_pad[rr] ? 1. : v;
             }
         }
@@ -98,7 +115,8 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
                 // tile-major with kW-wide tiles: (c / kW) * num_pix * kW + t * kW + c % kW
                 const int c = c0 + lane;
                 // raw: W itself (the background-aware likelihood takes log(phi W + b B))
-                logw[(size_t)(c / kW) * ((size_t)num_pix * kW) + (size_t)tt * kW + (c % kW)] = (T)(raw ? W : log(W));
+                logw[(size_t)(c / kW) * ((size_t)num_pix * kW) + (size_t)tt * kW + (c % kW)] =
+                    raw ? (T)W : log_stored<T, MT>(W);
             }
         }
         __syncthreads();
@@ -1086,13 +1104,24 @@ void launch_expand(hipStream_t s, const double *rots, const double *model, long 
                                                           pix_per_group, logw, upart, ubad, slot_rot, modes, raw);
 }
 
-void launch_expand(hipStream_t s, const double *rots, const double *model, long size,
+void launch_expand(hipStream_t s, const double *rots, const float *model, long size,
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad,
                    const int *slot_rot, int modes) {
     dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
-    expand_kernel<float, 2 * kRTile><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0,
-                                                             C, pix_per_group, logw, upart, ubad, slot_rot, modes, 0);
+    expand_kernel<float, 2 * kRTile, float><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix,
+                                                                    r0, C, pix_per_group, logw, upart, ubad,
+                                                                    slot_rot, modes, 0);
+}
+
+__global__ void copy_f32_kernel(const double *__restrict__ in, size_t n, float *__restrict__ out) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+
+void launch_copy_f32(hipStream_t s, const double *in, size_t n, float *out) {
+    if (n == 0) return;
+    copy_f32_kernel<<<grid_cap(cdiv(n, 256)), 256, 0, s>>>(in, n, out);
 }
 
 // FP32 copy of one orientation chunk of the tile-major posterior in 128-wide tiles:
@@ -1308,6 +1337,24 @@ void launch_frame_norm(hipStream_t s, const double *R, int num_rot, int num_data
                        const int *rmax, double *p_norm) {
     frame_norm_kernel<<<num_data, kBlock, 0, s>>>(R, (size_t)num_data * kRTile, num_rot, beta, blacklist, max_exp, rmax,
                                                   p_norm);
+}
+
+__global__ void subset_mass_kernel(const double *__restrict__ R, int num_data, const long *__restrict__ off,
+                                   const int *__restrict__ idx, double *__restrict__ mass) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= num_data) return;
+    double acc = 0.;
+    for (long k = off[d]; k < off[d + 1]; ++k) {
+        const int r = idx[k];
+        acc += R[(size_t)(r / kRTile) * ((size_t)num_data * kRTile) + (size_t)d * kRTile + r % kRTile];
+    }
+    mass[d] = acc;
+}
+
+void launch_subset_mass(hipStream_t s, const double *R, int num_data, const long *off, const int *idx,
+                        double *mass) {
+    if (num_data <= 0) return;
+    subset_mass_kernel<<<cdiv(num_data, 256), 256, 0, s>>>(R, num_data, off, idx, mass);
 }
 
 void launch_posterior(hipStream_t s, double *R, int num_rot, int num_data,

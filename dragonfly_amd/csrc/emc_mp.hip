@@ -45,18 +45,19 @@ namespace emc {
 //   EPI 1: out[m * ts + n] = acc * scale / row_div[m] if m < div_valid and row_div[m] > 0,
 //          else acc * scale   (row-major; tomogram of orientation m at pixel n; rows at or past
 //          div_valid are padding orientations)
-template <int NB, int EPI>
+template <typename T, int NB, int EPI>
 __global__ __launch_bounds__(kGemmThreads) void mp_gemm_kernel(
-    const _Float16 *__restrict__ A, size_t lda, const _Float16 *__restrict__ B0,
-    const _Float16 *__restrict__ B1, size_t ldb, int K, int m_tiles, int n_tiles, int M_valid,
+    const T *__restrict__ A, size_t lda, const T *__restrict__ B0,
+    const T *__restrict__ B1, size_t ldb, int K, int m_tiles, int n_tiles, int M_valid,
     int N_valid, double *__restrict__ out, size_t ts, const double *__restrict__ row_add,
     const double *__restrict__ row_div, int div_valid, double scale) {
-    extern __shared__ _Float16 smem[];
+    extern __shared__ char smem_raw[];
+    T *smem = reinterpret_cast<T *>(smem_raw);
     const GemmTile tile = gemm_block_tile(m_tiles, n_tiles);
     if (!tile.valid) return;
     floatx16 acc[2][4];
     // one-term products double-buffer LDS (2 x 72 KiB); two-term ones do not fit twice
-    gemm_mainloop<_Float16, NB, kMpGemmBK, NB == 1>(A, lda, B0, B1, ldb, K, tile.m0, tile.n0, smem, acc);
+    gemm_mainloop<T, NB, kMpGemmBK, NB == 1>(A, lda, B0, B1, ldb, K, tile.m0, tile.n0, smem, acc);
     gemm_for_each(tile.m0, tile.n0, acc, [&](size_t m, size_t n, float a) {
         if (m >= (size_t)M_valid || n >= (size_t)N_valid) return;
         const double x = (double)a;
@@ -71,16 +72,30 @@ __global__ __launch_bounds__(kGemmThreads) void mp_gemm_kernel(
 
 // ---- operand builders ---------------------------------------------------------------------
 
-// klik[d][pix] = count for the frame's mask < 1 photons (likelihood photon list, engine order).
+// Round a double to the 16-bit operand type: FP16 directly, BF16 through FP32.
+template <typename T>
+__device__ __forceinline__ T to_op(double x);
+template <>
+__device__ __forceinline__ _Float16 to_op<_Float16>(double x) { return (_Float16)x; }
+template <>
+__device__ __forceinline__ __bf16 to_op<__bf16>(double x) { return (__bf16)(float)x; }
+template <typename T>
+__device__ __forceinline__ double from_op(T x) { return (double)(float)x; }
+template <>
+__device__ __forceinline__ double from_op<_Float16>(_Float16 x) { return (double)x; }
+
+// klik[d][pix] = count for the frame's mask < 1 photons (likelihood photon list, engine order),
+// as T (integers are exact to 2048 in FP16, to 256 in BF16).
+template <typename T>
 __global__ __launch_bounds__(kBlock) void build_klik_kernel(const long *__restrict__ off,
                                                             const int *__restrict__ pix,
                                                             const double *__restrict__ cnt,
                                                             int num_data, size_t ldp,
-                                                            _Float16 *__restrict__ klik) {
+                                                            T *__restrict__ klik) {
     const int d = blockIdx.x;
     if (d >= num_data) return;
     for (long k = off[d] + threadIdx.x; k < off[d + 1]; k += blockDim.x)
-        klik[(size_t)d * ldp + pix[k]] = (_Float16)cnt[k];
+        klik[(size_t)d * ldp + pix[k]] = to_op<T>(cnt[k]);
 }
 
 // ktom[t][frame] = count for the pixel's mask < 2 photons (tomogram photon list).
@@ -138,14 +153,16 @@ __global__ __launch_bounds__(kBlock) void center_base_kernel(const long *__restr
 }
 
 // Expand for the mixed path: W as slice_gen3d, u_r and the non-finite flag as expand_kernel,
-// and the centred log W split into FP16 hi / lo rows lw[c][t] (lanes along pixels, so the
-// stores are contiguous without a transpose). mask >= 1 and non-finite entries store 0.
+// and the centred log W split into 16-bit (T: FP16 or BF16) hi / lo rows lw[c][t] (lanes
+// along pixels, so the stores are contiguous without a transpose). mask >= 1 and non-finite
+// entries store 0.
+template <typename T>
 __global__ __launch_bounds__(kBlock) void expand_mp_kernel(
     const double *__restrict__ rots, const double *__restrict__ model, long size,
     const double *__restrict__ qvals, const double *__restrict__ corr,
     const uint8_t *__restrict__ mask, const double *__restrict__ center_t, int num_pix, int r0,
-    int C, int pix_per_group, size_t ldp, int two_terms, _Float16 *__restrict__ lw_hi,
-    _Float16 *__restrict__ lw_lo, double *__restrict__ upart, int *__restrict__ ubad) {
+    int C, int pix_per_group, size_t ldp, int two_terms, T *__restrict__ lw_hi,
+    T *__restrict__ lw_lo, double *__restrict__ upart, int *__restrict__ ubad) {
     __shared__ double s_rot[kRTile * 9];
     __shared__ double s_tile[64][kRTile + 1];
     __shared__ double s_u[kWavesPerBlock][kRTile];
@@ -172,10 +189,10 @@ __global__ __launch_bounds__(kBlock) void expand_mp_kernel(
                 const double W = slice_value(model, size, &s_rot[rr * 9], q0, q1, q2, cr, center);
                 s_tile[lane][rr] = W;
                 double x = (good && W > 0.) ? log(W) - ct : 0.;
-                const _Float16 hi = (_Float16)x;
+                const T hi = to_op<T>(x);
                 const size_t at = (size_t)(c0 + rr) * ldp + t;
                 lw_hi[at] = hi;
-                if (two_terms) lw_lo[at] = (_Float16)(x - (double)hi);
+                if (two_terms) lw_lo[at] = to_op<T>(x - from_op<T>(hi));
             }
         }
         __syncthreads();
@@ -230,27 +247,34 @@ size_t gemm_lds_bytes(int nb) {
 int launch_mp_gemm(hipStream_t s, int nb, int epi, const _Float16 *A, size_t lda, const _Float16 *B0,
                    const _Float16 *B1, size_t ldb, int K, int Mp, int Np, int M_valid, int N_valid,
                    double *out, size_t ts, const double *row_add, const double *row_div,
-                   int div_valid, double scale) {
+                   int div_valid, double scale, int bf16) {
     if (Mp % kGemmBM || Np % kGemmBN || K % kMpGemmBK || lda % 8 || ldb % 8 || (size_t)K > lda ||
         (size_t)K > ldb || M_valid > Mp || N_valid > Np)
         return -1;
     const int m_tiles = Mp / kGemmBM, n_tiles = Np / kGemmBN;
     const dim3 grid(gemm_grid(m_tiles, n_tiles));
     const size_t lds = gemm_lds_bytes(nb);
-#define EMC_GEMM(NB, EPI)                                                                         \
-    mp_gemm_kernel<NB, EPI><<<grid, kGemmThreads, lds, s>>>(A, lda, B0, B1, ldb, K, m_tiles, n_tiles, \
-                                                            M_valid, N_valid, out, ts, row_add, row_div, div_valid, scale)
-    if (nb == 2 && epi == 0) EMC_GEMM(2, 0);
-    else if (nb == 1 && epi == 0) EMC_GEMM(1, 0);
-    else if (nb == 1 && epi == 1) EMC_GEMM(1, 1);
+#define EMC_GEMM(T, NB, EPI)                                                                      \
+    mp_gemm_kernel<T, NB, EPI><<<grid, kGemmThreads, lds, s>>>(                                   \
+        (const T *)A, lda, (const T *)B0, (const T *)B1, ldb, K, m_tiles, n_tiles, M_valid, N_valid, out, ts, \
+        row_add, row_div, div_valid, scale)
+    if (bf16 && nb == 2 && epi == 0) EMC_GEMM(__bf16, 2, 0);
+    else if (bf16 && nb == 1 && epi == 0) EMC_GEMM(__bf16, 1, 0);
+    else if (bf16) return -1;
+    else if (nb == 2 && epi == 0) EMC_GEMM(_Float16, 2, 0);
+    else if (nb == 1 && epi == 0) EMC_GEMM(_Float16, 1, 0);
+    else if (nb == 1 && epi == 1) EMC_GEMM(_Float16, 1, 1);
     else return -1;
 #undef EMC_GEMM
     return 0;
 }
 
 void launch_build_klik(hipStream_t s, const long *off, const int *pix, const double *cnt, int num_data,
-                       size_t ldp, _Float16 *klik) {
-    build_klik_kernel<<<num_data, kBlock, 0, s>>>(off, pix, cnt, num_data, ldp, klik);
+                       size_t ldp, _Float16 *klik, int bf16) {
+    if (bf16)
+        build_klik_kernel<__bf16><<<num_data, kBlock, 0, s>>>(off, pix, cnt, num_data, ldp, (__bf16 *)klik);
+    else
+        build_klik_kernel<_Float16><<<num_data, kBlock, 0, s>>>(off, pix, cnt, num_data, ldp, klik);
 }
 
 void launch_build_ktom(hipStream_t s, const long *toff, const int *tframe, const double *tcnt,
@@ -276,11 +300,16 @@ void launch_expand_mp(hipStream_t s, const double *rots, const double *model, lo
                       const double *qvals, const double *corr, const uint8_t *mask,
                       const double *center_t, int num_pix, int r0, int C, int pix_per_group,
                       size_t ldp, int two_terms, _Float16 *lw_hi, _Float16 *lw_lo, double *upart,
-                      int *ubad) {
+                      int *ubad, int bf16) {
     dim3 grid(C / kRTile, (num_pix + pix_per_group - 1) / pix_per_group);
-    expand_mp_kernel<<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, center_t, num_pix,
-                                             r0, C, pix_per_group, ldp, two_terms, lw_hi, lw_lo,
-                                             upart, ubad);
+    if (bf16)
+        expand_mp_kernel<__bf16><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, center_t,
+                                                         num_pix, r0, C, pix_per_group, ldp, two_terms,
+                                                         (__bf16 *)lw_hi, (__bf16 *)lw_lo, upart, ubad);
+    else
+        expand_mp_kernel<_Float16><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, center_t,
+                                                           num_pix, r0, C, pix_per_group, ldp, two_terms, lw_hi,
+                                                           lw_lo, upart, ubad);
 }
 
 void launch_posterior_fp16(hipStream_t s, const double *P, int num_data, int r0, int C, size_t ldd,

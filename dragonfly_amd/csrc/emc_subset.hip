@@ -463,7 +463,60 @@ __global__ __launch_bounds__(kBlock) void cell_posterior_kernel(
     for (int c = threadIdx.x; c < num_coarse; c += kBlock) out[(size_t)blockIdx.x * num_coarse + c] = (float)s_cell[c];
 }
 
+// ---- full-frame photon lists ----------------------------------------------------------------
+// Per engine pixel, the photons (mask < 2) of the full frames in frame order, split into
+// nparts ranges of full-frame index f (range x holds f with nF * x / nparts <= f). One wave
+// per pixel. Pass 0 counts the photons of every (pixel, range); pass 1 writes them from the
+// pixel's first output slot off2[t * nparts], keeping the pixel's frame order.
+__device__ __forceinline__ int full_part_of(int f, int nF, int nparts) {
+    int x = (int)(((long)f * nparts) / nF);  // then the largest x with nF * x / nparts <= f
+    while (x + 1 < nparts && (long)nF * (x + 1) / nparts <= f) ++x;
+    while (x > 0 && (long)nF * x / nparts > f) --x;
+    return x;
+}
+
+__global__ __launch_bounds__(kBlock) void full_lists_kernel(
+    int pass, const long *__restrict__ tom_off, const int *__restrict__ tom_frame,
+    const double *__restrict__ tom_cnt, const int *__restrict__ fof, int num_pix, int nF, int nparts,
+    int *__restrict__ counts, const long *__restrict__ off2, int *__restrict__ out_frame,
+    double *__restrict__ out_cnt) {
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (t >= num_pix) return;
+    const long k0 = tom_off[t], k1 = tom_off[t + 1];
+    long at = pass ? off2[(size_t)t * nparts] : 0;
+    int cnt_lane = 0;  // pass 0: lane x < nparts counts range x
+    for (long kb = k0; kb < k1; kb += 64) {
+        const long k = kb + lane;
+        const int f = k < k1 ? fof[tom_frame[k]] : -1;
+        const unsigned long long keep = __ballot(f >= 0);
+        if (pass == 0) {
+            const int x = f >= 0 ? full_part_of(f, nF, nparts) : -1;
+            for (int y = 0; y < nparts; ++y) {
+                const int n = __popcll(__ballot(x == y));
+                if (lane == y) cnt_lane += n;
+            }
+        } else {
+            if (f >= 0) {
+                const long pos = at + __popcll(keep & ((1ull << lane) - 1ull));
+                out_frame[pos] = f;
+                out_cnt[pos] = tom_cnt[k];
+            }
+            at += __popcll(keep);
+        }
+    }
+    if (pass == 0 && lane < nparts) counts[(size_t)t * nparts + lane] = cnt_lane;
+}
+
 // ---- launchers -----------------------------------------------------------------------------
+
+void launch_full_lists(hipStream_t s, int pass, const long *tom_off, const int *tom_frame, const double *tom_cnt,
+                       const int *fof, int num_pix, int nF, int nparts, int *counts, const long *off2,
+                       int *out_frame, double *out_cnt) {
+    if (num_pix <= 0) return;
+    full_lists_kernel<<<(num_pix + kWavesPerBlock - 1) / kWavesPerBlock, kBlock, 0, s>>>(
+        pass, tom_off, tom_frame, tom_cnt, fof, num_pix, nF, nparts, counts, off2, out_frame, out_cnt);
+}
 
 void launch_cell_posterior(hipStream_t s, double *R, int plain, int num_data, int num_rot, const long *pair_off,
                            const int *pair_tile, const FullRows &full, const int *slot_rot, const int *qmap,

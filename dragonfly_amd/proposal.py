@@ -230,6 +230,78 @@ class OrientationProposal:
         return idx[:num].cpu().numpy()
 
 
+def topk_device(Z, rows, n, k):
+    """Top-k column indices (descending, ties to the lower index) of the first `rows` rows of
+    an FP32 device matrix Z (row stride Z.shape[1]), with the softmax mass they hold, on the
+    GPU (emc_nn_topk). Returns host arrays ((rows, k) int32, (rows,) float32)."""
+    import torch
+    idx = torch.empty((max(rows, 1), k), dtype=torch.int32, device=Z.device)
+    mass = torch.empty(max(rows, 1), dtype=torch.float32, device=Z.device)
+    _ok(_lib.lib().emc_nn_topk(_stream(), _ptr(Z), Z.shape[1], rows, n, k, _ptr(idx), _ptr(mass)), 'top-k')
+    return idx[:rows].cpu().numpy(), mass[:rows].cpu().numpy()
+
+
+def union_cells(*tops):
+    """Per-frame union of several (num_frames, k_i) cell arrays, as CSR (off int64
+    (num_frames + 1), cells int32 ascending within a frame)."""
+    allc = np.sort(np.concatenate([np.asarray(t, 'i4') for t in tops], axis=1), axis=1)
+    keep = np.ones(allc.shape, bool)
+    keep[:, 1:] = allc[:, 1:] != allc[:, :-1]
+    off = np.zeros(allc.shape[0] + 1, 'i8')
+    np.cumsum(keep.sum(1), out=off[1:])
+    return off, allc[keep].astype('i4')
+
+
+class CoarseJudge:
+    """Judges orientation proposals by the likelihood: a full search over the coarse
+    orientations quat_gen(coarse_div) (every frame, every coarse orientation -- the coarse
+    orientation c is the centre of Voronoi cell c of the fine sampling) stopped after the
+    posterior (estep_only), at the same model, scale factors and beta as the fine iteration it
+    judges. A frame's proposal (a set of cells) is accepted when the coarse posterior puts at
+    least `threshold` of its mass inside it; otherwise the frame falls back to the full fine
+    search. The coarse posterior samples each cell at its centre, so a sharp fine peak between
+    centres lowers the coarse maximum and raises the mass outside any proposal: the judge errs
+    towards falling back."""
+
+    def __init__(self, itr, coarse_div, device=0, precision=0):
+        det = itr.dets[0]
+        self.eng = eng = _lib.Engine(device)
+        eng.set_precision(precision)
+        eng.set_detector(det.qvals, det.corr, det.raw_mask)
+        self.quats = _lib.quat_gen(coarse_div)
+        self.num_coarse = len(self.quats)
+        eng.set_quats(self.quats)
+        eng.set_model(itr.model.model1)
+        sp = itr.sparse_arrays()
+        eng.set_frames(sp['ones'], sp['multi'], sp['ones_accum'], sp['multi_accum'], sp['place_ones'],
+                       sp['place_multi'], sp['count_multi'])
+        eng.set_frame_params(itr.scale, itr.beta, itr.blacklist, itr.fcounts, itr.sum_fact, itr.mean_count[0],
+                             0)
+        eng.set_option('estep_only', 1)
+        self.stats = None
+
+    def estep(self, model1, scale, beta, iteration, need_scaling):
+        """Coarse posterior of every frame at this model / scale factors / beta."""
+        eng = self.eng
+        eng.set_model(model1)
+        eng.set_scale(scale)
+        eng.set_beta(beta)
+        self.stats = eng.maximize(iteration, need_scaling, 0, 0)
+        return self.stats
+
+    def argmax_cells(self):
+        """Each frame's most likely coarse orientation (cell), (num_frames, 1) int32; a frame
+        with no orientation (blacklisted) gets cell 0."""
+        return np.maximum(self.eng.get('rmax'), 0).astype('i4')[:, None]
+
+    def mass(self, rel_off, rel_quat):
+        """Coarse posterior mass inside each frame's cells rel_quat[rel_off[d]:rel_off[d + 1]]."""
+        return self.eng.subset_mass(rel_off, rel_quat)
+
+    def close(self):
+        self.eng.close()
+
+
 def refine_lists(top, mass, num_coarse, threshold=0.99, buf=None):
     """refine_sets flattened for Iterate.set_refine_lists: (rel_off, rel_quat, fallback, buf).
     rel_quat is a view of `buf` (reused when large enough, so repeated calls do not allocate
@@ -251,6 +323,30 @@ def refine_lists(top, mass, num_coarse, threshold=0.99, buf=None):
     for s in rel_off[:-1][fallback]:
         rel_quat[s:s + num_coarse] = every
     return rel_off, rel_quat, fallback, buf
+
+
+def _ranges(starts, lens):
+    """Concatenation of arange(s, s + n) over (s, n) pairs, int64."""
+    lens = np.asarray(lens, 'i8')
+    tot = int(lens.sum())
+    if tot == 0:
+        return np.zeros(0, 'i8')
+    first = np.cumsum(lens) - lens
+    return np.repeat(np.asarray(starts, 'i8') - first, lens) + np.arange(tot, dtype='i8')
+
+
+def refine_lists_csr(off, cells, accept, num_coarse=None):
+    """Refinement lists from per-frame proposals in CSR form (off, cells): a frame keeps its
+    cells where `accept`; elsewhere its list is the single entry -1, which emc_set_refine
+    reads as every cell (the full search). Returns (rel_off, rel_quat)."""
+    off = np.asarray(off, 'i8')
+    accept = np.asarray(accept, bool)
+    n = np.diff(off)
+    rel_off = np.zeros(len(n) + 1, 'i8')
+    np.cumsum(np.where(accept, n, 1), out=rel_off[1:])
+    rel_quat = np.full(int(rel_off[-1]), -1, 'i4')
+    rel_quat[_ranges(rel_off[:-1][accept], n[accept])] = np.asarray(cells, 'i4')[_ranges(off[:-1][accept], n[accept])]
+    return rel_off, rel_quat
 
 
 def refine_sets(top, mass, num_coarse, threshold=0.99):

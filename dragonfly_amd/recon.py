@@ -285,7 +285,13 @@ class EMCRecon:
         return self.sparse_prob()[0]
 
     def free(self):
+        """Release the engine. The model's host arrays are refreshed first and then detached
+        from it, so they stay readable (and assignable) afterwards."""
         if self.engine is not None:
+            model = self.iter.model if self.iter is not None else None
+            if model is not None and model._engine is self.engine:
+                model.sync()
+                model._engine = None
             self.engine.close()
             self.engine = None
 

@@ -141,7 +141,9 @@ int emc_set_frames_dense(emc_engine *e, int num_data, int num_pix, const int *fr
  * :364-525; applied in src/max_emc.c:264-274). quat_mapping[r] is the coarse cell of
  * orientation r (num_rot entries, values < num_coarse); frame d is evaluated only at the
  * orientations whose cell is listed in rel_quat[rel_off[d] .. rel_off[d + 1]). A null
- * quat_mapping switches refinement off; setting quaternions or frames also does.        */
+ * quat_mapping switches refinement off; setting quaternions or frames also does. A frame
+ * whose list is the single entry -1 takes every coarse cell (the full search) without
+ * listing them.                                                                          */
 int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const long *rel_off,
                    const int *rel_quat);
 /* arithmetic of the two photon contractions. No counterpart in the reference (which
@@ -149,8 +151,10 @@ int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const
  * 1 = dense frames only: likelihood and tomogram as FP16 products on matrix cores with
  * FP32 accumulation, log W centred per pixel and rounded once to FP16; 2 = as 1 with
  * log W carried as an FP16 hi + lo pair; 3 = any frames: log W and the posterior read by
- * the photon gathers in FP32 (half the bytes), sums in FP64. Every other step stays FP64.
- * Modes 1 and 2 need frames set with emc_set_frames_dense and counts <= 2048.           */
+ * the photon gathers in FP32 (half the bytes), sums in FP64; 4, 5 = as 1, 2 with the
+ * likelihood product's log W and counts in BF16 (the tomogram product stays FP16). Every
+ * other step stays FP64. Modes 1, 2, 4 and 5 need frames set with emc_set_frames_dense and
+ * counts <= 2048 (FP16) or <= 256 (BF16).                                                */
 int emc_set_precision(emc_engine *e, int mode);
 /* kernel-variant options, no counterpart in the reference (results are the same to the
  * parity tolerance; only speed changes): lik_tiles (1-4 orientation tiles per likelihood
@@ -163,8 +167,9 @@ int emc_set_precision(emc_engine *e, int mode);
  * back to back along a nearest-neighbour chain, 0: each cell padded to whole 64-orientation
  * tiles; takes effect at the next emc_set_refine), sparse_rows (frames per batch of
  * emc_get_sparse_prob after a subset iteration, 0: 256 MB of rows), full_part_frames
- * (frames per range of the subset path's full-frame tomogram, default 12500). Start values
- * come from EMC_<NAME>.      */
+ * (frames per range of the subset path's full-frame tomogram, default 12500), estep_only (1:
+ * emc_maximize stops after the posterior and its statistics -- no tomogram, merge or scale
+ * update; plain and mixed paths). Start values come from EMC_<NAME>.      */
 int emc_set_option(emc_engine *e, const char *name, int value);
 int emc_get_option(emc_engine *e, const char *name, int *value);
 /* per-frame state of struct iterate: scale, beta, blacklist, fcounts, sum_fact,
@@ -217,6 +222,13 @@ int emc_get_field(emc_engine *e, int field, void *dst, size_t count);
  * it gives an orientation-proposal model its training targets (the cell posteriors an
  * iteration produced) without moving the dense num_data x num_rot posterior.            */
 int emc_cell_posterior(emc_engine *e, int num_coarse, const int *quat_mapping, float *out);
+/* Posterior mass of listed orientations, per frame, after a plain (not refinement-subset)
+ * iteration: mass[d] = sum of P_dr over r in rel_quat[rel_off[d] .. rel_off[d + 1]) (rel_off:
+ * num_data + 1 ascending offsets; orientations < num_rot, each listed once). No counterpart
+ * in the reference: a full search at a coarse sampling, run with the option estep_only,
+ * judges orientation proposals by the likelihood -- the mass its posterior puts inside each
+ * frame's proposed cells.                                                                */
+int emc_subset_mass(emc_engine *e, const long *rel_off, const int *rel_quat, double *mass);
 /* Zero model2 and inter_weight (max_data allocate_memory, maximize.c:93-94).            */
 int emc_clear_merge(emc_engine *e);
 

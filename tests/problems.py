@@ -132,3 +132,35 @@ def write_config(p, folder, **keys):
     with open(path, 'w') as fp:
         fp.write('[emc]\n' + ''.join('%s = %s\n' % kv for kv in emc.items()))
     return path
+
+
+def from_synth(prob, quats=None, start_seed=4, beta='auto'):
+    """A problem dict (as build() returns) for a configuration made by synth.make_problem:
+    the configuration's detector, frames and orientations, a seeded start model and the
+    per-frame precompute; beta 'auto' is calc_beta's per-frame rule with scale factors
+    (iterate.c:66-83) or one value for every frame."""
+    import math
+    from dragonfly_amd import _lib
+    cfg, det, f = prob['cfg'], prob['det'], prob['frames']
+    mask = det['mask']
+    fc = frame_counts(f['ones'], f['multi'], f['place_ones'], f['place_multi'], f['count_multi'], mask)
+    sf = sum_log_factorials(f['multi'], f['place_multi'], f['count_multi'], mask)
+    D = len(f['ones'])
+    mean_count = fc.astype('f8').sum() / D
+    if beta == 'auto':
+        beta_v = np.array([math.exp(-6.5 * math.pow(c * 1e-5, 0.15)) for c in fc.tolist()])
+    else:
+        beta_v = np.full(D, float(beta))
+    fparams = dict(scale=np.ones(D), beta=beta_v, blacklist=np.zeros(D, 'u1'), fcounts=fc, sum_fact=sf,
+                   mean_count=mean_count, num_blacklist=0)
+    return dict(det=det, frames={k: f[k] for k in SPARSE_KEYS},
+                quats=_lib.quat_gen(cfg['num_div']) if quats is None else quats,
+                model1=synth.start_model(cfg['size'], mean_count, mask, seed=start_seed), fparams=fparams,
+                intens=prob['intens'], true_scale=f['true_scale'])
+
+
+def oracle_threads():
+    """Host threads for oracle runs that must stay fast (full orientation sets): the result
+    equals the one-thread rule whenever every frame keeps every orientation it evaluates."""
+    import os
+    return max(1, min(16, os.cpu_count() or 1))

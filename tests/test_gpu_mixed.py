@@ -1,6 +1,7 @@
 """Mixed-precision dense-frame path: the likelihood and tomogram contractions as FP16
-products on matrix cores (emc_set_precision modes 1 and 2) against the FP64 oracle and the
-plain FP64 engine path on the same dense integer frames.
+products on matrix cores (emc_set_precision modes 1 and 2; 4 and 5 with the likelihood
+product in BF16) against the FP64 oracle and the plain FP64 engine path on the same dense
+integer frames.
 
 Bar: the merged model and the updated model within 1e-3 relative error (max-norm) of FP64
 in the two-term mode; the average log-likelihood and mutual information within 1e-6 (two
@@ -39,8 +40,10 @@ def bright():
     return P.densify(P.build(P.tiny_config(num_data=300, mean_count=400.)))
 
 
-@pytest.mark.parametrize('mode', [2, 1])
+@pytest.mark.parametrize('mode', [2, 1, 5, 4])
 def test_tiny_dense_vs_oracle(bright, mode):
+    """Modes 1, 2: FP16 log W (one term, hi + lo); 4, 5: the same in BF16 (8 mantissa bits
+    per term, so one BF16 term moves log-likelihoods by ~1e-3 of a unit per photon)."""
     flags = dict(iteration=1, need_scaling=0)
     o = P.run_oracle(bright, flags)
     eng, g = _run(bright, mode, flags)
@@ -53,9 +56,15 @@ def test_tiny_dense_vs_oracle(bright, mode):
         assert e['model2'] <= 1e-3
         assert e['avg_likelihood'] <= 1e-6 and e['mutual_info'] <= 1e-6
         assert e['rmax_agree'] == 1.
-    else:
+    elif mode == 5:
+        assert e['model2'] <= 1e-2
+        assert e['avg_likelihood'] <= 1e-4 and e['mutual_info'] <= 1e-3
+    elif mode == 1:
         assert e['model2'] <= 2e-2
         assert e['avg_likelihood'] <= 1e-3 and e['mutual_info'] <= 1e-2
+    else:
+        assert e['model2'] <= 2e-1
+        assert e['avg_likelihood'] <= 1e-2 and e['mutual_info'] <= 1e-1
 
 
 def test_dense_zero_block_drops_same_orientations(bright):
@@ -143,7 +152,9 @@ def test_fp32_gathers_vs_oracle(scaled):
     e = _errs(g, o)
     print('fp32 gathers scaled=%s' % scaled, {k: '%.2e' % v for k, v in e.items()}, flush=True)
     assert g['num_chunks'] > 1
-    assert e['u'] <= 1e-12 and e['inter_weight'] <= 1e-12
+    # W is interpolated from an FP32 copy of the model in FP32 (u sums those W in FP64); the
+    # merge weights are FP64
+    assert e['u'] <= 1e-6 and e['inter_weight'] <= 1e-12
     assert e['model2'] <= 1e-4
     assert e['avg_likelihood'] <= 1e-6 and e['mutual_info'] <= 1e-5
 
@@ -184,6 +195,6 @@ def test_fp32_gathers_refinement_subsets():
     print('fp32 gathers, refinement subsets', {k: '%.2e' % v for k, v in e.items()}, flush=True)
     assert g['num_chunks'] > 1
     assert g['rmax'][4] == -1
-    assert e['u'] <= 1e-12 and e['inter_weight'] <= 1e-12
+    assert e['u'] <= 1e-6 and e['inter_weight'] <= 1e-12
     assert e['model2'] <= 1e-4
     assert e['avg_likelihood'] <= 1e-6 and e['mutual_info'] <= 1e-5

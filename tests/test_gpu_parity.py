@@ -495,3 +495,34 @@ def test_sparse_prob(tiny):
     okeep = o['prob'] > 1e-6
     assert np.array_equal(keep, okeep)
     assert P.rel_err(prob, o['prob'][okeep]) <= TOL
+
+
+def test_estep_only_and_subset_mass(tiny_scaled):
+    """The expectation-step-only iteration gives the full iteration's posterior and
+    per-frame statistics and leaves model2, inter_weight and the scale factors alone; the
+    per-frame mass over listed orientations is the sum of those posterior entries."""
+    p = tiny_scaled
+    flags = dict(iteration=2, need_scaling=1, update_scale=1)
+    p2 = dict(p, fparams=dict(p['fparams'], scale=np.asarray(p['true_scale'], 'f8').copy()))
+    full = P.engine_for(p2)
+    g = P.run_engine(full, flags)
+    full.close()
+    eng = P.engine_for(p2)
+    eng.set_option('estep_only', 1)
+    e = P.run_engine(eng, flags)
+    for k in ('max_exp', 'p_norm', 'likelihood', 'info', 'psum_d', 'prob'):
+        assert np.array_equal(e[k], g[k]), k
+    assert np.array_equal(e['rmax'], g['rmax'])
+    assert e['avg_likelihood'] == g['avg_likelihood'] and e['mutual_info'] == g['mutual_info']
+    assert not e['model2'].any() and not e['inter_weight'].any()
+    assert np.array_equal(e['scale'], p2['fparams']['scale'])
+    D, R = g['prob'].shape
+    rng = np.random.default_rng(11)
+    sets = [np.sort(rng.choice(R, rng.integers(0, 30), replace=False)).astype('i4') for _ in range(D)]
+    off = np.concatenate([[0], np.cumsum([len(x) for x in sets])]).astype('i8')
+    mass = eng.subset_mass(off, np.concatenate(sets))
+    want = np.array([g['prob'][d, s].sum() for d, s in enumerate(sets)])
+    assert np.allclose(mass, want, rtol=1e-14, atol=1e-300)
+    with pytest.raises(Exception):
+        eng.subset_mass(off, np.concatenate(sets) + R)   # orientation out of range
+    eng.close()

@@ -398,3 +398,20 @@ def test_reference_import_name_and_command():
     out = subprocess.run([sys.executable, '-m', 'dragonfly.recon', '--help'], cwd=root, capture_output=True,
                          text=True, timeout=120)
     assert out.returncode == 0 and 'niter' in out.stdout
+
+
+def test_union_and_csr_refine_lists():
+    """Proposal unions are each frame's distinct cells in ascending order; CSR refinement
+    lists keep an accepted frame's cells and give a rejected frame the single entry -1 (every
+    cell, emc_set_refine)."""
+    from dragonfly_amd import proposal
+    rng = np.random.default_rng(6)
+    a, b = rng.integers(0, 40, (25, 5)), rng.integers(0, 40, (25, 3))
+    off, cells = proposal.union_cells(a, b)
+    for d in range(25):
+        assert list(cells[off[d]:off[d + 1]]) == sorted(set(a[d]) | set(b[d]))
+    acc = rng.random(25) < 0.6
+    ro, rq = proposal.refine_lists_csr(off, cells, acc, 40)
+    for d in range(25):
+        want = cells[off[d]:off[d + 1]] if acc[d] else [-1]   # -1: every cell
+        assert np.array_equal(rq[ro[d]:ro[d + 1]], want)
