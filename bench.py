@@ -75,55 +75,60 @@ def cpu_model_name():
     return 'unknown'
 
 
-def cpu_baseline(prob, itr, threads, background=None, target_rot=(12800, 25600), frames=(64, 512)):
-    """Iteration 1 of the oracle engine (OpenMP restatement of the reference engine) timed on
-    orientation subsets (every k-th orientation, so that about target_rot orientations
-    remain) and frame subsets, fitted to the reference engine's cost structure and
-    extrapolated to the workload:
+def cpu_baseline(prob, itr, threads, background=None, target_rot=(12800, 25600), frames=512):
+    """Iteration 1 of the oracle engine (OpenMP restatement of the reference engine) on
+    `frames` frames and two orientation subsets (every k-th orientation, about target_rot
+    orientations), extrapolated to the workload with the reference engine's cost structure:
 
-        T(N, n) = alpha N + n (beta N + gamma N^2)
+        T(N, n) = alpha N + n (beta N + gamma N^2) + c
 
-    for N orientations and n frames: alpha N is the per-orientation expand and merge,
-    beta N the per-frame photon sums, and gamma N^2 the lookup of every (frame, orientation)
-    by a linear search of the frame's per-thread keep list of ~N / threads entries
-    (max_update.c:86-93). Three runs: (N1, n1), (N1, n2), (N2, n2). Returns pairs/s at the
-    workload's size, the sample text and the fit."""
+    for N orientations and n frames. alpha N is the per-orientation work (slice generation,
+    log, merge), beta N the per-frame photon sums and gamma N^2 the lookup of every (frame,
+    orientation) by a linear search of the frame's per-thread keep list of ~N / threads
+    entries (max_update.c:86-93); c is the rest (allocation, combining the thread volumes).
+    Each run splits its own wall time into these parts with the oracle's per-thread timers
+    (thread-seconds in per-orientation work over thread-seconds in the orientation loops), so
+    no difference of two noisy run times is taken. Returns pairs/s at the workload's size,
+    the sample text and the fit."""
     from oracle import core
     from dragonfly_amd import synth
     f = prob['frames']
     R = itr.quat.num_rot
     D = itr.tot_num_data - int((itr.blacklist != 0).sum())
-    ks = [max(1, int(round(R / t))) for t in target_rot]
-    runs = [(ks[0], frames[0]), (ks[0], frames[1]), (ks[1], frames[1])]
+    n = min(frames, itr.tot_num_data)
+    sub = synth.take_frames(f, n)
     pts = []
-    for k, n in runs:
-        sub = synth.take_frames(f, n)
+    for target in target_rot:
+        k = max(1, int(round(R / target)))
         quats = itr.quat.quats[::k].copy()
         fp = dict(scale=np.ones(n), beta=itr.beta_start[:n].copy(), blacklist=np.zeros(n, 'u1'),
                   fcounts=itr.fcounts[:n], sum_fact=itr.sum_fact[:n], mean_count=itr.mean_count[0])
         bg = None if background is None else dict(background=background['background'], bgscale=np.ones(n))
-        t0 = time.perf_counter()
-        core.maximize(prob['det'], quats, itr.model.model1, sub, fp,
-                      dict(iteration=1, need_scaling=itr.params.need_scaling, update_scale=1),
-                      num_threads=threads, background=bg)
-        pts.append((len(quats), n, time.perf_counter() - t0))
-        log('cpu baseline: %d frames x %d orientations, %d threads: %.2f s' % (n, len(quats), threads, pts[-1][2]))
-    (N1, n1, t1), (_, n2, t2), (N2, _, t3) = pts
-    b1 = (t2 - t1) / (n2 - n1)              # per-frame cost at N1
-    alpha = (t1 - n1 * b1) / N1
-    b2 = (t3 - alpha * N2) / n2             # per-frame cost at N2
-    # b = beta N + gamma N^2 through (N1, b1), (N2, b2)
-    gamma = (b2 / N2 - b1 / N1) / (N2 - N1)
-    beta = b1 / N1 - gamma * N1
-    t_full = alpha * R + D * (beta * R + gamma * R * R)
-    sample = ('iteration 1 of the oracle engine (OpenMP restatement of the reference) on %s; fit T = alpha N + '
-              'n (beta N + gamma N^2) (alpha: per-orientation expand and merge; gamma: keep-list linear '
-              'search) gives alpha = %.3g s, beta = %.3g s, gamma = %.3g s; extrapolated to %d frames x %d '
-              'orientations: %.0f s'
-              % (', '.join('%d frames x %d orientations %.1f s' % p for p in pts), alpha, beta, gamma, D, R, t_full))
-    fit = dict(points=[dict(orientations=p[0], frames=p[1], seconds=round(p[2], 3)) for p in pts],
-               alpha_s=float('%.4g' % alpha), beta_s=float('%.4g' % beta), gamma_s=float('%.4g' % gamma),
-               extrapolated_s=round(t_full, 1))
+        out = core.maximize(prob['det'], quats, itr.model.model1, sub, fp,
+                            dict(iteration=1, need_scaling=itr.params.need_scaling, update_scale=1),
+                            num_threads=threads, background=bg)
+        sec = out['seconds']
+        loops = sec[0] + sec[1] + sec[3]
+        share = sec[5] / sec[6] if sec[6] > 0 else 0.
+        pts.append(dict(orientations=len(quats), frames=n, seconds=round(sec[4], 3),
+                        per_orientation_s=round(share * loops, 3),
+                        per_frame_s=round((1. - share) * loops + sec[2], 3),
+                        other_s=round(sec[4] - loops - sec[2], 3)))
+        log('cpu baseline: %d frames x %d orientations, %d threads: %.2f s (%s)' % (n, len(quats), threads, sec[4], pts[-1]))
+    (N1, N2) = (pts[0]['orientations'], pts[1]['orientations'])
+    alpha = 0.5 * (pts[0]['per_orientation_s'] / N1 + pts[1]['per_orientation_s'] / N2)
+    b1, b2 = pts[0]['per_frame_s'] / n, pts[1]['per_frame_s'] / n
+    gamma = max((b2 / N2 - b1 / N1) / (N2 - N1), 0.)
+    beta = max(b1 / N1 - gamma * N1, 0.)
+    c = 0.5 * (pts[0]['other_s'] + pts[1]['other_s'])
+    t_full = alpha * R + D * (beta * R + gamma * R * R) + c
+    sample = ('iteration 1 of the oracle engine (OpenMP restatement of the reference, %d threads) on %d frames at '
+              '%d and %d of the %d orientations (%.1f s, %.1f s); fit T = alpha N + n (beta N + gamma N^2) + c '
+              'from each run\'s per-orientation / per-frame split: alpha = %.3g s, beta = %.3g s, gamma = %.3g s, '
+              'c = %.2f s; extrapolated to %d frames x %d orientations: %.0f s'
+              % (threads, n, N1, N2, R, pts[0]['seconds'], pts[1]['seconds'], alpha, beta, gamma, c, D, R, t_full))
+    fit = dict(points=pts, alpha_s=float('%.4g' % alpha), beta_s=float('%.4g' % beta),
+               gamma_s=float('%.4g' % gamma), c_s=round(c, 3), extrapolated_s=round(t_full, 1))
     return D * R / t_full, sample, fit
 
 

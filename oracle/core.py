@@ -36,7 +36,7 @@ class Result(C.Structure):
         ('u', _dp), ('max_exp', _dp), ('p_norm', _dp), ('rmax', _ip), ('likelihood', _dp),
         ('info', _dp), ('psum_d', _dp), ('prob_dense', _dp),
         ('rescale', C.c_double), ('mutual_info', C.c_double), ('avg_likelihood', C.c_double),
-        ('seconds', C.c_double * 5), ('quat_norm', _dp),
+        ('seconds', C.c_double * 7), ('quat_norm', _dp),
     ]
 
 

@@ -865,11 +865,13 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
 
     /* calculate_rescale (max_prob.c:3-64) */
     double total = 0.;
+    double orient_s = 0., loop_s = 0.;  /* thread-seconds: per-orientation work, whole loops */
 #pragma omp parallel num_threads(nthreads) default(shared)
     {
         double priv_total = 0.;
         double *view = malloc(P * sizeof(double));
-#pragma omp for schedule(static, 1)
+        const double l0 = orc_now();
+#pragma omp for schedule(static, 1) nowait
         for (int r = 0; r < Rn; ++r) {
             const int rotind = r / M, mode = r % M;
             orc_slice_gen3d(&p->quats[rotind * 5], p->model1 + mode * vol1, p->size, P, p->qvals, p->corr,
@@ -878,8 +880,13 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
                 if (p->mask[t] < 1) res->u[r] += view[t];
             priv_total += p->quats[rotind * 5 + 4] * res->u[r];
         }
+        const double dl = orc_now() - l0;
 #pragma omp critical(orc_total)
-        total += priv_total;
+        {
+            total += priv_total;
+            orient_s += dl;
+            loop_s += dl;
+        }
         free(view);
     }
     res->rescale = p->mean_count / total * M;
@@ -905,13 +912,16 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
         for (int d = 0; d < N; ++d) pmax[d] = ORC_MAX_EXP_START;
 
         /* calculate_prob (max_prob.c:66-213) */
+        double my_orient = 0., my_loop = 0., lt = orc_now();
 #pragma omp for schedule(static, 1)
         for (int r = 0; r < Rn; ++r) {
             const int rotind = r / M, mode = r % M;
+            const double ot = orc_now();
             orc_slice_gen3d(&p->quats[rotind * 5], p->model1 + mode * vol1, p->size, P, p->qvals, p->corr,
                             view);
             if (!bg_on)
                 for (int t = 0; t < P; ++t) view[t] = log(view[t]);
+            my_orient += orc_now() - ot;
             double logw = log(p->quats[rotind * 5 + 4]);
             for (int d = 0; d < N; ++d) {
                 if (p->blacklist[d]) continue;
@@ -964,6 +974,7 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
                 }
             }
         }
+        my_loop += orc_now() - lt;
         if (tid == 0) t_prob = orc_now();
 
         /* normalize_prob (max_prob.c:215-265) */
@@ -988,7 +999,8 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
         if (tid == 0) t_norm = orc_now();
 
         /* update_tomogram + merge_tomogram (max_update.c:3-230) */
-#pragma omp for schedule(static, 1)
+        lt = orc_now();
+#pragma omp for schedule(static, 1) nowait
         for (int r = 0; r < Rn; ++r) {
             const int rotind = r / M, mode = r % M;
             double w_r = p->quats[rotind * 5 + 4], psum_r = 0.;
@@ -1043,11 +1055,20 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
                 }
             }
             if (psum_r > 0.) {
+                const double ot = orc_now();
                 for (int t = 0; t < P; ++t) view[t] /= psum_r;
                 orc_slice_merge3d(&p->quats[rotind * 5], view, pmodel + mode * vol1, pweight + mode * vol1,
                                   p->size, P, p->qvals, p->corr, p->mask);
+                my_orient += orc_now() - ot;
             }
         }
+        my_loop += orc_now() - lt;
+#pragma omp critical(orc_timing)
+        {
+            orient_s += my_orient;
+            loop_s += my_loop;
+        }
+#pragma omp barrier
 
         /* combine_information_omp (max_prob.c:267-342) */
 #pragma omp critical(orc_model)
@@ -1112,6 +1133,8 @@ double orc_maximize(const orc_problem *p, orc_result *res) {
     res->seconds[2] = t_norm - t_prob;
     res->seconds[3] = t2 - t_norm;
     res->seconds[4] = orc_now() - t0;
+    res->seconds[5] = orient_s;
+    res->seconds[6] = loop_s;
     return avg;
 }
 

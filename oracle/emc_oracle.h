@@ -81,7 +81,9 @@ typedef struct orc_result {
     double rescale;
     double mutual_info;
     double avg_likelihood;
-    double seconds[5];        /* rescale, prob, norm, update, total */
+    double seconds[7];        /* wall: rescale, prob, norm, update, total; then thread-seconds
+                                 in the orientation loops spent on per-orientation work (slice
+                                 generation, log, merge) and in those loops altogether */
     double *quat_norm;        /* optional [num_data][num_modes] occupancies (max_update.c:122-124) */
 } orc_result;
 
