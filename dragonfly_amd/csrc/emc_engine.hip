@@ -277,7 +277,7 @@ struct emc_engine {
     int sparse_rows = 0;  // frames per batch of the subset sparse posterior (0: 256 MB of rows)
     int full_part_frames = kFramesPerTomoPart;  // frames per range of the full-frame tomogram
     int estep_only = 0;   // stop after the posterior: no tomogram, merge or scale update
-    DevBuf<float> model32;        // FP32 copy of model1 for the FP32-gather expand
+    DevBuf<float> model32;        // FP32 brick copy of model1 for the FP32-gather expand
     bool model32_valid = false;   // cleared whenever model1 changes
     DevBuf<long> mass_off;
     DevBuf<int> mass_idx;
@@ -319,10 +319,9 @@ namespace {
 // FP32 copy of model1 (every mode) for the FP32-gather expand, refreshed after model1 changed.
 int ensure_model32(emc_engine *e) {
     if (e->model32_valid) return 0;
-    const size_t vol = (size_t)e->size * e->size * e->size * e->num_modes;
     int rc;
-    if ((rc = e->model32.alloc(vol))) return rc;
-    emc::launch_copy_f32(e->stream, e->model1.p, vol, e->model32.p);
+    if ((rc = e->model32.alloc(emc::model_brick_elems(e->size, e->num_modes)))) return rc;
+    emc::launch_copy_f32_brick(e->stream, e->model1.p, e->size, e->num_modes, e->model32.p);
     HIP_TRY(hipGetLastError());
     e->model32_valid = true;
     return 0;

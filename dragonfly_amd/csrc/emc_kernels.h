@@ -19,6 +19,9 @@ void launch_expand(hipStream_t s, const double *rots, const float *model, long s
                    const int *slot_rot = nullptr, int modes = 1);
 // out[i] = (float)in[i], i < n
 void launch_copy_f32(hipStream_t s, const double *in, size_t n, float *out);
+// FP32 copy of `modes` volumes in 4^3-voxel bricks (the FP32-gather expand's model)
+size_t model_brick_elems(long size, int modes);
+void launch_copy_f32_brick(hipStream_t s, const double *in, long size, int modes, float *out);
 void launch_to_f32(hipStream_t s, const double *P, int num_data, int r0, int C, float *out);
 void launch_likelihood(hipStream_t s, const float *logw, int C, int num_pix, const long *off,
                        const int *pix, const double *cnt, const uint8_t *blacklist, int num_data,

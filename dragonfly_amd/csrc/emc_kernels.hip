@@ -49,9 +49,16 @@ __device__ __forceinline__ double slice_value_model(const double *__restrict__ m
                                                     double q0, double q1, double q2, double corr, double center) {
     return slice_value_flat(model, size, rot, q0, q1, q2, corr, center);
 }
+// the FP32 model is the brick copy (copy_f32_brick_kernel)
 __device__ __forceinline__ double slice_value_model(const float *__restrict__ model, long size, const double *rot,
                                                     double q0, double q1, double q2, double corr, double center) {
-    return slice_value_f32m(model, size, rot, q0, q1, q2, corr, center);
+    return slice_value_f32b(model, size, rot, q0, q1, q2, corr, center);
+}
+// elements of one mode's volume: size^3 for the FP64 model, the padded brick grid for the FP32 copy
+__device__ __forceinline__ size_t mode_stride(const double *, long size) { return (size_t)(size * size * size); }
+__device__ __forceinline__ size_t mode_stride(const float *, long size) {
+    const size_t nb = (size_t)model_bricks(size);
+    return nb * nb * nb * 64;
 }
 // log W as stored: FP64 log for an FP64 model; FP32 log of the FP32 interpolation otherwise
 // (values below FLT_MIN, i.e. DBL_MIN outside the grid, keep the FP64 log)
@@ -96,8 +103,7 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
             for (int k = 0; k < kRTile / kWavesPerBlock; ++k) {
                 const int rr = wave + kWavesPerBlock * k;
                 // combined orientation index r = rotation * modes + mode (max_prob.c:80-84)
-                const MT *mdl =
-                    modes > 1 ? model + (size_t)((r0 + c0 + rr) % modes) * (size_t)(size * size * size) : model;
+                const MT *mdl = modes > 1 ? model + (size_t)((r0 + c0 + rr) % modes) * mode_stride(model, size) : model;
                 const double v = slice_value_model(mdl, size, &s_rot[rr * 9], q0, q1, q2, cr, center);
                 s_tile[lane][rr] = s_pad[rr] ? 1. : v;
             }
@@ -1122,6 +1128,33 @@ __global__ void copy_f32_kernel(const double *__restrict__ in, size_t n, float *
 void launch_copy_f32(hipStream_t s, const double *in, size_t n, float *out) {
     if (n == 0) return;
     copy_f32_kernel<<<grid_cap(cdiv(n, 256)), 256, 0, s>>>(in, n, out);
+}
+
+// FP32 brick copy of `modes` volumes of size^3 (layout of brick_index; voxels of the padded
+// brick grid past size - 1 are 0 and never read): one thread per output element.
+__global__ void copy_f32_brick_kernel(const double *__restrict__ in, long size, int modes, float *__restrict__ out) {
+    const int nb = model_bricks(size);
+    const size_t per_mode = (size_t)nb * nb * nb * 64, n = per_mode * modes;
+    const size_t vol = (size_t)size * size * size;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t m = i / per_mode, j = i % per_mode, b = j >> 6;
+        const int l = (int)(j & 63);
+        const long x = (long)(b / ((size_t)nb * nb)) * 4 + (l >> 4);
+        const long y = (long)((b / nb) % nb) * 4 + ((l >> 2) & 3);
+        const long z = (long)(b % nb) * 4 + (l & 3);
+        out[i] = (x < size && y < size && z < size) ? (float)in[m * vol + (size_t)(x * size + y) * size + z] : 0.f;
+    }
+}
+
+size_t model_brick_elems(long size, int modes) {
+    const size_t nb = (size_t)model_bricks(size);
+    return nb * nb * nb * 64 * (size_t)modes;
+}
+
+void launch_copy_f32_brick(hipStream_t s, const double *in, long size, int modes, float *out) {
+    const size_t n = model_brick_elems(size, modes);
+    if (n == 0) return;
+    copy_f32_brick_kernel<<<grid_cap(cdiv(n, 256)), 256, 0, s>>>(in, size, modes, out);
 }
 
 // FP32 copy of one orientation chunk of the tile-major posterior in 128-wide tiles:

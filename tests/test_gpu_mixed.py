@@ -51,7 +51,18 @@ def test_tiny_dense_vs_oracle(bright, mode):
     e = _errs(g, o)
     print('mode', mode, {k: '%.2e' % v for k, v in e.items()}, flush=True)
     # the expand and merge are FP64 in both paths
-    assert e['u'] <= 1e-12 and e['inter_weight'] <= 1e-12 and e['rescale'] <= 1e-12
+    assert e['u'] <= 1e-12 and e['rescale'] <= 1e-12
+    if mode != 4:
+        assert e['inter_weight'] <= 1e-12
+    else:
+        # one BF16 term moves some orientations in or out of every frame's keep set, and the
+        # weights sum over the orientations some frame keeps (psum_r > 0): compare that set
+        # with the FP64 engine's, which the weights then follow
+        eng0, g0 = _run(bright, 0, flags)
+        eng0.close()
+        moved = np.mean((g['psum_r'] > 0) != (g0['psum_r'] > 0))
+        print('mode 4: orientations merged in one path only: %.4f' % moved, flush=True)
+        assert moved <= 0.01 and e['inter_weight'] <= 0.1
     if mode == 2:
         assert e['model2'] <= 1e-3
         assert e['avg_likelihood'] <= 1e-6 and e['mutual_info'] <= 1e-6

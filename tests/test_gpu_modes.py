@@ -145,7 +145,9 @@ def test_modes_fp32_gathers():
     for k in ('avg_likelihood', 'mutual_info'):
         e[k] = abs(g[k] - o[k]) / abs(o[k])
     print('modes fp32 gathers', {k: '%.2e' % v for k, v in e.items()}, flush=True)
-    assert e['u'] <= 1e-12 and e['inter_weight'] <= 1e-12
+    # W is interpolated from an FP32 copy of the model in FP32 (u sums those W in FP64); the
+    # merge weights are FP64
+    assert e['u'] <= 1e-6 and e['inter_weight'] <= 1e-12
     assert e['model2'] <= 1e-4 and e['quat_norm'] <= 1e-4
     assert e['avg_likelihood'] <= 1e-6 and e['mutual_info'] <= 1e-5
 
