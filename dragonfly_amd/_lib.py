@@ -86,6 +86,7 @@ def lib():
             'emc_update_model': (C.c_int, [C.c_void_p, C.c_double, _dp]),
             'emc_normalize_scale': (C.c_int, [C.c_void_p, _dp]),
             'emc_slice_gen': (C.c_int, [C.c_void_p, _dp, _dp]),
+            'emc_slice_gen_static': (C.c_int, [C.c_void_p, _dp, _dp]),
             'emc_subset_mass': (C.c_int, [C.c_void_p, _lp, _ip, _dp]),
             'emc_slice_merge': (C.c_int, [C.c_void_p, _dp, _dp]),
             'emc_get_field': (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),

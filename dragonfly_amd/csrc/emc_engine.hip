@@ -11,6 +11,7 @@
 #include <cctype>
 #include <climits>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -141,6 +142,31 @@ constexpr int kPixPerGroup = 1024;
 constexpr int kFramesPerPsumPart = 512;
 constexpr int kFramesPerTomoPart = 12500;  // default of the full_part_frames option
 constexpr int kModelParts = 1024;
+
+// Staging memory of emc_slice_gen_static: the rotation matrix at [0, 9) and the device copy
+// of one view from kStaticViewOffset on. Each GPU holds its own instance; its address on a
+// device comes from hipGetSymbolAddress with that device current (scratch_ptr below), never
+// from the host-side name.
+constexpr long kStaticViewOffset = 16;
+static __device__ __align__(256) double scratch[kStaticViewOffset + EMC_STATIC_MAX_PIX];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, resolved once per device (the caller has made the
+// device current). Returns 0, or an error code with the message set.
+int resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(EMC_E_ARG, "device %d beyond %d devices", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return 0;
+    void *ptr = nullptr;
+    hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !ptr)
+        return fail(EMC_E_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device, hipGetErrorString(err));
+    g_scratch_ptr[device] = (double *)ptr;
+    return 0;
+}
 
 }  // namespace
 
@@ -1365,6 +1391,13 @@ int emc_create(int device, emc_engine **out) {
         return fail(EMC_E_HIP, "hipStreamCreate failed: %s", hipGetErrorString(err));
     }
     for (auto &ev : e->ev) (void)hipEventCreate(&ev);
+    // staging memory of emc_slice_gen_static on this device (resolved once per device)
+    if (int rc = resolve_scratch(device)) {
+        for (auto &ev : e->ev) (void)hipEventDestroy(ev);
+        (void)hipStreamDestroy(e->stream);
+        delete e;
+        return rc;
+    }
     // start values of the kernel-variant options from EMC_<NAME> (upper case)
     for (const EngineOption &o : kOptions) {
         std::string env = "EMC_";
@@ -2282,6 +2315,29 @@ int emc_slice_gen(emc_engine *e, const double *quat, double *view_out) {
                           e->d_perm.p, e->num_pix, e->one_view.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(view_out, e->one_view.p, e->num_pix * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int emc_slice_gen_static(emc_engine *e, const double *quat, double *view_out) {
+    if (!e || !quat || !view_out) return fail(EMC_E_ARG, "emc_slice_gen_static: bad argument");
+    if (!e->num_pix || !e->size) return fail(EMC_E_STATE, "emc_slice_gen_static: detector/model not set");
+    if (e->num_pix > EMC_STATIC_MAX_PIX)
+        return fail(EMC_E_ARG, "emc_slice_gen_static: %d pixels exceed the static staging maximum of %d pixels",
+                    e->num_pix, EMC_STATIC_MAX_PIX);
+    HIP_TRY(hipSetDevice(e->device));
+    double *ptr = g_scratch_ptr[e->device];  // set by emc_create on this device
+    if (!ptr) return fail(EMC_E_STATE, "emc_slice_gen_static: staging memory of device %d not resolved", e->device);
+    double rot[9];
+    make_rot(quat, rot);
+    // the copy and the kernel are ordered on e->stream, and the stream is drained before rot
+    // goes out of scope
+    HIP_TRY(hipMemcpyAsync(ptr, rot, sizeof rot, hipMemcpyHostToDevice, e->stream));
+    emc::launch_slice_gen(e->stream, ptr, e->model1.p, e->size, e->qvals.p, e->corr.p, e->d_perm.p, e->num_pix,
+                          ptr + kStaticViewOffset);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(view_out, ptr + kStaticViewOffset, e->num_pix * sizeof(double), hipMemcpyDeviceToHost,
+                           e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     return 0;
 }

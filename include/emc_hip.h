@@ -199,6 +199,17 @@ int emc_normalize_scale(emc_engine *e, double *mean_scale);
  * Model.slice_gen (dragonfly/model.pyx:105-131 -> slice_gen3d model.c:11-59): view of
  * model1 at one quaternion.                                                             */
 int emc_slice_gen(emc_engine *e, const double *quat, double *view_out);
+/* emc_slice_gen without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * or per-orientation loops, harnesses that forbid hipMalloc) once the engine's detector and
+ * model are set. The rotation and the device copy of the view are staged in a fixed
+ * library array of EMC_STATIC_MAX_PIX + 16 doubles per GPU (8 MiB, held until the process
+ * ends) instead of engine buffers; num_pix above EMC_STATIC_MAX_PIX returns EMC_E_ARG.
+ * Same arguments, result and errors as emc_slice_gen otherwise. Every call on one GPU
+ * shares that array: calls that overlap on one GPU (several threads, several engines on
+ * the device) race and must be serialised by the caller.                                */
+#define EMC_STATIC_MAX_PIX 1048576
+int emc_slice_gen_static(emc_engine *e, const double *quat, double *view_out);
 /* Model.slice_merge (dragonfly/model.pyx:133-150 -> slice_merge3d model.c:171-244):
  * merge one view into model2 / inter_weight.                                            */
 int emc_slice_merge(emc_engine *e, const double *quat, const double *view);
