@@ -346,25 +346,27 @@ def bench_dense(args):
 def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
     """One beta setting of the learned-proposal workload (fine num_div 16, coarse num_div 4).
 
-    (1) Plain FP64 iteration k over every fine orientation from the true intensity at photon
-    scale; its posterior summed over the Voronoi cells of the coarse sampling gives the
-    training targets (the method's own output of an earlier iteration) and its updated model
-    and scales the input of iteration k + 1. (2) Plain FP64 iteration k + 1, timed
-    (`fp64_steps` times from the same input): the denominator and the reference model.
-    (3) The network (polar photon histogram -> coarse-cell logits) is trained on 90% of the
-    frames. (4) Iteration k + 1 from the same input with proposals, timed: a full search over
-    the 3240 coarse orientations (expectation step only) judges every proposal -- a frame keeps
-    its proposed cells (top-k of the network, joined with the coarse search's most likely cell)
-    when the coarse posterior puts at least `judge_threshold` of its mass inside them, and
-    falls back to the full fine search otherwise; then one refinement iteration over the kept
-    cells (FP32 gathers) and the model update. Each timed step includes the coarse pass, the
-    proposals, the lists and the iteration. (5) The same with the no-network proposals: each
-    frame's top-k cells of iteration k's cell posterior."""
+    (1) Plain FP64 iterations 1..k over every fine orientation from the true intensity at
+    photon scale (`--converge-iters` k, so that iteration k + 1 is a late iteration of a
+    converging run); iteration k's posterior summed over the Voronoi cells of the coarse
+    sampling gives the training targets (the method's own output of an earlier iteration), its
+    top cells the no-network proposals, and its updated model and scales the input of iteration
+    k + 1. (2) Plain FP64 iteration k + 1, timed (`fp64_steps` times from the same input): the
+    denominator and the reference model. (3) The network (polar photon histogram -> coarse-cell
+    logits) is trained on 90% of the frames. (4) Iteration k + 1 from the same input with
+    proposals, timed: a full search over the 3240 coarse orientations (expectation step only),
+    each frame's proposal (the network's top-k cells joined with the coarse search's top cells),
+    pass 1 over the proposals, the likelihood judge (mass outside each proposal estimated from
+    the coarse search against the fine maximum inside it; frames above 1 - judge_threshold fall
+    back to the full fine search) and the M-step (resumed from pass 1 when no frame falls back,
+    else a second pass), with FP32 gathers, and the model update. (5) The same with the
+    no-network proposals: iteration k's top-k cells in place of the network's."""
     import torch
     from dragonfly_amd import _lib, proposal
     det, mask = prob['det'], prob['det']['mask']
     true_model = np.ascontiguousarray(prob['intens'] * prob['frames']['rescale'])
-    k_top, tau = cfg['top_k'], args.judge_threshold
+    k_top, k_coarse = cfg['top_k'], args.coarse_k
+    tau = cfg['mass_threshold'] if args.judge_threshold is None else args.judge_threshold
 
     def recon_for(num_div, precision=0):
         recon, itr = build_recon(prob, dict(cfg, num_div=num_div), start=true_model, precision=precision)
@@ -372,16 +374,21 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
         itr.params.iteration = 1
         return recon, itr
 
-    # (1) iteration k, plain: targets and the input of iteration k + 1
+    # (1) iterations 1..k, plain: targets, previous proposals and the input of iteration k + 1
     recon, itr = recon_for(cfg['num_div'])
     D, R = itr.tot_num_data, itr.quat.num_rot
     qcoarse = _lib.quat_gen(cfg['coarse_div'])
     num_coarse = len(qcoarse)
     qmap = _lib.voronoi_subset(qcoarse, itr.quat.quats)
-    t0 = time.perf_counter()
-    recon.run_iteration()
-    log('beta x %g: iteration k (plain FP64, with set-up): %.3f s' % (beta_factor, time.perf_counter() - t0))
+    kk = max(1, args.converge_iters)
+    for it in range(1, kk + 1):
+        itr.params.iteration = it
+        t0 = time.perf_counter()
+        recon.run_iteration()
+        log('beta x %g: iteration %d (plain FP64): %.3f s, rms change %.3e' % (beta_factor, it, time.perf_counter() - t0,
+                                                                                itr.rms_change))
     post = recon.engine.cell_posterior(num_coarse, qmap)
+    prev_top, _ = recon.engine.cell_topk(num_coarse, qmap, k_top)
     model_k1 = itr.model.model1.copy()
     scale_k1 = np.array(itr.scale, dtype='f8')
 
@@ -389,7 +396,7 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
         it.model.model1 = model_k1
         it.scale[:] = scale_k1
         rec.engine.set_scale(scale_k1)
-        it.params.iteration = 2
+        it.params.iteration = kk + 1
 
     # (2) iteration k + 1, plain, timed
     plain_times = []
@@ -412,6 +419,7 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
     off, pix, cnt = proposal.photon_lists(prob['frames'], mask)
     X = net.features(off, pix, cnt, proposal.polar_bins(det['qvals'], mask))
     Y = torch.tensor(post, dtype=torch.float32, device='cuda')
+    del post
     ntr = int(0.9 * D)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -419,8 +427,6 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     log('beta x %g: trained %d epochs in %.1f s, loss %.4f' % (beta_factor, args.epochs, train_s, hist[-1]))
-    # previous iteration's cell posterior, kept on the device for the no-network proposals
-    logpost = torch.log(Y.clamp_min(1e-30))
     del Y
 
     # (4), (5) iteration k + 1 with judged proposals
@@ -432,25 +438,19 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
 
     def sources():
         return {'network': lambda: net.propose(X, D, k_top),
-                'previous_posterior': lambda: proposal.topk_device(logpost, D, num_coarse, k_top)[0]}
+                'previous_posterior': lambda: prev_top}
 
     def one_step(source):
-        """Judge, proposals, lists and the refinement iteration; returns (seconds, info)."""
+        """Judge, proposals, both passes and the model update; returns (seconds, info)."""
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        judge.estep(model_k1, scale_k1, beta_k1, 2, cfg['need_scaling'])
+        judge.estep(model_k1, scale_k1, beta_k1, kk + 1, cfg['need_scaling'])
         top = sources()[source]()
-        offs, cells = proposal.union_cells(top, judge.argmax_cells())
-        mass = judge.mass(offs, cells)
-        accept = mass >= tau
-        rel_off, rel_quat = proposal.refine_lists_csr(offs, cells, accept, num_coarse)
-        itr.set_refine_lists(cfg['coarse_div'], rel_off, rel_quat)
-        recon.engine.set_refine(itr.num_coarse, itr.quat_mapping, itr.rel_off, itr.rel_quat)
-        t1 = time.perf_counter()
-        recon.run_iteration()
+        tops = [top, judge.top_cells(k_coarse)] if k_coarse > 0 else [top]
+        info = proposal.judged_iteration(recon, itr, judge, tops, cfg['coarse_div'], tau)
         torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        return t2 - t0, dict(setup_s=t1 - t0, accept=accept, top=top, mass=mass)
+        info['top'] = top
+        return time.perf_counter() - t0, info
 
     results = {}
     for source, nsteps, nwarm in (('network', timed_steps, warmup), ('previous_posterior', args.baseline_steps, 1)):
@@ -466,27 +466,25 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
         step_s = float(np.median(times))
         acc = info['accept']
         held = np.arange(D) >= ntr
-        # iteration k's fine posterior mass inside the proposals that were kept
-        captured = np.take_along_axis(post, info['top'], 1).sum(1)
         res = dict(step_ms=round(step_s * 1e3, 1), steps_ms=[round(t * 1e3, 1) for t in times],
-                   num_chunks=recon.last_stats['num_chunks'],
-                   setup_ms=round(info['setup_s'] * 1e3, 1), speedup_vs_fp64=round(plain_s / step_s, 2),
+                   num_chunks=recon.last_stats['num_chunks'], passes=info['passes'],
+                   cells_per_frame=round(info['cells_per_frame'], 2), speedup_vs_fp64=round(plain_s / step_s, 2),
                    fallback_rate=round(float(1. - acc.mean()), 4),
                    heldout_fallback_rate=round(float(1. - acc[held].mean()), 4),
-                   judge_mass_accepted_mean=round(float(info['mass'][acc].mean()), 5) if acc.any() else None,
-                   prev_mass_in_topk_mean=round(float(captured.mean()), 5),
+                   judge_outside_accepted_mean=float('%.3g' % info['outside'][acc].mean()) if acc.any() else None,
                    model_rel_err=float('%.3g' % (np.abs(ml - model_plain).max() / np.abs(model_plain).max())),
                    model_rms_rel_err=float('%.3g' % np.sqrt(np.mean((ml - model_plain) ** 2) / np.mean(model_plain ** 2))),
                    rmax_agree=round(float(np.mean(recon.rmax == rmax_plain)), 5),
-                   stage_ms={k: round(v, 1) for k, v in recon.last_stats['stage_ms'].items()})
+                   stage_ms={k: round(v, 1) for k, v in recon.last_stats['stage_ms'].items()},
+                   pass1_stage_ms={k: round(v, 1) for k, v in info['estep_stage_ms'].items()})
         results[source] = res
-        log('beta x %g, %s: %s' % (beta_factor, source, json.dumps({k: v for k, v in res.items() if k != 'stage_ms'})))
+        log('beta x %g, %s: %s' % (beta_factor, source, json.dumps(res)))
     judge_ms = judge.stats['stage_ms'] if judge.stats else {}
     judge.close()
     recon.free()
     return dict(D=D, R=R, num_coarse=num_coarse, plain_s=plain_s, plain_times=plain_times, st_plain=st_plain,
                 train_s=train_s, epochs=args.epochs, loss=hist[-1], results=results,
-                judge_stage_ms={k: round(v, 1) for k, v in judge_ms.items()}, itr=itr)
+                judge_stage_ms={k: round(v, 1) for k, v in judge_ms.items()}, itr=itr, k=kk, tau=tau)
 
 
 def bench_learned(args):
@@ -539,11 +537,13 @@ def bench_learned(args):
         'dtype': ('f32 gathers, f64 sums' if args.precision == 3 else 'f64') + ' EMC / bf16 proposal network',
         'data': 'synthetic',
         'config': {'workload': '%s: %d frames (mean 200 photons, scale U(0.5, 1.5)), 251^3 model, 260x257 '
-                               'detector, fine num_div %d (%d orientations), coarse num_div %d (%d), top-%d '
-                               'proposals + coarse arg-max, judged by a coarse full search (fall back below '
-                               '%.4g of its mass), beta_start x %g (timed); beta_start x 1 beside it'
+                               'detector, fine num_div %d (%d orientations), coarse num_div %d (%d); iteration '
+                               '%d from %d plain iterations started at the true intensity; proposals = top-%d '
+                               'cells of the network + top-%d of a coarse full search; likelihood judge (fall '
+                               'back to the full search where the estimated mass outside exceeds %.4g); '
+                               'beta_start x %g (timed); beta_start x 1 beside it'
                                % (args.config, D, cfg['num_div'], R, cfg['coarse_div'], head['num_coarse'],
-                                  cfg['top_k'], args.judge_threshold, late),
+                                  head['k'] + 1, head['k'], cfg['top_k'], args.coarse_k, 1. - head['tau'], late),
                    'path': 'learned proposal + likelihood judge + sparse refinement', 'precision': args.precision,
                    'frames': D, 'num_rot': R},
         'speedup_vs_fp64': net['speedup_vs_fp64'], 'fallback_rate': net['fallback_rate'],
@@ -671,8 +671,13 @@ def main():
     ap.add_argument('--epochs', type=int, default=80, help='learned-proposal workload: training epochs')
     ap.add_argument('--wd', type=float, default=0., help='learned-proposal workload: L2 weight decay')
     ap.add_argument('--top-k', type=int, default=0, help='learned-proposal workload: cells proposed per frame')
-    ap.add_argument('--judge-threshold', type=float, default=0.999,
-                    help='learned-proposal workload: coarse-search posterior mass a proposal must hold')
+    ap.add_argument('--judge-threshold', type=float, default=None,
+                    help='learned-proposal workload: estimated posterior mass a proposal must hold '
+                         '(default: the configuration\'s mass_threshold, 0.99)')
+    ap.add_argument('--coarse-k', type=int, default=4,
+                    help='learned-proposal workload: cells of the coarse search joined to every proposal')
+    ap.add_argument('--converge-iters', type=int, default=3,
+                    help='learned-proposal workload: plain iterations before the timed one (k)')
     ap.add_argument('--judge-precision', type=int, default=3,
                     help='learned-proposal workload: arithmetic of the coarse judging search (0 FP64, 3 FP32 gathers)')
     ap.add_argument('--beta-factor', type=float, default=None,

@@ -88,6 +88,8 @@ def lib():
             'emc_slice_gen': (C.c_int, [C.c_void_p, _dp, _dp]),
             'emc_slice_gen_static': (C.c_int, [C.c_void_p, _dp, _dp]),
             'emc_subset_mass': (C.c_int, [C.c_void_p, _lp, _ip, _dp]),
+            'emc_judge_outside': (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, _dp]),
+            'emc_cell_topk': (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, _ip, C.POINTER(C.c_float)]),
             'emc_slice_merge': (C.c_int, [C.c_void_p, _dp, _dp]),
             'emc_get_field': (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
             'emc_get_sparse_prob': (C.c_int, [C.c_void_p, _ip, C.c_long, _ip, _dp, _lp]),
@@ -303,6 +305,27 @@ class Engine:
         out = np.empty(self.num_data)
         _check(lib().emc_subset_mass(self._h, _ptr(rel_off, _lp), _ptr(rel_quat, _ip) if rel_quat.size else None,
                                      _ptr(out)))
+        return out
+
+    def cell_topk(self, num_coarse, quat_mapping, k):
+        """Each frame's k most likely coarse cells after an iteration (descending mass) and the
+        mass they hold: ((num_data, k) int32, (num_data,) float32)."""
+        qm = np.ascontiguousarray(quat_mapping, dtype='i4')
+        if qm.shape != (self.num_rot,):
+            raise ValueError('quat_mapping must have num_rot entries')
+        idx = np.empty((self.num_data, int(k)), dtype='i4')
+        mass = np.empty(self.num_data, dtype='f4')
+        _check(lib().emc_cell_topk(self._h, int(num_coarse), _ptr(qm, _ip), int(k), _ptr(idx, _ip),
+                                   mass.ctypes.data_as(C.POINTER(C.c_float))))
+        return idx, mass
+
+    def judge_outside(self, coarse, prob_floor=0.):
+        """After an estep_only iteration on refinement subsets (the proposals) and a plain
+        estep_only iteration of `coarse` (one orientation per coarse cell, same frames): per
+        frame, the estimated posterior mass of the full fine search outside its cells (with
+        prob_floor > 0: in cells whose estimated posterior per orientation exceeds it)."""
+        out = np.empty(self.num_data)
+        _check(lib().emc_judge_outside(self._h, coarse._h, float(prob_floor), _ptr(out)))
         return out
 
     def slice_merge(self, quat, view):

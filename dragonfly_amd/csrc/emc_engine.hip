@@ -11,6 +11,7 @@
 #include <cctype>
 #include <climits>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -32,6 +33,10 @@ int fail(int code, const char *fmt, ...) {
     g_last_error = buf;
     return code;
 }
+
+// Counts device allocations and frees of the engine buffers (every engine): a captured graph
+// holds buffer addresses, so it is captured again once any buffer may have moved.
+std::atomic<unsigned long> g_alloc_gen{0};
 
 #define HIP_TRY(expr)                                                                        \
     do {                                                                                     \
@@ -72,6 +77,7 @@ struct DevBuf {
         }
         release();
         if (count == 0) return 0;
+        ++g_alloc_gen;
         hipError_t err = hipMalloc((void **)&p, count * sizeof(T));
         if (err != hipSuccess) {
             p = nullptr;
@@ -88,7 +94,10 @@ struct DevBuf {
         return 0;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            ++g_alloc_gen;
+            (void)hipFree(p);
+        }
         p = nullptr;
         n = cap = 0;
     }
@@ -276,6 +285,7 @@ struct emc_engine {
     struct BrickChunk {
         int r0 = -1, Cc = 0, num_units = 0;
         DevBuf<int> list;
+        DevBuf<unsigned long long> lmask;  // per list entry: the item's pixels with a corner in the brick
         DevBuf<int4> units;
     };
     std::vector<BrickChunk> bricks;
@@ -303,11 +313,28 @@ struct emc_engine {
     int sparse_rows = 0;  // frames per batch of the subset sparse posterior (0: 256 MB of rows)
     int full_part_frames = kFramesPerTomoPart;  // frames per range of the full-frame tomogram
     int estep_only = 0;   // stop after the posterior: no tomogram, merge or scale update
+    int mstep_only = 0;   // subset path: resume after an estep_only iteration with the M-step
+    int graph_mode = 0;   // subset path: run the iteration as a captured HIP graph
+    int fused = 0;        // subset path: persistent expand + pair likelihood kernel
+    DevBuf<long> fused_items;  // work-item offsets of every chunk
+    DevBuf<unsigned> fused_counter;
+    DevBuf<int> fused_done, fused_err;
+    std::vector<long> fused_base, fused_total, fused_key;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<long> graph_key;  // what the graph was captured for (buffers, sizes, options)
+    unsigned long layout_gen = 0;  // bumped by every setter that changes sizes, lists or options
+    // state generation: every setter (and model / scale update) increments it; an estep_only
+    // subset iteration records it, and mstep_only runs only while it is unchanged
+    unsigned long gen = 0, estep_gen = ~0ul;
     DevBuf<float> model32;        // FP32 brick copy of model1 for the FP32-gather expand
     bool model32_valid = false;   // cleared whenever model1 changes
     DevBuf<long> mass_off;
     DevBuf<int> mass_idx;
     DevBuf<double> mass_out;
+    // emc_judge_outside: per-cell constants, per-frame skip flags and estimates
+    DevBuf<double> judge_logn, judge_lwr, judge_out;
+    DevBuf<uint8_t> judge_skip;
+    std::vector<uint8_t> h_blacklist;  // host copy of the blacklist (emc_set_frame_params)
     std::vector<int> h_ctile0, h_ctile1;  // packed: first and last tile of each cell
     DevBuf<int> slot_cell;
     // full frames (every coarse cell): rows tile-major in RF; photon lists of mask < 2 pixels
@@ -595,7 +622,7 @@ int build_bricks(emc_engine *e, std::vector<emc_engine::BrickChunk> &out, const 
         bc.Cc = Cc;
         HIP_TRY(hipMemsetAsync(e->brick_counts.p, 0, nb * sizeof(int), s));
         emc::launch_brick_items(s, rots, r0, Cc, num_valid, e->qvals.p, e->mask.p, P, e->size, npatch, nbx, 0,
-                                e->brick_counts.p, nullptr, nullptr, nullptr, slot_rot, modes);
+                                e->brick_counts.p, nullptr, nullptr, nullptr, nullptr, slot_rot, modes);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(counts.data(), e->brick_counts.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
@@ -610,10 +637,11 @@ int build_bricks(emc_engine *e, std::vector<emc_engine::BrickChunk> &out, const 
         }
         if (tot >= INT32_MAX) return fail(EMC_E_ARG, "brick merge: too many work items");
         if ((rc = bc.list.alloc(tot > 0 ? tot : 1))) return rc;
+        if ((rc = bc.lmask.alloc(tot > 0 ? tot : 1))) return rc;
         if ((rc = e->brick_off.upload(off.data(), nb, s))) return rc;
         HIP_TRY(hipMemsetAsync(e->brick_fill.p, 0, nb * sizeof(int), s));
         emc::launch_brick_items(s, rots, r0, Cc, num_valid, e->qvals.p, e->mask.p, P, e->size, npatch, nbx, 1,
-                                nullptr, e->brick_off.p, e->brick_fill.p, bc.list.p, slot_rot, modes);
+                                nullptr, e->brick_off.p, e->brick_fill.p, bc.list.p, bc.lmask.p, slot_rot, modes);
         HIP_TRY(hipGetLastError());
         bc.num_units = (int)units.size();
         if (!units.empty() && (rc = bc.units.upload(units.data(), units.size(), s))) return rc;
@@ -629,12 +657,14 @@ int build_bricks(emc_engine *e) {
 }
 
 int merge_chunk(const emc_engine *e, const emc_engine::BrickChunk &bc, const double *rots, int num_valid,
-                const double *view, const double *psum, bool all_on, bool do_model, bool do_weight) {
+                const double *view, const double *psum, bool all_on, bool do_model, bool do_weight,
+                const int *gate = nullptr) {
     const int nbx = (int)((e->size + emc::kBrick - 1) / emc::kBrick);
-    emc::launch_merge_bricks(e->stream, bc.units.p, bc.num_units, bc.list.p, rots, bc.r0, num_valid, view, psum,
+    emc::launch_merge_bricks(e->stream, bc.units.p, bc.num_units, bc.list.p, bc.lmask.p, rots, bc.r0, num_valid,
+                             view, psum,
                              all_on, e->qvals.p, e->corr.p, e->mask.p, e->num_pix, e->size,
                              (e->num_pix + 63) / 64, nbx, e->model2.p, e->weight.p, do_model, do_weight,
-                             e->precision != 0);  // reduced-precision modes sum bricks in FP32
+                             e->precision != 0, gate);  // reduced-precision modes sum bricks in FP32
     return 0;
 }
 
@@ -954,6 +984,17 @@ int merge_chunk(emc_engine *e, int chunk, const double *view, bool all_on, bool 
 // Work buffers. The posterior matrix R/P holds all frames x all orientations (the keep
 // rule needs each frame's global maximum before any normalisation); log W / view hold one
 // orientation chunk of C columns, C chosen to fit the remaining memory.
+// Device bytes the merge's per-brick item lists will take beyond what they hold now, for
+// `cols` orientation columns: about 2.8 list entries (int item id + 64-bit pixel mask) per
+// (orientation, 64-pixel patch) at the benchmark geometries, budgeted at 4. The chunk buffers
+// are sized on what is left.
+size_t brick_list_extra(const std::vector<emc_engine::BrickChunk> &chunks, size_t cols, int num_pix) {
+    const size_t want = cols * (size_t)((num_pix + 63) / 64) * 4 * (sizeof(int) + sizeof(unsigned long long));
+    size_t held = 0;
+    for (const auto &bc : chunks) held += bc.list.cap * sizeof(int) + bc.lmask.cap * sizeof(unsigned long long);
+    return want > held ? want - held : 0;
+}
+
 int ensure_work(emc_engine *e) {
     int rc;
     const size_t N = e->num_data, Rp = e->Rp, P = e->num_pix;
@@ -963,7 +1004,7 @@ int ensure_work(emc_engine *e) {
     if ((rc = e->R.alloc(N * Rp))) return rc;
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
-    const size_t reserve = (size_t)4 << 30;
+    const size_t reserve = ((size_t)4 << 30) + brick_list_extra(e->bricks, Rp, P);
     size_t budget = freeb > reserve ? freeb - reserve : 0;
     const size_t cap = (size_t)120 << 30;
     budget += e->logw.cap * sizeof(double);
@@ -1093,6 +1134,17 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     const int nparts = e->full_parts;
     // memory: the plain path's rows are not needed here
     e->R.release();
+    if (e->Rsub.cap < (size_t)std::max<long>(e->num_pairs, 1) * emc::kRTile ||
+        (nF > 0 && e->RF.cap < (size_t)nF * T * emc::kRTile)) {
+        // the rows grow: hand the orientation-chunk buffers' room back first, so that the
+        // chunk width below is sized on what the new rows leave free
+        e->logw.release();
+        e->C = 0;
+        e->bricks_valid = false;
+        e->tom_partial.release();
+        e->logw32.release();
+        e->p32.release();
+    }
     if ((rc = e->Rsub.alloc((size_t)std::max<long>(e->num_pairs, 1) * emc::kRTile))) return rc;
     if (nF > 0) {
         if ((rc = e->RF.alloc((size_t)nF * T * emc::kRTile))) return rc;
@@ -1101,15 +1153,14 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
         if ((rc = e->psum_full.alloc(S))) return rc;
         if ((rc = e->scale_full.alloc(nF))) return rc;
         if ((rc = e->blacklist_full.alloc(nF))) return rc;
-    } else {
-        e->RF.release();
-    }
+    }  // with no full frames RF keeps its room: the next set of lists may have them again
     const bool g32 = e->precision == 3;
     if (g32 && (rc = e->Rsub32.alloc((size_t)std::max<long>(e->num_pairs, 1) * emc::kRTile))) return rc;
     if (g32 && (rc = ensure_model32(e))) return rc;
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
-    size_t budget = (freeb > ((size_t)4 << 30) ? freeb - ((size_t)4 << 30) : 0) + e->logw.cap * sizeof(double) +
+    const size_t reserve = ((size_t)4 << 30) + brick_list_extra(e->slot_bricks, S, P);
+    size_t budget = (freeb > reserve ? freeb - reserve : 0) + e->logw.cap * sizeof(double) +
                     e->tom_partial.cap * sizeof(double) + (e->logw32.cap + e->p32.cap) * sizeof(float);
     budget = std::min(budget, (size_t)120 << 30);
     // per slot column: log W / view, the full frames' partial views, and with FP32 gathers the
@@ -1120,7 +1171,11 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     if (e->max_chunk_cols > 0) cmax = std::min(cmax, (size_t)e->max_chunk_cols);
     cmax = cmax / emc::kRTile * emc::kRTile;
     if (cmax < (size_t)emc::kRTile) return fail(EMC_E_NOMEM, "no device memory for an orientation chunk");
-    const int C = (int)std::min(cmax, (size_t)S);
+    int C = (int)std::min(cmax, (size_t)S);
+    // keep the chunk width of the previous iteration while it still fits: the merge's brick
+    // lists are built per chunk, and lists that alternate between fewer and more full frames
+    // (proposals judged in two passes) would otherwise rebuild them every time
+    if (e->slot_C > 0 && e->slot_C < C && e->slot_bricks_valid) C = e->slot_C;
     if (C != e->slot_C) e->slot_bricks_valid = false;
     e->slot_C = C;
     if (e->logw.cap < (size_t)P * C) {
@@ -1158,8 +1213,13 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
 
     const int use_scale = opts->need_scaling && (opts->iteration > 1 || opts->known_scale);
     const int want_psum_d = opts->need_scaling && opts->update_scale;
+    const bool graph = e->graph_mode != 0;
+    if (graph && (e->estep_only || e->mstep_only))
+        return fail(EMC_E_STATE, "emc_maximize: the graph option runs whole iterations only");
     float ms[EMC_NUM_STAGES] = {};
+    bool capturing = false;  // enqueue into a graph: no events, no host synchronisation
     auto timed = [&](int stage, auto &&fn) -> int {
+        if (capturing) return fn();
         HIP_TRY(hipEventRecord(e->ev[0], s));
         int r = fn();
         if (r) return r;
@@ -1171,165 +1231,341 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
         ms[stage] += a;
         return 0;
     };
+    const uint8_t *dense = e->any_dense ? e->dense.p : nullptr;
+    // expectation step: expand, likelihood, per-frame maximum, normaliser and posterior, psum
+    // persistent expand + likelihood: work-item offsets of every chunk, built once per set of
+    // lists and chunk width (no upload inside the iteration, which a graph may hold)
+    const bool fused = e->fused != 0;
+    if (fused) {
+        const std::vector<long> fkey = {(long)e->layout_gen, C, S, (long)e->num_pairs};
+        if (fkey != e->fused_key) {
+            const int ng = (P + kPixPerGroup - 1) / kPixPerGroup;
+            std::vector<long> items;
+            e->fused_base.clear();
+            e->fused_total.clear();
+            int maxt = 0;
+            for (int s0 = 0; s0 < S; s0 += C) {
+                const int Cc = std::min(C, S - s0);
+                const int t0 = s0 / emc::kRTile, nt = Cc / emc::kRTile;
+                maxt = std::max(maxt, nt);
+                e->fused_base.push_back((long)items.size());
+                long acc = 0;
+                items.push_back(0);
+                for (int k = 0; k <= nt; ++k) {
+                    if (k < nt) acc += ng;
+                    if (k >= 1) {
+                        const long np = e->h_toff[t0 + k] - e->h_toff[t0 + k - 1];
+                        acc += (np + emc::kFusedPairBatch - 1) / emc::kFusedPairBatch;
+                    }
+                    items.push_back(acc);
+                }
+                e->fused_total.push_back(acc);
+            }
+            if ((rc = e->fused_items.upload(items.data(), items.size(), s))) return rc;
+            if ((rc = e->fused_done.alloc((size_t)maxt + 1))) return rc;
+            if ((rc = e->fused_counter.alloc(1))) return rc;
+            if ((rc = e->fused_err.alloc(1))) return rc;
+            emc::launch_fill_int(s, e->fused_err.p, 1, 0);
+            e->fused_key = fkey;
+        }
+    }
+    auto estep_work = [&]() -> int {
+            for (int s0 = 0; s0 < S; s0 += C) {
+                const int Cc = std::min(C, S - s0);
+                if (fused) {
+                    // expand and pair likelihood in one persistent launch (timed as likelihood)
+                    if ((rc = timed(1, [&] {
+                             const int t0 = s0 / emc::kRTile, nt = Cc / emc::kRTile, ci = s0 / C;
+                             const long *items = e->fused_items.p + e->fused_base[ci];
+                             if (g32)
+                                 emc::launch_fused_expand_likelihood(
+                                     s, e->rots_slot.p, e->model32.p, e->size, e->qvals.p, e->corr.p, e->mask.p, P, s0,
+                                     Cc, kPixPerGroup, e->logw32.p, e->upart.p, e->ubad.p, e->slot_rot.p, e->lik_off.p,
+                                     e->lik_pix.p, e->lik_cnt.p, e->pair_frame.p, e->pair_tile.p, e->tile_off.p,
+                                     e->tile_pair.p, t0, items, nt + 1, e->fused_total[ci], e->fused_counter.p,
+                                     e->fused_done.p, e->fused_err.p, e->Rsub.p);
+                             else
+                                 emc::launch_fused_expand_likelihood(
+                                     s, e->rots_slot.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p, P, s0,
+                                     Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p, e->slot_rot.p, e->lik_off.p,
+                                     e->lik_pix.p, e->lik_cnt.p, e->pair_frame.p, e->pair_tile.p, e->tile_off.p,
+                                     e->tile_pair.p, t0, items, nt + 1, e->fused_total[ci], e->fused_counter.p,
+                                     e->fused_done.p, e->fused_err.p, e->Rsub.p);
+                             emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, s0, S, e->u_slot.p,
+                                                  e->rbad_slot.p);
+                             if (nF > 0) {
+                                 if (g32)
+                                     emc::launch_likelihood_mapped(s, e->logw32.p, Cc, P, e->lik_off.p, e->lik_pix.p,
+                                                                   e->lik_cnt.p, e->blacklist.p, e->fidx.p, nF, e->RF.p,
+                                                                   s0);
+                                 else
+                                     emc::launch_likelihood_mapped(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p,
+                                                                   e->lik_cnt.p, e->blacklist.p, e->fidx.p, nF, e->RF.p,
+                                                                   s0);
+                             }
+                             return 0;
+                         })))
+                        return rc;
+                    continue;
+                }
+                if ((rc = timed(0, [&] {
+                         if (g32)
+                             emc::launch_expand(s, e->rots_slot.p, e->model32.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
+                                                P, s0, Cc, kPixPerGroup, e->logw32.p, e->upart.p, e->ubad.p, e->slot_rot.p);
+                         else
+                             emc::launch_expand(s, e->rots_slot.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
+                                                P, s0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p, e->slot_rot.p);
+                         emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, s0, S, e->u_slot.p, e->rbad_slot.p);
+                         return 0;
+                     })))
+                    return rc;
+                if ((rc = timed(1, [&] {
+                         const int t0 = s0 / emc::kRTile, t1 = (s0 + Cc) / emc::kRTile;
+                         if (g32) {
+                             emc::launch_pair_likelihood(s, e->logw32.p, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
+                                                         e->pair_frame.p, e->pair_tile.p, e->tile_off.p, e->tile_pair.p,
+                                                         e->num_pairs, t0, t1, e->Rsub.p);
+                             if (nF > 0)
+                                 emc::launch_likelihood_mapped(s, e->logw32.p, Cc, P, e->lik_off.p, e->lik_pix.p,
+                                                               e->lik_cnt.p, e->blacklist.p, e->fidx.p, nF, e->RF.p, s0);
+                         } else {
+                             emc::launch_pair_likelihood(s, e->logw.p, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
+                                                         e->pair_frame.p, e->pair_tile.p, e->tile_off.p, e->tile_pair.p,
+                                                         e->num_pairs, t0, t1, e->Rsub.p);
+                             if (nF > 0)
+                                 emc::launch_likelihood_mapped(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p,
+                                                               e->lik_cnt.p, e->blacklist.p, e->fidx.p, nF, e->RF.p, s0);
+                         }
+                         return 0;
+                     })))
+                    return rc;
+            }
+            if ((rc = timed(2, [&] {
+                     emc::launch_rescale(s, e->quats_slot.p, e->u_slot.p, S, e->mean_count, e->scalars.p);
+                     return 0;
+                 })))
+                return rc;
+            if ((rc = timed(3, [&] {
+                     emc::launch_pair_max(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, full, e->slot_rot.p,
+                                          e->slot_pack_built == 1 ? e->slot_cell.p : nullptr, e->relbits.p, e->rel_words,
+                                          e->logwr_slot.p, e->u_slot.p, e->rbad_slot.p, e->scale.p, use_scale, e->scalars.p,
+                                          e->blacklist.p, dense, N, e->max_exp.p, e->rmax.p);
+                     return 0;
+                 })))
+                return rc;
+            if ((rc = timed(4, [&] {
+                     emc::launch_pair_norm(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, full, e->slot_rot.p, e->beta.p,
+                                           e->blacklist.p, e->max_exp.p, e->rmax.p, N, e->p_norm.p);
+                     return 0;
+                 })))
+                return rc;
+            if ((rc = timed(5, [&] {
+                     emc::launch_pair_posterior(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, full, e->slot_rot.p,
+                                                e->logwr_slot.p, e->u_slot.p, e->beta.p, e->sum_fact.p, e->blacklist.p,
+                                                e->max_exp.p, e->rmax.p, e->p_norm.p, e->scalars.p, want_psum_d, N,
+                                                e->likelihood.p, e->info.p, e->psum_d.p);
+                     if (nF > 0) {
+                         emc::launch_frame_gather(s, e->scale.p, e->fidx.p, nF, e->scale_full.p);
+                         emc::launch_frame_gather(s, e->blacklist.p, e->fidx.p, nF, e->blacklist_full.p);
+                         emc::launch_psum_r(s, e->RF.p, S, nF, kFramesPerPsumPart, e->scale_full.p, opts->need_scaling,
+                                            e->blacklist_full.p, e->psum_full_part.p, e->psum_full.p);
+                     }
+                     emc::launch_pair_psum(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p, T, e->scale.p,
+                                           opts->need_scaling, nF > 0 ? e->psum_full.p : nullptr, e->psum_slot.p);
+                     emc::launch_slot_gather(s, e->psum_slot.p, e->rot_slot.p, Rn, e->psum_r.p);
+                     emc::launch_slot_gather(s, e->u_slot.p, e->rot_slot.p, Rn, e->u.p);
+                     if (g32) emc::launch_pair_to_f32(s, e->Rsub.p, e->num_pairs * emc::kRTile, e->Rsub32.p);
+                     return 0;
+                 })))
+                return rc;
+        return 0;
+    };
+    // all-orientation weights (they depend on the geometry only), computed once
+    auto cache_weights = [&]() -> int {
+        if (e->weight_all_valid) return 0;
+        int r2;
+        if ((r2 = e->weight_all.alloc(vol))) return r2;
+        emc::launch_fill(s, e->weight.p, vol, 0.);
+        for (const auto &bc : e->slot_bricks)
+            merge_chunk(e, bc, e->rots_slot.p, S, nullptr, e->psum_slot.p, true, false, true);
+        HIP_TRY(hipMemcpyAsync(e->weight_all.p, e->weight.p, vol * sizeof(double), hipMemcpyDeviceToDevice, s));
+        e->weight_all_valid = true;
+        return 0;
+    };
+    // M-step: tomogram and merge chunk by chunk. device_weights: the choice between the cached
+    // weights and merging them is made on the device (no read-back), as a graph needs
+    auto mstep_work = [&](bool device_weights) -> int {
+        bool cached_weight = false;
+        if (device_weights) {
+            emc::launch_count_unmerged(s, e->psum_r.p, Rn, e->d_count.p);
+            emc::launch_copy_if_zero(s, e->weight.p, e->weight_all.p, vol, e->d_count.p);
+            e->last_unmerged = -1;
+        } else {
+            // merge: bricks over the slot order (padding slots make no items)
+            if (!e->slot_bricks_valid) {
+                if ((rc = timed(8, [&] {
+                         return build_bricks(e, e->slot_bricks, e->rots_slot.p, S, C, S, e->slot_rot.p, 1);
+                     })))
+                    return rc;
+                e->slot_bricks_valid = true;
+            }
+            int unmerged = 0;
+            emc::launch_count_unmerged(s, e->psum_r.p, Rn, e->d_count.p);
+            HIP_TRY(hipMemcpyAsync(&unmerged, e->d_count.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            e->last_unmerged = unmerged;
+            cached_weight = unmerged == 0;
+            if (cached_weight && (rc = timed(8, [&] {
+                                      if (!e->weight_all_valid) return cache_weights();  // leaves them in weight too
+                                      HIP_TRY(hipMemcpyAsync(e->weight.p, e->weight_all.p, vol * sizeof(double),
+                                                             hipMemcpyDeviceToDevice, s));
+                                      return 0;
+                                  })))
+                return rc;
+        }
+        for (size_t k = 0; k < e->slot_bricks.size(); ++k) {
+            const auto &bc = e->slot_bricks[k];
+            const int t0 = bc.r0 / emc::kRTile, t1 = (bc.r0 + bc.Cc) / emc::kRTile;
+            if (nF > 0 && (rc = timed(6, [&] {
+                               if (g32) {
+                                   emc::launch_to_f32(s, e->RF.p, nF, bc.r0, bc.Cc, e->p32.p);
+                                   emc::launch_tomogram_parts_f32(s, e->p32.p, nF, bc.Cc, e->ftom_off2.p, nparts,
+                                                                  e->ftom_frame.p, e->ftom_cnt.p, P, e->tom_partial.p);
+                               } else {
+                                   emc::launch_tomogram_parts(s, e->RF.p, nF, bc.r0, bc.Cc, e->ftom_off2.p, nparts,
+                                                              e->ftom_frame.p, e->ftom_cnt.p, nullptr, S, P,
+                                                              e->tom_partial.p, e->logw.p, 0, 16, 0);
+                               }
+                               return 0;
+                           })))
+                return rc;
+            // partial frames' pairs added to the full frames' partial sums, divided by psum
+            if ((rc = timed(7, [&] {
+                     if (g32)
+                         emc::launch_pair_tomogram(s, e->Rsub32.p, e->tile_off.p, e->tile_pair.p, e->tile_frame.p,
+                                                   e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
+                                                   e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
+                                                   nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p);
+                     else
+                         emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->tile_frame.p,
+                                                   e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
+                                                   e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
+                                                   nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p);
+                     return 0;
+                 })))
+                return rc;
+            if ((rc = timed(8, [&] {
+                     if (!device_weights)
+                         return merge_chunk(e, bc, e->rots_slot.p, S, e->logw.p, e->psum_slot.p, false, true,
+                                            !cached_weight);
+                     // weights of the orientations that merge, only when some orientation does not
+                     // (gate: the count of those that do not); the model from every view
+                     merge_chunk(e, bc, e->rots_slot.p, S, nullptr, e->psum_slot.p, false, false, true, e->d_count.p);
+                     return merge_chunk(e, bc, e->rots_slot.p, S, e->logw.p, e->psum_slot.p, false, true, false);
+                 })))
+                return rc;
+        }
+        emc::launch_frame_stats(s, e->likelihood.p, e->info.p, e->blacklist.p, N, N - e->num_blacklist, e->scalars.p);
+        if (want_psum_d) emc::launch_scale(s, e->fcounts.p, e->psum_d.p, e->blacklist.p, N, e->scale.p);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    };
+    auto finish = [&]() -> int {
+        double host_scalars[3];
+        HIP_TRY(hipMemcpyAsync(host_scalars, e->scalars.p, sizeof host_scalars, hipMemcpyDeviceToHost, s));
+        int gave_up = 0;
+        if (fused) HIP_TRY(hipMemcpyAsync(&gave_up, e->fused_err.p, sizeof gave_up, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (gave_up) {
+            emc::launch_fill_int(s, e->fused_err.p, 1, 0);
+            return fail(EMC_E_HIP, "emc_maximize: a likelihood batch of the persistent kernel gave up waiting for "
+                                   "its tile's expand; the results of this iteration are not valid");
+        }
+        e->have_results = true;
+        e->subset_ran = true;
+        if (stats) {
+            stats->rescale = host_scalars[0];
+            stats->avg_likelihood = host_scalars[1];
+            stats->mutual_info = host_scalars[2];
+            for (int i = 0; i < EMC_NUM_STAGES; ++i) stats->stage_ms[i] = ms[i];
+            stats->num_chunks = (int)e->slot_bricks.size();
+        }
+        return 0;
+    };
+
+    if (graph) {
+        // the set-up that depends on the geometry only stays outside the graph
+        if (!e->slot_bricks_valid) {
+            if ((rc = build_bricks(e, e->slot_bricks, e->rots_slot.p, S, C, S, e->slot_rot.p, 1))) return rc;
+            e->slot_bricks_valid = true;
+        }
+        if ((rc = cache_weights())) return rc;
+        // the graph holds the kernels' arguments: capture again when a buffer moved or a size,
+        // list or option changed
+        const std::vector<long> key = {(long)g_alloc_gen.load(), (long)e->layout_gen, use_scale, want_psum_d,
+                                       opts->need_scaling, C, (long)e->slot_bricks.size(), (long)(intptr_t)e};
+        if (!e->graph_exec || key != e->graph_key) {
+            if (e->graph_exec) {
+                (void)hipGraphExecDestroy(e->graph_exec);
+                e->graph_exec = nullptr;
+            }
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            capturing = true;
+            emc::launch_fill(s, e->model2.p, vol, 0.);
+            emc::launch_fill(s, e->weight.p, vol, 0.);
+            rc = estep_work();
+            if (!rc) rc = mstep_work(true);
+            capturing = false;
+            hipGraph_t g = nullptr;
+            const hipError_t end = hipStreamEndCapture(s, &g);
+            if (rc || end != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc ? rc : fail(EMC_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(end));
+            }
+            const hipError_t inst = hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (inst != hipSuccess) {
+                e->graph_exec = nullptr;
+                return fail(EMC_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(inst));
+            }
+            e->graph_key = key;
+        }
+        HIP_TRY(hipGraphLaunch(e->graph_exec, s));
+        return finish();
+    }
+
     emc::launch_fill(s, e->model2.p, vol, 0.);
     emc::launch_fill(s, e->weight.p, vol, 0.);
-    const uint8_t *dense = e->any_dense ? e->dense.p : nullptr;
-    for (int s0 = 0; s0 < S; s0 += C) {
-        const int Cc = std::min(C, S - s0);
-        if ((rc = timed(0, [&] {
-                 if (g32)
-                     emc::launch_expand(s, e->rots_slot.p, e->model32.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
-                                        P, s0, Cc, kPixPerGroup, e->logw32.p, e->upart.p, e->ubad.p, e->slot_rot.p);
-                 else
-                     emc::launch_expand(s, e->rots_slot.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
-                                        P, s0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p, e->slot_rot.p);
-                 emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, s0, S, e->u_slot.p, e->rbad_slot.p);
-                 return 0;
-             })))
-            return rc;
-        if ((rc = timed(1, [&] {
-                 const int t0 = s0 / emc::kRTile, t1 = (s0 + Cc) / emc::kRTile;
-                 if (g32) {
-                     emc::launch_pair_likelihood(s, e->logw32.p, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
-                                                 e->pair_frame.p, e->pair_tile.p, e->tile_off.p, e->tile_pair.p,
-                                                 e->num_pairs, t0, t1, e->Rsub.p);
-                     if (nF > 0)
-                         emc::launch_likelihood_mapped(s, e->logw32.p, Cc, P, e->lik_off.p, e->lik_pix.p,
-                                                       e->lik_cnt.p, e->blacklist.p, e->fidx.p, nF, e->RF.p, s0);
-                 } else {
-                     emc::launch_pair_likelihood(s, e->logw.p, P, e->lik_off.p, e->lik_pix.p, e->lik_cnt.p,
-                                                 e->pair_frame.p, e->pair_tile.p, e->tile_off.p, e->tile_pair.p,
-                                                 e->num_pairs, t0, t1, e->Rsub.p);
-                     if (nF > 0)
-                         emc::launch_likelihood_mapped(s, e->logw.p, Cc, P, e->lik_off.p, e->lik_pix.p,
-                                                       e->lik_cnt.p, e->blacklist.p, e->fidx.p, nF, e->RF.p, s0);
-                 }
-                 return 0;
-             })))
-            return rc;
+    // mstep_only: the posterior, psum and per-frame sums of the estep_only iteration just run
+    // on this state are still in place; only the M-step below remains
+    const bool resume = e->mstep_only != 0;
+    if (resume && (e->estep_only || e->estep_gen != e->gen))
+        return fail(EMC_E_STATE, "emc_maximize: mstep_only needs an estep_only iteration of the current state");
+    if (!resume && (rc = estep_work())) return rc;
+    if (e->estep_only) {
+        // expectation step only: per-frame statistics of the posterior over the subsets;
+        // model2, inter_weight and the scale factors are left as they are
+        emc::launch_frame_stats(s, e->likelihood.p, e->info.p, e->blacklist.p, N, N - e->num_blacklist,
+                                e->scalars.p);
+        HIP_TRY(hipGetLastError());
+        double host_scalars[3];
+        HIP_TRY(hipMemcpyAsync(host_scalars, e->scalars.p, sizeof host_scalars, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        e->have_results = true;
+        e->subset_ran = true;
+        e->estep_gen = e->gen;
+        if (stats) {
+            stats->rescale = host_scalars[0];
+            stats->avg_likelihood = host_scalars[1];
+            stats->mutual_info = host_scalars[2];
+            for (int i = 0; i < EMC_NUM_STAGES; ++i) stats->stage_ms[i] = ms[i];
+            stats->num_chunks = (int)((S + C - 1) / C);
+        }
+        return 0;
     }
-    if ((rc = timed(2, [&] {
-             emc::launch_rescale(s, e->quats_slot.p, e->u_slot.p, S, e->mean_count, e->scalars.p);
-             return 0;
-         })))
-        return rc;
-    if ((rc = timed(3, [&] {
-             emc::launch_pair_max(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, full, e->slot_rot.p,
-                                  e->slot_pack_built == 1 ? e->slot_cell.p : nullptr, e->relbits.p, e->rel_words,
-                                  e->logwr_slot.p, e->u_slot.p, e->rbad_slot.p, e->scale.p, use_scale, e->scalars.p,
-                                  e->blacklist.p, dense, N, e->max_exp.p, e->rmax.p);
-             return 0;
-         })))
-        return rc;
-    if ((rc = timed(4, [&] {
-             emc::launch_pair_norm(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, full, e->slot_rot.p, e->beta.p,
-                                   e->blacklist.p, e->max_exp.p, e->rmax.p, N, e->p_norm.p);
-             return 0;
-         })))
-        return rc;
-    if ((rc = timed(5, [&] {
-             emc::launch_pair_posterior(s, e->Rsub.p, e->pair_off.p, e->pair_tile.p, full, e->slot_rot.p,
-                                        e->logwr_slot.p, e->u_slot.p, e->beta.p, e->sum_fact.p, e->blacklist.p,
-                                        e->max_exp.p, e->rmax.p, e->p_norm.p, e->scalars.p, want_psum_d, N,
-                                        e->likelihood.p, e->info.p, e->psum_d.p);
-             if (nF > 0) {
-                 emc::launch_frame_gather(s, e->scale.p, e->fidx.p, nF, e->scale_full.p);
-                 emc::launch_frame_gather(s, e->blacklist.p, e->fidx.p, nF, e->blacklist_full.p);
-                 emc::launch_psum_r(s, e->RF.p, S, nF, kFramesPerPsumPart, e->scale_full.p, opts->need_scaling,
-                                    e->blacklist_full.p, e->psum_full_part.p, e->psum_full.p);
-             }
-             emc::launch_pair_psum(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->pair_frame.p, T, e->scale.p,
-                                   opts->need_scaling, nF > 0 ? e->psum_full.p : nullptr, e->psum_slot.p);
-             emc::launch_slot_gather(s, e->psum_slot.p, e->rot_slot.p, Rn, e->psum_r.p);
-             emc::launch_slot_gather(s, e->u_slot.p, e->rot_slot.p, Rn, e->u.p);
-             if (g32) emc::launch_pair_to_f32(s, e->Rsub.p, e->num_pairs * emc::kRTile, e->Rsub32.p);
-             return 0;
-         })))
-        return rc;
-
-    // merge: bricks over the slot order (padding slots make no items)
-    if (!e->slot_bricks_valid) {
-        if ((rc = timed(8, [&] {
-                 return build_bricks(e, e->slot_bricks, e->rots_slot.p, S, C, S, e->slot_rot.p, 1);
-             })))
-            return rc;
-        e->slot_bricks_valid = true;
-    }
-    int unmerged = 0;
-    emc::launch_count_unmerged(s, e->psum_r.p, Rn, e->d_count.p);
-    HIP_TRY(hipMemcpyAsync(&unmerged, e->d_count.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    e->last_unmerged = unmerged;
-    const bool cached_weight = unmerged == 0;
-    if (cached_weight && (rc = timed(8, [&] {
-                              if (!e->weight_all_valid) {
-                                  int r2;
-                                  if ((r2 = e->weight_all.alloc(vol))) return r2;
-                                  for (const auto &bc : e->slot_bricks)
-                                      merge_chunk(e, bc, e->rots_slot.p, S, nullptr, e->psum_slot.p, true, false, true);
-                                  HIP_TRY(hipMemcpyAsync(e->weight_all.p, e->weight.p, vol * sizeof(double),
-                                                         hipMemcpyDeviceToDevice, s));
-                                  e->weight_all_valid = true;
-                              } else {
-                                  HIP_TRY(hipMemcpyAsync(e->weight.p, e->weight_all.p, vol * sizeof(double),
-                                                         hipMemcpyDeviceToDevice, s));
-                              }
-                              return 0;
-                          })))
-        return rc;
-    for (size_t k = 0; k < e->slot_bricks.size(); ++k) {
-        const auto &bc = e->slot_bricks[k];
-        const int t0 = bc.r0 / emc::kRTile, t1 = (bc.r0 + bc.Cc) / emc::kRTile;
-        if (nF > 0 && (rc = timed(6, [&] {
-                           if (g32) {
-                               emc::launch_to_f32(s, e->RF.p, nF, bc.r0, bc.Cc, e->p32.p);
-                               emc::launch_tomogram_parts_f32(s, e->p32.p, nF, bc.Cc, e->ftom_off2.p, nparts,
-                                                              e->ftom_frame.p, e->ftom_cnt.p, P, e->tom_partial.p);
-                           } else {
-                               emc::launch_tomogram_parts(s, e->RF.p, nF, bc.r0, bc.Cc, e->ftom_off2.p, nparts,
-                                                          e->ftom_frame.p, e->ftom_cnt.p, nullptr, S, P,
-                                                          e->tom_partial.p, e->logw.p, 0, 16, 0);
-                           }
-                           return 0;
-                       })))
-            return rc;
-        // partial frames' pairs added to the full frames' partial sums, divided by psum
-        if ((rc = timed(7, [&] {
-                 if (g32)
-                     emc::launch_pair_tomogram(s, e->Rsub32.p, e->tile_off.p, e->tile_pair.p, e->tile_frame.p,
-                                               e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
-                                               e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
-                                               nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p);
-                 else
-                     emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->tile_frame.p,
-                                               e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
-                                               e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
-                                               nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p);
-                 return 0;
-             })))
-            return rc;
-        if ((rc = timed(8, [&] {
-                 return merge_chunk(e, bc, e->rots_slot.p, S, e->logw.p, e->psum_slot.p, false, true,
-                                    !cached_weight);
-             })))
-            return rc;
-    }
-    emc::launch_frame_stats(s, e->likelihood.p, e->info.p, e->blacklist.p, N, N - e->num_blacklist, e->scalars.p);
-    if (want_psum_d) emc::launch_scale(s, e->fcounts.p, e->psum_d.p, e->blacklist.p, N, e->scale.p);
-    HIP_TRY(hipGetLastError());
-    double host_scalars[3];
-    HIP_TRY(hipMemcpyAsync(host_scalars, e->scalars.p, sizeof host_scalars, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    e->have_results = true;
-    e->subset_ran = true;
-    if (stats) {
-        stats->rescale = host_scalars[0];
-        stats->avg_likelihood = host_scalars[1];
-        stats->mutual_info = host_scalars[2];
-        for (int i = 0; i < EMC_NUM_STAGES; ++i) stats->stage_ms[i] = ms[i];
-        stats->num_chunks = (int)e->slot_bricks.size();
-    }
-    return 0;
+    e->estep_gen = ~0ul;  // the M-step below consumes the expectation step
+    if ((rc = mstep_work(false))) return rc;
+    return finish();
 }
 
 // Kernel-variant options (emc_set_option / EMC_<NAME>): the field, its allowed range, and
@@ -1358,6 +1594,9 @@ const EngineOption kOptions[] = {
     {"sparse_rows", &emc_engine::sparse_rows, 0, 1 << 30, false},
     {"full_part_frames", &emc_engine::full_part_frames, 1, 1 << 30, true},
     {"estep_only", &emc_engine::estep_only, 0, 1, false},
+    {"mstep_only", &emc_engine::mstep_only, 0, 1, false},
+    {"graph", &emc_engine::graph_mode, 0, 1, false},
+    {"fused", &emc_engine::fused, 0, 1, false},
 };
 
 int set_option(emc_engine *e, const EngineOption &o, int value) {
@@ -1411,6 +1650,8 @@ int emc_create(int device, emc_engine **out) {
 
 int emc_set_option(emc_engine *e, const char *name, int value) {
     if (!e || !name) return fail(EMC_E_ARG, "emc_set_option: null argument");
+    // the two pass switches leave the iteration's state as it is; every other option may not
+    if (strcmp(name, "estep_only") && strcmp(name, "mstep_only")) ++e->gen, ++e->layout_gen;
     for (const EngineOption &o : kOptions)
         if (!strcmp(o.name, name)) return set_option(e, o, value);
     return fail(EMC_E_ARG, "emc_set_option: unknown option '%s'", name);
@@ -1430,6 +1671,7 @@ void emc_destroy(emc_engine *e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
     (void)hipStreamSynchronize(e->stream);
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     for (DevBuf<double> *b : {&e->qvals, &e->corr, &e->quats, &e->rots, &e->logwr, &e->model1,
                               &e->model2, &e->weight, &e->lik_cnt, &e->tom_cnt, &e->scale,
                               &e->beta, &e->sum_fact, &e->logw, &e->upart, &e->u, &e->R,
@@ -1474,6 +1716,7 @@ int emc_synchronize(emc_engine *e) {
 
 int emc_set_detector(emc_engine *e, int num_pix, const double *qvals, const double *corr,
                      const uint8_t *raw_mask) {
+    if (e) ++e->gen, ++e->layout_gen;
     if (!e || num_pix <= 0 || !qvals || !corr || !raw_mask) return fail(EMC_E_ARG, "emc_set_detector: bad argument");
     HIP_TRY(hipSetDevice(e->device));
     std::vector<int> perm = morton_order(num_pix, qvals);
@@ -1537,6 +1780,7 @@ static int upload_quats(emc_engine *e) {
 }
 
 int emc_set_modes(emc_engine *e, int num_modes) {
+    if (e) ++e->gen, ++e->layout_gen;
     if (e) e->model32_valid = false;  // model1 changes below
     if (!e || num_modes < 1 || num_modes > EMC_MAX_MODES)
         return fail(EMC_E_ARG, "emc_set_modes: num_modes must be 1..%d", EMC_MAX_MODES);
@@ -1555,6 +1799,7 @@ int emc_set_modes(emc_engine *e, int num_modes) {
 }
 
 int emc_set_quats(emc_engine *e, int num_rot, const double *quats) {
+    if (e) ++e->gen, ++e->layout_gen;
     if (!e || num_rot <= 0 || !quats) return fail(EMC_E_ARG, "emc_set_quats: bad argument");
     HIP_TRY(hipSetDevice(e->device));
     e->h_quats.assign(quats, quats + 5 * (size_t)num_rot);
@@ -1571,6 +1816,7 @@ int emc_set_quats(emc_engine *e, int num_rot, const double *quats) {
 }
 
 int emc_set_model(emc_engine *e, long size, const double *model1) {
+    if (e) ++e->gen;  // the same buffer unless the size changes (then it is allocated anew)
     if (e) e->model32_valid = false;  // model1 changes below
     if (!e || size < 3 || !model1) return fail(EMC_E_ARG, "emc_set_model: bad argument");
     HIP_TRY(hipSetDevice(e->device));
@@ -1589,6 +1835,7 @@ int emc_set_model(emc_engine *e, long size, const double *model1) {
 }
 
 int emc_set_background(emc_engine *e, const double *background, const double *bgscale) {
+    if (e) ++e->gen, ++e->layout_gen;
     if (!e) return fail(EMC_E_ARG, "null engine");
     if (!background) {
         e->with_bg = false;
@@ -1630,6 +1877,7 @@ int emc_set_background(emc_engine *e, const double *background, const double *bg
 int emc_set_frames_sparse(emc_engine *e, int num_data, const int *ones, const int *multi,
                           const long *ones_accum, const long *multi_accum, const int *place_ones,
                           const int *place_multi, const int *count_multi) {
+    if (e) ++e->gen, ++e->layout_gen;
     if (!e || num_data <= 0 || !ones || !multi || !ones_accum || !multi_accum)
         return fail(EMC_E_ARG, "emc_set_frames_sparse: bad argument");
     long ones_total = ones_accum[num_data - 1] + ones[num_data - 1];
@@ -1658,6 +1906,7 @@ int emc_set_frames_sparse(emc_engine *e, int num_data, const int *ones, const in
 }
 
 int emc_set_frames_dense(emc_engine *e, int num_data, int num_pix, const int *frames) {
+    if (e) ++e->gen, ++e->layout_gen;
     if (!e || num_data <= 0 || num_pix <= 0 || !frames)
         return fail(EMC_E_ARG, "emc_set_frames_dense: bad argument");
     if (e->num_pix && num_pix != e->num_pix)
@@ -1687,6 +1936,7 @@ int emc_set_frames_dense(emc_engine *e, int num_data, int num_pix, const int *fr
 
 int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const long *rel_off,
                    const int *rel_quat) {
+    if (e) ++e->gen, ++e->layout_gen;
     if (!e) return fail(EMC_E_ARG, "null engine");
     if (!quat_mapping) {
         e->refine = false;
@@ -1765,7 +2015,44 @@ int emc_cell_posterior(emc_engine *e, int num_coarse, const int *quat_mapping, f
     return 0;
 }
 
+int emc_cell_topk(emc_engine *e, int num_coarse, const int *quat_mapping, int k, int *idx, float *mass) {
+    if (!e || !quat_mapping || !idx || !mass) return fail(EMC_E_ARG, "emc_cell_topk: null argument");
+    if (num_coarse <= 0 || num_coarse > 6144)
+        return fail(EMC_E_ARG, "emc_cell_topk: num_coarse %d (1 to 6144)", num_coarse);
+    if (k < 1 || k > 64 || k > num_coarse) return fail(EMC_E_ARG, "emc_cell_topk: k %d (1 to min(64, num_coarse))", k);
+    if (!e->have_results) return fail(EMC_E_STATE, "emc_cell_topk: no iteration has run");
+    HIP_TRY(hipSetDevice(e->device));
+    const int N = e->num_data, Rn = e->num_rot;
+    for (int r = 0; r < Rn; ++r)
+        if (quat_mapping[r] < 0 || quat_mapping[r] >= num_coarse)
+            return fail(EMC_E_ARG, "emc_cell_topk: quat_mapping[%d] = %d out of range", r, quat_mapping[r]);
+    const int batch = std::max(1, std::min(N, (int)((size_t)64 << 20) / (num_coarse * (int)sizeof(float))));
+    DevBuf<int> qm, didx;
+    DevBuf<float> dout, dmass;
+    int rc;
+    if ((rc = qm.upload(quat_mapping, Rn, e->stream))) return rc;
+    if ((rc = dout.alloc((size_t)batch * num_coarse))) return rc;
+    if ((rc = didx.alloc((size_t)batch * k))) return rc;
+    if ((rc = dmass.alloc((size_t)batch))) return rc;
+    const bool plain = !e->subset_ran;
+    const emc::FullRows full{e->num_full > 0 ? e->RF.p : nullptr, e->num_full > 0 ? e->ffull.p : nullptr,
+                             e->num_full, e->num_tiles};
+    for (int d0 = 0; d0 < N; d0 += batch) {
+        const int cnt = std::min(batch, N - d0);
+        emc::launch_cell_posterior(e->stream, plain ? e->R.p : e->Rsub.p, plain, N, Rn, e->pair_off.p,
+                                   e->pair_tile.p, full, e->slot_rot.p, qm.p, num_coarse, d0, cnt, dout.p);
+        emc::launch_prob_topk(e->stream, dout.p, num_coarse, cnt, num_coarse, k, didx.p, dmass.p);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(idx + (size_t)d0 * k, didx.p, (size_t)cnt * k * sizeof(int), hipMemcpyDeviceToHost,
+                               e->stream));
+        HIP_TRY(hipMemcpyAsync(mass + d0, dmass.p, (size_t)cnt * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    return 0;
+}
+
 int emc_set_precision(emc_engine *e, int mode) {
+    if (e) ++e->gen, ++e->layout_gen;
     if (!e) return fail(EMC_E_ARG, "null engine");
     if (mode < 0 || mode > 5) return fail(EMC_E_ARG, "emc_set_precision: mode %d (0 to 5)", mode);
     if (mode != e->precision) {
@@ -1778,6 +2065,7 @@ int emc_set_precision(emc_engine *e, int mode) {
 int emc_set_frame_params(emc_engine *e, const double *scale, const double *beta,
                          const uint8_t *blacklist, const int *fcounts, const double *sum_fact,
                          double mean_count, int num_blacklist) {
+    if (e) ++e->gen, ++e->layout_gen;
     if (!e || !e->num_data) return fail(EMC_E_STATE, "emc_set_frame_params: frames not set");
     if (!scale || !beta || !blacklist || !fcounts || !sum_fact)
         return fail(EMC_E_ARG, "emc_set_frame_params: bad argument");
@@ -1789,6 +2077,7 @@ int emc_set_frame_params(emc_engine *e, const double *scale, const double *beta,
     if ((rc = e->scale.upload(scale, N, e->stream))) return rc;
     if ((rc = e->beta.upload(beta, N, e->stream))) return rc;
     if ((rc = e->blacklist.upload(blacklist, N, e->stream))) return rc;
+    e->h_blacklist.assign(blacklist, blacklist + N);
     if ((rc = e->fcounts.upload(fcounts, N, e->stream))) return rc;
     if ((rc = e->sum_fact.upload(sum_fact, N, e->stream))) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1799,6 +2088,7 @@ int emc_set_frame_params(emc_engine *e, const double *scale, const double *beta,
 }
 
 int emc_set_beta(emc_engine *e, const double *beta) {
+    if (e) ++e->gen;
     if (!e || !beta || !e->beta.p) return fail(EMC_E_STATE, "emc_set_beta: frame parameters not set");
     HIP_TRY(hipSetDevice(e->device));
     HIP_TRY(hipMemcpyAsync(e->beta.p, beta, e->beta.n * sizeof(double), hipMemcpyHostToDevice, e->stream));
@@ -1807,6 +2097,7 @@ int emc_set_beta(emc_engine *e, const double *beta) {
 }
 
 int emc_set_scale(emc_engine *e, const double *scale) {
+    if (e) ++e->gen;
     if (!e || !scale || !e->scale.p) return fail(EMC_E_STATE, "emc_set_scale: frame parameters not set");
     HIP_TRY(hipSetDevice(e->device));
     HIP_TRY(hipMemcpyAsync(e->scale.p, scale, e->scale.n * sizeof(double), hipMemcpyHostToDevice, e->stream));
@@ -1814,7 +2105,51 @@ int emc_set_scale(emc_engine *e, const double *scale) {
     return 0;
 }
 
+int emc_judge_outside(emc_engine *e, emc_engine *coarse, double prob_floor, double *outside) {
+    if (!e || !coarse || !outside) return fail(EMC_E_ARG, "emc_judge_outside: null argument");
+    if (!(prob_floor >= 0.) || prob_floor >= 1.) return fail(EMC_E_ARG, "emc_judge_outside: prob_floor in [0, 1)");
+    if (!e->refine || !e->subset_ran || !e->have_results)
+        return fail(EMC_E_STATE, "emc_judge_outside: needs an iteration on refinement subsets");
+    if (!coarse->have_results || coarse->subset_ran || coarse->refine)
+        return fail(EMC_E_STATE, "emc_judge_outside: the coarse engine needs a plain iteration");
+    const int N = e->num_data, Nc = e->slot_num_coarse;
+    if (coarse->num_data != N || coarse->num_rot != Nc || coarse->num_modes != 1 || coarse->device != e->device)
+        return fail(EMC_E_ARG, "emc_judge_outside: the coarse engine must hold the same frames and one orientation "
+                               "per coarse cell (%d) on the same device", Nc);
+    if ((int)e->h_slot_qmap.size() != e->num_rot || (int)coarse->h_quats.size() != 5 * Nc)
+        return fail(EMC_E_STATE, "emc_judge_outside: orientation tables not set");
+    HIP_TRY(hipSetDevice(e->device));
+    // per cell: log of its fine orientation count, log of its mean fine weight over the centre's weight
+    std::vector<double> cnt(Nc, 0.), wsum(Nc, 0.), logn(Nc), lwr(Nc);
+    for (int r = 0; r < e->num_rot; ++r) {
+        cnt[e->h_slot_qmap[r]] += 1.;
+        wsum[e->h_slot_qmap[r]] += e->h_quats[5 * (size_t)r + 4];
+    }
+    for (int c = 0; c < Nc; ++c) {
+        const double wc = coarse->h_quats[5 * (size_t)c + 4];
+        logn[c] = cnt[c] > 0. ? std::log(cnt[c]) : -INFINITY;
+        lwr[c] = cnt[c] > 0. && wc > 0. ? std::log(wsum[c] / cnt[c] / wc) : 0.;
+    }
+    std::vector<uint8_t> skip(N);
+    for (int d = 0; d < N; ++d) skip[d] = (uint8_t)(e->h_ffull[d] >= 0 || e->h_blacklist[d] != 0);
+    int rc;
+    if ((rc = e->judge_logn.upload(logn.data(), Nc, e->stream))) return rc;
+    if ((rc = e->judge_lwr.upload(lwr.data(), Nc, e->stream))) return rc;
+    if ((rc = e->judge_skip.upload(skip.data(), N, e->stream))) return rc;
+    if ((rc = e->judge_out.alloc(N))) return rc;
+    // the coarse engine's posterior, maximum and normaliser are complete on its own stream
+    HIP_TRY(hipStreamSynchronize(coarse->stream));
+    emc::launch_judge_outside(e->stream, coarse->R.p, N, Nc, coarse->max_exp.p, coarse->p_norm.p, e->max_exp.p,
+                              e->p_norm.p, e->beta.p, e->judge_skip.p, e->relbits.p, e->rel_words, e->judge_logn.p,
+                              e->judge_lwr.p, prob_floor, e->judge_out.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(outside, e->judge_out.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
 int emc_clear_merge(emc_engine *e) {
+    if (e) ++e->gen;
     if (!e || !e->size) return fail(EMC_E_STATE, "emc_clear_merge: model not set");
     HIP_TRY(hipSetDevice(e->device));
     const size_t vol = (size_t)e->size * e->size * e->size * e->num_modes;
@@ -1979,8 +2314,10 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
     if (bgp && (e->precision != 0 || e->num_modes > 1 || e->refine))
         return fail(EMC_E_STATE, "emc_maximize: the background-aware path runs in FP64 with one mode, no refinement");
     if (g32) e->tomo_tile_mode = 2;  // the FP32 posterior feeds the frame-partitioned tomogram
-    if (e->estep_only && (bgp || (e->refine && e->subset_mode && e->subset_ready && !mp)))
-        return fail(EMC_E_STATE, "emc_maximize: estep_only runs on the plain and mixed paths only");
+    if (e->estep_only && bgp)
+        return fail(EMC_E_STATE, "emc_maximize: estep_only does not run on the background-aware path");
+    if (e->mstep_only && !(e->refine && e->subset_mode && e->subset_ready && !mp))
+        return fail(EMC_E_STATE, "emc_maximize: mstep_only runs on the refinement-subset path only");
     if (e->refine && e->subset_mode && e->subset_ready && !mp) return maximize_subset(e, opts, stats);
     e->subset_ran = false;
     if (mp && e->num_modes > 1)
@@ -2269,6 +2606,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
 }
 
 int emc_update_model(emc_engine *e, double alpha, double *rms_change) {
+    if (e) ++e->gen;
     if (e) e->model32_valid = false;  // model1 changes below
     if (!e || !e->size) return fail(EMC_E_STATE, "emc_update_model: model not set");
     HIP_TRY(hipSetDevice(e->device));
@@ -2287,6 +2625,7 @@ int emc_update_model(emc_engine *e, double alpha, double *rms_change) {
 }
 
 int emc_normalize_scale(emc_engine *e, double *mean_scale) {
+    if (e) ++e->gen;
     if (e) e->model32_valid = false;  // model1 changes below
     if (!e || !e->have_params || !e->size) return fail(EMC_E_STATE, "emc_normalize_scale: state not set");
     HIP_TRY(hipSetDevice(e->device));

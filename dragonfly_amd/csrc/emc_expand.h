@@ -1,0 +1,110 @@
+// Expand: log W of one 64-orientation tile over one pixel group, plus the tile's partial u_r
+// and "log W not finite on a mask < 1 pixel" flags. Shared by expand_kernel (one block per
+// (tile, group)) and the persistent expand + likelihood kernel of the refinement subsets.
+#pragma once
+
+#include <cfloat>
+
+#include "emc_common.h"
+#include "emc_geometry.h"
+
+namespace emc {
+
+// Lanes along pixels (neighbouring pixels hit neighbouring voxels), transpose through LDS,
+// store with lanes along orientations (coalesced rows of logW). u_r (max_prob.c:36-38) sums W
+// over mask < 1.
+__device__ __forceinline__ double slice_value_model(const double *__restrict__ model, long size, const double *rot,
+                                                    double q0, double q1, double q2, double corr, double center) {
+    return slice_value_flat(model, size, rot, q0, q1, q2, corr, center);
+}
+// the FP32 model is the brick copy (copy_f32_brick_kernel)
+__device__ __forceinline__ double slice_value_model(const float *__restrict__ model, long size, const double *rot,
+                                                    double q0, double q1, double q2, double corr, double center) {
+    return slice_value_f32b(model, size, rot, q0, q1, q2, corr, center);
+}
+// elements of one mode's volume: size^3 for the FP64 model, the padded brick grid for the FP32 copy
+__device__ __forceinline__ size_t mode_stride(const double *, long size) { return (size_t)(size * size * size); }
+__device__ __forceinline__ size_t mode_stride(const float *, long size) {
+    const size_t nb = (size_t)model_bricks(size);
+    return nb * nb * nb * 64;
+}
+// log W as stored: FP64 log for an FP64 model; FP32 log of the FP32 interpolation otherwise
+// (values below FLT_MIN, i.e. DBL_MIN outside the grid, keep the FP64 log)
+template <typename T, typename MT>
+__device__ __forceinline__ T log_stored(double W) {
+    if (sizeof(MT) == sizeof(float)) return (T)(W >= (double)FLT_MIN ? logf((float)W) : (float)log(W));
+    return (T)log(W);
+}
+
+// LDS of one expand block
+struct ExpandShared {
+    double rot[kRTile * 9];
+    double tile[64][kRTile + 1];
+    double u[kWavesPerBlock][kRTile];
+    int bad[kWavesPerBlock][kRTile];
+    int pad[kRTile];
+};
+
+// Tile starting at chunk column c0, pixel group g; called by all kBlock threads of a block.
+// MT: element type of the model read (double, or float for the FP32-gather path's copy)
+template <typename T, int kW, typename MT>
+__device__ __forceinline__ void expand_block(
+    ExpandShared &sh, int c0, int g, const double *__restrict__ rots, const MT *__restrict__ model, long size,
+    const double *__restrict__ qvals, const double *__restrict__ corr, const uint8_t *__restrict__ mask, int num_pix,
+    int r0, int C, int pix_per_group, T *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad,
+    const int *__restrict__ slot_rot, int modes, int raw) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kRTile * 9; i += kBlock) sh.rot[i] = rots[(size_t)(r0 + c0) * 9 + i];
+    // padding slots of the subset order (slot_rot < 0) get W = 1 without interpolation
+    if (tid < kRTile) sh.pad[tid] = slot_rot != nullptr && slot_rot[r0 + c0 + tid] < 0;
+    __syncthreads();
+    const double center = (double)(size / 2);
+    const int lane = tid & 63, wave = tid >> 6;
+    const int t_begin = g * pix_per_group;
+    const int t_end = min(num_pix, t_begin + pix_per_group);
+    double uacc = 0.;
+    int bad = 0;  // some mask < 1 pixel has W <= 0 (log W = -inf or NaN)
+    for (int tb = t_begin; tb < t_end; tb += 64) {
+        const int t = tb + lane;
+        if (t < t_end) {
+            const double q0 = qvals[3 * (size_t)t], q1 = qvals[3 * (size_t)t + 1],
+                         q2 = qvals[3 * (size_t)t + 2], cr = corr[t];
+#pragma unroll 4
+            for (int k = 0; k < kRTile / kWavesPerBlock; ++k) {
+                const int rr = wave + kWavesPerBlock * k;
+                // combined orientation index r = rotation * modes + mode (max_prob.c:80-84)
+                const MT *mdl = modes > 1 ? model + (size_t)((r0 + c0 + rr) % modes) * mode_stride(model, size) : model;
+                const double v = slice_value_model(mdl, size, &sh.rot[rr * 9], q0, q1, q2, cr, center);
+                sh.tile[lane][rr] = sh.pad[rr] ? 1. : v;
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < 64 / kWavesPerBlock; ++k) {
+            const int p = wave + kWavesPerBlock * k;
+            const int tt = tb + p;
+            if (tt < t_end) {
+                const double W = sh.tile[p][lane];
+                if (mask[tt] < 1) {
+                    uacc += W;
+                    bad |= !(W > 0.);
+                }
+                // tile-major with kW-wide tiles: (c / kW) * num_pix * kW + t * kW + c % kW
+                const int c = c0 + lane;
+                // raw: W itself (the background-aware likelihood takes log(phi W + b B))
+                logw[(size_t)(c / kW) * ((size_t)num_pix * kW) + (size_t)tt * kW + (c % kW)] =
+                    raw ? (T)W : log_stored<T, MT>(W);
+            }
+        }
+        __syncthreads();
+    }
+    sh.u[wave][lane] = uacc;
+    sh.bad[wave][lane] = bad;
+    __syncthreads();
+    if (tid < kRTile) {
+        upart[(size_t)g * C + c0 + tid] = sh.u[0][tid] + sh.u[1][tid] + sh.u[2][tid] + sh.u[3][tid];
+        ubad[(size_t)g * C + c0 + tid] = sh.bad[0][tid] | sh.bad[1][tid] | sh.bad[2][tid] | sh.bad[3][tid];
+    }
+    __syncthreads();  // the LDS is free again for the block's next piece of work
+}
+
+}  // namespace emc

@@ -59,6 +59,33 @@ void launch_frame_norm(hipStream_t s, const double *R, int num_rot, int num_data
 // mass[d] = sum of the posterior P_dr (tile-major R) over r in idx[off[d] .. off[d + 1])
 void launch_subset_mass(hipStream_t s, const double *R, int num_data, const long *off, const int *idx,
                         double *mass);
+// Estimated posterior mass outside each frame's cells (coarse full search Pc [num_coarse/64][N][64]
+// against the fine maximum Mf / normaliser pnf inside the cells); 0 where skip[d].
+// Top-k columns (descending, ties to the lower index; k <= 64, n <= 8192) of every row of a
+// non-negative FP32 matrix, with the sum of the selected entries (emc_nn.hip).
+void launch_prob_topk(hipStream_t s, const float *Z, size_t ldz, int rows, int n, int k, int *idx, float *mass);
+// Persistent expand + pair likelihood of one orientation chunk (emc_subset.hip): item_off
+// (nsteps + 1 offsets, nsteps = chunk tiles + 1) orders the work items; counter, done (one per
+// chunk tile) and err are device scratch; err[0] becomes 1 if a batch ever gave up waiting.
+void launch_fused_expand_likelihood(hipStream_t s, const double *rots, const double *model, long size,
+                                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix, int r0,
+                                    int C, int pix_per_group, double *logw, double *upart, int *ubad,
+                                    const int *slot_rot, const long *off, const int *pix, const double *cnt,
+                                    const int *pair_frame, const int *pair_tile, const long *tile_off,
+                                    const int *tile_pair, int t0, const long *item_off, int nsteps, long total_items,
+                                    unsigned *counter, int *done, int *err, double *G);
+void launch_fused_expand_likelihood(hipStream_t s, const double *rots, const float *model, long size,
+                                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix, int r0,
+                                    int C, int pix_per_group, float *logw, double *upart, int *ubad,
+                                    const int *slot_rot, const long *off, const int *pix, const double *cnt,
+                                    const int *pair_frame, const int *pair_tile, const long *tile_off,
+                                    const int *tile_pair, int t0, const long *item_off, int nsteps, long total_items,
+                                    unsigned *counter, int *done, int *err, double *G);
+constexpr int kFusedPairBatch = 64;  // pairs per likelihood item of the persistent kernel
+void launch_judge_outside(hipStream_t s, const double *Pc, int num_data, int num_coarse, const double *Mc,
+                          const double *pnc, const double *Mf, const double *pnf, const double *beta,
+                          const uint8_t *skip, const uint32_t *bits, int words, const double *log_n,
+                          const double *log_wr, double prob_floor, double *out);
 void launch_posterior(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *logwr, const double *u, const double *beta,
                       const double *sum_fact, const uint8_t *blacklist, const double *max_exp,
@@ -107,15 +134,20 @@ void launch_tomogram_reduce(hipStream_t s, const double *partial, int nparts, in
 void launch_tomogram_reduce_rmajor(hipStream_t s, const double *partial, int nparts, int r0, int C,
                                    const double *psum_r, int num_rot, int num_pix, double *view);
 // ---- brick merge (emc_merge.hip) ----
+// per-brick item lists: pass 0 counts, pass 1 writes the item ids and, beside each, the mask of
+// the item's pixels with a trilinear corner inside the brick
 void launch_brick_items(hipStream_t s, const double *rots, int r0, int Cc, int num_rot,
                         const double *qvals, const uint8_t *mask, int num_pix, long size, int npatch,
                         int nbx, int pass, int *counts, const long *off, int *fill, int *list,
-                        const int *slot_rot, int modes = 1);
+                        unsigned long long *lmask, const int *slot_rot, int modes = 1);
 void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const int *list,
-                         const double *rots, int r0, int num_rot, const double *view,
-                         const double *psum_r, int all_on, const double *qvals, const double *corr,
-                         const uint8_t *mask, int num_pix, long size, int npatch, int nbx,
-                         double *model2, double *weight, int do_model, int do_weight, int f32_sums = 0);
+                         const unsigned long long *lmask, const double *rots, int r0, int num_rot,
+                         const double *view, const double *psum_r, int all_on, const double *qvals,
+                         const double *corr, const uint8_t *mask, int num_pix, long size, int npatch, int nbx,
+                         double *model2, double *weight, int do_model, int do_weight, int f32_sums = 0,
+                         const int *gate = nullptr);
+// dst = src (n doubles) when *gate == 0
+void launch_copy_if_zero(hipStream_t s, double *dst, const double *src, size_t n, const int *gate);
 void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot, const double *view,
                   const double *psum_r, int all_on, const double *qvals, const double *corr,
                   const uint8_t *mask, int num_pix, long size, double *model2, double *weight,
@@ -132,6 +164,7 @@ void launch_frame_stats(hipStream_t s, const double *likelihood, const double *i
 void launch_scale(hipStream_t s, const int *fcounts, const double *psum_d, const uint8_t *blacklist,
                   int num_data, double *scale);
 void launch_fill(hipStream_t s, double *a, size_t n, double v);
+void launch_fill_int(hipStream_t s, int *a, size_t n, int v);
 void launch_update_model(hipStream_t s, double *model1, double *model2, const double *w, long size,
                          double alpha, double *part, int nparts, double *out_change, int modes = 1);
 void launch_normalize_scale(hipStream_t s, double *scale, const uint8_t *blacklist, int num_data,

@@ -16,6 +16,7 @@
 #include <climits>
 
 #include "emc_common.h"
+#include "emc_expand.h"
 #include "emc_geometry.h"
 #include "emc_kernels.h"
 
@@ -42,33 +43,7 @@ __device__ __forceinline__ void merge_value(double *__restrict__ model2, double 
 }
 
 // ---- expand: log W for one orientation chunk, plus partial u_r ---------------------------
-// Block = 64 orientations x one pixel group. Compute with lanes along pixels (neighbouring
-// pixels hit neighbouring voxels), transpose through LDS, store with lanes along
-// orientations (coalesced rows of logW). u_r (max_prob.c:36-38) sums W over mask < 1.
-__device__ __forceinline__ double slice_value_model(const double *__restrict__ model, long size, const double *rot,
-                                                    double q0, double q1, double q2, double corr, double center) {
-    return slice_value_flat(model, size, rot, q0, q1, q2, corr, center);
-}
-// the FP32 model is the brick copy (copy_f32_brick_kernel)
-__device__ __forceinline__ double slice_value_model(const float *__restrict__ model, long size, const double *rot,
-                                                    double q0, double q1, double q2, double corr, double center) {
-    return slice_value_f32b(model, size, rot, q0, q1, q2, corr, center);
-}
-// elements of one mode's volume: size^3 for the FP64 model, the padded brick grid for the FP32 copy
-__device__ __forceinline__ size_t mode_stride(const double *, long size) { return (size_t)(size * size * size); }
-__device__ __forceinline__ size_t mode_stride(const float *, long size) {
-    const size_t nb = (size_t)model_bricks(size);
-    return nb * nb * nb * 64;
-}
-// log W as stored: FP64 log for an FP64 model; FP32 log of the FP32 interpolation otherwise
-// (values below FLT_MIN, i.e. DBL_MIN outside the grid, keep the FP64 log)
-template <typename T, typename MT>
-__device__ __forceinline__ T log_stored(double W) {
-    if (sizeof(MT) == sizeof(float)) return (T)(W >= (double)FLT_MIN ? logf((float)W) : (float)log(W));
-    return (T)log(W);
-}
-
-// MT: element type of the model read (double, or float for the FP32-gather path's copy)
+// Block = 64 orientations x one pixel group (expand_block, emc_expand.h).
 template <typename T, int kW, typename MT = double>
 __global__ __launch_bounds__(kBlock) void expand_kernel(
     const double *__restrict__ rots, const MT *__restrict__ model, long size,
@@ -76,64 +51,9 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
     const uint8_t *__restrict__ mask, int num_pix, int r0, int C, int pix_per_group,
     T *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad,
     const int *__restrict__ slot_rot, int modes, int raw) {
-    __shared__ double s_rot[kRTile * 9];
-    __shared__ double s_tile[64][kRTile + 1];
-    __shared__ double s_u[kWavesPerBlock][kRTile];
-    __shared__ int s_bad[kWavesPerBlock][kRTile];
-    __shared__ int s_pad[kRTile];
-    const int tid = threadIdx.x;
-    const int c0 = blockIdx.x * kRTile;
-    const int g = blockIdx.y;
-    for (int i = tid; i < kRTile * 9; i += kBlock) s_rot[i] = rots[(size_t)(r0 + c0) * 9 + i];
-    // padding slots of the subset order (slot_rot < 0) get W = 1 without interpolation
-    if (tid < kRTile) s_pad[tid] = slot_rot != nullptr && slot_rot[r0 + c0 + tid] < 0;
-    __syncthreads();
-    const double center = (double)(size / 2);
-    const int lane = tid & 63, wave = tid >> 6;
-    const int t_begin = g * pix_per_group;
-    const int t_end = min(num_pix, t_begin + pix_per_group);
-    double uacc = 0.;
-    int bad = 0;  // some mask < 1 pixel has W <= 0 (log W = -inf or NaN)
-    for (int tb = t_begin; tb < t_end; tb += 64) {
-        const int t = tb + lane;
-        if (t < t_end) {
-            const double q0 = qvals[3 * (size_t)t], q1 = qvals[3 * (size_t)t + 1],
-                         q2 = qvals[3 * (size_t)t + 2], cr = corr[t];
-#pragma unroll 4
-            for (int k = 0; k < kRTile / kWavesPerBlock; ++k) {
-                const int rr = wave + kWavesPerBlock * k;
-                // combined orientation index r = rotation * modes + mode (max_prob.c:80-84)
-                const MT *mdl = modes > 1 ? model + (size_t)((r0 + c0 + rr) % modes) * mode_stride(model, size) : model;
-                const double v = slice_value_model(mdl, size, &s_rot[rr * 9], q0, q1, q2, cr, center);
-                s_tile[lane][rr] = s_pad[rr] ? 1. : v;
-            }
-        }
-        __syncthreads();
-        for (int k = 0; k < 64 / kWavesPerBlock; ++k) {
-            const int p = wave + kWavesPerBlock * k;
-            const int tt = tb + p;
-            if (tt < t_end) {
-                const double W = s_tile[p][lane];
-                if (mask[tt] < 1) {
-                    uacc += W;
-                    bad |= !(W > 0.);
-                }
-                // tile-major with kW-wide tiles: (c / kW) * num_pix * kW + t * kW + c % kW
-                const int c = c0 + lane;
-                // raw: W itself (the background-aware likelihood takes log(phi W + b B))
-                logw[(size_t)(c / kW) * ((size_t)num_pix * kW) + (size_t)tt * kW + (c % kW)] =
-                    raw ? (T)W : log_stored<T, MT>(W);
-            }
-        }
-        __syncthreads();
-    }
-    s_u[wave][lane] = uacc;
-    s_bad[wave][lane] = bad;
-    __syncthreads();
-    if (tid < kRTile) {
-        upart[(size_t)g * C + c0 + tid] = s_u[0][tid] + s_u[1][tid] + s_u[2][tid] + s_u[3][tid];
-        ubad[(size_t)g * C + c0 + tid] = s_bad[0][tid] | s_bad[1][tid] | s_bad[2][tid] | s_bad[3][tid];
-    }
+    __shared__ ExpandShared sh;
+    expand_block<T, kW, MT>(sh, blockIdx.x * kRTile, blockIdx.y, rots, model, size, qvals, corr, mask, num_pix, r0, C,
+                            pix_per_group, logw, upart, ubad, slot_rot, modes, raw);
 }
 
 // u[r0 + c] = sum over pixel groups, fixed order.
@@ -1120,6 +1040,18 @@ void launch_expand(hipStream_t s, const double *rots, const float *model, long s
                                                                     slot_rot, modes, 0);
 }
 
+__global__ void copy_if_zero_kernel(double *__restrict__ dst, const double *__restrict__ src, size_t n,
+                                    const int *__restrict__ gate) {
+    if (*gate != 0) return;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_copy_if_zero(hipStream_t s, double *dst, const double *src, size_t n, const int *gate) {
+    if (n == 0) return;
+    copy_if_zero_kernel<<<grid_cap(cdiv(n, 256)), 256, 0, s>>>(dst, src, n, gate);
+}
+
 __global__ void copy_f32_kernel(const double *__restrict__ in, size_t n, float *__restrict__ out) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         out[i] = (float)in[i];
@@ -1390,6 +1322,87 @@ void launch_subset_mass(hipStream_t s, const double *R, int num_data, const long
     subset_mass_kernel<<<cdiv(num_data, 256), 256, 0, s>>>(R, num_data, off, idx, mass);
 }
 
+// Posterior mass a frame's fine search would put outside its cells, estimated from a coarse
+// full search (every cell centre) against the fine maximum inside the cells. One wavefront
+// per frame; lanes stride over the coarse orientations. With the coarse posterior
+// Pc = exp(beta (Rc - Mc)) / pnc, a cell outside the frame's set is counted as n_c fine
+// orientations at the centre's log-likelihood with the centre's weight w_c replaced by the
+// cell's mean fine weight wbar_c:
+//   Z_out = sum_c n_c (wbar_c / w_c)^beta exp(beta (Rc - M)) = pnc exp(beta (Mc - M)) sum_c Pc g_c
+// with g_c = exp(log_n[c] + beta * log_wr[c]); the fine search inside holds Z_in = p_norm
+// (relative to its maximum M), and the estimate is Z_out / (Z_in + Z_out). Coarse centres
+// outside the coarse keep set (Pc = 0, more than 14 / beta below the coarse maximum) count 0.
+// Frames with skip[d] (full frames, blacklisted frames) get 0. With prob_floor > 0 only the
+// cells whose estimated posterior per fine orientation (the centre's term over Z_in + Z_out)
+// exceeds prob_floor are counted in the numerator: the mass outside that is spread thinner
+// than prob_floor per orientation is mass the tomogram drops anyway (P <= 1e-6,
+// max_update.c:119-120, 196).
+__global__ __launch_bounds__(kBlock) void judge_outside_kernel(
+    const double *__restrict__ Pc, int num_data, int num_coarse, const double *__restrict__ Mc,
+    const double *__restrict__ pnc, const double *__restrict__ Mf, const double *__restrict__ pnf,
+    const double *__restrict__ beta, const uint8_t *__restrict__ skip, const uint32_t *__restrict__ bits,
+    int words, const double *__restrict__ log_n, const double *__restrict__ log_wr, double prob_floor,
+    double *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int d = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (d >= num_data) return;
+    if (skip[d]) {
+        if (lane == 0) out[d] = 0.;
+        return;
+    }
+    const double b = beta[d];
+    const uint32_t *row = bits + (size_t)d * words;
+    auto pc = [&](int c) {
+        return Pc[(size_t)(c / kRTile) * ((size_t)num_data * kRTile) + (size_t)d * kRTile + c % kRTile];
+    };
+    double acc = 0.;
+    for (int c = lane; c < num_coarse; c += 64) {
+        if ((row[c >> 5] >> (c & 31)) & 1u) continue;
+        const double p = pc(c);
+        if (p > 0.) acc += p * exp(log_n[c] + b * log_wr[c]);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (!(acc > 0.)) {
+        if (lane == 0) out[d] = 0.;
+        return;
+    }
+    // Z_out = exp(lz) with the coarse sum acc; the share outside is Z_out / (Z_in + Z_out)
+    const double lz0 = log(pnc[d]) + b * (Mc[d] - Mf[d]);  // log of Z_out / acc
+    const double lz = lz0 + log(acc);
+    const double lr = log(pnf[d]) - lz;  // log Z_in / Z_out
+    if (lr > 700.) {
+        if (lane == 0) out[d] = 0.;
+        return;
+    }
+    const double share = 1. / (1. + exp(lr));
+    double v = share;
+    if (prob_floor > 0.) {
+        // log (Z_in + Z_out): the normaliser of the per-orientation estimates
+        const double lzt = lz + log1p(exp(lr));
+        double sig = 0.;
+        for (int c = lane; c < num_coarse; c += 64) {
+            if ((row[c >> 5] >> (c & 31)) & 1u) continue;
+            const double p = pc(c);
+            if (!(p > 0.)) continue;
+            const double term = p * exp(log_n[c] + b * log_wr[c]);    // the cell's share of acc
+            const double per = exp(lz0 + log(term) - log_n[c] - lzt);  // per fine orientation
+            if (per > prob_floor) sig += term;
+        }
+        for (int o = 32; o > 0; o >>= 1) sig += __shfl_xor(sig, o);
+        v = share * (sig / acc);
+    }
+    if (lane == 0) out[d] = v;
+}
+
+void launch_judge_outside(hipStream_t s, const double *Pc, int num_data, int num_coarse, const double *Mc,
+                          const double *pnc, const double *Mf, const double *pnf, const double *beta,
+                          const uint8_t *skip, const uint32_t *bits, int words, const double *log_n,
+                          const double *log_wr, double prob_floor, double *out) {
+    if (num_data <= 0) return;
+    judge_outside_kernel<<<cdiv(num_data, kWavesPerBlock), kBlock, 0, s>>>(
+        Pc, num_data, num_coarse, Mc, pnc, Mf, pnf, beta, skip, bits, words, log_n, log_wr, prob_floor, out);
+}
+
 void launch_posterior(hipStream_t s, double *R, int num_rot, int num_data,
                       const double *logwr, const double *u, const double *beta,
                       const double *sum_fact, const uint8_t *blacklist, const double *max_exp,
@@ -1545,6 +1558,16 @@ void launch_frame_stats(hipStream_t s, const double *likelihood, const double *i
 void launch_scale(hipStream_t s, const int *fcounts, const double *psum_d, const uint8_t *blacklist,
                   int num_data, double *scale) {
     scale_kernel<<<cdiv(num_data, 256), 256, 0, s>>>(fcounts, psum_d, blacklist, num_data, scale);
+}
+
+__global__ void fill_int_kernel(int *__restrict__ a, size_t n, int v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+
+void launch_fill_int(hipStream_t s, int *a, size_t n, int v) {
+    if (n == 0) return;
+    fill_int_kernel<<<grid_cap(cdiv(n, 256)), 256, 0, s>>>(a, n, v);
 }
 
 void launch_fill(hipStream_t s, double *a, size_t n, double v) {

@@ -6,7 +6,10 @@
 // a patch is a compact detector tile and its trilinear cells for one orientation fill a small
 // box that meets 1-8 bricks. Which bricks an item meets depends only on the rotations and the
 // detector, so the per-brick item lists are built once (brick_items_kernel, two passes:
-// count, then fill) and reused by every iteration. Each block then owns one brick (or a
+// count, then fill) and reused by every iteration. Each list entry carries the 64-bit mask of
+// the item's pixels that have a trilinear corner inside the brick: a patch's bounding box meets
+// about four 16^3 bricks, its pixels' corners about 1.3 each, so a block loads and interpolates
+// only the pixels it adds to. Each block then owns one brick (or a
 // range of a crowded brick's items), adds every corner contribution that falls inside the
 // brick into an LDS copy of it, and adds the brick to HBM once: one global atomic per touched
 // voxel per block instead of one per (orientation, pixel, corner).
@@ -47,7 +50,7 @@ __global__ __launch_bounds__(kBlock) void brick_items_kernel(
     const double *__restrict__ rots, int r0, int Cc, int num_rot, const double *__restrict__ qvals,
     const uint8_t *__restrict__ mask, int num_pix, long size, int npatch, int nbx, int pass,
     int *__restrict__ counts, const long *__restrict__ off, int *__restrict__ fill,
-    int *__restrict__ list, const int *__restrict__ slot_rot, int modes) {
+    int *__restrict__ list, unsigned long long *__restrict__ lmask, const int *__restrict__ slot_rot, int modes) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nb3 = nbx * nbx * nbx;
     const long items = (long)Cc * npatch, stride = (long)gridDim.x * kWavesPerBlock;
@@ -74,13 +77,23 @@ __global__ __launch_bounds__(kBlock) void brick_items_kernel(
         const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1;
         const int nb = (bx1 - bx0 + 1) * ny * nz;
         const int mbase = modes > 1 ? (r % modes) * nb3 : 0;
-        for (int i = lane; i < nb; i += 64) {
-            const int b = mbase + ((bx0 + i / (ny * nz)) * nbx + by0 + (i / nz) % ny) * nbx + bz0 + i % nz;
+        // the lane's corners span cells x .. x + 1 (likewise y, z): it meets the bricks from
+        // x / kBrick to (x + 1) / kBrick along each axis
+        const int cx0 = ok ? (int)cell.x / kBrick : INT_MAX, cx1 = ok ? ((int)cell.x + 1) / kBrick : -1;
+        const int cy0 = ok ? (int)cell.y / kBrick : INT_MAX, cy1 = ok ? ((int)cell.y + 1) / kBrick : -1;
+        const int cz0 = ok ? (int)cell.z / kBrick : INT_MAX, cz1 = ok ? ((int)cell.z + 1) / kBrick : -1;
+        for (int i = 0; i < nb; ++i) {
+            const int bx = bx0 + i / (ny * nz), by = by0 + (i / nz) % ny, bz = bz0 + i % nz;
+            const bool in = bx >= cx0 && bx <= cx1 && by >= cy0 && by <= cy1 && bz >= cz0 && bz <= cz1;
+            const unsigned long long m = __ballot(in);
+            if (m == 0 || lane != 0) continue;
+            const int b = mbase + (bx * nbx + by) * nbx + bz;
             if (pass == 0) {
                 atomicAdd(&counts[b], 1);
             } else {
-                const int at = atomicAdd(&fill[b], 1);
-                list[off[b] + at] = (int)item;
+                const long at = off[b] + atomicAdd(&fill[b], 1);
+                list[at] = (int)item;
+                lmask[at] = m;
             }
         }
     }
@@ -127,10 +140,10 @@ __device__ __forceinline__ void brick_add(const double *rot, double q0, double q
 // brick is added to the FP64 model once per work unit).
 template <typename A>
 __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
-    const int4 *__restrict__ units, const int *__restrict__ list, const double *__restrict__ rots,
-    int r0, int num_rot, const double *__restrict__ view, const double *__restrict__ psum_r, int all_on,
-    const double *__restrict__ qvals, const double *__restrict__ corr, const uint8_t *__restrict__ mask,
-    int num_pix, long size, int npatch, int nbx, double *__restrict__ model2) {
+    const int4 *__restrict__ units, const int *__restrict__ list, const unsigned long long *__restrict__ lmask,
+    const double *__restrict__ rots, int r0, int num_rot, const double *__restrict__ view,
+    const double *__restrict__ psum_r, int all_on, const double *__restrict__ qvals, const double *__restrict__ corr,
+    const uint8_t *__restrict__ mask, int num_pix, long size, int npatch, int nbx, double *__restrict__ model2) {
     constexpr int kVox = kBrick * kBrick * kBrick;
     constexpr int kRing = 128;
     __shared__ A s_m[kVox];
@@ -160,10 +173,12 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
     constexpr int kItemsAhead = 4;
     for (int k0 = u.y + wave; k0 < u.z; k0 += kWavesPerBlock * kItemsAhead) {
         int it[kItemsAhead];
+        bool mine[kItemsAhead];  // the lane's pixel has a corner in this brick
 #pragma unroll
         for (int a = 0; a < kItemsAhead; ++a) {
             const int k = k0 + a * kWavesPerBlock;
             it[a] = k < u.z ? list[k] : -1;
+            mine[a] = k < u.z && ((lmask[k] >> lane) & 1ull);
         }
         double wv[kItemsAhead];
         int rr[kItemsAhead];
@@ -176,10 +191,7 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
             const int r = r0 + c, t = p * 64 + lane;
             if (!all_on && !(psum_r[r] > 0.)) continue;
             rr[a] = r;
-            if (t < num_pix) {
-                const double v = view[(size_t)c * num_pix + t];
-                wv[a] = mask[t] <= 1 ? v : 0.;
-            }
+            if (mine[a]) wv[a] = view[(size_t)c * num_pix + t];  // a set bit: t < num_pix, mask[t] <= 1
         }
 #pragma unroll
         for (int a = 0; a < kItemsAhead; ++a) {
@@ -211,11 +223,14 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
 // into weight.
 template <bool kModel, bool kWeight>
 __global__ __launch_bounds__(kBlock) void merge_brick_kernel(
-    const int4 *__restrict__ units, const int *__restrict__ list, const double *__restrict__ rots,
-    int r0, int num_rot, const double *__restrict__ view, const double *__restrict__ psum_r,
-    int all_on, const double *__restrict__ qvals, const double *__restrict__ corr,
-    const uint8_t *__restrict__ mask, int num_pix, long size, int npatch, int nbx,
-    double *__restrict__ model2, double *__restrict__ weight) {
+    const int4 *__restrict__ units, const int *__restrict__ list, const unsigned long long *__restrict__ lmask,
+    const double *__restrict__ rots, int r0, int num_rot, const double *__restrict__ view,
+    const double *__restrict__ psum_r, int all_on, const double *__restrict__ qvals, const double *__restrict__ corr,
+    int num_pix, long size, int npatch, int nbx, double *__restrict__ model2, double *__restrict__ weight,
+    const int *__restrict__ gate) {
+    // gate: the number of orientations that merge nothing (psum_r = 0); none means the
+    // weights come from the all-orientation cache instead and this pass has nothing to do
+    if (gate && *gate == 0) return;
     constexpr int kVox = kBrick * kBrick * kBrick;
     __shared__ double s_m[kModel ? kVox : 1];
     __shared__ double s_w[kWeight ? kVox : 1];
@@ -233,12 +248,13 @@ __global__ __launch_bounds__(kBlock) void merge_brick_kernel(
     __syncthreads();
     const double center = (double)(size / 2);
     for (int k = u.y + wave; k < u.z; k += kWavesPerBlock) {
+        // a set bit: the lane's pixel (t < num_pix, mask[t] <= 1) has a corner in this brick
+        if (!((lmask[k] >> lane) & 1ull)) continue;
         const int item = list[k];
         const int c = item / npatch, p = item % npatch;
         const int r = r0 + c;
         if (!all_on && !(psum_r[r] > 0.)) continue;
         const int t = p * 64 + lane;
-        if (t >= num_pix || mask[t] > 1) continue;
         double w = 0.;
         if (kModel) {
             w = view[(size_t)c * num_pix + t] / corr[t];
@@ -278,32 +294,33 @@ __global__ __launch_bounds__(kBlock) void merge_brick_kernel(
 void launch_brick_items(hipStream_t s, const double *rots, int r0, int Cc, int num_rot,
                         const double *qvals, const uint8_t *mask, int num_pix, long size, int npatch,
                         int nbx, int pass, int *counts, const long *off, int *fill, int *list,
-                        const int *slot_rot, int modes) {
+                        unsigned long long *lmask, const int *slot_rot, int modes) {
     const long items = (long)Cc * npatch;
     const unsigned grid = stride_grid((items + kWavesPerBlock - 1) / kWavesPerBlock);
     brick_items_kernel<<<grid, kBlock, 0, s>>>(rots, r0, Cc, num_rot, qvals, mask, num_pix, size, npatch,
-                                               nbx, pass, counts, off, fill, list, slot_rot, modes);
+                                               nbx, pass, counts, off, fill, list, lmask, slot_rot, modes);
 }
 
 void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const int *list,
-                         const double *rots, int r0, int num_rot, const double *view,
-                         const double *psum_r, int all_on, const double *qvals, const double *corr,
-                         const uint8_t *mask, int num_pix, long size, int npatch, int nbx,
-                         double *model2, double *weight, int do_model, int do_weight, int f32_sums) {
+                         const unsigned long long *lmask, const double *rots, int r0, int num_rot,
+                         const double *view, const double *psum_r, int all_on, const double *qvals,
+                         const double *corr, const uint8_t *mask, int num_pix, long size, int npatch, int nbx,
+                         double *model2, double *weight, int do_model, int do_weight, int f32_sums,
+                         const int *gate) {
     if (num_units <= 0) return;
-#define EMC_MB(M, W)                                                                             \
-    merge_brick_kernel<M, W><<<num_units, kBlock, 0, s>>>(units, list, rots, r0, num_rot, view, psum_r, \
-                                                          all_on, qvals, corr, mask, num_pix, size, \
-                                                          npatch, nbx, model2, weight)
+#define EMC_MB(M, W)                                                                                        \
+    merge_brick_kernel<M, W><<<num_units, kBlock, 0, s>>>(units, list, lmask, rots, r0, num_rot, view, psum_r, \
+                                                          all_on, qvals, corr, num_pix, size, npatch, nbx,     \
+                                                          model2, weight, gate)
     if (do_model && do_weight) EMC_MB(true, true);
     else if (do_model && f32_sums)
-        merge_brick_model_kernel<float><<<num_units, kBlock, 0, s>>>(units, list, rots, r0, num_rot, view, psum_r,
-                                                                     all_on, qvals, corr, mask, num_pix, size,
-                                                                     npatch, nbx, model2);
+        merge_brick_model_kernel<float><<<num_units, kBlock, 0, s>>>(units, list, lmask, rots, r0, num_rot, view,
+                                                                     psum_r, all_on, qvals, corr, mask, num_pix,
+                                                                     size, npatch, nbx, model2);
     else if (do_model)
-        merge_brick_model_kernel<double><<<num_units, kBlock, 0, s>>>(units, list, rots, r0, num_rot, view, psum_r,
-                                                                      all_on, qvals, corr, mask, num_pix, size,
-                                                                      npatch, nbx, model2);
+        merge_brick_model_kernel<double><<<num_units, kBlock, 0, s>>>(units, list, lmask, rots, r0, num_rot, view,
+                                                                      psum_r, all_on, qvals, corr, mask, num_pix,
+                                                                      size, npatch, nbx, model2);
     else if (do_weight) EMC_MB(false, true);
 #undef EMC_MB
 }

@@ -226,7 +226,9 @@ __global__ __launch_bounds__(kBlock) void nn_polar_hist_kernel(
 
 // Indices of the k largest of z[row][0..n) in descending order (k <= 64), one wave per row:
 // k rounds of a wave-wide arg-max over the lane-strided row (ties: lower index). When mass is
-// given, mass[row] = the softmax probability of the selected entries.
+// given, mass[row] = the softmax probability of the selected entries (kProb: z are
+// non-negative masses, and mass[row] = their sum over the selected entries).
+template <bool kProb>
 __global__ __launch_bounds__(kBlock) void nn_topk_kernel(const float *__restrict__ Z, size_t ldz, int rows,
                                                          int n, int k, int *__restrict__ idx,
                                                          float *__restrict__ mass) {
@@ -238,8 +240,10 @@ __global__ __launch_bounds__(kBlock) void nn_topk_kernel(const float *__restrict
     for (int j = lane; j < n; j += 64) zmax = fmaxf(zmax, z[j]);
     for (int o = 32; o > 0; o >>= 1) zmax = fmaxf(zmax, __shfl_xor(zmax, o));
     float zsum = 0.f;
-    for (int j = lane; j < n; j += 64) zsum += __expf(z[j] - zmax);
-    for (int o = 32; o > 0; o >>= 1) zsum += __shfl_xor(zsum, o);
+    if (!kProb) {
+        for (int j = lane; j < n; j += 64) zsum += __expf(z[j] - zmax);
+        for (int o = 32; o > 0; o >>= 1) zsum += __shfl_xor(zsum, o);
+    }
     float captured = 0.f;
     unsigned long long taken_lo = 0;  // lane-local bitmap over this lane's (up to 64) columns
     unsigned long long taken_hi = 0;
@@ -270,9 +274,9 @@ __global__ __launch_bounds__(kBlock) void nn_topk_kernel(const float *__restrict
             else taken_hi |= 1ull << (slot - 64);
         }
         if (lane == 0) idx[(size_t)row * k + round] = (wi == INT_MAX) ? -1 : wi;
-        if (wi != INT_MAX) captured += __expf(wb - zmax);
+        if (wi != INT_MAX) captured += kProb ? wb : __expf(wb - zmax);
     }
-    if (mass && lane == 0) mass[row] = captured / zsum;
+    if (mass && lane == 0) mass[row] = kProb ? captured : captured / zsum;
 }
 
 // ---- launchers --------------------------------------------------------------------------------
@@ -344,7 +348,14 @@ void launch_nn_polar_hist(hipStream_t s, const long *off, const int *pix, const 
 }
 
 void launch_nn_topk(hipStream_t s, const float *Z, size_t ldz, int rows, int n, int k, int *idx, float *mass) {
-    nn_topk_kernel<<<(rows + kWavesPerBlock - 1) / kWavesPerBlock, kBlock, 0, s>>>(Z, ldz, rows, n, k, idx, mass);
+    nn_topk_kernel<false><<<(rows + kWavesPerBlock - 1) / kWavesPerBlock, kBlock, 0, s>>>(Z, ldz, rows, n, k, idx,
+                                                                                          mass);
+}
+
+void launch_prob_topk(hipStream_t s, const float *Z, size_t ldz, int rows, int n, int k, int *idx, float *mass) {
+    if (rows <= 0) return;
+    nn_topk_kernel<true><<<(rows + kWavesPerBlock - 1) / kWavesPerBlock, kBlock, 0, s>>>(Z, ldz, rows, n, k, idx,
+                                                                                         mass);
 }
 
 }  // namespace emc

@@ -18,9 +18,11 @@
 // same first arg-max (ties broken by the original orientation index), the same keep rule
 // (ordered by the original index), posterior, sums and tomogram. Only the order of additions
 // across frames in psum_r and the tomogram differs from the reference.
+#include <algorithm>
 #include <cfloat>
 
 #include "emc_common.h"
+#include "emc_expand.h"
 #include "emc_kernels.h"
 
 namespace emc {
@@ -524,6 +526,163 @@ void launch_cell_posterior(hipStream_t s, double *R, int plain, int num_data, in
     if (count <= 0) return;
     cell_posterior_kernel<<<count, kBlock, num_coarse * sizeof(double), s>>>(
         R, plain, num_data, num_rot, pair_off, pair_tile, full, slot_rot, qmap, num_coarse, d0, out);
+}
+
+// ---- persistent expand + likelihood ------------------------------------------------------
+// One launch per orientation chunk does the chunk's expand and its (frame, tile) pair
+// likelihoods. Work items are handed out in order from a counter: step k holds the expand
+// pieces (one per pixel group) of local tile k and then the pair batches of tile k - 1, so a
+// tile's expand is under way one step before its pairs are taken. A pair batch waits until
+// every expand piece of its tile has finished (per-tile counter, release / acquire at agent
+// scope) and then reads that tile's log W while it is recent in L2 / the Infinity Cache. Items
+// are taken by resident blocks only and an expand piece waits on nothing, so every item a
+// batch waits for is already running; the wait is bounded all the same (err[0] = 1 and the
+// batch is skipped) so that no wave can spin for ever. Batches of frames with many photons
+// take longer than others; blocks that finish early take the next items, which keeps the
+// uneven frames of a chunk load-balanced. The arithmetic is expand_block's and
+// pair_likelihood_kernel's, in the same order.
+constexpr long kFusedMaxSpin = 1L << 24;
+
+__global__ void fused_reset_kernel(unsigned *__restrict__ counter, int *__restrict__ done, int ntiles) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *counter = 0u;
+    if (i < ntiles) done[i] = 0;
+}
+
+template <int U, typename T, int kW, typename MT>
+__global__ __launch_bounds__(kBlock) void fused_expand_likelihood_kernel(
+    const double *__restrict__ rots, const MT *__restrict__ model, long size, const double *__restrict__ qvals,
+    const double *__restrict__ corr, const uint8_t *__restrict__ mask, int num_pix, int r0, int C, int pix_per_group,
+    int ngroups, T *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad,
+    const int *__restrict__ slot_rot, const long *__restrict__ off, const int *__restrict__ pix,
+    const double *__restrict__ cnt, const int *__restrict__ pair_frame, const int *__restrict__ pair_tile,
+    const long *__restrict__ tile_off, const int *__restrict__ tile_pair, int t0, const long *__restrict__ item_off,
+    int nsteps, unsigned *__restrict__ counter, int *__restrict__ done, int *__restrict__ err, double *__restrict__ G) {
+    constexpr int kSub = kW / kRTile;
+    __shared__ ExpandShared sh;
+    __shared__ long s_item;
+    __shared__ int s_go;
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const long total = item_off[nsteps];
+    const size_t tsW = (size_t)num_pix * kW;
+    for (;;) {
+        if (tid == 0) s_item = (long)atomicAdd(counter, 1u);
+        __syncthreads();
+        const long item = s_item;
+        __syncthreads();
+        if (item >= total) break;  // every block leaves here once the items run out
+        // step k: [ngroups expand pieces of tile k (k < number of tiles)] [batches of tile k - 1]
+        int lo = 0, hi = nsteps;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (item_off[mid] <= item) lo = mid;
+            else hi = mid;
+        }
+        const int k = lo;
+        const long within = item - item_off[k];
+        const int ntiles = nsteps - 1;
+        const int nexp = k < ntiles ? ngroups : 0;
+        if (within < nexp) {
+            expand_block<T, kW, MT>(sh, k * kRTile, (int)within, rots, model, size, qvals, corr, mask, num_pix, r0, C,
+                                    pix_per_group, logw, upart, ubad, slot_rot, 1, 0);
+            __threadfence();  // this thread's log W and u stores reach the device scope
+            __syncthreads();
+            if (tid == 0) __hip_atomic_fetch_add(&done[k], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            continue;
+        }
+        const int lt = k - 1;  // the batch's tile (local to the chunk)
+        const long b = within - nexp;
+        if (tid == 0) {
+            long spins = 0;
+            int go = 1;
+            while (__hip_atomic_load(&done[lt], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < ngroups) {
+                if (++spins > kFusedMaxSpin) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    go = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            s_go = go;
+        }
+        __syncthreads();
+        const int go = s_go;
+        __syncthreads();
+        if (!go) continue;
+        __threadfence();  // acquire side for every wave of the block
+        const long j0 = tile_off[t0 + lt] + b * kFusedPairBatch;
+        const long j1 = min(tile_off[t0 + lt + 1], j0 + kFusedPairBatch);
+        for (long j = j0 + wave; j < j1; j += kWavesPerBlock) {
+            const int kp = tile_pair[j];
+            const int d = pair_frame[kp];
+            const int lt2 = pair_tile[kp] - t0;
+            const T *__restrict__ col = logw + (size_t)(lt2 / kSub) * tsW + (lt2 % kSub) * kRTile + lane;
+            long q = off[d];
+            const long q1 = off[d + 1];
+            double acc = 0.;
+            for (; q + U <= q1; q += U) {
+                double v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) v[u] = (double)col[(size_t)pix[q + u] * kW];
+#pragma unroll
+                for (int u = 0; u < U; ++u) acc += cnt[q + u] * v[u];
+            }
+            for (; q < q1; ++q) acc += cnt[q] * (double)col[(size_t)pix[q] * kW];
+            G[(size_t)kp * kRTile + lane] = acc;
+        }
+    }
+}
+
+template <typename T, int kW, typename MT>
+static void fused_launch(hipStream_t s, const double *rots, const MT *model, long size, const double *qvals,
+                         const double *corr, const uint8_t *mask, int num_pix, int r0, int C, int pix_per_group,
+                         T *logw, double *upart, int *ubad, const int *slot_rot, const long *off, const int *pix,
+                         const double *cnt, const int *pair_frame, const int *pair_tile, const long *tile_off,
+                         const int *tile_pair, int t0, const long *item_off, int nsteps, long total_items,
+                         unsigned *counter, int *done, int *err, double *G) {
+    const int ntiles = nsteps - 1;
+    fused_reset_kernel<<<(ntiles + 255) / 256 + 1, 256, 0, s>>>(counter, done, ntiles);
+    if (total_items <= 0) return;
+    static int resident = 0;  // blocks of this kernel that fit the device at once
+    if (resident == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused_expand_likelihood_kernel<8, T, kW, MT>,
+                                                           kBlock, 0);
+        resident = std::max(1, cus * std::max(per_cu, 1));
+    }
+    const unsigned grid = (unsigned)std::min<long>(total_items, resident);
+    const int ngroups = (num_pix + pix_per_group - 1) / pix_per_group;
+    fused_expand_likelihood_kernel<8, T, kW, MT><<<grid, kBlock, 0, s>>>(
+        rots, model, size, qvals, corr, mask, num_pix, r0, C, pix_per_group, ngroups, logw, upart, ubad, slot_rot, off,
+        pix, cnt, pair_frame, pair_tile, tile_off, tile_pair, t0, item_off, nsteps, counter, done, err, G);
+}
+
+void launch_fused_expand_likelihood(hipStream_t s, const double *rots, const double *model, long size,
+                                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix, int r0,
+                                    int C, int pix_per_group, double *logw, double *upart, int *ubad,
+                                    const int *slot_rot, const long *off, const int *pix, const double *cnt,
+                                    const int *pair_frame, const int *pair_tile, const long *tile_off,
+                                    const int *tile_pair, int t0, const long *item_off, int nsteps, long total_items,
+                                    unsigned *counter, int *done, int *err, double *G) {
+    fused_launch<double, kRTile, double>(s, rots, model, size, qvals, corr, mask, num_pix, r0, C, pix_per_group, logw,
+                                         upart, ubad, slot_rot, off, pix, cnt, pair_frame, pair_tile, tile_off,
+                                         tile_pair, t0, item_off, nsteps, total_items, counter, done, err, G);
+}
+
+void launch_fused_expand_likelihood(hipStream_t s, const double *rots, const float *model, long size,
+                                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix, int r0,
+                                    int C, int pix_per_group, float *logw, double *upart, int *ubad,
+                                    const int *slot_rot, const long *off, const int *pix, const double *cnt,
+                                    const int *pair_frame, const int *pair_tile, const long *tile_off,
+                                    const int *tile_pair, int t0, const long *item_off, int nsteps, long total_items,
+                                    unsigned *counter, int *done, int *err, double *G) {
+    fused_launch<float, 2 * kRTile, float>(s, rots, model, size, qvals, corr, mask, num_pix, r0, C, pix_per_group,
+                                           logw, upart, ubad, slot_rot, off, pix, cnt, pair_frame, pair_tile, tile_off,
+                                           tile_pair, t0, item_off, nsteps, total_items, counter, done, err, G);
 }
 
 void launch_pair_likelihood(hipStream_t s, const double *logw, int num_pix, const long *off, const int *pix,

@@ -253,15 +253,13 @@ def union_cells(*tops):
 
 
 class CoarseJudge:
-    """Judges orientation proposals by the likelihood: a full search over the coarse
-    orientations quat_gen(coarse_div) (every frame, every coarse orientation -- the coarse
-    orientation c is the centre of Voronoi cell c of the fine sampling) stopped after the
-    posterior (estep_only), at the same model, scale factors and beta as the fine iteration it
-    judges. A frame's proposal (a set of cells) is accepted when the coarse posterior puts at
-    least `threshold` of its mass inside it; otherwise the frame falls back to the full fine
-    search. The coarse posterior samples each cell at its centre, so a sharp fine peak between
-    centres lowers the coarse maximum and raises the mass outside any proposal: the judge errs
-    towards falling back."""
+    """The full-search half of the likelihood judge: a search over the coarse orientations
+    quat_gen(coarse_div) (every frame, every coarse orientation -- the coarse orientation c is
+    the centre of Voronoi cell c of the fine sampling) stopped after the posterior
+    (estep_only), at the same model, scale factors and beta as the fine iteration it judges.
+    Its posterior proposes cells of its own (top_cells), and against the fine search's maximum
+    inside a frame's proposal it estimates the mass the full fine search would put outside it
+    (Engine.judge_outside; judged_iteration)."""
 
     def __init__(self, itr, coarse_div, device=0, precision=0):
         det = itr.dets[0]
@@ -289,6 +287,11 @@ class CoarseJudge:
         self.stats = eng.maximize(iteration, need_scaling, 0, 0)
         return self.stats
 
+    def top_cells(self, k):
+        """Each frame's k most likely coarse cells, (num_frames, k) int32."""
+        idx, _ = self.eng.cell_topk(self.num_coarse, np.arange(self.num_coarse, dtype='i4'), k)
+        return idx
+
     def argmax_cells(self):
         """Each frame's most likely coarse orientation (cell), (num_frames, 1) int32; a frame
         with no orientation (blacklisted) gets cell 0."""
@@ -300,6 +303,48 @@ class CoarseJudge:
 
     def close(self):
         self.eng.close()
+
+
+def judged_iteration(recon, itr, judge, tops, coarse_div, threshold=0.99):
+    """One EMC iteration (recon.run_iteration) on proposals judged by the likelihood.
+
+    Pass 1: the expectation step of the fine search restricted to each frame's proposed cells
+    (the union of the (num_frames, k_i) arrays in `tops`), which gives every frame its fine
+    maximum and normaliser inside the cells. The judge (a coarse full search already run at the
+    same model, scales and beta: CoarseJudge.estep) then estimates, per frame, the posterior
+    mass the full fine search would put outside the cells (Engine.judge_outside). Frames
+    estimated to hold at least `threshold` inside keep their cells; the rest fall back to the
+    full fine search. Pass 2: when every frame is kept, the M-step resumes from pass 1
+    (option mstep_only); otherwise the iteration runs again with the fall-back frames searched
+    in full. Returns a dict: accept (bool per frame), outside (estimates), passes (1 or 2)."""
+    eng = recon.engine
+    par = itr.params
+    off, cells = union_cells(*tops)
+    itr.set_refine_lists(coarse_div, off, cells)
+    eng.set_refine(itr.num_coarse, itr.quat_mapping, itr.rel_off, itr.rel_quat)
+    recon.update_beta()  # the beta of this iteration (run_iteration sets the same again)
+    eng.set_option('estep_only', 1)
+    try:
+        st1 = eng.maximize(par.iteration, par.need_scaling, par.update_scale, par.known_scale)
+    finally:
+        eng.set_option('estep_only', 0)
+    outside = eng.judge_outside(judge.eng)
+    accept = outside <= 1. - threshold
+    if accept.all():
+        eng.set_option('mstep_only', 1)
+        try:
+            recon.run_iteration()
+        finally:
+            eng.set_option('mstep_only', 0)
+        passes = 1
+    else:
+        rel_off, rel_quat = refine_lists_csr(off, cells, accept, itr.num_coarse)
+        itr.set_refine_lists(coarse_div, rel_off, rel_quat)
+        eng.set_refine(itr.num_coarse, itr.quat_mapping, itr.rel_off, itr.rel_quat)
+        recon.run_iteration()
+        passes = 2
+    return dict(accept=accept, outside=outside, passes=passes, cells_per_frame=float(np.diff(off).mean()),
+                estep_stage_ms=st1['stage_ms'])
 
 
 def refine_lists(top, mass, num_coarse, threshold=0.99, buf=None):

@@ -84,6 +84,7 @@ class EMCRecon:
         itr.num_blacklist = 0
         eng.set_frame_params(itr.scale, itr.beta_start, itr.blacklist, itr.fcounts,
                              itr.sum_fact, itr.mean_count[0], itr.num_blacklist)
+        self._engine_beta = None  # the engine now holds beta_start
         if det.with_bg:
             if itr.bgscale is None:
                 itr.parse_scale('', bg=True)
@@ -134,7 +135,12 @@ class EMCRecon:
         good = itr.blacklist != 1
         beta = itr.beta
         beta[good] = np.minimum(itr.beta_start[good] * factor, 1.)
-        self.engine.set_beta(beta)
+        # the engine's copy is refreshed only when beta changed (a set-up counts as a change
+        # of the iteration's state, which an expectation step already run would then not match)
+        last = getattr(self, '_engine_beta', None)
+        if last is None or last.shape != beta.shape or not np.array_equal(last, beta):
+            self.engine.set_beta(beta)
+            self._engine_beta = beta.copy()
         return beta[good].sum() / (itr.tot_num_data - itr.num_blacklist)
 
     # ---- outputs ---------------------------------------------------------------------

@@ -233,6 +233,12 @@ int emc_get_field(emc_engine *e, int field, void *dst, size_t count);
  * it gives an orientation-proposal model its training targets (the cell posteriors an
  * iteration produced) without moving the dense num_data x num_rot posterior.            */
 int emc_cell_posterior(emc_engine *e, int num_coarse, const int *quat_mapping, float *out);
+/* The k most likely coarse cells of every frame after an iteration: idx[d][0..k) the cells
+ * in descending order of posterior mass (ties to the lower cell), mass[d] the mass they hold
+ * (host memory; k <= 64, num_coarse <= 6144). No counterpart in the reference: the
+ * per-frame proposals of an earlier iteration's posterior, without moving the cell
+ * posteriors (emc_cell_posterior) to the host.                                          */
+int emc_cell_topk(emc_engine *e, int num_coarse, const int *quat_mapping, int k, int *idx, float *mass);
 /* Posterior mass of listed orientations, per frame, after a plain (not refinement-subset)
  * iteration: mass[d] = sum of P_dr over r in rel_quat[rel_off[d] .. rel_off[d + 1]) (rel_off:
  * num_data + 1 ascending offsets; orientations < num_rot, each listed once). No counterpart
@@ -240,6 +246,19 @@ int emc_cell_posterior(emc_engine *e, int num_coarse, const int *quat_mapping, f
  * judges orientation proposals by the likelihood -- the mass its posterior puts inside each
  * frame's proposed cells.                                                                */
 int emc_subset_mass(emc_engine *e, const long *rel_off, const int *rel_quat, double *mass);
+/* Likelihood judge of orientation proposals, after an iteration of e on refinement subsets
+ * run with the option estep_only (the frames' cells are the proposals) and a plain estep_only
+ * iteration of `coarse` on the same frames, scale factors and beta over one orientation per
+ * coarse cell (quat_gen of the coarse sampling, the cells of e's quat_mapping). Per frame,
+ * outside[d] (host, num_data) estimates the posterior mass the full fine search would put
+ * outside the frame's cells: each outside cell counts as its fine orientations at the centre's
+ * log-likelihood (the centre's weight replaced by the cell's mean fine weight), against the
+ * fine search's maximum and normaliser inside the cells. Full and blacklisted frames get 0.
+ * With prob_floor > 0 (< 1) only cells whose estimated posterior per fine orientation
+ * exceeds prob_floor count (mass spread thinner is dropped by the tomogram's P > 1e-6 cut).
+ * No counterpart in the reference: proposals are accepted or sent to the full search by this
+ * estimate (refinement in the reference, src/max_emc.c:264-274, takes its cells as given).  */
+int emc_judge_outside(emc_engine *e, emc_engine *coarse, double prob_floor, double *outside);
 /* Zero model2 and inter_weight (max_data allocate_memory, maximize.c:93-94).            */
 int emc_clear_merge(emc_engine *e);
 

@@ -526,3 +526,210 @@ def test_estep_only_and_subset_mass(tiny_scaled):
     with pytest.raises(Exception):
         eng.subset_mass(off, np.concatenate(sets) + R)   # orientation out of range
     eng.close()
+
+
+def _refine_setup(p, seed=5, ncells=10):
+    from dragonfly_amd import _lib
+    qc = _lib.quat_gen(1)
+    qmap = _lib.voronoi_subset(qc, p['quats'])
+    rng = np.random.default_rng(seed)
+    D = len(p['fparams']['beta'])
+    sets = [np.sort(rng.choice(len(qc), ncells, replace=False)).astype('i4') for _ in range(D)]
+    for d in range(3, D, 5):
+        sets[d] = np.array([-1], 'i4')  # the full search
+    off = np.zeros(D + 1, 'i8')
+    np.cumsum([len(x) for x in sets], out=off[1:])
+    return qc, qmap, off, np.concatenate(sets)
+
+
+@pytest.mark.parametrize('g32', [False, True])
+def test_estep_then_mstep_resume_equals_one_pass(tiny_scaled, g32):
+    """On refinement subsets, an estep_only iteration followed by an mstep_only one gives the
+    single full iteration's results (bit for bit: scales, posterior, per-frame and
+    per-orientation sums; model2 and weights to the order of their atomic additions);
+    mstep_only is refused without an estep_only iteration of the
+    current state (none run, or a setter called in between)."""
+    p = tiny_scaled
+    p = dict(p, fparams=dict(p['fparams'], scale=np.asarray(p['true_scale'], 'f8').copy()))
+    qc, qmap, off, cells = _refine_setup(p)
+    flags = dict(iteration=2, need_scaling=1, update_scale=1)
+    out = []
+    for two in (False, True):
+        eng = P.engine_for(p)
+        if g32:
+            eng.set_precision(3)
+        eng.set_refine(len(qc), qmap, off, cells)
+        if two:
+            with pytest.raises(Exception):
+                eng.set_option('mstep_only', 1)
+                eng.maximize(2, 1, 1, 0)
+            eng.set_option('mstep_only', 0)
+            eng.set_option('estep_only', 1)
+            eng.maximize(2, 1, 1, 0)
+            eng.set_option('estep_only', 0)
+            eng.set_option('mstep_only', 1)
+        out.append(P.run_engine(eng, flags))
+        if two:
+            # a second resume has nothing left to resume; nor has one after a setter
+            with pytest.raises(Exception):
+                eng.maximize(2, 1, 1, 0)
+            eng.set_option('mstep_only', 0)
+            eng.set_option('estep_only', 1)
+            eng.maximize(2, 1, 1, 0)
+            eng.set_option('estep_only', 0)
+            eng.set_scale(p['fparams']['scale'])
+            eng.set_option('mstep_only', 1)
+            with pytest.raises(Exception):
+                eng.maximize(2, 1, 1, 0)
+        eng.close()
+    a, b = out
+    for k in ('u', 'max_exp', 'p_norm', 'rmax', 'likelihood', 'info', 'psum_d', 'psum_r', 'scale', 'prob'):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ('avg_likelihood', 'mutual_info', 'rescale'):
+        assert a[k] == b[k], k
+    # the merge adds into the volume with atomics, whose order differs from run to run (and
+    # with FP32 gathers sums its bricks in FP32)
+    for k in ('model2', 'inter_weight'):
+        err = P.rel_err(a[k], b[k])
+        print('resume %s: %.1e' % (k, err), flush=True)
+        assert err <= (1e-5 if g32 else 1e-13), k
+
+
+def test_judge_outside_estimate(tiny_scaled):
+    """emc_judge_outside on the tiny problem (fine num_div 2, coarse num_div 1): the estimate
+    equals its definition recomputed from the two engines' results (the coarse posterior,
+    maxima and normalisers, the fine maxima and normalisers inside the cells, cell sizes and
+    weights), full frames get 0; and it follows the true outside mass of a full fine search --
+    frames estimated below 1e-3 hold less than 1e-2 outside their cells."""
+    from dragonfly_amd import _lib
+    p = tiny_scaled
+    p = dict(p, fparams=dict(p['fparams'], scale=np.asarray(p['true_scale'], 'f8').copy()))
+    qc, qmap, off, cells = _refine_setup(p, seed=9, ncells=24)
+    D = len(off) - 1
+    beta = p['fparams']['beta']
+    full = P.engine_for(p)
+    full.maximize(2, 1, 0, 0)
+    post = full.cell_posterior(len(qc), qmap).astype('f8')
+    full.close()
+    coarse = P.engine_for(dict(p, quats=qc))
+    coarse.set_option('estep_only', 1)
+    coarse.maximize(2, 1, 0, 0)
+    Pc = coarse.get('prob')
+    Mc, pnc = coarse.get('max_exp'), coarse.get('p_norm')
+    fine = P.engine_for(p)
+    fine.set_refine(len(qc), qmap, off, cells)
+    fine.set_option('estep_only', 1)
+    fine.maximize(2, 1, 0, 0)
+    Mf, pnf = fine.get('max_exp'), fine.get('p_norm')
+    est = fine.judge_outside(coarse)
+    fine.close()
+    coarse.close()
+    n = np.bincount(qmap, minlength=len(qc)).astype('f8')
+    wbar = np.bincount(qmap, weights=p['quats'][:, 4], minlength=len(qc)) / n
+    want = np.zeros(D)
+    inside = np.zeros((D, len(qc)), bool)
+    for d in range(D):
+        cs = cells[off[d]:off[d + 1]]
+        if len(cs) == 1 and cs[0] == -1:
+            inside[d] = True
+            continue
+        inside[d, cs] = True
+        g = n * (wbar / qc[:, 4]) ** beta[d]
+        acc = (Pc[d] * g)[~inside[d]].sum()
+        if acc > 0:
+            zout = pnc[d] * np.exp(beta[d] * (Mc[d] - Mf[d])) * acc
+            want[d] = zout / (pnf[d] + zout)
+    assert np.allclose(est, want, rtol=1e-10, atol=1e-300), np.abs(est - want).max()
+    assert (est[3::5] == 0).all()
+    true_out = np.clip(1. - (post * inside).sum(1), 0., 1.)
+    low = est <= 1e-3
+    print('judge: frames estimated <= 1e-3: %d of %d; their true outside mass max %.2e; correlation of logs %.3f'
+          % (low.sum(), D, true_out[low].max() if low.any() else 0.,
+             np.corrcoef(np.log(est[est > 0] + 1e-12), np.log(true_out[est > 0] + 1e-12))[0, 1]), flush=True)
+    assert low.any() and (true_out[low] <= 1e-2).all()
+
+
+@pytest.mark.parametrize('g32', [False, True])
+def test_graph_iteration_equals_eager(tiny_scaled, g32):
+    """The refinement-subset iteration run as a captured HIP graph (option graph) against the
+    same iterations run eagerly on a second engine: three iterations chained through
+    update_model and normalize_scale (the third replays the graph of the second, reading the
+    updated model and scales, with eager work and synchronisation in between), then new
+    refinement lists (the graph is captured again). Per-frame and per-orientation results equal; the merged volumes to the
+    order of their atomic additions."""
+    p = tiny_scaled
+    p = dict(p, fparams=dict(p['fparams'], scale=np.asarray(p['true_scale'], 'f8').copy()))
+    qc, qmap, off, cells = _refine_setup(p)
+    _, _, off2, cells2 = _refine_setup(p, seed=21, ncells=16)
+    engs = [P.engine_for(p), P.engine_for(p)]
+    for eng in engs:
+        if g32:
+            eng.set_precision(3)
+        eng.set_refine(len(qc), qmap, off, cells)
+    engs[1].set_option('graph', 1)
+    tol = 1e-5 if g32 else 1e-13
+    for it in (1, 2, 3, 4):
+        if it == 4:
+            for eng in engs:
+                eng.set_refine(len(qc), qmap, off2, cells2)
+        flags = dict(iteration=it, need_scaling=1, update_scale=1)
+        print('case: graph iteration %d (g32 %s)' % (it, g32), flush=True)
+        a, b = (P.run_engine(eng, flags) for eng in engs)
+        for k in ('u', 'max_exp', 'p_norm', 'rmax', 'likelihood', 'info', 'psum_d', 'psum_r', 'scale', 'prob'):
+            assert np.array_equal(a[k], b[k]), (it, k)
+        for k in ('avg_likelihood', 'mutual_info', 'rescale'):
+            assert a[k] == b[k], (it, k)
+        for k in ('model2', 'inter_weight'):
+            assert P.rel_err(a[k], b[k]) <= tol, (it, k, P.rel_err(a[k], b[k]))
+        rms = [eng.update_model(0.) for eng in engs]
+        assert abs(rms[0] - rms[1]) <= tol * max(rms[0], 1e-300)
+        for eng in engs:
+            eng.normalize_scale()
+        # the same input for the next iteration on both (the merges differ in their last bits)
+        engs[1].set_model(engs[0].get('model1'))
+        engs[1].set_scale(engs[0].get('scale'))
+    # the graph option refuses the two-pass switches
+    engs[1].set_option('estep_only', 1)
+    with pytest.raises(Exception):
+        engs[1].maximize(5, 1, 1, 0)
+    for eng in engs:
+        eng.close()
+
+
+@pytest.mark.parametrize('g32, chunked, graph', [(False, False, False), (True, False, False), (False, True, False),
+                                                 (True, True, True)])
+def test_fused_expand_likelihood_equals_separate(tiny_scaled, monkeypatch, g32, chunked, graph):
+    """The persistent expand + pair likelihood kernel (option fused) against the separate
+    expand and likelihood launches on the refinement subsets (partial and full frames), in one
+    or in 64-slot chunks, eagerly or as a captured graph: the same per-frame and
+    per-orientation results bit for bit (the same arithmetic in the same order), the merged
+    volumes to the order of their atomic additions."""
+    if chunked:
+        monkeypatch.setenv('EMC_MAX_CHUNK_COLS', '64')
+    p = tiny_scaled
+    p = dict(p, fparams=dict(p['fparams'], scale=np.asarray(p['true_scale'], 'f8').copy()))
+    qc, qmap, off, cells = _refine_setup(p, seed=3, ncells=12)
+    flags = dict(iteration=2, need_scaling=1, update_scale=1)
+    out = []
+    for fused in (0, 1):
+        eng = P.engine_for(p)
+        if g32:
+            eng.set_precision(3)
+        eng.set_refine(len(qc), qmap, off, cells)
+        eng.set_option('fused', fused)
+        if fused and graph:
+            eng.set_option('graph', 1)
+        g = P.run_engine(eng, flags)
+        if fused and graph:
+            # a replay of the captured graph from the same input gives the same again
+            eng.set_scale(p['fparams']['scale'])
+            g2 = P.run_engine(eng, flags)
+            assert np.array_equal(g2['max_exp'], g['max_exp']) and np.array_equal(g2['prob'], g['prob'])
+        out.append(g)
+        eng.close()
+    a, b = out
+    assert a['num_chunks'] == b['num_chunks'] and (a['num_chunks'] > 1) == chunked
+    for k in ('u', 'max_exp', 'p_norm', 'rmax', 'likelihood', 'info', 'psum_d', 'psum_r', 'scale', 'prob'):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ('model2', 'inter_weight'):
+        assert P.rel_err(a[k], b[k]) <= (1e-5 if g32 else 1e-13), k
