@@ -355,12 +355,14 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
     denominator and the reference model. (3) The network (polar photon histogram -> coarse-cell
     logits) is trained on 90% of the frames. (4) Iteration k + 1 from the same input with
     proposals, timed: a full search over the 3240 coarse orientations (expectation step only),
-    each frame's proposal (the network's top-k cells joined with the coarse search's top cells),
-    pass 1 over the proposals, the likelihood judge (mass outside each proposal estimated from
-    the coarse search against the fine maximum inside it; frames above 1 - judge_threshold fall
-    back to the full fine search) and the M-step (resumed from pass 1 when no frame falls back,
-    else a second pass), with FP32 gathers, and the model update. (5) The same with the
-    no-network proposals: iteration k's top-k cells in place of the network's."""
+    each frame's proposal (method 'learned': the network's top-k cells joined with iteration k's
+    top-k cells and the coarse search's top cells), pass 1 over the proposals, the likelihood
+    judge (mass outside each proposal estimated from the coarse search against the fine maximum
+    inside it; frames above 1 - judge_threshold fall back to the full fine search) and the
+    M-step (resumed from pass 1 when no frame falls back, else a second pass), with FP32
+    gathers, and the model update. (5) The same with the other proposal sources beside it:
+    'previous_posterior' (iteration k's cells, no network), 'network_only' (no posterior of an
+    earlier fine iteration) and 'coarse_only' (the coarse search's own top-k cells: neither)."""
     import torch
     from dragonfly_amd import _lib, proposal
     det, mask = prob['det'], prob['det']['mask']
@@ -436,24 +438,32 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
     recon, itr = recon_for(cfg['num_div'], precision=args.precision)
     start_k1(recon, itr)
 
-    def sources():
-        return {'network': lambda: net.propose(X, D, k_top),
-                'previous_posterior': lambda: prev_top}
+    def sources(source):
+        """The proposal arrays of a method (the coarse search's top cells are added to each)."""
+        if source == 'learned':
+            return [net.propose(X, D, k_top), prev_top]
+        if source == 'network_only':
+            return [net.propose(X, D, k_top)]
+        if source == 'previous_posterior':
+            return [prev_top]
+        return [judge.top_cells(k_top)]  # coarse_only: the coarse search's own top cells
 
     def one_step(source):
         """Judge, proposals, both passes and the model update; returns (seconds, info)."""
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         judge.estep(model_k1, scale_k1, beta_k1, kk + 1, cfg['need_scaling'])
-        top = sources()[source]()
-        tops = [top, judge.top_cells(k_coarse)] if k_coarse > 0 else [top]
+        tops = sources(source)
+        if k_coarse > 0 and source != 'coarse_only':
+            tops.append(judge.top_cells(k_coarse))
         info = proposal.judged_iteration(recon, itr, judge, tops, cfg['coarse_div'], tau)
         torch.cuda.synchronize()
-        info['top'] = top
         return time.perf_counter() - t0, info
 
     results = {}
-    for source, nsteps, nwarm in (('network', timed_steps, warmup), ('previous_posterior', args.baseline_steps, 1)):
+    methods = [('learned', timed_steps, warmup)] + [(m, args.baseline_steps, 1) for m in
+                                                   ('previous_posterior', 'network_only', 'coarse_only')]
+    for source, nsteps, nwarm in methods:
         for _ in range(max(nwarm, 1)):
             start_k1(recon, itr)
             one_step(source)
@@ -507,7 +517,7 @@ def bench_learned(args):
     if not args.single_beta and late != 1.:
         other = learned_case(prob, cfg, 1., args, max(1, min(args.steps, 3)), 1)
     D, R = head['D'], head['R']
-    net = head['results']['network']
+    net = head['results']['learned']
     step_s = net['step_ms'] * 1e-3
     cpu = None
     if not args.no_cpu:
@@ -539,12 +549,15 @@ def bench_learned(args):
         'config': {'workload': '%s: %d frames (mean 200 photons, scale U(0.5, 1.5)), 251^3 model, 260x257 '
                                'detector, fine num_div %d (%d orientations), coarse num_div %d (%d); iteration '
                                '%d from %d plain iterations started at the true intensity; proposals = top-%d '
-                               'cells of the network + top-%d of a coarse full search; likelihood judge (fall '
-                               'back to the full search where the estimated mass outside exceeds %.4g); '
+                               'cells of the network + top-%d of the previous iteration\'s posterior + top-%d of a '
+                               'coarse full search; likelihood judge (fall back to the full search where the '
+                               'estimated mass outside exceeds %.4g); '
                                'beta_start x %g (timed); beta_start x 1 beside it'
                                % (args.config, D, cfg['num_div'], R, cfg['coarse_div'], head['num_coarse'],
-                                  head['k'] + 1, head['k'], cfg['top_k'], args.coarse_k, 1. - head['tau'], late),
-                   'path': 'learned proposal + likelihood judge + sparse refinement', 'precision': args.precision,
+                                  head['k'] + 1, head['k'], cfg['top_k'], cfg['top_k'], args.coarse_k,
+                                  1. - head['tau'], late),
+                   'path': 'learned proposal (+ previous posterior) + likelihood judge + sparse refinement',
+                   'precision': args.precision,
                    'frames': D, 'num_rot': R},
         'speedup_vs_fp64': net['speedup_vs_fp64'], 'fallback_rate': net['fallback_rate'],
         'late_beta': summary(head),

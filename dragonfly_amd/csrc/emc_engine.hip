@@ -316,10 +316,7 @@ struct emc_engine {
     int mstep_only = 0;   // subset path: resume after an estep_only iteration with the M-step
     int graph_mode = 0;   // subset path: run the iteration as a captured HIP graph
     int fused = 0;        // subset path: persistent expand + pair likelihood kernel
-    DevBuf<long> fused_items;  // work-item offsets of every chunk
-    DevBuf<unsigned> fused_counter;
-    DevBuf<int> fused_done, fused_err;
-    std::vector<long> fused_base, fused_total, fused_key;
+    DevBuf<unsigned> fused_counter;  // the next tile of the persistent kernel
     hipGraphExec_t graph_exec = nullptr;
     std::vector<long> graph_key;  // what the graph was captured for (buffers, sizes, options)
     unsigned long layout_gen = 0;  // bumped by every setter that changes sizes, lists or options
@@ -1233,64 +1230,27 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     };
     const uint8_t *dense = e->any_dense ? e->dense.p : nullptr;
     // expectation step: expand, likelihood, per-frame maximum, normaliser and posterior, psum
-    // persistent expand + likelihood: work-item offsets of every chunk, built once per set of
-    // lists and chunk width (no upload inside the iteration, which a graph may hold)
     const bool fused = e->fused != 0;
-    if (fused) {
-        const std::vector<long> fkey = {(long)e->layout_gen, C, S, (long)e->num_pairs};
-        if (fkey != e->fused_key) {
-            const int ng = (P + kPixPerGroup - 1) / kPixPerGroup;
-            std::vector<long> items;
-            e->fused_base.clear();
-            e->fused_total.clear();
-            int maxt = 0;
-            for (int s0 = 0; s0 < S; s0 += C) {
-                const int Cc = std::min(C, S - s0);
-                const int t0 = s0 / emc::kRTile, nt = Cc / emc::kRTile;
-                maxt = std::max(maxt, nt);
-                e->fused_base.push_back((long)items.size());
-                long acc = 0;
-                items.push_back(0);
-                for (int k = 0; k <= nt; ++k) {
-                    if (k < nt) acc += ng;
-                    if (k >= 1) {
-                        const long np = e->h_toff[t0 + k] - e->h_toff[t0 + k - 1];
-                        acc += (np + emc::kFusedPairBatch - 1) / emc::kFusedPairBatch;
-                    }
-                    items.push_back(acc);
-                }
-                e->fused_total.push_back(acc);
-            }
-            if ((rc = e->fused_items.upload(items.data(), items.size(), s))) return rc;
-            if ((rc = e->fused_done.alloc((size_t)maxt + 1))) return rc;
-            if ((rc = e->fused_counter.alloc(1))) return rc;
-            if ((rc = e->fused_err.alloc(1))) return rc;
-            emc::launch_fill_int(s, e->fused_err.p, 1, 0);
-            e->fused_key = fkey;
-        }
-    }
+    if (fused && (rc = e->fused_counter.alloc(1))) return rc;
     auto estep_work = [&]() -> int {
             for (int s0 = 0; s0 < S; s0 += C) {
                 const int Cc = std::min(C, S - s0);
                 if (fused) {
                     // expand and pair likelihood in one persistent launch (timed as likelihood)
                     if ((rc = timed(1, [&] {
-                             const int t0 = s0 / emc::kRTile, nt = Cc / emc::kRTile, ci = s0 / C;
-                             const long *items = e->fused_items.p + e->fused_base[ci];
+                             const int t0 = s0 / emc::kRTile;
                              if (g32)
                                  emc::launch_fused_expand_likelihood(
                                      s, e->rots_slot.p, e->model32.p, e->size, e->qvals.p, e->corr.p, e->mask.p, P, s0,
                                      Cc, kPixPerGroup, e->logw32.p, e->upart.p, e->ubad.p, e->slot_rot.p, e->lik_off.p,
                                      e->lik_pix.p, e->lik_cnt.p, e->pair_frame.p, e->pair_tile.p, e->tile_off.p,
-                                     e->tile_pair.p, t0, items, nt + 1, e->fused_total[ci], e->fused_counter.p,
-                                     e->fused_done.p, e->fused_err.p, e->Rsub.p);
+                                     e->tile_pair.p, t0, e->fused_counter.p, e->Rsub.p);
                              else
                                  emc::launch_fused_expand_likelihood(
                                      s, e->rots_slot.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p, P, s0,
                                      Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p, e->slot_rot.p, e->lik_off.p,
                                      e->lik_pix.p, e->lik_cnt.p, e->pair_frame.p, e->pair_tile.p, e->tile_off.p,
-                                     e->tile_pair.p, t0, items, nt + 1, e->fused_total[ci], e->fused_counter.p,
-                                     e->fused_done.p, e->fused_err.p, e->Rsub.p);
+                                     e->tile_pair.p, t0, e->fused_counter.p, e->Rsub.p);
                              emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, s0, S, e->u_slot.p,
                                                   e->rbad_slot.p);
                              if (nF > 0) {
@@ -1473,14 +1433,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     auto finish = [&]() -> int {
         double host_scalars[3];
         HIP_TRY(hipMemcpyAsync(host_scalars, e->scalars.p, sizeof host_scalars, hipMemcpyDeviceToHost, s));
-        int gave_up = 0;
-        if (fused) HIP_TRY(hipMemcpyAsync(&gave_up, e->fused_err.p, sizeof gave_up, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        if (gave_up) {
-            emc::launch_fill_int(s, e->fused_err.p, 1, 0);
-            return fail(EMC_E_HIP, "emc_maximize: a likelihood batch of the persistent kernel gave up waiting for "
-                                   "its tile's expand; the results of this iteration are not valid");
-        }
         e->have_results = true;
         e->subset_ran = true;
         if (stats) {

@@ -64,24 +64,21 @@ void launch_subset_mass(hipStream_t s, const double *R, int num_data, const long
 // Top-k columns (descending, ties to the lower index; k <= 64, n <= 8192) of every row of a
 // non-negative FP32 matrix, with the sum of the selected entries (emc_nn.hip).
 void launch_prob_topk(hipStream_t s, const float *Z, size_t ldz, int rows, int n, int k, int *idx, float *mass);
-// Persistent expand + pair likelihood of one orientation chunk (emc_subset.hip): item_off
-// (nsteps + 1 offsets, nsteps = chunk tiles + 1) orders the work items; counter, done (one per
-// chunk tile) and err are device scratch; err[0] becomes 1 if a batch ever gave up waiting.
+// Persistent expand + pair likelihood of one orientation chunk of C columns (a multiple of
+// 64) starting at slot r0 = 64 t0 (emc_subset.hip): blocks take whole tiles from a counter
+// (device scratch, reset by the launch), expand them and evaluate their pairs.
 void launch_fused_expand_likelihood(hipStream_t s, const double *rots, const double *model, long size,
                                     const double *qvals, const double *corr, const uint8_t *mask, int num_pix, int r0,
                                     int C, int pix_per_group, double *logw, double *upart, int *ubad,
                                     const int *slot_rot, const long *off, const int *pix, const double *cnt,
                                     const int *pair_frame, const int *pair_tile, const long *tile_off,
-                                    const int *tile_pair, int t0, const long *item_off, int nsteps, long total_items,
-                                    unsigned *counter, int *done, int *err, double *G);
+                                    const int *tile_pair, int t0, unsigned *counter, double *G);
 void launch_fused_expand_likelihood(hipStream_t s, const double *rots, const float *model, long size,
                                     const double *qvals, const double *corr, const uint8_t *mask, int num_pix, int r0,
                                     int C, int pix_per_group, float *logw, double *upart, int *ubad,
                                     const int *slot_rot, const long *off, const int *pix, const double *cnt,
                                     const int *pair_frame, const int *pair_tile, const long *tile_off,
-                                    const int *tile_pair, int t0, const long *item_off, int nsteps, long total_items,
-                                    unsigned *counter, int *done, int *err, double *G);
-constexpr int kFusedPairBatch = 64;  // pairs per likelihood item of the persistent kernel
+                                    const int *tile_pair, int t0, unsigned *counter, double *G);
 void launch_judge_outside(hipStream_t s, const double *Pc, int num_data, int num_coarse, const double *Mc,
                           const double *pnc, const double *Mf, const double *pnf, const double *beta,
                           const uint8_t *skip, const uint32_t *bits, int words, const double *log_n,
@@ -164,7 +161,6 @@ void launch_frame_stats(hipStream_t s, const double *likelihood, const double *i
 void launch_scale(hipStream_t s, const int *fcounts, const double *psum_d, const uint8_t *blacklist,
                   int num_data, double *scale);
 void launch_fill(hipStream_t s, double *a, size_t n, double v);
-void launch_fill_int(hipStream_t s, int *a, size_t n, int v);
 void launch_update_model(hipStream_t s, double *model1, double *model2, const double *w, long size,
                          double alpha, double *part, int nparts, double *out_change, int modes = 1);
 void launch_normalize_scale(hipStream_t s, double *scale, const uint8_t *blacklist, int num_data,

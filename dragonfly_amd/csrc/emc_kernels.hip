@@ -1560,16 +1560,6 @@ void launch_scale(hipStream_t s, const int *fcounts, const double *psum_d, const
     scale_kernel<<<cdiv(num_data, 256), 256, 0, s>>>(fcounts, psum_d, blacklist, num_data, scale);
 }
 
-__global__ void fill_int_kernel(int *__restrict__ a, size_t n, int v) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        a[i] = v;
-}
-
-void launch_fill_int(hipStream_t s, int *a, size_t n, int v) {
-    if (n == 0) return;
-    fill_int_kernel<<<grid_cap(cdiv(n, 256)), 256, 0, s>>>(a, n, v);
-}
-
 void launch_fill(hipStream_t s, double *a, size_t n, double v) {
     fill_kernel<<<grid_cap(cdiv(n, 256)), 256, 0, s>>>(a, n, v);
 }

@@ -530,24 +530,14 @@ void launch_cell_posterior(hipStream_t s, double *R, int plain, int num_data, in
 
 // ---- persistent expand + likelihood ------------------------------------------------------
 // One launch per orientation chunk does the chunk's expand and its (frame, tile) pair
-// likelihoods. Work items are handed out in order from a counter: step k holds the expand
-// pieces (one per pixel group) of local tile k and then the pair batches of tile k - 1, so a
-// tile's expand is under way one step before its pairs are taken. A pair batch waits until
-// every expand piece of its tile has finished (per-tile counter, release / acquire at agent
-// scope) and then reads that tile's log W while it is recent in L2 / the Infinity Cache. Items
-// are taken by resident blocks only and an expand piece waits on nothing, so every item a
-// batch waits for is already running; the wait is bounded all the same (err[0] = 1 and the
-// batch is skipped) so that no wave can spin for ever. Batches of frames with many photons
-// take longer than others; blocks that finish early take the next items, which keeps the
-// uneven frames of a chunk load-balanced. The arithmetic is expand_block's and
-// pair_likelihood_kernel's, in the same order.
-constexpr long kFusedMaxSpin = 1L << 24;
-
-__global__ void fused_reset_kernel(unsigned *__restrict__ counter, int *__restrict__ done, int ntiles) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *counter = 0u;
-    if (i < ntiles) done[i] = 0;
-}
+// likelihoods. Each block takes whole 64-orientation tiles from a counter, one at a time:
+// it expands the tile over every pixel group into log W (expand_block) and then, with its
+// waves, evaluates every pair of that tile from the log W it has just written, while that is
+// still in L2 / the Infinity Cache. A tile's pairs can be many (frames of up to 20,000 photons
+// with subsets that favour some tiles) or few; blocks that finish early take the next tile, so
+// the uneven work of a chunk stays balanced without any block waiting on another. The
+// arithmetic is expand_block's and pair_likelihood_kernel's, in the same order.
+__global__ void fused_reset_kernel(unsigned *__restrict__ counter) { *counter = 0u; }
 
 template <int U, typename T, int kW, typename MT>
 __global__ __launch_bounds__(kBlock) void fused_expand_likelihood_kernel(
@@ -556,69 +546,31 @@ __global__ __launch_bounds__(kBlock) void fused_expand_likelihood_kernel(
     int ngroups, T *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad,
     const int *__restrict__ slot_rot, const long *__restrict__ off, const int *__restrict__ pix,
     const double *__restrict__ cnt, const int *__restrict__ pair_frame, const int *__restrict__ pair_tile,
-    const long *__restrict__ tile_off, const int *__restrict__ tile_pair, int t0, const long *__restrict__ item_off,
-    int nsteps, unsigned *__restrict__ counter, int *__restrict__ done, int *__restrict__ err, double *__restrict__ G) {
+    const long *__restrict__ tile_off, const int *__restrict__ tile_pair, int t0, int ntiles,
+    unsigned *__restrict__ counter, double *__restrict__ G) {
     constexpr int kSub = kW / kRTile;
     __shared__ ExpandShared sh;
-    __shared__ long s_item;
-    __shared__ int s_go;
+    __shared__ int s_tile;
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
-    const long total = item_off[nsteps];
     const size_t tsW = (size_t)num_pix * kW;
     for (;;) {
-        if (tid == 0) s_item = (long)atomicAdd(counter, 1u);
+        if (tid == 0) s_tile = (int)atomicAdd(counter, 1u);
         __syncthreads();
-        const long item = s_item;
+        const int lt = s_tile;  // local tile of the chunk
         __syncthreads();
-        if (item >= total) break;  // every block leaves here once the items run out
-        // step k: [ngroups expand pieces of tile k (k < number of tiles)] [batches of tile k - 1]
-        int lo = 0, hi = nsteps;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (item_off[mid] <= item) lo = mid;
-            else hi = mid;
-        }
-        const int k = lo;
-        const long within = item - item_off[k];
-        const int ntiles = nsteps - 1;
-        const int nexp = k < ntiles ? ngroups : 0;
-        if (within < nexp) {
-            expand_block<T, kW, MT>(sh, k * kRTile, (int)within, rots, model, size, qvals, corr, mask, num_pix, r0, C,
+        if (lt >= ntiles) break;  // every block leaves here once the tiles run out
+        for (int g = 0; g < ngroups; ++g)
+            expand_block<T, kW, MT>(sh, lt * kRTile, g, rots, model, size, qvals, corr, mask, num_pix, r0, C,
                                     pix_per_group, logw, upart, ubad, slot_rot, 1, 0);
-            __threadfence();  // this thread's log W and u stores reach the device scope
-            __syncthreads();
-            if (tid == 0) __hip_atomic_fetch_add(&done[k], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            continue;
-        }
-        const int lt = k - 1;  // the batch's tile (local to the chunk)
-        const long b = within - nexp;
-        if (tid == 0) {
-            long spins = 0;
-            int go = 1;
-            while (__hip_atomic_load(&done[lt], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < ngroups) {
-                if (++spins > kFusedMaxSpin) {
-                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    go = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-            s_go = go;
-        }
+        __threadfence();  // the tile's log W, written by every wave, is read below by every wave
         __syncthreads();
-        const int go = s_go;
-        __syncthreads();
-        if (!go) continue;
-        __threadfence();  // acquire side for every wave of the block
-        const long j0 = tile_off[t0 + lt] + b * kFusedPairBatch;
-        const long j1 = min(tile_off[t0 + lt + 1], j0 + kFusedPairBatch);
-        for (long j = j0 + wave; j < j1; j += kWavesPerBlock) {
+        const long j1 = tile_off[t0 + lt + 1];
+        for (long j = tile_off[t0 + lt] + wave; j < j1; j += kWavesPerBlock) {
             const int kp = tile_pair[j];
             const int d = pair_frame[kp];
-            const int lt2 = pair_tile[kp] - t0;
-            const T *__restrict__ col = logw + (size_t)(lt2 / kSub) * tsW + (lt2 % kSub) * kRTile + lane;
+            const T *__restrict__ col = logw + (size_t)(lt / kSub) * tsW + (lt % kSub) * kRTile + lane;
             long q = off[d];
             const long q1 = off[d + 1];
             double acc = 0.;
@@ -640,11 +592,10 @@ static void fused_launch(hipStream_t s, const double *rots, const MT *model, lon
                          const double *corr, const uint8_t *mask, int num_pix, int r0, int C, int pix_per_group,
                          T *logw, double *upart, int *ubad, const int *slot_rot, const long *off, const int *pix,
                          const double *cnt, const int *pair_frame, const int *pair_tile, const long *tile_off,
-                         const int *tile_pair, int t0, const long *item_off, int nsteps, long total_items,
-                         unsigned *counter, int *done, int *err, double *G) {
-    const int ntiles = nsteps - 1;
-    fused_reset_kernel<<<(ntiles + 255) / 256 + 1, 256, 0, s>>>(counter, done, ntiles);
-    if (total_items <= 0) return;
+                         const int *tile_pair, int t0, unsigned *counter, double *G) {
+    const int ntiles = C / kRTile;
+    if (ntiles <= 0) return;
+    fused_reset_kernel<<<1, 1, 0, s>>>(counter);
     static int resident = 0;  // blocks of this kernel that fit the device at once
     if (resident == 0) {
         int dev = 0, cus = 0, per_cu = 0;
@@ -654,11 +605,11 @@ static void fused_launch(hipStream_t s, const double *rots, const MT *model, lon
                                                            kBlock, 0);
         resident = std::max(1, cus * std::max(per_cu, 1));
     }
-    const unsigned grid = (unsigned)std::min<long>(total_items, resident);
+    const unsigned grid = (unsigned)std::min(ntiles, resident);
     const int ngroups = (num_pix + pix_per_group - 1) / pix_per_group;
     fused_expand_likelihood_kernel<8, T, kW, MT><<<grid, kBlock, 0, s>>>(
         rots, model, size, qvals, corr, mask, num_pix, r0, C, pix_per_group, ngroups, logw, upart, ubad, slot_rot, off,
-        pix, cnt, pair_frame, pair_tile, tile_off, tile_pair, t0, item_off, nsteps, counter, done, err, G);
+        pix, cnt, pair_frame, pair_tile, tile_off, tile_pair, t0, ntiles, counter, G);
 }
 
 void launch_fused_expand_likelihood(hipStream_t s, const double *rots, const double *model, long size,
@@ -666,11 +617,10 @@ void launch_fused_expand_likelihood(hipStream_t s, const double *rots, const dou
                                     int C, int pix_per_group, double *logw, double *upart, int *ubad,
                                     const int *slot_rot, const long *off, const int *pix, const double *cnt,
                                     const int *pair_frame, const int *pair_tile, const long *tile_off,
-                                    const int *tile_pair, int t0, const long *item_off, int nsteps, long total_items,
-                                    unsigned *counter, int *done, int *err, double *G) {
+                                    const int *tile_pair, int t0, unsigned *counter, double *G) {
     fused_launch<double, kRTile, double>(s, rots, model, size, qvals, corr, mask, num_pix, r0, C, pix_per_group, logw,
                                          upart, ubad, slot_rot, off, pix, cnt, pair_frame, pair_tile, tile_off,
-                                         tile_pair, t0, item_off, nsteps, total_items, counter, done, err, G);
+                                         tile_pair, t0, counter, G);
 }
 
 void launch_fused_expand_likelihood(hipStream_t s, const double *rots, const float *model, long size,
@@ -678,11 +628,10 @@ void launch_fused_expand_likelihood(hipStream_t s, const double *rots, const flo
                                     int C, int pix_per_group, float *logw, double *upart, int *ubad,
                                     const int *slot_rot, const long *off, const int *pix, const double *cnt,
                                     const int *pair_frame, const int *pair_tile, const long *tile_off,
-                                    const int *tile_pair, int t0, const long *item_off, int nsteps, long total_items,
-                                    unsigned *counter, int *done, int *err, double *G) {
+                                    const int *tile_pair, int t0, unsigned *counter, double *G) {
     fused_launch<float, 2 * kRTile, float>(s, rots, model, size, qvals, corr, mask, num_pix, r0, C, pix_per_group,
                                            logw, upart, ubad, slot_rot, off, pix, cnt, pair_frame, pair_tile, tile_off,
-                                           tile_pair, t0, item_off, nsteps, total_items, counter, done, err, G);
+                                           tile_pair, t0, counter, G);
 }
 
 void launch_pair_likelihood(hipStream_t s, const double *logw, int num_pix, const long *off, const int *pix,
