@@ -436,6 +436,8 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
     judge_itr.model.model1 = model_k1
     judge = proposal.CoarseJudge(judge_itr, cfg['coarse_div'], precision=args.judge_precision)
     recon, itr = recon_for(cfg['num_div'], precision=args.precision)
+    if args.fused:
+        recon.engine.set_option('fused', 1)
     start_k1(recon, itr)
 
     def sources(source):
@@ -453,16 +455,20 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         judge.estep(model_k1, scale_k1, beta_k1, kk + 1, cfg['need_scaling'])
+        t1 = time.perf_counter()
         tops = sources(source)
         if k_coarse > 0 and source != 'coarse_only':
             tops.append(judge.top_cells(k_coarse))
+        t2 = time.perf_counter()
         info = proposal.judged_iteration(recon, itr, judge, tops, cfg['coarse_div'], tau)
         torch.cuda.synchronize()
+        info['phase_ms'] = dict(coarse_search=round((t1 - t0) * 1e3, 1), proposals=round((t2 - t1) * 1e3, 1),
+                                **info['phase_ms'])
         return time.perf_counter() - t0, info
 
     results = {}
-    methods = [('learned', timed_steps, warmup)] + [(m, args.baseline_steps, 1) for m in
-                                                   ('previous_posterior', 'network_only', 'coarse_only')]
+    others = [m for m in ('previous_posterior', 'network_only', 'coarse_only') if m in args.methods.split(',')]
+    methods = [('learned', timed_steps, warmup)] + [(m, args.baseline_steps, 1) for m in others]
     for source, nsteps, nwarm in methods:
         for _ in range(max(nwarm, 1)):
             start_k1(recon, itr)
@@ -486,7 +492,8 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
                    model_rms_rel_err=float('%.3g' % np.sqrt(np.mean((ml - model_plain) ** 2) / np.mean(model_plain ** 2))),
                    rmax_agree=round(float(np.mean(recon.rmax == rmax_plain)), 5),
                    stage_ms={k: round(v, 1) for k, v in recon.last_stats['stage_ms'].items()},
-                   pass1_stage_ms={k: round(v, 1) for k, v in info['estep_stage_ms'].items()})
+                   pass1_stage_ms={k: round(v, 1) for k, v in info['estep_stage_ms'].items()},
+                   phase_ms=info['phase_ms'])
         results[source] = res
         log('beta x %g, %s: %s' % (beta_factor, source, json.dumps(res)))
     judge_ms = judge.stats['stage_ms'] if judge.stats else {}
@@ -689,6 +696,10 @@ def main():
                          '(default: the configuration\'s mass_threshold, 0.99)')
     ap.add_argument('--coarse-k', type=int, default=4,
                     help='learned-proposal workload: cells of the coarse search joined to every proposal')
+    ap.add_argument('--methods', default='previous_posterior,network_only,coarse_only',
+                    help='learned-proposal workload: proposal sources timed beside the headline')
+    ap.add_argument('--fused', action='store_true',
+                    help='learned-proposal workload: expand and pair likelihood in one persistent launch')
     ap.add_argument('--converge-iters', type=int, default=3,
                     help='learned-proposal workload: plain iterations before the timed one (k)')
     ap.add_argument('--judge-precision', type=int, default=3,

@@ -14,6 +14,7 @@
 #include <atomic>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/emc_hip.h"
@@ -805,7 +806,6 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
     }
     const std::vector<int> &ct0 = e->h_ctile0, &ct1 = e->h_ctile1;
     const int T = e->num_tiles;
-    std::vector<int> tiles;
     std::vector<long> &poff = e->h_poff;
     std::vector<int> &pframe = e->h_pframe, &ptile = e->h_ptile, &ffull = e->h_ffull;
     std::vector<int> fidx;
@@ -813,35 +813,67 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
     pframe.clear();
     ptile.clear();
     ffull.assign(N, -1);
-    // the frames' distinct cells come from their bit rows (built by emc_set_refine)
+    // the frames' distinct cells come from their bit rows (built by emc_set_refine); ranges of
+    // frames are listed by host threads, then joined in frame order
     const int words = e->rel_words;
     const std::vector<uint32_t> &bits = e->h_relbits;
-    std::vector<int> cells;
-    for (int d = 0; d < N; ++d) {
-        cells.clear();
-        const uint32_t *row = &bits[(size_t)d * words];
-        int n = 0;
-        for (int w = 0; w < words; ++w) n += __builtin_popcount(row[w]);
-        if (n < num_coarse)
+    const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int nparts_h = std::max(1, std::min(nthreads, N / 4096));
+    struct Part {
+        std::vector<int> frame, tile, full;
+        std::vector<long> count;  // pairs of each frame of the range
+    };
+    std::vector<Part> parts(nparts_h);
+    auto list_range = [&](int pi) {
+        Part &pt = parts[pi];
+        const int d0 = (int)((long)N * pi / nparts_h), d1 = (int)((long)N * (pi + 1) / nparts_h);
+        pt.count.assign(d1 - d0, 0);
+        std::vector<int> cells, tl;
+        for (int d = d0; d < d1; ++d) {
+            cells.clear();
+            const uint32_t *row = &bits[(size_t)d * words];
+            int n = 0;
+            for (int w = 0; w < words; ++w) n += __builtin_popcount(row[w]);
+            if (n == num_coarse) {
+                pt.full.push_back(d);
+                continue;
+            }
             for (int w = 0; w < words; ++w)
                 for (uint32_t x = row[w]; x; x &= x - 1) cells.push_back(w * 32 + __builtin_ctz(x));
-        if (n == num_coarse) {
-            ffull[d] = (int)fidx.size();
-            fidx.push_back(d);
-        } else {
             // the tiles holding the frame's cells, ascending and distinct (with packed cells
             // neighbouring cells share tiles)
-            tiles.clear();
+            tl.clear();
             for (int c : cells)
-                for (int t = ct0[c]; t <= ct1[c]; ++t) tiles.push_back(t);
-            std::sort(tiles.begin(), tiles.end());
-            tiles.erase(std::unique(tiles.begin(), tiles.end()), tiles.end());
-            for (int t : tiles) {
-                pframe.push_back(d);
-                ptile.push_back(t);
+                for (int t = ct0[c]; t <= ct1[c]; ++t) tl.push_back(t);
+            std::sort(tl.begin(), tl.end());
+            tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
+            for (int t : tl) {
+                pt.frame.push_back(d);
+                pt.tile.push_back(t);
             }
+            pt.count[d - d0] = (long)tl.size();
         }
-        poff[d + 1] = (long)pframe.size();
+    };
+    {
+        std::vector<std::thread> pool;
+        for (int pi = 1; pi < nparts_h; ++pi) pool.emplace_back(list_range, pi);
+        list_range(0);
+        for (auto &th : pool) th.join();
+    }
+    size_t total_pairs = 0;
+    for (const Part &pt : parts) total_pairs += pt.frame.size();
+    pframe.reserve(total_pairs);
+    ptile.reserve(total_pairs);
+    for (int pi = 0; pi < nparts_h; ++pi) {
+        const Part &pt = parts[pi];
+        const int d0 = (int)((long)N * pi / nparts_h);
+        for (size_t i = 0; i < pt.count.size(); ++i) poff[d0 + i + 1] = poff[d0 + i] + pt.count[i];
+        pframe.insert(pframe.end(), pt.frame.begin(), pt.frame.end());
+        ptile.insert(ptile.end(), pt.tile.begin(), pt.tile.end());
+        for (int d : pt.full) {
+            ffull[d] = (int)fidx.size();
+            fidx.push_back(d);
+        }
     }
     const long K = (long)pframe.size();
     if (K >= INT32_MAX) return fail(EMC_E_ARG, "emc_set_refine: too many (frame, tile) pairs");

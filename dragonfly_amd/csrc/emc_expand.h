@@ -12,33 +12,34 @@ namespace emc {
 
 // Lanes along pixels (neighbouring pixels hit neighbouring voxels), transpose through LDS,
 // store with lanes along orientations (coalesced rows of logW). u_r (max_prob.c:36-38) sums W
-// over mask < 1.
-__device__ __forceinline__ double slice_value_model(const double *__restrict__ model, long size, const double *rot,
-                                                    double q0, double q1, double q2, double corr, double center) {
-    return slice_value_flat(model, size, rot, q0, q1, q2, corr, center);
-}
-// the FP32 model is the brick copy (copy_f32_brick_kernel)
-__device__ __forceinline__ double slice_value_model(const float *__restrict__ model, long size, const double *rot,
-                                                    double q0, double q1, double q2, double corr, double center) {
-    return slice_value_f32b(model, size, rot, q0, q1, q2, corr, center);
-}
-// elements of one mode's volume: size^3 for the FP64 model, the padded brick grid for the FP32 copy
+// over mask < 1. The FP64 model is interpolated as slice_gen3d does (slice_value_flat); the
+// FP32 copy (halo bricks, copy_f32_brick_kernel) in FP32 from an FP32 rotation
+// (slice_value_f32h).
+// elements of one mode's volume: size^3 for the FP64 model, the brick grid for the FP32 copy
 __device__ __forceinline__ size_t mode_stride(const double *, long size) { return (size_t)(size * size * size); }
 __device__ __forceinline__ size_t mode_stride(const float *, long size) {
     const size_t nb = (size_t)model_bricks(size);
-    return nb * nb * nb * 64;
+    return nb * nb * nb * kHaloBrickFloats;
 }
-// log W as stored: FP64 log for an FP64 model; FP32 log of the FP32 interpolation otherwise
-// (values below FLT_MIN, i.e. DBL_MIN outside the grid, keep the FP64 log)
+// log(DBL_MIN) = -1022 ln 2: the stored log W of a pixel outside the grid
+constexpr float kLogDblMin = (float)(-1022.0 * 0.693147180559945309417232121458);
+// log W as stored: FP64 log for an FP64 model; for the FP32 copy the device's FP32 log of the
+// FP32 interpolation (W >= FLT_MIN), log(DBL_MIN) outside the grid, and the FP64 log of the
+// rest (0 gives -inf)
 template <typename T, typename MT>
 __device__ __forceinline__ T log_stored(double W) {
-    if (sizeof(MT) == sizeof(float)) return (T)(W >= (double)FLT_MIN ? logf((float)W) : (float)log(W));
+    if (sizeof(MT) == sizeof(float)) {
+        if (W >= (double)FLT_MIN) return (T)__logf((float)W);
+        if (W == DBL_MIN) return (T)kLogDblMin;
+        return (T)(float)log(W);
+    }
     return (T)log(W);
 }
 
 // LDS of one expand block
 struct ExpandShared {
     double rot[kRTile * 9];
+    float rotf[kRTile * 9];  // the same in FP32 (FP32 model copy)
     double tile[64][kRTile + 1];
     double u[kWavesPerBlock][kRTile];
     int bad[kWavesPerBlock][kRTile];
@@ -53,12 +54,18 @@ __device__ __forceinline__ void expand_block(
     const double *__restrict__ qvals, const double *__restrict__ corr, const uint8_t *__restrict__ mask, int num_pix,
     int r0, int C, int pix_per_group, T *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad,
     const int *__restrict__ slot_rot, int modes, int raw) {
+    constexpr bool f32 = sizeof(MT) == sizeof(float);
     const int tid = threadIdx.x;
-    for (int i = tid; i < kRTile * 9; i += kBlock) sh.rot[i] = rots[(size_t)(r0 + c0) * 9 + i];
+    for (int i = tid; i < kRTile * 9; i += kBlock) {
+        const double v = rots[(size_t)(r0 + c0) * 9 + i];
+        sh.rot[i] = v;
+        if (f32) sh.rotf[i] = (float)v;
+    }
     // padding slots of the subset order (slot_rot < 0) get W = 1 without interpolation
     if (tid < kRTile) sh.pad[tid] = slot_rot != nullptr && slot_rot[r0 + c0 + tid] < 0;
     __syncthreads();
     const double center = (double)(size / 2);
+    const int nb = model_bricks(size);
     const int lane = tid & 63, wave = tid >> 6;
     const int t_begin = g * pix_per_group;
     const int t_end = min(num_pix, t_begin + pix_per_group);
@@ -69,12 +76,17 @@ __device__ __forceinline__ void expand_block(
         if (t < t_end) {
             const double q0 = qvals[3 * (size_t)t], q1 = qvals[3 * (size_t)t + 1],
                          q2 = qvals[3 * (size_t)t + 2], cr = corr[t];
+            const float qf0 = (float)q0, qf1 = (float)q1, qf2 = (float)q2;
 #pragma unroll 4
             for (int k = 0; k < kRTile / kWavesPerBlock; ++k) {
                 const int rr = wave + kWavesPerBlock * k;
                 // combined orientation index r = rotation * modes + mode (max_prob.c:80-84)
                 const MT *mdl = modes > 1 ? model + (size_t)((r0 + c0 + rr) % modes) * mode_stride(model, size) : model;
-                const double v = slice_value_model(mdl, size, &sh.rot[rr * 9], q0, q1, q2, cr, center);
+                double v;
+                if constexpr (f32)
+                    v = slice_value_f32h(mdl, (int)size, nb, &sh.rotf[rr * 9], qf0, qf1, qf2, cr, (float)center);
+                else
+                    v = slice_value_flat(mdl, size, &sh.rot[rr * 9], q0, q1, q2, cr, center);
                 sh.tile[lane][rr] = sh.pad[rr] ? 1. : v;
             }
         }

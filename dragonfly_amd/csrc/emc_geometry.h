@@ -82,36 +82,40 @@ __device__ __forceinline__ double slice_value_flat(const double *__restrict__ mo
     return ok ? v * corr : DBL_MIN;
 }
 
-// FP32 model copy in 4 x 4 x 4 voxel bricks: brick (X/4, Y/4, Z/4) of an nb^3 grid of bricks
-// (nb = ceil(size / 4)) holds its 64 voxels contiguously, z fastest. A 128-byte cache line is
-// then a 2 x 4 x 4 block of voxels instead of 32 voxels along one z row, so the eight corners of
-// the cells a wavefront interpolates (a compact detector patch rotated into the volume, at any
-// orientation) fall in fewer lines.
+// FP32 model copy in bricks of 4 x 4 x 4 trilinear cells: brick (X/4, Y/4, Z/4) of an nb^3
+// grid (nb = (size - 2) / 4 + 1, the cells x <= size - 2 of the grid) holds the 5 x 5 x 5
+// corners of its cells (shared faces are stored twice), padded to 128 floats, z fastest. The
+// eight corners of any cell are then at fixed offsets {0, 1, 5, 6, 25, 26, 30, 31} from one
+// address, all within the brick's 512 bytes: one index computation per interpolation, and a
+// compact detector patch rotated into the volume at any orientation touches few cache lines.
 constexpr int kModelBrick = 4;
-__host__ __device__ __forceinline__ int model_bricks(long size) { return (int)((size + kModelBrick - 1) / kModelBrick); }
-__device__ __forceinline__ long brick_index(int nb, int x, int y, int z) {
-    return ((((long)(x >> 2) * nb + (y >> 2)) * nb + (z >> 2)) << 6) + (((x & 3) << 4) | ((y & 3) << 2) | (z & 3));
+constexpr int kHaloBrickFloats = 128;  // 5^3 = 125 corners, padded
+__host__ __device__ __forceinline__ int model_bricks(long size) { return (int)((size - 2) / kModelBrick + 1); }
+__device__ __forceinline__ long halo_index(int nb, int x, int y, int z) {
+    return (((long)(x >> 2) * nb + (y >> 2)) * nb + (z >> 2)) * kHaloBrickFloats + (x & 3) * 25 + (y & 3) * 5 + (z & 3);
 }
 
-// slice_value_flat on the FP32 brick copy: the rotated position, cell and fractions in FP64 (the
-// same cell as the FP64 path), the eight corner weights and the interpolation in FP32. DBL_MIN
-// outside the grid.
-__device__ __forceinline__ double slice_value_f32b(const float *__restrict__ model, long size,
-                                                   const double *rot, double q0, double q1, double q2,
-                                                   double corr, double center) {
-    Cell c{0, 0, 0, 0., 0., 0.};
-    const bool ok = locate(rot, q0, q1, q2, center, size, c);
-    const int nb = model_bricks(size);
-    const int x0 = ok ? c.x : 0, y0 = ok ? c.y : 0, z0 = ok ? c.z : 0;
-    // in the grid x0, y0, z0 <= size - 2, so the + 1 corners need no wrap
-    const float fx = (float)c.fx, fy = (float)c.fy, fz = (float)c.fz;
-    const float cx = 1.f - fx, cy = 1.f - fy, cz = 1.f - fz;
-    const float m000 = model[brick_index(nb, x0, y0, z0)], m001 = model[brick_index(nb, x0, y0, z0 + 1)];
-    const float m010 = model[brick_index(nb, x0, y0 + 1, z0)], m011 = model[brick_index(nb, x0, y0 + 1, z0 + 1)];
-    const float m100 = model[brick_index(nb, x0 + 1, y0, z0)], m101 = model[brick_index(nb, x0 + 1, y0, z0 + 1)];
-    const float m110 = model[brick_index(nb, x0 + 1, y0 + 1, z0)], m111 = model[brick_index(nb, x0 + 1, y0 + 1, z0 + 1)];
-    const float v = cx * (cy * (cz * m000 + fz * m001) + fy * (cz * m010 + fz * m011)) +
-                    fx * (cy * (cz * m100 + fz * m101) + fy * (cz * m110 + fz * m111));
+// Interpolated value of the FP32 brick copy at the rotated position of a pixel, in FP32: the
+// position R q + center, the truncated cell (toward zero, as model.c:31-33) and its fractions,
+// then the trilinear value as three rounds of linear interpolation (z, y, x), each a fused
+// multiply-add a + f (b - a). rot: the rotation as 9 floats; q: the pixel's q-vector in FP32.
+// DBL_MIN outside the grid (x, y or z outside [0, size - 2]).
+__device__ __forceinline__ double slice_value_f32h(const float *__restrict__ model, int size, int nb, const float *rot,
+                                                   float q0, float q1, float q2, double corr, float center) {
+    const float p0 = __builtin_fmaf(rot[0], q0, __builtin_fmaf(rot[1], q1, __builtin_fmaf(rot[2], q2, center)));
+    const float p1 = __builtin_fmaf(rot[3], q0, __builtin_fmaf(rot[4], q1, __builtin_fmaf(rot[5], q2, center)));
+    const float p2 = __builtin_fmaf(rot[6], q0, __builtin_fmaf(rot[7], q1, __builtin_fmaf(rot[8], q2, center)));
+    const int x = (int)p0, y = (int)p1, z = (int)p2;  // truncation toward zero
+    const int hi = size - 2;
+    const bool ok = x >= 0 && x <= hi && y >= 0 && y <= hi && z >= 0 && z <= hi;
+    const float fx = p0 - (float)x, fy = p1 - (float)y, fz = p2 - (float)z;
+    const float *b = model + (ok ? halo_index(nb, x, y, z) : 0);
+    const float m000 = b[0], m001 = b[1], m010 = b[5], m011 = b[6];
+    const float m100 = b[25], m101 = b[26], m110 = b[30], m111 = b[31];
+    const float a00 = __builtin_fmaf(fz, m001 - m000, m000), a01 = __builtin_fmaf(fz, m011 - m010, m010);
+    const float a10 = __builtin_fmaf(fz, m101 - m100, m100), a11 = __builtin_fmaf(fz, m111 - m110, m110);
+    const float b0 = __builtin_fmaf(fy, a01 - a00, a00), b1 = __builtin_fmaf(fy, a11 - a10, a10);
+    const float v = __builtin_fmaf(fx, b1 - b0, b0);
     return ok ? (double)v * corr : DBL_MIN;
 }
 

@@ -1062,25 +1062,30 @@ void launch_copy_f32(hipStream_t s, const double *in, size_t n, float *out) {
     copy_f32_kernel<<<grid_cap(cdiv(n, 256)), 256, 0, s>>>(in, n, out);
 }
 
-// FP32 brick copy of `modes` volumes of size^3 (layout of brick_index; voxels of the padded
-// brick grid past size - 1 are 0 and never read): one thread per output element.
+// FP32 brick copy of `modes` volumes of size^3 (halo_index layout: every brick holds the 5^3
+// corners of its 4^3 cells; corners past size - 1 and the padding are 0 and never read): one
+// thread per output element.
 __global__ void copy_f32_brick_kernel(const double *__restrict__ in, long size, int modes, float *__restrict__ out) {
     const int nb = model_bricks(size);
-    const size_t per_mode = (size_t)nb * nb * nb * 64, n = per_mode * modes;
+    const size_t per_mode = (size_t)nb * nb * nb * kHaloBrickFloats, n = per_mode * modes;
     const size_t vol = (size_t)size * size * size;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const size_t m = i / per_mode, j = i % per_mode, b = j >> 6;
-        const int l = (int)(j & 63);
-        const long x = (long)(b / ((size_t)nb * nb)) * 4 + (l >> 4);
-        const long y = (long)((b / nb) % nb) * 4 + ((l >> 2) & 3);
-        const long z = (long)(b % nb) * 4 + (l & 3);
-        out[i] = (x < size && y < size && z < size) ? (float)in[m * vol + (size_t)(x * size + y) * size + z] : 0.f;
+        const size_t m = i / per_mode, j = i % per_mode, b = j / kHaloBrickFloats;
+        const int l = (int)(j % kHaloBrickFloats);
+        float v = 0.f;
+        if (l < 125) {
+            const long x = (long)(b / ((size_t)nb * nb)) * 4 + l / 25;
+            const long y = (long)((b / nb) % nb) * 4 + (l / 5) % 5;
+            const long z = (long)(b % nb) * 4 + l % 5;
+            if (x < size && y < size && z < size) v = (float)in[m * vol + (size_t)(x * size + y) * size + z];
+        }
+        out[i] = v;
     }
 }
 
 size_t model_brick_elems(long size, int modes) {
     const size_t nb = (size_t)model_bricks(size);
-    return nb * nb * nb * 64 * (size_t)modes;
+    return nb * nb * nb * kHaloBrickFloats * (size_t)modes;
 }
 
 void launch_copy_f32_brick(hipStream_t s, const double *in, long size, int modes, float *out) {

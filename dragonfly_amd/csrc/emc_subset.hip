@@ -266,8 +266,11 @@ __global__ __launch_bounds__(kBlock) void pair_posterior_kernel(
 }
 
 // psum of every slot of a tile: the full frames' sum (psum_full, or 0) plus the sum over the
-// tile's partial-frame pairs (frames ascending) of P * phi' (phi' = scale if need_scaling
-// else 1; blacklisted frames have P = 0). One wave per tile.
+// tile's partial-frame pairs of P * phi' (phi' = scale if need_scaling else 1; blacklisted
+// frames have P = 0). One block per tile: wave w takes the batches of 64 pairs w, w + 4, ...,
+// its lanes load a batch's pair indices and factors together and pass them round by shuffles,
+// so the P-row loads of a batch are independent; the waves' partial sums are added in wave
+// order (a fixed order: the result does not depend on timing).
 __global__ __launch_bounds__(kBlock) void pair_psum_kernel(const double *__restrict__ P,
                                                            const long *__restrict__ tile_off,
                                                            const int *__restrict__ tile_pair,
@@ -275,16 +278,34 @@ __global__ __launch_bounds__(kBlock) void pair_psum_kernel(const double *__restr
                                                            const double *__restrict__ scale, int need_scaling,
                                                            const double *__restrict__ psum_full,
                                                            double *__restrict__ psum_slot) {
+    __shared__ double part[kWavesPerBlock][kRTile];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int T = blockIdx.x * kWavesPerBlock + wave;
-    if (T >= num_tiles) return;
-    double acc = psum_full ? psum_full[(size_t)T * kRTile + lane] : 0.;
-    for (long j = tile_off[T]; j < tile_off[T + 1]; ++j) {
-        const int k = tile_pair[j];
-        const double p = P[(size_t)k * kRTile + lane];
-        acc += need_scaling ? p * scale[pair_frame[k]] : p;
+    const int T = blockIdx.x;
+    const long j0 = tile_off[T], j1 = tile_off[T + 1];
+    double acc = 0.;
+    for (long b = j0 + (long)wave * 64; b < j1; b += (long)kWavesPerBlock * 64) {
+        const long j = b + lane;
+        int k = 0;
+        double f = 1.;
+        if (j < j1) {
+            k = tile_pair[j];
+            if (need_scaling) f = scale[pair_frame[k]];
+        }
+        const int n = (int)min((long)64, j1 - b);
+#pragma unroll 8
+        for (int i = 0; i < n; ++i) {
+            const int ki = __shfl(k, i);
+            const double fi = __shfl(f, i);
+            acc += P[(size_t)ki * kRTile + lane] * fi;
+        }
     }
-    psum_slot[(size_t)T * kRTile + lane] = acc;
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0) {
+        double sum = psum_full ? psum_full[(size_t)T * kRTile + lane] : 0.;
+        for (int w = 0; w < kWavesPerBlock; ++w) sum += part[w][lane];
+        psum_slot[(size_t)T * kRTile + lane] = sum;
+    }
 }
 
 // Tomogram of the chunk's tiles [t0, t0 + ntiles) over pixel ranges of kPairRange pixels:
@@ -685,7 +706,8 @@ void launch_pair_posterior(hipStream_t s, double *R, const long *pair_off, const
 void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                       const int *pair_frame, int num_tiles, const double *scale, int need_scaling,
                       const double *psum_full, double *psum_slot) {
-    pair_psum_kernel<<<(num_tiles + kWavesPerBlock - 1) / kWavesPerBlock, kBlock, 0, s>>>(
+    if (num_tiles <= 0) return;
+    pair_psum_kernel<<<num_tiles, kBlock, 0, s>>>(
         P, tile_off, tile_pair, pair_frame, num_tiles, scale, need_scaling, psum_full, psum_slot);
 }
 

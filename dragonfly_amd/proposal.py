@@ -305,11 +305,36 @@ class CoarseJudge:
         self.eng.close()
 
 
+def stack_cells(*tops):
+    """Per-frame lists of several (num_frames, k_i) cell arrays side by side, as CSR (off int64
+    (num_frames + 1), cells int32) of fixed width sum(k_i). A cell may appear more than once in a
+    frame's list; emc_set_refine takes a frame's set as the union of its list."""
+    X = np.ascontiguousarray(np.concatenate([np.asarray(t, 'i4') for t in tops], axis=1))
+    D, k = X.shape
+    return np.arange(D + 1, dtype='i8') * k, X.reshape(-1)
+
+
+def refine_rows(off, cells, accept):
+    """refine_lists_csr for lists of one fixed width (stack_cells): kept frames keep their
+    lists, the others get the single entry -1 (the full search)."""
+    accept = np.asarray(accept, bool)
+    D = len(accept)
+    k = int(off[1] - off[0]) if D else 0
+    X = np.asarray(cells, 'i4').reshape(D, k)
+    n = np.where(accept, k, 1)
+    rel_off = np.zeros(D + 1, 'i8')
+    np.cumsum(n, out=rel_off[1:])
+    out = np.full((D, k), -2, 'i4')
+    out[accept] = X[accept]
+    out[~accept, 0] = -1
+    return rel_off, out[out != -2]
+
+
 def judged_iteration(recon, itr, judge, tops, coarse_div, threshold=0.99):
     """One EMC iteration (recon.run_iteration) on proposals judged by the likelihood.
 
     Pass 1: the expectation step of the fine search restricted to each frame's proposed cells
-    (the union of the (num_frames, k_i) arrays in `tops`), which gives every frame its fine
+    (the union of its rows of the (num_frames, k_i) arrays in `tops`), which gives every frame its fine
     maximum and normaliser inside the cells. The judge (a coarse full search already run at the
     same model, scales and beta: CoarseJudge.estep) then estimates, per frame, the posterior
     mass the full fine search would put outside the cells (Engine.judge_outside). Frames
@@ -317,20 +342,25 @@ def judged_iteration(recon, itr, judge, tops, coarse_div, threshold=0.99):
     full fine search. Pass 2: when every frame is kept, the M-step resumes from pass 1
     (option mstep_only); otherwise the iteration runs again with the fall-back frames searched
     in full. Returns a dict: accept (bool per frame), outside (estimates), passes (1 or 2)."""
+    import time
     eng = recon.engine
     par = itr.params
-    off, cells = union_cells(*tops)
+    t = [time.perf_counter()]
+    off, cells = stack_cells(*tops)
     itr.set_refine_lists(coarse_div, off, cells)
     eng.set_refine(itr.num_coarse, itr.quat_mapping, itr.rel_off, itr.rel_quat)
     recon.update_beta()  # the beta of this iteration (run_iteration sets the same again)
+    t.append(time.perf_counter())
     eng.set_option('estep_only', 1)
     try:
         st1 = eng.maximize(par.iteration, par.need_scaling, par.update_scale, par.known_scale)
     finally:
         eng.set_option('estep_only', 0)
+    t.append(time.perf_counter())
     outside = eng.judge_outside(judge.eng)
     accept = outside <= 1. - threshold
     if accept.all():
+        t.append(time.perf_counter())
         eng.set_option('mstep_only', 1)
         try:
             recon.run_iteration()
@@ -338,13 +368,19 @@ def judged_iteration(recon, itr, judge, tops, coarse_div, threshold=0.99):
             eng.set_option('mstep_only', 0)
         passes = 1
     else:
-        rel_off, rel_quat = refine_lists_csr(off, cells, accept, itr.num_coarse)
+        rel_off, rel_quat = refine_rows(off, cells, accept)
         itr.set_refine_lists(coarse_div, rel_off, rel_quat)
         eng.set_refine(itr.num_coarse, itr.quat_mapping, itr.rel_off, itr.rel_quat)
+        t.append(time.perf_counter())
         recon.run_iteration()
         passes = 2
-    return dict(accept=accept, outside=outside, passes=passes, cells_per_frame=float(np.diff(off).mean()),
-                estep_stage_ms=st1['stage_ms'])
+    t.append(time.perf_counter())
+    host_ms = dict(zip(('lists', 'pass1', 'judge_and_lists', 'pass2'), [round((b - a) * 1e3, 1) for a, b in zip(t, t[1:])]))
+    # distinct cells per frame, on a sample of frames (the lists may repeat cells)
+    X = cells.reshape(len(off) - 1, -1)[:: max(1, (len(off) - 1) // 2000)]
+    distinct = (np.diff(np.sort(X, axis=1), axis=1) != 0).sum(1) + 1
+    return dict(accept=accept, outside=outside, passes=passes, cells_per_frame=float(distinct.mean()),
+                estep_stage_ms=st1['stage_ms'], phase_ms=host_ms)
 
 
 def refine_lists(top, mass, num_coarse, threshold=0.99, buf=None):

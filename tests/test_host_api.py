@@ -415,3 +415,25 @@ def test_union_and_csr_refine_lists():
     for d in range(25):
         want = cells[off[d]:off[d + 1]] if acc[d] else [-1]   # -1: every cell
         assert np.array_equal(rq[ro[d]:ro[d + 1]], want)
+
+
+def test_stacked_proposal_lists():
+    """stack_cells keeps every row side by side (repeats allowed: the engine takes a frame's set
+    as the union of its list); refine_rows gives refine_lists_csr's lists for them."""
+    from dragonfly_amd import proposal
+    rng = np.random.default_rng(4)
+    a = rng.integers(0, 50, (30, 5)).astype('i4')
+    b = rng.integers(0, 50, (30, 3)).astype('i4')
+    off, cells = proposal.stack_cells(a, b)
+    assert np.array_equal(off, np.arange(31) * 8)
+    assert np.array_equal(cells.reshape(30, 8), np.concatenate([a, b], axis=1))
+    acc = rng.random(30) > 0.3
+    ro, rq = proposal.refine_rows(off, cells, acc)
+    ro2, rq2 = proposal.refine_lists_csr(off, cells, acc, 50)
+    assert np.array_equal(ro, ro2) and np.array_equal(rq, rq2)
+    for d in range(30):
+        got = rq[ro[d]:ro[d + 1]]
+        if acc[d]:
+            assert set(got) == set(a[d]) | set(b[d])
+        else:
+            assert list(got) == [-1]
