@@ -354,6 +354,9 @@ struct emc_engine {
     DevBuf<long> pair_off, tile_off;
     DevBuf<double> rots_slot, quats_slot, logwr_slot, u_slot, psum_slot, Rsub;
     DevBuf<float> Rsub32;  // FP32 copy of the pair rows' posterior (FP32 gathers)
+    // pairs with some P > 1e-6, per tile (tile_off layout, sig_cnt per tile): the tomogram's
+    DevBuf<int> sig_pair, sig_frame, sig_cnt;
+    DevBuf<uint8_t> pair_flag;
     std::vector<BrickChunk> slot_bricks;
     bool slot_bricks_valid = false;
     // frame photon lists of mask < 2 pixels sorted by engine pixel, with per-range offsets
@@ -656,8 +659,16 @@ int build_bricks(emc_engine *e) {
 
 int merge_chunk(const emc_engine *e, const emc_engine::BrickChunk &bc, const double *rots, int num_valid,
                 const double *view, const double *psum, bool all_on, bool do_model, bool do_weight,
-                const int *gate = nullptr) {
+                const int *gate = nullptr, const float *view32 = nullptr) {
     const int nbx = (int)((e->size + emc::kBrick - 1) / emc::kBrick);
+    if (view32) {
+        // an FP32 view: model only, FP32 brick sums
+        if (!do_model || do_weight) return fail(EMC_E_STATE, "internal: an FP32 view merges the model only");
+        emc::launch_merge_model_f32(e->stream, bc.units.p, bc.num_units, bc.list.p, bc.lmask.p, rots, bc.r0, num_valid,
+                                    view32, psum, all_on, e->qvals.p, e->corr.p, e->mask.p, e->num_pix, e->size,
+                                    (e->num_pix + 63) / 64, nbx, e->model2.p);
+        return 0;
+    }
     emc::launch_merge_bricks(e->stream, bc.units.p, bc.num_units, bc.list.p, bc.lmask.p, rots, bc.r0, num_valid,
                              view, psum,
                              all_on, e->qvals.p, e->corr.p, e->mask.p, e->num_pix, e->size,
@@ -1185,6 +1196,10 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     }  // with no full frames RF keeps its room: the next set of lists may have them again
     const bool g32 = e->precision == 3;
     if (g32 && (rc = e->Rsub32.alloc((size_t)std::max<long>(e->num_pairs, 1) * emc::kRTile))) return rc;
+    if ((rc = e->sig_pair.alloc((size_t)std::max<long>(e->num_pairs, 1)))) return rc;
+    if ((rc = e->sig_frame.alloc((size_t)std::max<long>(e->num_pairs, 1)))) return rc;
+    if ((rc = e->pair_flag.alloc((size_t)std::max<long>(e->num_pairs, 1)))) return rc;
+    if ((rc = e->sig_cnt.alloc((size_t)std::max(T, 1)))) return rc;
     if (g32 && (rc = ensure_model32(e))) return rc;
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
@@ -1366,7 +1381,16 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                                            opts->need_scaling, nF > 0 ? e->psum_full.p : nullptr, e->psum_slot.p);
                      emc::launch_slot_gather(s, e->psum_slot.p, e->rot_slot.p, Rn, e->psum_r.p);
                      emc::launch_slot_gather(s, e->u_slot.p, e->rot_slot.p, Rn, e->u.p);
-                     if (g32) emc::launch_pair_to_f32(s, e->Rsub.p, e->num_pairs * emc::kRTile, e->Rsub32.p);
+                     if (g32) {
+                         emc::launch_pair_to_f32(s, e->Rsub.p, e->num_pairs * emc::kRTile, e->Rsub32.p);
+                         emc::launch_sig_pairs(s, e->Rsub32.p, e->num_pairs, e->tile_off.p, e->tile_pair.p,
+                                               e->tile_frame.p, T, e->pair_flag.p, e->sig_pair.p, e->sig_frame.p,
+                                               e->sig_cnt.p);
+                     } else {
+                         emc::launch_sig_pairs(s, e->Rsub.p, e->num_pairs, e->tile_off.p, e->tile_pair.p,
+                                               e->tile_frame.p, T, e->pair_flag.p, e->sig_pair.p, e->sig_frame.p,
+                                               e->sig_cnt.p);
+                     }
                      return 0;
                  })))
                 return rc;
@@ -1433,27 +1457,34 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                 return rc;
             // partial frames' pairs added to the full frames' partial sums, divided by psum
             if ((rc = timed(7, [&] {
+                     // FP32 gathers: an FP32 view (in the FP32 log W buffer, free in the M-step)
                      if (g32)
-                         emc::launch_pair_tomogram(s, e->Rsub32.p, e->tile_off.p, e->tile_pair.p, e->tile_frame.p,
-                                                   e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
-                                                   e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
-                                                   nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p);
+                         emc::launch_pair_tomogram(s, e->Rsub32.p, e->tile_off.p, e->sig_pair.p, e->sig_frame.p,
+                                                   e->sig_cnt.p, e->fsort_off.p, e->fsort_roff.p, e->nranges,
+                                                   e->fsort_pix.p, e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
+                                                   nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw32.p);
                      else
-                         emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->tile_pair.p, e->tile_frame.p,
-                                                   e->fsort_off.p, e->fsort_roff.p, e->nranges, e->fsort_pix.p,
-                                                   e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
+                         emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->sig_pair.p, e->sig_frame.p,
+                                                   e->sig_cnt.p, e->fsort_off.p, e->fsort_roff.p, e->nranges,
+                                                   e->fsort_pix.p, e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
                                                    nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p);
                      return 0;
                  })))
                 return rc;
             if ((rc = timed(8, [&] {
-                     if (!device_weights)
+                     const float *v32 = g32 ? e->logw32.p : nullptr;
+                     if (!device_weights && !g32)
                          return merge_chunk(e, bc, e->rots_slot.p, S, e->logw.p, e->psum_slot.p, false, true,
                                             !cached_weight);
                      // weights of the orientations that merge, only when some orientation does not
-                     // (gate: the count of those that do not); the model from every view
-                     merge_chunk(e, bc, e->rots_slot.p, S, nullptr, e->psum_slot.p, false, false, true, e->d_count.p);
-                     return merge_chunk(e, bc, e->rots_slot.p, S, e->logw.p, e->psum_slot.p, false, true, false);
+                     // (device: gate = the count of those that do not), then the model from every view
+                     if (device_weights)
+                         merge_chunk(e, bc, e->rots_slot.p, S, nullptr, e->psum_slot.p, false, false, true,
+                                     e->d_count.p);
+                     else if (!cached_weight)
+                         merge_chunk(e, bc, e->rots_slot.p, S, nullptr, e->psum_slot.p, false, false, true);
+                     return merge_chunk(e, bc, e->rots_slot.p, S, e->logw.p, e->psum_slot.p, false, true, false,
+                                        nullptr, v32);
                  })))
                 return rc;
         }

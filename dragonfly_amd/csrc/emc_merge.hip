@@ -137,11 +137,12 @@ __device__ __forceinline__ void brick_add(const double *rot, double q0, double q
 // wave compacts the nonzero (orientation, pixel) entries of its items into a 128-entry ring in
 // LDS and interpolates and adds them 64 at a time, one entry per lane. A is the type of the
 // brick's LDS sums: double for the FP64 path, float for the reduced-precision modes (the
-// brick is added to the FP64 model once per work unit).
-template <typename A>
+// brick is added to the FP64 model once per work unit). VT: the view's type (double, or float
+// where the reduced-precision pair tomogram writes it).
+template <typename A, typename VT = double>
 __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
     const int4 *__restrict__ units, const int *__restrict__ list, const unsigned long long *__restrict__ lmask,
-    const double *__restrict__ rots, int r0, int num_rot, const double *__restrict__ view,
+    const double *__restrict__ rots, int r0, int num_rot, const VT *__restrict__ view,
     const double *__restrict__ psum_r, int all_on, const double *__restrict__ qvals, const double *__restrict__ corr,
     const uint8_t *__restrict__ mask, int num_pix, long size, int npatch, int nbx, double *__restrict__ model2) {
     constexpr int kVox = kBrick * kBrick * kBrick;
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
             const int r = r0 + c, t = p * 64 + lane;
             if (!all_on && !(psum_r[r] > 0.)) continue;
             rr[a] = r;
-            if (mine[a]) wv[a] = view[(size_t)c * num_pix + t];  // a set bit: t < num_pix, mask[t] <= 1
+            if (mine[a]) wv[a] = (double)view[(size_t)c * num_pix + t];  // a set bit: t < num_pix, mask[t] <= 1
         }
 #pragma unroll
         for (int a = 0; a < kItemsAhead; ++a) {
@@ -323,6 +324,17 @@ void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const 
                                                                       size, npatch, nbx, model2);
     else if (do_weight) EMC_MB(false, true);
 #undef EMC_MB
+}
+
+void launch_merge_model_f32(hipStream_t s, const int4 *units, int num_units, const int *list,
+                            const unsigned long long *lmask, const double *rots, int r0, int num_rot,
+                            const float *view, const double *psum_r, int all_on, const double *qvals,
+                            const double *corr, const uint8_t *mask, int num_pix, long size, int npatch, int nbx,
+                            double *model2) {
+    if (num_units <= 0) return;
+    merge_brick_model_kernel<float, float><<<num_units, kBlock, 0, s>>>(units, list, lmask, rots, r0, num_rot, view,
+                                                                        psum_r, all_on, qvals, corr, mask, num_pix,
+                                                                        size, npatch, nbx, model2);
 }
 
 }  // namespace emc

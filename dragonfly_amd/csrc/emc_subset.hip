@@ -308,21 +308,75 @@ __global__ __launch_bounds__(kBlock) void pair_psum_kernel(const double *__restr
     }
 }
 
+// Pairs of every tile whose row holds some P > 1e-6 (the tomogram's condition, on the values
+// the tomogram reads), in the tile's order (frames ascending): sig_pair / sig_frame
+// [tile_off[T] + i] for i < sig_cnt[T]. At a late iteration a frame's posterior sits in one
+// or two of its proposed cells, so most pairs add nothing to any view.
+template <typename TP>
+__global__ __launch_bounds__(kBlock) void pair_sig_kernel(const TP *__restrict__ P, long num_pairs,
+                                                          uint8_t *__restrict__ flag) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long stride = (long)gridDim.x * kWavesPerBlock;
+    for (long k = (long)blockIdx.x * kWavesPerBlock + wave; k < num_pairs; k += stride) {
+        const unsigned long long b = __ballot((double)P[(size_t)k * kRTile + lane] > kProbMin);
+        if (lane == 0) flag[k] = b != 0ull;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void sig_compact_kernel(const long *__restrict__ tile_off,
+                                                             const int *__restrict__ tile_pair,
+                                                             const int *__restrict__ tile_frame,
+                                                             const uint8_t *__restrict__ flag, int *__restrict__ sig_pair,
+                                                             int *__restrict__ sig_frame, int *__restrict__ sig_cnt) {
+    __shared__ int s_w[kWavesPerBlock];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int T = blockIdx.x;
+    const long j0 = tile_off[T], j1 = tile_off[T + 1];
+    int base = 0;  // block-uniform count so far
+    for (long jb = j0; jb < j1; jb += kBlock) {
+        const long j = jb + threadIdx.x;
+        int k = 0, d = 0;
+        bool sig = false;
+        if (j < j1) {
+            k = tile_pair[j];
+            d = tile_frame[j];
+            sig = flag[k] != 0;
+        }
+        const unsigned long long b = __ballot(sig);
+        if (lane == 0) s_w[wave] = __popcll(b);
+        __syncthreads();
+        int before = base, total = base;
+        for (int w = 0; w < kWavesPerBlock; ++w) {
+            if (w < wave) before += s_w[w];
+            total += s_w[w];
+        }
+        if (sig) {
+            const long at = j0 + before + __popcll(b & ((1ull << lane) - 1ull));
+            sig_pair[at] = k;
+            sig_frame[at] = d;
+        }
+        base = total;
+        __syncthreads();  // s_w is rewritten by the next round
+    }
+    if (threadIdx.x == 0) sig_cnt[T] = base;
+}
+
 // Tomogram of the chunk's tiles [t0, t0 + ntiles) over pixel ranges of kPairRange pixels:
 // block (tile, range) starts from the full frames' partial sums (tomogram_part_kernel
 // layout partial[x][tile][t][64], added over the nparts frame ranges in order) or from zero,
-// adds K * P (P > 1e-6) of the tile's partial-frame pairs into LDS rows of the range's pixels
-// (frame photon lists sorted by pixel, range offsets roff[d][range]), divides by psum and
-// writes the orientation-major view[(T - t0) * 64 + j][t]. The tile's pairs are examined
-// kBlock at a time, one per thread, and only those with photons in the range are queued for
-// the waves, so a pair costs one row of P per range it has photons in.
-template <typename TP>
+// adds K * P (P > 1e-6) of the tile's significant partial-frame pairs (sig_pair, sig_cnt)
+// into LDS rows of the range's pixels (frame photon lists sorted by pixel, range offsets
+// roff[d][range]), divides by psum and writes the orientation-major view[(T - t0) * 64 + j][t]
+// (VT: double, or float for the FP32-sum merge). The tile's pairs are examined kBlock at a
+// time, one per thread, and only those with photons in the range are queued for the waves,
+// so a pair costs one row of P per range it has photons in.
+template <typename TP, typename VT>
 __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     const TP *__restrict__ P, const long *__restrict__ tile_off, const int *__restrict__ tile_pair,
-    const int *__restrict__ tile_frame, const long *__restrict__ foff, const int *__restrict__ roff,
-    int nranges, const int *__restrict__ fpix, const double *__restrict__ fcnt,
+    const int *__restrict__ tile_frame, const int *__restrict__ sig_cnt, const long *__restrict__ foff,
+    const int *__restrict__ roff, int nranges, const int *__restrict__ fpix, const double *__restrict__ fcnt,
     const double *__restrict__ psum_slot, int t0, int ntiles, int num_pix,
-    const double *__restrict__ partial, int nparts, double *__restrict__ view) {
+    const double *__restrict__ partial, int nparts, VT *__restrict__ view) {
     __shared__ double acc[kPairRange][kRTile + 1];
     __shared__ int s_k[kBlock];
     __shared__ long s_q0[kBlock], s_q1[kBlock];
@@ -334,8 +388,9 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     const int lt = xcd + kNumXcd * (j / nranges), rg = j % nranges;
     if (lt >= ntiles) return;
     const int T = t0 + lt;
-    const long j0 = tile_off[T], j1 = tile_off[T + 1];
-    if (j0 == j1 && !partial) return;  // no frame takes this tile: psum 0, the merge skips it
+    const long j0 = tile_off[T];
+    if (j0 == tile_off[T + 1] && !partial) return;  // no frame takes this tile: psum 0, the merge skips it
+    const long j1 = j0 + sig_cnt[T];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int p0 = rg * kPairRange;
     if (partial) {
@@ -405,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     for (int jj = wave; jj < kRTile; jj += kWavesPerBlock)
         for (int i = lane; i < kPairRange; i += 64) {
             const int t = p0 + i;
-            if (t < num_pix) view[((size_t)lt * kRTile + jj) * num_pix + t] = acc[i][jj];
+            if (t < num_pix) view[((size_t)lt * kRTile + jj) * num_pix + t] = (VT)acc[i][jj];
         }
 }
 
@@ -711,31 +766,56 @@ void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, cons
         P, tile_off, tile_pair, pair_frame, num_tiles, scale, need_scaling, psum_full, psum_slot);
 }
 
-template <typename TP>
-static void pair_tomogram_launch(hipStream_t s, const TP *P, const long *tile_off, const int *tile_pair,
-                                 const int *tile_frame, const long *foff, const int *roff, int nranges,
-                                 const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
-                                 int num_pix, const double *partial, int nparts, double *view) {
+template <typename TP, typename VT>
+static void pair_tomogram_launch(hipStream_t s, const TP *P, const long *tile_off, const int *sig_pair,
+                                 const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff,
+                                 int nranges, const int *fpix, const double *fcnt, const double *psum_slot, int t0,
+                                 int ntiles, int num_pix, const double *partial, int nparts, VT *view) {
     if (ntiles <= 0) return;
     const unsigned grid = (unsigned)((ntiles + kNumXcd - 1) / kNumXcd * kNumXcd) * (unsigned)nranges;
-    pair_tomogram_kernel<TP><<<grid, kBlock, 0, s>>>(P, tile_off, tile_pair, tile_frame, foff, roff, nranges, fpix,
-                                                     fcnt, psum_slot, t0, ntiles, num_pix, partial, nparts, view);
+    pair_tomogram_kernel<TP, VT><<<grid, kBlock, 0, s>>>(P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff,
+                                                         nranges, fpix, fcnt, psum_slot, t0, ntiles, num_pix, partial,
+                                                         nparts, view);
 }
 
-void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
-                          const int *tile_frame, const long *foff, const int *roff, int nranges, const int *fpix,
-                          const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
-                          const double *partial, int nparts, double *view) {
-    pair_tomogram_launch(s, P, tile_off, tile_pair, tile_frame, foff, roff, nranges, fpix, fcnt, psum_slot, t0,
-                         ntiles, num_pix, partial, nparts, view);
+void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *sig_pair,
+                          const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
+                          const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
+                          int num_pix, const double *partial, int nparts, double *view) {
+    pair_tomogram_launch(s, P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff, nranges, fpix, fcnt, psum_slot,
+                         t0, ntiles, num_pix, partial, nparts, view);
 }
 
-void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *tile_pair,
-                          const int *tile_frame, const long *foff, const int *roff, int nranges, const int *fpix,
-                          const double *fcnt, const double *psum_slot, int t0, int ntiles, int num_pix,
-                          const double *partial, int nparts, double *view) {
-    pair_tomogram_launch(s, P, tile_off, tile_pair, tile_frame, foff, roff, nranges, fpix, fcnt, psum_slot, t0,
-                         ntiles, num_pix, partial, nparts, view);
+void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *sig_pair,
+                          const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
+                          const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
+                          int num_pix, const double *partial, int nparts, float *view) {
+    pair_tomogram_launch(s, P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff, nranges, fpix, fcnt, psum_slot,
+                         t0, ntiles, num_pix, partial, nparts, view);
+}
+
+template <typename TP>
+static void sig_pairs_launch(hipStream_t s, const TP *P, long num_pairs, const long *tile_off, const int *tile_pair,
+                             const int *tile_frame, int num_tiles, uint8_t *flag, int *sig_pair, int *sig_frame,
+                             int *sig_cnt) {
+    if (num_tiles <= 0) return;
+    if (num_pairs > 0)
+        pair_sig_kernel<TP><<<stride_grid((num_pairs + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, s>>>(
+            P, num_pairs, flag);
+    sig_compact_kernel<<<num_tiles, kBlock, 0, s>>>(tile_off, tile_pair, tile_frame, flag, sig_pair, sig_frame,
+                                                    sig_cnt);
+}
+
+void launch_sig_pairs(hipStream_t s, const double *P, long num_pairs, const long *tile_off, const int *tile_pair,
+                      const int *tile_frame, int num_tiles, uint8_t *flag, int *sig_pair, int *sig_frame,
+                      int *sig_cnt) {
+    sig_pairs_launch(s, P, num_pairs, tile_off, tile_pair, tile_frame, num_tiles, flag, sig_pair, sig_frame, sig_cnt);
+}
+
+void launch_sig_pairs(hipStream_t s, const float *P, long num_pairs, const long *tile_off, const int *tile_pair,
+                      const int *tile_frame, int num_tiles, uint8_t *flag, int *sig_pair, int *sig_frame,
+                      int *sig_cnt) {
+    sig_pairs_launch(s, P, num_pairs, tile_off, tile_pair, tile_frame, num_tiles, flag, sig_pair, sig_frame, sig_cnt);
 }
 
 // FP32 copy of the pair rows (the posterior read by the FP32 pair tomogram).
