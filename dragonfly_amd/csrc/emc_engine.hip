@@ -357,6 +357,8 @@ struct emc_engine {
     // pairs with some P > 1e-6, per tile (tile_off layout, sig_cnt per tile): the tomogram's
     DevBuf<int> sig_pair, sig_frame, sig_cnt;
     DevBuf<uint8_t> pair_flag;
+    // per (chunk slot, 64-pixel patch): pixels with a nonzero view (pair tomogram -> merge)
+    DevBuf<unsigned long long> nzmask;
     std::vector<BrickChunk> slot_bricks;
     bool slot_bricks_valid = false;
     // frame photon lists of mask < 2 pixels sorted by engine pixel, with per-range offsets
@@ -659,21 +661,23 @@ int build_bricks(emc_engine *e) {
 
 int merge_chunk(const emc_engine *e, const emc_engine::BrickChunk &bc, const double *rots, int num_valid,
                 const double *view, const double *psum, bool all_on, bool do_model, bool do_weight,
-                const int *gate = nullptr, const float *view32 = nullptr) {
+                const int *gate = nullptr, const float *view32 = nullptr,
+                const unsigned long long *nzmask = nullptr) {
     const int nbx = (int)((e->size + emc::kBrick - 1) / emc::kBrick);
+    if ((view32 || nzmask) && (!do_model || do_weight))
+        return fail(EMC_E_STATE, "internal: an FP32 or masked view merges the model only");
     if (view32) {
-        // an FP32 view: model only, FP32 brick sums
-        if (!do_model || do_weight) return fail(EMC_E_STATE, "internal: an FP32 view merges the model only");
+        // an FP32 view: FP32 brick sums
         emc::launch_merge_model_f32(e->stream, bc.units.p, bc.num_units, bc.list.p, bc.lmask.p, rots, bc.r0, num_valid,
                                     view32, psum, all_on, e->qvals.p, e->corr.p, e->mask.p, e->num_pix, e->size,
-                                    (e->num_pix + 63) / 64, nbx, e->model2.p);
+                                    (e->num_pix + 63) / 64, nbx, e->model2.p, nzmask);
         return 0;
     }
     emc::launch_merge_bricks(e->stream, bc.units.p, bc.num_units, bc.list.p, bc.lmask.p, rots, bc.r0, num_valid,
                              view, psum,
                              all_on, e->qvals.p, e->corr.p, e->mask.p, e->num_pix, e->size,
                              (e->num_pix + 63) / 64, nbx, e->model2.p, e->weight.p, do_model, do_weight,
-                             e->precision != 0, gate);  // reduced-precision modes sum bricks in FP32
+                             e->precision != 0, gate, nzmask);  // reduced-precision modes sum bricks in FP32
     return 0;
 }
 
@@ -1210,7 +1214,8 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     // per slot column: log W / view, the full frames' partial views, and with FP32 gathers the
     // FP32 log W and full-frame posterior columns
     const size_t per_col = (size_t)P * sizeof(double) * (1 + (nF > 0 ? nparts : 0)) +
-                           (g32 ? ((size_t)P + nF) * sizeof(float) : 0);
+                           (g32 ? ((size_t)P + nF) * sizeof(float) : 0) +
+                           (size_t)((P + 63) / 64) * sizeof(unsigned long long);
     size_t cmax = budget / per_col;
     if (e->max_chunk_cols > 0) cmax = std::min(cmax, (size_t)e->max_chunk_cols);
     cmax = cmax / emc::kRTile * emc::kRTile;
@@ -1238,6 +1243,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     const int ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
     if ((rc = e->upart.alloc((size_t)ngroups * C))) return rc;
     if ((rc = e->ubad.alloc((size_t)ngroups * C))) return rc;
+    if ((rc = e->nzmask.alloc((size_t)C * ((P + 63) / 64)))) return rc;
     if ((rc = e->u_slot.alloc(S))) return rc;
     if ((rc = e->rbad_slot.alloc(S))) return rc;
     if ((rc = e->psum_slot.alloc(S))) return rc;
@@ -1455,19 +1461,21 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                                return 0;
                            })))
                 return rc;
-            // partial frames' pairs added to the full frames' partial sums, divided by psum
+            // partial frames' pairs added to the full frames' partial sums, divided by psum;
+            // with nonzero masks when the model merges alone (only nonzero views are stored)
+            unsigned long long *nz = g32 || device_weights || cached_weight ? e->nzmask.p : nullptr;
             if ((rc = timed(7, [&] {
                      // FP32 gathers: an FP32 view (in the FP32 log W buffer, free in the M-step)
                      if (g32)
                          emc::launch_pair_tomogram(s, e->Rsub32.p, e->tile_off.p, e->sig_pair.p, e->sig_frame.p,
                                                    e->sig_cnt.p, e->fsort_off.p, e->fsort_roff.p, e->nranges,
                                                    e->fsort_pix.p, e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
-                                                   nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw32.p);
+                                                   nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw32.p, nz);
                      else
                          emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->sig_pair.p, e->sig_frame.p,
                                                    e->sig_cnt.p, e->fsort_off.p, e->fsort_roff.p, e->nranges,
                                                    e->fsort_pix.p, e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
-                                                   nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p);
+                                                   nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p, nz);
                      return 0;
                  })))
                 return rc;
@@ -1475,7 +1483,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                      const float *v32 = g32 ? e->logw32.p : nullptr;
                      if (!device_weights && !g32)
                          return merge_chunk(e, bc, e->rots_slot.p, S, e->logw.p, e->psum_slot.p, false, true,
-                                            !cached_weight);
+                                            !cached_weight, nullptr, nullptr, nz);
                      // weights of the orientations that merge, only when some orientation does not
                      // (device: gate = the count of those that do not), then the model from every view
                      if (device_weights)
@@ -1484,7 +1492,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                      else if (!cached_weight)
                          merge_chunk(e, bc, e->rots_slot.p, S, nullptr, e->psum_slot.p, false, false, true);
                      return merge_chunk(e, bc, e->rots_slot.p, S, e->logw.p, e->psum_slot.p, false, true, false,
-                                        nullptr, v32);
+                                        nullptr, v32, nz);
                  })))
                 return rc;
         }

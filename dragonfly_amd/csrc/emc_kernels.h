@@ -142,13 +142,13 @@ void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const 
                          const double *view, const double *psum_r, int all_on, const double *qvals,
                          const double *corr, const uint8_t *mask, int num_pix, long size, int npatch, int nbx,
                          double *model2, double *weight, int do_model, int do_weight, int f32_sums = 0,
-                         const int *gate = nullptr);
-// model-only merge with FP32 brick sums from an FP32 view
+                         const int *gate = nullptr, const unsigned long long *nzmask = nullptr);
+// model-only merge with FP32 brick sums from an FP32 view (nzmask: as launch_merge_bricks')
 void launch_merge_model_f32(hipStream_t s, const int4 *units, int num_units, const int *list,
                             const unsigned long long *lmask, const double *rots, int r0, int num_rot,
                             const float *view, const double *psum_r, int all_on, const double *qvals,
                             const double *corr, const uint8_t *mask, int num_pix, long size, int npatch, int nbx,
-                            double *model2);
+                            double *model2, const unsigned long long *nzmask = nullptr);
 // dst = src (n doubles) when *gate == 0
 void launch_copy_if_zero(hipStream_t s, double *dst, const double *src, size_t n, const int *gate);
 void launch_merge(hipStream_t s, const double *rots, int r0, int C, int num_rot, const double *view,
@@ -231,15 +231,18 @@ void launch_pair_posterior(hipStream_t s, double *R, const long *pair_off, const
 void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                       const int *pair_frame, int num_tiles, const double *scale, int need_scaling,
                       const double *psum_full, double *psum_slot);
-// sig_pair / sig_frame / sig_cnt: the tiles' significant pairs (launch_sig_pairs)
+// sig_pair / sig_frame / sig_cnt: the tiles' significant pairs (launch_sig_pairs). nzmask
+// (nullable): per (chunk slot, 64-pixel patch) the mask of pixels with a nonzero view; with it
+// only those view entries are written.
 void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *sig_pair,
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
-                          int num_pix, const double *partial, int nparts, double *view);
+                          int num_pix, const double *partial, int nparts, double *view,
+                          unsigned long long *nzmask);
 void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *sig_pair,
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
-                          int num_pix, const double *partial, int nparts, float *view);
+                          int num_pix, const double *partial, int nparts, float *view, unsigned long long *nzmask);
 // pairs with some P > 1e-6 in their row, per tile in the tile's order (flag: num_pairs bytes;
 // sig_pair / sig_frame: tile_off layout; sig_cnt: per tile)
 void launch_sig_pairs(hipStream_t s, const double *P, long num_pairs, const long *tile_off, const int *tile_pair,

@@ -138,13 +138,17 @@ __device__ __forceinline__ void brick_add(const double *rot, double q0, double q
 // LDS and interpolates and adds them 64 at a time, one entry per lane. A is the type of the
 // brick's LDS sums: double for the FP64 path, float for the reduced-precision modes (the
 // brick is added to the FP64 model once per work unit). VT: the view's type (double, or float
-// where the reduced-precision pair tomogram writes it).
+// where the reduced-precision pair tomogram writes it). nzmask (nullable): per item (chunk
+// orientation c, patch p) at c * npatch + p, the pixels whose view is nonzero (the only view
+// entries written); an item none of whose masked pixels is nonzero is passed over without a
+// view read, and psum_r is not consulted (an orientation with psum_r = 0 has no nonzero view).
 template <typename A, typename VT = double>
 __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
     const int4 *__restrict__ units, const int *__restrict__ list, const unsigned long long *__restrict__ lmask,
     const double *__restrict__ rots, int r0, int num_rot, const VT *__restrict__ view,
     const double *__restrict__ psum_r, int all_on, const double *__restrict__ qvals, const double *__restrict__ corr,
-    const uint8_t *__restrict__ mask, int num_pix, long size, int npatch, int nbx, double *__restrict__ model2) {
+    const uint8_t *__restrict__ mask, int num_pix, long size, int npatch, int nbx, double *__restrict__ model2,
+    const unsigned long long *__restrict__ nzmask) {
     constexpr int kVox = kBrick * kBrick * kBrick;
     constexpr int kRing = 128;
     __shared__ A s_m[kVox];
@@ -174,12 +178,17 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
     constexpr int kItemsAhead = 4;
     for (int k0 = u.y + wave; k0 < u.z; k0 += kWavesPerBlock * kItemsAhead) {
         int it[kItemsAhead];
-        bool mine[kItemsAhead];  // the lane's pixel has a corner in this brick
+        unsigned long long lm[kItemsAhead];  // pixels with a corner in this brick
 #pragma unroll
         for (int a = 0; a < kItemsAhead; ++a) {
             const int k = k0 + a * kWavesPerBlock;
             it[a] = k < u.z ? list[k] : -1;
-            mine[a] = k < u.z && ((lmask[k] >> lane) & 1ull);
+            lm[a] = k < u.z ? lmask[k] : 0ull;
+        }
+        if (nzmask) {
+#pragma unroll
+            for (int a = 0; a < kItemsAhead; ++a)
+                if (it[a] >= 0) lm[a] &= nzmask[it[a]];  // item id = c * npatch + p
         }
         double wv[kItemsAhead];
         int rr[kItemsAhead];
@@ -187,12 +196,13 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
         for (int a = 0; a < kItemsAhead; ++a) {
             wv[a] = 0.;  // the view; divided by corr when its entry is drained
             rr[a] = -1;
-            if (it[a] < 0) continue;
+            if (it[a] < 0 || (nzmask && lm[a] == 0ull)) continue;
             const int c = it[a] / npatch, p = it[a] % npatch;
             const int r = r0 + c, t = p * 64 + lane;
-            if (!all_on && !(psum_r[r] > 0.)) continue;
+            if (!nzmask && !all_on && !(psum_r[r] > 0.)) continue;
             rr[a] = r;
-            if (mine[a]) wv[a] = (double)view[(size_t)c * num_pix + t];  // a set bit: t < num_pix, mask[t] <= 1
+            // a set bit: t < num_pix, mask[t] <= 1
+            if ((lm[a] >> lane) & 1ull) wv[a] = (double)view[(size_t)c * num_pix + t];
         }
 #pragma unroll
         for (int a = 0; a < kItemsAhead; ++a) {
@@ -307,7 +317,7 @@ void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const 
                          const double *view, const double *psum_r, int all_on, const double *qvals,
                          const double *corr, const uint8_t *mask, int num_pix, long size, int npatch, int nbx,
                          double *model2, double *weight, int do_model, int do_weight, int f32_sums,
-                         const int *gate) {
+                         const int *gate, const unsigned long long *nzmask) {
     if (num_units <= 0) return;
 #define EMC_MB(M, W)                                                                                        \
     merge_brick_kernel<M, W><<<num_units, kBlock, 0, s>>>(units, list, lmask, rots, r0, num_rot, view, psum_r, \
@@ -317,11 +327,11 @@ void launch_merge_bricks(hipStream_t s, const int4 *units, int num_units, const 
     else if (do_model && f32_sums)
         merge_brick_model_kernel<float><<<num_units, kBlock, 0, s>>>(units, list, lmask, rots, r0, num_rot, view,
                                                                      psum_r, all_on, qvals, corr, mask, num_pix,
-                                                                     size, npatch, nbx, model2);
+                                                                     size, npatch, nbx, model2, nzmask);
     else if (do_model)
         merge_brick_model_kernel<double><<<num_units, kBlock, 0, s>>>(units, list, lmask, rots, r0, num_rot, view,
                                                                       psum_r, all_on, qvals, corr, mask, num_pix,
-                                                                      size, npatch, nbx, model2);
+                                                                      size, npatch, nbx, model2, nzmask);
     else if (do_weight) EMC_MB(false, true);
 #undef EMC_MB
 }
@@ -330,11 +340,11 @@ void launch_merge_model_f32(hipStream_t s, const int4 *units, int num_units, con
                             const unsigned long long *lmask, const double *rots, int r0, int num_rot,
                             const float *view, const double *psum_r, int all_on, const double *qvals,
                             const double *corr, const uint8_t *mask, int num_pix, long size, int npatch, int nbx,
-                            double *model2) {
+                            double *model2, const unsigned long long *nzmask) {
     if (num_units <= 0) return;
     merge_brick_model_kernel<float, float><<<num_units, kBlock, 0, s>>>(units, list, lmask, rots, r0, num_rot, view,
                                                                         psum_r, all_on, qvals, corr, mask, num_pix,
-                                                                        size, npatch, nbx, model2);
+                                                                        size, npatch, nbx, model2, nzmask);
 }
 
 }  // namespace emc

@@ -376,7 +376,9 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     const int *__restrict__ tile_frame, const int *__restrict__ sig_cnt, const long *__restrict__ foff,
     const int *__restrict__ roff, int nranges, const int *__restrict__ fpix, const double *__restrict__ fcnt,
     const double *__restrict__ psum_slot, int t0, int ntiles, int num_pix,
-    const double *__restrict__ partial, int nparts, VT *__restrict__ view) {
+    const double *__restrict__ partial, int nparts, VT *__restrict__ view,
+    unsigned long long *__restrict__ nzmask) {
+    static_assert(kPairRange == 64, "a pixel range is one 64-pixel merge patch");
     __shared__ double acc[kPairRange][kRTile + 1];
     __shared__ int s_k[kBlock];
     __shared__ long s_q0[kBlock], s_q1[kBlock];
@@ -389,9 +391,14 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     if (lt >= ntiles) return;
     const int T = t0 + lt;
     const long j0 = tile_off[T];
-    if (j0 == tile_off[T + 1] && !partial) return;  // no frame takes this tile: psum 0, the merge skips it
-    const long j1 = j0 + sig_cnt[T];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int npatch = (num_pix + 63) / 64;
+    if (j0 == tile_off[T + 1] && !partial) {
+        // no frame takes this tile: psum 0, views 0 (unwritten), the merge skips the items
+        if (nzmask && tid < kRTile) nzmask[((size_t)lt * kRTile + tid) * npatch + rg] = 0ull;
+        return;
+    }
+    const long j1 = j0 + sig_cnt[T];
     const int p0 = rg * kPairRange;
     if (partial) {
         const size_t tsW = (size_t)num_pix * kRTile, tsPart = tsW * ntiles;
@@ -456,12 +463,20 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     for (int i = wave; i < kPairRange; i += kWavesPerBlock)
         acc[i][lane] = ps > 0. ? acc[i][lane] / ps : acc[i][lane];
     __syncthreads();
-    // transpose: lanes along pixels
-    for (int jj = wave; jj < kRTile; jj += kWavesPerBlock)
-        for (int i = lane; i < kPairRange; i += 64) {
-            const int t = p0 + i;
-            if (t < num_pix) view[((size_t)lt * kRTile + jj) * num_pix + t] = (VT)acc[i][jj];
+    // transpose: lanes along pixels. With nzmask, item (slot, patch rg) records which of its
+    // pixels have a nonzero view and only those are stored: the merge reads no other.
+    for (int jj = wave; jj < kRTile; jj += kWavesPerBlock) {
+        const int t = p0 + lane;
+        const double v = t < num_pix ? acc[lane][jj] : 0.;
+        const size_t at = ((size_t)lt * kRTile + jj) * num_pix + t;
+        if (nzmask) {
+            const unsigned long long m = __ballot(v != 0.);
+            if (lane == 0) nzmask[((size_t)lt * kRTile + jj) * npatch + rg] = m;
+            if (v != 0.) view[at] = (VT)v;
+        } else if (t < num_pix) {
+            view[at] = (VT)v;
         }
+    }
 }
 
 // Scatter of partial-frame pair values to a dense (num_data x num_rot) matrix (zeroed by the
@@ -770,28 +785,31 @@ template <typename TP, typename VT>
 static void pair_tomogram_launch(hipStream_t s, const TP *P, const long *tile_off, const int *sig_pair,
                                  const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff,
                                  int nranges, const int *fpix, const double *fcnt, const double *psum_slot, int t0,
-                                 int ntiles, int num_pix, const double *partial, int nparts, VT *view) {
+                                 int ntiles, int num_pix, const double *partial, int nparts, VT *view,
+                                 unsigned long long *nzmask) {
     if (ntiles <= 0) return;
     const unsigned grid = (unsigned)((ntiles + kNumXcd - 1) / kNumXcd * kNumXcd) * (unsigned)nranges;
     pair_tomogram_kernel<TP, VT><<<grid, kBlock, 0, s>>>(P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff,
                                                          nranges, fpix, fcnt, psum_slot, t0, ntiles, num_pix, partial,
-                                                         nparts, view);
+                                                         nparts, view, nzmask);
 }
 
 void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *sig_pair,
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
-                          int num_pix, const double *partial, int nparts, double *view) {
+                          int num_pix, const double *partial, int nparts, double *view,
+                          unsigned long long *nzmask) {
     pair_tomogram_launch(s, P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff, nranges, fpix, fcnt, psum_slot,
-                         t0, ntiles, num_pix, partial, nparts, view);
+                         t0, ntiles, num_pix, partial, nparts, view, nzmask);
 }
 
 void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *sig_pair,
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
-                          int num_pix, const double *partial, int nparts, float *view) {
+                          int num_pix, const double *partial, int nparts, float *view,
+                          unsigned long long *nzmask) {
     pair_tomogram_launch(s, P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff, nranges, fpix, fcnt, psum_slot,
-                         t0, ntiles, num_pix, partial, nparts, view);
+                         t0, ntiles, num_pix, partial, nparts, view, nzmask);
 }
 
 template <typename TP>
