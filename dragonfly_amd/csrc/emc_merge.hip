@@ -173,51 +173,63 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
         }
         head += count;
     };
-    // items are taken kItemsAhead at a time: their ids, then their views and masks, are loaded
-    // before the first is queued, so a wave has that many independent reads in flight
+    // a wave reads the ids and pixel masks of 64 list entries at once (lane = entry), masks
+    // them with the nonzero masks (or drops orientations with psum_r = 0), and then walks only
+    // the live items, kItemsAhead at a time: their views are loaded before the first is
+    // queued, so a wave has that many independent reads in flight
     constexpr int kItemsAhead = 4;
-    for (int k0 = u.y + wave; k0 < u.z; k0 += kWavesPerBlock * kItemsAhead) {
-        int it[kItemsAhead];
-        unsigned long long lm[kItemsAhead];  // pixels with a corner in this brick
-#pragma unroll
-        for (int a = 0; a < kItemsAhead; ++a) {
-            const int k = k0 + a * kWavesPerBlock;
-            it[a] = k < u.z ? list[k] : -1;
-            lm[a] = k < u.z ? lmask[k] : 0ull;
+    for (int kb = u.y + wave * 64; kb < u.z; kb += kWavesPerBlock * 64) {
+        const int k = kb + lane;
+        int id = -1;
+        unsigned long long m = 0ull;
+        if (k < u.z) {
+            id = list[k];
+            m = lmask[k];
         }
-        if (nzmask) {
-#pragma unroll
-            for (int a = 0; a < kItemsAhead; ++a)
-                if (it[a] >= 0) lm[a] &= nzmask[it[a]];  // item id = c * npatch + p
+        if (id >= 0) {
+            if (nzmask)
+                m &= nzmask[id];  // item id = c * npatch + p
+            else if (!all_on && !(psum_r[r0 + id / npatch] > 0.))
+                m = 0ull;
         }
-        double wv[kItemsAhead];
-        int rr[kItemsAhead];
+        unsigned long long live = __ballot(m != 0ull);
+        while (live) {
+            int ids[kItemsAhead];
+            unsigned long long ms[kItemsAhead];
 #pragma unroll
-        for (int a = 0; a < kItemsAhead; ++a) {
-            wv[a] = 0.;  // the view; divided by corr when its entry is drained
-            rr[a] = -1;
-            if (it[a] < 0 || (nzmask && lm[a] == 0ull)) continue;
-            const int c = it[a] / npatch, p = it[a] % npatch;
-            const int r = r0 + c, t = p * 64 + lane;
-            if (!nzmask && !all_on && !(psum_r[r] > 0.)) continue;
-            rr[a] = r;
-            // a set bit: t < num_pix, mask[t] <= 1
-            if ((lm[a] >> lane) & 1ull) wv[a] = (double)view[(size_t)c * num_pix + t];
-        }
-#pragma unroll
-        for (int a = 0; a < kItemsAhead; ++a) {
-            if (rr[a] < 0) continue;
-            const int t = (it[a] % npatch) * 64 + lane;
-            const bool nz = wv[a] != 0.;  // a zero view adds exactly nothing
-            const unsigned long long bal = __ballot(nz);
-            if (nz) {
-                const int at = (tail + __popcll(bal & ((1ull << lane) - 1))) & (kRing - 1);
-                q_r[wave][at] = rr[a];
-                q_t[wave][at] = t;
-                q_w[wave][at] = wv[a];
+            for (int a = 0; a < kItemsAhead; ++a) {
+                ids[a] = -1;
+                ms[a] = 0ull;
+                if (live) {
+                    const int i = __ffsll((long long)live) - 1;
+                    live &= live - 1ull;
+                    ids[a] = __builtin_amdgcn_readlane(id, i);
+                    ms[a] = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(m >> 32), i) << 32) |
+                            (unsigned)__builtin_amdgcn_readlane((int)m, i);
+                }
             }
-            tail += __popcll(bal);
-            if (tail - head >= 64) drain(64);
+            double wv[kItemsAhead];
+#pragma unroll
+            for (int a = 0; a < kItemsAhead; ++a) {
+                wv[a] = 0.;  // the view; divided by corr when its entry is drained
+                // a set bit: t < num_pix, mask[t] <= 1
+                if (ids[a] >= 0 && ((ms[a] >> lane) & 1ull))
+                    wv[a] = (double)view[(size_t)(ids[a] / npatch) * num_pix + (ids[a] % npatch) * 64 + lane];
+            }
+#pragma unroll
+            for (int a = 0; a < kItemsAhead; ++a) {
+                if (ids[a] < 0) continue;
+                const bool nz = wv[a] != 0.;  // a zero view adds exactly nothing
+                const unsigned long long bal = __ballot(nz);
+                if (nz) {
+                    const int at = (tail + __popcll(bal & ((1ull << lane) - 1))) & (kRing - 1);
+                    q_r[wave][at] = r0 + ids[a] / npatch;
+                    q_t[wave][at] = (ids[a] % npatch) * 64 + lane;
+                    q_w[wave][at] = wv[a];
+                }
+                tail += __popcll(bal);
+                if (tail - head >= 64) drain(64);
+            }
         }
     }
     if (tail > head) drain(tail - head);
