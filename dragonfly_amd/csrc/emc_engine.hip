@@ -238,6 +238,15 @@ struct emc_engine {
     // upper bound on the orientation chunk width (0: memory decides)
     int max_chunk_cols = 0;
     DevBuf<double> tom_partial;
+    DevBuf<float> tom_partial32;  // FP32 partial views of the full frames (subset path, FP32 gathers)
+    // log W of every slot (one orientation chunk), left by an expectation step of the subset
+    // path with the current model and geometry and not written over since: a later expectation
+    // step (a second pass over new lists) skips the expand. Cleared by every change of model,
+    // detector, orientations, modes, precision or options, by the M-step (the view takes the
+    // buffer), and by the plain path.
+    bool logw_resident = false;
+    const void *res_logw = nullptr, *res_u = nullptr, *res_rbad = nullptr;
+    int res_S = 0;
     // tile form of the tomogram list: tiles of 128 engine pixels -> frames -> photons
     DevBuf<long> ft_off, fe_off;
     DevBuf<int> ft_frame, fe_code;
@@ -695,6 +704,7 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
                             e->slot_pack_built == (int)pack &&
                             std::equal(e->h_slot_qmap.begin(), e->h_slot_qmap.end(), qmap);
     if (!same_slots) {
+        e->logw_resident = false;  // a new slot order
         std::vector<int> cnt(num_coarse, 0), tile0(num_coarse + 1, 0);
         for (int r = 0; r < Rn; ++r) ++cnt[qmap[r]];
         int T = 0;
@@ -1178,15 +1188,26 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     const int nparts = e->full_parts;
     // memory: the plain path's rows are not needed here
     e->R.release();
-    if (e->Rsub.cap < (size_t)std::max<long>(e->num_pairs, 1) * emc::kRTile ||
-        (nF > 0 && e->RF.cap < (size_t)nF * T * emc::kRTile)) {
+    const size_t rsub_need = (size_t)std::max<long>(e->num_pairs, 1) * emc::kRTile;
+    const size_t rf_need = nF > 0 ? (size_t)nF * T * emc::kRTile : 0;
+    if (e->Rsub.cap < rsub_need || e->RF.cap < rf_need) {
         // the rows grow: hand the orientation-chunk buffers' room back first, so that the
-        // chunk width below is sized on what the new rows leave free
-        e->logw.release();
-        e->C = 0;
-        e->bricks_valid = false;
+        // chunk width below is sized on what the new rows leave free; a resident log W (the
+        // second pass of a judged iteration) stays when the rows fit beside it
+        size_t freeb = 0, totalb = 0;
+        HIP_TRY(hipMemGetInfo(&freeb, &totalb));
+        const size_t grow = (e->Rsub.cap < rsub_need ? rsub_need : 0) * sizeof(double) +
+                            (e->RF.cap < rf_need ? rf_need : 0) * sizeof(double);
+        const bool keep_logw = e->logw_resident && freeb > grow + ((size_t)8 << 30);
+        if (!keep_logw) {
+            e->logw.release();
+            e->C = 0;
+            e->bricks_valid = false;
+            e->logw32.release();
+            e->logw_resident = false;
+        }
         e->tom_partial.release();
-        e->logw32.release();
+        e->tom_partial32.release();
         e->p32.release();
     }
     if ((rc = e->Rsub.alloc((size_t)std::max<long>(e->num_pairs, 1) * emc::kRTile))) return rc;
@@ -1209,12 +1230,15 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     const size_t reserve = ((size_t)4 << 30) + brick_list_extra(e->slot_bricks, S, P);
     size_t budget = (freeb > reserve ? freeb - reserve : 0) + e->logw.cap * sizeof(double) +
-                    e->tom_partial.cap * sizeof(double) + (e->logw32.cap + e->p32.cap) * sizeof(float);
+                    e->tom_partial.cap * sizeof(double) +
+                    (e->logw32.cap + e->p32.cap + e->tom_partial32.cap) * sizeof(float);
     budget = std::min(budget, (size_t)120 << 30);
-    // per slot column: log W / view, the full frames' partial views, and with FP32 gathers the
-    // FP32 log W and full-frame posterior columns
-    const size_t per_col = (size_t)P * sizeof(double) * (1 + (nF > 0 ? nparts : 0)) +
-                           (g32 ? ((size_t)P + nF) * sizeof(float) : 0) +
+    // per slot column: log W / view and the full frames' partial views (FP64; with FP32 gathers
+    // all FP32, log W and the view sharing one buffer, plus the full frames' FP32 posterior
+    // column), and the view's nonzero masks
+    const size_t parts_col = nF > 0 ? (size_t)nparts : 0;
+    const size_t per_col = (g32 ? (size_t)P * sizeof(float) * (1 + parts_col) + (size_t)nF * sizeof(float)
+                                : (size_t)P * sizeof(double) * (1 + parts_col)) +
                            (size_t)((P + 63) / 64) * sizeof(unsigned long long);
     size_t cmax = budget / per_col;
     if (e->max_chunk_cols > 0) cmax = std::min(cmax, (size_t)e->max_chunk_cols);
@@ -1227,13 +1251,20 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     if (e->slot_C > 0 && e->slot_C < C && e->slot_bricks_valid) C = e->slot_C;
     if (C != e->slot_C) e->slot_bricks_valid = false;
     e->slot_C = C;
-    if (e->logw.cap < (size_t)P * C) {
+    if (g32 ? e->logw.cap > 0 : e->logw.cap < (size_t)P * C) {
+        // the FP64 log W / view buffer: none with FP32 gathers (log W and the view are FP32)
         e->logw.release();
         e->C = 0;
         e->bricks_valid = false;
-        if ((rc = e->logw.alloc((size_t)P * C))) return rc;
+        if (!g32 && (rc = e->logw.alloc((size_t)P * C))) return rc;
     }
-    if (nF > 0 && (rc = e->tom_partial.alloc((size_t)nparts * P * (size_t)C))) return rc;
+    if (g32) {
+        e->tom_partial.release();
+        if (nF > 0 && (rc = e->tom_partial32.alloc((size_t)nparts * P * (size_t)C))) return rc;
+    } else {
+        e->tom_partial32.release();
+        if (nF > 0 && (rc = e->tom_partial.alloc((size_t)nparts * P * (size_t)C))) return rc;
+    }
     if (g32) {
         // 128-wide tiles: room for the chunk rounded up to 128 slots
         const size_t C128 = round_up((size_t)C, 128);
@@ -1285,6 +1316,12 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     // expectation step: expand, likelihood, per-frame maximum, normaliser and posterior, psum
     const bool fused = e->fused != 0;
     if (fused && (rc = e->fused_counter.alloc(1))) return rc;
+    // log W of every slot still in place from the previous expectation step (one chunk)
+    const void *logw_buf = g32 ? (const void *)e->logw32.p : (const void *)e->logw.p;
+    const bool one_chunk = C >= S;
+    const bool resident = one_chunk && !fused && !graph && e->logw_resident && e->res_logw == logw_buf &&
+                          e->res_u == e->u_slot.p && e->res_rbad == e->rbad_slot.p && e->res_S == S;
+    e->logw_resident = false;  // set again below once the expand has run
     auto estep_work = [&]() -> int {
             for (int s0 = 0; s0 < S; s0 += C) {
                 const int Cc = std::min(C, S - s0);
@@ -1321,17 +1358,27 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                         return rc;
                     continue;
                 }
-                if ((rc = timed(0, [&] {
-                         if (g32)
-                             emc::launch_expand(s, e->rots_slot.p, e->model32.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
-                                                P, s0, Cc, kPixPerGroup, e->logw32.p, e->upart.p, e->ubad.p, e->slot_rot.p);
-                         else
-                             emc::launch_expand(s, e->rots_slot.p, e->model1.p, e->size, e->qvals.p, e->corr.p, e->mask.p,
-                                                P, s0, Cc, kPixPerGroup, e->logw.p, e->upart.p, e->ubad.p, e->slot_rot.p);
-                         emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, s0, S, e->u_slot.p, e->rbad_slot.p);
-                         return 0;
-                     })))
+                if (!resident && (rc = timed(0, [&] {
+                                      if (g32)
+                                          emc::launch_expand(s, e->rots_slot.p, e->model32.p, e->size, e->qvals.p,
+                                                             e->corr.p, e->mask.p, P, s0, Cc, kPixPerGroup,
+                                                             e->logw32.p, e->upart.p, e->ubad.p, e->slot_rot.p);
+                                      else
+                                          emc::launch_expand(s, e->rots_slot.p, e->model1.p, e->size, e->qvals.p,
+                                                             e->corr.p, e->mask.p, P, s0, Cc, kPixPerGroup,
+                                                             e->logw.p, e->upart.p, e->ubad.p, e->slot_rot.p);
+                                      emc::launch_u_reduce(s, e->upart.p, e->ubad.p, ngroups, Cc, s0, S, e->u_slot.p,
+                                                           e->rbad_slot.p);
+                                      return 0;
+                                  })))
                     return rc;
+                if (one_chunk && !capturing) {
+                    e->logw_resident = true;
+                    e->res_logw = logw_buf;
+                    e->res_u = e->u_slot.p;
+                    e->res_rbad = e->rbad_slot.p;
+                    e->res_S = S;
+                }
                 if ((rc = timed(1, [&] {
                          const int t0 = s0 / emc::kRTile, t1 = (s0 + Cc) / emc::kRTile;
                          if (g32) {
@@ -1417,6 +1464,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     // M-step: tomogram and merge chunk by chunk. device_weights: the choice between the cached
     // weights and merging them is made on the device (no read-back), as a graph needs
     auto mstep_work = [&](bool device_weights) -> int {
+        e->logw_resident = false;  // the views go into the log W buffer
         bool cached_weight = false;
         if (device_weights) {
             emc::launch_count_unmerged(s, e->psum_r.p, Rn, e->d_count.p);
@@ -1452,7 +1500,8 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                                if (g32) {
                                    emc::launch_to_f32(s, e->RF.p, nF, bc.r0, bc.Cc, e->p32.p);
                                    emc::launch_tomogram_parts_f32(s, e->p32.p, nF, bc.Cc, e->ftom_off2.p, nparts,
-                                                                  e->ftom_frame.p, e->ftom_cnt.p, P, e->tom_partial.p);
+                                                                  e->ftom_frame.p, e->ftom_cnt.p, P,
+                                                                  e->tom_partial32.p);
                                } else {
                                    emc::launch_tomogram_parts(s, e->RF.p, nF, bc.r0, bc.Cc, e->ftom_off2.p, nparts,
                                                               e->ftom_frame.p, e->ftom_cnt.p, nullptr, S, P,
@@ -1470,7 +1519,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                          emc::launch_pair_tomogram(s, e->Rsub32.p, e->tile_off.p, e->sig_pair.p, e->sig_frame.p,
                                                    e->sig_cnt.p, e->fsort_off.p, e->fsort_roff.p, e->nranges,
                                                    e->fsort_pix.p, e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
-                                                   nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw32.p, nz);
+                                                   nF > 0 ? e->tom_partial32.p : nullptr, nparts, e->logw32.p, nz);
                      else
                          emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->sig_pair.p, e->sig_frame.p,
                                                    e->sig_cnt.p, e->fsort_off.p, e->fsort_roff.p, e->nranges,
@@ -1675,7 +1724,7 @@ int emc_create(int device, emc_engine **out) {
 int emc_set_option(emc_engine *e, const char *name, int value) {
     if (!e || !name) return fail(EMC_E_ARG, "emc_set_option: null argument");
     // the two pass switches leave the iteration's state as it is; every other option may not
-    if (strcmp(name, "estep_only") && strcmp(name, "mstep_only")) ++e->gen, ++e->layout_gen;
+    if (strcmp(name, "estep_only") && strcmp(name, "mstep_only")) ++e->gen, ++e->layout_gen, e->logw_resident = false;
     for (const EngineOption &o : kOptions)
         if (!strcmp(o.name, name)) return set_option(e, o, value);
     return fail(EMC_E_ARG, "emc_set_option: unknown option '%s'", name);
@@ -1741,6 +1790,7 @@ int emc_synchronize(emc_engine *e) {
 int emc_set_detector(emc_engine *e, int num_pix, const double *qvals, const double *corr,
                      const uint8_t *raw_mask) {
     if (e) ++e->gen, ++e->layout_gen;
+    if (e) e->logw_resident = false;
     if (!e || num_pix <= 0 || !qvals || !corr || !raw_mask) return fail(EMC_E_ARG, "emc_set_detector: bad argument");
     HIP_TRY(hipSetDevice(e->device));
     std::vector<int> perm = morton_order(num_pix, qvals);
@@ -1805,7 +1855,7 @@ static int upload_quats(emc_engine *e) {
 
 int emc_set_modes(emc_engine *e, int num_modes) {
     if (e) ++e->gen, ++e->layout_gen;
-    if (e) e->model32_valid = false;  // model1 changes below
+    if (e) e->model32_valid = false, e->logw_resident = false;  // model1 changes below
     if (!e || num_modes < 1 || num_modes > EMC_MAX_MODES)
         return fail(EMC_E_ARG, "emc_set_modes: num_modes must be 1..%d", EMC_MAX_MODES);
     if (num_modes == e->num_modes) return 0;
@@ -1824,6 +1874,7 @@ int emc_set_modes(emc_engine *e, int num_modes) {
 
 int emc_set_quats(emc_engine *e, int num_rot, const double *quats) {
     if (e) ++e->gen, ++e->layout_gen;
+    if (e) e->logw_resident = false;
     if (!e || num_rot <= 0 || !quats) return fail(EMC_E_ARG, "emc_set_quats: bad argument");
     HIP_TRY(hipSetDevice(e->device));
     e->h_quats.assign(quats, quats + 5 * (size_t)num_rot);
@@ -1841,7 +1892,7 @@ int emc_set_quats(emc_engine *e, int num_rot, const double *quats) {
 
 int emc_set_model(emc_engine *e, long size, const double *model1) {
     if (e) ++e->gen;  // the same buffer unless the size changes (then it is allocated anew)
-    if (e) e->model32_valid = false;  // model1 changes below
+    if (e) e->model32_valid = false, e->logw_resident = false;  // model1 changes below
     if (!e || size < 3 || !model1) return fail(EMC_E_ARG, "emc_set_model: bad argument");
     HIP_TRY(hipSetDevice(e->device));
     const size_t vol = (size_t)size * size * size * e->num_modes;  // every mode's volume
@@ -1860,6 +1911,7 @@ int emc_set_model(emc_engine *e, long size, const double *model1) {
 
 int emc_set_background(emc_engine *e, const double *background, const double *bgscale) {
     if (e) ++e->gen, ++e->layout_gen;
+    if (e) e->logw_resident = false;
     if (!e) return fail(EMC_E_ARG, "null engine");
     if (!background) {
         e->with_bg = false;
@@ -2077,6 +2129,7 @@ int emc_cell_topk(emc_engine *e, int num_coarse, const int *quat_mapping, int k,
 
 int emc_set_precision(emc_engine *e, int mode) {
     if (e) ++e->gen, ++e->layout_gen;
+    if (e) e->logw_resident = false;
     if (!e) return fail(EMC_E_ARG, "null engine");
     if (mode < 0 || mode > 5) return fail(EMC_E_ARG, "emc_set_precision: mode %d (0 to 5)", mode);
     if (mode != e->precision) {
@@ -2344,6 +2397,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
         return fail(EMC_E_STATE, "emc_maximize: mstep_only runs on the refinement-subset path only");
     if (e->refine && e->subset_mode && e->subset_ready && !mp) return maximize_subset(e, opts, stats);
     e->subset_ran = false;
+    e->logw_resident = false;  // the plain path's chunks use the log W buffers
     if (mp && e->num_modes > 1)
         return fail(EMC_E_STATE, "emc_maximize: the mixed-precision path takes one mode");
     if (mp) {
@@ -2631,7 +2685,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
 
 int emc_update_model(emc_engine *e, double alpha, double *rms_change) {
     if (e) ++e->gen;
-    if (e) e->model32_valid = false;  // model1 changes below
+    if (e) e->model32_valid = false, e->logw_resident = false;  // model1 changes below
     if (!e || !e->size) return fail(EMC_E_STATE, "emc_update_model: model not set");
     HIP_TRY(hipSetDevice(e->device));
     int rc;
@@ -2650,7 +2704,7 @@ int emc_update_model(emc_engine *e, double alpha, double *rms_change) {
 
 int emc_normalize_scale(emc_engine *e, double *mean_scale) {
     if (e) ++e->gen;
-    if (e) e->model32_valid = false;  // model1 changes below
+    if (e) e->model32_valid = false, e->logw_resident = false;  // model1 changes below
     if (!e || !e->have_params || !e->size) return fail(EMC_E_STATE, "emc_normalize_scale: state not set");
     HIP_TRY(hipSetDevice(e->device));
     int rc;

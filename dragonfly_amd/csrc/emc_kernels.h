@@ -41,6 +41,10 @@ void launch_likelihood_mapped(hipStream_t s, const float *logw, int C, int num_p
 void launch_tomogram_parts_f32(hipStream_t s, const float *P32, int num_data, int C, const long *toff2,
                                int nparts, const int *tframe, const double *tcnt, int num_pix,
                                double *partial);
+// the same with FP32 partial sums (each the FP64 sum rounded once)
+void launch_tomogram_parts_f32(hipStream_t s, const float *P32, int num_data, int C, const long *toff2,
+                               int nparts, const int *tframe, const double *tcnt, int num_pix,
+                               float *partial);
 void launch_u_reduce(hipStream_t s, const double *upart, const int *ubad, int ngroups, int C,
                      int r0, int num_rot, double *u, int *rbad);
 void launch_rescale(hipStream_t s, const double *quats, const double *u, int num_rot,
@@ -242,7 +246,7 @@ void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, 
 void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *sig_pair,
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
-                          int num_pix, const double *partial, int nparts, float *view, unsigned long long *nzmask);
+                          int num_pix, const float *partial, int nparts, float *view, unsigned long long *nzmask);
 // pairs with some P > 1e-6 in their row, per tile in the tile's order (flag: num_pairs bytes;
 // sig_pair / sig_frame: tile_off layout; sig_cnt: per tile)
 void launch_sig_pairs(hipStream_t s, const double *P, long num_pairs, const long *tile_off, const int *tile_pair,

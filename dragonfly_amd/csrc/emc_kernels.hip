@@ -1159,10 +1159,10 @@ __global__ __launch_bounds__(kBlock) void likelihood_w128_kernel(
 // Frame-partitioned tomogram over an FP32 posterior in 128-wide tiles (see to_f32_w128_kernel):
 // each lane gathers two orientations per photon; partial sums in FP64, written in the 64-wide
 // tile-major partial layout of tomogram_part_kernel.
-template <int U>
+template <int U, typename OT>
 __global__ __launch_bounds__(kBlock) void tomogram_part_w128_kernel(
     const float *__restrict__ P32, size_t tsP, int C, const long *__restrict__ toff2, int nparts,
-    const int *__restrict__ tframe, const double *__restrict__ tcnt, int num_pix, double *__restrict__ partial) {
+    const int *__restrict__ tframe, const double *__restrict__ tcnt, int num_pix, OT *__restrict__ partial) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int part = blockIdx.x % nparts;
@@ -1193,9 +1193,9 @@ __global__ __launch_bounds__(kBlock) void tomogram_part_w128_kernel(
     const int c = tile * 128 + 2 * lane;
     if (c < C) {
         const size_t tsW = (size_t)num_pix * kRTile, tsPart = tsW * (C / kRTile);
-        double2 *dst = reinterpret_cast<double2 *>(partial + part * tsPart + (size_t)(c / kRTile) * tsW +
-                                                   (size_t)t * kRTile + (c % kRTile));
-        *dst = make_double2(a0, a1);
+        OT *dst = partial + part * tsPart + (size_t)(c / kRTile) * tsW + (size_t)t * kRTile + (c % kRTile);
+        dst[0] = (OT)a0;
+        dst[1] = (OT)a1;
     }
 }
 
@@ -1506,8 +1506,16 @@ void launch_tomogram_parts_f32(hipStream_t s, const float *P32, int num_data, in
                                double *partial) {
     const int ngroups = (int)cdiv(num_pix, kWavesPerBlock);
     dim3 grid(ngroups * nparts, cdiv(C, 128));
-    tomogram_part_w128_kernel<16><<<grid, kBlock, 0, s>>>(P32, (size_t)num_data * 128, C, toff2, nparts, tframe,
-                                                          tcnt, num_pix, partial);
+    tomogram_part_w128_kernel<16, double><<<grid, kBlock, 0, s>>>(P32, (size_t)num_data * 128, C, toff2, nparts,
+                                                                  tframe, tcnt, num_pix, partial);
+}
+
+void launch_tomogram_parts_f32(hipStream_t s, const float *P32, int num_data, int C, const long *toff2,
+                               int nparts, const int *tframe, const double *tcnt, int num_pix, float *partial) {
+    const int ngroups = (int)cdiv(num_pix, kWavesPerBlock);
+    dim3 grid(ngroups * nparts, cdiv(C, 128));
+    tomogram_part_w128_kernel<16, float><<<grid, kBlock, 0, s>>>(P32, (size_t)num_data * 128, C, toff2, nparts,
+                                                                 tframe, tcnt, num_pix, partial);
 }
 
 void launch_tomogram_reduce_rmajor(hipStream_t s, const double *partial, int nparts, int r0, int C,

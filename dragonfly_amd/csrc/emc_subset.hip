@@ -370,13 +370,13 @@ __global__ __launch_bounds__(kBlock) void sig_compact_kernel(const long *__restr
 // (VT: double, or float for the FP32-sum merge). The tile's pairs are examined kBlock at a
 // time, one per thread, and only those with photons in the range are queued for the waves,
 // so a pair costs one row of P per range it has photons in.
-template <typename TP, typename VT>
+template <typename TP, typename VT, typename PT>
 __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     const TP *__restrict__ P, const long *__restrict__ tile_off, const int *__restrict__ tile_pair,
     const int *__restrict__ tile_frame, const int *__restrict__ sig_cnt, const long *__restrict__ foff,
     const int *__restrict__ roff, int nranges, const int *__restrict__ fpix, const double *__restrict__ fcnt,
     const double *__restrict__ psum_slot, int t0, int ntiles, int num_pix,
-    const double *__restrict__ partial, int nparts, VT *__restrict__ view,
+    const PT *__restrict__ partial, int nparts, VT *__restrict__ view,
     unsigned long long *__restrict__ nzmask) {
     static_assert(kPairRange == 64, "a pixel range is one 64-pixel merge patch");
     __shared__ double acc[kPairRange][kRTile + 1];
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
             double a = 0.;
             if (t < num_pix) {
                 const size_t at = (size_t)lt * tsW + (size_t)t * kRTile + lane;
-                for (int x = 0; x < nparts; ++x) a += partial[x * tsPart + at];
+                for (int x = 0; x < nparts; ++x) a += (double)partial[x * tsPart + at];
             }
             acc[i][lane] = a;
         }
@@ -781,15 +781,15 @@ void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, cons
         P, tile_off, tile_pair, pair_frame, num_tiles, scale, need_scaling, psum_full, psum_slot);
 }
 
-template <typename TP, typename VT>
+template <typename TP, typename VT, typename PT>
 static void pair_tomogram_launch(hipStream_t s, const TP *P, const long *tile_off, const int *sig_pair,
                                  const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff,
                                  int nranges, const int *fpix, const double *fcnt, const double *psum_slot, int t0,
-                                 int ntiles, int num_pix, const double *partial, int nparts, VT *view,
+                                 int ntiles, int num_pix, const PT *partial, int nparts, VT *view,
                                  unsigned long long *nzmask) {
     if (ntiles <= 0) return;
     const unsigned grid = (unsigned)((ntiles + kNumXcd - 1) / kNumXcd * kNumXcd) * (unsigned)nranges;
-    pair_tomogram_kernel<TP, VT><<<grid, kBlock, 0, s>>>(P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff,
+    pair_tomogram_kernel<TP, VT, PT><<<grid, kBlock, 0, s>>>(P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff,
                                                          nranges, fpix, fcnt, psum_slot, t0, ntiles, num_pix, partial,
                                                          nparts, view, nzmask);
 }
@@ -806,7 +806,7 @@ void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, 
 void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *sig_pair,
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
-                          int num_pix, const double *partial, int nparts, float *view,
+                          int num_pix, const float *partial, int nparts, float *view,
                           unsigned long long *nzmask) {
     pair_tomogram_launch(s, P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff, nranges, fpix, fcnt, psum_slot,
                          t0, ntiles, num_pix, partial, nparts, view, nzmask);

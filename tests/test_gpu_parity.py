@@ -595,6 +595,55 @@ def test_estep_then_mstep_resume_equals_one_pass(tiny_scaled, g32):
         assert err <= (1e-5 if g32 else 1e-13), k
 
 
+@pytest.mark.parametrize('g32', [False, True])
+def test_second_pass_reuses_resident_log_w(tiny_scaled, g32):
+    """An estep_only pass over one set of refinement lists, then a whole iteration over another
+    set on the same slots (the judged iteration's second pass) skips the expand, log W of every
+    slot being still in place (one orientation chunk): its results equal those of an engine
+    that runs the second lists alone (bit for bit up to the merge's atomic order), and a
+    model change in between makes the expand run again."""
+    p = tiny_scaled
+    p = dict(p, fparams=dict(p['fparams'], scale=np.asarray(p['true_scale'], 'f8').copy()))
+    qc, qmap, off1, cells1 = _refine_setup(p, seed=5)
+    _, _, off2, cells2 = _refine_setup(p, seed=9, ncells=12)
+    flags = dict(iteration=2, need_scaling=1, update_scale=1)
+    out = []
+    for two in (False, True):
+        eng = P.engine_for(p)
+        if g32:
+            eng.set_precision(3)
+        if two:
+            eng.set_refine(len(qc), qmap, off1, cells1)
+            eng.set_option('estep_only', 1)
+            st1 = eng.maximize(2, 1, 1, 0)
+            eng.set_option('estep_only', 0)
+            assert st1['stage_ms']['expand'] > 0.
+            eng.set_scale(p['fparams']['scale'])
+        eng.set_refine(len(qc), qmap, off2, cells2)
+        r = P.run_engine(eng, flags)
+        out.append(r)
+        if two:
+            assert r['stage_ms']['expand'] == 0., 'the second pass ran the expand'
+            # after the iteration (views written over log W) and a model change: expand again
+            eng.set_model(p['model1'])
+            eng.set_scale(p['fparams']['scale'])
+            eng.set_option('estep_only', 1)
+            st3 = eng.maximize(2, 1, 1, 0)
+            assert st3['stage_ms']['expand'] > 0.
+        else:
+            assert r['stage_ms']['expand'] > 0.
+        eng.close()
+    a, b = out
+    for k in ('u', 'max_exp', 'p_norm', 'rmax', 'likelihood', 'info', 'psum_d', 'psum_r', 'scale', 'prob'):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ('avg_likelihood', 'mutual_info', 'rescale'):
+        assert a[k] == b[k], k
+    for k in ('model2', 'inter_weight'):
+        err = P.rel_err(b[k], a[k])
+        print('resident log W %s: %.1e' % (k, err), flush=True)
+        assert err <= (1e-5 if g32 else 1e-13), k
+
+
 def test_judge_outside_estimate(tiny_scaled):
     """emc_judge_outside on the tiny problem (fine num_div 2, coarse num_div 1): the estimate
     equals its definition recomputed from the two engines' results (the coarse posterior,
