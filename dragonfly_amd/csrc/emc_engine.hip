@@ -151,6 +151,7 @@ std::vector<int> morton_order(int num_pix, const double *qvals) {
 constexpr int kPixPerGroup = 1024;
 constexpr int kFramesPerPsumPart = 512;
 constexpr int kFramesPerTomoPart = 12500;  // default of the full_part_frames option
+constexpr int kSigFullRowsPerTile = 1024;  // full-frame rows read by the pair tomogram, mean per tile
 constexpr int kModelParts = 1024;
 
 // Staging memory of emc_slice_gen_static: the rotation matrix at [0, 9) and the device copy
@@ -368,6 +369,11 @@ struct emc_engine {
     DevBuf<uint8_t> pair_flag;
     // per (chunk slot, 64-pixel patch): pixels with a nonzero view (pair tomogram -> merge)
     DevBuf<unsigned long long> nzmask;
+    // full frames' rows with some P > 1e-6, per tile (T * nF layout), their count per tile and
+    // in total: the pair tomogram reads them from RF when they are few
+    DevBuf<uint8_t> full_flag;
+    DevBuf<int> sigF_row, sigF_frame, sigF_cnt;
+    DevBuf<long> sigF_total;
     std::vector<BrickChunk> slot_bricks;
     bool slot_bricks_valid = false;
     // frame photon lists of mask < 2 pixels sorted by engine pixel, with per-range offsets
@@ -1225,6 +1231,14 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     if ((rc = e->sig_frame.alloc((size_t)std::max<long>(e->num_pairs, 1)))) return rc;
     if ((rc = e->pair_flag.alloc((size_t)std::max<long>(e->num_pairs, 1)))) return rc;
     if ((rc = e->sig_cnt.alloc((size_t)std::max(T, 1)))) return rc;
+    if (nF > 0) {
+        const size_t rows = (size_t)nF * T;
+        if ((rc = e->full_flag.alloc(rows))) return rc;
+        if ((rc = e->sigF_row.alloc(rows))) return rc;
+        if ((rc = e->sigF_frame.alloc(rows))) return rc;
+        if ((rc = e->sigF_cnt.alloc((size_t)T))) return rc;
+        if ((rc = e->sigF_total.alloc(1))) return rc;
+    }
     if (g32 && (rc = ensure_model32(e))) return rc;
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
@@ -1434,6 +1448,9 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                                            opts->need_scaling, nF > 0 ? e->psum_full.p : nullptr, e->psum_slot.p);
                      emc::launch_slot_gather(s, e->psum_slot.p, e->rot_slot.p, Rn, e->psum_r.p);
                      emc::launch_slot_gather(s, e->u_slot.p, e->rot_slot.p, Rn, e->u.p);
+                     if (nF > 0)
+                         emc::launch_sig_full_rows(s, e->RF.p, nF, T, e->fidx.p, e->full_flag.p, e->sigF_row.p,
+                                                   e->sigF_frame.p, e->sigF_cnt.p, e->sigF_total.p);
                      if (g32) {
                          emc::launch_pair_to_f32(s, e->Rsub.p, e->num_pairs * emc::kRTile, e->Rsub32.p);
                          emc::launch_sig_pairs(s, e->Rsub32.p, e->num_pairs, e->tile_off.p, e->tile_pair.p,
@@ -1493,10 +1510,25 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                                   })))
                 return rc;
         }
+        // the full frames' significant rows go to the pair tomogram directly when they average
+        // at most kSigFullRowsPerTile a tile (its blocks walk every row of their tile); else
+        // their tomogram is the plain path's, into partial views (also inside a graph, which
+        // cannot read the count back)
+        bool sig_full = false;
+        if (nF > 0 && !device_weights) {
+            long tot = 0;
+            HIP_TRY(hipMemcpyAsync(&tot, e->sigF_total.p, sizeof(long), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            sig_full = tot <= (long)kSigFullRowsPerTile * T;
+        }
+        const emc::SigFullRows frows =
+            sig_full ? emc::SigFullRows{e->RF.p, nF, e->sigF_row.p, e->sigF_frame.p, e->sigF_cnt.p}
+                     : emc::SigFullRows{};
+        const bool parts = nF > 0 && !sig_full;
         for (size_t k = 0; k < e->slot_bricks.size(); ++k) {
             const auto &bc = e->slot_bricks[k];
             const int t0 = bc.r0 / emc::kRTile, t1 = (bc.r0 + bc.Cc) / emc::kRTile;
-            if (nF > 0 && (rc = timed(6, [&] {
+            if (parts && (rc = timed(6, [&] {
                                if (g32) {
                                    emc::launch_to_f32(s, e->RF.p, nF, bc.r0, bc.Cc, e->p32.p);
                                    emc::launch_tomogram_parts_f32(s, e->p32.p, nF, bc.Cc, e->ftom_off2.p, nparts,
@@ -1519,12 +1551,14 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                          emc::launch_pair_tomogram(s, e->Rsub32.p, e->tile_off.p, e->sig_pair.p, e->sig_frame.p,
                                                    e->sig_cnt.p, e->fsort_off.p, e->fsort_roff.p, e->nranges,
                                                    e->fsort_pix.p, e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
-                                                   nF > 0 ? e->tom_partial32.p : nullptr, nparts, e->logw32.p, nz);
+                                                   parts ? e->tom_partial32.p : nullptr, nparts, e->logw32.p, nz,
+                                                   frows);
                      else
                          emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->sig_pair.p, e->sig_frame.p,
                                                    e->sig_cnt.p, e->fsort_off.p, e->fsort_roff.p, e->nranges,
                                                    e->fsort_pix.p, e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
-                                                   nF > 0 ? e->tom_partial.p : nullptr, nparts, e->logw.p, nz);
+                                                   parts ? e->tom_partial.p : nullptr, nparts, e->logw.p, nz,
+                                                   frows);
                      return 0;
                  })))
                 return rc;

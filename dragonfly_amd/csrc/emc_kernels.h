@@ -235,6 +235,17 @@ void launch_pair_posterior(hipStream_t s, double *R, const long *pair_off, const
 void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                       const int *pair_frame, int num_tiles, const double *scale, int need_scaling,
                       const double *psum_full, double *psum_slot);
+// Significant full-frame rows per tile (launch_sig_full_rows), read by the pair tomogram from
+// RF in place of partial sums; RF null: none.
+struct SigFullRows {
+    const double *RF = nullptr;
+    int nF = 0;
+    const int *row = nullptr, *frame = nullptr, *cnt = nullptr;
+};
+// flag: num_tiles * nF bytes; sigF_row / sigF_frame: num_tiles * nF; sigF_cnt: num_tiles;
+// total: the sum of sigF_cnt
+void launch_sig_full_rows(hipStream_t s, const double *RF, int nF, int num_tiles, const int *fidx, uint8_t *flag,
+                          int *sigF_row, int *sigF_frame, int *sigF_cnt, long *total);
 // sig_pair / sig_frame / sig_cnt: the tiles' significant pairs (launch_sig_pairs). nzmask
 // (nullable): per (chunk slot, 64-pixel patch) the mask of pixels with a nonzero view; with it
 // only those view entries are written.
@@ -242,11 +253,12 @@ void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, 
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
                           int num_pix, const double *partial, int nparts, double *view,
-                          unsigned long long *nzmask);
+                          unsigned long long *nzmask, const SigFullRows &fr = SigFullRows());
 void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *sig_pair,
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
-                          int num_pix, const float *partial, int nparts, float *view, unsigned long long *nzmask);
+                          int num_pix, const float *partial, int nparts, float *view, unsigned long long *nzmask,
+                          const SigFullRows &fr = SigFullRows());
 // pairs with some P > 1e-6 in their row, per tile in the tile's order (flag: num_pairs bytes;
 // sig_pair / sig_frame: tile_off layout; sig_cnt: per tile)
 void launch_sig_pairs(hipStream_t s, const double *P, long num_pairs, const long *tile_off, const int *tile_pair,

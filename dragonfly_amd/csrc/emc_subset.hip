@@ -361,15 +361,64 @@ __global__ __launch_bounds__(kBlock) void sig_compact_kernel(const long *__restr
     if (threadIdx.x == 0) sig_cnt[T] = base;
 }
 
+// Full frames' rows RF[T][f][64] with some P > 1e-6, per tile T, f ascending:
+// sigF_row / sigF_frame[T * nF + i] = f / fidx[f] for i < sigF_cnt[T] (flag: one byte per row,
+// pair_sig_kernel over the T * nF rows).
+__global__ __launch_bounds__(kBlock) void full_sig_compact_kernel(const uint8_t *__restrict__ flag, int nF,
+                                                                  const int *__restrict__ fidx,
+                                                                  int *__restrict__ sigF_row,
+                                                                  int *__restrict__ sigF_frame,
+                                                                  int *__restrict__ sigF_cnt) {
+    __shared__ int s_w[kWavesPerBlock];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t base_row = (size_t)blockIdx.x * nF;
+    int base = 0;
+    for (int fb = 0; fb < nF; fb += kBlock) {
+        const int f = fb + threadIdx.x;
+        const bool sig = f < nF && flag[base_row + f] != 0;
+        const unsigned long long b = __ballot(sig);
+        if (lane == 0) s_w[wave] = __popcll(b);
+        __syncthreads();
+        int before = base, total = base;
+        for (int w = 0; w < kWavesPerBlock; ++w) {
+            if (w < wave) before += s_w[w];
+            total += s_w[w];
+        }
+        if (sig) {
+            const size_t at = base_row + before + __popcll(b & ((1ull << lane) - 1ull));
+            sigF_row[at] = f;
+            sigF_frame[at] = fidx[f];
+        }
+        base = total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sigF_cnt[blockIdx.x] = base;
+}
+
+// *total = sum of cnt[0 .. n) (one block)
+__global__ __launch_bounds__(kBlock) void sum_counts_kernel(const int *__restrict__ cnt, int n, long *__restrict__ total) {
+    __shared__ long s[kBlock];
+    long a = 0;
+    for (int i = threadIdx.x; i < n; i += kBlock) a += cnt[i];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = s[0];
+}
+
 // Tomogram of the chunk's tiles [t0, t0 + ntiles) over pixel ranges of kPairRange pixels:
 // block (tile, range) starts from the full frames' partial sums (tomogram_part_kernel
 // layout partial[x][tile][t][64], added over the nparts frame ranges in order) or from zero,
 // adds K * P (P > 1e-6) of the tile's significant partial-frame pairs (sig_pair, sig_cnt)
-// into LDS rows of the range's pixels (frame photon lists sorted by pixel, range offsets
-// roff[d][range]), divides by psum and writes the orientation-major view[(T - t0) * 64 + j][t]
-// (VT: double, or float for the FP32-sum merge). The tile's pairs are examined kBlock at a
-// time, one per thread, and only those with photons in the range are queued for the waves,
-// so a pair costs one row of P per range it has photons in.
+// and, with RF, of its significant full-frame rows (sigF_*: rows RF[T][f][64], in place of the
+// partial sums) into LDS rows of the range's pixels (frame photon lists sorted by pixel, range
+// offsets roff[d][range]), divides by psum and writes the orientation-major
+// view[(T - t0) * 64 + j][t] (VT: double, or float for the FP32-sum merge). The tile's rows
+// are examined kBlock at a time, one per thread, and only those with photons in the range are
+// queued for the waves, so a row costs one read per range it has photons in.
 template <typename TP, typename VT, typename PT>
 __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     const TP *__restrict__ P, const long *__restrict__ tile_off, const int *__restrict__ tile_pair,
@@ -377,10 +426,12 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     const int *__restrict__ roff, int nranges, const int *__restrict__ fpix, const double *__restrict__ fcnt,
     const double *__restrict__ psum_slot, int t0, int ntiles, int num_pix,
     const PT *__restrict__ partial, int nparts, VT *__restrict__ view,
-    unsigned long long *__restrict__ nzmask) {
+    unsigned long long *__restrict__ nzmask, const double *__restrict__ RF, int nF,
+    const int *__restrict__ sigF_row, const int *__restrict__ sigF_frame, const int *__restrict__ sigF_cnt) {
     static_assert(kPairRange == 64, "a pixel range is one 64-pixel merge patch");
     __shared__ double acc[kPairRange][kRTile + 1];
-    __shared__ int s_k[kBlock];
+    __shared__ long s_row[kBlock];  // element offset of the row (P or RF)
+    __shared__ bool s_full[kBlock];
     __shared__ long s_q0[kBlock], s_q1[kBlock];
     __shared__ int s_n;
     // XCD-aware order: workgroups go round-robin to the kNumXcd XCDs, so XCD x takes tiles
@@ -393,12 +444,14 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     const long j0 = tile_off[T];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int npatch = (num_pix + 63) / 64;
-    if (j0 == tile_off[T + 1] && !partial) {
+    const bool rows_full = RF != nullptr;
+    if (j0 == tile_off[T + 1] && !partial && !rows_full) {
         // no frame takes this tile: psum 0, views 0 (unwritten), the merge skips the items
         if (nzmask && tid < kRTile) nzmask[((size_t)lt * kRTile + tid) * npatch + rg] = 0ull;
         return;
     }
-    const long j1 = j0 + sig_cnt[T];
+    const long nP = sig_cnt[T], nRows = nP + (rows_full ? sigF_cnt[T] : 0);
+    const size_t frow0 = (size_t)T * nF;
     const int p0 = rg * kPairRange;
     if (partial) {
         const size_t tsW = (size_t)num_pix * kRTile, tsPart = tsW * ntiles;
@@ -414,18 +467,28 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     } else {
         for (int i = tid; i < kPairRange * (kRTile + 1); i += kBlock) (&acc[0][0])[i] = 0.;
     }
-    for (long jb = j0; jb < j1; jb += kBlock) {
+    for (long jb = 0; jb < nRows; jb += kBlock) {
         if (tid == 0) s_n = 0;
         __syncthreads();
         const long j = jb + tid;
-        if (j < j1) {
-            const int k = tile_pair[j];
-            const int d = tile_frame[j];
+        if (j < nRows) {
+            long row;
+            int d;
+            const bool full = j >= nP;
+            if (!full) {
+                row = (long)tile_pair[j0 + j] * kRTile;
+                d = tile_frame[j0 + j];
+            } else {
+                const size_t i = frow0 + (size_t)(j - nP);
+                row = ((long)frow0 + sigF_row[i]) * kRTile;
+                d = sigF_frame[i];
+            }
             const int *__restrict__ rr = roff + (size_t)d * (nranges + 1);
             const int q0 = rr[rg], q1 = rr[rg + 1];
             if (q0 < q1) {
                 const int at = atomicAdd(&s_n, 1);
-                s_k[at] = k;
+                s_row[at] = row;
+                s_full[at] = full;
                 s_q0[at] = foff[d] + q0;
                 s_q1[at] = foff[d] + q1;
             }
@@ -445,7 +508,7 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
                 const int i = i0 + u * kWavesPerBlock;
                 qa[u] = i < n ? s_q0[i] : 0;
                 qb[u] = i < n ? s_q1[i] : 0;
-                p[u] = i < n ? (double)P[(size_t)s_k[i] * kRTile + lane] : 0.;
+                p[u] = i < n ? (s_full[i] ? RF[s_row[i] + lane] : (double)P[s_row[i] + lane]) : 0.;
                 px0[u] = i < n ? fpix[qa[u]] : 0;
                 c0[u] = i < n ? fcnt[qa[u]] : 0.;
             }
@@ -786,30 +849,31 @@ static void pair_tomogram_launch(hipStream_t s, const TP *P, const long *tile_of
                                  const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff,
                                  int nranges, const int *fpix, const double *fcnt, const double *psum_slot, int t0,
                                  int ntiles, int num_pix, const PT *partial, int nparts, VT *view,
-                                 unsigned long long *nzmask) {
+                                 unsigned long long *nzmask, const SigFullRows &fr) {
     if (ntiles <= 0) return;
     const unsigned grid = (unsigned)((ntiles + kNumXcd - 1) / kNumXcd * kNumXcd) * (unsigned)nranges;
     pair_tomogram_kernel<TP, VT, PT><<<grid, kBlock, 0, s>>>(P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff,
-                                                         nranges, fpix, fcnt, psum_slot, t0, ntiles, num_pix, partial,
-                                                         nparts, view, nzmask);
+                                                             nranges, fpix, fcnt, psum_slot, t0, ntiles, num_pix,
+                                                             partial, nparts, view, nzmask, fr.RF, fr.nF, fr.row,
+                                                             fr.frame, fr.cnt);
 }
 
 void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *sig_pair,
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
                           int num_pix, const double *partial, int nparts, double *view,
-                          unsigned long long *nzmask) {
+                          unsigned long long *nzmask, const SigFullRows &fr) {
     pair_tomogram_launch(s, P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff, nranges, fpix, fcnt, psum_slot,
-                         t0, ntiles, num_pix, partial, nparts, view, nzmask);
+                         t0, ntiles, num_pix, partial, nparts, view, nzmask, fr);
 }
 
 void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *sig_pair,
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
                           int num_pix, const float *partial, int nparts, float *view,
-                          unsigned long long *nzmask) {
+                          unsigned long long *nzmask, const SigFullRows &fr) {
     pair_tomogram_launch(s, P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff, nranges, fpix, fcnt, psum_slot,
-                         t0, ntiles, num_pix, partial, nparts, view, nzmask);
+                         t0, ntiles, num_pix, partial, nparts, view, nzmask, fr);
 }
 
 template <typename TP>
@@ -828,6 +892,16 @@ void launch_sig_pairs(hipStream_t s, const double *P, long num_pairs, const long
                       const int *tile_frame, int num_tiles, uint8_t *flag, int *sig_pair, int *sig_frame,
                       int *sig_cnt) {
     sig_pairs_launch(s, P, num_pairs, tile_off, tile_pair, tile_frame, num_tiles, flag, sig_pair, sig_frame, sig_cnt);
+}
+
+void launch_sig_full_rows(hipStream_t s, const double *RF, int nF, int num_tiles, const int *fidx, uint8_t *flag,
+                          int *sigF_row, int *sigF_frame, int *sigF_cnt, long *total) {
+    if (num_tiles <= 0 || nF <= 0) return;
+    const long rows = (long)num_tiles * nF;
+    pair_sig_kernel<double><<<stride_grid((rows + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, s>>>(RF, rows,
+                                                                                                         flag);
+    full_sig_compact_kernel<<<num_tiles, kBlock, 0, s>>>(flag, nF, fidx, sigF_row, sigF_frame, sigF_cnt);
+    sum_counts_kernel<<<1, kBlock, 0, s>>>(sigF_cnt, num_tiles, total);
 }
 
 void launch_sig_pairs(hipStream_t s, const float *P, long num_pairs, const long *tile_off, const int *tile_pair,
