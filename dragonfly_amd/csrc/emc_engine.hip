@@ -104,6 +104,51 @@ struct DevBuf {
     }
 };
 
+// Page-locked host memory (grow only) that the per-iteration lists are built in, so that
+// their uploads run at the host link's speed without a staging copy.
+template <typename T>
+struct PinnedBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() { release(); }
+    int reserve(size_t count) {
+        count = std::max<size_t>(count, 1);
+        if (p && count <= cap) return 0;
+        release();
+        const size_t want = count + count / 4;  // room for lists that grow a little
+        if (hipHostMalloc((void **)&p, want * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return fail(EMC_E_NOMEM, "hipHostMalloc of %zu bytes failed", want * sizeof(T));
+        }
+        cap = want;
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Host threads for a loop over n items, at least per items each, at most 16.
+int host_threads(long n, long per) {
+    const long hw = std::max(1u, std::thread::hardware_concurrency());
+    return (int)std::max(1L, std::min(std::min(16L, hw), n / std::max(per, 1L)));
+}
+
+// fn(part, i0, i1) over nparts contiguous ranges of [0, n), part 0 on the calling thread.
+template <typename F>
+void parallel_ranges(long n, int nparts, F &&fn) {
+    std::vector<std::thread> pool;
+    for (int pi = 1; pi < nparts; ++pi)
+        pool.emplace_back([&fn, n, nparts, pi] { fn(pi, n * pi / nparts, n * (pi + 1) / nparts); });
+    fn(0, 0L, n / nparts);
+    for (auto &th : pool) th.join();
+}
+
 void make_rot(const double *q, double *rot) {
     // make_rot_quat, dragonfly/src/model.c:759-786
     double q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
@@ -356,9 +401,10 @@ struct emc_engine {
     DevBuf<uint8_t> blacklist_full;
     bool full_lists_valid = false;
     // host scratch of the per-iteration subset set-up, kept so that its pages stay mapped
-    std::vector<uint32_t> h_relbits;
+    PinnedBuf<uint32_t> h_relbits;
     std::vector<long> h_poff, h_toff, h_ftom_off2;
-    std::vector<int> h_pframe, h_ptile, h_tpair, h_ffull, h_fof;
+    std::vector<int> h_ffull, h_fof;
+    PinnedBuf<int> h_pframe, h_ptile, h_tpair, h_tframe;
     long num_pairs = 0;
     DevBuf<int> slot_rot, rot_slot, pair_frame, pair_tile, tile_pair, tile_frame, rbad_slot;
     DevBuf<long> pair_off, tile_off;
@@ -837,102 +883,101 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
     }
     const std::vector<int> &ct0 = e->h_ctile0, &ct1 = e->h_ctile1;
     const int T = e->num_tiles;
-    std::vector<long> &poff = e->h_poff;
-    std::vector<int> &pframe = e->h_pframe, &ptile = e->h_ptile, &ffull = e->h_ffull;
-    std::vector<int> fidx;
-    poff.assign(N + 1, 0);
-    pframe.clear();
-    ptile.clear();
-    ffull.assign(N, -1);
-    // the frames' distinct cells come from their bit rows (built by emc_set_refine); ranges of
-    // frames are listed by host threads, then joined in frame order
+    // (frame, tile) pairs from the frames' bit rows (built by emc_set_refine): host threads list
+    // ranges of frames (a frame's distinct tiles, ascending), the frame offsets follow, then
+    // the threads write the pairs and, from per-thread tile counts, the tile-major copy in
+    // frame order
     const int words = e->rel_words;
-    const std::vector<uint32_t> &bits = e->h_relbits;
-    const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const int nparts_h = std::max(1, std::min(nthreads, N / 4096));
+    const uint32_t *bits = e->h_relbits.p;
+    const int nth = host_threads(N, 2048);
     struct Part {
-        std::vector<int> frame, tile, full;
-        std::vector<long> count;  // pairs of each frame of the range
+        std::vector<int> tiles;   // the range's frames' tiles, frame after frame
+        std::vector<long> hist;  // pairs per tile
     };
-    std::vector<Part> parts(nparts_h);
-    auto list_range = [&](int pi) {
+    std::vector<Part> parts(nth);
+    std::vector<int> ntl(N);  // tiles per frame, -1 for a full frame
+    parallel_ranges(N, nth, [&](int pi, long d0, long d1) {
         Part &pt = parts[pi];
-        const int d0 = (int)((long)N * pi / nparts_h), d1 = (int)((long)N * (pi + 1) / nparts_h);
-        pt.count.assign(d1 - d0, 0);
-        std::vector<int> cells, tl;
-        for (int d = d0; d < d1; ++d) {
-            cells.clear();
-            const uint32_t *row = &bits[(size_t)d * words];
+        std::vector<int> tl;
+        for (long d = d0; d < d1; ++d) {
+            const uint32_t *row = bits + (size_t)d * words;
             int n = 0;
             for (int w = 0; w < words; ++w) n += __builtin_popcount(row[w]);
             if (n == num_coarse) {
-                pt.full.push_back(d);
+                ntl[d] = -1;
                 continue;
             }
-            for (int w = 0; w < words; ++w)
-                for (uint32_t x = row[w]; x; x &= x - 1) cells.push_back(w * 32 + __builtin_ctz(x));
             // the tiles holding the frame's cells, ascending and distinct (with packed cells
             // neighbouring cells share tiles)
             tl.clear();
-            for (int c : cells)
-                for (int t = ct0[c]; t <= ct1[c]; ++t) tl.push_back(t);
+            for (int w = 0; w < words; ++w)
+                for (uint32_t x = row[w]; x; x &= x - 1) {
+                    const int c = w * 32 + __builtin_ctz(x);
+                    for (int t = ct0[c]; t <= ct1[c]; ++t) tl.push_back(t);
+                }
             std::sort(tl.begin(), tl.end());
             tl.erase(std::unique(tl.begin(), tl.end()), tl.end());
-            for (int t : tl) {
-                pt.frame.push_back(d);
-                pt.tile.push_back(t);
-            }
-            pt.count[d - d0] = (long)tl.size();
+            ntl[d] = (int)tl.size();
+            pt.tiles.insert(pt.tiles.end(), tl.begin(), tl.end());
         }
-    };
-    {
-        std::vector<std::thread> pool;
-        for (int pi = 1; pi < nparts_h; ++pi) pool.emplace_back(list_range, pi);
-        list_range(0);
-        for (auto &th : pool) th.join();
-    }
-    size_t total_pairs = 0;
-    for (const Part &pt : parts) total_pairs += pt.frame.size();
-    pframe.reserve(total_pairs);
-    ptile.reserve(total_pairs);
-    for (int pi = 0; pi < nparts_h; ++pi) {
-        const Part &pt = parts[pi];
-        const int d0 = (int)((long)N * pi / nparts_h);
-        for (size_t i = 0; i < pt.count.size(); ++i) poff[d0 + i + 1] = poff[d0 + i] + pt.count[i];
-        pframe.insert(pframe.end(), pt.frame.begin(), pt.frame.end());
-        ptile.insert(ptile.end(), pt.tile.begin(), pt.tile.end());
-        for (int d : pt.full) {
+    });
+    std::vector<long> &poff = e->h_poff;
+    std::vector<int> &ffull = e->h_ffull;
+    std::vector<int> fidx;
+    poff.assign(N + 1, 0);
+    ffull.assign(N, -1);
+    for (int d = 0; d < N; ++d) {
+        poff[d + 1] = poff[d] + std::max(ntl[d], 0);
+        if (ntl[d] < 0) {
             ffull[d] = (int)fidx.size();
             fidx.push_back(d);
         }
     }
-    const long K = (long)pframe.size();
+    const long K = poff[N];
     if (K >= INT32_MAX) return fail(EMC_E_ARG, "emc_set_refine: too many (frame, tile) pairs");
+    if ((rc = e->h_pframe.reserve(K)) || (rc = e->h_ptile.reserve(K)) || (rc = e->h_tpair.reserve(K)) ||
+        (rc = e->h_tframe.reserve(K)))
+        return rc;
+    int *pframe = e->h_pframe.p, *ptile = e->h_ptile.p, *tpair = e->h_tpair.p, *tframe = e->h_tframe.p;
+    pframe[0] = ptile[0] = tpair[0] = tframe[0] = 0;  // one element at least is uploaded
+    parallel_ranges(N, nth, [&](int pi, long d0, long d1) {
+        Part &pt = parts[pi];
+        pt.hist.assign(T, 0);
+        size_t i = 0;
+        for (long d = d0; d < d1; ++d)
+            for (long k = poff[d]; k < poff[d + 1]; ++k) {
+                const int t = pt.tiles[i++];
+                pframe[k] = (int)d;
+                ptile[k] = t;
+                ++pt.hist[t];
+            }
+    });
     std::vector<long> &toff = e->h_toff;
     toff.assign(T + 1, 0);
-    for (long k = 0; k < K; ++k) ++toff[ptile[k] + 1];
-    for (int t = 0; t < T; ++t) toff[t + 1] += toff[t];
-    std::vector<int> &tpair = e->h_tpair;
-    tpair.resize(K > 0 ? K : 1);
-    std::vector<long> tfill(toff.begin(), toff.end() - 1);
-    for (long k = 0; k < K; ++k) tpair[tfill[ptile[k]]++] = (int)k;
-    if (pframe.empty()) {
-        pframe.push_back(0);
-        ptile.push_back(0);
+    for (int t = 0; t < T; ++t) {
+        long at = toff[t];
+        for (Part &pt : parts) {
+            const long n = pt.hist[t];
+            pt.hist[t] = at;  // the part's first position in tile t
+            at += n;
+        }
+        toff[t + 1] = at;
     }
+    parallel_ranges(N, nth, [&](int pi, long d0, long d1) {
+        std::vector<long> &at = parts[pi].hist;
+        for (long k = poff[d0]; k < poff[d1]; ++k) {
+            const long j = at[ptile[k]]++;
+            tpair[j] = (int)k;
+            tframe[j] = pframe[k];  // the frame beside the pair: the pair tomogram reads it with it
+        }
+    });
+    const size_t Ku = (size_t)std::max(K, 1L);
     if ((rc = e->pair_off.upload(poff.data(), poff.size(), s))) return rc;
-    if ((rc = e->pair_frame.upload(pframe.data(), pframe.size(), s))) return rc;
-    if ((rc = e->pair_tile.upload(ptile.data(), ptile.size(), s))) return rc;
+    if ((rc = e->pair_frame.upload(pframe, Ku, s))) return rc;
+    if ((rc = e->pair_tile.upload(ptile, Ku, s))) return rc;
     if ((rc = e->tile_off.upload(toff.data(), toff.size(), s))) return rc;
-    if ((rc = e->tile_pair.upload(tpair.data(), tpair.size(), s))) return rc;
-    {
-        // frame of every entry of the tile-major list, so the pair tomogram reads it with the
-        // pair index instead of after it
-        std::vector<int> tframe(tpair.size());
-        for (size_t j = 0; j < tpair.size(); ++j) tframe[j] = pframe[tpair[j]];
-        if ((rc = e->tile_frame.upload(tframe.data(), tframe.size(), s))) return rc;
-        HIP_TRY(hipStreamSynchronize(s));
-    }
+    if ((rc = e->tile_pair.upload(tpair, Ku, s))) return rc;
+    if ((rc = e->tile_frame.upload(tframe, Ku, s))) return rc;
     if ((rc = e->ffull.upload(ffull.data(), ffull.size(), s))) return rc;
     if (!fidx.empty() && (rc = e->fidx.upload(fidx.data(), fidx.size(), s))) return rc;
     HIP_TRY(hipStreamSynchronize(s));
@@ -2067,26 +2112,38 @@ int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const
         qm[r] = quat_mapping[r];
     }
     if (rel_off[0] != 0) return fail(EMC_E_ARG, "emc_set_refine: rel_off[0] must be 0");
-    std::vector<uint32_t> &bits = e->h_relbits;
-    bits.assign((size_t)N * words, 0u);
-    for (int d = 0; d < N; ++d) {
+    for (int d = 0; d < N; ++d)
         if (rel_off[d + 1] < rel_off[d]) return fail(EMC_E_ARG, "emc_set_refine: rel_off not ascending");
-        if (rel_off[d + 1] - rel_off[d] == 1 && rel_quat[rel_off[d]] == -1) {
-            // the single entry -1: every coarse cell (the full search)
-            uint32_t *row = &bits[(size_t)d * words];
-            for (int w = 0; w < words; ++w) row[w] = ~0u;
-            if (num_coarse & 31) row[words - 1] = (1u << (num_coarse & 31)) - 1u;
-            continue;
-        }
-        for (long k = rel_off[d]; k < rel_off[d + 1]; ++k) {
-            const int c = rel_quat[k];
-            if (c < 0 || c >= num_coarse) return fail(EMC_E_ARG, "emc_set_refine: rel_quat %d out of range", c);
-            bits[(size_t)d * words + (c >> 5)] |= 1u << (c & 31);
-        }
-    }
     int rc;
+    // one bit row per frame, built by host threads over ranges of frames
+    if ((rc = e->h_relbits.reserve((size_t)N * words))) return rc;
+    uint32_t *bits = e->h_relbits.p;
+    const int nth = host_threads(N, 2048);
+    std::vector<long> bad(nth, -1);  // first entry out of range in each range of frames
+    parallel_ranges(N, nth, [&](int pi, long d0, long d1) {
+        std::memset(bits + (size_t)d0 * words, 0, (size_t)(d1 - d0) * words * sizeof(uint32_t));
+        for (long d = d0; d < d1; ++d) {
+            uint32_t *row = bits + (size_t)d * words;
+            if (rel_off[d + 1] - rel_off[d] == 1 && rel_quat[rel_off[d]] == -1) {
+                // the single entry -1: every coarse cell (the full search)
+                for (int w = 0; w < words; ++w) row[w] = ~0u;
+                if (num_coarse & 31) row[words - 1] = (1u << (num_coarse & 31)) - 1u;
+                continue;
+            }
+            for (long k = rel_off[d]; k < rel_off[d + 1]; ++k) {
+                const int c = rel_quat[k];
+                if (c < 0 || c >= num_coarse) {
+                    if (bad[pi] < 0) bad[pi] = k;
+                    continue;
+                }
+                row[c >> 5] |= 1u << (c & 31);
+            }
+        }
+    });
+    for (long k : bad)
+        if (k >= 0) return fail(EMC_E_ARG, "emc_set_refine: rel_quat %d out of range", rel_quat[k]);
     if ((rc = e->qmap.upload(qm.data(), qm.size(), e->stream))) return rc;
-    if ((rc = e->relbits.upload(bits.data(), bits.size(), e->stream))) return rc;
+    if ((rc = e->relbits.upload(bits, (size_t)N * words, e->stream))) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->rel_words = words;
     e->refine = true;
