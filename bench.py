@@ -438,6 +438,8 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
     recon, itr = recon_for(cfg['num_div'], precision=args.precision)
     if args.fused:
         recon.engine.set_option('fused', 1)
+    if args.sig_full_rows is not None:
+        recon.engine.set_option('sig_full_rows', args.sig_full_rows)
     start_k1(recon, itr)
 
     def sources(source):
@@ -698,6 +700,9 @@ def main():
                     help='learned-proposal workload: cells of the coarse search joined to every proposal')
     ap.add_argument('--methods', default='previous_posterior,network_only,coarse_only',
                     help='learned-proposal workload: proposal sources timed beside the headline')
+    ap.add_argument('--sig-full-rows', type=int, default=None,
+                    help='learned-proposal workload: mean significant full-frame rows per tile up to which the '
+                         'pair tomogram reads them directly (engine option sig_full_rows)')
     ap.add_argument('--fused', action='store_true',
                     help='learned-proposal workload: expand and pair likelihood in one persistent launch')
     ap.add_argument('--converge-iters', type=int, default=3,

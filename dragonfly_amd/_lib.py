@@ -364,7 +364,8 @@ class Engine:
     def set_option(self, name, value):
         """Kernel-variant option (emc_set_option): lik_tiles, lik_unroll, lik_xcd, tomo_tiles,
         tomo_tile_mode, tomo_parts, tomo_width32, tomo_unroll, tomo_seq, max_chunk_cols,
-        merge_mode, subset, slot_pack, lik_split, sparse_rows, full_part_frames."""
+        merge_mode, subset, slot_pack, lik_split, sparse_rows, full_part_frames, estep_only,
+        mstep_only, graph, fused, sig_full_rows."""
         _check(lib().emc_set_option(self._h, name.encode(), int(value)))
 
     def get_option(self, name):

@@ -368,6 +368,7 @@ struct emc_engine {
     int slot_pack = 1, slot_pack_built = -1;
     int sparse_rows = 0;  // frames per batch of the subset sparse posterior (0: 256 MB of rows)
     int full_part_frames = kFramesPerTomoPart;  // frames per range of the full-frame tomogram
+    int sig_full_rows = kSigFullRowsPerTile;  // full-frame rows per tile the pair tomogram takes
     int estep_only = 0;   // stop after the posterior: no tomogram, merge or scale update
     int mstep_only = 0;   // subset path: resume after an estep_only iteration with the M-step
     int graph_mode = 0;   // subset path: run the iteration as a captured HIP graph
@@ -1556,7 +1557,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                 return rc;
         }
         // the full frames' significant rows go to the pair tomogram directly when they average
-        // at most kSigFullRowsPerTile a tile (its blocks walk every row of their tile); else
+        // at most sig_full_rows a tile (its blocks walk every row of their tile); else
         // their tomogram is the plain path's, into partial views (also inside a graph, which
         // cannot read the count back)
         bool sig_full = false;
@@ -1564,7 +1565,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
             long tot = 0;
             HIP_TRY(hipMemcpyAsync(&tot, e->sigF_total.p, sizeof(long), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
-            sig_full = tot <= (long)kSigFullRowsPerTile * T;
+            sig_full = tot <= (long)e->sig_full_rows * T;
         }
         const emc::SigFullRows frows =
             sig_full ? emc::SigFullRows{e->RF.p, nF, e->sigF_row.p, e->sigF_frame.p, e->sigF_cnt.p}
@@ -1749,6 +1750,7 @@ const EngineOption kOptions[] = {
     {"mstep_only", &emc_engine::mstep_only, 0, 1, false},
     {"graph", &emc_engine::graph_mode, 0, 1, false},
     {"fused", &emc_engine::fused, 0, 1, false},
+    {"sig_full_rows", &emc_engine::sig_full_rows, 0, 1 << 20, false},
 };
 
 int set_option(emc_engine *e, const EngineOption &o, int value) {

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
     const uint8_t *__restrict__ mask, int num_pix, int r0, int C, int pix_per_group,
     T *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad,
     const int *__restrict__ slot_rot, int modes, int raw) {
-    __shared__ ExpandShared sh;
+    __shared__ ExpandShared<MT> sh;
     expand_block<T, kW, MT>(sh, blockIdx.x * kRTile, blockIdx.y, rots, model, size, qvals, corr, mask, num_pix, r0, C,
                             pix_per_group, logw, upart, ubad, slot_rot, modes, raw);
 }

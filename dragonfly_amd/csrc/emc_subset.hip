@@ -703,7 +703,7 @@ __global__ __launch_bounds__(kBlock) void fused_expand_likelihood_kernel(
     const long *__restrict__ tile_off, const int *__restrict__ tile_pair, int t0, int ntiles,
     unsigned *__restrict__ counter, double *__restrict__ G) {
     constexpr int kSub = kW / kRTile;
-    __shared__ ExpandShared sh;
+    __shared__ ExpandShared<MT> sh;
     __shared__ int s_tile;
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);

@@ -169,7 +169,11 @@ int emc_set_precision(emc_engine *e, int mode);
  * emc_get_sparse_prob after a subset iteration, 0: 256 MB of rows), full_part_frames
  * (frames per range of the subset path's full-frame tomogram, default 12500), estep_only (1:
  * emc_maximize stops after the posterior and its statistics -- no tomogram, merge or scale
- * update; plain and mixed paths). Start values come from EMC_<NAME>.      */
+ * update), mstep_only (1: on the subset path, the M-step of the last estep_only iteration
+ * when nothing changed since), graph (1: the subset iteration captured and replayed as a HIP
+ * graph), fused (1: expand and pair likelihood in one persistent launch), sig_full_rows (mean
+ * significant full-frame rows per tile up to which the subset path's pair tomogram reads them
+ * directly instead of partial views; default 1024). Start values come from EMC_<NAME>.      */
 int emc_set_option(emc_engine *e, const char *name, int value);
 int emc_get_option(emc_engine *e, const char *name, int *value);
 /* per-frame state of struct iterate: scale, beta, blacklist, fcounts, sum_fact,
