@@ -221,6 +221,53 @@ def roofline(prob, itr, stage_tot, steps, nchunks, pairs, workload, value_bytes=
     return out
 
 
+def roofline_learned(res, slots, num_pix, size, num_photons, num_frames, workload):
+    """HBM roofline of the judged proposal iteration's longest kernel among the expand (first
+    pass), the full frames' tomogram, the pair tomogram and the merge, each one launch per
+    iteration (one orientation chunk), timed with HIP events on the engine's stream, on
+    compulsory bytes (every operand moved once; FP32 gathers):
+    expand: the FP32 model copy read (halo bricks, 512 B per 4^3 cells) and log W written
+      (4 B per orientation x pixel);
+    tomogram (full frames): their FP32 posterior read (4 B per frame x orientation), their
+      photon lists (int32 frame + f64 count) and the partial views written (4 B per
+      orientation x pixel);
+    reduce (pair tomogram): the partial views read, the photon lists of every frame, the
+      nonzero masks (8 B per orientation x 64-pixel patch) and the nonzero view entries
+      written (4 B each; the engine counts them);
+    merge: the nonzero masks and nonzero view entries read and the merged volume read and
+      written (16 B per voxel)."""
+    st, st1 = res['stage_ms'], res['pass1_stage_ms']
+    nnz = float(res.get('view_nonzero') or 0.)
+    npatch = (num_pix + 63) // 64
+    nb = (size - 2) // 4 + 1
+    full = res['fallback_rate'] * num_frames
+    photons_full = num_photons * res['fallback_rate']
+    dense = 4. * slots * num_pix
+    masks = 8. * slots * npatch
+    comp = {'expand': 512. * nb ** 3 + dense,
+            'tomogram': 4. * full * slots + 12. * photons_full + dense,
+            'reduce': (dense if full > 0 else 0.) + 12. * num_photons + masks + 4. * nnz,
+            'merge': masks + 4. * nnz + 16. * float(size) ** 3}
+    ms = {'expand': st1.get('expand', 0.), 'tomogram': st.get('tomogram', 0.), 'reduce': st.get('reduce', 0.),
+          'merge': st.get('merge', 0.)}
+    kernels = {'expand': 'expand_kernel', 'tomogram': 'tomogram_part_w128_kernel', 'reduce': 'pair_tomogram_kernel',
+               'merge': 'merge_brick_model_kernel'}
+    stage = max(ms, key=ms.get)
+    per = {k: dict(kernel=kernels[k], launch_ms=round(ms[k], 3), compulsory_bytes=round(comp[k]),
+                   frac=round(comp[k] / (ms[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ms[k] > 0 else None)
+           for k in ms}
+    achieved = comp[stage] / (ms[stage] * 1e-3) / 1e9
+    traffic, note = load_traffic(workload, kernels[stage]) if workload else (None, 'not the default workload size')
+    out = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+               frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None if traffic is None else round(traffic),
+               kernel=kernels[stage], stage=stage, launch_ms=round(ms[stage], 3), compulsory_bytes=round(comp[stage]),
+               view_nonzero=nnz, traffic_source=note, stages=per)
+    if traffic is not None:
+        out['traffic_over_compulsory'] = round(traffic / comp[stage], 2)
+        out['hbm_frac_measured'] = round(traffic / (ms[stage] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    return out
+
+
 def roofline_fixed_stages(stage_ms, num_chunks, slots, num_pix, size, value_bytes):
     """HBM roofline of the longer of the two per-orientation stages of an iteration, expand
     and merge, per launch (one launch per orientation chunk), on compulsory bytes: expand
@@ -493,6 +540,7 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
                    model_rel_err=float('%.3g' % (np.abs(ml - model_plain).max() / np.abs(model_plain).max())),
                    model_rms_rel_err=float('%.3g' % np.sqrt(np.mean((ml - model_plain) ** 2) / np.mean(model_plain ** 2))),
                    rmax_agree=round(float(np.mean(recon.rmax == rmax_plain)), 5),
+                   view_nonzero=recon.last_stats.get('view_nonzero', 0.),
                    stage_ms={k: round(v, 1) for k, v in recon.last_stats['stage_ms'].items()},
                    pass1_stage_ms={k: round(v, 1) for k, v in info['estep_stage_ms'].items()},
                    phase_ms=info['phase_ms'])
@@ -501,7 +549,10 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
     judge_ms = judge.stats['stage_ms'] if judge.stats else {}
     judge.close()
     recon.free()
+    f = prob['frames']
+    num_photons = int((mask[f['place_ones']] < 2).sum() + (mask[f['place_multi']] < 2).sum())
     return dict(D=D, R=R, num_coarse=num_coarse, plain_s=plain_s, plain_times=plain_times, st_plain=st_plain,
+                num_photons=num_photons,
                 train_s=train_s, epochs=args.epochs, loss=hist[-1], results=results,
                 judge_stage_ms={k: round(v, 1) for k, v in judge_ms.items()}, itr=itr, k=kk, tau=tau)
 
@@ -521,6 +572,7 @@ def bench_learned(args):
     if args.top_k:
         cfg['top_k'] = args.top_k
     late = float(cfg.get('beta_factor', 1.)) if args.beta_factor is None else args.beta_factor
+    default_size = args.num_data is None and args.num_div is None
     head = learned_case(prob, cfg, late, args, args.steps, args.warmup)
     other = None
     if not args.single_beta and late != 1.:
@@ -571,8 +623,11 @@ def bench_learned(args):
         'speedup_vs_fp64': net['speedup_vs_fp64'], 'fallback_rate': net['fallback_rate'],
         'late_beta': summary(head),
         'config_beta': summary(dict(other, beta_factor=1.)) if other else None,
-        'roofline': roofline_fixed_stages(net['stage_ms'], net['num_chunks'], R, len(prob['det']['mask']),
-                                          cfg['size'], 4 if args.precision == 3 else 8),
+        'roofline': (roofline_learned(net, R, len(prob['det']['mask']), cfg['size'], head['num_photons'], D,
+                                      'cfg5' if default_size else None)
+                     if args.precision == 3 and net['num_chunks'] == 1 else
+                     roofline_fixed_stages(net['stage_ms'], net['num_chunks'], R, len(prob['det']['mask']),
+                                           cfg['size'], 4 if args.precision == 3 else 8)),
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)

@@ -36,7 +36,8 @@ class IterOpts(C.Structure):
 
 class IterStats(C.Structure):
     _fields_ = [('avg_likelihood', C.c_double), ('mutual_info', C.c_double),
-                ('rescale', C.c_double), ('stage_ms', C.c_double * len(STAGES)), ('num_chunks', C.c_int)]
+                ('rescale', C.c_double), ('stage_ms', C.c_double * len(STAGES)), ('num_chunks', C.c_int),
+                ('view_nonzero', C.c_double)]
 
 
 class EMCError(RuntimeError):
@@ -274,7 +275,7 @@ class Engine:
         _check(lib().emc_maximize(self._h, C.byref(opts), C.byref(st)))
         return dict(avg_likelihood=st.avg_likelihood, mutual_info=st.mutual_info,
                     rescale=st.rescale, stage_ms=dict(zip(STAGES, list(st.stage_ms))),
-                    num_chunks=st.num_chunks)
+                    num_chunks=st.num_chunks, view_nonzero=st.view_nonzero)
 
     def update_model(self, alpha=0.):
         rms = C.c_double()

@@ -421,6 +421,8 @@ struct emc_engine {
     DevBuf<uint8_t> full_flag;
     DevBuf<int> sigF_row, sigF_frame, sigF_cnt;
     DevBuf<long> sigF_total;
+    DevBuf<unsigned long long> nz_count;  // nonzero view entries of the last subset iteration
+    bool nz_counted = false;
     std::vector<BrickChunk> slot_bricks;
     bool slot_bricks_valid = false;
     // frame photon lists of mask < 2 pixels sorted by engine pixel, with per-range offsets
@@ -1335,6 +1337,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     if ((rc = e->upart.alloc((size_t)ngroups * C))) return rc;
     if ((rc = e->ubad.alloc((size_t)ngroups * C))) return rc;
     if ((rc = e->nzmask.alloc((size_t)C * ((P + 63) / 64)))) return rc;
+    if ((rc = e->nz_count.alloc(1))) return rc;
     if ((rc = e->u_slot.alloc(S))) return rc;
     if ((rc = e->rbad_slot.alloc(S))) return rc;
     if ((rc = e->psum_slot.alloc(S))) return rc;
@@ -1571,6 +1574,8 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
             sig_full ? emc::SigFullRows{e->RF.p, nF, e->sigF_row.p, e->sigF_frame.p, e->sigF_cnt.p}
                      : emc::SigFullRows{};
         const bool parts = nF > 0 && !sig_full;
+        e->nz_counted = false;
+        emc::launch_zero_u64(s, e->nz_count.p);
         for (size_t k = 0; k < e->slot_bricks.size(); ++k) {
             const auto &bc = e->slot_bricks[k];
             const int t0 = bc.r0 / emc::kRTile, t1 = (bc.r0 + bc.Cc) / emc::kRTile;
@@ -1608,6 +1613,10 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                      return 0;
                  })))
                 return rc;
+            if (nz) {
+                emc::launch_popcount(s, nz, (size_t)bc.Cc * ((P + 63) / 64), e->nz_count.p);
+                e->nz_counted = true;
+            }
             if ((rc = timed(8, [&] {
                      const float *v32 = g32 ? e->logw32.p : nullptr;
                      if (!device_weights && !g32)
@@ -1642,6 +1651,13 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
             stats->mutual_info = host_scalars[2];
             for (int i = 0; i < EMC_NUM_STAGES; ++i) stats->stage_ms[i] = ms[i];
             stats->num_chunks = (int)e->slot_bricks.size();
+            stats->view_nonzero = 0.;
+            if (e->nz_counted) {
+                unsigned long long nnz = 0;
+                HIP_TRY(hipMemcpyAsync(&nnz, e->nz_count.p, sizeof nnz, hipMemcpyDeviceToHost, s));
+                HIP_TRY(hipStreamSynchronize(s));
+                stats->view_nonzero = (double)nnz;
+            }
         }
         return 0;
     };
@@ -1713,6 +1729,7 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
             stats->mutual_info = host_scalars[2];
             for (int i = 0; i < EMC_NUM_STAGES; ++i) stats->stage_ms[i] = ms[i];
             stats->num_chunks = (int)((S + C - 1) / C);
+            stats->view_nonzero = 0.;
         }
         return 0;
     }
@@ -2632,6 +2649,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
             const float ms[EMC_NUM_STAGES] = {ms_expand, ms_lik, ms_rescale, ms_max, ms_norm, ms_post, 0.f, 0.f, 0.f};
             for (int i = 0; i < EMC_NUM_STAGES; ++i) stats->stage_ms[i] = ms[i];
             stats->num_chunks = (e->Rp + C - 1) / C;
+            stats->view_nonzero = 0.;
         }
         return 0;
     }
@@ -2772,6 +2790,7 @@ int emc_maximize(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *stats
                                           ms_post, ms_tomo, ms_reduce, ms_merge};
         for (int i = 0; i < EMC_NUM_STAGES; ++i) stats->stage_ms[i] = ms[i];
         stats->num_chunks = (e->Rp + C - 1) / C;
+        stats->view_nonzero = 0.;
     }
     return 0;
 }

@@ -235,6 +235,9 @@ void launch_pair_posterior(hipStream_t s, double *R, const long *pair_off, const
 void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                       const int *pair_frame, int num_tiles, const double *scale, int need_scaling,
                       const double *psum_full, double *psum_slot);
+// *x = 0; *total += set bits of m[0 .. n)
+void launch_zero_u64(hipStream_t s, unsigned long long *x);
+void launch_popcount(hipStream_t s, const unsigned long long *m, size_t n, unsigned long long *total);
 // Significant full-frame rows per tile (launch_sig_full_rows), read by the pair tomogram from
 // RF in place of partial sums; RF null: none.
 struct SigFullRows {

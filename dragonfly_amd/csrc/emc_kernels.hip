@@ -52,8 +52,22 @@ __global__ __launch_bounds__(kBlock) void expand_kernel(
     T *__restrict__ logw, double *__restrict__ upart, int *__restrict__ ubad,
     const int *__restrict__ slot_rot, int modes, int raw) {
     __shared__ ExpandShared<MT> sh;
-    expand_block<T, kW, MT>(sh, blockIdx.x * kRTile, blockIdx.y, rots, model, size, qvals, corr, mask, num_pix, r0, C,
-                            pix_per_group, logw, upart, ubad, slot_rot, modes, raw);
+    // XCD-aware order: workgroups go round-robin to the kNumXcd XCDs, so the b-th block of XCD
+    // x takes tile x * per + b % per of its own contiguous range of tiles (neighbouring
+    // orientations: the slot order follows a chain of neighbouring cells) and pixel group
+    // b / per; the blocks resident on an XCD then read one region of the model through its L2
+    const int ntiles = C / kRTile, per = (ntiles + kNumXcd - 1) / kNumXcd;
+    const int xcd = blockIdx.x % kNumXcd, b = blockIdx.x / kNumXcd;
+    const int tile = xcd * per + b % per, g = b / per;
+    if (tile >= ntiles || (long)g * pix_per_group >= num_pix) return;
+    expand_block<T, kW, MT>(sh, tile * kRTile, g, rots, model, size, qvals, corr, mask, num_pix, r0, C, pix_per_group,
+                            logw, upart, ubad, slot_rot, modes, raw);
+}
+
+// grid of expand_kernel: kNumXcd ranges of per tiles, each over every pixel group
+static unsigned expand_grid(int C, int num_pix, int pix_per_group) {
+    const long per = (C / kRTile + kNumXcd - 1) / kNumXcd;
+    return (unsigned)(kNumXcd * per * ((num_pix + pix_per_group - 1) / pix_per_group));
 }
 
 // u[r0 + c] = sum over pixel groups, fixed order.
@@ -1025,8 +1039,7 @@ void launch_expand(hipStream_t s, const double *rots, const double *model, long 
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, double *logw, double *upart, int *ubad,
                    const int *slot_rot, int modes, int raw) {
-    dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
-    expand_kernel<double, kRTile><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0, C,
+    expand_kernel<double, kRTile><<<expand_grid(C, num_pix, pix_per_group), kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix, r0, C,
                                                           pix_per_group, logw, upart, ubad, slot_rot, modes, raw);
 }
 
@@ -1034,8 +1047,7 @@ void launch_expand(hipStream_t s, const double *rots, const float *model, long s
                    const double *qvals, const double *corr, const uint8_t *mask, int num_pix,
                    int r0, int C, int pix_per_group, float *logw, double *upart, int *ubad,
                    const int *slot_rot, int modes) {
-    dim3 grid(C / kRTile, cdiv(num_pix, pix_per_group));
-    expand_kernel<float, 2 * kRTile, float><<<grid, kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix,
+    expand_kernel<float, 2 * kRTile, float><<<expand_grid(C, num_pix, pix_per_group), kBlock, 0, s>>>(rots, model, size, qvals, corr, mask, num_pix,
                                                                     r0, C, pix_per_group, logw, upart, ubad,
                                                                     slot_rot, modes, 0);
 }

@@ -395,6 +395,31 @@ __global__ __launch_bounds__(kBlock) void full_sig_compact_kernel(const uint8_t 
     if (threadIdx.x == 0) sigF_cnt[blockIdx.x] = base;
 }
 
+// *total += set bits of m[0 .. n) (the caller zeroes *total)
+__global__ __launch_bounds__(kBlock) void popcount_kernel(const unsigned long long *__restrict__ m, size_t n,
+                                                          unsigned long long *__restrict__ total) {
+    __shared__ unsigned long long s[kBlock];
+    unsigned long long a = 0;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        a += (unsigned long long)__popcll(m[i]);
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && s[0]) atomicAdd(total, s[0]);
+}
+
+__global__ void zero_u64_kernel(unsigned long long *x) { *x = 0ull; }
+
+void launch_zero_u64(hipStream_t s, unsigned long long *x) { zero_u64_kernel<<<1, 1, 0, s>>>(x); }
+
+void launch_popcount(hipStream_t s, const unsigned long long *m, size_t n, unsigned long long *total) {
+    if (n == 0) return;
+    popcount_kernel<<<(unsigned)std::min<size_t>((n + kBlock - 1) / kBlock, 2048), kBlock, 0, s>>>(m, n, total);
+}
+
 // *total = sum of cnt[0 .. n) (one block)
 __global__ __launch_bounds__(kBlock) void sum_counts_kernel(const int *__restrict__ cnt, int n, long *__restrict__ total) {
     __shared__ long s[kBlock];

@@ -58,6 +58,8 @@ typedef struct emc_iter_stats {
                               expand, likelihood, rescale, max, norm, posterior, tomogram
                               (gather), tomogram reduce, merge */
     int num_chunks;        /* orientation chunks the iteration was split into (launches per stage) */
+    double view_nonzero;   /* nonzero view entries (orientation x pixel) the merge read, on the
+                              refinement-subset path when the views are stored sparsely; else 0 */
 } emc_iter_stats;
 
 /* Per-frame / per-orientation / volume fields readable with emc_get_field(). */
