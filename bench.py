@@ -487,6 +487,8 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
         recon.engine.set_option('fused', 1)
     if args.sig_full_rows is not None:
         recon.engine.set_option('sig_full_rows', args.sig_full_rows)
+    if args.unit_items is not None:
+        recon.engine.set_option('unit_items', args.unit_items)
     start_k1(recon, itr)
 
     def sources(source):
@@ -758,6 +760,9 @@ def main():
     ap.add_argument('--sig-full-rows', type=int, default=None,
                     help='learned-proposal workload: mean significant full-frame rows per tile up to which the '
                          'pair tomogram reads them directly (engine option sig_full_rows)')
+    ap.add_argument('--unit-items', type=int, default=None,
+                    help='learned-proposal workload: brick-merge list entries per work unit (engine option '
+                         'unit_items)')
     ap.add_argument('--fused', action='store_true',
                     help='learned-proposal workload: expand and pair likelihood in one persistent launch')
     ap.add_argument('--converge-iters', type=int, default=3,

@@ -369,6 +369,7 @@ struct emc_engine {
     int sparse_rows = 0;  // frames per batch of the subset sparse posterior (0: 256 MB of rows)
     int full_part_frames = kFramesPerTomoPart;  // frames per range of the full-frame tomogram
     int sig_full_rows = kSigFullRowsPerTile;  // full-frame rows per tile the pair tomogram takes
+    int unit_items = emc::kUnitItems;         // brick merge: list entries per work unit (block)
     int estep_only = 0;   // stop after the posterior: no tomogram, merge or scale update
     int mstep_only = 0;   // subset path: resume after an estep_only iteration with the M-step
     int graph_mode = 0;   // subset path: run the iteration as a captured HIP graph
@@ -697,9 +698,9 @@ int build_bricks(emc_engine *e, std::vector<emc_engine::BrickChunk> &out, const 
         std::vector<int4> units;
         for (size_t b = 0; b < nb; ++b) {
             off[b] = tot;
-            for (int k = 0; k < counts[b]; k += emc::kUnitItems)
+            for (int k = 0; k < counts[b]; k += e->unit_items)
                 units.push_back(make_int4((int)b, (int)(tot + k),
-                                          (int)(tot + std::min(counts[b], k + emc::kUnitItems)), 0));
+                                          (int)(tot + std::min(counts[b], k + e->unit_items)), 0));
             tot += counts[b];
         }
         if (tot >= INT32_MAX) return fail(EMC_E_ARG, "brick merge: too many work items");
@@ -1768,6 +1769,7 @@ const EngineOption kOptions[] = {
     {"graph", &emc_engine::graph_mode, 0, 1, false},
     {"fused", &emc_engine::fused, 0, 1, false},
     {"sig_full_rows", &emc_engine::sig_full_rows, 0, 1 << 20, false},
+    {"unit_items", &emc_engine::unit_items, 64, 1 << 20, false},
 };
 
 int set_option(emc_engine *e, const EngineOption &o, int value) {
@@ -1777,6 +1779,8 @@ int set_option(emc_engine *e, const EngineOption &o, int value) {
         e->lists_dirty = true;
         e->full_lists_valid = false;
     }
+    if (e->*(o.field) != value && o.field == &emc_engine::unit_items)
+        e->bricks_valid = e->slot_bricks_valid = false;  // the merge's work units are cut again
     e->*(o.field) = value;
     return 0;
 }

@@ -186,24 +186,28 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
             id = list[k];
             m = lmask[k];
         }
+        // item id = c * npatch + p (chunk orientation c, patch p), split once per lane
+        const int ic = id >= 0 ? id / npatch : 0, ip = id >= 0 ? id - ic * npatch : 0;
         if (id >= 0) {
             if (nzmask)
-                m &= nzmask[id];  // item id = c * npatch + p
-            else if (!all_on && !(psum_r[r0 + id / npatch] > 0.))
+                m &= nzmask[id];
+            else if (!all_on && !(psum_r[r0 + ic] > 0.))
                 m = 0ull;
         }
         unsigned long long live = __ballot(m != 0ull);
         while (live) {
-            int ids[kItemsAhead];
+            int cs[kItemsAhead], ps[kItemsAhead];
             unsigned long long ms[kItemsAhead];
 #pragma unroll
             for (int a = 0; a < kItemsAhead; ++a) {
-                ids[a] = -1;
+                cs[a] = -1;
+                ps[a] = 0;
                 ms[a] = 0ull;
                 if (live) {
                     const int i = __ffsll((long long)live) - 1;
                     live &= live - 1ull;
-                    ids[a] = __builtin_amdgcn_readlane(id, i);
+                    cs[a] = __builtin_amdgcn_readlane(ic, i);
+                    ps[a] = __builtin_amdgcn_readlane(ip, i);
                     ms[a] = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(m >> 32), i) << 32) |
                             (unsigned)__builtin_amdgcn_readlane((int)m, i);
                 }
@@ -213,18 +217,18 @@ __global__ __launch_bounds__(kBlock) void merge_brick_model_kernel(
             for (int a = 0; a < kItemsAhead; ++a) {
                 wv[a] = 0.;  // the view; divided by corr when its entry is drained
                 // a set bit: t < num_pix, mask[t] <= 1
-                if (ids[a] >= 0 && ((ms[a] >> lane) & 1ull))
-                    wv[a] = (double)view[(size_t)(ids[a] / npatch) * num_pix + (ids[a] % npatch) * 64 + lane];
+                if (cs[a] >= 0 && ((ms[a] >> lane) & 1ull))
+                    wv[a] = (double)view[(size_t)cs[a] * num_pix + ps[a] * 64 + lane];
             }
 #pragma unroll
             for (int a = 0; a < kItemsAhead; ++a) {
-                if (ids[a] < 0) continue;
+                if (cs[a] < 0) continue;
                 const bool nz = wv[a] != 0.;  // a zero view adds exactly nothing
                 const unsigned long long bal = __ballot(nz);
                 if (nz) {
                     const int at = (tail + __popcll(bal & ((1ull << lane) - 1))) & (kRing - 1);
-                    q_r[wave][at] = r0 + ids[a] / npatch;
-                    q_t[wave][at] = (ids[a] % npatch) * 64 + lane;
+                    q_r[wave][at] = r0 + cs[a];
+                    q_t[wave][at] = ps[a] * 64 + lane;
                     q_w[wave][at] = wv[a];
                 }
                 tail += __popcll(bal);

@@ -454,7 +454,10 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     unsigned long long *__restrict__ nzmask, const double *__restrict__ RF, int nF,
     const int *__restrict__ sigF_row, const int *__restrict__ sigF_frame, const int *__restrict__ sigF_cnt) {
     static_assert(kPairRange == 64, "a pixel range is one 64-pixel merge patch");
-    __shared__ double acc[kPairRange][kRTile + 1];
+    // LDS sums in the view's type: FP64 for an FP64 view, FP32 for an FP32 one (half the LDS,
+    // more resident blocks; each sum has a few terms and is stored in FP32 anyway)
+    using A = VT;
+    __shared__ A acc[kPairRange][kRTile + 1];
     __shared__ long s_row[kBlock];  // element offset of the row (P or RF)
     __shared__ bool s_full[kBlock];
     __shared__ long s_q0[kBlock], s_q1[kBlock];
@@ -487,10 +490,10 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
                 const size_t at = (size_t)lt * tsW + (size_t)t * kRTile + lane;
                 for (int x = 0; x < nparts; ++x) a += (double)partial[x * tsPart + at];
             }
-            acc[i][lane] = a;
+            acc[i][lane] = (A)a;
         }
     } else {
-        for (int i = tid; i < kPairRange * (kRTile + 1); i += kBlock) (&acc[0][0])[i] = 0.;
+        for (int i = tid; i < kPairRange * (kRTile + 1); i += kBlock) (&acc[0][0])[i] = (A)0;
     }
     for (long jb = 0; jb < nRows; jb += kBlock) {
         if (tid == 0) s_n = 0;
@@ -540,22 +543,36 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
 #pragma unroll
             for (int u = 0; u < kQueueRows; ++u) {
                 if (!(p[u] > kProbMin)) continue;  // also the entries past the queue (p = 0)
-                unsafeAtomicAdd(&acc[px0[u] - p0][lane], c0[u] * p[u]);
-                for (long q = qa[u] + 1; q < qb[u]; ++q) unsafeAtomicAdd(&acc[fpix[q] - p0][lane], fcnt[q] * p[u]);
+                unsafeAtomicAdd(&acc[px0[u] - p0][lane], (A)(c0[u] * p[u]));
+                for (long q = qa[u] + 1; q < qb[u]; ++q)
+                    unsafeAtomicAdd(&acc[fpix[q] - p0][lane], (A)(fcnt[q] * p[u]));
             }
         }
         __syncthreads();  // every wave has read the queue before it is refilled
     }
+    // psum of the tile's slots (FP32 views: its FP32 reciprocal, one division per slot)
+    __shared__ double s_ps[kRTile];
+    __shared__ float s_inv[kRTile];
+    if (tid < kRTile) {
+        const double ps = psum_slot[(size_t)T * kRTile + tid];
+        s_ps[tid] = ps;
+        s_inv[tid] = ps > 0. ? (float)(1. / ps) : 1.f;
+    }
     __syncthreads();
-    const double ps = psum_slot[(size_t)T * kRTile + lane];
-    for (int i = wave; i < kPairRange; i += kWavesPerBlock)
-        acc[i][lane] = ps > 0. ? acc[i][lane] / ps : acc[i][lane];
-    __syncthreads();
-    // transpose: lanes along pixels. With nzmask, item (slot, patch rg) records which of its
-    // pixels have a nonzero view and only those are stored: the merge reads no other.
+    // divide by psum while transposing (lanes along pixels). With nzmask, item (slot, patch rg)
+    // records which of its pixels have a nonzero view and only those are stored: the merge
+    // reads no other.
     for (int jj = wave; jj < kRTile; jj += kWavesPerBlock) {
         const int t = p0 + lane;
-        const double v = t < num_pix ? acc[lane][jj] : 0.;
+        double v = 0.;
+        if (t < num_pix) {
+            if constexpr (sizeof(VT) == sizeof(float)) {
+                v = (double)(acc[lane][jj] * s_inv[jj]);
+            } else {
+                const double ps = s_ps[jj];
+                v = ps > 0. ? (double)acc[lane][jj] / ps : (double)acc[lane][jj];
+            }
+        }
         const size_t at = ((size_t)lt * kRTile + jj) * num_pix + t;
         if (nzmask) {
             const unsigned long long m = __ballot(v != 0.);
