@@ -1292,9 +1292,12 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     size_t freeb = 0, totalb = 0;
     HIP_TRY(hipMemGetInfo(&freeb, &totalb));
     const size_t reserve = ((size_t)4 << 30) + brick_list_extra(e->slot_bricks, S, P);
+    // free memory plus the chunk buffers sized below (the full frames' ones only when there are
+    // full frames: else they keep their room for the next lists that have some)
     size_t budget = (freeb > reserve ? freeb - reserve : 0) + e->logw.cap * sizeof(double) +
-                    e->tom_partial.cap * sizeof(double) +
-                    (e->logw32.cap + e->p32.cap + e->tom_partial32.cap) * sizeof(float);
+                    e->logw32.cap * sizeof(float) +
+                    (nF > 0 ? e->tom_partial.cap * sizeof(double) + (e->p32.cap + e->tom_partial32.cap) * sizeof(float)
+                            : 0);
     budget = std::min(budget, (size_t)120 << 30);
     // per slot column: log W / view and the full frames' partial views (FP64; with FP32 gathers
     // all FP32, log W and the view sharing one buffer, plus the full frames' FP32 posterior
@@ -1314,25 +1317,38 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
     if (e->slot_C > 0 && e->slot_C < C && e->slot_bricks_valid) C = e->slot_C;
     if (C != e->slot_C) e->slot_bricks_valid = false;
     e->slot_C = C;
-    if (g32 ? e->logw.cap > 0 : e->logw.cap < (size_t)P * C) {
+    // the chunk buffers at the sizes the budget assumed: every buffer larger than that (a wider
+    // chunk before) is released before any is allocated, since the budget counted all of it as
+    // free; a size of 0 (no full frames) leaves the buffer as it is
+    auto shrink = [](auto &buf, size_t n) {
+        if (n > 0 && buf.cap > n) buf.release();
+    };
+    auto fit = [](auto &buf, size_t n) -> int { return n > 0 ? buf.alloc(n) : 0; };
+    const size_t C128 = round_up((size_t)C, 128);  // FP32 log W: 128-wide tiles
+    if (g32 ? e->logw.cap > 0 : e->logw.cap != (size_t)P * C) {
         // the FP64 log W / view buffer: none with FP32 gathers (log W and the view are FP32)
         e->logw.release();
         e->C = 0;
         e->bricks_valid = false;
         if (!g32 && (rc = e->logw.alloc((size_t)P * C))) return rc;
     }
+    if (g32 && e->logw32.cap > (size_t)P * C128) e->logw_resident = false;  // released below
     if (g32) {
+        const size_t n_part = nF > 0 ? (size_t)nparts * P * (size_t)C : 0, n_logw = (size_t)P * C128,
+                     n_p32 = nF > 0 ? (size_t)nF * C128 : 0;
         e->tom_partial.release();
-        if (nF > 0 && (rc = e->tom_partial32.alloc((size_t)nparts * P * (size_t)C))) return rc;
+        shrink(e->logw32, n_logw);
+        shrink(e->tom_partial32, n_part);
+        shrink(e->p32, n_p32);
+        if ((rc = fit(e->logw32, n_logw)) || (rc = fit(e->tom_partial32, n_part)) || (rc = fit(e->p32, n_p32)))
+            return rc;
     } else {
         e->tom_partial32.release();
-        if (nF > 0 && (rc = e->tom_partial.alloc((size_t)nparts * P * (size_t)C))) return rc;
-    }
-    if (g32) {
-        // 128-wide tiles: room for the chunk rounded up to 128 slots
-        const size_t C128 = round_up((size_t)C, 128);
-        if ((rc = e->logw32.alloc((size_t)P * C128))) return rc;
-        if (nF > 0 && (rc = e->p32.alloc((size_t)nF * C128))) return rc;
+        e->logw32.release();
+        e->p32.release();
+        const size_t n_part = nF > 0 ? (size_t)nparts * P * (size_t)C : 0;
+        shrink(e->tom_partial, n_part);
+        if ((rc = fit(e->tom_partial, n_part))) return rc;
     }
     const int ngroups = (P + kPixPerGroup - 1) / kPixPerGroup;
     if ((rc = e->upart.alloc((size_t)ngroups * C))) return rc;
