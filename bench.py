@@ -511,7 +511,7 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
         if k_coarse > 0 and source != 'coarse_only':
             tops.append(judge.top_cells(k_coarse))
         t2 = time.perf_counter()
-        info = proposal.judged_iteration(recon, itr, judge, tops, cfg['coarse_div'], tau)
+        info = proposal.judged_iteration(recon, itr, judge, tops, cfg['coarse_div'], tau, widen=args.widen)
         torch.cuda.synchronize()
         info['phase_ms'] = dict(coarse_search=round((t1 - t0) * 1e3, 1), proposals=round((t2 - t1) * 1e3, 1),
                                 **info['phase_ms'])
@@ -536,8 +536,9 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
         res = dict(step_ms=round(step_s * 1e3, 1), steps_ms=[round(t * 1e3, 1) for t in times],
                    num_chunks=recon.last_stats['num_chunks'], passes=info['passes'],
                    cells_per_frame=round(info['cells_per_frame'], 2), speedup_vs_fp64=round(plain_s / step_s, 2),
-                   fallback_rate=round(float(1. - acc.mean()), 4),
-                   heldout_fallback_rate=round(float(1. - acc[held].mean()), 4),
+                   fallback_rate=round(float(1. - acc.mean() - info['widened'].mean()), 4),
+                   widened_rate=round(float(info['widened'].mean()), 4),
+                   heldout_fallback_rate=round(float(1. - (acc | info['widened'])[held].mean()), 4),
                    judge_outside_accepted_mean=float('%.3g' % info['outside'][acc].mean()) if acc.any() else None,
                    model_rel_err=float('%.3g' % (np.abs(ml - model_plain).max() / np.abs(model_plain).max())),
                    model_rms_rel_err=float('%.3g' % np.sqrt(np.mean((ml - model_plain) ** 2) / np.mean(model_plain ** 2))),
@@ -760,6 +761,9 @@ def main():
     ap.add_argument('--sig-full-rows', type=int, default=None,
                     help='learned-proposal workload: mean significant full-frame rows per tile up to which the '
                          'pair tomogram reads them directly (engine option sig_full_rows)')
+    ap.add_argument('--widen', type=int, default=0,
+                    help='learned-proposal workload: at most this many coarse cells the judge adds to a rejected '
+                         'frame before it falls back to the full search (0: no widening)')
     ap.add_argument('--unit-items', type=int, default=None,
                     help='learned-proposal workload: brick-merge list entries per work unit (engine option '
                          'unit_items)')

@@ -90,6 +90,8 @@ def lib():
             'emc_slice_gen_static': (C.c_int, [C.c_void_p, _dp, _dp]),
             'emc_subset_mass': (C.c_int, [C.c_void_p, _lp, _ip, _dp]),
             'emc_judge_outside': (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, _dp]),
+            'emc_judge_widen': (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int, _dp, _ip, _lp]),
+            'emc_judge_widen_cells': (C.c_int, [C.c_void_p, _ip, C.c_long]),
             'emc_cell_topk': (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, _ip, C.POINTER(C.c_float)]),
             'emc_slice_merge': (C.c_int, [C.c_void_p, _dp, _dp]),
             'emc_get_field': (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
@@ -328,6 +330,22 @@ class Engine:
         out = np.empty(self.num_data)
         _check(lib().emc_judge_outside(self._h, coarse._h, float(prob_floor), _ptr(out)))
         return out
+
+    def judge_widen(self, coarse, outside_max, max_add, target=None):
+        """judge_outside (prob_floor 0) and, for the frames above outside_max, the fewest coarse
+        cells to add for the estimate to fall to target (default outside_max / 10: the estimate
+        of a widened set reads cells at the coarse sampling only): (outside, num_add, cells) with
+        num_add (num_data,) i4 = 0 (kept as proposed), k (k cells), -1 (the full search) and
+        cells the added cells frame after frame."""
+        out = np.empty(self.num_data)
+        num = np.empty(self.num_data, dtype='i4')
+        tot = C.c_long(0)
+        target = outside_max / 10. if target is None else target
+        _check(lib().emc_judge_widen(self._h, coarse._h, float(outside_max), float(target), int(max_add), _ptr(out),
+                                     _ptr(num, _ip), C.byref(tot)))
+        cells = np.empty(max(tot.value, 1), dtype='i4')
+        _check(lib().emc_judge_widen_cells(self._h, _ptr(cells, _ip), cells.size))
+        return out, num, cells[:tot.value]
 
     def slice_merge(self, quat, view):
         quat = np.ascontiguousarray(quat, dtype='f8')

@@ -423,6 +423,11 @@ struct emc_engine {
     DevBuf<int> sigF_row, sigF_frame, sigF_cnt;
     DevBuf<long> sigF_total;
     DevBuf<unsigned long long> nz_count;  // nonzero view entries of the last subset iteration
+    // cells the judge adds to rejected frames (emc_judge_widen): per frame (stride max_add),
+    // their counts, and compacted frame after frame
+    DevBuf<int> judge_nadd, judge_add, judge_cells;
+    DevBuf<long> judge_off;
+    long judge_cells_n = -1;
     bool nz_counted = false;
     std::vector<BrickChunk> slot_bricks;
     bool slot_bricks_valid = false;
@@ -2312,21 +2317,21 @@ int emc_set_scale(emc_engine *e, const double *scale) {
     return 0;
 }
 
-int emc_judge_outside(emc_engine *e, emc_engine *coarse, double prob_floor, double *outside) {
-    if (!e || !coarse || !outside) return fail(EMC_E_ARG, "emc_judge_outside: null argument");
-    if (!(prob_floor >= 0.) || prob_floor >= 1.) return fail(EMC_E_ARG, "emc_judge_outside: prob_floor in [0, 1)");
+// Checks and per-cell inputs of the likelihood judge: the two engines' states, each cell's log
+// fine orientation count and log mean fine weight over its centre's weight, and the frames it
+// skips (full or blacklisted); leaves the coarse engine's results complete.
+static int judge_setup(emc_engine *e, emc_engine *coarse, const char *who) {
     if (!e->refine || !e->subset_ran || !e->have_results)
-        return fail(EMC_E_STATE, "emc_judge_outside: needs an iteration on refinement subsets");
+        return fail(EMC_E_STATE, "%s: needs an iteration on refinement subsets", who);
     if (!coarse->have_results || coarse->subset_ran || coarse->refine)
-        return fail(EMC_E_STATE, "emc_judge_outside: the coarse engine needs a plain iteration");
+        return fail(EMC_E_STATE, "%s: the coarse engine needs a plain iteration", who);
     const int N = e->num_data, Nc = e->slot_num_coarse;
     if (coarse->num_data != N || coarse->num_rot != Nc || coarse->num_modes != 1 || coarse->device != e->device)
-        return fail(EMC_E_ARG, "emc_judge_outside: the coarse engine must hold the same frames and one orientation "
-                               "per coarse cell (%d) on the same device", Nc);
+        return fail(EMC_E_ARG, "%s: the coarse engine must hold the same frames and one orientation "
+                               "per coarse cell (%d) on the same device", who, Nc);
     if ((int)e->h_slot_qmap.size() != e->num_rot || (int)coarse->h_quats.size() != 5 * Nc)
-        return fail(EMC_E_STATE, "emc_judge_outside: orientation tables not set");
+        return fail(EMC_E_STATE, "%s: orientation tables not set", who);
     HIP_TRY(hipSetDevice(e->device));
-    // per cell: log of its fine orientation count, log of its mean fine weight over the centre's weight
     std::vector<double> cnt(Nc, 0.), wsum(Nc, 0.), logn(Nc), lwr(Nc);
     for (int r = 0; r < e->num_rot; ++r) {
         cnt[e->h_slot_qmap[r]] += 1.;
@@ -2346,12 +2351,71 @@ int emc_judge_outside(emc_engine *e, emc_engine *coarse, double prob_floor, doub
     if ((rc = e->judge_out.alloc(N))) return rc;
     // the coarse engine's posterior, maximum and normaliser are complete on its own stream
     HIP_TRY(hipStreamSynchronize(coarse->stream));
+    return 0;
+}
+
+int emc_judge_outside(emc_engine *e, emc_engine *coarse, double prob_floor, double *outside) {
+    if (!e || !coarse || !outside) return fail(EMC_E_ARG, "emc_judge_outside: null argument");
+    if (!(prob_floor >= 0.) || prob_floor >= 1.) return fail(EMC_E_ARG, "emc_judge_outside: prob_floor in [0, 1)");
+    int rc;
+    if ((rc = judge_setup(e, coarse, "emc_judge_outside"))) return rc;
+    const int N = e->num_data, Nc = e->slot_num_coarse;
     emc::launch_judge_outside(e->stream, coarse->R.p, N, Nc, coarse->max_exp.p, coarse->p_norm.p, e->max_exp.p,
                               e->p_norm.p, e->beta.p, e->judge_skip.p, e->relbits.p, e->rel_words, e->judge_logn.p,
                               e->judge_lwr.p, prob_floor, e->judge_out.p);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(outside, e->judge_out.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int emc_judge_widen(emc_engine *e, emc_engine *coarse, double outside_max, double target, int max_add,
+                    double *outside, int *num_add, long *total) {
+    if (!e || !coarse || !outside || !num_add || !total) return fail(EMC_E_ARG, "emc_judge_widen: null argument");
+    if (!(outside_max > 0.) || !(outside_max < 1.)) return fail(EMC_E_ARG, "emc_judge_widen: outside_max in (0, 1)");
+    if (!(target > 0.) || !(target <= outside_max))
+        return fail(EMC_E_ARG, "emc_judge_widen: target in (0, outside_max]");
+    if (max_add < 1) return fail(EMC_E_ARG, "emc_judge_widen: max_add %d < 1", max_add);
+    if (e->slot_num_coarse > emc::kWidenCells)
+        return fail(EMC_E_ARG, "emc_judge_widen: %d coarse cells (at most %d)", e->slot_num_coarse, emc::kWidenCells);
+    int rc;
+    if ((rc = judge_setup(e, coarse, "emc_judge_widen"))) return rc;
+    const int N = e->num_data, Nc = e->slot_num_coarse;
+    max_add = std::min(max_add, Nc);
+    hipStream_t s = e->stream;
+    if ((rc = e->judge_nadd.alloc(N)) || (rc = e->judge_add.alloc((size_t)N * max_add))) return rc;
+    emc::launch_judge_widen(s, coarse->R.p, N, Nc, coarse->max_exp.p, coarse->p_norm.p, e->max_exp.p, e->p_norm.p,
+                            e->beta.p, e->judge_skip.p, e->relbits.p, e->rel_words, e->judge_logn.p, e->judge_lwr.p,
+                            outside_max, target, max_add, e->judge_out.p, e->judge_nadd.p, e->judge_add.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(outside, e->judge_out.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(num_add, e->judge_nadd.p, (size_t)N * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // the added cells frame after frame, kept for emc_judge_widen_cells
+    std::vector<long> off(N + 1, 0);
+    for (int d = 0; d < N; ++d) off[d + 1] = off[d] + std::max(num_add[d], 0);
+    *total = off[N];
+    e->judge_cells_n = off[N];
+    if ((rc = e->judge_off.upload(off.data(), off.size(), s)) ||
+        (rc = e->judge_cells.alloc((size_t)std::max(off[N], 1L))))
+        return rc;
+    emc::launch_widen_compact(s, e->judge_add.p, e->judge_nadd.p, e->judge_off.p, N, max_add, e->judge_cells.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int emc_judge_widen_cells(emc_engine *e, int *cells, long capacity) {
+    if (!e || (!cells && e->judge_cells_n > 0)) return fail(EMC_E_ARG, "emc_judge_widen_cells: null argument");
+    if (e->judge_cells_n < 0) return fail(EMC_E_STATE, "emc_judge_widen_cells: no emc_judge_widen before");
+    if (capacity < e->judge_cells_n)
+        return fail(EMC_E_ARG, "emc_judge_widen_cells: capacity %ld < %ld cells", capacity, e->judge_cells_n);
+    HIP_TRY(hipSetDevice(e->device));
+    if (e->judge_cells_n > 0) {
+        HIP_TRY(hipMemcpyAsync(cells, e->judge_cells.p, (size_t)e->judge_cells_n * sizeof(int), hipMemcpyDeviceToHost,
+                               e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
     return 0;
 }
 

@@ -83,6 +83,15 @@ void launch_fused_expand_likelihood(hipStream_t s, const double *rots, const flo
                                     const int *slot_rot, const long *off, const int *pix, const double *cnt,
                                     const int *pair_frame, const int *pair_tile, const long *tile_off,
                                     const int *tile_pair, int t0, unsigned *counter, double *G);
+// the fewest cells to add to each frame the judge rejects (judge_widen_kernel); num_coarse
+// <= kWidenCells; add: num_data * max_add; then compacted to the CSR offsets off
+constexpr int kWidenCells = 4096;
+void launch_judge_widen(hipStream_t s, const double *Pc, int num_data, int num_coarse, const double *Mc,
+                        const double *pnc, const double *Mf, const double *pnf, const double *beta, const uint8_t *skip,
+                        const uint32_t *bits, int words, const double *log_n, const double *log_wr, double outside_max,
+                        double target, int max_add, double *out, int *num_add, int *add);
+void launch_widen_compact(hipStream_t s, const int *add, const int *num_add, const long *off, int num_data,
+                          int max_add, int *out);
 void launch_judge_outside(hipStream_t s, const double *Pc, int num_data, int num_coarse, const double *Mc,
                           const double *pnc, const double *Mf, const double *pnf, const double *beta,
                           const uint8_t *skip, const uint32_t *bits, int words, const double *log_n,

@@ -1411,6 +1411,137 @@ __global__ __launch_bounds__(kBlock) void judge_outside_kernel(
     if (lane == 0) out[d] = v;
 }
 
+// For the frames judge_outside_kernel (prob_floor 0) puts above outside_max: the fewest coarse
+// cells outside the frame's set after whose addition the estimate, still with the fine
+// maximum and normaliser of the set alone (both only grow with more cells, so the estimate is
+// an upper bound), is at most target -- the cells of largest share of Z_out, found by
+// bisection on the share. One wavefront per frame, a lane holding cells lane + 64 j
+// (num_coarse <= 64 * kWidenPerLane). out: the estimate; num_add: 0 (kept, skipped, or no mass
+// outside), k (add[d * max_add + i], i < k, cells ascending), -1 (more than max_add cells, or
+// a non-finite normaliser: the full search).
+constexpr int kWidenPerLane = 64;
+static_assert(64 * kWidenPerLane == kWidenCells, "a wavefront's cells");
+__global__ __launch_bounds__(kBlock) void judge_widen_kernel(
+    const double *__restrict__ Pc, int num_data, int num_coarse, const double *__restrict__ Mc,
+    const double *__restrict__ pnc, const double *__restrict__ Mf, const double *__restrict__ pnf,
+    const double *__restrict__ beta, const uint8_t *__restrict__ skip, const uint32_t *__restrict__ bits,
+    int words, const double *__restrict__ log_n, const double *__restrict__ log_wr, double outside_max, double target,
+    int max_add, double *__restrict__ out, int *__restrict__ num_add, int *__restrict__ add) {
+    const int lane = threadIdx.x & 63;
+    const int d = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (d >= num_data) return;
+    if (skip[d]) {
+        if (lane == 0) out[d] = 0., num_add[d] = 0;
+        return;
+    }
+    const double b = beta[d];
+    const uint32_t *row = bits + (size_t)d * words;
+    // each outside cell's share of Z_out (up to the common factor), as judge_outside_kernel sums it
+    double acc = 0.;
+    float lt[kWidenPerLane];  // its log (-inf: inside the set or no posterior)
+#pragma unroll
+    for (int j = 0; j < kWidenPerLane; ++j) {
+        const int c = lane + 64 * j;
+        lt[j] = -INFINITY;
+        if (c < num_coarse && !((row[c >> 5] >> (c & 31)) & 1u)) {
+            const double p = Pc[(size_t)(c / kRTile) * ((size_t)num_data * kRTile) + (size_t)d * kRTile + c % kRTile];
+            if (p > 0.) {
+                const double term = p * exp(log_n[c] + b * log_wr[c]);
+                acc += term;
+                lt[j] = (float)log(term);
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (!(acc > 0.)) {
+        if (lane == 0) out[d] = 0., num_add[d] = 0;
+        return;
+    }
+    const double lz0 = log(pnc[d]) + b * (Mc[d] - Mf[d]);  // log of Z_out / acc
+    const double lr = log(pnf[d]) - (lz0 + log(acc));    // log Z_in / Z_out
+    const double share = lr > 700. ? 0. : 1. / (1. + exp(lr));
+    if (lane == 0) out[d] = share;
+    if (!(share > outside_max)) {
+        if (lane == 0) num_add[d] = 0;
+        return;
+    }
+    // the remaining sum allowed: lz0 + log(rem) <= log(target / (1 - target)) + log Z_in
+    const double lallow = log(target / (1. - target)) + log(pnf[d]) - lz0;
+    float lmax = -INFINITY, lmin = INFINITY;
+#pragma unroll
+    for (int j = 0; j < kWidenPerLane; ++j)
+        if (lt[j] > -INFINITY) lmax = fmaxf(lmax, lt[j]), lmin = fminf(lmin, lt[j]);
+    for (int o = 32; o > 0; o >>= 1) {
+        lmax = fmaxf(lmax, __shfl_xor(lmax, o));
+        lmin = fminf(lmin, __shfl_xor(lmin, o));
+    }
+    if (!isfinite(lallow) || !isfinite(lmax)) {
+        if (lane == 0) num_add[d] = -1;
+        return;
+    }
+    // shares relative to the largest: s = lt - lmax <= 0; find the largest cut t with
+    // sum over s < t of exp(s) <= exp(lallow - lmax); cells with s >= t are added
+    const float g = (float)(lallow - lmax);
+    float lo = lmin - lmax - 1.f, hi = 0.5f;  // sum below lo: 0 (fits); below hi: everything (does not)
+    for (int it = 0; it < 28; ++it) {
+        const float mid = 0.5f * (lo + hi);
+        float f = 0.f;
+#pragma unroll
+        for (int j = 0; j < kWidenPerLane; ++j) {
+            const float sj = lt[j] - lmax;
+            if (sj < mid) f += __expf(sj);  // -inf entries add 0
+        }
+        for (int o = 32; o > 0; o >>= 1) f += __shfl_xor(f, o);
+        if (f > 0.f && __logf(f) > g)
+            hi = mid;
+        else
+            lo = mid;
+    }
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < kWidenPerLane; ++j) cnt += (lt[j] > -INFINITY && lt[j] - lmax >= lo) ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if (cnt > max_add) {
+        if (lane == 0) num_add[d] = -1;
+        return;
+    }
+    int base = 0;
+#pragma unroll
+    for (int j = 0; j < kWidenPerLane; ++j) {
+        const bool take = lt[j] > -INFINITY && lt[j] - lmax >= lo;
+        const unsigned long long m = __ballot(take);
+        if (take) add[(size_t)d * max_add + base + __popcll(m & ((1ull << lane) - 1ull))] = lane + 64 * j;
+        base += __popcll(m);
+    }
+    if (lane == 0) num_add[d] = cnt;
+}
+
+// out[off[d] + i] = add[d * max_add + i] for i < num_add[d] (> 0)
+__global__ void widen_compact_kernel(const int *__restrict__ add, const int *__restrict__ num_add,
+                                     const long *__restrict__ off, int num_data, int max_add, int *__restrict__ out) {
+    const int d = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (d >= num_data) return;
+    const int n = num_add[d];
+    for (int i = lane; i < n; i += 64) out[off[d] + i] = add[(size_t)d * max_add + i];
+}
+
+void launch_judge_widen(hipStream_t s, const double *Pc, int num_data, int num_coarse, const double *Mc,
+                        const double *pnc, const double *Mf, const double *pnf, const double *beta, const uint8_t *skip,
+                        const uint32_t *bits, int words, const double *log_n, const double *log_wr, double outside_max,
+                        double target, int max_add, double *out, int *num_add, int *add) {
+    if (num_data <= 0) return;
+    judge_widen_kernel<<<cdiv(num_data, kWavesPerBlock), kBlock, 0, s>>>(Pc, num_data, num_coarse, Mc, pnc, Mf, pnf,
+                                                                         beta, skip, bits, words, log_n, log_wr,
+                                                                         outside_max, target, max_add, out, num_add,
+                                                                         add);
+}
+
+void launch_widen_compact(hipStream_t s, const int *add, const int *num_add, const long *off, int num_data,
+                          int max_add, int *out) {
+    if (num_data <= 0) return;
+    widen_compact_kernel<<<cdiv(num_data, kWavesPerBlock), kBlock, 0, s>>>(add, num_add, off, num_data, max_add, out);
+}
+
 void launch_judge_outside(hipStream_t s, const double *Pc, int num_data, int num_coarse, const double *Mc,
                           const double *pnc, const double *Mf, const double *pnf, const double *beta,
                           const uint8_t *skip, const uint32_t *bits, int words, const double *log_n,

@@ -330,7 +330,7 @@ def refine_rows(off, cells, accept):
     return rel_off, out[out != -2]
 
 
-def judged_iteration(recon, itr, judge, tops, coarse_div, threshold=0.99):
+def judged_iteration(recon, itr, judge, tops, coarse_div, threshold=0.99, widen=0):
     """One EMC iteration (recon.run_iteration) on proposals judged by the likelihood.
 
     Pass 1: the expectation step of the fine search restricted to each frame's proposed cells
@@ -339,9 +339,15 @@ def judged_iteration(recon, itr, judge, tops, coarse_div, threshold=0.99):
     same model, scales and beta: CoarseJudge.estep) then estimates, per frame, the posterior
     mass the full fine search would put outside the cells (Engine.judge_outside). Frames
     estimated to hold at least `threshold` inside keep their cells; the rest fall back to the
-    full fine search. Pass 2: when every frame is kept, the M-step resumes from pass 1
-    (option mstep_only); otherwise the iteration runs again with the fall-back frames searched
-    in full. Returns a dict: accept (bool per frame), outside (estimates), passes (1 or 2)."""
+    full fine search. With `widen` > 0 a rejected frame instead gets the fewest further coarse
+    cells (at most `widen`) after which the judge's estimate, an upper bound taken with the
+    fine maximum and normaliser of its first cells, is at most (1 - threshold) / 10
+    (Engine.judge_widen: the added cells are judged at the coarse sampling only, hence the
+    margin), and falls back to the full search only when more are needed. Pass 2:
+    when every frame is kept, the M-step resumes from pass 1 (option mstep_only); otherwise the
+    iteration runs again with the widened and fall-back frames. Returns a dict: accept (bool
+    per frame: kept as proposed), widened (bool per frame), outside (estimates), passes (1 or
+    2)."""
     import time
     eng = recon.engine
     par = itr.params
@@ -357,8 +363,13 @@ def judged_iteration(recon, itr, judge, tops, coarse_div, threshold=0.99):
     finally:
         eng.set_option('estep_only', 0)
     t.append(time.perf_counter())
-    outside = eng.judge_outside(judge.eng)
-    accept = outside <= 1. - threshold
+    if widen > 0:
+        outside, num_add, added = eng.judge_widen(judge.eng, 1. - threshold, widen)
+        accept = num_add == 0
+    else:
+        outside = eng.judge_outside(judge.eng)
+        accept = outside <= 1. - threshold
+        num_add = np.where(accept, 0, -1)
     if accept.all():
         t.append(time.perf_counter())
         eng.set_option('mstep_only', 1)
@@ -368,7 +379,10 @@ def judged_iteration(recon, itr, judge, tops, coarse_div, threshold=0.99):
             eng.set_option('mstep_only', 0)
         passes = 1
     else:
-        rel_off, rel_quat = refine_rows(off, cells, accept)
+        if widen > 0:
+            rel_off, rel_quat = widen_lists_csr(off, cells, num_add, added)
+        else:
+            rel_off, rel_quat = refine_rows(off, cells, accept)
         itr.set_refine_lists(coarse_div, rel_off, rel_quat)
         eng.set_refine(itr.num_coarse, itr.quat_mapping, itr.rel_off, itr.rel_quat)
         t.append(time.perf_counter())
@@ -379,7 +393,8 @@ def judged_iteration(recon, itr, judge, tops, coarse_div, threshold=0.99):
     # distinct cells per frame, on a sample of frames (the lists may repeat cells)
     X = cells.reshape(len(off) - 1, -1)[:: max(1, (len(off) - 1) // 2000)]
     distinct = (np.diff(np.sort(X, axis=1), axis=1) != 0).sum(1) + 1
-    return dict(accept=accept, outside=outside, passes=passes, cells_per_frame=float(distinct.mean()),
+    return dict(accept=accept, widened=num_add > 0, outside=outside, passes=passes,
+                cells_per_frame=float(distinct.mean()),
                 estep_stage_ms=st1['stage_ms'], phase_ms=host_ms)
 
 
@@ -427,6 +442,26 @@ def refine_lists_csr(off, cells, accept, num_coarse=None):
     np.cumsum(np.where(accept, n, 1), out=rel_off[1:])
     rel_quat = np.full(int(rel_off[-1]), -1, 'i4')
     rel_quat[_ranges(rel_off[:-1][accept], n[accept])] = np.asarray(cells, 'i4')[_ranges(off[:-1][accept], n[accept])]
+    return rel_off, rel_quat
+
+
+def widen_lists_csr(off, cells, num_add, add):
+    """Refinement lists after the judge's widening (Engine.judge_widen): a frame with num_add 0
+    keeps its cells, one with k > 0 gets its cells followed by its k added cells (`add`, frame
+    after frame), one with -1 the single entry -1 (the full search). Returns (rel_off,
+    rel_quat)."""
+    off = np.asarray(off, 'i8')
+    num_add = np.asarray(num_add, 'i8')
+    n = np.diff(off)
+    full = num_add < 0
+    grow = np.maximum(num_add, 0)
+    rel_off = np.zeros(len(n) + 1, 'i8')
+    np.cumsum(np.where(full, 1, n + grow), out=rel_off[1:])
+    rel_quat = np.full(int(rel_off[-1]), -1, 'i4')
+    keep = ~full
+    rel_quat[_ranges(rel_off[:-1][keep], n[keep])] = np.asarray(cells, 'i4')[_ranges(off[:-1][keep], n[keep])]
+    wide = grow > 0
+    rel_quat[_ranges(rel_off[:-1][wide] + n[wide], grow[wide])] = np.asarray(add, 'i4')
     return rel_off, rel_quat
 
 

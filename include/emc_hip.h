@@ -265,6 +265,20 @@ int emc_subset_mass(emc_engine *e, const long *rel_off, const int *rel_quat, dou
  * No counterpart in the reference: proposals are accepted or sent to the full search by this
  * estimate (refinement in the reference, src/max_emc.c:264-274, takes its cells as given).  */
 int emc_judge_outside(emc_engine *e, emc_engine *coarse, double prob_floor, double *outside);
+/* The same judge (prob_floor 0), and for every frame it puts above outside_max (in (0, 1)) the
+ * fewest coarse cells outside the frame's set whose addition brings the estimate, still taken
+ * with the fine maximum and normaliser of the set alone (an upper bound, since both only grow
+ * with more cells), to at most target (in (0, outside_max]): the cells of largest share of
+ * the outside mass.
+ * outside[d] as emc_judge_outside; num_add[d] (host, num_data) = 0 (at most outside_max, or a
+ * full or blacklisted frame), k > 0 (k cells to add), -1 (more than max_add cells: the full
+ * search); *total = the cells to add over all frames, kept in the engine for
+ * emc_judge_widen_cells. At most 4096 coarse cells. No counterpart in the reference.        */
+int emc_judge_widen(emc_engine *e, emc_engine *coarse, double outside_max, double target, int max_add,
+                    double *outside, int *num_add, long *total);
+/* The cells of the last emc_judge_widen, frame after frame, ascending within a frame
+ * (capacity >= its *total).                                                               */
+int emc_judge_widen_cells(emc_engine *e, int *cells, long capacity);
 /* Zero model2 and inter_weight (max_data allocate_memory, maximize.c:93-94).            */
 int emc_clear_merge(emc_engine *e);
 

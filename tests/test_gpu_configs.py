@@ -267,6 +267,7 @@ def test_cfg5_geometry_judged_proposals():
     eng.set_option('estep_only', 1)
     eng.maximize(4, 1, 1, 0)
     est = eng.judge_outside(judge)
+    _, num_add, added = eng.judge_widen(judge, 1e-3, 256)
     judge.close()
     accept = est <= 1e-3
     rel_off, rel_quat = proposal.refine_lists_csr(off, cells, accept, Nc)
@@ -289,3 +290,29 @@ def test_cfg5_geometry_judged_proposals():
     assert accept.mean() > 0.5
     assert true_out[accept].mean() <= 1e-3 and np.quantile(true_out[accept], 0.99) <= 1e-2
     assert err <= 1e-3
+    # the judge widening the rejected frames' cells (at most 256 more) instead of the full search
+    assert np.array_equal(num_add == 0, accept)
+    wide = num_add > 0
+    w_off, w_cells = proposal.widen_lists_csr(off, cells, num_add, added)
+    eng = P.engine_for(dict(p, model1=model_k))
+    eng.set_scale(scale_k)
+    eng.set_refine(Nc, qmap, w_off, w_cells)
+    eng.maximize(4, 1, 1, 0)
+    eng.update_model(0.)
+    model_w, rmax_w = eng.get('model1'), eng.get('rmax')
+    eng.close()
+    inside_w = np.zeros(post4.shape, bool)
+    fr = np.repeat(np.arange(len(w_off) - 1), np.diff(w_off))
+    sel = w_cells >= 0
+    inside_w[fr[sel], w_cells[sel]] = True
+    inside_w[num_add < 0] = True
+    true_w = np.clip(1. - (post4 * inside_w).sum(1), 0., 1.)
+    err_w = np.abs(model_w - model_plain).max() / np.abs(model_plain).max()
+    print('cfg5 geometry, widened: %.4f of the frames widened, %.4f full; true outside of widened mean %.2e / '
+          'q99 %.2e; model max-abs %.2e, rmax agree %.4f' % (wide.mean(), (num_add < 0).mean(),
+                                                              true_w[wide].mean() if wide.any() else 0.,
+                                                              np.quantile(true_w[wide], 0.99) if wide.any() else 0.,
+                                                              err_w, np.mean(rmax_w == rmax_plain)), flush=True)
+    assert wide.any()
+    assert true_w[wide].mean() <= 1e-3 and np.quantile(true_w[wide], 0.99) <= 1e-2
+    assert err_w <= 1e-3

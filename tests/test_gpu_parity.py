@@ -671,6 +671,8 @@ def test_judge_outside_estimate(tiny_scaled):
     fine.maximize(2, 1, 0, 0)
     Mf, pnf = fine.get('max_exp'), fine.get('p_norm')
     est = fine.judge_outside(coarse)
+    tau = 1e-3
+    est_w, num_add, added = fine.judge_widen(coarse, tau, 16, target=tau / 4)
     fine.close()
     coarse.close()
     n = np.bincount(qmap, minlength=len(qc)).astype('f8')
@@ -690,6 +692,37 @@ def test_judge_outside_estimate(tiny_scaled):
             want[d] = zout / (pnf[d] + zout)
     assert np.allclose(est, want, rtol=1e-10, atol=1e-300), np.abs(est - want).max()
     assert (est[3::5] == 0).all()
+    # widening: the same estimate; frames at most tau keep their cells; a rejected frame gets
+    # the cells of largest share of the outside sum whose removal leaves at most the remainder
+    # allowed by the target tau / 4 (the shortest such prefix of the shares in decreasing
+    # order), or -1 past 16
+    assert np.array_equal(est_w, est)
+    a = 0
+    nwide = 0
+    for d in range(D):
+        k = int(num_add[d])
+        if est[d] <= tau:
+            assert k == 0, d
+            continue
+        g = n * (wbar / qc[:, 4]) ** beta[d]
+        term = np.where(inside[d], 0., Pc[d] * g)
+        allow = (tau / 4) / (1. - tau / 4) * pnf[d] / (pnc[d] * np.exp(beta[d] * (Mc[d] - Mf[d])))
+        order = np.argsort(-term, kind='stable')
+        rem = term.sum() - np.cumsum(term[order])
+        kmin = int(np.argmax(rem <= allow * (1. + 1e-9))) + 1
+        if k < 0:
+            assert kmin > 16 - 1, (d, kmin)  # more than max_add needed (up to the cut's float rounding)
+            continue
+        got = added[a:a + k]
+        a += k
+        nwide += 1
+        assert np.all(np.diff(got) > 0) and not inside[d, got].any()
+        assert abs(k - kmin) <= 1, (d, k, kmin)
+        left = term.sum() - term[got].sum()
+        assert left <= allow * (1. + 1e-5), (d, left, allow)
+        assert term[got].min() >= np.delete(term, got).max() * (1. - 1e-5)
+    assert a == len(added)
+    print('judge widen: %d frames widened, %d to the full search' % (nwide, (num_add < 0).sum()), flush=True)
     true_out = np.clip(1. - (post * inside).sum(1), 0., 1.)
     low = est <= 1e-3
     print('judge: frames estimated <= 1e-3: %d of %d; their true outside mass max %.2e; correlation of logs %.3f'

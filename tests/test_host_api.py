@@ -417,6 +417,31 @@ def test_union_and_csr_refine_lists():
         assert np.array_equal(rq[ro[d]:ro[d + 1]], want)
 
 
+def test_widened_proposal_lists():
+    """widen_lists_csr: kept frames keep their cells, widened ones get theirs followed by the
+    added cells, full ones the single entry -1."""
+    from dragonfly_amd import proposal
+    rng = np.random.default_rng(7)
+    n = rng.integers(1, 6, 40)
+    off = np.zeros(41, 'i8')
+    np.cumsum(n, out=off[1:])
+    cells = rng.integers(0, 60, off[-1]).astype('i4')
+    num_add = rng.choice([-1, 0, 0, 1, 3, 7], 40).astype('i4')
+    add = rng.integers(0, 60, int(np.maximum(num_add, 0).sum())).astype('i4')
+    ro, rq = proposal.widen_lists_csr(off, cells, num_add, add)
+    a = 0
+    for d in range(40):
+        got = list(rq[ro[d]:ro[d + 1]])
+        own = list(cells[off[d]:off[d + 1]])
+        if num_add[d] < 0:
+            assert got == [-1]
+        else:
+            k = int(num_add[d])
+            assert got == own + list(add[a:a + k])
+            a += k
+    assert a == len(add)
+
+
 def test_stacked_proposal_lists():
     """stack_cells keeps every row side by side (repeats allowed: the engine takes a frame's set
     as the union of its list); refine_rows gives refine_lists_csr's lists for them."""
