@@ -505,7 +505,7 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
         """Judge, proposals, both passes and the model update; returns (seconds, info)."""
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        judge.estep(model_k1, scale_k1, beta_k1, kk + 1, cfg['need_scaling'])
+        judge.estep(recon.engine, scale_k1, beta_k1, kk + 1, cfg['need_scaling'])  # the fine engine's model
         t1 = time.perf_counter()
         tops = sources(source)
         if k_coarse > 0 and source != 'coarse_only':

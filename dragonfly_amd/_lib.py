@@ -66,6 +66,7 @@ def lib():
             'emc_set_background': (C.c_int, [C.c_void_p, _dp, _dp]),
             'emc_set_quats': (C.c_int, [C.c_void_p, C.c_int, _dp]),
             'emc_set_model': (C.c_int, [C.c_void_p, C.c_long, _dp]),
+            'emc_copy_model': (C.c_int, [C.c_void_p, C.c_void_p]),
             'emc_set_frames_sparse': (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _lp, _lp, _ip, _ip, _ip]),
             'emc_set_frames_dense': (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip]),
             'emc_voronoi_subset': (C.c_int, [C.c_int, _dp, C.c_int, _dp, _ip]),
@@ -330,6 +331,10 @@ class Engine:
         out = np.empty(self.num_data)
         _check(lib().emc_judge_outside(self._h, coarse._h, float(prob_floor), _ptr(out)))
         return out
+
+    def copy_model(self, src):
+        """This engine's model1 := src's (same device and mode count), copied on the device."""
+        _check(lib().emc_copy_model(self._h, src._h))
 
     def judge_widen(self, coarse, outside_max, max_add, target=None):
         """judge_outside (prob_floor 0) and, for the frames above outside_max, the fewest coarse

@@ -2032,6 +2032,30 @@ int emc_set_model(emc_engine *e, long size, const double *model1) {
     return 0;
 }
 
+int emc_copy_model(emc_engine *dst, const emc_engine *src) {
+    if (dst) ++dst->gen;
+    if (dst) dst->model32_valid = false, dst->logw_resident = false;  // model1 changes below
+    if (!dst || !src) return fail(EMC_E_ARG, "emc_copy_model: null engine");
+    if (!src->size || !src->model1.p) return fail(EMC_E_STATE, "emc_copy_model: the source holds no model");
+    if (src->device != dst->device || src->num_modes != dst->num_modes)
+        return fail(EMC_E_ARG, "emc_copy_model: engines on different devices or with different mode counts");
+    HIP_TRY(hipSetDevice(dst->device));
+    const long size = src->size;
+    const size_t vol = (size_t)size * size * size * dst->num_modes;
+    int rc;
+    if ((rc = dst->model1.alloc(vol)) || (rc = dst->model2.alloc(vol)) || (rc = dst->weight.alloc(vol))) return rc;
+    // the source's model is complete on its own stream before the copy on the destination's
+    HIP_TRY(hipStreamSynchronize(src->stream));
+    HIP_TRY(hipMemcpyAsync(dst->model1.p, src->model1.p, vol * sizeof(double), hipMemcpyDeviceToDevice, dst->stream));
+    emc::launch_fill(dst->stream, dst->model2.p, vol, 0.);
+    emc::launch_fill(dst->stream, dst->weight.p, vol, 0.);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(dst->stream));
+    if (size != dst->size) dst->weight_all_valid = false, dst->bricks_valid = false;
+    dst->size = size;
+    return 0;
+}
+
 int emc_set_background(emc_engine *e, const double *background, const double *bgscale) {
     if (e) ++e->gen, ++e->layout_gen;
     if (e) e->logw_resident = false;

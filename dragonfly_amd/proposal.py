@@ -279,9 +279,13 @@ class CoarseJudge:
         self.stats = None
 
     def estep(self, model1, scale, beta, iteration, need_scaling):
-        """Coarse posterior of every frame at this model / scale factors / beta."""
+        """Coarse posterior of every frame at this model / scale factors / beta; model1 is a
+        host array or an Engine holding the model (copied on the device)."""
         eng = self.eng
-        eng.set_model(model1)
+        if isinstance(model1, np.ndarray):
+            eng.set_model(model1)
+        else:
+            eng.copy_model(model1)
         eng.set_scale(scale)
         eng.set_beta(beta)
         self.stats = eng.maximize(iteration, need_scaling, 0, 0)

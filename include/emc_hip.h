@@ -127,6 +127,11 @@ int emc_set_quats(emc_engine *e, int num_rot, const double *quats);
 /* model: struct model model1[num_modes][size^3] (dragonfly/src/model.h:15-20,
  *        Model.allocate dragonfly/model.pyx:43-103)                                   */
 int emc_set_model(emc_engine *e, long size, const double *model1);
+/* emc_set_model with src's model1 (same device and mode count), copied on the device: a
+ * second engine over the same frames (the coarse search that judges orientation proposals)
+ * takes the current model without a round trip through the host. No counterpart in the
+ * reference.                                                                            */
+int emc_copy_model(emc_engine *dst, const emc_engine *src);
 /* sparse photon frames: struct dataset ones, multi, place_ones, place_multi, count_multi
  *        (dragonfly/src/emcfile.h:12-32, parse_binarydataset emcfile.c:3-57)          */
 int emc_set_frames_sparse(emc_engine *e, int num_data, const int *ones, const int *multi,

@@ -644,6 +644,23 @@ def test_second_pass_reuses_resident_log_w(tiny_scaled, g32):
         assert err <= (1e-5 if g32 else 1e-13), k
 
 
+def test_copy_model_between_engines(tiny_scaled):
+    """emc_copy_model gives the destination engine the source's model1 bit for bit, and an
+    iteration from it equals one from the same model set from the host."""
+    p = tiny_scaled
+    a = P.engine_for(p)
+    m = a.get('model1')
+    b = P.engine_for(dict(p, model1=np.full_like(p['model1'], 3.)))
+    b.copy_model(a)
+    assert np.array_equal(b.get('model1'), m)
+    ra = P.run_engine(a, dict(iteration=1))
+    rb = P.run_engine(b, dict(iteration=1))
+    for k in ('u', 'max_exp', 'p_norm', 'likelihood', 'prob'):
+        assert np.array_equal(ra[k], rb[k]), k
+    a.close()
+    b.close()
+
+
 def test_judge_outside_estimate(tiny_scaled):
     """emc_judge_outside on the tiny problem (fine num_div 2, coarse num_div 1): the estimate
     equals its definition recomputed from the two engines' results (the coarse posterior,
