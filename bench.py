@@ -11,6 +11,7 @@ from HIP events on the engine's stream, and a CPU baseline (the oracle's OpenMP 
 of the reference engine) timed on a bounded orientation sample.
 """
 import argparse
+import concurrent.futures
 import json
 import os
 import sys
@@ -497,24 +498,31 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
             return [net.propose(X, D, k_top), prev_top]
         if source == 'network_only':
             return [net.propose(X, D, k_top)]
-        if source == 'previous_posterior':
-            return [prev_top]
-        return [judge.top_cells(k_top)]  # coarse_only: the coarse search's own top cells
+        return [prev_top]  # previous_posterior (coarse_only takes the coarse search's own top cells)
+
+    pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
 
     def one_step(source):
-        """Judge, proposals, both passes and the model update; returns (seconds, info)."""
+        """Judge, proposals, both passes and the model update; returns (seconds, info). The
+        coarse search runs on its own engine and stream from a second host thread (taking the
+        fine engine's model on the device) while this one computes the proposals that do not
+        need it (network, previous posterior)."""
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        judge.estep(recon.engine, scale_k1, beta_k1, kk + 1, cfg['need_scaling'])  # the fine engine's model
+        coarse = pool.submit(judge.estep, recon.engine, scale_k1, beta_k1, kk + 1, cfg['need_scaling'])
+        tops = sources(source) if source != 'coarse_only' else []
         t1 = time.perf_counter()
-        tops = sources(source)
-        if k_coarse > 0 and source != 'coarse_only':
-            tops.append(judge.top_cells(k_coarse))
+        coarse.result()
         t2 = time.perf_counter()
+        if source == 'coarse_only':
+            tops = [judge.top_cells(k_top)]
+        elif k_coarse > 0:
+            tops.append(judge.top_cells(k_coarse))
+        t3 = time.perf_counter()
         info = proposal.judged_iteration(recon, itr, judge, tops, cfg['coarse_div'], tau, widen=args.widen)
         torch.cuda.synchronize()
-        info['phase_ms'] = dict(coarse_search=round((t1 - t0) * 1e3, 1), proposals=round((t2 - t1) * 1e3, 1),
-                                **info['phase_ms'])
+        info['phase_ms'] = dict(proposals=round((t1 - t0) * 1e3, 1), coarse_search=round((t2 - t0) * 1e3, 1),
+                                coarse_top=round((t3 - t2) * 1e3, 1), **info['phase_ms'])
         return time.perf_counter() - t0, info
 
     results = {}
@@ -549,6 +557,7 @@ def learned_case(prob, cfg, beta_factor, args, timed_steps, warmup):
                    phase_ms=info['phase_ms'])
         results[source] = res
         log('beta x %g, %s: %s' % (beta_factor, source, json.dumps(res)))
+    pool.shutdown()
     judge_ms = judge.stats['stage_ms'] if judge.stats else {}
     judge.close()
     recon.free()
