@@ -22,20 +22,8 @@ __device__ __forceinline__ size_t mode_stride(const float *, long size) {
     const size_t nb = (size_t)model_bricks(size);
     return nb * nb * nb * kHaloBrickFloats;
 }
-// log(DBL_MIN) = -1022 ln 2: the stored log W of a pixel outside the grid
+// log(DBL_MIN) = -1022 ln 2: the stored log W of a pixel outside the grid (FP32 model copy)
 constexpr float kLogDblMin = (float)(-1022.0 * 0.693147180559945309417232121458);
-// log W as stored: FP64 log for an FP64 model; for the FP32 copy the device's FP32 log of the
-// FP32 interpolation (W >= FLT_MIN), log(DBL_MIN) outside the grid, and the FP64 log of the
-// rest (0 gives -inf)
-template <typename T, typename MT>
-__device__ __forceinline__ T log_stored(double W) {
-    if (sizeof(MT) == sizeof(float)) {
-        if (W >= (double)FLT_MIN) return (T)__logf((float)W);
-        if (W == DBL_MIN) return (T)kLogDblMin;
-        return (T)(float)log(W);
-    }
-    return (T)log(W);
-}
 
 // LDS of one expand block. With the FP32 model copy (MT float) the rotations are FP32 and the
 // transposition tile holds W as FP32 (W = DBL_MIN outside the grid as -inf): 21 KB instead of
@@ -77,16 +65,16 @@ __device__ __forceinline__ void expand_block(
         if (t < t_end) {
             const double q0 = qvals[3 * (size_t)t], q1 = qvals[3 * (size_t)t + 1],
                          q2 = qvals[3 * (size_t)t + 2], cr = corr[t];
-            const float qf0 = (float)q0, qf1 = (float)q1, qf2 = (float)q2;
+            const float qf0 = (float)q0, qf1 = (float)q1, qf2 = (float)q2, crf = (float)cr;
 #pragma unroll 4
             for (int k = 0; k < kRTile / kWavesPerBlock; ++k) {
                 const int rr = wave + kWavesPerBlock * k;
                 // combined orientation index r = rotation * modes + mode (max_prob.c:80-84)
                 const MT *mdl = modes > 1 ? model + (size_t)((r0 + c0 + rr) % modes) * mode_stride(model, size) : model;
                 if constexpr (f32) {
-                    const double v =
-                        slice_value_f32h(mdl, (int)size, nb, &sh.rot[rr * 9], qf0, qf1, qf2, cr, (float)center);
-                    sh.tile[lane][rr] = sh.pad[rr] ? 1.f : (v == DBL_MIN ? -INFINITY : (float)v);
+                    const float v =
+                        slice_value_f32h(mdl, (int)size, nb, &sh.rot[rr * 9], qf0, qf1, qf2, crf, (float)center);
+                    sh.tile[lane][rr] = sh.pad[rr] ? 1.f : v;
                 } else {
                     const double v = slice_value_flat(mdl, size, &sh.rot[rr * 9], q0, q1, q2, cr, center);
                     sh.tile[lane][rr] = sh.pad[rr] ? 1. : v;
@@ -98,18 +86,27 @@ __device__ __forceinline__ void expand_block(
             const int p = wave + kWavesPerBlock * k;
             const int tt = tb + p;
             if (tt < t_end) {
-                const auto Wt = sh.tile[p][lane];
-                const bool outside = f32 && Wt == -INFINITY;  // FP32 tile: outside the grid
-                const double W = outside ? DBL_MIN : (double)Wt;
-                if (mask[tt] < 1) {
-                    uacc += W;
-                    bad |= !(W > 0.);
-                }
                 // tile-major with kW-wide tiles: (c / kW) * num_pix * kW + t * kW + c % kW
                 const int c = c0 + lane;
-                // raw: W itself (the background-aware likelihood takes log(phi W + b B))
-                logw[(size_t)(c / kW) * ((size_t)num_pix * kW) + (size_t)tt * kW + (c % kW)] =
-                    raw ? (T)W : log_stored<T, MT>(W);
+                T *dst = logw + (size_t)(c / kW) * ((size_t)num_pix * kW) + (size_t)tt * kW + (c % kW);
+                if constexpr (f32) {
+                    // FP32 tile: -inf outside the grid (W = DBL_MIN there: not bad, log W = log(DBL_MIN))
+                    const float Wt = sh.tile[p][lane];
+                    const bool outside = Wt == -INFINITY;
+                    if (mask[tt] < 1) {
+                        uacc += outside ? DBL_MIN : (double)Wt;
+                        bad |= !outside && !(Wt > 0.f);
+                    }
+                    *dst = (T)(outside ? kLogDblMin : (Wt >= FLT_MIN ? __logf(Wt) : logf(Wt)));
+                } else {
+                    const double W = sh.tile[p][lane];
+                    if (mask[tt] < 1) {
+                        uacc += W;
+                        bad |= !(W > 0.);
+                    }
+                    // raw: W itself (the background-aware likelihood takes log(phi W + b B))
+                    *dst = raw ? (T)W : (T)log(W);
+                }
             }
         }
         __syncthreads();

@@ -95,13 +95,14 @@ __device__ __forceinline__ long halo_index(int nb, int x, int y, int z) {
     return (((long)(x >> 2) * nb + (y >> 2)) * nb + (z >> 2)) * kHaloBrickFloats + (x & 3) * 25 + (y & 3) * 5 + (z & 3);
 }
 
-// Interpolated value of the FP32 brick copy at the rotated position of a pixel, in FP32: the
-// position R q + center, the truncated cell (toward zero, as model.c:31-33) and its fractions,
-// then the trilinear value as three rounds of linear interpolation (z, y, x), each a fused
-// multiply-add a + f (b - a). rot: the rotation as 9 floats; q: the pixel's q-vector in FP32.
-// DBL_MIN outside the grid (x, y or z outside [0, size - 2]).
-__device__ __forceinline__ double slice_value_f32h(const float *__restrict__ model, int size, int nb, const float *rot,
-                                                   float q0, float q1, float q2, double corr, float center) {
+// Interpolated value of the FP32 brick copy at the rotated position of a pixel times the
+// pixel's corr, all in FP32: the position R q + center, the truncated cell (toward zero, as
+// model.c:31-33) and its fractions, then the trilinear value as three rounds of linear
+// interpolation (z, y, x), each a fused multiply-add a + f (b - a). rot: the rotation as 9
+// floats; q: the pixel's q-vector in FP32. -inf outside the grid (x, y or z outside
+// [0, size - 2]), where the reference takes DBL_MIN.
+__device__ __forceinline__ float slice_value_f32h(const float *__restrict__ model, int size, int nb, const float *rot,
+                                                  float q0, float q1, float q2, float corr, float center) {
     const float p0 = __builtin_fmaf(rot[0], q0, __builtin_fmaf(rot[1], q1, __builtin_fmaf(rot[2], q2, center)));
     const float p1 = __builtin_fmaf(rot[3], q0, __builtin_fmaf(rot[4], q1, __builtin_fmaf(rot[5], q2, center)));
     const float p2 = __builtin_fmaf(rot[6], q0, __builtin_fmaf(rot[7], q1, __builtin_fmaf(rot[8], q2, center)));
@@ -116,7 +117,7 @@ __device__ __forceinline__ double slice_value_f32h(const float *__restrict__ mod
     const float a10 = __builtin_fmaf(fz, m101 - m100, m100), a11 = __builtin_fmaf(fz, m111 - m110, m110);
     const float b0 = __builtin_fmaf(fy, a01 - a00, a00), b1 = __builtin_fmaf(fy, a11 - a10, a10);
     const float v = __builtin_fmaf(fx, b1 - b0, b0);
-    return ok ? (double)v * corr : DBL_MIN;
+    return ok ? v * corr : -INFINITY;
 }
 
 }  // namespace emc
