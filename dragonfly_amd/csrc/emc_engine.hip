@@ -140,13 +140,23 @@ int host_threads(long n, long per) {
 }
 
 // fn(part, i0, i1) over nparts contiguous ranges of [0, n), part 0 on the calling thread.
+// Returns EMC_E_NOMEM (with the message) when a part ran out of host memory, else 0: an
+// exception must not leave a host thread.
 template <typename F>
-void parallel_ranges(long n, int nparts, F &&fn) {
+int parallel_ranges(long n, int nparts, F &&fn) {
+    std::atomic<bool> failed{false};
+    auto run = [&fn, &failed, n, nparts](int pi) {
+        try {
+            fn(pi, n * pi / nparts, n * (pi + 1) / nparts);
+        } catch (const std::exception &) {
+            failed = true;
+        }
+    };
     std::vector<std::thread> pool;
-    for (int pi = 1; pi < nparts; ++pi)
-        pool.emplace_back([&fn, n, nparts, pi] { fn(pi, n * pi / nparts, n * (pi + 1) / nparts); });
-    fn(0, 0L, n / nparts);
+    for (int pi = 1; pi < nparts; ++pi) pool.emplace_back(run, pi);
+    run(0);
     for (auto &th : pool) th.join();
+    return failed ? fail(EMC_E_NOMEM, "host memory ran out while building lists") : 0;
 }
 
 void make_rot(const double *q, double *rot) {
@@ -905,7 +915,7 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
     };
     std::vector<Part> parts(nth);
     std::vector<int> ntl(N);  // tiles per frame, -1 for a full frame
-    parallel_ranges(N, nth, [&](int pi, long d0, long d1) {
+    if ((rc = parallel_ranges(N, nth, [&](int pi, long d0, long d1) {
         Part &pt = parts[pi];
         std::vector<int> tl;
         for (long d = d0; d < d1; ++d) {
@@ -929,7 +939,8 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
             ntl[d] = (int)tl.size();
             pt.tiles.insert(pt.tiles.end(), tl.begin(), tl.end());
         }
-    });
+    })))
+        return rc;
     std::vector<long> &poff = e->h_poff;
     std::vector<int> &ffull = e->h_ffull;
     std::vector<int> fidx;
@@ -949,7 +960,7 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
         return rc;
     int *pframe = e->h_pframe.p, *ptile = e->h_ptile.p, *tpair = e->h_tpair.p, *tframe = e->h_tframe.p;
     pframe[0] = ptile[0] = tpair[0] = tframe[0] = 0;  // one element at least is uploaded
-    parallel_ranges(N, nth, [&](int pi, long d0, long d1) {
+    if ((rc = parallel_ranges(N, nth, [&](int pi, long d0, long d1) {
         Part &pt = parts[pi];
         pt.hist.assign(T, 0);
         size_t i = 0;
@@ -960,7 +971,8 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
                 ptile[k] = t;
                 ++pt.hist[t];
             }
-    });
+    })))
+        return rc;
     std::vector<long> &toff = e->h_toff;
     toff.assign(T + 1, 0);
     for (int t = 0; t < T; ++t) {
@@ -972,14 +984,15 @@ int build_subset(emc_engine *e, int num_coarse, const int *qmap, const long *rel
         }
         toff[t + 1] = at;
     }
-    parallel_ranges(N, nth, [&](int pi, long d0, long d1) {
+    if ((rc = parallel_ranges(N, nth, [&](int pi, long d0, long d1) {
         std::vector<long> &at = parts[pi].hist;
         for (long k = poff[d0]; k < poff[d1]; ++k) {
             const long j = at[ptile[k]]++;
             tpair[j] = (int)k;
             tframe[j] = pframe[k];  // the frame beside the pair: the pair tomogram reads it with it
         }
-    });
+    })))
+        return rc;
     const size_t Ku = (size_t)std::max(K, 1L);
     if ((rc = e->pair_off.upload(poff.data(), poff.size(), s))) return rc;
     if ((rc = e->pair_frame.upload(pframe, Ku, s))) return rc;
@@ -2188,7 +2201,7 @@ int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const
     uint32_t *bits = e->h_relbits.p;
     const int nth = host_threads(N, 2048);
     std::vector<long> bad(nth, -1);  // first entry out of range in each range of frames
-    parallel_ranges(N, nth, [&](int pi, long d0, long d1) {
+    if ((rc = parallel_ranges(N, nth, [&](int pi, long d0, long d1) {
         std::memset(bits + (size_t)d0 * words, 0, (size_t)(d1 - d0) * words * sizeof(uint32_t));
         for (long d = d0; d < d1; ++d) {
             uint32_t *row = bits + (size_t)d * words;
@@ -2207,7 +2220,8 @@ int emc_set_refine(emc_engine *e, int num_coarse, const int *quat_mapping, const
                 row[c >> 5] |= 1u << (c & 31);
             }
         }
-    });
+    })))
+        return rc;
     for (long k : bad)
         if (k >= 0) return fail(EMC_E_ARG, "emc_set_refine: rel_quat %d out of range", rel_quat[k]);
     if ((rc = e->qmap.upload(qm.data(), qm.size(), e->stream))) return rc;
