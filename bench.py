@@ -76,9 +76,9 @@ def cpu_model_name():
     return 'unknown'
 
 
-def cpu_baseline(prob, itr, threads, background=None, target_rot=(12800, 25600), frames=512):
+def cpu_baseline(prob, itr, threads, background=None, target_rot=(12800, 25600, 51200), frames=512):
     """Iteration 1 of the oracle engine (OpenMP restatement of the reference engine) on
-    `frames` frames and two orientation subsets (every k-th orientation, about target_rot
+    `frames` frames and three orientation subsets (every k-th orientation, about target_rot
     orientations), extrapolated to the workload with the reference engine's cost structure:
 
         T(N, n) = alpha N + n (beta N + gamma N^2) + c
@@ -116,18 +116,26 @@ def cpu_baseline(prob, itr, threads, background=None, target_rot=(12800, 25600),
                         per_frame_s=round((1. - share) * loops + sec[2], 3),
                         other_s=round(sec[4] - loops - sec[2], 3)))
         log('cpu baseline: %d frames x %d orientations, %d threads: %.2f s (%s)' % (n, len(quats), threads, sec[4], pts[-1]))
-    (N1, N2) = (pts[0]['orientations'], pts[1]['orientations'])
-    alpha = 0.5 * (pts[0]['per_orientation_s'] / N1 + pts[1]['per_orientation_s'] / N2)
-    b1, b2 = pts[0]['per_frame_s'] / n, pts[1]['per_frame_s'] / n
-    gamma = max((b2 / N2 - b1 / N1) / (N2 - N1), 0.)
-    beta = max(b1 / N1 - gamma * N1, 0.)
-    c = 0.5 * (pts[0]['other_s'] + pts[1]['other_s'])
+    # per-orientation work: alpha = mean of its share per orientation; per-frame work per
+    # orientation b / N = beta + gamma N, least squares over the points with beta, gamma >= 0
+    Ns = np.array([p_['orientations'] for p_ in pts], 'f8')
+    alpha = float(np.mean([p_['per_orientation_s'] for p_ in pts] / Ns))
+    y = np.array([p_['per_frame_s'] / n for p_ in pts]) / Ns
+    A = np.stack([np.ones_like(Ns), Ns], axis=1)
+    beta, gamma = np.linalg.lstsq(A, y, rcond=None)[0]
+    if gamma < 0.:
+        beta, gamma = float(y.mean()), 0.
+    elif beta < 0.:
+        beta, gamma = 0., float((Ns * y).sum() / (Ns * Ns).sum())
+    beta, gamma = float(beta), float(gamma)
+    c = float(np.mean([p_['other_s'] for p_ in pts]))
     t_full = alpha * R + D * (beta * R + gamma * R * R) + c
     sample = ('iteration 1 of the oracle engine (OpenMP restatement of the reference, %d threads) on %d frames at '
-              '%d and %d of the %d orientations (%.1f s, %.1f s); fit T = alpha N + n (beta N + gamma N^2) + c '
-              'from each run\'s per-orientation / per-frame split: alpha = %.3g s, beta = %.3g s, gamma = %.3g s, '
-              'c = %.2f s; extrapolated to %d frames x %d orientations: %.0f s'
-              % (threads, n, N1, N2, R, pts[0]['seconds'], pts[1]['seconds'], alpha, beta, gamma, c, D, R, t_full))
+              '%s of the %d orientations (%s s); fit T = alpha N + n (beta N + gamma N^2) + c from each run\'s '
+              'per-orientation / per-frame split (least squares, beta and gamma >= 0): alpha = %.3g s, beta = %.3g s, '
+              'gamma = %.3g s, c = %.2f s; extrapolated to %d frames x %d orientations: %.0f s'
+              % (threads, n, ', '.join('%d' % N for N in Ns), R, ', '.join('%.1f' % p_['seconds'] for p_ in pts),
+                 alpha, beta, gamma, c, D, R, t_full))
     fit = dict(points=pts, alpha_s=float('%.4g' % alpha), beta_s=float('%.4g' % beta),
                gamma_s=float('%.4g' % gamma), c_s=round(c, 3), extrapolated_s=round(t_full, 1))
     return D * R / t_full, sample, fit
