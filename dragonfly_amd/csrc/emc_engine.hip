@@ -207,6 +207,7 @@ constexpr int kPixPerGroup = 1024;
 constexpr int kFramesPerPsumPart = 512;
 constexpr int kFramesPerTomoPart = 12500;  // default of the full_part_frames option
 constexpr int kSigFullRowsPerTile = 1024;  // full-frame rows read by the pair tomogram, mean per tile
+constexpr int kMaxBucketRanges = 8192;     // pixel ranges of the pair tomogram's buckets (LDS counters)
 constexpr int kModelParts = 1024;
 
 // Staging memory of emc_slice_gen_static: the rotation matrix at [0, 9) and the device copy
@@ -380,6 +381,7 @@ struct emc_engine {
     int full_part_frames = kFramesPerTomoPart;  // frames per range of the full-frame tomogram
     int sig_full_rows = kSigFullRowsPerTile;  // full-frame rows per tile the pair tomogram takes
     int unit_items = emc::kUnitItems;         // brick merge: list entries per work unit (block)
+    int tomo_buckets = 1;                     // pair tomogram over per-(tile, range) buckets
     int estep_only = 0;   // stop after the posterior: no tomogram, merge or scale update
     int mstep_only = 0;   // subset path: resume after an estep_only iteration with the M-step
     int graph_mode = 0;   // subset path: run the iteration as a captured HIP graph
@@ -447,6 +449,10 @@ struct emc_engine {
     DevBuf<long> fsort_off;
     DevBuf<int> fsort_pix, fsort_roff;
     DevBuf<double> fsort_cnt;
+    // each frame's distinct pixel ranges (fsort order): fr_off per frame, the range and its
+    // photons [fr_q0, fr_q1) per entry; and the pair tomogram's per-(tile, range) buckets
+    DevBuf<long> fr_off, fr_q0, fr_q1, bk_tot, bk_base;
+    DevBuf<int> fr_rg, bk_loc, bk_pair, bk_fr;
 
 };
 
@@ -1060,8 +1066,8 @@ int build_fsort(emc_engine *e) {
     const int N = e->num_data, P = e->num_pix;
     const int nr = (P + emc::kPairRange - 1) / emc::kPairRange;
     const uint8_t *mask = e->h_mask.data();
-    std::vector<long> off(N + 1, 0);
-    std::vector<int> pix, roff((size_t)N * (nr + 1));
+    std::vector<long> off(N + 1, 0), r_off(N + 1, 0), r_q0, r_q1;
+    std::vector<int> pix, roff((size_t)N * (nr + 1)), r_rg;
     std::vector<double> cnt;
     std::vector<std::pair<int, double>> fr;
     for (int d = 0; d < N; ++d) {
@@ -1081,11 +1087,28 @@ int build_fsort(emc_engine *e) {
             while (q < (int)fr.size() && fr[q].first < rg * emc::kPairRange) ++q;
             roff[(size_t)d * (nr + 1) + rg] = q;
         }
+        // the frame's distinct ranges and their photons
+        const long base = (long)pix.size();
+        for (size_t a = 0; a < fr.size();) {
+            const int rg = fr[a].first / emc::kPairRange;
+            size_t b = a + 1;
+            while (b < fr.size() && fr[b].first / emc::kPairRange == rg) ++b;
+            r_rg.push_back(rg);
+            r_q0.push_back(base + (long)a);
+            r_q1.push_back(base + (long)b);
+            a = b;
+        }
+        r_off[d + 1] = (long)r_rg.size();
         for (const auto &x : fr) {
             pix.push_back(x.first);
             cnt.push_back(x.second);
         }
         off[d + 1] = (long)pix.size();
+    }
+    if (r_rg.empty()) {
+        r_rg.push_back(0);
+        r_q0.push_back(0);
+        r_q1.push_back(0);
     }
     if (pix.empty()) {
         pix.push_back(0);
@@ -1097,6 +1120,9 @@ int build_fsort(emc_engine *e) {
     if ((rc = e->fsort_pix.upload(pix.data(), pix.size(), s))) return rc;
     if ((rc = e->fsort_cnt.upload(cnt.data(), cnt.size(), s))) return rc;
     if ((rc = e->fsort_roff.upload(roff.data(), roff.size(), s))) return rc;
+    if ((rc = e->fr_off.upload(r_off.data(), r_off.size(), s)) || (rc = e->fr_rg.upload(r_rg.data(), r_rg.size(), s)) ||
+        (rc = e->fr_q0.upload(r_q0.data(), r_q0.size(), s)) || (rc = e->fr_q1.upload(r_q1.data(), r_q1.size(), s)))
+        return rc;
     HIP_TRY(hipStreamSynchronize(s));
     e->nranges = nr;
     e->fsort_valid = true;
@@ -1611,6 +1637,38 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
         const bool parts = nF > 0 && !sig_full;
         e->nz_counted = false;
         emc::launch_zero_u64(s, e->nz_count.p);
+        // per-(tile, range) buckets of the significant pairs' photons (eager only: their size is
+        // read back; without room for them the blocks walk their tiles' pairs instead)
+        emc::PairBuckets bkv;
+        if (e->tomo_buckets && !device_weights && !sig_full && e->nranges <= kMaxBucketRanges) {
+            const int nr = e->nranges;
+            if ((rc = timed(7, [&]() -> int {
+                     int r2;
+                     if ((r2 = e->bk_loc.alloc((size_t)T * (nr + 1))) || (r2 = e->bk_tot.alloc((size_t)T)) ||
+                         (r2 = e->bk_base.alloc((size_t)T + 1)))
+                         return r2;
+                     emc::launch_pair_buckets(s, T, e->tile_off.p, e->sig_pair.p, e->sig_frame.p, e->sig_cnt.p,
+                                              e->fr_off.p, e->fr_rg.p, nr, e->bk_loc.p, e->bk_tot.p, e->bk_base.p,
+                                              nullptr, nullptr, 0);
+                     HIP_TRY(hipGetLastError());
+                     long total = 0;
+                     HIP_TRY(hipMemcpyAsync(&total, e->bk_base.p + T, sizeof(long), hipMemcpyDeviceToHost, s));
+                     HIP_TRY(hipStreamSynchronize(s));
+                     const size_t n = (size_t)std::max(total, 1L);
+                     if (e->bk_pair.alloc(n) || e->bk_fr.alloc(n)) {
+                         e->bk_pair.release();  // no room: the blocks walk their tiles' pairs
+                         e->bk_fr.release();
+                         return 0;
+                     }
+                     emc::launch_pair_buckets(s, T, e->tile_off.p, e->sig_pair.p, e->sig_frame.p, e->sig_cnt.p,
+                                              e->fr_off.p, e->fr_rg.p, nr, e->bk_loc.p, e->bk_tot.p, e->bk_base.p,
+                                              e->bk_pair.p, e->bk_fr.p, 1);
+                     HIP_TRY(hipGetLastError());
+                     bkv = emc::PairBuckets{e->bk_base.p, e->bk_loc.p, e->bk_pair.p, e->bk_fr.p, e->fr_q0.p, e->fr_q1.p};
+                     return 0;
+                 })))
+                return rc;
+        }
         for (size_t k = 0; k < e->slot_bricks.size(); ++k) {
             const auto &bc = e->slot_bricks[k];
             const int t0 = bc.r0 / emc::kRTile, t1 = (bc.r0 + bc.Cc) / emc::kRTile;
@@ -1638,13 +1696,13 @@ int maximize_subset(emc_engine *e, const emc_iter_opts *opts, emc_iter_stats *st
                                                    e->sig_cnt.p, e->fsort_off.p, e->fsort_roff.p, e->nranges,
                                                    e->fsort_pix.p, e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
                                                    parts ? e->tom_partial32.p : nullptr, nparts, e->logw32.p, nz,
-                                                   frows);
+                                                   frows, bkv);
                      else
                          emc::launch_pair_tomogram(s, e->Rsub.p, e->tile_off.p, e->sig_pair.p, e->sig_frame.p,
                                                    e->sig_cnt.p, e->fsort_off.p, e->fsort_roff.p, e->nranges,
                                                    e->fsort_pix.p, e->fsort_cnt.p, e->psum_slot.p, t0, t1 - t0, P,
                                                    parts ? e->tom_partial.p : nullptr, nparts, e->logw.p, nz,
-                                                   frows);
+                                                   frows, bkv);
                      return 0;
                  })))
                 return rc;
@@ -1804,6 +1862,7 @@ const EngineOption kOptions[] = {
     {"fused", &emc_engine::fused, 0, 1, false},
     {"sig_full_rows", &emc_engine::sig_full_rows, 0, 1 << 20, false},
     {"unit_items", &emc_engine::unit_items, 64, 1 << 20, false},
+    {"tomo_buckets", &emc_engine::tomo_buckets, 0, 1, false},
 };
 
 int set_option(emc_engine *e, const EngineOption &o, int value) {

@@ -244,6 +244,19 @@ void launch_pair_posterior(hipStream_t s, double *R, const long *pair_off, const
 void launch_pair_psum(hipStream_t s, const double *P, const long *tile_off, const int *tile_pair,
                       const int *pair_frame, int num_tiles, const double *scale, int need_scaling,
                       const double *psum_full, double *psum_slot);
+// Per-(tile, range) buckets of the significant pairs' frame ranges (launch_pair_buckets), read
+// by the pair tomogram instead of walking each tile's pairs; base null: none.
+struct PairBuckets {
+    const long *base = nullptr;  // [num_tiles + 1] first entry of each tile
+    const int *loc = nullptr;    // [num_tiles][nranges + 1] offsets of the ranges in their tile
+    const int *pair = nullptr, *fr = nullptr;  // entries: pair, frame range
+    const long *q0 = nullptr, *q1 = nullptr;   // frame range f: photons [q0[f], q1[f])
+};
+// pass 0: counts into loc (and each tile's total), then base; pass 1: the entries
+// (bk_pair / bk_fr sized base[num_tiles]). Dynamic LDS of (nr + 1) ints per block.
+void launch_pair_buckets(hipStream_t s, int num_tiles, const long *tile_off, const int *sig_pair,
+                         const int *sig_frame, const int *sig_cnt, const long *fr_off, const int *fr_rg, int nr,
+                         int *loc, long *tile_tot, long *base, int *bk_pair, int *bk_fr, int pass);
 // *x = 0; *total += set bits of m[0 .. n)
 void launch_zero_u64(hipStream_t s, unsigned long long *x);
 void launch_popcount(hipStream_t s, const unsigned long long *m, size_t n, unsigned long long *total);
@@ -265,12 +278,13 @@ void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, 
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
                           int num_pix, const double *partial, int nparts, double *view,
-                          unsigned long long *nzmask, const SigFullRows &fr = SigFullRows());
+                          unsigned long long *nzmask, const SigFullRows &fr = SigFullRows(),
+                          const PairBuckets &bk = PairBuckets());
 void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *sig_pair,
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
                           int num_pix, const float *partial, int nparts, float *view, unsigned long long *nzmask,
-                          const SigFullRows &fr = SigFullRows());
+                          const SigFullRows &fr = SigFullRows(), const PairBuckets &bk = PairBuckets());
 // pairs with some P > 1e-6 in their row, per tile in the tile's order (flag: num_pairs bytes;
 // sig_pair / sig_frame: tile_off layout; sig_cnt: per tile)
 void launch_sig_pairs(hipStream_t s, const double *P, long num_pairs, const long *tile_off, const int *tile_pair,

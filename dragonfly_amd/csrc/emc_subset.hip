@@ -434,6 +434,120 @@ __global__ __launch_bounds__(kBlock) void sum_counts_kernel(const int *__restric
     if (threadIdx.x == 0) *total = s[0];
 }
 
+// Per-(tile, pixel range) buckets of the significant pairs' photons, so that a pair-tomogram
+// block reads only the entries of its own range instead of every pair of its tile: an entry
+// is (pair, frame range f), the frame's photons [fr_q0[f], fr_q1[f]) in range fr_rg[f] (the
+// frame's distinct ranges, fsort order). Count: one block per tile, LDS counters per range
+// (nr + 1 of them, dynamic LDS), then their exclusive scan: loc[T * (nr + 1) + rg] and the
+// tile's total.
+__global__ __launch_bounds__(kBlock) void bucket_count_kernel(const long *__restrict__ tile_off,
+                                                              const int *__restrict__ sig_frame,
+                                                              const int *__restrict__ sig_cnt,
+                                                              const long *__restrict__ fr_off,
+                                                              const int *__restrict__ fr_rg, int nr,
+                                                              int *__restrict__ loc, long *__restrict__ tile_tot) {
+    extern __shared__ int cnt[];
+    __shared__ int part[kBlock];
+    const int T = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int i = tid; i <= nr; i += kBlock) cnt[i] = 0;
+    __syncthreads();
+    const long j0 = tile_off[T];
+    const int n = sig_cnt[T];
+    for (int i = wave; i < n; i += kWavesPerBlock) {
+        const int d = sig_frame[j0 + i];
+        for (long f = fr_off[d] + lane; f < fr_off[d + 1]; f += 64) atomicAdd(&cnt[fr_rg[f]], 1);
+    }
+    __syncthreads();
+    // exclusive scan of cnt[0 .. nr): each thread a chunk, then the chunks' totals
+    const int per = (nr + kBlock - 1) / kBlock, a = tid * per, b = min(nr, a + per);
+    int sum = 0;
+    for (int i = a; i < b; ++i) sum += cnt[i];
+    part[tid] = sum;
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int w = 0; w < kBlock; ++w) {
+            const int v = part[w];
+            part[w] = run;
+            run += v;
+        }
+        tile_tot[T] = run;
+        loc[(size_t)T * (nr + 1) + nr] = run;
+    }
+    __syncthreads();
+    int run = part[tid];
+    for (int i = a; i < b; ++i) {
+        const int v = cnt[i];
+        loc[(size_t)T * (nr + 1) + i] = run;
+        run += v;
+    }
+}
+
+// base[0] = 0, base[t + 1] = base[t] + tot[t] (one block)
+__global__ __launch_bounds__(kBlock) void tile_scan_kernel(const long *__restrict__ tot, int n, long *__restrict__ base) {
+    __shared__ long part[kBlock];
+    const int tid = threadIdx.x, per = (n + kBlock - 1) / kBlock, a = tid * per, b = min(n, a + per);
+    long sum = 0;
+    for (int i = a; i < b; ++i) sum += tot[i];
+    part[tid] = sum;
+    __syncthreads();
+    if (tid == 0) {
+        long run = 0;
+        for (int w = 0; w < kBlock; ++w) {
+            const long v = part[w];
+            part[w] = run;
+            run += v;
+        }
+        base[n] = run;
+    }
+    __syncthreads();
+    long run = part[tid];
+    for (int i = a; i < b; ++i) {
+        base[i] = run;
+        run += tot[i];
+    }
+}
+
+// Fill: the same walk, each entry at base[T] + loc[T][rg] + (its rank, from LDS counters)
+__global__ __launch_bounds__(kBlock) void bucket_fill_kernel(const long *__restrict__ tile_off,
+                                                             const int *__restrict__ sig_pair,
+                                                             const int *__restrict__ sig_frame,
+                                                             const int *__restrict__ sig_cnt,
+                                                             const long *__restrict__ fr_off,
+                                                             const int *__restrict__ fr_rg, int nr,
+                                                             const int *__restrict__ loc, const long *__restrict__ base,
+                                                             int *__restrict__ bk_pair, int *__restrict__ bk_fr) {
+    extern __shared__ int fill[];
+    const int T = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int i = tid; i < nr; i += kBlock) fill[i] = loc[(size_t)T * (nr + 1) + i];
+    __syncthreads();
+    const long j0 = tile_off[T], b0 = base[T];
+    const int n = sig_cnt[T];
+    for (int i = wave; i < n; i += kWavesPerBlock) {
+        const int k = sig_pair[j0 + i], d = sig_frame[j0 + i];
+        for (long f = fr_off[d] + lane; f < fr_off[d + 1]; f += 64) {
+            const long at = b0 + atomicAdd(&fill[fr_rg[f]], 1);
+            bk_pair[at] = k;
+            bk_fr[at] = (int)f;
+        }
+    }
+}
+
+void launch_pair_buckets(hipStream_t s, int num_tiles, const long *tile_off, const int *sig_pair,
+                         const int *sig_frame, const int *sig_cnt, const long *fr_off, const int *fr_rg, int nr,
+                         int *loc, long *tile_tot, long *base, int *bk_pair, int *bk_fr, int pass) {
+    if (num_tiles <= 0) return;
+    const size_t lds = (size_t)(nr + 1) * sizeof(int);
+    if (pass == 0) {
+        bucket_count_kernel<<<num_tiles, kBlock, lds, s>>>(tile_off, sig_frame, sig_cnt, fr_off, fr_rg, nr, loc,
+                                                          tile_tot);
+        tile_scan_kernel<<<1, kBlock, 0, s>>>(tile_tot, num_tiles, base);
+    } else {
+        bucket_fill_kernel<<<num_tiles, kBlock, lds, s>>>(tile_off, sig_pair, sig_frame, sig_cnt, fr_off, fr_rg, nr,
+                                                         loc, base, bk_pair, bk_fr);
+    }
+}
+
 // Tomogram of the chunk's tiles [t0, t0 + ntiles) over pixel ranges of kPairRange pixels:
 // block (tile, range) starts from the full frames' partial sums (tomogram_part_kernel
 // layout partial[x][tile][t][64], added over the nparts frame ranges in order) or from zero,
@@ -452,7 +566,8 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
     const double *__restrict__ psum_slot, int t0, int ntiles, int num_pix,
     const PT *__restrict__ partial, int nparts, VT *__restrict__ view,
     unsigned long long *__restrict__ nzmask, const double *__restrict__ RF, int nF,
-    const int *__restrict__ sigF_row, const int *__restrict__ sigF_frame, const int *__restrict__ sigF_cnt) {
+    const int *__restrict__ sigF_row, const int *__restrict__ sigF_frame, const int *__restrict__ sigF_cnt,
+    const PairBuckets bk) {
     static_assert(kPairRange == 64, "a pixel range is one 64-pixel merge patch");
     // LDS sums in the view's type: FP64 for an FP64 view, FP32 for an FP32 one (half the LDS,
     // more resident blocks; each sum has a few terms and is stored in FP32 anyway)
@@ -478,7 +593,18 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
         if (nzmask && tid < kRTile) nzmask[((size_t)lt * kRTile + tid) * npatch + rg] = 0ull;
         return;
     }
-    const long nP = sig_cnt[T], nRows = nP + (rows_full ? sigF_cnt[T] : 0);
+    // with buckets: this (tile, range)'s entries [e0, e0 + nRows); else every significant row
+    // of the tile, filtered below by its photons in the range
+    const bool buckets = bk.base != nullptr;
+    long e0 = 0, nP, nRows;
+    if (buckets) {
+        const int *loc = bk.loc + (size_t)T * (nranges + 1);
+        e0 = bk.base[T] + loc[rg];
+        nP = nRows = loc[rg + 1] - loc[rg];
+    } else {
+        nP = sig_cnt[T];
+        nRows = nP + (rows_full ? sigF_cnt[T] : 0);
+    }
     const size_t frow0 = (size_t)T * nF;
     const int p0 = rg * kPairRange;
     if (partial) {
@@ -499,7 +625,17 @@ __global__ __launch_bounds__(kBlock) void pair_tomogram_kernel(
         if (tid == 0) s_n = 0;
         __syncthreads();
         const long j = jb + tid;
-        if (j < nRows) {
+        if (buckets) {
+            if (j < nRows) {
+                // every entry has photons in the range: its queue position is its own
+                const int f = bk.fr[e0 + j];
+                s_row[tid] = (long)bk.pair[e0 + j] * kRTile;
+                s_full[tid] = false;
+                s_q0[tid] = bk.q0[f];
+                s_q1[tid] = bk.q1[f];
+            }
+            if (tid == 0) s_n = (int)min((long)kBlock, nRows - jb);
+        } else if (j < nRows) {
             long row;
             int d;
             const bool full = j >= nP;
@@ -891,31 +1027,31 @@ static void pair_tomogram_launch(hipStream_t s, const TP *P, const long *tile_of
                                  const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff,
                                  int nranges, const int *fpix, const double *fcnt, const double *psum_slot, int t0,
                                  int ntiles, int num_pix, const PT *partial, int nparts, VT *view,
-                                 unsigned long long *nzmask, const SigFullRows &fr) {
+                                 unsigned long long *nzmask, const SigFullRows &fr, const PairBuckets &bk) {
     if (ntiles <= 0) return;
     const unsigned grid = (unsigned)((ntiles + kNumXcd - 1) / kNumXcd * kNumXcd) * (unsigned)nranges;
     pair_tomogram_kernel<TP, VT, PT><<<grid, kBlock, 0, s>>>(P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff,
                                                              nranges, fpix, fcnt, psum_slot, t0, ntiles, num_pix,
                                                              partial, nparts, view, nzmask, fr.RF, fr.nF, fr.row,
-                                                             fr.frame, fr.cnt);
+                                                             fr.frame, fr.cnt, bk);
 }
 
 void launch_pair_tomogram(hipStream_t s, const double *P, const long *tile_off, const int *sig_pair,
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
                           int num_pix, const double *partial, int nparts, double *view,
-                          unsigned long long *nzmask, const SigFullRows &fr) {
+                          unsigned long long *nzmask, const SigFullRows &fr, const PairBuckets &bk) {
     pair_tomogram_launch(s, P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff, nranges, fpix, fcnt, psum_slot,
-                         t0, ntiles, num_pix, partial, nparts, view, nzmask, fr);
+                         t0, ntiles, num_pix, partial, nparts, view, nzmask, fr, bk);
 }
 
 void launch_pair_tomogram(hipStream_t s, const float *P, const long *tile_off, const int *sig_pair,
                           const int *sig_frame, const int *sig_cnt, const long *foff, const int *roff, int nranges,
                           const int *fpix, const double *fcnt, const double *psum_slot, int t0, int ntiles,
                           int num_pix, const float *partial, int nparts, float *view,
-                          unsigned long long *nzmask, const SigFullRows &fr) {
+                          unsigned long long *nzmask, const SigFullRows &fr, const PairBuckets &bk) {
     pair_tomogram_launch(s, P, tile_off, sig_pair, sig_frame, sig_cnt, foff, roff, nranges, fpix, fcnt, psum_slot,
-                         t0, ntiles, num_pix, partial, nparts, view, nzmask, fr);
+                         t0, ntiles, num_pix, partial, nparts, view, nzmask, fr, bk);
 }
 
 template <typename TP>
