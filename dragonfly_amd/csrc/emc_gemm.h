@@ -81,8 +81,10 @@ __device__ __forceinline__ GemmTile gemm_block_tile(int m_tiles, int n_tiles) {
 // Accumulates the block's tile into acc (zeroed here). smem holds
 // gemm_lds_elems<T>(NB, BK, DB ? 2 : 1); K advances BK elements per step. With DB the steps
 // alternate between two LDS stages: step k's fragments are read from one while step k + 1's
-// operands are stored to the other, one barrier per step.
-template <typename T, int NB, int BK = kGemmBK, bool DB = false>
+// operands are stored to the other, one barrier per step. With PF the fragments of the next
+// 16 elements of K are read from LDS while the current ones feed the MFMAs, the reads spread
+// between the MFMAs (scheduling groups: 0x100 LDS read, 0x008 MFMA).
+template <typename T, int NB, int BK = kGemmBK, bool DB = false, bool PF = false>
 __device__ __forceinline__ void gemm_mainloop(const T *__restrict__ A, size_t lda,
                                               const T *__restrict__ B0, const T *__restrict__ B1,
                                               size_t ldb, int K, size_t m0, size_t n0, T *smem,
@@ -130,7 +132,52 @@ __device__ __forceinline__ void gemm_mainloop(const T *__restrict__ A, size_t ld
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
 
     const int frag_row = lane & 31, frag_k = (lane >> 5) * 8;
+    constexpr int kReads = 2 + 4 * NB, kMfmas = 8 * NB, kPerRead = kMfmas / kReads;
+    auto compute_pf = [&](const T *sA) {
+        const T *sB = sA + kGemmBM * kRow;
+        vec8 fa[2][2], fb[2][NB][4];
+        auto read = [&](int kk, int buf) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                fa[buf][i] = *reinterpret_cast<const vec8 *>(sA + (wm * 64 + i * 32 + frag_row) * kRow + kk + frag_k);
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    fb[buf][b][j] = *reinterpret_cast<const vec8 *>(
+                        sB + (b * kGemmBN + wn * 128 + j * 32 + frag_row) * kRow + kk + frag_k);
+        };
+        read(0, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, kReads, 0);
+#pragma unroll
+        for (int st = 0; st < BK / 16; ++st) {
+            const int cur = st & 1;
+            const bool next = st + 1 < BK / 16;
+            if (next) read((st + 1) * 16, cur ^ 1);
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::run(fa[cur][i], fb[cur][b][j], acc[i][j]);
+            if (next) {
+#pragma unroll
+                for (int r = 0; r < kReads; ++r) {
+                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x008, kPerRead, 0);
+                }
+                if (kMfmas > kPerRead * kReads)
+                    __builtin_amdgcn_sched_group_barrier(0x008, kMfmas - kPerRead * kReads, 0);
+            } else {
+                __builtin_amdgcn_sched_group_barrier(0x008, kMfmas, 0);
+            }
+        }
+    };
     auto compute = [&](const T *sA) {
+        if constexpr (PF) {
+            compute_pf(sA);
+            return;
+        }
         const T *sB = sA + kGemmBM * kRow;
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {

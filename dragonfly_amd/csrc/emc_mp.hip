@@ -56,8 +56,9 @@ __global__ __launch_bounds__(kGemmThreads) void mp_gemm_kernel(
     const GemmTile tile = gemm_block_tile(m_tiles, n_tiles);
     if (!tile.valid) return;
     floatx16 acc[2][4];
-    // one-term products double-buffer LDS (2 x 72 KiB); two-term ones do not fit twice
-    gemm_mainloop<T, NB, kMpGemmBK, NB == 1>(A, lda, B0, B1, ldb, K, tile.m0, tile.n0, smem, acc);
+    // one-term products double-buffer LDS (2 x 72 KiB) and their fragment reads (the two-term
+    // ones fit neither: LDS, registers)
+    gemm_mainloop<T, NB, kMpGemmBK, NB == 1, NB == 1>(A, lda, B0, B1, ldb, K, tile.m0, tile.n0, smem, acc);
     gemm_for_each(tile.m0, tile.n0, acc, [&](size_t m, size_t n, float a) {
         if (m >= (size_t)M_valid || n >= (size_t)N_valid) return;
         const double x = (double)a;
