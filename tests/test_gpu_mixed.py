@@ -170,12 +170,20 @@ def test_fp32_gathers_vs_oracle(scaled):
     assert e['avg_likelihood'] <= 1e-6 and e['mutual_info'] <= 1e-5
 
 
-def test_fp32_gathers_refinement_subsets():
+@pytest.mark.parametrize('variant', ['default', 'one-chunk', 'no-buckets', 'full-parts', 'full-direct',
+                                     'small-units'])
+def test_fp32_gathers_refinement_subsets(variant, monkeypatch):
     """Precision 3 on the sparse subset path: partial frames (10 of 60 cells), full frames
-    (every cell) and an empty set, in 128-slot chunks, against the oracle's refinement branch
-    (tolerances of the FP32 gathers above)."""
+    (every cell) and an empty set, in 128-slot chunks (or one), against the oracle's refinement
+    branch (tolerances of the FP32 gathers above); also with each kernel variant of the sparse
+    M-step: the pair tomogram walking its tiles' pairs instead of per-(tile, range) buckets,
+    the full frames through partial views or read directly, merge units of 64 entries."""
     import os
     from oracle import core
+    env = {'no-buckets': {'EMC_TOMO_BUCKETS': '0'}, 'full-parts': {'EMC_SIG_FULL_ROWS': '0'},
+           'full-direct': {'EMC_SIG_FULL_ROWS': str(1 << 20)}, 'small-units': {'EMC_UNIT_ITEMS': '64'}}
+    for k, v in env.get(variant, {}).items():
+        monkeypatch.setenv(k, v)
     from dragonfly_amd import _lib
     cfg = P.tiny_config(num_data=120, mean_count=200.)
     p = P.build(cfg)
@@ -193,18 +201,19 @@ def test_fp32_gathers_refinement_subsets():
     fp = dict(p['fparams'], scale=np.ones(D))
     o = core.maximize(p['det'], p['quats'], p['model1'], p['frames'], fp, flags, dense_prob=True, refine=ref)
     o['scale'] = fp['scale']
-    os.environ['EMC_MAX_CHUNK_COLS'] = '128'
+    if variant != 'one-chunk':
+        os.environ['EMC_MAX_CHUNK_COLS'] = '128'
     try:
         eng = P.engine_for(p)
     finally:
-        del os.environ['EMC_MAX_CHUNK_COLS']
+        os.environ.pop('EMC_MAX_CHUNK_COLS', None)
     eng.set_precision(3)
     eng.set_refine(len(qc), qmap, off, ref['rel_quat'])
     g = P.run_engine(eng, flags)
     eng.close()
     e = _errs(g, o)
     print('fp32 gathers, refinement subsets', {k: '%.2e' % v for k, v in e.items()}, flush=True)
-    assert g['num_chunks'] > 1
+    assert (g['num_chunks'] == 1) if variant == 'one-chunk' else (g['num_chunks'] > 1)
     assert g['rmax'][4] == -1
     assert e['u'] <= 1e-6 and e['inter_weight'] <= 1e-12
     assert e['model2'] <= 1e-4

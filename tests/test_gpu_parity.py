@@ -239,19 +239,33 @@ def test_dense_integer_frames(tiny, zero_block):
     _compare(o, g, 'dense zero_block=%s' % zero_block)
 
 
-@pytest.mark.parametrize('form', ['subset', 'subset-chunked', 'subset-padded', 'masked'])
+# kernel-variant options of the subset path's M-step (start values from EMC_<NAME>)
+SUBSET_VARIANTS = {
+    'no-buckets': {'EMC_TOMO_BUCKETS': '0'},                # pair tomogram walks its tiles' pairs
+    'full-parts': {'EMC_SIG_FULL_ROWS': '0'},               # full frames through partial views
+    'full-direct': {'EMC_SIG_FULL_ROWS': str(1 << 20)},     # full frames' rows read directly
+    'full-parts-no-buckets': {'EMC_SIG_FULL_ROWS': '0', 'EMC_TOMO_BUCKETS': '0'},
+    'small-units': {'EMC_UNIT_ITEMS': '64'},                # merge units of 64 list entries
+}
+
+
+@pytest.mark.parametrize('form', ['subset', 'subset-chunked', 'subset-padded', 'masked'] +
+                         ['subset-' + k for k in SUBSET_VARIANTS])
 def test_refinement_subsets(tiny, monkeypatch, form):
     """emc_set_refine: per-frame coarse cells (Voronoi of quat_gen(1)), one frame with an
     empty set and every fourth frame with every cell (a full frame: the fall-back to the plain
     search, with repeated cells in its list), against the oracle's refinement branch;
     evaluated sparsely over (frame, tile) pairs (in one or in 64-slot chunks; cells packed back
-    to back, or each padded to whole tiles), or over every pair with the excluded ones masked."""
+    to back, or each padded to whole tiles), or over every pair with the excluded ones masked;
+    and with each kernel variant of the sparse M-step (SUBSET_VARIANTS)."""
     if form == 'subset-chunked':
         monkeypatch.setenv('EMC_MAX_CHUNK_COLS', '64')
     if form == 'subset-padded':
         monkeypatch.setenv('EMC_SLOT_PACK', '0')
     if form == 'masked':
         monkeypatch.setenv('EMC_SUBSET', '0')
+    for k, v in SUBSET_VARIANTS.get(form[len('subset-'):], {}).items():
+        monkeypatch.setenv(k, v)
     from oracle import core
     from dragonfly_amd import _lib
     qc = _lib.quat_gen(1)
