@@ -180,7 +180,10 @@ int emc_set_precision(emc_engine *e, int mode);
  * when nothing changed since), graph (1: the subset iteration captured and replayed as a HIP
  * graph), fused (1: expand and pair likelihood in one persistent launch), sig_full_rows (mean
  * significant full-frame rows per tile up to which the subset path's pair tomogram reads them
- * directly instead of partial views; default 1024). Start values come from EMC_<NAME>.      */
+ * directly instead of partial views; default 1024), unit_items (brick merge: list entries per
+ * work unit, >= 64), tomo_buckets (1, default: the subset path's pair tomogram reads per-(tile,
+ * pixel range) buckets of its significant pairs; 0: each block walks its tile's pairs). Start
+ * values come from EMC_<NAME>.                                                               */
 int emc_set_option(emc_engine *e, const char *name, int value);
 int emc_get_option(emc_engine *e, const char *name, int *value);
 /* per-frame state of struct iterate: scale, beta, blacklist, fcounts, sum_fact,
