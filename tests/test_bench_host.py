@@ -80,3 +80,34 @@ def test_roofline_of_the_judged_iteration_units():
     # no fall-back frames: the pair tomogram reads no partial views
     assert st['reduce']['compulsory_bytes'] == 12 * 500 + 8 * 64 + 4 * 10 ** 9
     assert st['tomogram']['frac'] is None
+
+
+def test_cpu_baseline_fit_on_a_tiny_problem():
+    """bench.cpu_baseline end to end on the tiny geometry (3240 orientations, 60 frames, the
+    oracle on 2 threads at three orientation subsets of 32 frames): each run's split adds up
+    to its wall time, the fitted coefficients are non-negative, and the reported rate is the
+    workload's pairs over the fitted time T = alpha N + n (beta N + gamma N^2) + c."""
+    from types import SimpleNamespace
+    import numpy as np
+    import problems as P
+    p = P.build(P.tiny_config(num_data=60, num_div=4))
+    fp = p['fparams']
+    R = len(p['quats'])
+    itr = SimpleNamespace(quat=SimpleNamespace(num_rot=R, quats=p['quats']), tot_num_data=60,
+                          blacklist=np.zeros(60, 'u1'), beta_start=fp['beta'], fcounts=fp['fcounts'],
+                          sum_fact=fp['sum_fact'], mean_count=[fp['mean_count']],
+                          model=SimpleNamespace(model1=p['model1']), params=SimpleNamespace(need_scaling=0))
+    value, sample, fit = bench.cpu_baseline(p, itr, threads=2, target_rot=(400, 800, 1600), frames=32)
+    pts = fit['points']
+    assert [q['orientations'] for q in pts] == [len(p['quats'][::k]) for k in (8, 4, 2)]
+    for q in pts:
+        assert q['frames'] == 32
+        parts = q['per_orientation_s'] + q['per_frame_s'] + q['other_s']
+        assert parts == pytest.approx(q['seconds'], abs=5e-3)
+    a, b, g, c = fit['alpha_s'], fit['beta_s'], fit['gamma_s'], fit['c_s']
+    assert a > 0. and b >= 0. and g >= 0.
+    # the reported coefficients carry 4 significant digits, c and the time are rounded
+    t = a * R + 60 * (b * R + g * R * R) + c
+    assert fit['extrapolated_s'] == pytest.approx(t, rel=1e-3, abs=0.051)
+    assert value == pytest.approx(60 * R / t, rel=1e-2)
+    assert '%d frames' % 32 in sample and '%d orientations' % R in sample
