@@ -2016,6 +2016,7 @@ int emc_set_detector(emc_engine *e, int num_pix, const double *qvals, const doub
     e->have_results = false;
     e->weight_all_valid = false;
     e->bricks_valid = false;
+    e->slot_bricks_valid = false;  // the refinement subsets' brick lists hold pixels of the old detector
     HIP_TRY(hipStreamSynchronize(e->stream));
     return 0;
 }
@@ -2099,7 +2100,7 @@ int emc_set_model(emc_engine *e, long size, const double *model1) {
     emc::launch_fill(e->stream, e->weight.p, vol, 0.);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(e->stream));
-    if (size != e->size) e->weight_all_valid = false, e->bricks_valid = false;
+    if (size != e->size) e->weight_all_valid = false, e->bricks_valid = e->slot_bricks_valid = false;
     e->size = size;
     return 0;
 }
@@ -2123,7 +2124,7 @@ int emc_copy_model(emc_engine *dst, const emc_engine *src) {
     emc::launch_fill(dst->stream, dst->weight.p, vol, 0.);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(dst->stream));
-    if (size != dst->size) dst->weight_all_valid = false, dst->bricks_valid = false;
+    if (size != dst->size) dst->weight_all_valid = false, dst->bricks_valid = dst->slot_bricks_valid = false;
     dst->size = size;
     return 0;
 }
